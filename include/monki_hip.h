@@ -1,0 +1,90 @@
+/*
+ * C-ABI of libmonki_hip.so: MONKI photon transport on one gfx950 GPU.
+ *
+ * Counterpart of the reference Metal bridge ABI v2 (kernels/metal/bridge.m:26-150), which
+ * python/monki/execution/metal.py:125-156 binds with ctypes. Plain pointers and sizes only.
+ * A handle owns the device buffers of one prepared scene. One caller per handle: calls on a
+ * handle must not overlap. Every call is synchronous and returns after the device is done.
+ *
+ * Failure flags reported in error text (same meanings as kernels/metal/transport.metal:420-457):
+ *   1 nonfinite probability, 2 detector traversal limit, 4 direction sampling limit,
+ *   8 invalid Stokes vector, 16 history step limit.
+ */
+#ifndef MONKI_HIP_H
+#define MONKI_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ABI version; a Python host refuses a library whose version differs.
+ * Replaces monki_metal_abi_version (kernels/metal/bridge.m:26-28). */
+int monki_hip_abi_version(void);
+
+/* Validate packed float32 tables (layout v1, monki/execution/hip_tables.py), select the
+ * device, upload tables, and reserve tally banks: tally_count floats per round (all shards)
+ * for round_capacity rounds. Returns NULL and fills error on failure.
+ * Replaces monki_metal_create (kernels/metal/bridge.m:30-84); no kernel source: the
+ * kernels are compiled for gfx950 ahead of time. */
+void *monki_hip_create(uint32_t device, const float *tables, size_t table_count,
+                       size_t tally_count, uint32_t round_capacity, char *error,
+                       size_t error_size);
+
+/* Run one group of rounds. controls[8] = photon count of this dispatch, first flattened
+ * photon, first round (1-based), seed, step limit, profile flag, photons per round,
+ * rounds in the group. Tallies are zeroed, traced in float32, reduced over shards in
+ * float64 on the device, and copied to out: rounds x (tally_count / shards) doubles.
+ * Returns 0, or nonzero with error text (the round group is then rejected).
+ * Replaces monki_metal_run (kernels/metal/bridge.m:88-133), whose out is float per shard. */
+int monki_hip_run(void *handle, const uint32_t *controls, double *out, char *error,
+                  size_t error_size);
+
+/* Same as monki_hip_run, with transport arithmetic in float64 on the same Philox streams.
+ * No reference counterpart: the FP64 build the FP32 error is measured against. */
+int monki_hip_run_fp64(void *handle, const uint32_t *controls, double *out, char *error,
+                       size_t error_size);
+
+/* Per-photon records of the first min(record_count, controls[0]) photons of a dispatch
+ * (8 floats each: atmospheric events, surface events, end status, final weight, sum of
+ * Stokes-I detector contributions times mu0/mu, sum of detector weights, sum of weighted
+ * path contributions, last layer). precision is 32 or 64. Tallies are also returned as
+ * monki_hip_run does. No reference counterpart: used to check histories photon by photon. */
+int monki_hip_trace(void *handle, const uint32_t *controls, uint32_t precision, double *out,
+                    float *records, size_t record_count, char *error, size_t error_size);
+
+/* Event counters of the last run (monotonic since create): 0 photons, 1 atmospheric events,
+ * 2 surface events, 3 free-path cell crossings, 4 explicit-equivalent detector cells,
+ * 5 phase-table entries read, 6 rejection-sampling attempts, 7 tally atomics issued.
+ * Copies min(count, 8) values. No reference counterpart (the reference counts nothing). */
+int monki_hip_counters(void *handle, uint64_t *out, size_t count);
+
+/* Accumulated device time of transport dispatches when the profile flag was set
+ * (kernels/metal/bridge.m:121-123, 135-138). */
+double monki_hip_gpu_seconds(void *handle);
+
+/* Seconds of the transport kernel alone in the last run, from HIP events on its stream. */
+double monki_hip_kernel_seconds(void *handle);
+
+/* Device name of the handle (kernels/metal/bridge.m:140-143). */
+const char *monki_hip_device_name(void *handle);
+
+/* Device-helper probes for tests (the reference's probe_kernel technique,
+ * tests/integration/test_metal.py:23-40, as fixed entry points). No reference counterpart.
+ * monki_hip_probe_scatter: per case 8 doubles (uo, u1, phio, phi1, depol, I, Q, U) in,
+ * 12 doubles out (cos_sca, C1, S1, C2, S2, F11/4, F22/4, F33/4, F12/4, I', Q', U') computed in
+ * float32 as transport does for a Rayleigh event. monki_hip_probe_philox: per case 6 words
+ * (counter[4], key[2]) in, 4 words out. Return 0 on success. */
+int monki_hip_probe_scatter(uint32_t device, const double *cases, size_t count, double *out);
+int monki_hip_probe_philox(uint32_t device, const uint32_t *cases, size_t count, uint32_t *out);
+
+/* Release device buffers; NULL is accepted (kernels/metal/bridge.m:145-150). */
+void monki_hip_close(void *handle);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

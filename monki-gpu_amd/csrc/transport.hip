@@ -1,0 +1,1063 @@
+// MONKI forward photon transport on gfx950: one lane per photon history.
+//
+// Physics follows Victor Trees's Fortran transport (fortran/monki/upstream/PROGRAM/mc.f90:314-1474
+// and its helper modules), with the counter-based random streams and float32 arithmetic of the
+// reference GPU kernel (kernels/metal/transport.metal:339-458). Every device routine is a
+// template over the arithmetic type so the same code runs in float32 (product) and float64
+// (precision reference on identical Philox streams).
+//
+// Layout on the device: packed float32 tables (layout v1, see monki/execution/hip_tables.py),
+// float32 tally banks [round][shard][stokes | paths | weights], per-lane layer history in LDS.
+
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "monki_hip.h"
+
+namespace {
+
+constexpr int kMaxLayers = 64;
+constexpr int kMaxSpecies = 4;
+constexpr int kMaxViews = 3;
+constexpr int kHeaderWords = 64;
+constexpr int kLayoutVersion = 1;
+constexpr int kBlock32 = 256;
+constexpr int kBlock64 = 128;
+constexpr int kCounters = 8;
+constexpr int kTraceWords = 8;
+constexpr uint32_t kSamplingAttempts = 4096;
+
+enum FailureFlag : unsigned {
+  kNonfiniteProbability = 1u,
+  kDetectorLimit = 2u,
+  kSamplingLimit = 4u,
+  kInvalidStokes = 8u,
+  kHistoryLimit = 16u,
+};
+
+enum Status : int { kRunning = 0, kEscaped = 1, kAbsorbed = 2, kFailed = 3 };
+
+struct SpeciesTables {
+  int lut_size;
+  float ssalb;
+  // cosine[0, n) ascending; cosine[n, 2n) = 1 - cos; cosine[2n, 3n) = 1 + cos.
+  const float *cosine, *cdf, *f11, *f22, *f33, *f12;
+};
+
+// Scene constants and table pointers, passed by value as a kernel argument.
+struct Scene {
+  int nx, ny, nz, nmat, views, species, paths, uniform_top, shards;
+  int tally_count;  // floats per round per shard
+  int stokes_count, path_count;
+  float xunit, yunit, depol, theta0, phi0, phi;
+  float view[kMaxViews];
+  const float *zunit, *albedo, *beta, *cumfrac, *ssalb_gas, *height_above, *tau_above;
+  SpeciesTables sp[kMaxSpecies];
+};
+
+struct Controls {
+  uint32_t count, first, round, seed, max_steps, profile, per_round, rounds;
+};
+
+// ---------------------------------------------------------------------------------------
+// Philox4x32-10 (Salmon et al. 2011, Random123 constants). Counter (photon, round, block,
+// stream), key (seed, 0): identical to transport.metal:64-92 for stream 0, so a photon's
+// draws do not depend on dispatch size, sharding or grouping.
+// ---------------------------------------------------------------------------------------
+struct Words {
+  uint32_t x, y, z, w;
+};
+
+__host__ __device__ inline Words philox(Words c, uint32_t k0, uint32_t k1) {
+  for (int i = 0; i < 10; ++i) {
+    const uint64_t p0 = uint64_t(0xD2511F53u) * c.x;
+    const uint64_t p1 = uint64_t(0xCD9E8D57u) * c.z;
+    c = Words{uint32_t(p1 >> 32) ^ c.y ^ k0, uint32_t(p1), uint32_t(p0 >> 32) ^ c.w ^ k1,
+              uint32_t(p0)};
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return c;
+}
+
+struct Stream {
+  uint32_t photon, round, seed, stream, block;
+  Words words;
+  int lane;
+};
+
+// Open (0,1) float32 grid of half steps: ((w >> 9) + 0.5) * 2^-23 (transport.metal:89-91).
+__device__ inline float uniform(Stream &rng) {
+  if (rng.lane == 4) {
+    rng.words = philox(Words{rng.photon, rng.round, rng.block++, rng.stream}, rng.seed, 0u);
+    rng.lane = 0;
+  }
+  uint32_t w = rng.lane == 0 ? rng.words.x
+               : rng.lane == 1 ? rng.words.y
+               : rng.lane == 2 ? rng.words.z
+                               : rng.words.w;
+  ++rng.lane;
+  return (float(w >> 9) + 0.5f) * 0x1p-23f;
+}
+
+// Per-lane event counters, reduced per wave before one global atomic each.
+struct Counts {
+  uint32_t photons, events, surface, walk, detector, lut, attempts, atomics;
+};
+
+template <typename R> struct Elements {
+  R f11, f22, f33, f12;
+};
+
+template <typename R> struct Stokes {
+  R i, q, u;
+};
+
+template <typename R> struct Photon {
+  int i, j, k;
+  R rx, ry, rz;  // distances to the faces being approached (mc.f90:280-300)
+  R a, b, c;     // direction cosines, c positive upwards
+  R phi;         // azimuth of the direction (rad)
+};
+
+template <typename R> __device__ inline R clamp1(R x) {
+  return x < R(-1) ? R(-1) : (x > R(1) ? R(1) : x);
+}
+
+// Largest index low with axis[low] <= value, in [0, n-2] (transport.metal:98-108).
+__device__ inline int bracket(const float *__restrict__ axis, int n, float value, uint32_t &reads) {
+  int low = 0, high = n - 1;
+  while (high - low > 1) {
+    const int middle = (low + high) >> 1;
+    ++reads;
+    if (axis[middle] <= value)
+      low = middle;
+    else
+      high = middle;
+  }
+  return low;
+}
+
+template <typename R>
+__device__ inline int bracket_r(const float *__restrict__ axis, int n, R value, uint32_t &reads) {
+  int low = 0, high = n - 1;
+  while (high - low > 1) {
+    const int middle = (low + high) >> 1;
+    ++reads;
+    if (R(axis[middle]) <= value)
+      low = middle;
+    else
+      high = middle;
+  }
+  return low;
+}
+
+// ray_scat_matrix / mie_scat_matrix (mc_routines_general_mod.f90:74-136, 257-388):
+// F11, F22, F33, F12 divided by 4 (radiance normalized for incident flux pi).
+template <typename R>
+__device__ inline Elements<R> elements(const Scene &s, int species, R cosine, uint32_t &reads) {
+  cosine = clamp1(cosine);
+  Elements<R> f;
+  if (species < 0) {
+    const R depol = R(s.depol);
+    const R delta = (R(1) - depol) / (R(1) + depol / R(2));
+    const R diagonal = delta * R(0.75) * (R(1) + cosine * cosine);
+    f.f11 = diagonal + R(1) - delta;
+    f.f22 = diagonal;
+    f.f33 = delta * R(1.5) * cosine;
+    f.f12 = -delta * R(0.75) * (R(1) - cosine * cosine);
+  } else {
+    const SpeciesTables &t = s.sp[species];
+    const int low = bracket_r<R>(t.cosine, t.lut_size, cosine, reads);
+    const R c0 = R(t.cosine[low]), c1 = R(t.cosine[low + 1]);
+    const R w = (cosine - c0) / (c1 - c0);
+    reads += 10;
+    f.f11 = R(t.f11[low]) + w * (R(t.f11[low + 1]) - R(t.f11[low]));
+    f.f22 = R(t.f22[low]) + w * (R(t.f22[low + 1]) - R(t.f22[low]));
+    f.f33 = R(t.f33[low]) + w * (R(t.f33[low + 1]) - R(t.f33[low]));
+    f.f12 = R(t.f12[low]) + w * (R(t.f12[low + 1]) - R(t.f12[low]));
+  }
+  if (s.nmat == 1) f.f12 = R(0);
+  f.f11 *= R(0.25);
+  f.f22 *= R(0.25);
+  f.f33 *= R(0.25);
+  f.f12 *= R(0.25);
+  return f;
+}
+
+template <typename R> struct Rotation {
+  R c1, s1, c2, s2;  // cos/sin of twice the rotation angles i1, i2
+};
+
+// stokes_rot_mat (mc_routines_general_mod.f90:942-1174): coplanar, both-vertical,
+// scattered-vertical, incident-vertical and general cases. The angular degeneracy
+// threshold is 2e-7 in float32 (as transport.metal:132-137) and 1e-16 in float64 (as mc.f90).
+template <typename R>
+__device__ inline Rotation<R> rotation(R cosine, R uo, R u1, R phio, R phi1) {
+  const R eps = sizeof(R) == 4 ? R(2e-7) : R(1e-16);
+  const R pi = R(3.14159265358979323846);
+  const R dphi = fabs(phio - phi1);
+  const R rem = fmod(dphi, pi);
+  const bool vertical_o = sizeof(R) == 4 ? fabs(uo) == R(1) : fabs(uo) > R(1) - R(1e-16);
+  const bool vertical_1 = sizeof(R) == 4 ? fabs(u1) == R(1) : fabs(u1) > R(1) - R(1e-16);
+  if (fmin(rem, pi - rem) < eps || (vertical_o && vertical_1)) return Rotation<R>{R(1), R(0), R(1), R(0)};
+
+  const R sino = sqrt(fmax(R(0), R(1) - uo * uo));
+  const R sin1 = sqrt(fmax(R(0), R(1) - u1 * u1));
+  const R cpo = cos(phio), spo = sin(phio), cp1 = cos(phi1), sp1 = sin(phi1);
+  R cosi1, cosi2, q;
+  if (vertical_1) {
+    cosi1 = copysign(R(1), u1);
+    cosi2 = -cosi1 * (cp1 * cpo + sp1 * spo);
+    q = sino * spo * cp1 - sino * cpo * sp1;
+  } else if (vertical_o) {
+    cosi2 = copysign(R(1), uo);
+    cosi1 = -cosi2 * (cp1 * cpo + sp1 * spo);
+    q = spo * sin1 * cp1 - cpo * sin1 * sp1;
+  } else {
+    const R e = R(1e-16);
+    const R sc = R(1) - cosine * cosine;
+    cosi1 = (u1 - uo * cosine) / sqrt(fmax(e, (R(1) - uo * uo) * sc));
+    cosi2 = (uo - u1 * cosine) / sqrt(fmax(e, (R(1) - u1 * u1) * sc));
+    q = sino * spo * sin1 * cp1 - sino * cpo * sin1 * sp1;
+  }
+  cosi1 = clamp1(cosi1);
+  cosi2 = clamp1(cosi2);
+  const R sini1 = copysign(sqrt(fmax(R(0), R(1) - cosi1 * cosi1)), q);
+  const R sini2 = copysign(sqrt(fmax(R(0), R(1) - cosi2 * cosi2)), q);
+  return Rotation<R>{cosi1 * cosi1 - sini1 * sini1, R(2) * sini1 * cosi1,
+                     cosi2 * cosi2 - sini2 * sini2, R(2) * sini2 * cosi2};
+}
+
+// matmul_stokes_rot then P * S (mc_routines_general_mod.f90:1184-1242), I/Q/U only.
+template <typename R>
+__device__ inline Stokes<R> scatter(const Elements<R> &f, const Rotation<R> &r, const Stokes<R> &s) {
+  const R rq = r.c1 * s.q - r.s1 * s.u;
+  const R ru = r.s1 * s.q + r.c1 * s.u;
+  const R q = f.f12 * s.i + f.f22 * rq;
+  const R u = f.f33 * ru;
+  return Stokes<R>{f.f11 * s.i + f.f12 * rq, r.c2 * q - r.s2 * u, r.s2 * q + r.c2 * u};
+}
+
+// rayl_scatter_pol / mie_scatter_pol_bound, forward sign -1 (mc_routines_general_mod.f90:
+// 492-932): rejection sampling of (cos theta, sigma) from F11 + F12 (Q cos 2s - U sin 2s),
+// uniform cosine under the Rayleigh bound F11(1), inverse bound-CDF for tabulated species.
+template <typename R>
+__device__ inline bool sample_direction(const Scene &s, int species, const Stokes<R> &st, R uo,
+                                        Stream &rng, R &u1, R &dphi, R &sampled, Counts &n) {
+  const R pi = R(3.14159265358979323846);
+  const R eps = R(1e-16);
+  for (uint32_t attempt = 0; attempt < kSamplingAttempts; ++attempt) {
+    ++n.attempts;
+    // sin^2 of the scattering angle as (1 - cos)(1 + cos) from factors that keep their
+    // precision near both peaks: exact for Rayleigh (cos = 2u - 1), interpolated from the
+    // 1 -/+ cos tables for tabulated species.
+    R cosine, sin2;
+    if (species < 0) {
+      const float u = uniform(rng);
+      cosine = R(2) * R(u) - R(1);
+      sin2 = R(4) * R(u) * (R(1) - R(u));
+    } else {
+      const SpeciesTables &t = s.sp[species];
+      const float draw = uniform(rng);
+      const int low = bracket(t.cdf, t.lut_size, draw, n.lut);
+      const R d0 = R(t.cdf[low]), d1 = R(t.cdf[low + 1]);
+      const R w = (R(draw) - d0) / (d1 - d0);
+      const float *omc = t.cosine + t.lut_size, *opc = t.cosine + 2 * t.lut_size;
+      const R one_minus = R(omc[low]) + w * (R(omc[low + 1]) - R(omc[low]));
+      const R one_plus = R(opc[low]) + w * (R(opc[low + 1]) - R(opc[low]));
+      cosine = R(t.cosine[low]) + w * (R(t.cosine[low + 1]) - R(t.cosine[low]));
+      sin2 = one_minus * one_plus;
+      n.lut += 8;
+    }
+    const R sigma = R(2) * pi * R(uniform(rng)) - pi;
+    const Elements<R> f = elements<R>(s, species, cosine, n.lut);
+    const R pdf = f.f11 + f.f12 * (st.q * cos(R(2) * sigma) - st.u * sin(R(2) * sigma));
+    const R bound =
+        species < 0 ? elements<R>(s, -1, R(1), n.lut).f11 : f.f11 + fabs(f.f12);
+    if (R(uniform(rng)) * bound > pdf) continue;
+
+    sampled = cosine;
+    const R sin_in = sqrt(fmax(eps, R(1) - uo * uo));
+    const R sin_sca = sqrt(fmax(eps, sin2));
+    u1 = clamp1(uo * cosine + sin_in * sin_sca * cos(sigma));
+    const R frac = (cosine - uo * u1) / (sin_in * sqrt(fmax(eps, R(1) - u1 * u1)));
+    dphi = -acos(clamp1(frac)) * copysign(R(1), sigma);
+    return true;
+  }
+  return false;
+}
+
+template <typename R>
+__device__ inline int cell_index(const Scene &s, int i, int j, int k) {
+  return i + s.nx * (j + s.ny * k);
+}
+
+// New direction: reverse the face distances whose axis changed sign (mc.f90:1418-1421).
+template <typename R>
+__device__ inline void change_direction(const Scene &s, Photon<R> &p, R a, R b, R c, R phi) {
+  if (p.a * a < R(0)) p.rx = fmax(R(0), R(s.xunit) - p.rx);
+  if (p.b * b < R(0)) p.ry = fmax(R(0), R(s.yunit) - p.ry);
+  if (p.c * c < R(0)) p.rz = fmax(R(0), R(s.zunit[p.k]) - p.rz);
+  p.a = a;
+  p.b = b;
+  p.c = c;
+  p.phi = phi;
+}
+
+template <typename R> __device__ inline R boundary_distance(const Photon<R> &p) {
+  const R eps = R(1e-16);
+  const R dx = p.rx / fmax(fabs(p.a), eps);
+  const R dy = p.ry / fmax(fabs(p.b), eps);
+  const R dz = p.rz / fmax(fabs(p.c), eps);
+  return fmin(dx, fmin(dy, dz));
+}
+
+// ADVANCE_internal (mc.f90:1531-1580): move to the nearest face, x before y before z on ties,
+// periodic in x and y. Returns the distance travelled.
+template <typename R> __device__ inline R advance(const Scene &s, Photon<R> &p) {
+  const R eps = R(1e-16);
+  const R aa = fabs(p.a), ab = fabs(p.b), ac = fabs(p.c);
+  const R dx = p.rx / fmax(aa, eps);
+  const R dy = p.ry / fmax(ab, eps);
+  const R dz = p.rz / fmax(ac, eps);
+  const R d = fmin(dx, fmin(dy, dz));
+  if (dx == d) {
+    p.i += p.a >= R(0) ? 1 : -1;
+    p.i = p.i < 0 ? s.nx - 1 : (p.i >= s.nx ? 0 : p.i);
+    p.rx = R(s.xunit);
+    p.ry = fmax(R(0), p.ry - ab * d);
+    p.rz = fmax(R(0), p.rz - ac * d);
+  } else if (dy == d) {
+    p.j += p.b >= R(0) ? 1 : -1;
+    p.j = p.j < 0 ? s.ny - 1 : (p.j >= s.ny ? 0 : p.j);
+    p.rx = fmax(R(0), p.rx - aa * d);
+    p.ry = R(s.yunit);
+    p.rz = fmax(R(0), p.rz - ac * d);
+  } else {
+    p.k += p.c >= R(0) ? 1 : -1;
+    p.rx = fmax(R(0), p.rx - aa * d);
+    p.ry = fmax(R(0), p.ry - ab * d);
+    if (p.k >= 0 && p.k < s.nz) p.rz = R(s.zunit[p.k]);
+  }
+  return d;
+}
+
+// Column reached by a straight ray through uniform layers (transport.metal:270-279 and the
+// analytical register mapping of mc_routines_cartesian_mod.f90:1076-1105).
+template <typename R>
+__device__ inline int projected_column(int index, R remaining, R direction, R distance, R unit,
+                                       int count) {
+  const R local = direction >= R(0) ? unit - remaining : remaining;
+  R position = fmod(R(index) * unit + local + direction * distance, unit * R(count));
+  if (position < R(0)) position += unit * R(count);
+  const int column = int(position / unit);
+  return column < 0 ? 0 : (column < count - 1 ? column : count - 1);
+}
+
+struct DetectorHit {
+  int i, j;
+  bool ok;
+};
+
+// calc_radiances_and_paths_pol (mc_routines_cartesian_mod.f90:849-1222) for one view:
+// optical depth from the event to the top of the last physical layer and the TOA pixel.
+// Cells are walked one by one below uniform_top; above it the vertical prefix gives the exact
+// slant integral (transport.metal:299-313).
+template <typename R>
+__device__ inline DetectorHit detector_walk(const Scene &s, const Photon<R> &event, R va, R vb,
+                                            R mu, R &tau, uint32_t max_steps, Counts &n) {
+  Photon<R> ray = event;
+  change_direction<R>(s, ray, va, vb, mu, R(s.phi));
+  tau = R(0);
+  uint32_t steps = 0;
+  while (ray.k >= 0 && ray.k < s.nz - 1) {
+    if (++steps > max_steps) return DetectorHit{0, 0, false};
+    if (ray.k >= s.uniform_top) {
+      const R distance = (ray.rz + R(s.height_above[ray.k])) / mu;
+      tau += (ray.rz * R(s.beta[cell_index<R>(s, ray.i, ray.j, ray.k)]) +
+              R(s.tau_above[ray.k])) / mu;
+      // Explicit-equivalent cell count: layers left plus horizontal faces crossed.
+      const R hx = fabs(va) * distance, hy = fabs(vb) * distance;
+      n.detector += uint32_t(s.nz - 1 - ray.k) +
+                    uint32_t((R(s.xunit) - ray.rx + hx) / R(s.xunit)) +
+                    uint32_t((R(s.yunit) - ray.ry + hy) / R(s.yunit));
+      const int i = projected_column<R>(ray.i, ray.rx, va, distance, R(s.xunit), s.nx);
+      const int j = projected_column<R>(ray.j, ray.ry, vb, distance, R(s.yunit), s.ny);
+      return DetectorHit{i, j, true};
+    }
+    const int cell = cell_index<R>(s, ray.i, ray.j, ray.k);
+    const R d = advance<R>(s, ray);
+    tau += d * R(s.beta[cell]);
+    ++n.detector;
+  }
+  return DetectorHit{ray.i, ray.j, true};
+}
+
+template <typename T> __device__ inline void add_tally(T *target, T value) {
+  __hip_atomic_fetch_add(target, value, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+template <typename R> struct Trace {
+  R events, surface, status, weight, stokes_i, weights, paths, layer;
+};
+
+// Local estimate for one view: detector walk, attenuation and all tallies of this event.
+template <typename R, bool TRACE>
+__device__ inline bool register_view(const Scene &s, const Photon<R> &p, int view,
+                                     const Stokes<R> &probability, const R *history, int hstride,
+                                     R *__restrict__ bank, uint32_t max_steps, Counts &n,
+                                     Trace<R> &trace) {
+  if (probability.i <= R(1e-16)) return true;
+  const R theta = R(s.view[view]);
+  const R mu = cos(theta), sv = sin(theta);
+  const R va = sv * cos(R(s.phi)), vb = sv * sin(R(s.phi));
+  R tau;
+  const DetectorHit hit = detector_walk<R>(s, p, va, vb, mu, tau, max_steps, n);
+  if (!hit.ok) return false;
+
+  const R attenuation = exp(-tau);
+  const R ci = probability.i * attenuation;
+  const R geometry = fabs(cos(R(s.theta0))) / mu;  // mu0 / mu (mc.f90:1456-1474)
+  const int pixel = hit.i + s.nx * (hit.j + s.ny * view);
+  add_tally<R>(bank + s.nmat * pixel, ci * geometry);
+  if (s.nmat > 1) {
+    add_tally<R>(bank + s.nmat * pixel + 1, probability.q * attenuation * geometry);
+    add_tally<R>(bank + s.nmat * pixel + 2, probability.u * attenuation * geometry);
+  }
+  n.atomics += s.nmat;
+  if (TRACE) {
+    trace.stokes_i += ci * geometry;
+  }
+  if (s.paths) {
+    // Detector-ray lengths per layer: the partial event layer, then full layers to TOA.
+    const R rz = p.c * mu < R(0) ? fmax(R(0), R(s.zunit[p.k]) - p.rz) : p.rz;
+    R *paths = bank + s.stokes_count + hit.i + s.nx * hit.j;
+    const int layer_stride = s.nx * s.ny;
+    const int view_offset = layer_stride * s.nz * view;
+    R total = R(0);
+    for (int k = 0; k < s.nz; ++k) {
+      R length = history[k * hstride];
+      if (k == p.k && k < s.nz - 1)
+        length += rz / mu;
+      else if (k > p.k && k < s.nz - 1)
+        length += R(s.zunit[k]) / mu;
+      if (length > R(0)) {
+        add_tally<R>(paths + view_offset + layer_stride * k, ci * length);
+        ++n.atomics;
+      }
+      if (TRACE) total += ci * length;
+    }
+    add_tally<R>(bank + s.stokes_count + s.path_count + pixel, ci);
+    ++n.atomics;
+    if (TRACE) {
+      trace.weights += ci;
+      trace.paths += total;
+    }
+  }
+  return true;
+}
+
+template <typename R> __device__ inline bool finite3(const Stokes<R> &v) {
+  return isfinite(v.i) && isfinite(v.q) && isfinite(v.u);
+}
+
+// One photon history (mc.f90:314-1436).
+template <typename R, bool TRACE>
+__device__ int trace_photon(const Scene &s, const Controls &c, uint32_t lane,
+                            R *__restrict__ tallies, unsigned *failure, R *history,
+                            int hstride, Counts &n, Trace<R> &trace) {
+  const R eps = R(1e-16);
+  const R pi = R(3.14159265358979323846);
+  const uint32_t identity = c.first + lane;
+  const uint32_t round_offset = identity / c.per_round;
+  const uint32_t photon = identity % c.per_round;
+  R *bank = tallies + size_t(round_offset * uint32_t(s.shards) + lane % uint32_t(s.shards)) *
+                          size_t(s.tally_count);
+  Stream rng{photon, c.round + round_offset, c.seed, 0u, 0u, Words{0, 0, 0, 0}, 4};
+
+  // Launch at TOA in the vacuum layer of column photon mod (nx*ny) (mc.f90:351-420).
+  const int column = int(photon % uint32_t(s.nx * s.ny));
+  Photon<R> p;
+  p.i = column % s.nx;
+  p.j = column / s.nx;
+  p.k = s.nz - 1;
+  const R t0 = R(s.theta0), p0 = R(s.phi0);
+  p.a = sin(t0) * cos(p0);
+  p.b = sin(t0) * sin(p0);
+  p.c = cos(t0);
+  p.phi = p0;
+  p.rx = R(s.xunit) * R(uniform(rng));
+  p.ry = R(s.yunit) * R(uniform(rng));
+  p.rz = R(s.zunit[p.k]);
+  for (int k = 0; k < s.nz; ++k) history[k * hstride] = R(0);
+  Stokes<R> st{R(1), R(0), R(0)};
+  R weight = R(1);
+  R remains = -log(R(uniform(rng)));
+  ++n.photons;
+
+  for (uint32_t step = 0; step < c.max_steps; ++step) {
+    // Free path: event inside this cell or move to the next face (mc.f90:476-677).
+    const R extinction =
+        (p.k < s.nz - 1 && p.k >= 0) ? R(s.beta[cell_index<R>(s, p.i, p.j, p.k)]) : R(0);
+    const R event_distance = extinction > eps ? remains / extinction : R(INFINITY);
+    bool surface = false;
+    if (event_distance < boundary_distance<R>(p)) {
+      p.rx = fmax(R(0), p.rx - fabs(p.a) * event_distance);
+      p.ry = fmax(R(0), p.ry - fabs(p.b) * event_distance);
+      p.rz = fmax(R(0), p.rz - fabs(p.c) * event_distance);
+      if (s.paths) history[p.k * hstride] += event_distance;
+    } else {
+      const int old_layer = p.k;
+      const R d = advance<R>(s, p);
+      ++n.walk;
+      if (s.paths) history[old_layer * hstride] += d;
+      remains -= d * extinction;
+      if (p.k >= s.nz) return kEscaped;
+      if (p.k >= 0) continue;
+      surface = true;
+      p.k = 0;
+      p.rz = R(0);
+    }
+
+    // Surface albedo or species choice with absorption weight (mc.f90:681-727, 1028-1074).
+    int species = -1;
+    if (surface) {
+      weight *= R(s.albedo[p.i + s.nx * p.j]);
+      ++n.surface;
+      if (TRACE) trace.surface += R(1);
+    } else {
+      const float draw = uniform(rng);
+      const int cell = cell_index<R>(s, p.i, p.j, p.k);
+      const int cells = s.nx * s.ny * s.nz;
+      for (int t = 0; t < s.species; ++t) {
+        if (draw < s.cumfrac[t * cells + cell]) {
+          species = t;
+          break;
+        }
+      }
+      weight *= species < 0 ? R(s.ssalb_gas[p.k]) : R(s.sp[species].ssalb);
+      ++n.events;
+      if (TRACE) trace.events += R(1);
+    }
+    if (weight < eps) return kAbsorbed;
+
+    // Local estimate toward every view (mc.f90:738-822 surface, 1109-1207 atmosphere).
+    for (int view = 0; view < s.views; ++view) {
+      const R theta = R(s.view[view]);
+      const R mu = cos(theta);
+      Stokes<R> prob;
+      if (surface) {
+        prob = Stokes<R>{st.i * mu * weight, R(0), R(0)};
+      } else {
+        const R cosine = clamp1(p.c * mu + sqrt(fmax(R(0), R(1) - p.c * p.c)) * sin(theta) *
+                                               cos(p.phi - R(s.phi)));
+        const Elements<R> f = elements<R>(s, species, cosine, n.lut);
+        const Rotation<R> r = s.nmat > 1 ? rotation<R>(cosine, p.c, mu, p.phi, R(s.phi))
+                                         : Rotation<R>{R(1), R(0), R(1), R(0)};
+        prob = scatter<R>(f, r, st);
+        prob.i *= weight;
+        prob.q *= weight;
+        prob.u *= weight;
+      }
+      if (!finite3(prob)) {
+        atomicOr(failure, kNonfiniteProbability);
+        return kFailed;
+      }
+      if (!register_view<R, TRACE>(s, p, view, prob, history, hstride, bank, c.max_steps, n, trace)) {
+        atomicOr(failure, kDetectorLimit);
+        return kFailed;
+      }
+    }
+
+    // Continuation direction (mc.f90:1266-1431; Lambertian GROUND_LAMB2 for the surface).
+    R mu, phi;
+    if (surface) {
+      mu = sqrt(R(uniform(rng)));
+      phi = R(2) * pi * R(uniform(rng));
+      st = Stokes<R>{R(1), R(0), R(0)};
+    } else {
+      R dphi, cosine;
+      if (!sample_direction<R>(s, species, st, p.c, rng, mu, dphi, cosine, n)) {
+        atomicOr(failure, kSamplingLimit);
+        return kFailed;
+      }
+      phi = p.phi + dphi;
+      // The sampled scattering cosine is used directly, as in mc.f90.
+      if (s.nmat > 1) {
+        const Elements<R> f = elements<R>(s, species, cosine, n.lut);
+        const Rotation<R> r = rotation<R>(cosine, p.c, mu, p.phi, phi);
+        st = scatter<R>(f, r, st);
+        const R norm = st.i;
+        st.i = R(1);
+        st.q /= norm;
+        st.u /= norm;
+        if (!finite3(st) || sqrt(st.q * st.q + st.u * st.u) > R(1.01)) {
+          atomicOr(failure, kInvalidStokes);
+          return kFailed;
+        }
+      }
+      remains = -log(R(uniform(rng)));
+    }
+    const R sine = sqrt(fmax(R(0), R(1) - mu * mu));
+    change_direction<R>(s, p, cos(phi) * sine, sin(phi) * sine, mu, phi);
+    if (TRACE) {
+      trace.weight = weight;
+      trace.layer = R(p.k);
+    }
+  }
+  atomicOr(failure, kHistoryLimit);
+  return kFailed;
+}
+
+__device__ inline unsigned long long wave_sum(unsigned long long v) {
+  for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, 64);
+  return v;
+}
+
+template <typename R, bool TRACE>
+__global__ void __launch_bounds__(kBlock32) transport_kernel(Scene s, Controls c, R *tallies,
+                                                             unsigned *failure,
+                                                             unsigned long long *counters,
+                                                             float *records, uint32_t record_count) {
+  extern __shared__ unsigned char lds_raw[];
+  R *history = reinterpret_cast<R *>(lds_raw) + threadIdx.x;
+  const int hstride = blockDim.x;
+  const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+  Counts n{0, 0, 0, 0, 0, 0, 0, 0};
+  if (lane < c.count) {
+    Trace<R> trace{R(0), R(0), R(0), R(1), R(0), R(0), R(0), R(0)};
+    const int status = trace_photon<R, TRACE>(s, c, lane, tallies, failure, history, hstride, n, trace);
+    if (TRACE && lane < record_count) {
+      float *out = records + size_t(lane) * kTraceWords;
+      out[0] = float(trace.events);
+      out[1] = float(trace.surface);
+      out[2] = float(status);
+      out[3] = float(trace.weight);
+      out[4] = float(trace.stokes_i);
+      out[5] = float(trace.weights);
+      out[6] = float(trace.paths);
+      out[7] = float(trace.layer);
+    }
+  }
+  const uint32_t values[kCounters] = {n.photons, n.events, n.surface, n.walk,
+                                      n.detector, n.lut, n.attempts, n.atomics};
+#pragma unroll
+  for (int i = 0; i < kCounters; ++i) {
+    const unsigned long long total = wave_sum(static_cast<unsigned long long>(values[i]));
+    if ((threadIdx.x & 63) == 0 && total) atomicAdd(counters + i, total);
+  }
+}
+
+// Sum the shard banks of each round in float64: out[r][t] = sum_s banks[r][s][t].
+template <typename B>
+__global__ void reduce_shards(const B *__restrict__ banks, double *__restrict__ out,
+                              int tally_count, int shards, int rounds) {
+  const size_t index = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (index >= size_t(tally_count) * rounds) return;
+  const size_t r = index / tally_count, t = index % tally_count;
+  const B *base = banks + r * size_t(shards) * tally_count + t;
+  double sum = 0.0;
+  for (int sh = 0; sh < shards; ++sh) sum += double(base[size_t(sh) * tally_count]);
+  out[index] = sum;
+}
+
+// ---------------------------------------------------------------------------------------
+// Host side.
+// ---------------------------------------------------------------------------------------
+struct Session {
+  int device = -1;
+  char name[256] = {0};
+  Scene scene{};
+  float *tables = nullptr;
+  void *banks = nullptr;  // float or double banks, sized for double
+  double *reduced = nullptr;
+  unsigned *failure = nullptr;
+  unsigned long long *counters = nullptr;
+  float *records = nullptr;
+  size_t record_capacity = 0;
+  size_t tally_count = 0;  // per round, all shards
+  uint32_t round_capacity = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t start = nullptr, stop = nullptr;
+  double gpu_seconds = 0.0;
+  double kernel_seconds = 0.0;
+};
+
+void set_error(char *error, size_t size, const char *fmt, const char *detail = "") {
+  if (error && size) snprintf(error, size, fmt, detail);
+}
+
+bool check(hipError_t status, char *error, size_t size, const char *what) {
+  if (status == hipSuccess) return true;
+  if (error && size) snprintf(error, size, "%s: %s", what, hipGetErrorString(status));
+  return false;
+}
+
+void release(Session *s) {
+  if (!s) return;
+  if (s->device >= 0) (void)hipSetDevice(s->device);
+  if (s->start) (void)hipEventDestroy(s->start);
+  if (s->stop) (void)hipEventDestroy(s->stop);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  (void)hipFree(s->tables);
+  (void)hipFree(s->banks);
+  (void)hipFree(s->reduced);
+  (void)hipFree(s->failure);
+  (void)hipFree(s->counters);
+  (void)hipFree(s->records);
+  delete s;
+}
+
+bool word_int(const float *t, int index, int lo, int hi, int &out) {
+  const float v = t[index];
+  if (!(v == floorf(v)) || v < float(lo) || v > float(hi)) return false;
+  out = int(v);
+  return true;
+}
+
+// Parse and validate the header; pointers are offsets into `base` (host or device).
+bool parse_tables(const float *t, size_t count, const float *base, Scene &s, char *error, size_t size) {
+  if (count < size_t(kHeaderWords) || t[0] != float(kLayoutVersion)) {
+    set_error(error, size, "Unsupported packed table layout");
+    return false;
+  }
+  for (size_t i = 0; i < count; ++i) {
+    if (!isfinite(t[i])) {
+      set_error(error, size, "Nonfinite packed transport tables");
+      return false;
+    }
+  }
+  int length = 0;
+  if (!word_int(t, 1, 1, 1 << 16, s.nx) || !word_int(t, 2, 1, 1 << 16, s.ny) ||
+      !word_int(t, 3, 2, kMaxLayers, s.nz) || !word_int(t, 4, 1, 3, s.nmat) ||
+      !word_int(t, 5, 1, kMaxViews, s.views) || !word_int(t, 6, 0, kMaxSpecies, s.species) ||
+      !word_int(t, 7, 0, 1, s.paths) || !word_int(t, 8, 0, kMaxLayers, s.uniform_top) ||
+      !word_int(t, 9, 1, 64, s.shards) || !word_int(t, 26, kHeaderWords, 1 << 24, length)) {
+    set_error(error, size, "Invalid packed table dimensions");
+    return false;
+  }
+  if (s.nmat == 2 || size_t(length) != count) {
+    set_error(error, size, "Packed table size or Stokes size mismatch");
+    return false;
+  }
+  const long cells = long(s.nx) * s.ny * s.nz;
+  s.xunit = t[10];
+  s.yunit = t[11];
+  s.depol = t[12];
+  s.theta0 = t[13];
+  s.phi0 = t[14];
+  s.phi = t[15];
+  for (int v = 0; v < kMaxViews; ++v) s.view[v] = v < s.views ? t[16 + v] : 0.0f;
+  struct Span {
+    int word;
+    long length;
+    const float **target;
+  } spans[] = {{19, s.nz, &s.zunit},        {20, long(s.nx) * s.ny, &s.albedo},
+               {21, cells, &s.beta},         {22, cells * s.species, &s.cumfrac},
+               {23, s.nz, &s.ssalb_gas},     {24, s.nz, &s.height_above},
+               {25, s.nz, &s.tau_above}};
+  for (const Span &span : spans) {
+    int offset = 0;
+    if (!word_int(t, span.word, kHeaderWords, length, offset) || offset + span.length > length) {
+      set_error(error, size, "Packed table offset out of range");
+      return false;
+    }
+    *span.target = base + offset;
+  }
+  for (int sp = 0; sp < kMaxSpecies; ++sp) {
+    SpeciesTables &tab = s.sp[sp];
+    tab = SpeciesTables{0, 0.0f, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (sp >= s.species) continue;
+    const int h = 32 + 8 * sp;
+    if (!word_int(t, h, 2, 1 << 20, tab.lut_size)) {
+      set_error(error, size, "Invalid scattering LUT size");
+      return false;
+    }
+    tab.ssalb = t[h + 1];
+    const float **targets[6] = {&tab.cosine, &tab.cdf, &tab.f11, &tab.f22, &tab.f33, &tab.f12};
+    for (int a = 0; a < 6; ++a) {
+      int offset = 0;
+      const int extent = a == 0 ? 3 * tab.lut_size : tab.lut_size;
+      if (!word_int(t, h + 2 + a, kHeaderWords, length, offset) || offset + extent > length) {
+        set_error(error, size, "Scattering table offset out of range");
+        return false;
+      }
+      *targets[a] = base + offset;
+    }
+    // Ascending cosines spanning [-1, 1] and a CDF from 0 to 1 (metal_tables.py:421-425).
+    const float *cs = t + (tab.cosine - base), *cd = t + (tab.cdf - base);
+    if (fabsf(cs[0] + 1.0f) > 1e-6f || fabsf(cs[tab.lut_size - 1] - 1.0f) > 1e-6f ||
+        fabsf(cd[0]) > 1e-6f || fabsf(cd[tab.lut_size - 1] - 1.0f) > 1e-6f) {
+      set_error(error, size, "Invalid scattering angle grid or CDF");
+      return false;
+    }
+    for (int a = 1; a < tab.lut_size; ++a) {
+      if (!(cs[a] > cs[a - 1]) || cd[a] < cd[a - 1]) {
+        set_error(error, size, "Invalid scattering angle grid or CDF");
+        return false;
+      }
+    }
+  }
+  for (int v = 0; v < s.views; ++v) {
+    if (!(cosf(s.view[v]) > 0.0f)) {
+      set_error(error, size, "Viewing angles must remain above the horizon");
+      return false;
+    }
+  }
+  s.stokes_count = s.nmat * s.nx * s.ny * s.views;
+  s.path_count = s.nx * s.ny * s.nz * s.views;
+  s.tally_count = s.stokes_count + (s.paths ? s.path_count + s.nx * s.ny * s.views : 0);
+  return true;
+}
+
+template <typename R, bool TRACE>
+int launch(Session *h, const uint32_t *controls, double *out, float *records, size_t record_count,
+           char *error, size_t error_size) {
+  if (!h || !controls || !out) {
+    set_error(error, error_size, "Invalid HIP session or arguments");
+    return 1;
+  }
+  Controls c{controls[0], controls[1], controls[2], controls[3],
+             controls[4], controls[5], controls[6], controls[7]};
+  const Scene &s = h->scene;
+  const size_t per_round = size_t(s.tally_count) * size_t(s.shards);
+  if (!c.per_round || !c.rounds || c.rounds > h->round_capacity || !c.max_steps ||
+      per_round != h->tally_count || uint64_t(c.first) + c.count > uint64_t(c.per_round) * c.rounds) {
+    set_error(error, error_size, "Invalid HIP round group");
+    return 1;
+  }
+  if (!check(hipSetDevice(h->device), error, error_size, "hipSetDevice")) return 1;
+  const size_t active = per_round * c.rounds;
+  if (!check(hipMemsetAsync(h->banks, 0, active * sizeof(R), h->stream), error, error_size,
+             "clear tallies") ||
+      !check(hipMemsetAsync(h->failure, 0, sizeof(unsigned), h->stream), error, error_size,
+             "clear failure flag"))
+    return 1;
+  if (TRACE) {
+    if (record_count > h->record_capacity) {
+      (void)hipFree(h->records);
+      h->records = nullptr;
+      h->record_capacity = 0;
+      if (!check(hipMalloc(&h->records, record_count * kTraceWords * sizeof(float)), error,
+                 error_size, "allocate trace records"))
+        return 1;
+      h->record_capacity = record_count;
+    }
+  }
+  const int block = sizeof(R) == 4 ? kBlock32 : kBlock64;
+  const size_t lds = size_t(s.nz) * block * sizeof(R);
+  const uint32_t grid = (c.count + block - 1) / block;
+  R *banks = static_cast<R *>(h->banks);
+  (void)hipEventRecord(h->start, h->stream);
+  if (grid) {
+    hipLaunchKernelGGL((transport_kernel<R, TRACE>), dim3(grid), dim3(block), lds, h->stream, s, c,
+                       banks, h->failure, h->counters, TRACE ? h->records : nullptr,
+                       uint32_t(TRACE ? record_count : 0));
+  }
+  (void)hipEventRecord(h->stop, h->stream);
+  if (!check(hipGetLastError(), error, error_size, "transport launch")) return 1;
+  const size_t reduced = size_t(s.tally_count) * c.rounds;
+  hipLaunchKernelGGL(reduce_shards<R>, dim3((reduced + 255) / 256), dim3(256), 0, h->stream, banks,
+                     h->reduced, s.tally_count, s.shards, int(c.rounds));
+  if (!check(hipGetLastError(), error, error_size, "shard reduction launch")) return 1;
+  unsigned flag = 0;
+  if (!check(hipMemcpyAsync(&flag, h->failure, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream),
+             error, error_size, "copy failure flag") ||
+      !check(hipStreamSynchronize(h->stream), error, error_size, "transport"))
+    return 1;
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, h->start, h->stop);
+  h->kernel_seconds = ms * 1e-3;
+  if (c.profile) h->gpu_seconds += ms * 1e-3;
+  if (flag) {
+    if (error && error_size)
+      snprintf(error, error_size, "HIP transport failed (flag %u); round rejected", flag);
+    return 1;
+  }
+  if (!check(hipMemcpy(out, h->reduced, reduced * sizeof(double), hipMemcpyDeviceToHost), error,
+             error_size, "copy tallies"))
+    return 1;
+  if (TRACE && records && record_count) {
+    const size_t n = record_count < c.count ? record_count : c.count;
+    if (!check(hipMemcpy(records, h->records, n * kTraceWords * sizeof(float), hipMemcpyDeviceToHost),
+               error, error_size, "copy trace records"))
+      return 1;
+  }
+  return 0;
+}
+
+// Single-purpose probes of the device helpers, for tests against golden vectors.
+// Case: uo, u1, phio, phi1, depol, I, Q, U -> cos_sca, C1, S1, C2, S2, F11, F22, F33, F12 (/4),
+// and P*S (I, Q, U), evaluated in float32 exactly as transport does for a Rayleigh event.
+__global__ void probe_scatter_kernel(const double *in, double *out, int cases) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= cases) return;
+  const double *c = in + 8 * n;
+  const float uo = float(c[0]), u1 = float(c[1]), phio = float(c[2]), phi1 = float(c[3]);
+  const float cosine = clamp1(uo * u1 + sqrtf(fmaxf(0.0f, 1.0f - uo * uo)) *
+                                            sqrtf(fmaxf(0.0f, 1.0f - u1 * u1)) * cosf(phio - phi1));
+  Scene s{};
+  s.depol = float(c[4]);
+  s.nmat = 3;
+  uint32_t reads = 0;
+  const Elements<float> f = elements<float>(s, -1, cosine, reads);
+  const Rotation<float> r = rotation<float>(cosine, uo, u1, phio, phi1);
+  const Stokes<float> p = scatter<float>(f, r, Stokes<float>{float(c[5]), float(c[6]), float(c[7])});
+  double *o = out + 12 * n;
+  const float values[12] = {cosine, r.c1, r.s1, r.c2, r.s2, f.f11, f.f22, f.f33, f.f12, p.i, p.q, p.u};
+  for (int i = 0; i < 12; ++i) o[i] = double(values[i]);
+}
+
+__global__ void probe_philox_kernel(const uint32_t *in, uint32_t *out, int cases) {
+  const int n = blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= cases) return;
+  const uint32_t *c = in + 6 * n;
+  const Words w = philox(Words{c[0], c[1], c[2], c[3]}, c[4], c[5]);
+  out[4 * n + 0] = w.x;
+  out[4 * n + 1] = w.y;
+  out[4 * n + 2] = w.z;
+  out[4 * n + 3] = w.w;
+}
+
+template <typename T>
+int run_probe(uint32_t device, void (*kernel)(const T *, T *, int), const T *in, size_t in_words,
+              T *out, size_t out_words, int cases) {
+  int devices = 0;
+  if (hipGetDeviceCount(&devices) != hipSuccess || device >= uint32_t(devices)) return 1;
+  T *din = nullptr, *dout = nullptr;
+  int status = 1;
+  if (hipSetDevice(int(device)) == hipSuccess && hipMalloc(&din, in_words * sizeof(T)) == hipSuccess &&
+      hipMalloc(&dout, out_words * sizeof(T)) == hipSuccess &&
+      hipMemcpy(din, in, in_words * sizeof(T), hipMemcpyHostToDevice) == hipSuccess) {
+    hipLaunchKernelGGL(kernel, dim3((cases + 63) / 64), dim3(64), 0, 0, din, dout, cases);
+    if (hipGetLastError() == hipSuccess && hipDeviceSynchronize() == hipSuccess &&
+        hipMemcpy(out, dout, out_words * sizeof(T), hipMemcpyDeviceToHost) == hipSuccess)
+      status = 0;
+  }
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  return status;
+}
+
+}  // namespace
+
+extern "C" {
+
+int monki_hip_abi_version(void) { return 1; }
+
+void *monki_hip_create(uint32_t device, const float *tables, size_t table_count, size_t tally_count,
+                       uint32_t round_capacity, char *error, size_t error_size) {
+  if (!tables || !round_capacity) {
+    set_error(error, error_size, "Invalid HIP create arguments");
+    return nullptr;
+  }
+  Scene parsed{};
+  if (!parse_tables(tables, table_count, nullptr, parsed, error, error_size)) return nullptr;
+  const size_t per_round = size_t(parsed.tally_count) * size_t(parsed.shards);
+  if (tally_count != per_round) {
+    set_error(error, error_size, "Tally count does not match the packed tables");
+    return nullptr;
+  }
+  int devices = 0;
+  if (hipGetDeviceCount(&devices) != hipSuccess || device >= uint32_t(devices)) {
+    set_error(error, error_size, "Requested HIP device is unavailable");
+    return nullptr;
+  }
+  Session *h = new Session();
+  h->device = int(device);
+  h->tally_count = tally_count;
+  h->round_capacity = round_capacity;
+  hipDeviceProp_t prop{};
+  if (!check(hipSetDevice(h->device), error, error_size, "hipSetDevice") ||
+      !check(hipGetDeviceProperties(&prop, h->device), error, error_size, "device properties") ||
+      !check(hipStreamCreate(&h->stream), error, error_size, "stream") ||
+      !check(hipEventCreate(&h->start), error, error_size, "event") ||
+      !check(hipEventCreate(&h->stop), error, error_size, "event") ||
+      !check(hipMalloc(&h->tables, table_count * sizeof(float)), error, error_size, "allocate tables") ||
+      !check(hipMalloc(&h->banks, per_round * round_capacity * sizeof(double)), error, error_size,
+             "allocate tally banks") ||
+      !check(hipMalloc(&h->reduced, size_t(parsed.tally_count) * round_capacity * sizeof(double)),
+             error, error_size, "allocate reduced tallies") ||
+      !check(hipMalloc(&h->failure, sizeof(unsigned)), error, error_size, "allocate flag") ||
+      !check(hipMalloc(&h->counters, kCounters * sizeof(unsigned long long)), error, error_size,
+             "allocate counters") ||
+      !check(hipMemcpy(h->tables, tables, table_count * sizeof(float), hipMemcpyHostToDevice), error,
+             error_size, "upload tables") ||
+      !check(hipMemset(h->counters, 0, kCounters * sizeof(unsigned long long)), error, error_size,
+             "clear counters")) {
+    release(h);
+    return nullptr;
+  }
+  snprintf(h->name, sizeof(h->name), "%s (%s)", prop.name, prop.gcnArchName);
+  parse_tables(tables, table_count, h->tables, h->scene, nullptr, 0);
+  return h;
+}
+
+int monki_hip_run(void *handle, const uint32_t *controls, double *out, char *error, size_t error_size) {
+  return launch<float, false>(static_cast<Session *>(handle), controls, out, nullptr, 0, error,
+                              error_size);
+}
+
+int monki_hip_run_fp64(void *handle, const uint32_t *controls, double *out, char *error,
+                       size_t error_size) {
+  return launch<double, false>(static_cast<Session *>(handle), controls, out, nullptr, 0, error,
+                               error_size);
+}
+
+int monki_hip_trace(void *handle, const uint32_t *controls, uint32_t precision, double *out,
+                    float *records, size_t record_count, char *error, size_t error_size) {
+  if (!records || !record_count) {
+    set_error(error, error_size, "Trace needs a record buffer");
+    return 1;
+  }
+  if (precision == 32)
+    return launch<float, true>(static_cast<Session *>(handle), controls, out, records, record_count,
+                               error, error_size);
+  if (precision == 64)
+    return launch<double, true>(static_cast<Session *>(handle), controls, out, records,
+                                record_count, error, error_size);
+  set_error(error, error_size, "Trace precision must be 32 or 64");
+  return 1;
+}
+
+int monki_hip_counters(void *handle, uint64_t *out, size_t count) {
+  Session *h = static_cast<Session *>(handle);
+  if (!h || !out) return 1;
+  unsigned long long values[kCounters] = {0};
+  if (hipSetDevice(h->device) != hipSuccess ||
+      hipMemcpy(values, h->counters, sizeof(values), hipMemcpyDeviceToHost) != hipSuccess)
+    return 1;
+  for (size_t i = 0; i < count && i < size_t(kCounters); ++i) out[i] = values[i];
+  return 0;
+}
+
+double monki_hip_gpu_seconds(void *handle) {
+  return handle ? static_cast<Session *>(handle)->gpu_seconds : 0.0;
+}
+
+double monki_hip_kernel_seconds(void *handle) {
+  return handle ? static_cast<Session *>(handle)->kernel_seconds : 0.0;
+}
+
+const char *monki_hip_device_name(void *handle) {
+  return handle ? static_cast<Session *>(handle)->name : "";
+}
+
+int monki_hip_probe_scatter(uint32_t device, const double *cases, size_t count, double *out) {
+  if (!cases || !out || count > (1u << 20)) return 1;
+  return run_probe<double>(device, probe_scatter_kernel, cases, 8 * count, out, 12 * count, int(count));
+}
+
+int monki_hip_probe_philox(uint32_t device, const uint32_t *cases, size_t count, uint32_t *out) {
+  if (!cases || !out || count > (1u << 20)) return 1;
+  return run_probe<uint32_t>(device, probe_philox_kernel, cases, 6 * count, out, 4 * count, int(count));
+}
+
+void monki_hip_close(void *handle) { release(static_cast<Session *>(handle)); }
+
+}  // extern "C"

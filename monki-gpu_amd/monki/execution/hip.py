@@ -1,0 +1,421 @@
+"""MI355X transport backend: prepared tables resident in HBM, histories traced on the GPU.
+
+Counterpart of the reference ``Metal`` backend (``python/monki/execution/metal.py:23-349``):
+same settings validation before any device access, same session lifecycle (prepare replaces,
+close is idempotent, any failure closes the session), same grouping of equal rounds into one
+dispatch with independent random streams and tally banks, and the same ``RawTallies``
+output. The native side is ``libmonki_hip.so`` (``include/monki_hip.h``), compiled ahead of
+time for gfx950. There is no CPU transport fallback: without the library or a GPU,
+``prepare`` raises.
+"""
+
+import ctypes
+import hashlib
+import platform
+from dataclasses import replace
+from pathlib import Path
+from typing import List, Optional
+
+import numpy as np
+
+from ..input import ReferenceCase, SceneCase
+from ..metadata import file_hash, scene_metadata
+from .backend import Backend
+from .hip_tables import pack_tables, tally_count
+from .tallies import BatchControls, RawTallies
+
+PACKAGE_ROOT = Path(__file__).resolve().parents[2]
+LIBRARY = PACKAGE_ROOT / "lib" / "libmonki_hip.so"
+ABI_VERSION = 1
+TRACE_WORDS = 8
+COUNTER_NAMES = (
+    "photons",
+    "atmospheric_events",
+    "surface_events",
+    "walk_cells",
+    "detector_cells",
+    "table_reads",
+    "sampling_attempts",
+    "tally_atomics",
+)
+
+FLOAT_POINTER = ctypes.POINTER(ctypes.c_float)
+DOUBLE_POINTER = ctypes.POINTER(ctypes.c_double)
+UINT_POINTER = ctypes.POINTER(ctypes.c_uint32)
+U64_POINTER = ctypes.POINTER(ctypes.c_uint64)
+
+
+def load_library(path: Path = LIBRARY):
+    """Load libmonki_hip.so and declare its C-ABI; raise if it is missing or stale."""
+
+    if not Path(path).is_file():
+        raise FileNotFoundError(
+            f"{path} is missing: build it with python -c 'import __graft_entry__ as g; g.build()'"
+        )
+
+    native = ctypes.CDLL(str(path))
+    abi = getattr(native, "monki_hip_abi_version", None)
+
+    if abi is None or abi() != ABI_VERSION:
+        raise RuntimeError("Rebuild libmonki_hip.so: its ABI version does not match")
+
+    native.monki_hip_create.argtypes = [
+        ctypes.c_uint32,
+        FLOAT_POINTER,
+        ctypes.c_size_t,
+        ctypes.c_size_t,
+        ctypes.c_uint32,
+        ctypes.c_char_p,
+        ctypes.c_size_t,
+    ]
+    native.monki_hip_create.restype = ctypes.c_void_p
+
+    for name in ("monki_hip_run", "monki_hip_run_fp64"):
+        function = getattr(native, name)
+        function.argtypes = [ctypes.c_void_p, UINT_POINTER, DOUBLE_POINTER, ctypes.c_char_p, ctypes.c_size_t]
+        function.restype = ctypes.c_int
+
+    native.monki_hip_trace.argtypes = [
+        ctypes.c_void_p,
+        UINT_POINTER,
+        ctypes.c_uint32,
+        DOUBLE_POINTER,
+        FLOAT_POINTER,
+        ctypes.c_size_t,
+        ctypes.c_char_p,
+        ctypes.c_size_t,
+    ]
+    native.monki_hip_trace.restype = ctypes.c_int
+    native.monki_hip_counters.argtypes = [ctypes.c_void_p, U64_POINTER, ctypes.c_size_t]
+    native.monki_hip_counters.restype = ctypes.c_int
+    native.monki_hip_gpu_seconds.argtypes = [ctypes.c_void_p]
+    native.monki_hip_gpu_seconds.restype = ctypes.c_double
+    native.monki_hip_kernel_seconds.argtypes = [ctypes.c_void_p]
+    native.monki_hip_kernel_seconds.restype = ctypes.c_double
+    native.monki_hip_device_name.argtypes = [ctypes.c_void_p]
+    native.monki_hip_device_name.restype = ctypes.c_char_p
+    native.monki_hip_close.argtypes = [ctypes.c_void_p]
+    native.monki_hip_close.restype = None
+    return native
+
+
+class HIP(Backend):
+    """Retain device tables across rounds; all photon histories execute on the GPU.
+
+    Transport is float32 (``precision=64`` runs the same kernels in float64 on the same
+    random streams); per-round shard reduction is float64 on the device; cross-round
+    statistics are float64 on the host. One caller per instance.
+    """
+
+    def __init__(
+        self,
+        device: int = 0,
+        *,
+        save_path: bool = True,
+        profile: bool = False,
+        seed: int = 23274,
+        photons_per_dispatch: int = 1 << 24,
+        tally_shards: int = 32,
+        max_steps: int = 100_000,
+        precision: int = 32,
+        library: Path = LIBRARY,
+    ):
+        for name, value, minimum, maximum in (
+            ("device", device, 0, 2**32 - 1),
+            ("seed", seed, 0, 2**32 - 1),
+            ("tally_shards", tally_shards, 1, 64),
+            ("photons_per_dispatch", photons_per_dispatch, 1, 1 << 30),
+            ("max_steps", max_steps, 1, 100_000),
+        ):
+            if type(value) is not int or not minimum <= value <= maximum:
+                raise ValueError(f"{name} must be an integer in [{minimum}, {maximum}]")
+
+        for name, value in (("save_path", save_path), ("profile", profile)):
+            if type(value) is not bool:
+                raise ValueError(f"{name} must be a bool")
+
+        if precision not in (32, 64) or type(precision) is not int:
+            raise ValueError("precision must be 32 or 64")
+
+        self.device = device
+        self.save_path = save_path
+        self.profile = profile
+        self.seed = seed
+        self.photons_per_dispatch = photons_per_dispatch
+        self.tally_shards = tally_shards
+        self.max_steps = max_steps
+        self.precision = precision
+        self.library = Path(library)
+        self.metadata = None
+        self.provenance: dict = {}
+        self._native = None
+        self._handle = None
+        self._shape = (0, 0, 0, 0, 0)
+        self._configuration = None
+        self._round = 0
+        self._tally_count = 0
+        self._round_capacity = 1
+        self._packed: Optional[np.ndarray] = None
+
+    # -- lifecycle ----------------------------------------------------------------------
+
+    def prepare(self, scene: object) -> None:
+        """Pack and upload the scene tables; reserve per-round tally banks on the device."""
+
+        self.close()
+
+        if isinstance(scene, ReferenceCase):
+            raise NotImplementedError(
+                "HIP prepares in-memory transport tables (SceneCase); optical preparation of "
+                "legacy positional fixtures is not part of this backend"
+            )
+
+        if not isinstance(scene, SceneCase):
+            raise TypeError("HIP.prepare expects a SceneCase")
+
+        if scene.backward:
+            raise NotImplementedError("HIP backward transport is not implemented")
+
+        tables = replace(scene.scene, save_path=self.save_path)
+        packed, self._shape = pack_tables(tables, shards=self.tally_shards)
+        nmat, nx, ny, nz, views = self._shape
+        per_round = tally_count(nmat, nx, ny, nz, views, self.save_path)
+        self._tally_count = per_round
+        bank_bytes = per_round * self.tally_shards * (4 if self.precision == 32 else 8)
+        self._round_capacity = max(
+            1,
+            min(
+                32,
+                self.photons_per_dispatch // (scene.photons_per_column * nx * ny),
+                (256 * 1024 * 1024) // bank_bytes,
+            ),
+        )
+        native = load_library(self.library)
+        error = ctypes.create_string_buffer(8192)
+        handle = native.monki_hip_create(
+            self.device,
+            packed.ctypes.data_as(FLOAT_POINTER),
+            packed.size,
+            per_round * self.tally_shards,
+            self._round_capacity,
+            error,
+            len(error),
+        )
+
+        if not handle:
+            raise RuntimeError(error.value.decode() or "HIP session creation failed")
+
+        self._native = native
+        self._handle = handle
+        self._packed = packed
+        self.metadata = scene_metadata(tables, scene.photons_per_column)
+        self._configuration = self._settings()
+        self._round = 0
+        self.provenance = {
+            "backend": "HIP",
+            "device": native.monki_hip_device_name(handle).decode(),
+            "case": scene.name,
+            "scene": tables.description,
+            "rng": "Philox4x32-10; seed/round/photon/draw-block",
+            "seed": self.seed,
+            "transport_precision": f"float{self.precision}",
+            "round_precision": "float64",
+            "photons_per_dispatch": self.photons_per_dispatch,
+            "tally_shards": self.tally_shards,
+            "round_capacity": self._round_capacity,
+            "max_steps": self.max_steps,
+            "library_sha256": file_hash(self.library),
+            "tables_sha256": hashlib.sha256(packed.tobytes()).hexdigest(),
+            "python_version": platform.python_version(),
+            "gpu_dispatches": 0,
+        }
+
+    def run_batch(self, controls: object) -> RawTallies:
+        """Execute exactly one complete statistical round."""
+
+        return self._run_group([controls])[0]
+
+    def run_batches(self, controls: list) -> list:
+        """Schedule equal rounds together with independent accumulators (metal.py:219-242)."""
+
+        self._validate_requests(controls)
+        results = []
+        first = 0
+        _, nx, ny, _, _ = self._shape
+
+        while first < len(controls):
+            photons = controls[first].photons_per_column * nx * ny
+            capacity = max(1, min(self._round_capacity, self.photons_per_dispatch // photons))
+            last = first + 1
+
+            while last < min(len(controls), first + capacity):
+                if controls[last].photons_per_column != controls[first].photons_per_column:
+                    break
+
+                last += 1
+
+            results.extend(self._run_group(controls[first:last]))
+            first = last
+
+        return results
+
+    def close(self) -> None:
+        """Release device buffers; returned arrays stay valid."""
+
+        if self._handle is not None and self._native is not None:
+            self._native.monki_hip_close(self._handle)
+
+        self._handle = None
+        self._native = None
+        self.metadata = None
+        self._configuration = None
+
+    # -- diagnostics --------------------------------------------------------------------
+
+    def counters(self) -> dict:
+        """Event counters accumulated since prepare (monotonic)."""
+
+        self._require_session()
+        values = (ctypes.c_uint64 * len(COUNTER_NAMES))()
+
+        if self._native.monki_hip_counters(self._handle, values, len(values)):
+            raise RuntimeError("HIP counter readback failed")
+
+        return dict(zip(COUNTER_NAMES, (int(v) for v in values)))
+
+    def kernel_seconds(self) -> float:
+        """Transport-kernel seconds of the last dispatch (HIP events on its stream)."""
+
+        self._require_session()
+        return float(self._native.monki_hip_kernel_seconds(self._handle))
+
+    def trace(self, controls: BatchControls, records: int, precision: int = 32):
+        """Run one round and return (RawTallies-like totals, per-photon records).
+
+        Records hold, per photon: atmospheric events, surface events, end status
+        (1 escaped, 2 absorbed, 3 failed), final weight, sum of Stokes-I contributions
+        times mu0/mu, sum of detector weights, sum of weighted path contributions, last layer.
+        Does not advance the session's round counter.
+        """
+
+        self._require_session()
+        _, nx, ny, _, _ = self._shape
+        per_round = controls.photons_per_column * nx * ny
+        count = per_round
+        command = (ctypes.c_uint32 * 8)(
+            count, 0, controls.round_index, self.seed, self.max_steps, 0, per_round, 1
+        )
+        out = np.zeros(self._tally_count, dtype=np.float64)
+        buffer = np.zeros((records, TRACE_WORDS), dtype=np.float32)
+        error = ctypes.create_string_buffer(8192)
+        status = self._native.monki_hip_trace(
+            self._handle,
+            command,
+            precision,
+            out.ctypes.data_as(DOUBLE_POINTER),
+            buffer.ctypes.data_as(FLOAT_POINTER),
+            records,
+            error,
+            len(error),
+        )
+
+        if status:
+            raise RuntimeError(error.value.decode())
+
+        return out, buffer[: min(records, count)]
+
+    # -- internals ----------------------------------------------------------------------
+
+    def _require_session(self):
+        if self._handle is None or self._native is None:
+            raise RuntimeError("HIP must be prepared before running a batch")
+
+    def _validate_requests(self, controls: list) -> None:
+        self._require_session()
+
+        if self._settings() != self._configuration:
+            raise ValueError("HIP settings changed; reprepare before running another batch")
+
+        _, nx, ny, _, _ = self._shape
+
+        for offset, request in enumerate(controls, start=1):
+            if not isinstance(request, BatchControls):
+                raise TypeError("run_batch expects BatchControls")
+
+            if request.round_index != self._round + offset:
+                raise ValueError("round_index must be sequential, starting at one")
+
+            if request.photons_per_column * nx * ny * len(controls) >= 2**32:
+                raise ValueError("HIP photon identity exceeds its 32-bit round range")
+
+    def _run_group(self, controls: list) -> List[RawTallies]:
+        """Trace a group of equal rounds and split the float64 totals per round."""
+
+        self._validate_requests(controls)
+        nmat, nx, ny, nz, views = self._shape
+        per_round = controls[0].photons_per_column * nx * ny
+        photon_count = per_round * len(controls)
+        total = np.zeros((len(controls), self._tally_count), dtype=np.float64)
+        scratch = np.empty_like(total)
+        error = ctypes.create_string_buffer(8192)
+        run = self._native.monki_hip_run if self.precision == 32 else self._native.monki_hip_run_fp64
+
+        try:
+            for first in range(0, photon_count, self.photons_per_dispatch):
+                count = min(self.photons_per_dispatch, photon_count - first)
+                command = (ctypes.c_uint32 * 8)(
+                    count,
+                    first,
+                    controls[0].round_index,
+                    self.seed,
+                    self.max_steps,
+                    int(self.profile),
+                    per_round,
+                    len(controls),
+                )
+                status = run(self._handle, command, scratch.ctypes.data_as(DOUBLE_POINTER), error, len(error))
+
+                if status or not np.all(np.isfinite(scratch)):
+                    raise RuntimeError(error.value.decode() or "Nonfinite HIP tallies")
+
+                total += scratch
+                self.provenance["gpu_dispatches"] += 1
+
+            if self.profile:
+                self.provenance["gpu_seconds"] = self._native.monki_hip_gpu_seconds(self._handle)
+
+            results = []
+            stokes_count = nmat * nx * ny * views
+
+            for request, values in zip(controls, total):
+                stokes = values[:stokes_count].reshape((nmat, nx, ny, views), order="F").copy()
+                paths = weights = None
+
+                if self.save_path:
+                    path_count = nx * ny * nz * views
+                    paths = (
+                        values[stokes_count : stokes_count + path_count]
+                        .reshape((nx, ny, nz, views), order="F")
+                        .copy()
+                    )
+                    weights = values[stokes_count + path_count :].reshape((nx, ny, views), order="F").copy()
+
+                results.append(RawTallies(request, stokes, paths, weights))
+
+            self._round = controls[-1].round_index
+            return results
+        except BaseException:
+            self.close()
+            raise
+
+    def _settings(self) -> tuple:
+        return (
+            self.device,
+            self.save_path,
+            self.seed,
+            self.photons_per_dispatch,
+            self.max_steps,
+            self.profile,
+            self.tally_shards,
+            self.precision,
+        )

@@ -1,0 +1,145 @@
+"""Validate and pack prepared transport tables into the float32 layout of ``libmonki_hip``.
+
+Counterpart of the reference's ``python/monki/execution/metal_tables.py:6-133``: a
+versioned float32 buffer with a fixed header, then the tables. Differences: several
+tabulated species (cumulative extinction fractions per species, gas last and implicit),
+explicit offsets in the header instead of a fixed order, and scene constants (sizes,
+geometry, shard count) as header words rather than compiled-in defines.
+
+Header (64 float32 words; integers are stored exactly, all < 2**24):
+
+====  =====================================================================
+0     layout version (1)
+1-9   nx, ny, nz (incl. vacuum), nmat, views, species, paths, uniform_top, shards
+10-15 xunit, yunit (km), depol, propagation zenith, solar azimuth, view azimuth (rad)
+16-18 view zeniths (rad)
+19-25 offsets: zunit, albedo, beta, cumulative fractions, gas albedo,
+      height above, optical depth above
+26    total buffer length
+32+8s species s: LUT size, albedo, offsets of cosine block (cos, 1-cos, 1+cos),
+      CDF, F11, F22, F33, F12
+====  =====================================================================
+
+Grids are x-fastest (Fortran order). ``height_above[k]``/``tau_above[k]`` are the thickness
+and vertical optical depth of the physical layers above layer k, integrated from the same
+float32 values the kernel reads (``metal_tables.py:93-107``); ``uniform_top`` is one above the
+highest layer whose extinction or species mix varies across columns.
+"""
+
+import numpy as np
+
+HEADER_WORDS = 64
+LAYOUT_VERSION = 1
+MAX_LAYERS = 64
+MAX_SPECIES = 4
+MAX_VIEWS = 3
+MAX_OFFSET = 2**24
+
+
+def pack_tables(scene, shards: int = 32):
+    """Return (float32 buffer, (nmat, nx, ny, nz, views)) for ``monki_hip_create``."""
+
+    scene.validate()
+    nx, ny, nz, nmat = scene.nx, scene.ny, scene.nz, scene.nmat
+    views = scene.angles
+    nspecies = len(scene.species)
+
+    if nz > MAX_LAYERS:
+        raise ValueError(f"at most {MAX_LAYERS} layers including vacuum are supported")
+
+    if nspecies > MAX_SPECIES:
+        raise NotImplementedError(f"at most {MAX_SPECIES} tabulated species are supported")
+
+    if views > MAX_VIEWS:
+        raise ValueError(f"at most {MAX_VIEWS} view zenith angles are supported")
+
+    if type(shards) is not int or not 1 <= shards <= 64:
+        raise ValueError("shards must be an integer in [1, 64]")
+
+    beta32 = scene.beta.astype(np.float32)
+    z32 = scene.zunit.astype(np.float32)
+    by_layer = beta32.reshape(nx * ny, nz, order="F")
+    varying = scene.varying_layers()
+    uniform_top = int(varying[-1] + 1) if varying.size else 0
+    height_above = np.zeros(nz)
+    tau_above = np.zeros(nz)
+
+    for layer in range(nz - 3, -1, -1):
+        height_above[layer] = height_above[layer + 1] + float(z32[layer + 1])
+        tau_above[layer] = tau_above[layer + 1] + float(z32[layer + 1]) * float(by_layer[0, layer + 1])
+
+    cumulative = np.cumsum(scene.fractions[:nspecies], axis=0)
+    parts = []
+    offset = HEADER_WORDS
+
+    def place(array):
+        nonlocal offset
+        array = np.asarray(array, dtype=np.float64).ravel(order="F")
+        start = offset
+        parts.append(array)
+        offset += array.size
+        return start
+
+    header = np.zeros(HEADER_WORDS)
+    header[0] = LAYOUT_VERSION
+    header[1:10] = [nx, ny, nz, nmat, views, nspecies, int(scene.save_path), uniform_top, shards]
+    header[10:16] = [
+        scene.xunit,
+        scene.yunit,
+        scene.depol,
+        np.deg2rad(scene.propagation_zenith),
+        np.deg2rad(scene.solar_azimuth),
+        np.deg2rad(scene.view_azimuth),
+    ]
+    header[16 : 16 + views] = np.deg2rad(scene.view_zeniths)
+    header[19] = place(scene.zunit)
+    header[20] = place(scene.albedo)
+    header[21] = place(scene.beta)
+    header[22] = place(np.moveaxis(cumulative, 0, -1)) if nspecies else offset
+    header[23] = place(scene.ssalb_gas)
+    header[24] = place(height_above)
+    header[25] = place(tau_above)
+
+    for index, sp in enumerate(scene.species):
+        cdf = sp.bound_cdf(nmat) if sp.cdf is None else sp.cdf
+        base = 32 + 8 * index
+        header[base : base + 2] = [sp.cosines.size, sp.ssalb]
+        # Cosine block: cos, then 1 - cos and 1 + cos, which keep float32 precision near the
+        # forward and backward peaks where 1 - cos^2 would cancel.
+        header[base + 2] = place(np.concatenate([sp.cosines, 1.0 - sp.cosines, 1.0 + sp.cosines]))
+        header[base + 3] = place(cdf)
+        header[base + 4] = place(sp.f11)
+        header[base + 5] = place(sp.f22)
+        header[base + 6] = place(sp.f33)
+        header[base + 7] = place(sp.f12)
+
+    header[26] = offset
+
+    if offset >= MAX_OFFSET:
+        raise ValueError("scene tables exceed the 2**24-word float32 header range")
+
+    packed = np.concatenate([header, *parts]).astype(np.float32)
+
+    if not np.all(np.isfinite(packed)):
+        raise ValueError("Transport tables overflow float32 precision")
+
+    for sp in scene.species:
+        if np.any(np.diff(sp.cosines.astype(np.float32)) <= 0):
+            raise ValueError("Scattering angles are not distinct at float32 precision")
+
+    if np.any(np.cos(np.deg2rad(np.asarray(scene.view_zeniths)).astype(np.float32)) <= 0):
+        raise ValueError("Viewing angles must remain above the horizon in float32")
+
+    assert cumulative.shape == (nspecies, nx, ny, nz) and packed.size == offset
+    return packed, (nmat, nx, ny, nz, views)
+
+
+def tally_count(nmat: int, nx: int, ny: int, nz: int, views: int, paths: bool) -> int:
+    """Float count of one round's tallies: Stokes, then paths and weights when saved."""
+
+    count = nmat * nx * ny * views
+
+    if paths:
+        count += nx * ny * (nz + 1) * views
+
+    return count

@@ -1,0 +1,535 @@
+/*
+ * CPU oracle for MONKI photon transport -- TEST INFRASTRUCTURE ONLY.
+ *
+ * This file is the checker, never the product: only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load it. It restates Victor Trees's Fortran transport in
+ * float64 C, routine by routine, with the Philox4x32-10 draw order of the reference GPU kernel
+ * (kernels/metal/transport.metal) so that histories can be compared photon by photon:
+ *
+ *   launch ............................ mc.f90:351-464
+ *   free path, event vs face ........... mc.f90:476-677, ADVANCE_internal mc.f90:1531-1580
+ *   Lambertian surface ................. mc.f90:681-1007, GROUND_LAMB2 general_mod:31-65
+ *   species choice, absorption ......... mc.f90:1028-1074
+ *   local estimate per view ............ mc.f90:1109-1207
+ *   detector walk, register mapping .... mc_routines_cartesian_mod.f90:849-1222, ADVANCE :1233
+ *   Rayleigh matrix .................... mc_routines_general_mod.f90:74-136
+ *   tabulated matrix, interpolation .... mc_routines_general_mod.f90:257-388, interpol_sp.f:105
+ *   direction sampling ................. mc_routines_general_mod.f90:492-932
+ *   rotation matrices .................. mc_routines_general_mod.f90:942-1242
+ *   mu0/mu scaling ..................... mc.f90:1456-1474
+ *
+ * Differences from the Fortran, all deliberate: Philox streams instead of RANDOM_NUMBER
+ * (the Fortran's compiler RNG has no counter form); tabulated-element brackets by binary
+ * search (equal to the equal-angle index estimate of mie_scat_matrix on equal-angle grids);
+ * the sampling loop is bounded at 4096 attempts (transport.metal:184) instead of unbounded.
+ * The detector walk is always explicit, cell by cell, with the analytical pixel mapping
+ * of mc_routines_cartesian_mod.f90:1076-1105: no uniform-top shortcut.
+ *
+ * Input is the packed float32 table buffer of the product (layout v1), widened to double.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define MAX_LAYERS 64
+#define MAX_SPECIES 4
+#define MAX_VIEWS 3
+#define HEADER 64
+#define EPS 1.0e-16
+#define PI 3.14159265358979323846
+#define TRACE_WORDS 8
+#define ATTEMPTS 4096
+
+typedef struct {
+  int lut;
+  double ssalb;
+  const float *cosine, *cdf, *f11, *f22, *f33, *f12;
+} species_t;
+
+typedef struct {
+  int nx, ny, nz, nmat, views, species, paths;
+  double xunit, yunit, depol, theta0, phi0, phi;
+  double view[MAX_VIEWS];
+  const float *zunit, *albedo, *beta, *cumfrac, *ssalb_gas;
+  species_t sp[MAX_SPECIES];
+  int stokes_count, path_count, tally_count;
+} scene_t;
+
+/* ---- Philox4x32-10 (Random123 constants) ------------------------------------------- */
+void oracle_philox(const uint32_t counter[4], const uint32_t key[2], uint32_t out[4]) {
+  uint32_t x = counter[0], y = counter[1], z = counter[2], w = counter[3];
+  uint32_t k0 = key[0], k1 = key[1];
+  for (int i = 0; i < 10; ++i) {
+    uint64_t p0 = (uint64_t)0xD2511F53u * x, p1 = (uint64_t)0xCD9E8D57u * z;
+    uint32_t nx = (uint32_t)(p1 >> 32) ^ y ^ k0, ny = (uint32_t)p1;
+    uint32_t nz = (uint32_t)(p0 >> 32) ^ w ^ k1, nw = (uint32_t)p0;
+    x = nx, y = ny, z = nz, w = nw;
+    k0 += 0x9E3779B9u, k1 += 0xBB67AE85u;
+  }
+  out[0] = x, out[1] = y, out[2] = z, out[3] = w;
+}
+
+typedef struct {
+  uint32_t photon, round, seed, stream, block, words[4];
+  int lane;
+} rng_t;
+
+static double draw(rng_t *r) {
+  if (r->lane == 4) {
+    uint32_t c[4] = {r->photon, r->round, r->block++, r->stream}, k[2] = {r->seed, 0u};
+    oracle_philox(c, k, r->words);
+    r->lane = 0;
+  }
+  return ((double)(r->words[r->lane++] >> 9) + 0.5) * 0x1p-23;
+}
+
+/* ---- Scattering matrices ------------------------------------------------------------ */
+/* Elements F11, F22, F33, F12 divided by 4; species < 0 is Rayleigh (ray_scat_matrix). */
+void oracle_rayleigh(double depol, double c, double f[4]) {
+  double s2 = 1.0 - c * c;
+  if (s2 < EPS) s2 = 0.0;
+  double delta = (1.0 - depol) / (1.0 + depol / 2.0);
+  f[0] = (delta * 0.75 * (1.0 + c * c) + (1.0 - delta)) / 4.0;
+  f[1] = delta * 0.75 * (1.0 + c * c) / 4.0;
+  f[2] = delta * 1.5 * c / 4.0;
+  f[3] = -delta * 0.75 * s2 / 4.0;
+}
+
+/* linpol_dp_single on the bracketing pair (interpol_sp.f:105-152). */
+static double lerp2(double x0, double x1, double y0, double y1, double x) {
+  if (x == x1) return y1;
+  if (x == x0) return y0;
+  return (y1 - y0) / (x1 - x0) * (x - x0) + y0;
+}
+
+static int bracket_cos(const float *axis, int n, double v) {
+  int lo = 0, hi = n - 1;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) / 2;
+    if ((double)axis[mid] <= v)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+void oracle_tabulated(const float *cosine, const float *f11, const float *f22, const float *f33,
+                      const float *f12, int n, double c, double f[4]) {
+  if (c < -1.0) c = -1.0;
+  if (c > 1.0) c = 1.0;
+  int lo = bracket_cos(cosine, n, c);
+  double x0 = cosine[lo], x1 = cosine[lo + 1];
+  f[0] = lerp2(x0, x1, f11[lo], f11[lo + 1], c) / 4.0;
+  f[1] = lerp2(x0, x1, f22[lo], f22[lo + 1], c) / 4.0;
+  f[2] = lerp2(x0, x1, f33[lo], f33[lo + 1], c) / 4.0;
+  f[3] = lerp2(x0, x1, f12[lo], f12[lo + 1], c) / 4.0;
+}
+
+static void elements(const scene_t *s, int species, double c, double f[4]) {
+  if (species < 0) {
+    if (c < -1.0) c = -1.0;
+    if (c > 1.0) c = 1.0;
+    oracle_rayleigh(s->depol, c, f);
+  } else {
+    const species_t *t = &s->sp[species];
+    oracle_tabulated(t->cosine, t->f11, t->f22, t->f33, t->f12, t->lut, c, f);
+  }
+  if (s->nmat == 1) f[3] = 0.0;
+}
+
+/* stokes_rot_mat: out = (C1, S1, C2, S2) = (L1(2,2), L1(3,2), L2(2,2), L2(3,2)). */
+void oracle_rotation(double cos_sca, double uo, double u1, double phio, double phi1, double out[4]) {
+  double dphi = fabs(phio - phi1), rem = fmod(dphi, PI);
+  out[0] = 1.0, out[1] = 0.0, out[2] = 1.0, out[3] = 0.0;
+  if (rem < EPS || fabs(rem - PI) < EPS) return;
+  if (fabs(uo) > 1.0 - EPS && fabs(u1) > 1.0 - EPS) return;
+  double fac = 1.0 - uo * uo, sino = fac < EPS ? 0.0 : sqrt(fac);
+  fac = 1.0 - u1 * u1;
+  double sin1 = fac < EPS ? 0.0 : sqrt(fac);
+  double wo[2] = {sino * cos(phio), sino * sin(phio)}, w1[2] = {sin1 * cos(phi1), sin1 * sin(phi1)};
+  double cosi1, cosi2, q;
+  if (fabs(uo) > 1.0 - EPS || fabs(u1) > 1.0 - EPS) {
+    double wop[2] = {cos(phio), sin(phio)}, w1p[2] = {cos(phi1), sin(phi1)};
+    if (fabs(u1) > 1.0 - EPS) {
+      cosi1 = u1 >= 0 ? 1.0 : -1.0;
+      cosi2 = -cosi1 * (w1p[0] * wop[0] + w1p[1] * wop[1]);
+      q = wo[1] * w1p[0] - wo[0] * w1p[1];
+    } else {
+      cosi2 = uo >= 0 ? 1.0 : -1.0;
+      cosi1 = -cosi2 * (w1p[0] * wop[0] + w1p[1] * wop[1]);
+      q = wop[1] * w1[0] - wop[0] * w1[1];
+    }
+  } else {
+    double a = (1.0 - uo * uo) * (1.0 - cos_sca * cos_sca);
+    double b = (1.0 - u1 * u1) * (1.0 - cos_sca * cos_sca);
+    cosi1 = (u1 - uo * cos_sca) / sqrt(a > EPS ? a : EPS);
+    cosi2 = (uo - u1 * cos_sca) / sqrt(b > EPS ? b : EPS);
+    q = wo[1] * w1[0] - wo[0] * w1[1];
+  }
+  double sign = q >= 0 ? 1.0 : -1.0;
+  fac = 1.0 - cosi1 * cosi1;
+  double sini1 = fac < EPS ? 0.0 : sqrt(fac) * sign;
+  fac = 1.0 - cosi2 * cosi2;
+  double sini2 = fac < EPS ? 0.0 : sqrt(fac) * sign;
+  out[0] = cosi1 * cosi1 - sini1 * sini1;
+  out[1] = 2.0 * sini1 * cosi1;
+  out[2] = cosi2 * cosi2 - sini2 * sini2;
+  out[3] = 2.0 * sini2 * cosi2;
+}
+
+/* matmul_stokes_rot then P * S for I, Q, U (mc_routines_general_mod.f90:1184-1242). */
+void oracle_scatter(const double f[4], const double r[4], const double s[3], double out[3]) {
+  double C1 = r[0], S1 = r[1], C2 = r[2], S2 = r[3];
+  double F11 = f[0], F22 = f[1], F33 = f[2], F12 = f[3];
+  double P[3][3] = {{F11, F12 * C1, -F12 * S1},
+                    {C2 * F12, C1 * C2 * F22 - S1 * S2 * F33, -C2 * S1 * F22 - C1 * S2 * F33},
+                    {S2 * F12, C1 * S2 * F22 + C2 * S1 * F33, C1 * C2 * F33 - S1 * S2 * F22}};
+  for (int i = 0; i < 3; ++i) out[i] = P[i][0] * s[0] + P[i][1] * s[1] + P[i][2] * s[2];
+}
+
+/* Spherical trigonometry of rayl_scatter_pol / mie_scatter_pol_bound after acceptance. */
+void oracle_new_direction(double cos_in, double cos_sca, double sigma, double *cos_out, double *dphi) {
+  double a = 1.0 - cos_in * cos_in, b = 1.0 - cos_sca * cos_sca;
+  double sin_in = sqrt(a > EPS ? a : EPS), sin_sca = sqrt(b > EPS ? b : EPS);
+  double co = cos_in * cos_sca + sin_in * sin_sca * cos(sigma);
+  if (co > 1.0) co = 1.0;
+  if (co < -1.0) co = -1.0;
+  double c = 1.0 - co * co, sin_out = sqrt(c > EPS ? c : EPS);
+  double frac = (cos_sca - cos_in * co) / (sin_in * sin_out);
+  double d;
+  if (frac >= 1.0 - EPS)
+    d = 0.0;
+  else if (frac <= -1.0 + EPS)
+    d = PI;
+  else
+    d = acos(frac) * (sigma >= 0 ? 1.0 : -1.0);
+  *cos_out = co;
+  *dphi = -d;
+}
+
+/* Rejection sampling, forward sign -1. Returns 0 after ATTEMPTS rejections. */
+static int sample(const scene_t *s, int species, const double st[3], double cos_in, rng_t *r,
+                  double *cos_out, double *dphi, double *cos_sca, uint64_t *attempts) {
+  double bound_ray[4];
+  oracle_rayleigh(s->depol, 1.0, bound_ray);
+  for (int attempt = 0; attempt < ATTEMPTS; ++attempt) {
+    ++*attempts;
+    double c;
+    if (species < 0) {
+      c = 2.0 * draw(r) - 1.0;
+    } else {
+      const species_t *t = &s->sp[species];
+      double u = draw(r);
+      int lo = bracket_cos(t->cdf, t->lut, u);
+      c = lerp2(t->cdf[lo], t->cdf[lo + 1], t->cosine[lo], t->cosine[lo + 1], u);
+    }
+    double sigma = 2.0 * PI * draw(r) - PI;
+    double f[4];
+    elements(s, species, c, f);
+    double pdf = f[0] + f[3] * (st[1] * cos(2.0 * sigma) - st[2] * sin(2.0 * sigma));
+    double bound = species < 0 ? bound_ray[0] : f[0] + fabs(f[3]);
+    if (draw(r) * bound > pdf) continue;
+    *cos_sca = c;
+    oracle_new_direction(cos_in, c, sigma, cos_out, dphi);
+    return 1;
+  }
+  return 0;
+}
+
+/* ---- Geometry ----------------------------------------------------------------------- */
+typedef struct {
+  int i, j, k;
+  double x, y, z, a, b, c;
+} pos_t;
+
+static double advance(const scene_t *s, pos_t *p) {
+  double aa = fabs(p->a), ab = fabs(p->b), ac = fabs(p->c);
+  double dx = p->x / (aa > EPS ? aa : EPS), dy = p->y / (ab > EPS ? ab : EPS);
+  double dz = p->z / (ac > EPS ? ac : EPS);
+  double d = fmin(dx, fmin(dy, dz));
+  if (dx == d) {
+    p->i = ((p->i + (p->a >= 0 ? 1 : -1)) % s->nx + s->nx) % s->nx;
+    p->x = s->xunit;
+    p->y = fmax(0.0, p->y - ab * d);
+    p->z = fmax(0.0, p->z - ac * d);
+  } else if (dy == d) {
+    p->j = ((p->j + (p->b >= 0 ? 1 : -1)) % s->ny + s->ny) % s->ny;
+    p->x = fmax(0.0, p->x - aa * d);
+    p->y = s->yunit;
+    p->z = fmax(0.0, p->z - ac * d);
+  } else {
+    p->k += p->c >= 0 ? 1 : -1;
+    p->x = fmax(0.0, p->x - aa * d);
+    p->y = fmax(0.0, p->y - ab * d);
+    if (p->k >= 0 && p->k < s->nz) p->z = s->zunit[p->k];
+  }
+  return d;
+}
+
+static int cell(const scene_t *s, int i, int j, int k) { return i + s->nx * (j + s->ny * k); }
+
+typedef struct {
+  double *tallies;
+  double *record;
+  uint64_t *counters;
+} sink_t;
+
+/* calc_radiances_and_paths_pol for one event and all views. Returns 0 on a walk limit. */
+static int detect(const scene_t *s, const pos_t *ev, double prob[MAX_VIEWS][3],
+                  const double *history, uint32_t max_steps, sink_t *out) {
+  for (int v = 0; v < s->views; ++v) {
+    if (!(prob[v][0] > EPS)) continue;
+    double th = s->view[v];
+    double va = cos(s->phi) * sin(th), vb = sin(s->phi) * sin(th), vc = cos(th);
+    pos_t r = *ev;
+    r.a = va, r.b = vb, r.c = vc;
+    if (va * ev->a < 0) r.x = fmax(0.0, s->xunit - r.x);
+    if (vb * ev->b < 0) r.y = fmax(0.0, s->yunit - r.y);
+    if (vc * ev->c < 0) r.z = fmax(0.0, s->zunit[r.k] - r.z);
+    double x_old = r.x, y_old = r.y, z_old = r.z;
+    double tau = 0.0, lengths[MAX_LAYERS] = {0};
+    uint32_t steps = 0;
+    while (r.k >= 0 && r.k < s->nz - 1) {
+      if (++steps > max_steps) return 0;
+      int k_old = r.k;
+      double beta = s->beta[cell(s, r.i, r.j, r.k)];
+      double d = advance(s, &r);
+      tau += d * beta;
+      lengths[k_old] += d;
+      ++out->counters[4];
+    }
+    /* Analytical register mapping (mc_routines_cartesian_mod.f90:1076-1105), TOA grid. */
+    double height = 0.0;
+    for (int k = 0; k < ev->k; ++k) height += s->zunit[k];
+    height += (vc > 0 ? s->zunit[ev->k] - z_old : z_old);
+    double top = 0.0;
+    for (int k = 0; k < s->nz - 1; ++k) top += s->zunit[k];
+    height = top - height;
+    double dxh = fabs(va) * height / fabs(vc), dyh = fabs(vb) * height / fabs(vc);
+    int di = (int)((s->xunit - x_old + dxh) / s->xunit);
+    int dj = (int)((s->yunit - y_old + dyh) / s->yunit);
+    int si = va >= 0 ? 1 : -1, sj = vb >= 0 ? 1 : -1;
+    int pi = (((ev->i + si * di) % s->nx) + s->nx) % s->nx;
+    int pj = (((ev->j + sj * dj) % s->ny) + s->ny) % s->ny;
+
+    double att = exp(-tau), mu0mu = fabs(cos(s->theta0)) / fabs(vc);
+    double c1 = prob[v][0] * att;
+    int pixel = pi + s->nx * (pj + s->ny * v);
+    for (int m = 0; m < s->nmat; ++m) out->tallies[s->nmat * pixel + m] += prob[v][m] * att * mu0mu;
+    if (out->record) out->record[4] += c1 * mu0mu;
+    if (s->paths) {
+      double total = 0.0;
+      for (int k = 0; k < s->nz; ++k) {
+        double add = c1 * (history[k] + lengths[k]);
+        out->tallies[s->stokes_count + pi + s->nx * (pj + s->ny * (k + s->nz * v))] += add;
+        total += add;
+      }
+      out->tallies[s->stokes_count + s->path_count + pixel] += c1;
+      if (out->record) out->record[5] += c1, out->record[6] += total;
+    }
+  }
+  return 1;
+}
+
+enum { ESCAPED = 1, ABSORBED = 2, FAILED = 3 };
+
+/* One photon history. Returns the end status; *flag receives failure bits. */
+static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uint32_t seed,
+                          uint32_t max_steps, sink_t *out, unsigned *flag) {
+  rng_t r = {photon, round, seed, 0u, 0u, {0, 0, 0, 0}, 4};
+  int column = (int)(photon % (uint32_t)(s->nx * s->ny));
+  pos_t p;
+  p.i = column % s->nx;
+  p.j = column / s->nx;
+  p.k = s->nz - 1;
+  p.a = sin(s->theta0) * cos(s->phi0);
+  p.b = sin(s->theta0) * sin(s->phi0);
+  p.c = cos(s->theta0);
+  double phi_in = s->phi0;
+  p.x = s->xunit * draw(&r);
+  p.y = s->yunit * draw(&r);
+  p.z = s->zunit[p.k];
+  double st[3] = {1.0, 0.0, 0.0}, weight = 1.0;
+  double history[MAX_LAYERS] = {0};
+  double remains = -log(fmax(draw(&r), EPS));
+  ++out->counters[0];
+  double *rec = out->record;
+
+  for (uint32_t step = 0; step < max_steps; ++step) {
+    double beta = (p.k < s->nz - 1 && p.k >= 0) ? s->beta[cell(s, p.i, p.j, p.k)] : 0.0;
+    int surface = 0;
+    int event = 0;
+    if (beta > EPS) {
+      double dist = remains / beta;
+      double ax = fabs(p.a), ay = fabs(p.b), az = fabs(p.c);
+      double xc = p.x / (ax > EPS ? ax : EPS), yc = p.y / (ay > EPS ? ay : EPS);
+      double zc = p.z / (az > EPS ? az : EPS);
+      if (dist < xc && dist < yc && dist < zc) {
+        p.x = fmax(0.0, p.x - ax * dist);
+        p.y = fmax(0.0, p.y - ay * dist);
+        p.z = fmax(0.0, p.z - az * dist);
+        history[p.k] += dist;
+        event = 1;
+      }
+    }
+    if (!event) {
+      int k_old = p.k;
+      double d = advance(s, &p);
+      ++out->counters[3];
+      history[k_old] += d;
+      remains -= d * beta;
+      if (p.k >= s->nz) return ESCAPED;
+      if (p.k >= 0) continue;
+      surface = 1;
+      p.k = 0;
+      p.z = 0.0;
+    }
+
+    int species = -1;
+    if (surface) {
+      weight *= s->albedo[p.i + s->nx * p.j];
+      ++out->counters[2];
+      if (rec) rec[1] += 1.0;
+    } else {
+      double u = draw(&r);
+      int c = cell(s, p.i, p.j, p.k), cells = s->nx * s->ny * s->nz;
+      for (int t = 0; t < s->species; ++t) {
+        if (u < s->cumfrac[t * cells + c]) {
+          species = t;
+          break;
+        }
+      }
+      weight *= species < 0 ? s->ssalb_gas[p.k] : s->sp[species].ssalb;
+      ++out->counters[1];
+      if (rec) rec[0] += 1.0;
+    }
+    if (weight < EPS) return ABSORBED;
+
+    double prob[MAX_VIEWS][3];
+    for (int v = 0; v < s->views; ++v) {
+      double vc = cos(s->view[v]);
+      if (surface) {
+        prob[v][0] = st[0] * vc * weight, prob[v][1] = 0.0, prob[v][2] = 0.0;
+      } else {
+        double a = 1.0 - p.c * p.c, b = 1.0 - vc * vc;
+        double cs = p.c * vc + sqrt(a > EPS ? a : EPS) * sqrt(b > EPS ? b : EPS) * cos(phi_in - s->phi);
+        double f[4], rot[4];
+        elements(s, species, cs, f);
+        if (s->nmat > 1)
+          oracle_rotation(cs, p.c, vc, phi_in, s->phi, rot);
+        else
+          rot[0] = 1.0, rot[1] = 0.0, rot[2] = 1.0, rot[3] = 0.0;
+        oracle_scatter(f, rot, st, prob[v]);
+        for (int m = 0; m < 3; ++m) prob[v][m] *= weight;
+        if (prob[v][0] < 1.0e-100) prob[v][0] = prob[v][1] = prob[v][2] = 0.0;
+      }
+      if (!isfinite(prob[v][0]) || !isfinite(prob[v][1]) || !isfinite(prob[v][2])) {
+        *flag |= 1u;
+        return FAILED;
+      }
+    }
+    if (!detect(s, &p, prob, history, max_steps, out)) {
+      *flag |= 2u;
+      return FAILED;
+    }
+
+    double cos_out, phi_out;
+    if (surface) {
+      cos_out = sqrt(draw(&r));
+      phi_out = 2.0 * PI * draw(&r);
+      st[0] = 1.0, st[1] = 0.0, st[2] = 0.0;
+    } else {
+      double dphi, cos_sca;
+      if (!sample(s, species, st, p.c, &r, &cos_out, &dphi, &cos_sca, &out->counters[6])) {
+        *flag |= 4u;
+        return FAILED;
+      }
+      phi_out = phi_in + dphi;
+      if (s->nmat > 1) {
+        double f[4], rot[4], ns[3];
+        elements(s, species, cos_sca, f);
+        oracle_rotation(cos_sca, p.c, cos_out, phi_in, phi_out, rot);
+        oracle_scatter(f, rot, st, ns);
+        st[0] = 1.0, st[1] = ns[1] / ns[0], st[2] = ns[2] / ns[0];
+        if (!isfinite(st[1]) || !isfinite(st[2]) || sqrt(st[1] * st[1] + st[2] * st[2]) > 1.01) {
+          *flag |= 8u;
+          return FAILED;
+        }
+      }
+      remains = -log(fmax(draw(&r), EPS));
+    }
+    double sine = sqrt(fmax(0.0, 1.0 - cos_out * cos_out));
+    double a = cos(phi_out) * sine, b = sin(phi_out) * sine, c = cos_out;
+    if (a * p.a < 0) p.x = fmax(0.0, s->xunit - p.x);
+    if (b * p.b < 0) p.y = fmax(0.0, s->yunit - p.y);
+    if (c * p.c < 0) p.z = fmax(0.0, s->zunit[p.k] - p.z);
+    p.a = a, p.b = b, p.c = c;
+    phi_in = phi_out;
+    if (rec) rec[3] = weight, rec[7] = p.k;
+  }
+  *flag |= 16u;
+  return FAILED;
+}
+
+static int word(const float *t, int index) { return (int)t[index]; }
+
+/* Parse the packed float32 tables (layout v1). Returns 0 on success. */
+static int parse(const float *t, size_t count, scene_t *s) {
+  if (count < HEADER || t[0] != 1.0f || (size_t)t[26] != count) return 1;
+  s->nx = word(t, 1), s->ny = word(t, 2), s->nz = word(t, 3), s->nmat = word(t, 4);
+  s->views = word(t, 5), s->species = word(t, 6), s->paths = word(t, 7);
+  if (s->nz > MAX_LAYERS || s->views > MAX_VIEWS || s->species > MAX_SPECIES) return 1;
+  s->xunit = t[10], s->yunit = t[11], s->depol = t[12];
+  s->theta0 = t[13], s->phi0 = t[14], s->phi = t[15];
+  for (int v = 0; v < s->views; ++v) s->view[v] = t[16 + v];
+  s->zunit = t + word(t, 19), s->albedo = t + word(t, 20), s->beta = t + word(t, 21);
+  s->cumfrac = t + word(t, 22), s->ssalb_gas = t + word(t, 23);
+  for (int i = 0; i < s->species; ++i) {
+    int h = 32 + 8 * i;
+    species_t *p = &s->sp[i];
+    p->lut = word(t, h), p->ssalb = t[h + 1];
+    p->cosine = t + word(t, h + 2), p->cdf = t + word(t, h + 3), p->f11 = t + word(t, h + 4);
+    p->f22 = t + word(t, h + 5), p->f33 = t + word(t, h + 6), p->f12 = t + word(t, h + 7);
+  }
+  s->stokes_count = s->nmat * s->nx * s->ny * s->views;
+  s->path_count = s->nx * s->ny * s->nz * s->views;
+  s->tally_count = s->stokes_count + (s->paths ? s->path_count + s->nx * s->ny * s->views : 0);
+  return 0;
+}
+
+int oracle_tally_count(const float *tables, size_t count) {
+  scene_t s;
+  return parse(tables, count, &s) ? -1 : s.tally_count;
+}
+
+/*
+ * Run photons [first, first + count) of a round group with the product's control words
+ * (count, first, round, seed, max_steps, -, photons per round, rounds). tallies receives
+ * rounds x tally_count doubles (zeroed here). records (optional) receives TRACE_WORDS
+ * doubles for the first record_count photons of the range. counters[8] are accumulated.
+ * Returns the failure flags (0 on success).
+ */
+unsigned oracle_run(const float *tables, size_t table_count, const uint32_t *controls,
+                    double *tallies, double *records, size_t record_count, uint64_t *counters) {
+  scene_t s;
+  if (parse(tables, table_count, &s)) return 0x80000000u;
+  uint32_t count = controls[0], first = controls[1], round = controls[2], seed = controls[3];
+  uint32_t max_steps = controls[4], per_round = controls[6], rounds = controls[7];
+  memset(tallies, 0, sizeof(double) * (size_t)s.tally_count * rounds);
+  unsigned flag = 0;
+  for (uint32_t lane = 0; lane < count && !flag; ++lane) {
+    uint32_t identity = first + lane;
+    uint32_t offset = identity / per_round, photon = identity % per_round;
+    double *rec = (records && lane < record_count) ? records + (size_t)lane * TRACE_WORDS : NULL;
+    if (rec) {
+      memset(rec, 0, sizeof(double) * TRACE_WORDS);
+      rec[3] = 1.0;
+    }
+    sink_t out = {tallies + (size_t)offset * s.tally_count, rec, counters};
+    int status = photon_history(&s, photon, round + offset, seed, max_steps, &out, &flag);
+    if (rec) rec[2] = status;
+  }
+  return flag;
+}

@@ -1,0 +1,311 @@
+"""GPU parity: the HIP path against the float64 oracle, the reference Fortran and itself.
+
+Mirrors the reference's tests/integration/test_metal.py: Philox known answers and device
+helpers against the Fortran (probe entry points instead of spliced kernels), no CPU fallback,
+invariance of chunking, shards and grouped rounds, truncated histories rejected, invalid
+devices rejected, validation before dispatch. Histories are compared photon by photon with the
+oracle on identical Philox streams; full-size runs are checked through invariants.
+"""
+
+import ctypes
+import json
+from dataclasses import replace
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = Path(__file__).with_name("golden")
+
+
+def small_scenes():
+    from monki import configs
+
+    return {
+        "clear-I": configs.clear_sky(nx=6, ny=4, nmat=1),
+        "clear-IQU": configs.clear_sky(nx=6, ny=4, nmat=3),
+        "layered-cloud": configs.layered_cloud(nx=3, ny=2, nmat=3),
+        "box-cloud": configs.box_cloud(nx=8, ny=6, nmat=3, box=(3, 2), offset=(2, 1)),
+    }
+
+
+SCENES = small_scenes()
+
+
+def case(name, photons_per_column=200, rounds=32):
+    from monki import SceneCase
+
+    return SceneCase(name, SCENES[name], photons_per_column, rounds)
+
+
+def test_gpu_philox_matches_published_known_answers():
+    from monki.execution.hip import load_library
+
+    native = load_library()
+    cases = np.array(
+        [[0, 0, 0, 0, 0, 0], [0xFFFFFFFF] * 6,
+         [0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344, 0xA4093822, 0x299F31D0]],
+        dtype=np.uint32,
+    )  # fmt: skip
+    out = np.zeros((3, 4), dtype=np.uint32)
+    assert native.monki_hip_probe_philox(
+        0, cases.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), 3,
+        out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)),
+    ) == 0  # fmt: skip
+    np.testing.assert_array_equal(out[0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8])
+    np.testing.assert_array_equal(out[1], [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD])
+    np.testing.assert_array_equal(out[2], [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1])
+
+
+def test_gpu_polarization_helpers_match_reference_fortran():
+    from monki.execution.hip import load_library
+
+    native = load_library()
+    vectors = json.loads((GOLDEN / "helper_vectors.json").read_text())
+    inputs = np.ascontiguousarray(vectors["geometry_inputs"], dtype=np.float64)
+    expected = np.array(vectors["geometry_outputs"])
+    out = np.zeros((len(inputs), 12))
+    assert native.monki_hip_probe_scatter(
+        0, inputs.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), len(inputs),
+        out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+    ) == 0  # fmt: skip
+    # float32 device arithmetic against the float64 Fortran (test_metal.py:103 tolerance).
+    np.testing.assert_allclose(out[:, 5:], expected[:, 5:], rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(out[:, :5], expected[:, :5], rtol=1e-5, atol=2e-6)
+
+
+def trace_both(oracle, name, photons_per_column, precision=32, round_index=1):
+    from monki.execution import HIP, BatchControls
+    from monki.execution.hip_tables import pack_tables
+
+    scene = SCENES[name]
+    per_round = photons_per_column * scene.nx * scene.ny
+
+    with HIP(tally_shards=1) as backend:
+        backend.prepare(case(name, photons_per_column))
+        gpu_tallies, gpu = backend.trace(BatchControls(round_index, photons_per_column), per_round, precision)
+
+    packed, _ = pack_tables(scene, shards=1)
+    cpu_tallies, cpu, _, flag = oracle.run(
+        packed, [per_round, 0, round_index, 23274, 100000, 0, per_round, 1], records=per_round
+    )
+    assert flag == 0
+    return gpu_tallies, gpu.astype(np.float64), cpu_tallies[0], cpu
+
+
+def matching(gpu, cpu, rtol):
+    """Photons whose records agree in every field: event counts, end status, final weight,
+    Stokes-I, weight and path contributions, last layer."""
+
+    close = np.abs(gpu - cpu) <= rtol * np.abs(cpu) + 1e-9
+    return np.all(close, axis=1)
+
+
+# Float32 transport follows the float64 histories until a float32 rounding flips a rejection,
+# event-versus-face or tie decision; afterwards the two histories are independent samples.
+# Required fraction of photons with identical histories, per scene.
+FLOAT32_AGREEMENT = {"clear-I": 0.995, "clear-IQU": 0.995, "layered-cloud": 0.97, "box-cloud": 0.97}
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+def test_float32_histories_match_float64_oracle(oracle, name):
+    gpu_tallies, gpu, cpu_tallies, cpu = trace_both(oracle, name, 200)
+    assert np.all(cpu[:, 2] != 3) and np.all(gpu[:, 2] != 3)
+    same = matching(gpu, cpu, rtol=1e-4)
+    print(name, "identical histories:", same.mean())
+    assert same.mean() >= FLOAT32_AGREEMENT[name], same.mean()
+    # Tally totals: the difference is bounded by what the diverged photons contributed.
+    diverged = ~same
+    scene = SCENES[name]
+    stokes = scene.nmat * scene.nx * scene.ny * scene.angles
+    bound = np.sum(np.abs(gpu[diverged, 4]) + np.abs(cpu[diverged, 4]))
+    i_gpu = gpu_tallies[:stokes][:: scene.nmat].sum()
+    i_cpu = cpu_tallies[:stokes][:: scene.nmat].sum()
+    assert abs(i_gpu - i_cpu) <= bound + 1e-4 * abs(i_cpu), (i_gpu, i_cpu, bound)
+    paths = slice(stokes, stokes + scene.nx * scene.ny * scene.nz * scene.angles)
+    bound = np.sum(np.abs(gpu[diverged, 6]) + np.abs(cpu[diverged, 6]))
+    assert abs(gpu_tallies[paths].sum() - cpu_tallies[paths].sum()) <= bound + 1e-4 * cpu_tallies[paths].sum()
+
+
+@pytest.mark.parametrize("name", ["clear-IQU", "layered-cloud", "box-cloud"])
+def test_float64_build_matches_oracle(oracle, name):
+    gpu_tallies, gpu, cpu_tallies, cpu = trace_both(oracle, name, 100, precision=64)
+    # Records are stored in float32: agreement to float32 resolution.
+    same = matching(gpu, cpu, rtol=1e-6)
+    print(name, "identical histories:", same.mean())
+    assert same.mean() >= 0.999, same.mean()
+    scene = SCENES[name]
+    np.testing.assert_allclose(gpu_tallies, cpu_tallies, rtol=1e-9 if same.all() else 1e-2, atol=1e-12)
+
+
+def test_hip_never_uses_cpu_transport():
+    from monki.execution import HIP, BatchControls
+
+    with HIP() as backend:
+        backend.prepare(case("clear-IQU"))
+        first = backend.run_batch(BatchControls(1, 200))
+        snapshot = first.stokes.copy()
+        second = backend.run_batch(BatchControls(2, 200))
+        assert backend.provenance["gpu_dispatches"] == 2
+        assert "gfx950" in backend.provenance["device"]
+        assert np.all(first.stokes[0] > 0)
+        assert not np.array_equal(first.stokes, second.stokes)
+        assert backend.counters()["photons"] == 2 * 200 * 24
+        backend.prepare(case("clear-IQU"))
+        replay = backend.run_batch(BatchControls(1, 200))
+        np.testing.assert_allclose(first.stokes, replay.stokes, rtol=2e-5, atol=2e-5)
+
+    np.testing.assert_array_equal(first.stokes, snapshot)
+
+    with pytest.raises(RuntimeError, match="prepared"):
+        backend.run_batch(BatchControls(1, 200))
+
+
+def test_chunking_preserves_photon_streams():
+    from monki.execution import HIP, BatchControls
+
+    with HIP(photons_per_dispatch=1000) as small, HIP() as large:
+        small.prepare(case("box-cloud"))
+        large.prepare(case("box-cloud"))
+        a = small.run_batch(BatchControls(1, 200))
+        b = large.run_batch(BatchControls(1, 200))
+        assert small.provenance["gpu_dispatches"] == 10
+        np.testing.assert_allclose(a.stokes, b.stokes, rtol=2e-5, atol=2e-5)
+        np.testing.assert_allclose(a.weighted_paths, b.weighted_paths, rtol=2e-5, atol=2e-5)
+
+
+@pytest.mark.parametrize("save_path", [False, True])
+def test_shard_reduction_preserves_all_estimators(save_path):
+    from monki.execution import HIP, BatchControls
+
+    with HIP(tally_shards=1, save_path=save_path) as one, HIP(tally_shards=64, save_path=save_path, profile=True) as many:
+        one.prepare(case("layered-cloud"))
+        many.prepare(case("layered-cloud"))
+        a = one.run_batch(BatchControls(1, 200))
+        b = many.run_batch(BatchControls(1, 200))
+        np.testing.assert_allclose(a.stokes, b.stokes, rtol=2e-5, atol=2e-5)
+
+        if save_path:
+            np.testing.assert_allclose(a.weighted_paths, b.weighted_paths, rtol=2e-5, atol=2e-5)
+            np.testing.assert_allclose(a.weights, b.weights, rtol=2e-5, atol=2e-5)
+        else:
+            assert a.weighted_paths is None
+
+        assert many.provenance["gpu_seconds"] > 0
+        assert "gpu_seconds" not in one.provenance
+
+
+def test_grouped_rounds_preserve_individual_streams():
+    from monki.execution import HIP, BatchControls
+
+    requests = [BatchControls(index, 200) for index in range(1, 36)]
+
+    with HIP() as grouped, HIP() as sequential:
+        grouped.prepare(case("clear-IQU"))
+        sequential.prepare(case("clear-IQU"))
+        actual = grouped.run_batches(requests)
+        expected = [sequential.run_batch(request) for request in requests]
+        assert grouped.provenance["gpu_dispatches"] == 2
+        assert sequential.provenance["gpu_dispatches"] == 35
+
+        for a, b in zip(actual, expected):
+            assert a.controls == b.controls
+            np.testing.assert_allclose(a.stokes, b.stokes, rtol=2e-5, atol=2e-5)
+            np.testing.assert_allclose(a.weighted_paths, b.weighted_paths, rtol=2e-5, atol=2e-5)
+
+
+def test_truncated_histories_fail_and_release_session():
+    from monki.execution import HIP, BatchControls
+
+    with HIP(max_steps=1) as backend:
+        backend.prepare(case("clear-IQU"))
+
+        with pytest.raises(RuntimeError, match="round rejected"):
+            backend.run_batches([BatchControls(1, 200), BatchControls(2, 200)])
+
+        assert backend._handle is None
+        assert backend.metadata is None
+
+
+def test_invalid_device_does_not_fall_back():
+    from monki.execution import HIP
+
+    with HIP(device=2**32 - 1) as backend:
+        with pytest.raises(RuntimeError, match="unavailable"):
+            backend.prepare(case("clear-IQU"))
+
+        assert backend._handle is None
+
+
+def test_batch_validation_precedes_dispatch():
+    from monki.execution import HIP, BatchControls
+
+    with HIP() as backend:
+        backend.prepare(case("clear-IQU"))
+
+        with pytest.raises(ValueError, match="sequential"):
+            backend.run_batches([BatchControls(1, 200), BatchControls(3, 200)])
+
+        assert backend.provenance["gpu_dispatches"] == 0
+        setattr(backend, "seed", 1)
+
+        with pytest.raises(ValueError, match="reprepare"):
+            backend.run_batch(BatchControls(1, 200))
+
+
+def test_simulate_matches_reference_fortran(tmp_path):
+    """Plain HIP against the reference Fortran transport: |d| < 3 combined SE per pixel/view
+    (Bonferroni bound of compare()), chi-square of the normalized differences reported."""
+
+    from monki import simulate
+    from monki.execution import HIP
+    from monki.metadata import scene_metadata
+    from monki.validation import check_result, combined_se_statistics, compare
+    from oracle.oracle import run_reference
+
+    name = "box-cloud"
+    rounds, ppc = 40, 100
+    gpu = simulate(case(name, ppc, rounds), HIP(seed=881), tmp_path / "gpu.nc")
+    assert check_result(gpu) == []
+    scene = SCENES[name]
+    tallies, _, _ = run_reference(scene, ppc, rounds, threads=4)
+    nmat, nx, ny, views = scene.nmat, scene.nx, scene.ny, scene.angles
+    stokes = tallies[:, : nmat * nx * ny * views].reshape(rounds, nmat, nx, ny, views, order="F")
+    normalized = stokes / ppc
+    mean = normalized.mean(axis=0)
+    se = normalized.std(axis=0, ddof=1) / np.sqrt(rounds)
+    a = gpu.radiance[..., 0].transpose(3, 2, 1, 0)
+    b = gpu.standard_error[..., 0].transpose(3, 2, 1, 0)
+    stats = combined_se_statistics(a, b, mean, se, floor=1e-12)
+    print(json.dumps(stats))
+    assert stats["max_abs_z"] < 5.0
+    assert 0.7 < stats["chi_square_per_dof"] < 1.4
+    assert stats["fraction_above_3"] < 0.01
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+def test_full_size_layered_cloud_round_invariants(precision):
+    """The benchmark scene at full size (30 x 20 x 35, 10,000 photons per column)."""
+
+    from monki import SceneCase, configs
+    from monki.execution import HIP, BatchControls
+
+    scene = configs.layered_cloud()
+
+    with HIP(precision=precision) as backend:
+        backend.prepare(SceneCase("layered-cloud", scene, 10_000, 1))
+        batch = backend.run_batch(BatchControls(1, 10_000))
+        counts = backend.counters()
+
+    assert counts["photons"] == 10_000 * 600
+    assert counts["atmospheric_events"] > 10 * counts["photons"]
+    i, q, u = batch.stokes
+    assert np.all(np.isfinite(batch.stokes)) and np.all(i > 0)
+    assert np.all(np.hypot(q, u) <= i)
+    # Horizontally uniform scene: every pixel estimates the same radiance.
+    per_view = i.reshape(-1, 3) / 10_000
+    assert np.all(np.abs(per_view / per_view.mean(axis=0) - 1) < 0.1)
+    amf = batch.weighted_paths[:, :, :-1, :] / batch.weights[:, :, None, :]
+    assert np.all(np.isfinite(amf)) and np.all(amf >= 0)

@@ -1,0 +1,439 @@
+"""Host logic without a GPU: the reference's unit-test semantics plus the C-ABI boundary.
+
+Re-expresses tests/unit/{test_convergence,test_execution,test_inputs,test_validation}.py of the
+reference for this package, and checks table packing, the transport.bin v1 export, NetCDF
+products and that libmonki_hip.so exports every symbol include/monki_hip.h declares.
+"""
+
+import ctypes
+import re
+from dataclasses import replace
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+# -- convergence (tests/unit/test_convergence.py) ----------------------------------------
+
+
+def test_online_amf_error_matches_explicit_correlated_ratio():
+    from monki.convergence import AMFMoments
+    from monki.execution.tallies import BatchControls, RawTallies
+
+    rng = np.random.default_rng(123)
+    weights = rng.uniform(1, 4, (100, 1, 1, 2))
+    paths = weights[:, :, :, None, :] * rng.uniform(1, 5, (100, 1, 1, 4, 2))
+    moments = AMFMoments()
+
+    for index in range(100):
+        moments.add(RawTallies(BatchControls(index + 1, 100), np.zeros((3, 1, 1, 2)), paths[index], weights[index]))
+
+    thickness = np.array([0.5, 2.0, 5.0])
+    actual, error = moments.estimate(thickness)
+    ratio = paths[:, :, :, :-1, :].mean(axis=0) / weights.mean(axis=0)[:, :, None, :]
+    residual = paths[:, :, :, :-1, :] - ratio * weights[:, :, :, None, :]
+    expected = residual.std(axis=0, ddof=1) / 10 / weights.mean(axis=0)[:, :, None, :]
+    np.testing.assert_allclose(actual, ratio / thickness[None, None, :, None])
+    np.testing.assert_allclose(error, expected / thickness[None, None, :, None], rtol=1e-12)
+
+
+@pytest.mark.parametrize(
+    "settings",
+    [{"relative_error": 0}, {"relative_error": np.nan}, {"minimum_rounds": 31},
+     {"check_every": 0}, {"consecutive_checks": True}],
+)  # fmt: skip
+def test_invalid_stopping_policy(settings):
+    from monki import AMFConvergence
+
+    with pytest.raises(ValueError):
+        AMFConvergence(**settings)
+
+
+class _Case:
+    photons_per_column = 100
+
+    def __init__(self, rounds):
+        self.rounds = rounds
+
+
+def test_late_outlier_resets_precision_streak():
+    from monki import AMFConvergence
+    from monki.convergence import AMFStopping
+    from monki.execution.tallies import RawTallies
+
+    class Synthetic:
+        def run_batches(self, requests):
+            return [
+                RawTallies(r, np.ones((3, 1, 1, 1)), np.full((1, 1, 2, 1), 100.0 if r.round_index == 48 else 1.0), np.ones((1, 1, 1)))
+                for r in requests
+            ]
+
+    stopping = AMFStopping(AMFConvergence(minimum_rounds=32, check_every=16), np.array([1.0]))
+    assert len(list(stopping.batches(Synthetic(), _Case(64)))) == 64
+    assert not stopping.converged
+    assert stopping.history[0]["streak"] == 1
+    assert stopping.history[1]["streak"] == 0
+
+
+def test_zero_amf_cannot_satisfy_relative_precision():
+    from monki import AMFConvergence
+    from monki.convergence import AMFStopping
+    from monki.execution.tallies import RawTallies
+
+    class Synthetic:
+        def run_batches(self, requests):
+            return [RawTallies(r, np.ones((3, 1, 1, 1)), np.zeros((1, 1, 2, 1)), np.ones((1, 1, 1))) for r in requests]
+
+    stopping = AMFStopping(AMFConvergence(minimum_rounds=32, consecutive_checks=1), np.array([1.0]))
+    list(stopping.batches(Synthetic(), _Case(32)))
+    assert not stopping.converged
+    assert stopping.history[-1]["max_relative_se"] is None
+
+
+# -- execution (tests/unit/test_execution.py) --------------------------------------------
+
+
+def test_backend_is_abstract():
+    from monki.execution import Backend
+
+    with pytest.raises(TypeError, match="abstract"):
+        Backend()
+
+
+@pytest.mark.parametrize("missing", ["prepare", "run_batch", "close"])
+def test_every_lifecycle_method_is_required(missing):
+    from monki.execution import Backend
+
+    methods = {"prepare": lambda self, s: None, "run_batch": lambda self, c: object(), "close": lambda self: None}
+    del methods[missing]
+
+    with pytest.raises(TypeError, match=missing):
+        type("Incomplete", (Backend,), methods)()
+
+
+def test_context_manager_closes_on_success_and_failure():
+    from monki.execution import Backend
+
+    class Recording(Backend):
+        def __init__(self):
+            self.events = []
+
+        def prepare(self, scene):
+            self.events.append(("prepare", scene))
+
+        def run_batch(self, controls):
+            self.events.append(("batch", controls))
+            return controls
+
+        def close(self):
+            self.events.append(("close", None))
+
+    backend = Recording()
+
+    with backend as strategy:
+        strategy.prepare(1)
+        assert strategy.run_batch(2) == 2
+
+    assert backend.events == [("prepare", 1), ("batch", 2), ("close", None)]
+
+    with pytest.raises(RuntimeError, match="failed"), backend:
+        raise RuntimeError("simulation failed")
+
+    assert backend.events.count(("close", None)) == 2
+
+
+@pytest.mark.parametrize(
+    "settings",
+    [{"device": -1}, {"device": True}, {"seed": -1}, {"seed": True}, {"seed": 2**32},
+     {"photons_per_dispatch": 0}, {"tally_shards": 0}, {"tally_shards": 65},
+     {"tally_shards": True}, {"profile": "yes"}, {"max_steps": 0}, {"max_steps": 100_001},
+     {"save_path": "yes"}, {"precision": 16}],
+)  # fmt: skip
+def test_invalid_hip_settings_fail_before_gpu_access(settings):
+    from monki.execution import HIP
+
+    with pytest.raises(ValueError):
+        HIP(**settings)
+
+
+def test_hip_does_not_accept_cloud_settings():
+    from monki.execution import HIP
+
+    with pytest.raises(TypeError):
+        HIP(provider="aws")
+
+
+def test_missing_library_fails_loudly(tmp_path):
+    from monki import SceneCase, configs
+    from monki.execution import HIP
+
+    backend = HIP(library=tmp_path / "missing.so")
+
+    with pytest.raises(FileNotFoundError, match="build"):
+        backend.prepare(SceneCase("x", configs.clear_sky(nx=2, ny=2), 10, 1))
+
+    assert backend._handle is None
+
+
+def test_stale_library_is_rejected_before_native_call(monkeypatch):
+    from monki import SceneCase, configs
+    from monki.execution import HIP
+
+    class Stale:
+        monki_hip_abi_version = staticmethod(lambda: 0)
+
+    monkeypatch.setattr(ctypes, "CDLL", lambda path: Stale())
+
+    with HIP() as backend, pytest.raises(RuntimeError, match="Rebuild"):
+        backend.prepare(SceneCase("x", configs.clear_sky(nx=2, ny=2), 10, 1))
+
+
+def test_reference_case_and_backward_are_rejected_explicitly():
+    from monki import ReferenceCase
+    from monki.execution import HIP
+
+    with HIP() as backend, pytest.raises(NotImplementedError, match="SceneCase"):
+        backend.prepare(ReferenceCase("clear", "testrun/MONKI_MLS_cloudfree_340_340_test.in"))
+
+
+# -- inputs (tests/unit/test_inputs.py) ---------------------------------------------------
+
+
+@pytest.mark.parametrize(
+    "changes",
+    [{"name": "bad name"}, {"input_file": "/abs.in"}, {"input_file": "../x.in"}, {"photons_per_column": 0},
+     {"rounds": 2**31}, {"surface_albedo": 1.5}, {"view_zeniths": ()}, {"solar_zenith": 90},
+     {"view_azimuth": 360}, {"backward": 1}],
+)  # fmt: skip
+def test_reference_case_validation(changes):
+    from monki import ReferenceCase
+
+    with pytest.raises(ValueError):
+        replace(ReferenceCase("clear", "testrun/clear.in"), **changes)
+
+
+def test_batch_controls_are_positive_32_bit():
+    from monki.execution import BatchControls
+
+    for bad in ((0, 1), (1, 0), (1, 2**31), (True, 1)):
+        with pytest.raises(ValueError):
+            BatchControls(*bad)
+
+
+# -- scene tables ---------------------------------------------------------------------------
+
+
+def test_scene_validation_rejects_inconsistent_tables():
+    from monki import configs
+
+    scene = configs.clear_sky(nx=2, ny=2)
+    scene.validate()
+    broken = replace(scene, fractions=scene.fractions * 0.5)
+
+    with pytest.raises(ValueError, match="fractions"):
+        broken.validate()
+
+    with pytest.raises(NotImplementedError):
+        replace(scene, nmat=4).validate()
+
+    with pytest.raises(ValueError, match="zenith"):
+        replace(scene, view_zeniths=(0.0, 90.0)).validate()
+
+
+def test_transport_bin_round_trip_and_layout():
+    from monki import configs
+    from monki.scene import TransportScene
+
+    scene = configs.layered_cloud(nx=3, ny=2)
+    raw = scene.transport_bin()
+    # worker.f90:1102-1120 header: version, nx, ny, nz, nmat, angles, species, source,
+    # detector, register, save_path, backward.
+    assert list(raw[:12]) == [1, 3, 2, 35, 3, 3, 2, 35, 35, 35, 1, 0]
+    assert raw[15] == 140.0  # propagation zenith = 180 - SZA (worker.f90:789)
+    back = TransportScene.from_transport_bin(raw)
+    np.testing.assert_array_equal(back.beta, scene.beta)
+    np.testing.assert_array_equal(back.fractions, scene.fractions)
+    np.testing.assert_array_equal(back.species[0].f11, scene.species[0].f11)
+    np.testing.assert_allclose(back.species[1].cdf, scene.species[1].bound_cdf(3))
+
+
+def test_packed_tables_uniform_top_and_prefixes():
+    from monki import configs
+    from monki.execution.hip_tables import pack_tables
+
+    box = configs.box_cloud()
+    packed, (nmat, nx, ny, nz, views) = pack_tables(box)
+    # The box cloud occupies layers 3-6: the detector walks explicitly below layer 7.
+    assert packed[8] == 7
+    clear, _ = pack_tables(configs.clear_sky())
+    assert clear[8] == 0
+    height = clear[int(clear[24]) : int(clear[24]) + nz]
+    tau = clear[int(clear[25]) : int(clear[25]) + nz]
+    # Above the last physical layer nothing is left; at layer 0 the whole column above it.
+    assert height[nz - 2] == 0 and tau[nz - 2] == 0
+    zunit = configs.clear_sky().zunit
+    assert height[0] == pytest.approx(zunit[1 : nz - 1].sum(), rel=1e-6)
+    # Moving a varying layer to the last physical layer disables the shortcut everywhere.
+    scene = configs.box_cloud()
+    scene.beta[0, 0, nz - 2] += 0.01
+    changed, _ = pack_tables(scene)
+    assert changed[8] == nz - 1
+
+
+def test_packed_tables_reject_oversized_scenes():
+    from monki import configs
+    from monki.execution.hip_tables import pack_tables
+
+    with pytest.raises(ValueError, match="shards"):
+        pack_tables(configs.clear_sky(nx=2, ny=2), shards=65)
+
+
+# -- validation and the NetCDF product (tests/unit/test_validation.py) ---------------------
+
+
+@pytest.fixture
+def result():
+    from monki import SimulationResult
+
+    values = {
+        "radiance": np.array([[[[[0.5], [0.1], [-0.05]]]]]),
+        "std_radiance": np.full((1, 1, 1, 3, 1), 0.001),
+        "DOP": np.array([[[[np.hypot(0.1, 0.05) / 0.5]]]]),
+        "wav": np.array([340.0]),
+        "theta": np.array([30.0]),
+        "n_rounds": np.array([32]),
+        "n_photons_per_column": np.array([2000]),
+    }
+    dims = {
+        "radiance": ("nvza", "ny", "nx", "nmat", "nwavs"),
+        "std_radiance": ("nvza", "ny", "nx", "nmat", "nwavs"),
+        "DOP": ("nvza", "ny", "nx", "nwavs"),
+        "wav": ("nwavs",),
+        "theta": ("nvza",),
+        "n_rounds": ("single",),
+        "n_photons_per_column": ("single",),
+    }
+    return SimulationResult(Path("synthetic.nc"), values, dims, {})
+
+
+def changed(result, name, value):
+    return replace(result, values={**result.values, name: value})
+
+
+def test_identical_products_pass_both_modes(result):
+    from monki.validation import check_result, compare
+
+    assert check_result(result) == []
+    assert compare(result, result, mode="deterministic").passed
+    assert compare(result, result).passed
+
+
+def test_wrong_but_physical_radiance_fails(result):
+    from monki.validation import check_result, compare
+
+    candidate = changed(result, "radiance", result.radiance * 1.5)
+    assert check_result(candidate) == []
+    assert not compare(result, candidate).passed
+
+
+def test_dop_and_units_mismatches_fail(result):
+    from monki.validation import check_result, compare
+
+    assert "DOP is inconsistent with Stokes components" in check_result(changed(result, "DOP", result.values["DOP"] * 2))
+    other = replace(result, attributes={"radiance": {"units": "W"}})
+    assert not compare(result, other).passed
+
+
+def test_combined_se_statistics_excludes_zero_uncertainty():
+    from monki.validation import combined_se_statistics
+
+    stats = combined_se_statistics([1.0, 2.0, 0.0], [0.1, 0.1, 0.0], [1.1, 2.0, 0.0], [0.1, 0.1, 0.0])
+    assert stats["count"] == 2
+    assert stats["max_abs_z"] == pytest.approx(0.1 / np.hypot(0.1, 0.1))
+
+
+def test_netcdf_product_round_trip_and_exclusive_create(tmp_path, result):
+    from monki import SimulationResult
+
+    product = replace(result, path=tmp_path / "out.nc", attributes={"radiance": {"units": "-"}})
+    product.save()
+    loaded = SimulationResult.load(tmp_path / "out.nc")
+    np.testing.assert_array_equal(loaded.radiance, product.radiance)
+    assert loaded.dimensions["radiance"] == product.dimensions["radiance"]
+    assert loaded.attributes["radiance"]["units"] == "-"
+
+    with pytest.raises(FileExistsError):
+        product.save()
+
+
+def test_simulate_with_a_recording_backend(tmp_path):
+    """simulate() statistics on synthetic tallies: mean, SE with 999 through round 30, AMF."""
+
+    from monki import SceneCase, configs, simulate
+    from monki.execution import Backend
+    from monki.execution.tallies import RawTallies
+    from monki.metadata import scene_metadata
+
+    scene = configs.clear_sky(nx=2, ny=1, nmat=3)
+    rng = np.random.default_rng(5)
+
+    class Synthetic(Backend):
+        def prepare(self, case):
+            self.metadata = scene_metadata(case.scene, case.photons_per_column)
+            self.provenance = {"backend": "synthetic"}
+
+        def run_batch(self, controls):
+            stokes = np.abs(rng.normal(10, 1, (3, 2, 1, 3))) * [[[[1]]], [[[-0.1]]], [[[0.01]]]]
+            paths = np.abs(rng.normal(5, 0.5, (2, 1, scene.nz, 3)))
+            return RawTallies(controls, stokes, paths, np.full((2, 1, 3), 2.0))
+
+        def close(self):
+            pass
+
+    case = SceneCase("synthetic", scene, 100, 40)
+    out = simulate(case, Synthetic(), tmp_path / "s.nc")
+    assert out.radiance.shape == (3, 1, 2, 3, 1)
+    assert np.all(out.standard_error < 999) and np.all(out.standard_error > 0)
+    assert (tmp_path / "s.run.json").is_file()
+
+    with pytest.raises(FileExistsError):
+        simulate(case, Synthetic(), tmp_path / "s.nc")
+
+    short = simulate(replace(case, rounds=2), Synthetic(), tmp_path / "short.nc")
+    assert np.all(short.standard_error == 999)
+
+
+# -- the C-ABI ------------------------------------------------------------------------------
+
+
+def declared_symbols():
+    header = (ROOT / "include" / "monki_hip.h").read_text()
+    return sorted(set(re.findall(r"\b(monki_hip_\w+)\s*\(", header)))
+
+
+def test_library_exports_every_declared_symbol():
+    from monki.execution.hip import LIBRARY
+
+    if not LIBRARY.is_file():
+        pytest.fail("libmonki_hip.so is not built (python monki-gpu_amd/build.py)")
+
+    native = ctypes.CDLL(str(LIBRARY))
+    symbols = declared_symbols()
+    assert len(symbols) >= 12
+
+    for name in symbols:
+        assert hasattr(native, name), name
+
+    assert native.monki_hip_abi_version() == 1
+
+
+def test_library_binding_declares_every_symbol():
+    from monki.execution.hip import load_library
+
+    native = load_library()
+
+    for name in declared_symbols():
+        assert getattr(native, name) is not None

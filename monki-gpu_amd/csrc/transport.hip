@@ -158,23 +158,39 @@ __device__ inline int bracket_r(const float *__restrict__ axis, int n, R value, 
 
 // ray_scat_matrix / mie_scat_matrix (mc_routines_general_mod.f90:74-136, 257-388):
 // F11, F22, F33, F12 divided by 4 (radiance normalized for incident flux pi).
+// The scattering angle is given as cos together with 1 - cos and 1 + cos. Tabulated elements
+// interpolate linearly in cos as the Fortran does; near the peaks the interpolation weight is
+// taken from 1 - cos (or 1 + cos) against the table's exact 1 - cos (1 + cos) nodes, because a
+// float32 cos cannot resolve the forward peak (F11 changes ~1e8 per unit cos there).
 template <typename R>
-__device__ inline Elements<R> elements(const Scene &s, int species, R cosine, uint32_t &reads) {
+__device__ inline Elements<R> elements(const Scene &s, int species, R cosine, R one_minus,
+                                       R one_plus, uint32_t &reads) {
   cosine = clamp1(cosine);
   Elements<R> f;
   if (species < 0) {
     const R depol = R(s.depol);
     const R delta = (R(1) - depol) / (R(1) + depol / R(2));
-    const R diagonal = delta * R(0.75) * (R(1) + cosine * cosine);
+    const R sin2 = one_minus * one_plus;
+    const R diagonal = delta * R(0.75) * (R(2) - sin2);
     f.f11 = diagonal + R(1) - delta;
     f.f22 = diagonal;
     f.f33 = delta * R(1.5) * cosine;
-    f.f12 = -delta * R(0.75) * (R(1) - cosine * cosine);
+    f.f12 = -delta * R(0.75) * sin2;
   } else {
     const SpeciesTables &t = s.sp[species];
-    const int low = bracket_r<R>(t.cosine, t.lut_size, cosine, reads);
-    const R c0 = R(t.cosine[low]), c1 = R(t.cosine[low + 1]);
-    const R w = (cosine - c0) / (c1 - c0);
+    const int n = t.lut_size;
+    const int low = bracket_r<R>(t.cosine, n, cosine, reads);
+    R w;
+    if (cosine > R(0.5)) {
+      const R o0 = R(t.cosine[n + low]), o1 = R(t.cosine[n + low + 1]);
+      w = (o0 - one_minus) / (o0 - o1);
+    } else if (cosine < R(-0.5)) {
+      const R o0 = R(t.cosine[2 * n + low]), o1 = R(t.cosine[2 * n + low + 1]);
+      w = (one_plus - o0) / (o1 - o0);
+    } else {
+      const R c0 = R(t.cosine[low]), c1 = R(t.cosine[low + 1]);
+      w = (cosine - c0) / (c1 - c0);
+    }
     reads += 10;
     f.f11 = R(t.f11[low]) + w * (R(t.f11[low + 1]) - R(t.f11[low]));
     f.f22 = R(t.f22[low]) + w * (R(t.f22[low + 1]) - R(t.f22[low]));
@@ -194,11 +210,12 @@ template <typename R> struct Rotation {
 };
 
 // stokes_rot_mat (mc_routines_general_mod.f90:942-1174): coplanar, both-vertical,
-// scattered-vertical, incident-vertical and general cases. The angular degeneracy
-// threshold is 2e-7 in float32 (as transport.metal:132-137) and 1e-16 in float64 (as mc.f90).
+// scattered-vertical, incident-vertical and general cases. The coplanarity threshold is
+// 1e-16 in float64 (as mc.f90) and 1e-6 in float32, above the float32 spacing of azimuths up
+// to 2 pi, so exact forward/backward coplanar geometry is recognised after rounding.
 template <typename R>
 __device__ inline Rotation<R> rotation(R cosine, R uo, R u1, R phio, R phi1) {
-  const R eps = sizeof(R) == 4 ? R(2e-7) : R(1e-16);
+  const R eps = sizeof(R) == 4 ? R(1e-6) : R(1e-16);
   const R pi = R(3.14159265358979323846);
   const R dphi = fabs(phio - phi1);
   const R rem = fmod(dphi, pi);
@@ -206,8 +223,10 @@ __device__ inline Rotation<R> rotation(R cosine, R uo, R u1, R phio, R phi1) {
   const bool vertical_1 = sizeof(R) == 4 ? fabs(u1) == R(1) : fabs(u1) > R(1) - R(1e-16);
   if (fmin(rem, pi - rem) < eps || (vertical_o && vertical_1)) return Rotation<R>{R(1), R(0), R(1), R(0)};
 
-  const R sino = sqrt(fmax(R(0), R(1) - uo * uo));
-  const R sin1 = sqrt(fmax(R(0), R(1) - u1 * u1));
+  const R tiny = R(1e-16);
+  const R fo = R(1) - uo * uo, f1 = R(1) - u1 * u1;
+  const R sino = fo < tiny ? R(0) : sqrt(fo);
+  const R sin1 = f1 < tiny ? R(0) : sqrt(f1);
   const R cpo = cos(phio), spo = sin(phio), cp1 = cos(phi1), sp1 = sin(phi1);
   R cosi1, cosi2, q;
   if (vertical_1) {
@@ -225,10 +244,16 @@ __device__ inline Rotation<R> rotation(R cosine, R uo, R u1, R phio, R phi1) {
     cosi2 = (uo - u1 * cosine) / sqrt(fmax(e, (R(1) - u1 * u1) * sc));
     q = sino * spo * sin1 * cp1 - sino * cpo * sin1 * sp1;
   }
-  cosi1 = clamp1(cosi1);
-  cosi2 = clamp1(cosi2);
-  const R sini1 = copysign(sqrt(fmax(R(0), R(1) - cosi1 * cosi1)), q);
-  const R sini2 = copysign(sqrt(fmax(R(0), R(1) - cosi2 * cosi2)), q);
+  // float32 cancellation can push |cos i| past one; clamp as transport.metal:158-159 does.
+  // float64 keeps the unclamped Fortran form.
+  if (sizeof(R) == 4) {
+    cosi1 = clamp1(cosi1);
+    cosi2 = clamp1(cosi2);
+  }
+  const R sign = q >= R(0) ? R(1) : R(-1);
+  const R g1 = R(1) - cosi1 * cosi1, g2 = R(1) - cosi2 * cosi2;
+  const R sini1 = g1 < tiny ? R(0) : sqrt(g1) * sign;
+  const R sini2 = g2 < tiny ? R(0) : sqrt(g2) * sign;
   return Rotation<R>{cosi1 * cosi1 - sini1 * sini1, R(2) * sini1 * cosi1,
                      cosi2 * cosi2 - sini2 * sini2, R(2) * sini2 * cosi2};
 }
@@ -248,7 +273,8 @@ __device__ inline Stokes<R> scatter(const Elements<R> &f, const Rotation<R> &r, 
 // uniform cosine under the Rayleigh bound F11(1), inverse bound-CDF for tabulated species.
 template <typename R>
 __device__ inline bool sample_direction(const Scene &s, int species, const Stokes<R> &st, R uo,
-                                        Stream &rng, R &u1, R &dphi, R &sampled, Counts &n) {
+                                        Stream &rng, R &u1, R &dphi, R &sampled,
+                                        R &sampled_minus, R &sampled_plus, Counts &n) {
   const R pi = R(3.14159265358979323846);
   const R eps = R(1e-16);
   for (uint32_t attempt = 0; attempt < kSamplingAttempts; ++attempt) {
@@ -256,11 +282,13 @@ __device__ inline bool sample_direction(const Scene &s, int species, const Stoke
     // sin^2 of the scattering angle as (1 - cos)(1 + cos) from factors that keep their
     // precision near both peaks: exact for Rayleigh (cos = 2u - 1), interpolated from the
     // 1 -/+ cos tables for tabulated species.
-    R cosine, sin2;
+    R cosine, sin2, one_minus, one_plus;
     if (species < 0) {
       const float u = uniform(rng);
       cosine = R(2) * R(u) - R(1);
-      sin2 = R(4) * R(u) * (R(1) - R(u));
+      one_minus = R(2) - R(2) * R(u);
+      one_plus = R(2) * R(u);
+      sin2 = one_minus * one_plus;
     } else {
       const SpeciesTables &t = s.sp[species];
       const float draw = uniform(rng);
@@ -268,25 +296,31 @@ __device__ inline bool sample_direction(const Scene &s, int species, const Stoke
       const R d0 = R(t.cdf[low]), d1 = R(t.cdf[low + 1]);
       const R w = (R(draw) - d0) / (d1 - d0);
       const float *omc = t.cosine + t.lut_size, *opc = t.cosine + 2 * t.lut_size;
-      const R one_minus = R(omc[low]) + w * (R(omc[low + 1]) - R(omc[low]));
-      const R one_plus = R(opc[low]) + w * (R(opc[low + 1]) - R(opc[low]));
+      one_minus = R(omc[low]) + w * (R(omc[low + 1]) - R(omc[low]));
+      one_plus = R(opc[low]) + w * (R(opc[low + 1]) - R(opc[low]));
       cosine = R(t.cosine[low]) + w * (R(t.cosine[low + 1]) - R(t.cosine[low]));
       sin2 = one_minus * one_plus;
       n.lut += 8;
     }
     const R sigma = R(2) * pi * R(uniform(rng)) - pi;
-    const Elements<R> f = elements<R>(s, species, cosine, n.lut);
+    const Elements<R> f = elements<R>(s, species, cosine, one_minus, one_plus, n.lut);
     const R pdf = f.f11 + f.f12 * (st.q * cos(R(2) * sigma) - st.u * sin(R(2) * sigma));
     const R bound =
-        species < 0 ? elements<R>(s, -1, R(1), n.lut).f11 : f.f11 + fabs(f.f12);
+        species < 0 ? elements<R>(s, -1, R(1), R(0), R(2), n.lut).f11 : f.f11 + fabs(f.f12);
     if (R(uniform(rng)) * bound > pdf) continue;
 
     sampled = cosine;
+    sampled_minus = one_minus;
+    sampled_plus = one_plus;
     const R sin_in = sqrt(fmax(eps, R(1) - uo * uo));
     const R sin_sca = sqrt(fmax(eps, sin2));
-    u1 = clamp1(uo * cosine + sin_in * sin_sca * cos(sigma));
-    const R frac = (cosine - uo * u1) / (sin_in * sqrt(fmax(eps, R(1) - u1 * u1)));
-    dphi = -acos(clamp1(frac)) * copysign(R(1), sigma);
+    const R cs = cos(sigma), ss = sin(sigma);
+    u1 = clamp1(uo * cosine + sin_in * sin_sca * cs);
+    // Azimuth change of the spherical triangle: the Fortran's acos((cos - uo u1) /
+    // (sin_in sin_out)) written as an atan2 of its sine and cosine, which stays accurate for
+    // the small azimuth changes of forward-peaked scattering (acos near 1 does not).
+    const R magnitude = atan2(sin_sca * fabs(ss), cosine * sin_in - uo * sin_sca * cs);
+    dphi = -copysign(magnitude, sigma);
     return true;
   }
   return false;
@@ -466,23 +500,28 @@ template <typename R> __device__ inline bool finite3(const Stokes<R> &v) {
   return isfinite(v.i) && isfinite(v.q) && isfinite(v.u);
 }
 
-// One photon history (mc.f90:314-1436).
-template <typename R, bool TRACE>
-__device__ int trace_photon(const Scene &s, const Controls &c, uint32_t lane,
-                            R *__restrict__ tallies, unsigned *failure, R *history,
-                            int hstride, Counts &n, Trace<R> &trace) {
-  const R eps = R(1e-16);
-  const R pi = R(3.14159265358979323846);
-  const uint32_t identity = c.first + lane;
+// State of one photon history between events.
+template <typename R> struct History {
+  Photon<R> p;
+  Stokes<R> st;
+  R weight, remains;
+  Stream rng;
+  R *bank;
+  uint32_t steps;
+};
+
+// Launch at TOA in the vacuum layer of column photon mod (nx*ny) (mc.f90:351-464).
+template <typename R>
+__device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, uint32_t lane,
+                              R *__restrict__ tallies, R *history, int hstride, History<R> &h) {
+  const uint32_t identity = c.first + id;
   const uint32_t round_offset = identity / c.per_round;
   const uint32_t photon = identity % c.per_round;
-  R *bank = tallies + size_t(round_offset * uint32_t(s.shards) + lane % uint32_t(s.shards)) *
-                          size_t(s.tally_count);
-  Stream rng{photon, c.round + round_offset, c.seed, 0u, 0u, Words{0, 0, 0, 0}, 4};
-
-  // Launch at TOA in the vacuum layer of column photon mod (nx*ny) (mc.f90:351-420).
+  h.bank = tallies + size_t(round_offset * uint32_t(s.shards) + lane % uint32_t(s.shards)) *
+                         size_t(s.tally_count);
+  h.rng = Stream{photon, c.round + round_offset, c.seed, 0u, 0u, Words{0, 0, 0, 0}, 4};
   const int column = int(photon % uint32_t(s.nx * s.ny));
-  Photon<R> p;
+  Photon<R> &p = h.p;
   p.i = column % s.nx;
   p.j = column / s.nx;
   p.k = s.nz - 1;
@@ -491,127 +530,154 @@ __device__ int trace_photon(const Scene &s, const Controls &c, uint32_t lane,
   p.b = sin(t0) * sin(p0);
   p.c = cos(t0);
   p.phi = p0;
-  p.rx = R(s.xunit) * R(uniform(rng));
-  p.ry = R(s.yunit) * R(uniform(rng));
+  p.rx = R(s.xunit) * R(uniform(h.rng));
+  p.ry = R(s.yunit) * R(uniform(h.rng));
   p.rz = R(s.zunit[p.k]);
   for (int k = 0; k < s.nz; ++k) history[k * hstride] = R(0);
-  Stokes<R> st{R(1), R(0), R(0)};
-  R weight = R(1);
-  R remains = -log(R(uniform(rng)));
-  ++n.photons;
+  h.st = Stokes<R>{R(1), R(0), R(0)};
+  h.weight = R(1);
+  h.remains = -log(R(uniform(h.rng)));
+  h.steps = 0;
+}
 
-  for (uint32_t step = 0; step < c.max_steps; ++step) {
-    // Free path: event inside this cell or move to the next face (mc.f90:476-677).
+// Advance one photon to its next collision or surface hit and process it (mc.f90:476-1431).
+// Returns kRunning while the history continues, else its end status.
+template <typename R, bool TRACE>
+__device__ int next_event(const Scene &s, const Controls &c, unsigned *failure, R *history,
+                          int hstride, History<R> &h, Counts &n, Trace<R> &trace) {
+  const R eps = R(1e-16);
+  const R pi = R(3.14159265358979323846);
+  Photon<R> &p = h.p;
+  bool surface = false;
+
+  // Free path: event inside this cell or move to the next face (mc.f90:476-677).
+  for (;;) {
+    if (++h.steps > c.max_steps) {
+      atomicOr(failure, kHistoryLimit);
+      return kFailed;
+    }
     const R extinction =
         (p.k < s.nz - 1 && p.k >= 0) ? R(s.beta[cell_index<R>(s, p.i, p.j, p.k)]) : R(0);
-    const R event_distance = extinction > eps ? remains / extinction : R(INFINITY);
-    bool surface = false;
+    const R event_distance = extinction > eps ? h.remains / extinction : R(INFINITY);
     if (event_distance < boundary_distance<R>(p)) {
       p.rx = fmax(R(0), p.rx - fabs(p.a) * event_distance);
       p.ry = fmax(R(0), p.ry - fabs(p.b) * event_distance);
       p.rz = fmax(R(0), p.rz - fabs(p.c) * event_distance);
       if (s.paths) history[p.k * hstride] += event_distance;
-    } else {
-      const int old_layer = p.k;
-      const R d = advance<R>(s, p);
-      ++n.walk;
-      if (s.paths) history[old_layer * hstride] += d;
-      remains -= d * extinction;
-      if (p.k >= s.nz) return kEscaped;
-      if (p.k >= 0) continue;
-      surface = true;
-      p.k = 0;
-      p.rz = R(0);
+      break;
     }
+    const int old_layer = p.k;
+    const R d = advance<R>(s, p);
+    ++n.walk;
+    if (s.paths) history[old_layer * hstride] += d;
+    h.remains -= d * extinction;
+    if (p.k >= s.nz) return kEscaped;
+    if (p.k >= 0) continue;
+    surface = true;
+    p.k = 0;
+    p.rz = R(0);
+    break;
+  }
 
-    // Surface albedo or species choice with absorption weight (mc.f90:681-727, 1028-1074).
-    int species = -1;
+  // Surface albedo or species choice with absorption weight (mc.f90:681-727, 1028-1074).
+  int species = -1;
+  if (surface) {
+    h.weight *= R(s.albedo[p.i + s.nx * p.j]);
+    ++n.surface;
+    if (TRACE) trace.surface += R(1);
+  } else {
+    const float draw = uniform(h.rng);
+    const int cell = cell_index<R>(s, p.i, p.j, p.k);
+    const int cells = s.nx * s.ny * s.nz;
+    for (int t = 0; t < s.species; ++t) {
+      if (draw < s.cumfrac[t * cells + cell]) {
+        species = t;
+        break;
+      }
+    }
+    h.weight *= species < 0 ? R(s.ssalb_gas[p.k]) : R(s.sp[species].ssalb);
+    ++n.events;
+    if (TRACE) trace.events += R(1);
+  }
+  if (h.weight < eps) return kAbsorbed;
+
+  // Local estimate toward every view (mc.f90:738-822 surface, 1109-1207 atmosphere).
+  for (int view = 0; view < s.views; ++view) {
+    const R theta = R(s.view[view]);
+    const R mu = cos(theta);
+    Stokes<R> prob;
     if (surface) {
-      weight *= R(s.albedo[p.i + s.nx * p.j]);
-      ++n.surface;
-      if (TRACE) trace.surface += R(1);
+      prob = Stokes<R>{h.st.i * mu * h.weight, R(0), R(0)};
     } else {
-      const float draw = uniform(rng);
-      const int cell = cell_index<R>(s, p.i, p.j, p.k);
-      const int cells = s.nx * s.ny * s.nz;
-      for (int t = 0; t < s.species; ++t) {
-        if (draw < s.cumfrac[t * cells + cell]) {
-          species = t;
-          break;
-        }
-      }
-      weight *= species < 0 ? R(s.ssalb_gas[p.k]) : R(s.sp[species].ssalb);
-      ++n.events;
-      if (TRACE) trace.events += R(1);
+      // Sines floored at sqrt(1e-16) as mc.f90:1112-1113 does: the forward peak of a Mie
+      // table is steep enough for this to matter in float64.
+      const R cosine = clamp1(p.c * mu + sqrt(fmax(eps, R(1) - p.c * p.c)) *
+                                             sqrt(fmax(eps, R(1) - mu * mu)) * cos(p.phi - R(s.phi)));
+      // 1 -/+ cos from the chord between the photon and view directions: accurate to the
+      // direction's own rounding even within a fraction of a degree of the peaks.
+      const R vx = sin(theta) * cos(R(s.phi)), vy = sin(theta) * sin(R(s.phi));
+      const R dx = p.a - vx, dy = p.b - vy, dz = p.c - mu;
+      const R ex = p.a + vx, ey = p.b + vy, ez = p.c + mu;
+      const R one_minus = R(0.5) * (dx * dx + dy * dy + dz * dz);
+      const R one_plus = R(0.5) * (ex * ex + ey * ey + ez * ez);
+      const Elements<R> f = elements<R>(s, species, cosine, one_minus, one_plus, n.lut);
+      const Rotation<R> r = s.nmat > 1 ? rotation<R>(cosine, p.c, mu, p.phi, R(s.phi))
+                                       : Rotation<R>{R(1), R(0), R(1), R(0)};
+      prob = scatter<R>(f, r, h.st);
+      prob.i *= h.weight;
+      prob.q *= h.weight;
+      prob.u *= h.weight;
     }
-    if (weight < eps) return kAbsorbed;
-
-    // Local estimate toward every view (mc.f90:738-822 surface, 1109-1207 atmosphere).
-    for (int view = 0; view < s.views; ++view) {
-      const R theta = R(s.view[view]);
-      const R mu = cos(theta);
-      Stokes<R> prob;
-      if (surface) {
-        prob = Stokes<R>{st.i * mu * weight, R(0), R(0)};
-      } else {
-        const R cosine = clamp1(p.c * mu + sqrt(fmax(R(0), R(1) - p.c * p.c)) * sin(theta) *
-                                               cos(p.phi - R(s.phi)));
-        const Elements<R> f = elements<R>(s, species, cosine, n.lut);
-        const Rotation<R> r = s.nmat > 1 ? rotation<R>(cosine, p.c, mu, p.phi, R(s.phi))
-                                         : Rotation<R>{R(1), R(0), R(1), R(0)};
-        prob = scatter<R>(f, r, st);
-        prob.i *= weight;
-        prob.q *= weight;
-        prob.u *= weight;
-      }
-      if (!finite3(prob)) {
-        atomicOr(failure, kNonfiniteProbability);
-        return kFailed;
-      }
-      if (!register_view<R, TRACE>(s, p, view, prob, history, hstride, bank, c.max_steps, n, trace)) {
-        atomicOr(failure, kDetectorLimit);
-        return kFailed;
-      }
+    if (!finite3(prob)) {
+      atomicOr(failure, kNonfiniteProbability);
+      return kFailed;
     }
-
-    // Continuation direction (mc.f90:1266-1431; Lambertian GROUND_LAMB2 for the surface).
-    R mu, phi;
-    if (surface) {
-      mu = sqrt(R(uniform(rng)));
-      phi = R(2) * pi * R(uniform(rng));
-      st = Stokes<R>{R(1), R(0), R(0)};
-    } else {
-      R dphi, cosine;
-      if (!sample_direction<R>(s, species, st, p.c, rng, mu, dphi, cosine, n)) {
-        atomicOr(failure, kSamplingLimit);
-        return kFailed;
-      }
-      phi = p.phi + dphi;
-      // The sampled scattering cosine is used directly, as in mc.f90.
-      if (s.nmat > 1) {
-        const Elements<R> f = elements<R>(s, species, cosine, n.lut);
-        const Rotation<R> r = rotation<R>(cosine, p.c, mu, p.phi, phi);
-        st = scatter<R>(f, r, st);
-        const R norm = st.i;
-        st.i = R(1);
-        st.q /= norm;
-        st.u /= norm;
-        if (!finite3(st) || sqrt(st.q * st.q + st.u * st.u) > R(1.01)) {
-          atomicOr(failure, kInvalidStokes);
-          return kFailed;
-        }
-      }
-      remains = -log(R(uniform(rng)));
-    }
-    const R sine = sqrt(fmax(R(0), R(1) - mu * mu));
-    change_direction<R>(s, p, cos(phi) * sine, sin(phi) * sine, mu, phi);
-    if (TRACE) {
-      trace.weight = weight;
-      trace.layer = R(p.k);
+    if (!register_view<R, TRACE>(s, p, view, prob, history, hstride, h.bank, c.max_steps, n, trace)) {
+      atomicOr(failure, kDetectorLimit);
+      return kFailed;
     }
   }
-  atomicOr(failure, kHistoryLimit);
-  return kFailed;
+
+  // Continuation direction (mc.f90:1266-1431; Lambertian GROUND_LAMB2 for the surface).
+  R mu, phi;
+  if (surface) {
+    mu = sqrt(R(uniform(h.rng)));
+    phi = R(2) * pi * R(uniform(h.rng));
+    h.st = Stokes<R>{R(1), R(0), R(0)};
+  } else {
+    R dphi, cosine, one_minus, one_plus;
+    if (!sample_direction<R>(s, species, h.st, p.c, h.rng, mu, dphi, cosine, one_minus, one_plus, n)) {
+      atomicOr(failure, kSamplingLimit);
+      return kFailed;
+    }
+    phi = p.phi + dphi;
+    // float32 keeps the azimuth in [-pi, pi]: every use is periodic, and an azimuth that
+    // random-walks away from zero loses the resolution the coplanarity test needs.
+    if (sizeof(R) == 4) phi = remainder(phi, R(2) * pi);
+    // The sampled scattering cosine is used directly, as in mc.f90.
+    if (s.nmat > 1) {
+      const Elements<R> f = elements<R>(s, species, cosine, one_minus, one_plus, n.lut);
+      const Rotation<R> r = rotation<R>(cosine, p.c, mu, p.phi, phi);
+      Stokes<R> st = scatter<R>(f, r, h.st);
+      const R norm = st.i;
+      st.i = R(1);
+      st.q /= norm;
+      st.u /= norm;
+      if (!finite3(st) || sqrt(st.q * st.q + st.u * st.u) > R(1.01)) {
+        atomicOr(failure, kInvalidStokes);
+        return kFailed;
+      }
+      h.st = st;
+    }
+    h.remains = -log(R(uniform(h.rng)));
+  }
+  const R sine = sqrt(fmax(R(0), R(1) - mu * mu));
+  change_direction<R>(s, p, cos(phi) * sine, sin(phi) * sine, mu, phi);
+  if (TRACE) {
+    trace.weight = h.weight;
+    trace.layer = R(p.k);
+  }
+  return kRunning;
 }
 
 __device__ inline unsigned long long wave_sum(unsigned long long v) {
@@ -619,9 +685,21 @@ __device__ inline unsigned long long wave_sum(unsigned long long v) {
   return v;
 }
 
+__device__ inline void flush_counts(Counts &n, unsigned long long *counters) {
+  const uint32_t values[kCounters] = {n.photons, n.events, n.surface, n.walk,
+                                      n.detector, n.lut, n.attempts, n.atomics};
+  for (int i = 0; i < kCounters; ++i)
+    if (values[i]) atomicAdd(counters + i, static_cast<unsigned long long>(values[i]));
+  n = Counts{0, 0, 0, 0, 0, 0, 0, 0};
+}
+
+// Persistent transport: every lane traces photons one event at a time and takes the next
+// photon id from a wave-aggregated work counter as soon as its history ends, so a wave
+// never idles behind its longest history. Photon ids (and with them the random streams and
+// the order of draws) do not depend on which lane runs them.
 template <typename R, bool TRACE>
 __global__ void __launch_bounds__(kBlock32) transport_kernel(Scene s, Controls c, R *tallies,
-                                                             unsigned *failure,
+                                                             unsigned *failure, unsigned *work,
                                                              unsigned long long *counters,
                                                              float *records, uint32_t record_count) {
   extern __shared__ unsigned char lds_raw[];
@@ -629,11 +707,34 @@ __global__ void __launch_bounds__(kBlock32) transport_kernel(Scene s, Controls c
   const int hstride = blockDim.x;
   const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
   Counts n{0, 0, 0, 0, 0, 0, 0, 0};
-  if (lane < c.count) {
-    Trace<R> trace{R(0), R(0), R(0), R(1), R(0), R(0), R(0), R(0)};
-    const int status = trace_photon<R, TRACE>(s, c, lane, tallies, failure, history, hstride, n, trace);
-    if (TRACE && lane < record_count) {
-      float *out = records + size_t(lane) * kTraceWords;
+  History<R> h;
+  Trace<R> trace;
+  bool active = false;
+  uint32_t id = 0;
+
+  for (;;) {
+    const unsigned long long idle = __ballot(!active);
+    if (idle) {
+      const int leader = __ffsll(static_cast<long long>(idle)) - 1;
+      uint32_t base = 0;
+      if (__lane_id() == leader) base = atomicAdd(work, uint32_t(__popcll(idle)));
+      base = __shfl(base, leader, 64);
+      if (!active) {
+        id = base + uint32_t(__popcll(idle & __lanemask_lt()));
+        if (id < c.count) {
+          launch_photon<R>(s, c, id, lane, tallies, history, hstride, h);
+          trace = Trace<R>{R(0), R(0), R(0), R(1), R(0), R(0), R(0), R(0)};
+          ++n.photons;
+          active = true;
+        }
+      }
+    }
+    if (!active) break;
+    const int status = next_event<R, TRACE>(s, c, failure, history, hstride, h, n, trace);
+    if (status == kRunning) continue;
+    active = false;
+    if (TRACE && id < record_count) {
+      float *out = records + size_t(id) * kTraceWords;
       out[0] = float(trace.events);
       out[1] = float(trace.surface);
       out[2] = float(status);
@@ -643,13 +744,16 @@ __global__ void __launch_bounds__(kBlock32) transport_kernel(Scene s, Controls c
       out[6] = float(trace.paths);
       out[7] = float(trace.layer);
     }
+    // Keep the 32-bit lane counters far from overflow on very long dispatches.
+    if ((n.walk | n.detector | n.lut | n.attempts | n.atomics) >= (1u << 30)) flush_counts(n, counters);
   }
+
   const uint32_t values[kCounters] = {n.photons, n.events, n.surface, n.walk,
                                       n.detector, n.lut, n.attempts, n.atomics};
 #pragma unroll
   for (int i = 0; i < kCounters; ++i) {
     const unsigned long long total = wave_sum(static_cast<unsigned long long>(values[i]));
-    if ((threadIdx.x & 63) == 0 && total) atomicAdd(counters + i, total);
+    if (__lane_id() == 0 && total) atomicAdd(counters + i, total);
   }
 }
 
@@ -676,8 +780,9 @@ struct Session {
   float *tables = nullptr;
   void *banks = nullptr;  // float or double banks, sized for double
   double *reduced = nullptr;
-  unsigned *failure = nullptr;
+  unsigned *failure = nullptr;  // failure[0]: flags; failure[1]: work counter
   unsigned long long *counters = nullptr;
+  int compute_units = 0;
   float *records = nullptr;
   size_t record_capacity = 0;
   size_t tally_count = 0;  // per round, all shards
@@ -827,6 +932,7 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
   const Scene &s = h->scene;
   const size_t per_round = size_t(s.tally_count) * size_t(s.shards);
   if (!c.per_round || !c.rounds || c.rounds > h->round_capacity || !c.max_steps ||
+      c.count > (1u << 31) ||
       per_round != h->tally_count || uint64_t(c.first) + c.count > uint64_t(c.per_round) * c.rounds) {
     set_error(error, error_size, "Invalid HIP round group");
     return 1;
@@ -835,8 +941,8 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
   const size_t active = per_round * c.rounds;
   if (!check(hipMemsetAsync(h->banks, 0, active * sizeof(R), h->stream), error, error_size,
              "clear tallies") ||
-      !check(hipMemsetAsync(h->failure, 0, sizeof(unsigned), h->stream), error, error_size,
-             "clear failure flag"))
+      !check(hipMemsetAsync(h->failure, 0, 2 * sizeof(unsigned), h->stream), error, error_size,
+             "clear failure flag and work counter"))
     return 1;
   if (TRACE) {
     if (record_count > h->record_capacity) {
@@ -851,12 +957,19 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
   }
   const int block = sizeof(R) == 4 ? kBlock32 : kBlock64;
   const size_t lds = size_t(s.nz) * block * sizeof(R);
-  const uint32_t grid = (c.count + block - 1) / block;
+  // Persistent grid: enough resident blocks to fill every CU, never more than the work.
+  int resident = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, TRACE>, block,
+                                                  lds) != hipSuccess || resident < 1)
+    resident = 1;
+  const uint64_t wanted = (uint64_t(c.count) + block - 1) / block;
+  const uint64_t filled = uint64_t(resident) * uint64_t(h->compute_units > 0 ? h->compute_units : 1);
+  const uint32_t grid = uint32_t(wanted < filled ? wanted : filled);
   R *banks = static_cast<R *>(h->banks);
   (void)hipEventRecord(h->start, h->stream);
   if (grid) {
     hipLaunchKernelGGL((transport_kernel<R, TRACE>), dim3(grid), dim3(block), lds, h->stream, s, c,
-                       banks, h->failure, h->counters, TRACE ? h->records : nullptr,
+                       banks, h->failure, h->failure + 1, h->counters, TRACE ? h->records : nullptr,
                        uint32_t(TRACE ? record_count : 0));
   }
   (void)hipEventRecord(h->stop, h->stream);
@@ -905,7 +1018,7 @@ __global__ void probe_scatter_kernel(const double *in, double *out, int cases) {
   s.depol = float(c[4]);
   s.nmat = 3;
   uint32_t reads = 0;
-  const Elements<float> f = elements<float>(s, -1, cosine, reads);
+  const Elements<float> f = elements<float>(s, -1, cosine, 1.0f - cosine, 1.0f + cosine, reads);
   const Rotation<float> r = rotation<float>(cosine, uo, u1, phio, phi1);
   const Stokes<float> p = scatter<float>(f, r, Stokes<float>{float(c[5]), float(c[6]), float(c[7])});
   double *o = out + 12 * n;
@@ -983,7 +1096,7 @@ void *monki_hip_create(uint32_t device, const float *tables, size_t table_count,
              "allocate tally banks") ||
       !check(hipMalloc(&h->reduced, size_t(parsed.tally_count) * round_capacity * sizeof(double)),
              error, error_size, "allocate reduced tallies") ||
-      !check(hipMalloc(&h->failure, sizeof(unsigned)), error, error_size, "allocate flag") ||
+      !check(hipMalloc(&h->failure, 2 * sizeof(unsigned)), error, error_size, "allocate flag") ||
       !check(hipMalloc(&h->counters, kCounters * sizeof(unsigned long long)), error, error_size,
              "allocate counters") ||
       !check(hipMemcpy(h->tables, tables, table_count * sizeof(float), hipMemcpyHostToDevice), error,
@@ -994,6 +1107,7 @@ void *monki_hip_create(uint32_t device, const float *tables, size_t table_count,
     return nullptr;
   }
   snprintf(h->name, sizeof(h->name), "%s (%s)", prop.name, prop.gcnArchName);
+  h->compute_units = prop.multiProcessorCount;
   parse_tables(tables, table_count, h->tables, h->scene, nullptr, 0);
   return h;
 }

@@ -21,11 +21,15 @@
  * Differences from the Fortran, all deliberate: Philox streams instead of RANDOM_NUMBER
  * (the Fortran's compiler RNG has no counter form); tabulated-element brackets by binary
  * search (equal to the equal-angle index estimate of mie_scat_matrix on equal-angle grids);
- * the sampling loop is bounded at 4096 attempts (transport.metal:184) instead of unbounded.
+ * the sampling loop is bounded at 4096 attempts (transport.metal:184) instead of unbounded;
+ * the azimuth change after scattering is the same spherical-triangle angle written as an
+ * atan2 instead of an acos (well conditioned for small angles).
  * The detector walk is always explicit, cell by cell, with the analytical pixel mapping
  * of mc_routines_cartesian_mod.f90:1076-1105: no uniform-top shortcut.
  *
- * Input is the packed float32 table buffer of the product (layout v1), widened to double.
+ * Input is the packed float32 table buffer of the product (layout v1), widened to double. Its
+ * cosine block carries 1 - cos and 1 + cos next to cos, so sin^2 of a sampled scattering angle
+ * keeps float64 precision near the forward peak although the cosines are stored in float32.
  */
 #include <math.h>
 #include <stdint.h>
@@ -127,16 +131,36 @@ void oracle_tabulated(const float *cosine, const float *f11, const float *f22, c
   f[3] = lerp2(x0, x1, f12[lo], f12[lo + 1], c) / 4.0;
 }
 
-static void elements(const scene_t *s, int species, double c, double f[4]) {
+/* Linear interpolation in cos (mie_scat_matrix) with the weight taken against the packed
+ * 1 - cos (1 + cos) nodes near the forward (backward) peak: the same affine interpolation,
+ * with node positions that keep float64 precision although cos is stored in float32.
+ * one_minus / one_plus are 1 - c and 1 + c of the scattering angle. */
+static void elements_at(const scene_t *s, int species, double c, double one_minus, double one_plus,
+                        double f[4]) {
   if (species < 0) {
     if (c < -1.0) c = -1.0;
     if (c > 1.0) c = 1.0;
     oracle_rayleigh(s->depol, c, f);
   } else {
     const species_t *t = &s->sp[species];
-    oracle_tabulated(t->cosine, t->f11, t->f22, t->f33, t->f12, t->lut, c, f);
+    if (c < -1.0) c = -1.0;
+    if (c > 1.0) c = 1.0;
+    int n = t->lut, lo = bracket_cos(t->cosine, n, c);
+    double w;
+    if (c > 0.5)
+      w = ((double)t->cosine[n + lo] - one_minus) / ((double)t->cosine[n + lo] - t->cosine[n + lo + 1]);
+    else if (c < -0.5)
+      w = (one_plus - t->cosine[2 * n + lo]) / ((double)t->cosine[2 * n + lo + 1] - t->cosine[2 * n + lo]);
+    else
+      w = (c - t->cosine[lo]) / ((double)t->cosine[lo + 1] - t->cosine[lo]);
+    const float *tab[4] = {t->f11, t->f22, t->f33, t->f12};
+    for (int e = 0; e < 4; ++e) f[e] = (tab[e][lo] + w * ((double)tab[e][lo + 1] - tab[e][lo])) / 4.0;
   }
   if (s->nmat == 1) f[3] = 0.0;
+}
+
+static void elements(const scene_t *s, int species, double c, double f[4]) {
+  elements_at(s, species, c, 1.0 - c, 1.0 + c, f);
 }
 
 /* stokes_rot_mat: out = (C1, S1, C2, S2) = (L1(2,2), L1(3,2), L2(2,2), L2(3,2)). */
@@ -189,50 +213,63 @@ void oracle_scatter(const double f[4], const double r[4], const double s[3], dou
   for (int i = 0; i < 3; ++i) out[i] = P[i][0] * s[0] + P[i][1] * s[1] + P[i][2] * s[2];
 }
 
-/* Spherical trigonometry of rayl_scatter_pol / mie_scatter_pol_bound after acceptance. */
-void oracle_new_direction(double cos_in, double cos_sca, double sigma, double *cos_out, double *dphi) {
-  double a = 1.0 - cos_in * cos_in, b = 1.0 - cos_sca * cos_sca;
-  double sin_in = sqrt(a > EPS ? a : EPS), sin_sca = sqrt(b > EPS ? b : EPS);
+/* Spherical trigonometry of rayl_scatter_pol / mie_scatter_pol_bound after acceptance, with
+ * sin^2 of the scattering angle given (the Fortran's 1 - cos^2, evaluated as (1-cos)(1+cos)). */
+static void new_direction(double cos_in, double cos_sca, double sin2, double sigma, double *cos_out,
+                          double *dphi) {
+  double a = 1.0 - cos_in * cos_in;
+  double sin_in = sqrt(a > EPS ? a : EPS), sin_sca = sqrt(sin2 > EPS ? sin2 : EPS);
   double co = cos_in * cos_sca + sin_in * sin_sca * cos(sigma);
   if (co > 1.0) co = 1.0;
   if (co < -1.0) co = -1.0;
-  double c = 1.0 - co * co, sin_out = sqrt(c > EPS ? c : EPS);
-  double frac = (cos_sca - cos_in * co) / (sin_in * sin_out);
-  double d;
-  if (frac >= 1.0 - EPS)
-    d = 0.0;
-  else if (frac <= -1.0 + EPS)
-    d = PI;
-  else
-    d = acos(frac) * (sigma >= 0 ? 1.0 : -1.0);
+  /* |dphi| = acos((cos_sca - cos_in cos_out) / (sin_in sin_out)) of the Fortran, evaluated as
+   * the atan2 of the same angle's sine and cosine: acos near +-1 loses half the digits, and the
+   * forward peak of a cloud amplifies that loss along a history. */
+  double magnitude = atan2(sin_sca * fabs(sin(sigma)), cos_sca * sin_in - cos_in * sin_sca * cos(sigma));
   *cos_out = co;
-  *dphi = -d;
+  *dphi = sigma >= 0 ? -magnitude : magnitude;
+}
+
+void oracle_new_direction(double cos_in, double cos_sca, double sigma, double *cos_out, double *dphi) {
+  new_direction(cos_in, cos_sca, 1.0 - cos_sca * cos_sca, sigma, cos_out, dphi);
 }
 
 /* Rejection sampling, forward sign -1. Returns 0 after ATTEMPTS rejections. */
 static int sample(const scene_t *s, int species, const double st[3], double cos_in, rng_t *r,
-                  double *cos_out, double *dphi, double *cos_sca, uint64_t *attempts) {
+                  double *cos_out, double *dphi, double *cos_sca, double sampled[2],
+                  uint64_t *attempts) {
   double bound_ray[4];
   oracle_rayleigh(s->depol, 1.0, bound_ray);
   for (int attempt = 0; attempt < ATTEMPTS; ++attempt) {
     ++*attempts;
-    double c;
+    /* sin^2 = (1 - c)(1 + c): exact for Rayleigh (c = 2u - 1); for tabulated species both
+     * factors are interpolated from the 1 - cos and 1 + cos rows of the packed cosine block,
+     * which keep the precision of the float64 angle grid near the peaks. */
+    double c, sin2, one_minus, one_plus;
     if (species < 0) {
-      c = 2.0 * draw(r) - 1.0;
+      double u = draw(r);
+      c = 2.0 * u - 1.0;
+      one_minus = 2.0 - 2.0 * u;
+      one_plus = 2.0 * u;
     } else {
       const species_t *t = &s->sp[species];
       double u = draw(r);
       int lo = bracket_cos(t->cdf, t->lut, u);
+      const float *omc = t->cosine + t->lut, *opc = t->cosine + 2 * t->lut;
       c = lerp2(t->cdf[lo], t->cdf[lo + 1], t->cosine[lo], t->cosine[lo + 1], u);
+      one_minus = lerp2(t->cdf[lo], t->cdf[lo + 1], omc[lo], omc[lo + 1], u);
+      one_plus = lerp2(t->cdf[lo], t->cdf[lo + 1], opc[lo], opc[lo + 1], u);
     }
+    sin2 = one_minus * one_plus;
     double sigma = 2.0 * PI * draw(r) - PI;
     double f[4];
-    elements(s, species, c, f);
+    elements_at(s, species, c, one_minus, one_plus, f);
     double pdf = f[0] + f[3] * (st[1] * cos(2.0 * sigma) - st[2] * sin(2.0 * sigma));
     double bound = species < 0 ? bound_ray[0] : f[0] + fabs(f[3]);
     if (draw(r) * bound > pdf) continue;
     *cos_sca = c;
-    oracle_new_direction(cos_in, c, sigma, cos_out, dphi);
+    sampled[0] = one_minus, sampled[1] = one_plus;
+    new_direction(cos_in, c, sin2, sigma, cos_out, dphi);
     return 1;
   }
   return 0;
@@ -441,15 +478,15 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
       phi_out = 2.0 * PI * draw(&r);
       st[0] = 1.0, st[1] = 0.0, st[2] = 0.0;
     } else {
-      double dphi, cos_sca;
-      if (!sample(s, species, st, p.c, &r, &cos_out, &dphi, &cos_sca, &out->counters[6])) {
+      double dphi, cos_sca, sampled[2];
+      if (!sample(s, species, st, p.c, &r, &cos_out, &dphi, &cos_sca, sampled, &out->counters[6])) {
         *flag |= 4u;
         return FAILED;
       }
       phi_out = phi_in + dphi;
       if (s->nmat > 1) {
         double f[4], rot[4], ns[3];
-        elements(s, species, cos_sca, f);
+        elements_at(s, species, cos_sca, sampled[0], sampled[1], f);
         oracle_rotation(cos_sca, p.c, cos_out, phi_in, phi_out, rot);
         oracle_scatter(f, rot, st, ns);
         st[0] = 1.0, st[1] = ns[1] / ns[0], st[2] = ns[2] / ns[0];

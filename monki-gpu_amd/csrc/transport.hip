@@ -7,7 +7,7 @@
 // (precision reference on identical Philox streams).
 //
 // Layout on the device: packed float32 tables (layout v1, see monki/execution/hip_tables.py),
-// float32 tally banks [round][shard][stokes | paths | weights], per-lane layer history in LDS.
+// float32 tally banks [round][shard][view][pixel][layer paths | Stokes], per-lane layer paths in LDS.
 
 #include <hip/hip_runtime.h>
 
@@ -51,11 +51,15 @@ struct SpeciesTables {
 // Scene constants and table pointers, passed by value as a kernel argument.
 struct Scene {
   int nx, ny, nz, nmat, views, species, paths, uniform_top, shards;
-  int tally_count;  // floats per round per shard
+  int tally_count;  // output values per round: Stokes | paths | weights
   int stokes_count, path_count;
+  // Device banks per round and shard: one record per (view, pixel) holding the layer paths
+  // then the Stokes components.
+  int bank_count;
   float xunit, yunit, depol, theta0, phi0, phi;
   float view[kMaxViews];
   const float *zunit, *albedo, *beta, *cumfrac, *ssalb_gas, *height_above, *tau_above;
+  const float *uniform;  // per layer: 1 if every column has the same extinction and mix
   SpeciesTables sp[kMaxSpecies];
 };
 
@@ -244,12 +248,10 @@ __device__ inline Rotation<R> rotation(R cosine, R uo, R u1, R phio, R phi1) {
     cosi2 = (uo - u1 * cosine) / sqrt(fmax(e, (R(1) - u1 * u1) * sc));
     q = sino * spo * sin1 * cp1 - sino * cpo * sin1 * sp1;
   }
-  // float32 cancellation can push |cos i| past one; clamp as transport.metal:158-159 does.
-  // float64 keeps the unclamped Fortran form.
-  if (sizeof(R) == 4) {
-    cosi1 = clamp1(cosi1);
-    cosi2 = clamp1(cosi2);
-  }
+  // Cancellation, or the 1e-16 floor under a near-zero scattering angle, can push |cos i|
+  // past one; clamp as transport.metal:158-159 does (the Fortran only warns, :1142-1167).
+  cosi1 = clamp1(cosi1);
+  cosi2 = clamp1(cosi2);
   const R sign = q >= R(0) ? R(1) : R(-1);
   const R g1 = R(1) - cosi1 * cosi1, g2 = R(1) - cosi2 * cosi2;
   const R sini1 = g1 < tiny ? R(0) : sqrt(g1) * sign;
@@ -381,6 +383,30 @@ template <typename R> __device__ inline R advance(const Scene &s, Photon<R> &p) 
   return d;
 }
 
+// Move a distance along one horizontal axis inside a horizontally uniform layer: the photon
+// keeps its direction, so the cell index and the distance to the approached face follow from
+// the periodic position directly (positions in float64; a near-horizontal photon may travel
+// thousands of cells in one flight). Returns the number of faces crossed.
+template <typename R>
+__device__ inline uint32_t slide_axis(int &index, R &remaining, R direction, R distance, float unit,
+                                      int count) {
+  if (direction == R(0)) return 0;
+  const double u = double(unit);
+  const double offset = direction > R(0) ? u - double(remaining) : double(remaining);
+  const double travel = double(fabs(direction)) * double(distance);
+  const double moved = direction > R(0) ? offset + travel : u - offset + travel;
+  const uint32_t crossed = uint32_t(fmin(moved / u, 4.0e9));
+  const double span = u * double(count);
+  double x = fmod(double(index) * u + offset + double(direction) * double(distance), span);
+  if (x < 0.0) x += span;
+  int cell = int(x / u);
+  cell = cell < 0 ? 0 : (cell >= count ? count - 1 : cell);
+  const double local = fmin(fmax(x - double(cell) * u, 0.0), u);
+  index = cell;
+  remaining = R(direction > R(0) ? u - local : local);
+  return crossed;
+}
+
 // Column reached by a straight ray through uniform layers (transport.metal:270-279 and the
 // analytical register mapping of mc_routines_cartesian_mod.f90:1076-1105).
 template <typename R>
@@ -440,86 +466,42 @@ template <typename R> struct Trace {
   R events, surface, status, weight, stokes_i, weights, paths, layer;
 };
 
-// Local estimate for one view: detector walk, attenuation and all tallies of this event.
-template <typename R, bool TRACE>
-__device__ inline bool register_view(const Scene &s, const Photon<R> &p, int view,
-                                     const Stokes<R> &probability, const R *history, int hstride,
-                                     R *__restrict__ bank, uint32_t max_steps, Counts &n,
-                                     Trace<R> &trace) {
-  if (probability.i <= R(1e-16)) return true;
-  const R theta = R(s.view[view]);
-  const R mu = cos(theta), sv = sin(theta);
-  const R va = sv * cos(R(s.phi)), vb = sv * sin(R(s.phi));
-  R tau;
-  const DetectorHit hit = detector_walk<R>(s, p, va, vb, mu, tau, max_steps, n);
-  if (!hit.ok) return false;
-
-  const R attenuation = exp(-tau);
-  const R ci = probability.i * attenuation;
-  const R geometry = fabs(cos(R(s.theta0))) / mu;  // mu0 / mu (mc.f90:1456-1474)
-  const int pixel = hit.i + s.nx * (hit.j + s.ny * view);
-  add_tally<R>(bank + s.nmat * pixel, ci * geometry);
-  if (s.nmat > 1) {
-    add_tally<R>(bank + s.nmat * pixel + 1, probability.q * attenuation * geometry);
-    add_tally<R>(bank + s.nmat * pixel + 2, probability.u * attenuation * geometry);
-  }
-  n.atomics += s.nmat;
-  if (TRACE) {
-    trace.stokes_i += ci * geometry;
-  }
-  if (s.paths) {
-    // Detector-ray lengths per layer: the partial event layer, then full layers to TOA.
-    const R rz = p.c * mu < R(0) ? fmax(R(0), R(s.zunit[p.k]) - p.rz) : p.rz;
-    R *paths = bank + s.stokes_count + hit.i + s.nx * hit.j;
-    const int layer_stride = s.nx * s.ny;
-    const int view_offset = layer_stride * s.nz * view;
-    R total = R(0);
-    for (int k = 0; k < s.nz; ++k) {
-      R length = history[k * hstride];
-      if (k == p.k && k < s.nz - 1)
-        length += rz / mu;
-      else if (k > p.k && k < s.nz - 1)
-        length += R(s.zunit[k]) / mu;
-      if (length > R(0)) {
-        add_tally<R>(paths + view_offset + layer_stride * k, ci * length);
-        ++n.atomics;
-      }
-      if (TRACE) total += ci * length;
-    }
-    add_tally<R>(bank + s.stokes_count + s.path_count + pixel, ci);
-    ++n.atomics;
-    if (TRACE) {
-      trace.weights += ci;
-      trace.paths += total;
-    }
-  }
-  return true;
-}
-
-template <typename R> __device__ inline bool finite3(const Stokes<R> &v) {
-  return isfinite(v.i) && isfinite(v.q) && isfinite(v.u);
-}
-
-// State of one photon history between events.
+// State of one photon history between events. The layer paths of the photon (path_layers
+// of mc.f90) live in LDS, one row of layers per lane.
 template <typename R> struct History {
   Photon<R> p;
   Stokes<R> st;
   R weight, remains;
   Stream rng;
-  R *bank;
+  uint32_t round_offset;
   uint32_t steps;
+  int species;  // -1 gas, else tabulated species of the pending event
+  bool surface;
+  R path_total;  // sum of the layer paths, for the per-photon records
 };
+
+// Local estimates of one event, waiting in registers for the wave's cooperative tally.
+template <typename R> struct Pending {
+  bool has;
+  int layer;
+  int pixel[kMaxViews];  // (i + nx j), or -1 when the view gets nothing from this event
+  R weight[kMaxViews];   // C1 = I * exp(-tau): detector weight of the view
+  R ray[kMaxViews];      // partial detector-ray path inside the event layer
+  R stokes[kMaxViews][3];  // Stokes contribution times mu0/mu
+};
+
+template <typename R> __device__ inline bool finite3(const Stokes<R> &v) {
+  return isfinite(v.i) && isfinite(v.q) && isfinite(v.u);
+}
 
 // Launch at TOA in the vacuum layer of column photon mod (nx*ny) (mc.f90:351-464).
 template <typename R>
-__device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, uint32_t lane,
-                              R *__restrict__ tallies, R *history, int hstride, History<R> &h) {
+__device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R *history,
+                              History<R> &h) {
   const uint32_t identity = c.first + id;
-  const uint32_t round_offset = identity / c.per_round;
+  h.round_offset = identity / c.per_round;
   const uint32_t photon = identity % c.per_round;
-  h.bank = tallies + size_t(round_offset * uint32_t(s.shards) + lane % uint32_t(s.shards)) *
-                         size_t(s.tally_count);
-  h.rng = Stream{photon, c.round + round_offset, c.seed, 0u, 0u, Words{0, 0, 0, 0}, 4};
+  h.rng = Stream{photon, c.round + h.round_offset, c.seed, 0u, 0u, Words{0, 0, 0, 0}, 4};
   const int column = int(photon % uint32_t(s.nx * s.ny));
   Photon<R> &p = h.p;
   p.i = column % s.nx;
@@ -533,24 +515,25 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, ui
   p.rx = R(s.xunit) * R(uniform(h.rng));
   p.ry = R(s.yunit) * R(uniform(h.rng));
   p.rz = R(s.zunit[p.k]);
-  for (int k = 0; k < s.nz; ++k) history[k * hstride] = R(0);
+  for (int k = 0; k < s.nz; ++k) history[k] = R(0);
   h.st = Stokes<R>{R(1), R(0), R(0)};
   h.weight = R(1);
   h.remains = -log(R(uniform(h.rng)));
   h.steps = 0;
+  h.path_total = R(0);
 }
 
-// Advance one photon to its next collision or surface hit and process it (mc.f90:476-1431).
-// Returns kRunning while the history continues, else its end status.
+// Advance one photon to its next collision or surface hit (mc.f90:476-677), apply the albedo
+// or the species' single-scattering albedo (mc.f90:681-727, 1028-1074), and evaluate the local
+// estimate toward every view (mc.f90:738-822, 1109-1207; mc_routines_cartesian_mod.f90:849-1222)
+// into `out`. Returns kRunning while the history continues, else its end status.
 template <typename R, bool TRACE>
-__device__ int next_event(const Scene &s, const Controls &c, unsigned *failure, R *history,
-                          int hstride, History<R> &h, Counts &n, Trace<R> &trace) {
+__device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *history,
+                       History<R> &h, Pending<R> &out, Counts &n, Trace<R> &trace) {
   const R eps = R(1e-16);
-  const R pi = R(3.14159265358979323846);
   Photon<R> &p = h.p;
   bool surface = false;
 
-  // Free path: event inside this cell or move to the next face (mc.f90:476-677).
   for (;;) {
     if (++h.steps > c.max_steps) {
       atomicOr(failure, kHistoryLimit);
@@ -559,17 +542,52 @@ __device__ int next_event(const Scene &s, const Controls &c, unsigned *failure, 
     const R extinction =
         (p.k < s.nz - 1 && p.k >= 0) ? R(s.beta[cell_index<R>(s, p.i, p.j, p.k)]) : R(0);
     const R event_distance = extinction > eps ? h.remains / extinction : R(INFINITY);
+    if (s.uniform[p.k] != 0.0f) {
+      // Horizontally uniform layer: straight to the event or to the next z face. The path is
+      // the one the cell-by-cell walk takes (x/y faces change nothing here).
+      const R to_face = p.rz / fmax(fabs(p.c), eps);
+      const bool event = event_distance < to_face;
+      const R d = event ? event_distance : to_face;
+      n.walk += slide_axis<R>(p.i, p.rx, p.a, d, s.xunit, s.nx) +
+                slide_axis<R>(p.j, p.ry, p.b, d, s.yunit, s.ny);
+      if (s.paths) {
+        history[p.k] += d;
+        h.path_total += d;
+      }
+      if (event) {
+        p.rz = fmax(R(0), p.rz - fabs(p.c) * d);
+        break;
+      }
+      ++n.walk;
+      h.remains -= d * extinction;
+      p.k += p.c >= R(0) ? 1 : -1;
+      if (p.k >= s.nz) return kEscaped;
+      if (p.k >= 0) {
+        p.rz = R(s.zunit[p.k]);
+        continue;
+      }
+      surface = true;
+      p.k = 0;
+      p.rz = R(0);
+      break;
+    }
     if (event_distance < boundary_distance<R>(p)) {
       p.rx = fmax(R(0), p.rx - fabs(p.a) * event_distance);
       p.ry = fmax(R(0), p.ry - fabs(p.b) * event_distance);
       p.rz = fmax(R(0), p.rz - fabs(p.c) * event_distance);
-      if (s.paths) history[p.k * hstride] += event_distance;
+      if (s.paths) {
+        history[p.k] += event_distance;
+        h.path_total += event_distance;
+      }
       break;
     }
     const int old_layer = p.k;
     const R d = advance<R>(s, p);
     ++n.walk;
-    if (s.paths) history[old_layer * hstride] += d;
+    if (s.paths) {
+      history[old_layer] += d;
+      h.path_total += d;
+    }
     h.remains -= d * extinction;
     if (p.k >= s.nz) return kEscaped;
     if (p.k >= 0) continue;
@@ -579,7 +597,6 @@ __device__ int next_event(const Scene &s, const Controls &c, unsigned *failure, 
     break;
   }
 
-  // Surface albedo or species choice with absorption weight (mc.f90:681-727, 1028-1074).
   int species = -1;
   if (surface) {
     h.weight *= R(s.albedo[p.i + s.nx * p.j]);
@@ -599,10 +616,16 @@ __device__ int next_event(const Scene &s, const Controls &c, unsigned *failure, 
     ++n.events;
     if (TRACE) trace.events += R(1);
   }
+  h.species = species;
+  h.surface = surface;
   if (h.weight < eps) return kAbsorbed;
 
-  // Local estimate toward every view (mc.f90:738-822 surface, 1109-1207 atmosphere).
-  for (int view = 0; view < s.views; ++view) {
+  out.has = true;
+  out.layer = p.k;
+  const R geometry0 = fabs(cos(R(s.theta0)));
+  for (int view = 0; view < kMaxViews; ++view) {
+    out.pixel[view] = -1;
+    if (view >= s.views) continue;
     const R theta = R(s.view[view]);
     const R mu = cos(theta);
     Stokes<R> prob;
@@ -632,21 +655,50 @@ __device__ int next_event(const Scene &s, const Controls &c, unsigned *failure, 
       atomicOr(failure, kNonfiniteProbability);
       return kFailed;
     }
-    if (!register_view<R, TRACE>(s, p, view, prob, history, hstride, h.bank, c.max_steps, n, trace)) {
+    if (prob.i <= eps) continue;
+    const R va = sin(theta) * cos(R(s.phi)), vb = sin(theta) * sin(R(s.phi));
+    R tau;
+    const DetectorHit hit = detector_walk<R>(s, p, va, vb, mu, tau, c.max_steps, n);
+    if (!hit.ok) {
       atomicOr(failure, kDetectorLimit);
       return kFailed;
     }
+    const R attenuation = exp(-tau);
+    const R geometry = geometry0 / mu;  // mu0 / mu (mc.f90:1456-1474)
+    const R ci = prob.i * attenuation;
+    out.pixel[view] = hit.i + s.nx * hit.j;
+    out.weight[view] = ci;
+    // Detector ray inside the event layer: the distance to its top face along the view.
+    const R rz = p.c * mu < R(0) ? fmax(R(0), R(s.zunit[p.k]) - p.rz) : p.rz;
+    out.ray[view] = rz / mu;
+    out.stokes[view][0] = ci * geometry;
+    out.stokes[view][1] = prob.q * attenuation * geometry;
+    out.stokes[view][2] = prob.u * attenuation * geometry;
+    if (TRACE) {
+      trace.stokes_i += ci * geometry;
+      if (s.paths) {
+        trace.weights += ci;
+        trace.paths += ci * (h.path_total + (rz + R(s.height_above[p.k])) / mu);
+      }
+    }
   }
+  return kRunning;
+}
 
-  // Continuation direction (mc.f90:1266-1431; Lambertian GROUND_LAMB2 for the surface).
+// Continuation direction after the event (mc.f90:1266-1431; GROUND_LAMB2 at the surface).
+template <typename R, bool TRACE>
+__device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, Counts &n,
+                              Trace<R> &trace) {
+  const R pi = R(3.14159265358979323846);
+  Photon<R> &p = h.p;
   R mu, phi;
-  if (surface) {
+  if (h.surface) {
     mu = sqrt(R(uniform(h.rng)));
     phi = R(2) * pi * R(uniform(h.rng));
     h.st = Stokes<R>{R(1), R(0), R(0)};
   } else {
     R dphi, cosine, one_minus, one_plus;
-    if (!sample_direction<R>(s, species, h.st, p.c, h.rng, mu, dphi, cosine, one_minus, one_plus, n)) {
+    if (!sample_direction<R>(s, h.species, h.st, p.c, h.rng, mu, dphi, cosine, one_minus, one_plus, n)) {
       atomicOr(failure, kSamplingLimit);
       return kFailed;
     }
@@ -656,7 +708,7 @@ __device__ int next_event(const Scene &s, const Controls &c, unsigned *failure, 
     if (sizeof(R) == 4) phi = remainder(phi, R(2) * pi);
     // The sampled scattering cosine is used directly, as in mc.f90.
     if (s.nmat > 1) {
-      const Elements<R> f = elements<R>(s, species, cosine, one_minus, one_plus, n.lut);
+      const Elements<R> f = elements<R>(s, h.species, cosine, one_minus, one_plus, n.lut);
       const Rotation<R> r = rotation<R>(cosine, p.c, mu, p.phi, phi);
       Stokes<R> st = scatter<R>(f, r, h.st);
       const R norm = st.i;
@@ -680,6 +732,69 @@ __device__ int next_event(const Scene &s, const Controls &c, unsigned *failure, 
   return kRunning;
 }
 
+template <typename R> __device__ inline R broadcast(R value, int lane);
+
+template <> __device__ inline float broadcast<float>(float value, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(value), lane));
+}
+
+template <> __device__ inline double broadcast<double>(double value, int lane) {
+  const long long bits = __double_as_longlong(value);
+  const int lo = __builtin_amdgcn_readlane(int(bits & 0xffffffffll), lane);
+  const int hi = __builtin_amdgcn_readlane(int(bits >> 32), lane);
+  return __longlong_as_double((static_cast<long long>(hi) << 32) | (unsigned(lo)));
+}
+
+__device__ inline int broadcast_int(int value, int lane) {
+  return __builtin_amdgcn_readlane(value, lane);
+}
+
+// Wave-cooperative tally. The wave takes the pending events of its lanes one at a time and
+// adds, per view, one record [layer paths 0..nz-1 | Stokes 0..nmat-1] of the event's pixel in
+// a single atomic instruction: lane t adds layer t's C1 * (path_layers + path_layers_angle)
+// (mc_routines_cartesian_mod.f90:1173-1186, the photon's layer paths read from the source
+// lane's LDS row) or Stokes component t - nz. Contiguous lanes make each instruction a few
+// cache-line requests instead of one request per value.
+template <typename R>
+__device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int shard,
+                                  const Pending<R> &pending, uint32_t round_offset,
+                                  const R *wave_history, int row, Counts &n) {
+  const int lane = __lane_id();
+  const int record = s.paths ? s.nz + s.nmat : s.nmat;
+  const int plane = s.nx * s.ny;
+  for (uint64_t mask = __ballot(pending.has); mask; mask &= mask - 1) {
+    const int src = __ffsll(static_cast<long long>(mask)) - 1;
+    const int layer = broadcast_int(pending.layer, src);
+    const uint32_t round = uint32_t(broadcast_int(int(round_offset), src));
+    R *bank = banks + (size_t(round) * s.shards + shard) * size_t(s.bank_count);
+    const R *source = wave_history + src * row;
+    for (int view = 0; view < s.views; ++view) {
+      const int pixel = broadcast_int(pending.pixel[view], src);
+      if (pixel < 0) continue;
+      const R weight = broadcast<R>(pending.weight[view], src);
+      const R ray = broadcast<R>(pending.ray[view], src);
+      const R stokes0 = broadcast<R>(pending.stokes[view][0], src);
+      const R stokes1 = broadcast<R>(pending.stokes[view][1], src);
+      const R stokes2 = broadcast<R>(pending.stokes[view][2], src);
+      R value = R(0);
+      if (s.paths && lane < s.nz) {
+        const R mu = cos(R(s.view[view]));
+        const R length = lane == layer ? ray
+                         : (lane > layer && lane < s.nz - 1) ? R(s.zunit[lane]) / mu
+                                                            : R(0);
+        value = weight * (source[lane] + length);
+      } else {
+        const int m = lane - (s.paths ? s.nz : 0);
+        value = m == 0 ? stokes0 : (m == 1 ? stokes1 : stokes2);
+      }
+      if (lane < record && value != R(0)) {
+        add_tally<R>(bank + (size_t(view) * plane + pixel) * record + lane, value);
+        ++n.atomics;
+      }
+    }
+  }
+}
+
 __device__ inline unsigned long long wave_sum(unsigned long long v) {
   for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, 64);
   return v;
@@ -693,19 +808,25 @@ __device__ inline void flush_counts(Counts &n, unsigned long long *counters) {
   n = Counts{0, 0, 0, 0, 0, 0, 0, 0};
 }
 
-// Persistent transport: every lane traces photons one event at a time and takes the next
-// photon id from a wave-aggregated work counter as soon as its history ends, so a wave
-// never idles behind its longest history. Photon ids (and with them the random streams and
-// the order of draws) do not depend on which lane runs them.
+// Persistent transport. Each iteration, every lane advances its photon to the next event and
+// evaluates the local estimates (divergent work), the wave then tallies all pending events
+// cooperatively, and the lanes sample their continuation directions. A lane whose history
+// ended takes the next photon id from a wave-aggregated work counter, so a wave never idles
+// behind its longest history. Photon ids (and with them the random streams and the order of
+// draws) do not depend on which lane runs them. Tally banks are chosen per wave.
 template <typename R, bool TRACE>
-__global__ void __launch_bounds__(kBlock32) transport_kernel(Scene s, Controls c, R *tallies,
+__global__ void __launch_bounds__(kBlock32) transport_kernel(Scene s, Controls c, R *banks,
                                                              unsigned *failure, unsigned *work,
                                                              unsigned long long *counters,
                                                              float *records, uint32_t record_count) {
   extern __shared__ unsigned char lds_raw[];
-  R *history = reinterpret_cast<R *>(lds_raw) + threadIdx.x;
-  const int hstride = blockDim.x;
-  const uint32_t lane = blockIdx.x * blockDim.x + threadIdx.x;
+  // Layer rows of odd length: lanes reading one row, or each its own row at one layer,
+  // touch distinct LDS banks.
+  const int row = s.nz | 1;
+  R *history = reinterpret_cast<R *>(lds_raw) + threadIdx.x * row;
+  const R *wave_history = reinterpret_cast<R *>(lds_raw) + (threadIdx.x & ~63u) * row;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int shard = int(wave % uint32_t(s.shards));
   Counts n{0, 0, 0, 0, 0, 0, 0, 0};
   History<R> h;
   Trace<R> trace;
@@ -722,16 +843,22 @@ __global__ void __launch_bounds__(kBlock32) transport_kernel(Scene s, Controls c
       if (!active) {
         id = base + uint32_t(__popcll(idle & __lanemask_lt()));
         if (id < c.count) {
-          launch_photon<R>(s, c, id, lane, tallies, history, hstride, h);
+          launch_photon<R>(s, c, id, history, h);
           trace = Trace<R>{R(0), R(0), R(0), R(1), R(0), R(0), R(0), R(0)};
           ++n.photons;
           active = true;
         }
       }
     }
-    if (!active) break;
-    const int status = next_event<R, TRACE>(s, c, failure, history, hstride, h, n, trace);
-    if (status == kRunning) continue;
+    if (__ballot(active) == 0) break;
+
+    Pending<R> pending;
+    pending.has = false;
+    int status = kRunning;
+    if (active) status = collide<R, TRACE>(s, c, failure, history, h, pending, n, trace);
+    tally_wave<R>(s, banks, shard, pending, h.round_offset, wave_history, row, n);
+    if (active && status == kRunning) status = scatter_photon<R, TRACE>(s, failure, h, n, trace);
+    if (!active || status == kRunning) continue;
     active = false;
     if (TRACE && id < record_count) {
       float *out = records + size_t(id) * kTraceWords;
@@ -759,15 +886,44 @@ __global__ void __launch_bounds__(kBlock32) transport_kernel(Scene s, Controls c
 
 // Sum the shard banks of each round in float64: out[r][t] = sum_s banks[r][s][t].
 template <typename B>
-__global__ void reduce_shards(const B *__restrict__ banks, double *__restrict__ out,
-                              int tally_count, int shards, int rounds) {
+__global__ void reduce_shards(const B *__restrict__ banks, double *__restrict__ out, int bank_count,
+                              int shards, int rounds) {
   const size_t index = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (index >= size_t(tally_count) * rounds) return;
-  const size_t r = index / tally_count, t = index % tally_count;
-  const B *base = banks + r * size_t(shards) * tally_count + t;
+  if (index >= size_t(bank_count) * rounds) return;
+  const size_t r = index / bank_count, t = index % bank_count;
+  const B *base = banks + r * size_t(shards) * bank_count + t;
   double sum = 0.0;
-  for (int sh = 0; sh < shards; ++sh) sum += double(base[size_t(sh) * tally_count]);
+  for (int sh = 0; sh < shards; ++sh) sum += double(base[size_t(sh) * bank_count]);
   out[index] = sum;
+}
+
+// Output tallies of each round, in Fortran order: Stokes (nmat, nx, ny, views), layer paths
+// (nx, ny, nz, views) and detector weights (nx, ny, views), from the banks' per-(view, pixel)
+// records. The weight C1 of every estimate is its Stokes-I contribution over mu0/mu.
+__global__ void assemble_tallies(Scene s, const double *__restrict__ reduced,
+                                 double *__restrict__ out, int rounds) {
+  const int plane = s.nx * s.ny;
+  const int record = s.paths ? s.nz + s.nmat : s.nmat;
+  const size_t per_round = size_t(plane) * s.views * record;
+  const size_t index = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (index >= per_round * rounds) return;
+  const size_t r = index / per_round, t = index % per_round;
+  const int slot = int(t % record);
+  const int pixel = int((t / record) % plane);
+  const int view = int(t / (size_t(record) * plane));
+  const double value = reduced[r * size_t(s.bank_count) + t];
+  double *dst = out + r * size_t(s.tally_count);
+  const int stokes_at = s.paths ? s.nz : 0;
+  if (slot >= stokes_at) {
+    const int m = slot - stokes_at;
+    dst[m + size_t(s.nmat) * (pixel + size_t(plane) * view)] = value;
+    if (s.paths && m == 0) {
+      const double geometry = fabs(cos(double(s.theta0))) / cos(double(s.view[view]));
+      dst[size_t(s.stokes_count) + s.path_count + pixel + size_t(plane) * view] = value / geometry;
+    }
+  } else {
+    dst[size_t(s.stokes_count) + pixel + size_t(plane) * (slot + size_t(s.nz) * view)] = value;
+  }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -779,7 +935,8 @@ struct Session {
   Scene scene{};
   float *tables = nullptr;
   void *banks = nullptr;  // float or double banks, sized for double
-  double *reduced = nullptr;
+  double *reduced = nullptr;  // per round: summed banks
+  double *assembled = nullptr;  // per round: output tallies
   unsigned *failure = nullptr;  // failure[0]: flags; failure[1]: work counter
   unsigned long long *counters = nullptr;
   int compute_units = 0;
@@ -812,6 +969,7 @@ void release(Session *s) {
   (void)hipFree(s->tables);
   (void)hipFree(s->banks);
   (void)hipFree(s->reduced);
+  (void)hipFree(s->assembled);
   (void)hipFree(s->failure);
   (void)hipFree(s->counters);
   (void)hipFree(s->records);
@@ -865,7 +1023,7 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
   } spans[] = {{19, s.nz, &s.zunit},        {20, long(s.nx) * s.ny, &s.albedo},
                {21, cells, &s.beta},         {22, cells * s.species, &s.cumfrac},
                {23, s.nz, &s.ssalb_gas},     {24, s.nz, &s.height_above},
-               {25, s.nz, &s.tau_above}};
+               {25, s.nz, &s.tau_above},     {27, s.nz, &s.uniform}};
   for (const Span &span : spans) {
     int offset = 0;
     if (!word_int(t, span.word, kHeaderWords, length, offset) || offset + span.length > length) {
@@ -917,6 +1075,7 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
   s.stokes_count = s.nmat * s.nx * s.ny * s.views;
   s.path_count = s.nx * s.ny * s.nz * s.views;
   s.tally_count = s.stokes_count + (s.paths ? s.path_count + s.nx * s.ny * s.views : 0);
+  s.bank_count = s.nx * s.ny * s.views * (s.paths ? s.nz + s.nmat : s.nmat);
   return true;
 }
 
@@ -930,10 +1089,10 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
   Controls c{controls[0], controls[1], controls[2], controls[3],
              controls[4], controls[5], controls[6], controls[7]};
   const Scene &s = h->scene;
-  const size_t per_round = size_t(s.tally_count) * size_t(s.shards);
+  const size_t per_round = size_t(s.bank_count) * size_t(s.shards);
   if (!c.per_round || !c.rounds || c.rounds > h->round_capacity || !c.max_steps ||
       c.count > (1u << 31) ||
-      per_round != h->tally_count || uint64_t(c.first) + c.count > uint64_t(c.per_round) * c.rounds) {
+      uint64_t(c.first) + c.count > uint64_t(c.per_round) * c.rounds) {
     set_error(error, error_size, "Invalid HIP round group");
     return 1;
   }
@@ -956,7 +1115,7 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
     }
   }
   const int block = sizeof(R) == 4 ? kBlock32 : kBlock64;
-  const size_t lds = size_t(s.nz) * block * sizeof(R);
+  const size_t lds = size_t(s.nz | 1) * block * sizeof(R);
   // Persistent grid: enough resident blocks to fill every CU, never more than the work.
   int resident = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, TRACE>, block,
@@ -974,9 +1133,13 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
   }
   (void)hipEventRecord(h->stop, h->stream);
   if (!check(hipGetLastError(), error, error_size, "transport launch")) return 1;
+  const size_t summed = size_t(s.bank_count) * c.rounds;
+  hipLaunchKernelGGL(reduce_shards<R>, dim3((summed + 255) / 256), dim3(256), 0, h->stream, banks,
+                     h->reduced, s.bank_count, s.shards, int(c.rounds));
+  const size_t assembled = size_t(s.bank_count) * c.rounds;
+  hipLaunchKernelGGL(assemble_tallies, dim3((assembled + 255) / 256), dim3(256), 0, h->stream, s,
+                     h->reduced, h->assembled, int(c.rounds));
   const size_t reduced = size_t(s.tally_count) * c.rounds;
-  hipLaunchKernelGGL(reduce_shards<R>, dim3((reduced + 255) / 256), dim3(256), 0, h->stream, banks,
-                     h->reduced, s.tally_count, s.shards, int(c.rounds));
   if (!check(hipGetLastError(), error, error_size, "shard reduction launch")) return 1;
   unsigned flag = 0;
   if (!check(hipMemcpyAsync(&flag, h->failure, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream),
@@ -992,7 +1155,7 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
       snprintf(error, error_size, "HIP transport failed (flag %u); round rejected", flag);
     return 1;
   }
-  if (!check(hipMemcpy(out, h->reduced, reduced * sizeof(double), hipMemcpyDeviceToHost), error,
+  if (!check(hipMemcpy(out, h->assembled, reduced * sizeof(double), hipMemcpyDeviceToHost), error,
              error_size, "copy tallies"))
     return 1;
   if (TRACE && records && record_count) {
@@ -1071,8 +1234,8 @@ void *monki_hip_create(uint32_t device, const float *tables, size_t table_count,
   }
   Scene parsed{};
   if (!parse_tables(tables, table_count, nullptr, parsed, error, error_size)) return nullptr;
-  const size_t per_round = size_t(parsed.tally_count) * size_t(parsed.shards);
-  if (tally_count != per_round) {
+  const size_t per_round = size_t(parsed.bank_count) * size_t(parsed.shards);
+  if (tally_count != size_t(parsed.tally_count) * size_t(parsed.shards)) {
     set_error(error, error_size, "Tally count does not match the packed tables");
     return nullptr;
   }
@@ -1083,7 +1246,7 @@ void *monki_hip_create(uint32_t device, const float *tables, size_t table_count,
   }
   Session *h = new Session();
   h->device = int(device);
-  h->tally_count = tally_count;
+  h->tally_count = per_round;
   h->round_capacity = round_capacity;
   hipDeviceProp_t prop{};
   if (!check(hipSetDevice(h->device), error, error_size, "hipSetDevice") ||
@@ -1094,8 +1257,10 @@ void *monki_hip_create(uint32_t device, const float *tables, size_t table_count,
       !check(hipMalloc(&h->tables, table_count * sizeof(float)), error, error_size, "allocate tables") ||
       !check(hipMalloc(&h->banks, per_round * round_capacity * sizeof(double)), error, error_size,
              "allocate tally banks") ||
-      !check(hipMalloc(&h->reduced, size_t(parsed.tally_count) * round_capacity * sizeof(double)),
+      !check(hipMalloc(&h->reduced, size_t(parsed.bank_count) * round_capacity * sizeof(double)),
              error, error_size, "allocate reduced tallies") ||
+      !check(hipMalloc(&h->assembled, size_t(parsed.tally_count) * round_capacity * sizeof(double)),
+             error, error_size, "allocate output tallies") ||
       !check(hipMalloc(&h->failure, 2 * sizeof(unsigned)), error, error_size, "allocate flag") ||
       !check(hipMalloc(&h->counters, kCounters * sizeof(unsigned long long)), error, error_size,
              "allocate counters") ||

@@ -16,6 +16,8 @@ Header (64 float32 words; integers are stored exactly, all < 2**24):
 19-25 offsets: zunit, albedo, beta, cumulative fractions, gas albedo,
       height above, optical depth above
 26    total buffer length
+27    offset of the per-layer uniformity flags (1: extinction and species mix are the same in
+      every column of the layer, so the free path crosses it without cell-by-cell steps)
 32+8s species s: LUT size, albedo, offsets of cosine block (cos, 1-cos, 1+cos),
       CDF, F11, F22, F33, F12
 ====  =====================================================================
@@ -99,6 +101,9 @@ def pack_tables(scene, shards: int = 32):
     header[23] = place(scene.ssalb_gas)
     header[24] = place(height_above)
     header[25] = place(tau_above)
+    uniform = np.ones(nz)
+    uniform[varying] = 0.0
+    header[27] = place(uniform)
 
     for index, sp in enumerate(scene.species):
         cdf = sp.bound_cdf(nmat) if sp.cdf is None else sp.cdf

@@ -23,7 +23,8 @@
  * search (equal to the equal-angle index estimate of mie_scat_matrix on equal-angle grids);
  * the sampling loop is bounded at 4096 attempts (transport.metal:184) instead of unbounded;
  * the azimuth change after scattering is the same spherical-triangle angle written as an
- * atan2 instead of an acos (well conditioned for small angles).
+ * atan2 instead of an acos (well conditioned for small angles); rotation cosines are clamped
+ * to [-1, 1].
  * The detector walk is always explicit, cell by cell, with the analytical pixel mapping
  * of mc_routines_cartesian_mod.f90:1076-1105: no uniform-top shortcut.
  *
@@ -192,6 +193,10 @@ void oracle_rotation(double cos_sca, double uo, double u1, double phio, double p
     cosi2 = (uo - u1 * cos_sca) / sqrt(b > EPS ? b : EPS);
     q = wo[1] * w1[0] - wo[0] * w1[1];
   }
+  /* |cos i| can pass one when the scattering angle is near zero (the denominators above are
+   * floored at 1e-16); clamped as transport.metal:158-159 does, where the Fortran warns. */
+  cosi1 = cosi1 > 1.0 ? 1.0 : (cosi1 < -1.0 ? -1.0 : cosi1);
+  cosi2 = cosi2 > 1.0 ? 1.0 : (cosi2 < -1.0 ? -1.0 : cosi2);
   double sign = q >= 0 ? 1.0 : -1.0;
   fac = 1.0 - cosi1 * cosi1;
   double sini1 = fac < EPS ? 0.0 : sqrt(fac) * sign;

@@ -14,7 +14,14 @@ per view) over the transport kernel's average duration (HIP events on its stream
 the 8 TB/s HBM peak. cpu_baseline: the reference's Fortran transport (mc.f90, amdflang -O2
 -fopenmp) on a bounded sample of the same scene on the host cores.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--amf]
+traffic: HBM bytes per transport launch from two rocprofv3 --pmc runs (FETCH_SIZE, then
+WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads for a plain 1 GiB
+device copy (--copy-calibration); --summarize-traffic DIR turns the four runs into
+profiles/traffic_layered_cloud.json, which later bench runs report.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--amf] [--no-cpu]
+       python bench.py --copy-calibration
+       python bench.py --summarize-traffic DIR
 """
 
 import argparse
@@ -96,6 +103,114 @@ def time_to_amf_se(scene, max_rounds: int = 1024) -> dict:
     return {"seconds": elapsed, "status": receipt["status"], "rounds": receipt["completed_rounds"]}
 
 
+COPY_BYTES = 1 << 30
+TRAFFIC_FILE = ROOT / "profiles" / "traffic_layered_cloud.json"
+
+
+def copy_calibration() -> None:
+    """A plain device-to-device copy of COPY_BYTES, three times, for counter calibration."""
+
+    import torch
+
+    source = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    target = torch.empty_like(source)
+
+    for _ in range(3):
+        target.copy_(source)
+
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": COPY_BYTES, "copies": 3}), flush=True)
+
+
+def _counter_values(directory: Path, name: str, match) -> list:
+    """Per-dispatch counter values of matching kernels from one rocprofv3 --pmc run (CSV or
+    the SQLite database that rocprofv3 writes by default)."""
+
+    import csv
+    import sqlite3
+
+    table = directory / f"{name}_counter_collection.csv"
+
+    if table.is_file():
+        with table.open() as handle:
+            rows = [row for row in csv.DictReader(handle) if match(row["Kernel_Name"])]
+
+        return [float(row["Counter_Value"]) for row in rows]
+
+    with sqlite3.connect(directory / f"{name}_results.db") as db:
+        rows = db.execute("SELECT kernel_name, value FROM counters_collection").fetchall()
+
+    return [float(value) for kernel, value in rows if match(kernel)]
+
+
+def summarize_traffic(directory: Path) -> dict:
+    """HBM bytes per transport launch from the four --pmc runs written under ``directory``.
+
+    Expected runs (rocprofv3 -d DIR -o NAME): transport_fetch, transport_write, copy_fetch and
+    copy_write. Counter values are KiB per dispatch.
+    """
+
+    def read(name: str, match) -> list:
+        return _counter_values(directory, name, match)
+
+    transport = lambda name: "transport_kernel" in name  # noqa: E731
+    copy = lambda name: "copy" in name.lower() or "elementwise" in name.lower()  # noqa: E731
+    fetch = read("transport_fetch", transport)
+    write = read("transport_write", transport)
+    copy_fetch = max(read("copy_fetch", copy))
+    copy_write = max(read("copy_write", copy))
+    fetch_factor = COPY_BYTES / (copy_fetch * 1024.0)
+    write_factor = COPY_BYTES / (copy_write * 1024.0)
+    fetch_bytes = sum(fetch) / len(fetch) * 1024.0 * fetch_factor
+    write_bytes = sum(write) / len(write) * 1024.0 * write_factor
+    summary = {
+        "hbm_bytes_per_launch": fetch_bytes + write_bytes,
+        "fetch_bytes_per_launch": fetch_bytes,
+        "write_bytes_per_launch": write_bytes,
+        "fetch_factor": fetch_factor,
+        "write_factor": write_factor,
+        "transport_dispatches": [len(fetch), len(write)],
+        "copy_kib": [copy_fetch, copy_write],
+        "commands": [
+            "rocprofv3 --pmc FETCH_SIZE -d DIR -o transport_fetch -- python3 bench.py --steps 2 --warmup 0 --no-cpu",
+            "rocprofv3 --pmc WRITE_SIZE -d DIR -o transport_write -- python3 bench.py --steps 2 --warmup 0 --no-cpu",
+            "rocprofv3 --pmc FETCH_SIZE -d DIR -o copy_fetch -- python3 bench.py --copy-calibration",
+            "rocprofv3 --pmc WRITE_SIZE -d DIR -o copy_write -- python3 bench.py --copy-calibration",
+            "python3 bench.py --summarize-traffic DIR",
+        ],
+    }
+    TRAFFIC_FILE.write_text(json.dumps(summary, indent=1) + "\n")
+    return summary
+
+
+def summarize_profile(database: Path) -> str:
+    """Markdown summary of a rocprofv3 --kernel-trace --stats run (its SQLite output)."""
+
+    import sqlite3
+
+    with sqlite3.connect(database) as db:
+        top = db.execute("SELECT name, total_calls, total_duration, average, percentage FROM top_kernels").fetchall()
+        dispatches = db.execute(
+            "SELECT name, duration, grid_x, workgroup_x, lds_size, scratch_size, vgpr_count, "
+            "accum_vgpr_count, sgpr_count FROM kernels ORDER BY start"
+        ).fetchall()
+
+    def short(name: str) -> str:
+        name = name.replace("(anonymous namespace)::", "").removeprefix("void ")
+        return name.split("(")[0]
+
+    lines = ["| kernel | calls | total ms | average ms | % |", "|---|---|---|---|---|"]
+    lines += [f"| `{short(n)}` | {c} | {t / 1e3:.3f} | {a / 1e3:.3f} | {p:.3f} |" for n, c, t, a, p in top]
+    lines += ["", "| kernel | duration ms | grid | block | LDS B | scratch B | VGPR (as reported) | AGPR | SGPR |",
+              "|---|---|---|---|---|---|---|---|---|"]  # fmt: skip
+    lines += [
+        f"| `{short(n)}` | {d / 1e6:.3f} | {g} | {w} | {lds} | {sc} | {v} | {acc} | {sg} |"
+        for n, d, g, w, lds, sc, v, acc, sg in dispatches
+        if "transport" in n or "reduce" in n or "assemble" in n
+    ]
+    return "\n".join(lines) + "\n"
+
+
 def main() -> int:
     parser = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     parser.add_argument("--gpus", type=int, default=1)
@@ -103,7 +218,22 @@ def main() -> int:
     parser.add_argument("--warmup", type=int, default=2)
     parser.add_argument("--amf", action="store_true", help="also time AMF convergence at 1%%")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    parser.add_argument("--copy-calibration", action="store_true", help="run the copy kernel only")
+    parser.add_argument("--summarize-traffic", type=Path, metavar="DIR")
+    parser.add_argument("--summarize-profile", type=Path, metavar="DB")
     args = parser.parse_args()
+
+    if args.summarize_profile:
+        print(summarize_profile(args.summarize_profile), end="")
+        return 0
+
+    if args.copy_calibration:
+        copy_calibration()
+        return 0
+
+    if args.summarize_traffic:
+        print(json.dumps(summarize_traffic(args.summarize_traffic)), flush=True)
+        return 0
 
     if args.gpus != 1:
         print(
@@ -143,10 +273,9 @@ def main() -> int:
     kernel_average = kernel_seconds / args.steps
     achieved = bytes_per_launch / kernel_average / 1e9
     traffic = None
-    traffic_file = ROOT / "profiles" / "traffic_layered_cloud.json"
 
-    if traffic_file.is_file():
-        traffic = json.loads(traffic_file.read_text()).get("hbm_bytes_per_launch")
+    if TRAFFIC_FILE.is_file():
+        traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
 
     line = {
         "metric": METRIC,

@@ -19,6 +19,9 @@ FLAGS = [
     "-shared",
     "-munsafe-fp-atomics",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # Keep SimplifyCFG from sinking stores of different fields of the photon state into one
+    # store through a selected pointer: that pins the whole state in scratch memory.
+    "-mllvm", "-simplifycfg-sink-common=false",
     "-Wall",
 ]  # fmt: skip
 

@@ -58,6 +58,11 @@ struct Scene {
   int bank_count;
   float xunit, yunit, depol, theta0, phi0, phi;
   float view[kMaxViews];
+  // Geometry derived once on the host in float64 from the float32 angles: launch direction,
+  // |cos theta0|, and per view cos, sin, the horizontal direction and mu0/mu.
+  double launch_a, launch_b, launch_c, mu0;
+  double view_mu[kMaxViews], view_sine[kMaxViews], view_a[kMaxViews], view_b[kMaxViews];
+  double view_geometry[kMaxViews], view_inv_mu[kMaxViews];
   const float *zunit, *albedo, *beta, *cumfrac, *ssalb_gas, *height_above, *tau_above;
   const float *uniform;  // per layer: 1 if every column has the same extinction and mix
   SpeciesTables sp[kMaxSpecies];
@@ -480,14 +485,27 @@ template <typename R> struct History {
   R path_total;  // sum of the layer paths, for the per-photon records
 };
 
-// Local estimates of one event, waiting in registers for the wave's cooperative tally.
+// Local estimates of one event, parked in LDS until the wave's cooperative tally reads them.
+// Struct of arrays over the block's lanes: lane writes are contiguous, and the tally's reads
+// of one source lane are broadcasts.
+constexpr int kPendingReals = 1 + 3 * kMaxViews;
+constexpr int kPendingWords = 2 + kMaxViews;
+
 template <typename R> struct Pending {
-  bool has;
-  int layer;
-  int pixel[kMaxViews];  // (i + nx j), or -1 when the view gets nothing from this event
-  R weight[kMaxViews];   // C1 = I * exp(-tau): detector weight of the view
-  R ray[kMaxViews];      // partial detector-ray path inside the event layer
-  R stokes[kMaxViews][3];  // Stokes contribution times mu0/mu
+  R *real;     // rz | weight x views | q x views | u x views
+  int *word;   // layer | round offset | pixel x views
+  int stride;  // lanes per block
+  // Vertical extent of the detector ray inside the event layer.
+  __device__ R &rz() const { return real[0]; }
+  // C1 = I exp(-tau), the detector weight of the view; Q and U contributions times mu0/mu.
+  __device__ R &weight(int v) const { return real[(1 + v) * stride]; }
+  __device__ R &q(int v) const { return real[(1 + kMaxViews + v) * stride]; }
+  __device__ R &u(int v) const { return real[(1 + 2 * kMaxViews + v) * stride]; }
+  __device__ int &layer() const { return word[0]; }
+  __device__ int &round() const { return word[stride]; }
+  // Output pixel i + nx j, or -1 when the view gets nothing from this event.
+  __device__ int &pixel(int v) const { return word[(2 + v) * stride]; }
+  __device__ Pending at(int lane) const { return Pending{real + lane, word + lane, stride}; }
 };
 
 template <typename R> __device__ inline bool finite3(const Stokes<R> &v) {
@@ -507,11 +525,10 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
   p.i = column % s.nx;
   p.j = column / s.nx;
   p.k = s.nz - 1;
-  const R t0 = R(s.theta0), p0 = R(s.phi0);
-  p.a = sin(t0) * cos(p0);
-  p.b = sin(t0) * sin(p0);
-  p.c = cos(t0);
-  p.phi = p0;
+  p.a = R(s.launch_a);
+  p.b = R(s.launch_b);
+  p.c = R(s.launch_c);
+  p.phi = R(s.phi0);
   p.rx = R(s.xunit) * R(uniform(h.rng));
   p.ry = R(s.yunit) * R(uniform(h.rng));
   p.rz = R(s.zunit[p.k]);
@@ -529,7 +546,7 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
 // into `out`. Returns kRunning while the history continues, else its end status.
 template <typename R, bool TRACE>
 __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *history,
-                       History<R> &h, Pending<R> &out, Counts &n, Trace<R> &trace) {
+                       History<R> &h, const Pending<R> &out, Counts &n, Trace<R> &trace) {
   const R eps = R(1e-16);
   Photon<R> &p = h.p;
   bool surface = false;
@@ -620,25 +637,26 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
   h.surface = surface;
   if (h.weight < eps) return kAbsorbed;
 
-  out.has = true;
-  out.layer = p.k;
-  const R geometry0 = fabs(cos(R(s.theta0)));
-  for (int view = 0; view < kMaxViews; ++view) {
-    out.pixel[view] = -1;
-    if (view >= s.views) continue;
-    const R theta = R(s.view[view]);
-    const R mu = cos(theta);
+  out.layer() = p.k;
+  out.round() = int(h.round_offset);
+  // Detector ray inside the event layer: the distance to its top face (views point upward).
+  const R rz = p.c < R(0) ? fmax(R(0), R(s.zunit[p.k]) - p.rz) : p.rz;
+  out.rz() = rz;
+#pragma unroll 1
+  for (int view = 0; view < s.views; ++view) {
+    out.pixel(view) = -1;
+    const R mu = R(s.view_mu[view]);
     Stokes<R> prob;
     if (surface) {
       prob = Stokes<R>{h.st.i * mu * h.weight, R(0), R(0)};
     } else {
       // Sines floored at sqrt(1e-16) as mc.f90:1112-1113 does: the forward peak of a Mie
       // table is steep enough for this to matter in float64.
-      const R cosine = clamp1(p.c * mu + sqrt(fmax(eps, R(1) - p.c * p.c)) *
-                                             sqrt(fmax(eps, R(1) - mu * mu)) * cos(p.phi - R(s.phi)));
+      const R cosine = clamp1(p.c * mu + sqrt(fmax(eps, R(1) - p.c * p.c)) * R(s.view_sine[view]) *
+                                             cos(p.phi - R(s.phi)));
       // 1 -/+ cos from the chord between the photon and view directions: accurate to the
       // direction's own rounding even within a fraction of a degree of the peaks.
-      const R vx = sin(theta) * cos(R(s.phi)), vy = sin(theta) * sin(R(s.phi));
+      const R vx = R(s.view_a[view]), vy = R(s.view_b[view]);
       const R dx = p.a - vx, dy = p.b - vy, dz = p.c - mu;
       const R ex = p.a + vx, ey = p.b + vy, ez = p.c + mu;
       const R one_minus = R(0.5) * (dx * dx + dy * dy + dz * dz);
@@ -656,24 +674,20 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
       return kFailed;
     }
     if (prob.i <= eps) continue;
-    const R va = sin(theta) * cos(R(s.phi)), vb = sin(theta) * sin(R(s.phi));
     R tau;
-    const DetectorHit hit = detector_walk<R>(s, p, va, vb, mu, tau, c.max_steps, n);
+    const DetectorHit hit =
+        detector_walk<R>(s, p, R(s.view_a[view]), R(s.view_b[view]), mu, tau, c.max_steps, n);
     if (!hit.ok) {
       atomicOr(failure, kDetectorLimit);
       return kFailed;
     }
     const R attenuation = exp(-tau);
-    const R geometry = geometry0 / mu;  // mu0 / mu (mc.f90:1456-1474)
+    const R geometry = R(s.view_geometry[view]);  // mu0 / mu (mc.f90:1456-1474)
     const R ci = prob.i * attenuation;
-    out.pixel[view] = hit.i + s.nx * hit.j;
-    out.weight[view] = ci;
-    // Detector ray inside the event layer: the distance to its top face along the view.
-    const R rz = p.c * mu < R(0) ? fmax(R(0), R(s.zunit[p.k]) - p.rz) : p.rz;
-    out.ray[view] = rz / mu;
-    out.stokes[view][0] = ci * geometry;
-    out.stokes[view][1] = prob.q * attenuation * geometry;
-    out.stokes[view][2] = prob.u * attenuation * geometry;
+    out.pixel(view) = hit.i + s.nx * hit.j;
+    out.weight(view) = ci;
+    out.q(view) = prob.q * attenuation * geometry;
+    out.u(view) = prob.u * attenuation * geometry;
     if (TRACE) {
       trace.stokes_i += ci * geometry;
       if (s.paths) {
@@ -756,36 +770,34 @@ __device__ inline int broadcast_int(int value, int lane) {
 // lane's LDS row) or Stokes component t - nz. Contiguous lanes make each instruction a few
 // cache-line requests instead of one request per value.
 template <typename R>
-__device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int shard,
-                                  const Pending<R> &pending, uint32_t round_offset,
-                                  const R *wave_history, int row, Counts &n) {
+__device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int shard, bool has,
+                                  const Pending<R> &wave_pending, const R *wave_history, int row,
+                                  R lane_layer, Counts &n) {
   const int lane = __lane_id();
   const int record = s.paths ? s.nz + s.nmat : s.nmat;
+  const int stokes_at = s.paths ? s.nz : 0;
+  const bool path_lane = s.paths && lane < s.nz;
   const int plane = s.nx * s.ny;
-  for (uint64_t mask = __ballot(pending.has); mask; mask &= mask - 1) {
+  for (uint64_t mask = __ballot(has); mask; mask &= mask - 1) {
     const int src = __ffsll(static_cast<long long>(mask)) - 1;
-    const int layer = broadcast_int(pending.layer, src);
-    const uint32_t round = uint32_t(broadcast_int(int(round_offset), src));
-    R *bank = banks + (size_t(round) * s.shards + shard) * size_t(s.bank_count);
-    const R *source = wave_history + src * row;
+    const Pending<R> source = wave_pending.at(src);
+    const int layer = source.layer();
+    const R rz = source.rz();
+    R *bank = banks + (size_t(source.round()) * s.shards + shard) * size_t(s.bank_count);
+    const R path = path_lane ? wave_history[src * row + lane] : R(0);
+#pragma unroll 1
     for (int view = 0; view < s.views; ++view) {
-      const int pixel = broadcast_int(pending.pixel[view], src);
+      const int pixel = source.pixel(view);
       if (pixel < 0) continue;
-      const R weight = broadcast<R>(pending.weight[view], src);
-      const R ray = broadcast<R>(pending.ray[view], src);
-      const R stokes0 = broadcast<R>(pending.stokes[view][0], src);
-      const R stokes1 = broadcast<R>(pending.stokes[view][1], src);
-      const R stokes2 = broadcast<R>(pending.stokes[view][2], src);
-      R value = R(0);
-      if (s.paths && lane < s.nz) {
-        const R mu = cos(R(s.view[view]));
-        const R length = lane == layer ? ray
-                         : (lane > layer && lane < s.nz - 1) ? R(s.zunit[lane]) / mu
-                                                            : R(0);
-        value = weight * (source[lane] + length);
+      const R weight = source.weight(view);
+      const R inv_mu = R(s.view_inv_mu[view]);
+      R value;
+      if (path_lane) {
+        const R length = lane == layer ? rz * inv_mu : (lane > layer ? lane_layer * inv_mu : R(0));
+        value = weight * (path + length);
       } else {
-        const int m = lane - (s.paths ? s.nz : 0);
-        value = m == 0 ? stokes0 : (m == 1 ? stokes1 : stokes2);
+        const int m = lane - stokes_at;
+        value = m == 0 ? weight * R(s.view_geometry[view]) : (m == 1 ? source.q(view) : source.u(view));
       }
       if (lane < record && value != R(0)) {
         add_tally<R>(bank + (size_t(view) * plane + pixel) * record + lane, value);
@@ -814,8 +826,12 @@ __device__ inline void flush_counts(Counts &n, unsigned long long *counters) {
 // ended takes the next photon id from a wave-aggregated work counter, so a wave never idles
 // behind its longest history. Photon ids (and with them the random streams and the order of
 // draws) do not depend on which lane runs them. Tally banks are chosen per wave.
+// Three waves per SIMD is what the LDS rows of a 256-lane block allow; the register budget is
+// capped to match (float32), which measured 1.35x faster than the compiler's own choice of
+// two waves per SIMD on the layered-cloud round.
 template <typename R, bool TRACE>
-__global__ void __launch_bounds__(kBlock32) transport_kernel(Scene s, Controls c, R *banks,
+__global__ void __launch_bounds__(kBlock32)
+    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 1))) transport_kernel(Scene s, Controls c, R *banks,
                                                              unsigned *failure, unsigned *work,
                                                              unsigned long long *counters,
                                                              float *records, uint32_t record_count) {
@@ -827,6 +843,15 @@ __global__ void __launch_bounds__(kBlock32) transport_kernel(Scene s, Controls c
   const R *wave_history = reinterpret_cast<R *>(lds_raw) + (threadIdx.x & ~63u) * row;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const int shard = int(wave % uint32_t(s.shards));
+  // Thickness of this lane's layer in the cooperative tally (the vacuum layer adds no path).
+  const R lane_layer = __lane_id() < s.nz - 1 ? R(s.zunit[__lane_id()]) : R(0);
+  const int block = int(blockDim.x);
+  const Pending<R> wave_pending{
+      reinterpret_cast<R *>(lds_raw) + size_t(block) * row + (threadIdx.x & ~63u),
+      reinterpret_cast<int *>(reinterpret_cast<R *>(lds_raw) + size_t(block) * (row + kPendingReals)) +
+          (threadIdx.x & ~63u),
+      block};
+  const Pending<R> pending = wave_pending.at(__lane_id());
   Counts n{0, 0, 0, 0, 0, 0, 0, 0};
   History<R> h;
   Trace<R> trace;
@@ -852,11 +877,15 @@ __global__ void __launch_bounds__(kBlock32) transport_kernel(Scene s, Controls c
     }
     if (__ballot(active) == 0) break;
 
-    Pending<R> pending;
-    pending.has = false;
     int status = kRunning;
     if (active) status = collide<R, TRACE>(s, c, failure, history, h, pending, n, trace);
-    tally_wave<R>(s, banks, shard, pending, h.round_offset, wave_history, row, n);
+    // Lanes read each other's LDS rows below: keep every access of this wave in order.
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    tally_wave<R>(s, banks, shard, active && status == kRunning, wave_pending, wave_history, row,
+                  lane_layer, n);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
     if (active && status == kRunning) status = scatter_photon<R, TRACE>(s, failure, h, n, trace);
     if (!active || status == kRunning) continue;
     active = false;
@@ -918,7 +947,7 @@ __global__ void assemble_tallies(Scene s, const double *__restrict__ reduced,
     const int m = slot - stokes_at;
     dst[m + size_t(s.nmat) * (pixel + size_t(plane) * view)] = value;
     if (s.paths && m == 0) {
-      const double geometry = fabs(cos(double(s.theta0))) / cos(double(s.view[view]));
+      const double geometry = s.view_geometry[view];
       dst[size_t(s.stokes_count) + s.path_count + pixel + size_t(plane) * view] = value / geometry;
     }
   } else {
@@ -1016,6 +1045,20 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
   s.phi0 = t[14];
   s.phi = t[15];
   for (int v = 0; v < kMaxViews; ++v) s.view[v] = v < s.views ? t[16 + v] : 0.0f;
+  const double t0 = s.theta0, p0 = s.phi0, ph = s.phi;
+  s.launch_a = sin(t0) * cos(p0);
+  s.launch_b = sin(t0) * sin(p0);
+  s.launch_c = cos(t0);
+  s.mu0 = fabs(cos(t0));
+  for (int v = 0; v < kMaxViews; ++v) {
+    const double theta = s.view[v], mu = cos(theta);
+    s.view_mu[v] = mu;
+    s.view_sine[v] = sqrt(fmax(1e-16, 1.0 - mu * mu));
+    s.view_a[v] = sin(theta) * cos(ph);
+    s.view_b[v] = sin(theta) * sin(ph);
+    s.view_geometry[v] = v < s.views ? s.mu0 / mu : 0.0;
+    s.view_inv_mu[v] = v < s.views ? 1.0 / mu : 0.0;
+  }
   struct Span {
     int word;
     long length;
@@ -1115,7 +1158,7 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
     }
   }
   const int block = sizeof(R) == 4 ? kBlock32 : kBlock64;
-  const size_t lds = size_t(s.nz | 1) * block * sizeof(R);
+  const size_t lds = (size_t(s.nz | 1) + kPendingReals) * block * sizeof(R) + size_t(kPendingWords) * block * sizeof(int);
   // Persistent grid: enough resident blocks to fill every CU, never more than the work.
   int resident = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, TRACE>, block,
@@ -1271,7 +1314,12 @@ void *monki_hip_create(uint32_t device, const float *tables, size_t table_count,
     release(h);
     return nullptr;
   }
-  snprintf(h->name, sizeof(h->name), "%s (%s)", prop.name, prop.gcnArchName);
+  // Some drivers report an empty marketing name: fall back to the architecture and CU count.
+  if (prop.name[0])
+    snprintf(h->name, sizeof(h->name), "%s (%s, %d CUs)", prop.name, prop.gcnArchName,
+             prop.multiProcessorCount);
+  else
+    snprintf(h->name, sizeof(h->name), "%s, %d CUs", prop.gcnArchName, prop.multiProcessorCount);
   h->compute_units = prop.multiProcessorCount;
   parse_tables(tables, table_count, h->tables, h->scene, nullptr, 0);
   return h;

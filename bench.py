@@ -86,21 +86,66 @@ def cpu_baseline(scene, events_per_photon: float, target_seconds: float = 15.0) 
 
 
 def time_to_amf_se(scene, max_rounds: int = 1024) -> dict:
+    """Wall time of simulate(..., convergence=AMFConvergence(0.01, 128, 32, 3)) to converged."""
+
     import tempfile
 
     from monki import AMFConvergence, SceneCase, simulate
     from monki.execution import HIP
+    from monki.execution.backend import Backend
+
+    class Progress(Backend):
+        """Delegates to HIP and prints one line per group of rounds (the run takes minutes)."""
+
+        def __init__(self):
+            self.inner = HIP()
+            self.rounds = 0
+            self.start = time.perf_counter()
+
+        def prepare(self, case):
+            self.inner.prepare(case)
+
+        @property
+        def metadata(self):
+            return self.inner.metadata
+
+        @property
+        def provenance(self):
+            return self.inner.provenance
+
+        def run_batch(self, controls):
+            return self.run_batches([controls])[0]
+
+        def run_batches(self, controls):
+            batches = self.inner.run_batches(controls)
+            self.rounds += len(batches)
+            print(f"amf: {self.rounds} rounds, {time.perf_counter() - self.start:.1f} s", file=sys.stderr, flush=True)
+            return batches
+
+        def close(self):
+            self.inner.close()
 
     policy = AMFConvergence(0.01, 128, 32, 3)
     case = SceneCase("layered-cloud", scene, PHOTONS_PER_COLUMN, max_rounds)
 
     with tempfile.TemporaryDirectory() as directory:
         start = time.perf_counter()
-        simulate(case, HIP(), Path(directory) / "amf.nc", convergence=policy)
+        simulate(case, Progress(), Path(directory) / "amf.nc", convergence=policy)
         elapsed = time.perf_counter() - start
         receipt = json.loads((Path(directory) / "amf.run.json").read_text())
 
-    return {"seconds": elapsed, "status": receipt["status"], "rounds": receipt["completed_rounds"]}
+    checks = receipt.get("checks") or []
+    worst = checks[-1]["max_relative_se"] if checks else None
+    result = {"seconds": elapsed, "status": receipt["status"], "rounds": receipt["completed_rounds"],
+              "policy": "AMFConvergence(0.01, minimum 128, every 32, 3 passes)", "max_relative_se": worst}  # fmt: skip
+
+    if worst and receipt["status"] != "converged":
+        # SE falls as rounds^-1/2: rounds and seconds the same run needs to reach 1%.
+        needed = receipt["completed_rounds"] * (worst / 0.01) ** 2
+        result["projected_rounds"] = needed
+        result["projected_seconds"] = elapsed * needed / receipt["completed_rounds"]
+
+    return result
 
 
 COPY_BYTES = 1 << 30

@@ -242,7 +242,7 @@ void oracle_new_direction(double cos_in, double cos_sca, double sigma, double *c
 /* Rejection sampling, forward sign -1. Returns 0 after ATTEMPTS rejections. */
 static int sample(const scene_t *s, int species, const double st[3], double cos_in, rng_t *r,
                   double *cos_out, double *dphi, double *cos_sca, double sampled[2],
-                  uint64_t *attempts) {
+                  uint64_t *attempts, double *sigma_out) {
   double bound_ray[4];
   oracle_rayleigh(s->depol, 1.0, bound_ray);
   for (int attempt = 0; attempt < ATTEMPTS; ++attempt) {
@@ -273,6 +273,7 @@ static int sample(const scene_t *s, int species, const double st[3], double cos_
     double bound = species < 0 ? bound_ray[0] : f[0] + fabs(f[3]);
     if (draw(r) * bound > pdf) continue;
     *cos_sca = c;
+    if (sigma_out) *sigma_out = sigma;
     sampled[0] = one_minus, sampled[1] = one_plus;
     new_direction(cos_in, c, sin2, sigma, cos_out, dphi);
     return 1;
@@ -484,7 +485,7 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
       st[0] = 1.0, st[1] = 0.0, st[2] = 0.0;
     } else {
       double dphi, cos_sca, sampled[2];
-      if (!sample(s, species, st, p.c, &r, &cos_out, &dphi, &cos_sca, sampled, &out->counters[6])) {
+      if (!sample(s, species, st, p.c, &r, &cos_out, &dphi, &cos_sca, sampled, &out->counters[6], NULL)) {
         *flag |= 4u;
         return FAILED;
       }
@@ -538,6 +539,29 @@ static int parse(const float *t, size_t count, scene_t *s) {
   s->stokes_count = s->nmat * s->nx * s->ny * s->views;
   s->path_count = s->nx * s->ny * s->nz * s->views;
   s->tally_count = s->stokes_count + (s->paths ? s->path_count + s->nx * s->ny * s->views : 0);
+  return 0;
+}
+
+/*
+ * Draw `count` scattering directions for species `species` (-1: gas) of the packed tables
+ * with incident Stokes (1, q, u) and incident direction cosine cos_in, sample i on the Philox
+ * stream (photon i, round 0, seed). out receives (cos_sca, sigma, cos_out, dphi) per sample.
+ * Returns 0, or -1 on a table or sampling failure.
+ */
+int oracle_sample(const float *tables, size_t table_count, int species, double q, double u,
+                  double cos_in, uint32_t seed, uint32_t count, double *out) {
+  scene_t s;
+  if (parse(tables, table_count, &s) || species >= s.species) return -1;
+  const double st[3] = {1.0, q, u};
+  for (uint32_t i = 0; i < count; ++i) {
+    rng_t r = {i, 0u, seed, 0u, 0u, {0, 0, 0, 0}, 4};
+    double cos_out, dphi, cos_sca, sigma, sampled[2];
+    uint64_t attempts = 0;
+    if (!sample(&s, species, st, cos_in, &r, &cos_out, &dphi, &cos_sca, sampled, &attempts, &sigma))
+      return -1;
+    double *o = out + 4 * (size_t)i;
+    o[0] = cos_sca, o[1] = sigma, o[2] = cos_out, o[3] = dphi;
+  }
   return 0;
 }
 

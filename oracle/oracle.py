@@ -49,6 +49,8 @@ class Oracle:
         lib.oracle_run.argtypes = [_F, ctypes.c_size_t, _U, _D, _D, ctypes.c_size_t, _U64]
         lib.oracle_run.restype = ctypes.c_uint
         lib.oracle_tally_count.argtypes = [_F, ctypes.c_size_t]
+        lib.oracle_sample.argtypes = [_F, ctypes.c_size_t, ctypes.c_int] + [ctypes.c_double] * 3 + [ctypes.c_uint32, ctypes.c_uint32, _D]
+        lib.oracle_sample.restype = ctypes.c_int
         lib.oracle_tally_count.restype = ctypes.c_int
         lib.oracle_philox.argtypes = [_U, _U, _U]
         lib.oracle_rayleigh.argtypes = [ctypes.c_double, ctypes.c_double, _D]
@@ -97,6 +99,21 @@ class Oracle:
         return co.value, dphi.value
 
     # -- transport ----------------------------------------------------------------------
+    def sample(self, packed, species: int, q: float, u: float, cos_in: float, count: int, seed: int = 1):
+        """Sampled (cos_sca, sigma, cos_out, dphi) rows of the oracle's rejection sampler."""
+
+        packed = np.ascontiguousarray(packed, dtype=np.float32)
+        out = np.zeros((count, 4), dtype=np.float64)
+        status = self.lib.oracle_sample(
+            packed.ctypes.data_as(_F), packed.size, species, q, u, cos_in, seed, count,
+            out.ctypes.data_as(_D),
+        )  # fmt: skip
+
+        if status:
+            raise ValueError("oracle sampling failed")
+
+        return out
+
     def run(self, packed, controls, records: int = 0):
         """Run photons with the product's control words; return (tallies, records, counters, flag)."""
 

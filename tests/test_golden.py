@@ -130,3 +130,184 @@ def test_bound_cdf_matches_trapezoid_definition():
     steps = np.concatenate([[0], np.cumsum(np.diff(mu) * (bound[1:] + bound[:-1]) / 2)])
     np.testing.assert_allclose(pol.bound_cdf(3), steps / steps[-1], rtol=1e-14)
     np.testing.assert_allclose(pol.bound_cdf(1), (mu + 1) / 2, atol=1e-15)
+
+
+# ---- Direction sampling: distributions fixed by the phase matrix ---------------------------
+
+
+def _chi_square_p(counts, probabilities):
+    from scipy.stats import chi2
+
+    expected = probabilities * counts.sum()
+    statistic = float(((counts - expected) ** 2 / expected).sum())
+    return statistic, float(chi2.sf(statistic, len(counts) - 1))
+
+
+def _packed(scene):
+    from monki.execution.hip_tables import pack_tables
+
+    return pack_tables(scene, shards=1)[0]
+
+
+def test_rayleigh_sampling_follows_f11(oracle):
+    """Unpolarized light: cos(theta) has density F11(c) / 2 (mc_routines_general_mod.f90:492-670)."""
+
+    from monki import configs
+
+    scene = configs.clear_sky(nx=2, ny=2, nmat=1)
+    rows = oracle.sample(_packed(scene), -1, 0.0, 0.0, 0.3, 200_000)
+    delta = (1 - scene.depol) / (1 + scene.depol / 2)
+    edges = np.linspace(-1, 1, 41)
+    primitive = delta * 0.75 * (edges + edges**3 / 3) + (1 - delta) * edges
+    counts, _ = np.histogram(rows[:, 0], edges)
+    statistic, p = _chi_square_p(counts, np.diff(primitive) / 2.0)
+    assert p > 1e-3, statistic
+
+
+@pytest.mark.parametrize("q, u", [(0.8, 0.0), (0.0, 0.8), (-0.5, -0.6)])
+def test_rayleigh_sampling_of_sigma_follows_linear_polarization(oracle, q, u):
+    """Polarized light: p(sigma) = (2 - Delta Q cos 2 sigma + Delta U sin 2 sigma) / 4 pi, the
+    cos-integral of F11 + F12 (Q cos 2 sigma - U sin 2 sigma) with the forward sign -1
+    (mc_routines_general_mod.f90:583-587). A flipped sign of either term fails."""
+
+    from monki import configs
+
+    scene = configs.clear_sky(nx=2, ny=2, nmat=3)
+    rows = oracle.sample(_packed(scene), -1, q, u, -0.4, 200_000)
+    delta = (1 - scene.depol) / (1 + scene.depol / 2)
+    edges = np.linspace(-np.pi, np.pi, 25)
+    primitive = 2 * edges - delta * q * np.sin(2 * edges) / 2 - delta * u * np.cos(2 * edges) / 2
+    counts, _ = np.histogram(rows[:, 1], edges)
+    statistic, p = _chi_square_p(counts, np.diff(primitive) / (4 * np.pi))
+    assert p > 1e-3, statistic
+
+
+@pytest.mark.parametrize("species", [0, 1])
+def test_tabulated_sampling_follows_trapezoid_cdf(oracle, species):
+    """nmat = 1: every draw is accepted, so the probability of a node interval is its trapezoid
+    area of F11 (main_routines_general_mod.f90:822-866 with mie_scatter_pol_bound)."""
+
+    from monki import configs
+
+    scene = configs.layered_cloud(nx=2, ny=2, nmat=1)
+    sp = scene.species[species]
+    cosines = sp.cosines.astype(np.float32).astype(np.float64)
+    f11 = sp.f11.astype(np.float32).astype(np.float64)
+    rows = oracle.sample(_packed(scene), species, 0.0, 0.0, 0.7, 200_000)
+    nodes = np.concatenate([np.arange(0, cosines.size - 1, 9), [cosines.size - 1]])
+    areas = np.concatenate([[0.0], np.cumsum(np.diff(cosines) * (f11[1:] + f11[:-1]) / 2)])
+    probabilities = np.diff(areas[nodes]) / areas[-1]
+    counts, _ = np.histogram(rows[:, 0], cosines[nodes])
+    statistic, p = _chi_square_p(counts, probabilities)
+    assert p > 1e-3, statistic
+
+
+def test_polarized_mie_sampling_follows_bounded_proposal(oracle):
+    """nmat = 3: the cosine is drawn piecewise uniformly from the trapezoid CDF of F11 + |F12|
+    and accepted with probability F11 / (F11 + |F12|) for unpolarized light, so a node interval
+    has probability (its bound area / width) times the integral of F11 / (F11 + |F12|)."""
+
+    from monki import configs
+
+    scene = configs.layered_cloud(nx=2, ny=2, nmat=3)
+    sp = scene.species[0]
+    c = sp.cosines.astype(np.float32).astype(np.float64)
+    f11 = sp.f11.astype(np.float32).astype(np.float64)
+    f12 = sp.f12.astype(np.float32).astype(np.float64)
+    rows = oracle.sample(_packed(scene), 0, 0.0, 0.0, 0.2, 200_000)
+    bound = f11 + np.abs(f12)
+    weights = np.empty(c.size - 1)
+
+    for n in range(c.size - 1):
+        t = np.linspace(0, 1, 65)
+        a = f11[n] + t * (f11[n + 1] - f11[n])
+        b = f12[n] + t * (f12[n + 1] - f12[n])
+        area = (c[n + 1] - c[n]) * (bound[n] + bound[n + 1]) / 2
+        weights[n] = area * np.trapezoid(a / (a + np.abs(b)), t)
+
+    nodes = np.concatenate([np.arange(0, c.size - 1, 9), [c.size - 1]])
+    cumulative = np.concatenate([[0.0], np.cumsum(weights)])
+    probabilities = np.diff(cumulative[nodes]) / cumulative[-1]
+    counts, _ = np.histogram(rows[:, 0], c[nodes])
+    statistic, p = _chi_square_p(counts, probabilities)
+    assert p > 1e-3, statistic
+
+
+# ---- Whole histories: closed forms and the reference Fortran --------------------------------
+
+
+def absorbing_column(optical_depth=0.6, albedo=0.3, solar_zenith=35.0, nmat=1):
+    """One periodic column, three 1-km layers, pure absorption (single-scattering albedo 0)."""
+
+    from monki.scene import VACUUM_THICKNESS_KM, TransportScene
+
+    zunit = np.array([1.0, 1.0, 1.0, VACUUM_THICKNESS_KM])
+    beta = np.zeros((1, 1, 4))
+    beta[0, 0, :3] = optical_depth / 3.0
+    return TransportScene(
+        nx=1, ny=1, nz=4, nmat=nmat, xunit=1.0, yunit=1.0, zunit=zunit, beta=beta,
+        fractions=np.ones((1, 1, 1, 4)), ssalb_gas=np.zeros(4), albedo=np.full((1, 1), albedo),
+        depol=0.03, solar_zenith=solar_zenith, solar_azimuth=0.0, view_azimuth=0.0,
+        view_zeniths=(0.0, 30.0, 60.0),
+    )  # fmt: skip
+
+
+def absorbing_column_expectation(scene, optical_depth):
+    """Only the surface reflection contributes: I = A mu0 exp(-tau/mu0 - tau/mu) per photon
+    (Lambertian estimate S1 mu w, times mu0/mu, mc.f90:738-822 and 1456-1474), and every
+    contribution's layer path is dz/mu0 + dz/mu, so each layer AMF is 1/mu0 + 1/mu exactly."""
+
+    mu0 = np.cos(np.deg2rad(scene.solar_zenith))
+    mu = np.cos(np.deg2rad(scene.view_zeniths))
+    radiance = scene.albedo[0, 0] * mu0 * np.exp(-optical_depth / mu0 - optical_depth / mu)
+    return radiance, 1 / mu0 + 1 / mu
+
+
+def test_oracle_absorbing_column_closed_form(oracle):
+    tau = 0.6
+    scene = absorbing_column(tau)
+    photons = 100_000
+    tallies, _, _, flag = oracle.run(_packed(scene), [photons, 0, 1, 7, 100000, 0, photons, 1])
+    assert flag == 0
+    radiance, amf = absorbing_column_expectation(scene, tau)
+    stokes = tallies[0, :3] / photons
+    paths = tallies[0, 3 : 3 + 12].reshape(4, 3, order="F")
+    weights = tallies[0, 15:18]
+    # The surface is reached with probability exp(-tau/mu0): binomial standard error.
+    p = np.exp(-tau / np.cos(np.deg2rad(scene.solar_zenith)))
+    np.testing.assert_allclose(stokes, radiance, rtol=5 * np.sqrt((1 - p) / (p * photons)))
+    # Angles reach the transport as float32 radians: AMFs carry their rounding (~1e-7).
+    np.testing.assert_allclose(paths[:3] / weights / 1.0, np.broadcast_to(amf, (3, 3)), rtol=1e-6)
+    # The vacuum layer holds the launch path only (the detector walk stops below it).
+    mu0 = np.cos(np.deg2rad(scene.solar_zenith))
+    np.testing.assert_allclose(paths[3] / weights, scene.zunit[3] / mu0, rtol=1e-6)
+
+
+@pytest.mark.skipif(not (Path(__file__).parents[1] / "oracle" / "_ref" / "monki_ref").is_file(),
+                    reason="reference Fortran build missing (make -C oracle ref)")  # fmt: skip
+@pytest.mark.parametrize("name", ["clear-IQU", "box-cloud"])
+def test_oracle_matches_reference_fortran_statistically(oracle, name):
+    """The float64 oracle against the reference's unchanged mc.f90 (its own compiler random
+    numbers): radiance and layer paths per pixel/view within combined standard errors."""
+
+    from monki import configs
+    from monki.validation import combined_se_statistics
+    from oracle.oracle import run_reference
+
+    scene = {
+        "clear-IQU": configs.clear_sky(nx=3, ny=2, nmat=3),
+        "box-cloud": configs.box_cloud(nx=4, ny=3, nmat=3, box=(2, 1), offset=(1, 1)),
+    }[name]
+    rounds, ppc = 40, 60
+    per_round = ppc * scene.nx * scene.ny
+    ours, _, _, flag = oracle.run(_packed(scene), [per_round * rounds, 0, 1, 5, 100000, 0, per_round, rounds])
+    assert flag == 0
+    theirs, _, _ = run_reference(scene, ppc, rounds, threads=4)
+    a, b = ours / ppc, theirs / ppc
+    stats = combined_se_statistics(
+        a.mean(axis=0), a.std(axis=0, ddof=1) / np.sqrt(rounds),
+        b.mean(axis=0), b.std(axis=0, ddof=1) / np.sqrt(rounds), floor=1e-12,
+    )  # fmt: skip
+    print(name, stats)
+    assert stats["max_abs_z"] < 5.5
+    assert 0.75 < stats["chi_square_per_dof"] < 1.35

@@ -20,7 +20,7 @@
 extern "C" {
 #endif
 
-/* ABI version; a Python host refuses a library whose version differs.
+/* ABI version (2); a Python host refuses a library whose version differs.
  * Replaces monki_metal_abi_version (kernels/metal/bridge.m:26-28). */
 int monki_hip_abi_version(void);
 
@@ -82,6 +82,76 @@ int monki_hip_probe_philox(uint32_t device, const uint32_t *cases, size_t count,
 
 /* Release device buffers; NULL is accepted (kernels/metal/bridge.m:145-150). */
 void monki_hip_close(void *handle);
+
+/* ---- Learned proposal (guided tracer). No reference counterpart: the reference samples
+ * directions from the phase function only (mc_routines_general_mod.f90:492-932).
+ * A proposal handle owns a BF16 MLP (12 -> 4 x 128 SiLU -> 40: an 8-component von
+ * Mises-Fisher mixture over the outgoing direction), its float32 master parameters and Adam
+ * state, and a device buffer of training samples. */
+
+/* Number of float32 master parameters (56,360). */
+size_t monki_hip_proposal_parameter_count(void);
+
+/* Create a proposal on a device with seeded He initialization; NULL and error on failure. */
+void *monki_hip_proposal_create(uint32_t device, uint32_t seed, char *error, size_t error_size);
+
+/* Replace or read the float32 master parameters (count = parameter count). */
+int monki_hip_proposal_set_parameters(void *proposal, const float *params, size_t count,
+                                      char *error, size_t error_size);
+int monki_hip_proposal_get_parameters(void *proposal, float *params, size_t count);
+
+/* Reserve room for `events` training samples and for the per-photon totals of dispatches of up
+ * to `photons` photons; drops the current samples. */
+int monki_hip_proposal_reserve(void *proposal, size_t events, size_t photons, char *error,
+                               size_t error_size);
+
+/* Replace the samples: features [count][12], unit directions [count][3], weights [count]. */
+int monki_hip_proposal_set_samples(void *proposal, const float *features, const float *directions,
+                                   const float *weights, size_t count, char *error,
+                                   size_t error_size);
+
+/* Sample count and sum of sample weights; copy the first `count` samples back. */
+int monki_hip_proposal_samples(void *proposal, uint64_t *count, double *weight_sum);
+int monki_hip_proposal_read_samples(void *proposal, float *features, float *directions,
+                                    float *weights, size_t count);
+
+/* Gradient (float32, parameter order) and value of -sum w log q(direction) / sum w over the
+ * first `count` samples. */
+int monki_hip_proposal_gradients(void *proposal, size_t count, float *grad, double *loss,
+                                 char *error, size_t error_size);
+
+/* `steps` Adam steps over random minibatches of `batch` samples; losses[steps] receives each
+ * step's minibatch loss (may be NULL). */
+int monki_hip_proposal_train(void *proposal, uint32_t steps, uint32_t batch, float learning_rate,
+                             uint64_t seed, double *losses, char *error, size_t error_size);
+
+/* Network outputs [count][40] for features [count][12] (the path transport uses). */
+int monki_hip_proposal_forward(void *proposal, const float *features, size_t count, float *out,
+                               char *error, size_t error_size);
+
+void monki_hip_proposal_close(void *proposal);
+
+/* Attach a proposal to a transport session (NULL detaches) with the probability alpha of
+ * drawing directions from the phase function, for the scattering events of the kinds in
+ * species_mask (bit 0 gas, bit s + 1 tabulated species s) up to scattering order max_order;
+ * other events are sampled as in monki_hip_run. The proposal must outlive the attachment. */
+int monki_hip_attach_proposal(void *handle, void *proposal, float alpha, uint32_t species_mask,
+                              uint32_t max_order, char *error, size_t error_size);
+
+/* monki_hip_run with the attached proposal: directions from the phase function with
+ * probability alpha, else from the mixture q; weights times p / (alpha p + (1 - alpha) q). */
+int monki_hip_run_guided(void *handle, const uint32_t *controls, double *out, char *error,
+                         size_t error_size);
+
+/* monki_hip_trace for the guided tracer (float32), same record layout. */
+int monki_hip_trace_guided(void *handle, const uint32_t *controls, double *out, float *records,
+                           size_t record_count, char *error, size_t error_size);
+
+/* monki_hip_run (plain float32 transport) that also logs every scattering event into the
+ * proposal's samples: features, direction after the event, and as weight the detector weight
+ * the photon contributed after the event. controls[0] must not exceed the reserved photons. */
+int monki_hip_run_logged(void *handle, const uint32_t *controls, double *out, void *proposal,
+                         char *error, size_t error_size);
 
 #ifdef __cplusplus
 }

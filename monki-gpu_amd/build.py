@@ -9,7 +9,7 @@ from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent
-SOURCE = HERE / "csrc" / "transport.hip"
+SOURCES = [HERE / "csrc" / "transport.hip", HERE / "csrc" / "proposal.hip"]
 OUTPUT = HERE / "lib" / "libmonki_hip.so"
 FLAGS = [
     "--offload-arch=gfx950",
@@ -29,7 +29,7 @@ FLAGS = [
 def build(verbose: bool = False) -> Path:
     OUTPUT.parent.mkdir(parents=True, exist_ok=True)
     temporary = OUTPUT.with_suffix(".so.tmp")
-    command = ["hipcc", *FLAGS, "-I", str(ROOT / "include"), str(SOURCE), "-o", str(temporary)]
+    command = ["hipcc", *FLAGS, "-I", str(ROOT / "include"), *map(str, SOURCES), "-o", str(temporary)]
 
     if verbose:
         print(" ".join(command), flush=True)

@@ -1,0 +1,282 @@
+// Learned scattering-direction proposal: a BF16 MLP (12 inputs, 4 hidden layers of 128 SiLU
+// units, 40 outputs) whose outputs parameterize an 8-component von Mises-Fisher mixture over
+// the outgoing direction. No reference counterpart: the reference samples directions from the
+// phase function only (mc_routines_general_mod.f90:492-932).
+//
+// The network runs on the matrix cores with the batch on the N side of
+// v_mfma_f32_32x32x8_bf16: D = W (M x K) * H (K x 32 samples). Lane t of the wave holds, for
+// sample column n = t % 32 and half h = t / 32,
+//   B operand (K x 32): rows 8 s + 4 h + e, e = 0..3, of k-step s;
+//   D (32 x 32):        rows 8 q + 4 h + i of register 4 q + i.
+// So the accumulator rows of M-block mb, registers 4 q .. 4 q + 3, are exactly the B operand of
+// k-step 4 mb + q of the next layer: layers chain in registers, with no transpose. Weights are
+// stored pre-swizzled in A-operand order (lane t holds W[32 mb + t % 32][8 s + 4 h + e]) so
+// each operand load is one coalesced 8-byte load per lane.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace monki_net {
+
+constexpr int kInputs = 12;
+constexpr int kHidden = 128;
+constexpr int kComponents = 8;
+constexpr int kOutputs = 5 * kComponents;  // per component: mean direction (3), kappa, logit
+constexpr int kLayers = 5;
+constexpr float kKappaMin = 0.05f;
+constexpr float kKappaMax = 2000.0f;
+
+// True and padded shapes of layer l (rows = outputs, cols = inputs).
+__host__ __device__ constexpr int true_rows(int l) { return l == 4 ? kOutputs : kHidden; }
+__host__ __device__ constexpr int true_cols(int l) { return l == 0 ? kInputs : kHidden; }
+__host__ __device__ constexpr int rows(int l) { return l == 4 ? 64 : kHidden; }
+__host__ __device__ constexpr int cols(int l) { return l == 0 ? 16 : kHidden; }
+
+// Master parameters (float32): per layer, W row-major [true_rows][true_cols], then b.
+__host__ __device__ constexpr int param_offset(int l) {
+  return l == 0 ? 0 : param_offset(l - 1) + true_rows(l - 1) * true_cols(l - 1) + true_rows(l - 1);
+}
+constexpr int kParams = param_offset(kLayers);
+
+// Packed bf16 operands, in units of 4 bf16 (one lane's operand of one block).
+__host__ __device__ constexpr int packed_offset(int l) {
+  return l == 0 ? 0 : packed_offset(l - 1) + (rows(l - 1) / 32) * (cols(l - 1) / 8) * 64;
+}
+constexpr int kPacked = packed_offset(kLayers);
+// Transposed operands W^T (for the backward pass) of layers 1..4: M' = cols(l), K' = rows(l).
+__host__ __device__ constexpr int packed_t_offset(int l) {
+  return l <= 1 ? 0 : packed_t_offset(l - 1) + (cols(l - 1) / 32) * (rows(l - 1) / 8) * 64;
+}
+constexpr int kPackedT = packed_t_offset(kLayers);
+// Padded biases (float32).
+__host__ __device__ constexpr int bias_offset(int l) { return l == 0 ? 0 : bias_offset(l - 1) + rows(l - 1); }
+constexpr int kBiases = bias_offset(kLayers);
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
+// Device view of one network.
+struct Net {
+  const bf16x4 *packed;    // kPacked operands
+  const bf16x4 *packed_t;  // kPackedT operands (training only)
+  const float *bias;       // kBiases
+};
+
+// float32 -> bfloat16, round to nearest even (finite inputs).
+__host__ __device__ inline uint16_t bf16_bits(float x) {
+  uint32_t u;
+  __builtin_memcpy(&u, &x, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return uint16_t(u >> 16);
+}
+
+__host__ __device__ inline float bf16_value(uint16_t bits) {
+  const uint32_t u = uint32_t(bits) << 16;
+  float x;
+  __builtin_memcpy(&x, &u, 4);
+  return x;
+}
+
+__device__ inline bf16x4 pack4(float a, float b, float c, float d) {
+  return bf16x4{short(bf16_bits(a)), short(bf16_bits(b)), short(bf16_bits(c)), short(bf16_bits(d))};
+}
+
+__device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
+
+__device__ inline float silu_grad(float z) {
+  const float s = 1.0f / (1.0f + __expf(-z));
+  return s * (1.0f + z * (1.0f - s));
+}
+
+// One dense layer on one 32-sample block: x (Steps k-steps of B operands) -> next-layer B
+// operands (SiLU) or raw outputs. Blocks = rows / 32. z_out, when given, receives the
+// pre-activations in D order [block][register][lane].
+template <int Steps, int Blocks, bool Activate>
+__device__ inline void dense(const bf16x4 *__restrict__ w, const float *__restrict__ bias,
+                             const bf16x4 (&x)[16], bf16x4 (&y)[16], floatx16 (&raw)[2],
+                             float *__restrict__ z_out) {
+  const int lane = __lane_id();
+  const int h = lane >> 5;
+#pragma unroll
+  for (int mb = 0; mb < Blocks; ++mb) {
+    floatx16 c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int s = 0; s < Steps; ++s)
+      c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w[(mb * Steps + s) * 64 + lane], x[s], c, 0, 0, 0);
+#pragma unroll
+    for (int v = 0; v < 16; ++v) c[v] += bias[32 * mb + 8 * (v >> 2) + 4 * h + (v & 3)];
+    if (z_out) {
+#pragma unroll
+      for (int v = 0; v < 16; ++v) z_out[(mb * 16 + v) * 64 + lane] = c[v];
+    }
+    if (Activate) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q)
+        y[4 * mb + q] = pack4(silu(c[4 * q]), silu(c[4 * q + 1]), silu(c[4 * q + 2]), silu(c[4 * q + 3]));
+    } else {
+      raw[mb] = c;
+    }
+    // One M-block at a time: keeps the scheduler from hoisting the weight loads of every
+    // block ahead, which would need all of them in registers at once.
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Forward pass of one 32-sample block. x: input B operands (k-steps 0, 1; rows >= 12 zero).
+// raw: the 64 output rows in D order (rows >= 40 are padding). z (optional): pre-activations
+// of the four hidden layers, 4 x 4 x 16 x 64 floats.
+__device__ inline void forward_block(const Net &net, const bf16x4 (&x)[16], floatx16 (&raw)[2],
+                                     float *__restrict__ z) {
+  bf16x4 a[16], b[16];
+  dense<2, 4, true>(net.packed + packed_offset(0), net.bias + bias_offset(0), x, a, raw,
+                    z ? z : nullptr);
+  dense<16, 4, true>(net.packed + packed_offset(1), net.bias + bias_offset(1), a, b, raw,
+                     z ? z + 1 * 4096 : nullptr);
+  dense<16, 4, true>(net.packed + packed_offset(2), net.bias + bias_offset(2), b, a, raw,
+                     z ? z + 2 * 4096 : nullptr);
+  dense<16, 4, true>(net.packed + packed_offset(3), net.bias + bias_offset(3), a, b, raw,
+                     z ? z + 3 * 4096 : nullptr);
+  dense<16, 2, false>(net.packed + packed_offset(4), net.bias + bias_offset(4), b, a, raw, nullptr);
+}
+
+// Output m of the block's sample column n = lane % 32, from the D registers of the wave.
+// Every lane calls it with the same m; lanes read the value of sample (lane % 32).
+template <int M> __device__ inline float output_row(const floatx16 (&raw)[2]) {
+  constexpr int mb = M / 32, q = (M % 32) / 8, h = (M % 8) / 4, i = M % 4;
+  const int source = (__lane_id() & 31) + 32 * h;
+  return __shfl(raw[mb][4 * q + i], source, 64);
+}
+
+// Gather the 40 outputs of sample (32 nb + lane % 32) into every lane of the wave; lanes keep
+// them when lane / 32 == nb.
+template <int M = 0>
+__device__ inline void gather_outputs(const floatx16 (&raw)[2], bool keep, float (&out)[kOutputs]) {
+  if constexpr (M < kOutputs) {
+    const float value = output_row<M>(raw);
+    if (keep) out[M] = value;
+    gather_outputs<M + 1>(raw, keep, out);
+  }
+}
+
+// Input B operands of block nb from per-lane features f[12] (sample of lane 32 nb + n).
+__device__ inline void scatter_inputs(const float (&f)[kInputs], int nb, bf16x4 (&x)[16]) {
+  const int lane = __lane_id();
+  const int source = 32 * nb + (lane & 31);
+  const bool upper = lane >= 32;
+  float v[16];
+#pragma unroll
+  for (int k = 0; k < kInputs; ++k) v[k] = __shfl(f[k], source, 64);
+#pragma unroll
+  for (int k = kInputs; k < 16; ++k) v[k] = 0.0f;
+  // rows 8 s + 4 h + e
+#pragma unroll
+  for (int s = 0; s < 2; ++s) {
+    float e0 = upper ? v[8 * s + 4] : v[8 * s + 0];
+    float e1 = upper ? v[8 * s + 5] : v[8 * s + 1];
+    float e2 = upper ? v[8 * s + 6] : v[8 * s + 2];
+    float e3 = upper ? v[8 * s + 7] : v[8 * s + 3];
+    x[s] = pack4(e0, e1, e2, e3);
+  }
+}
+
+// ---- von Mises-Fisher mixture -----------------------------------------------------------
+// Component k of the 40 outputs: o[5k..5k+2] unnormalized mean direction, o[5k+3] kappa
+// (softplus, clamped to [kKappaMin, kKappaMax]), o[5k+4] mixture logit (softmax).
+struct Mixture {
+  float mx[kComponents], my[kComponents], mz[kComponents];
+  float kappa[kComponents], weight[kComponents];
+};
+
+__host__ __device__ inline float softplus(float x) { return x > 20.0f ? x : log1pf(expf(x)); }
+
+__device__ inline Mixture mixture_from(const float *o) {
+  Mixture m;
+  float top = -INFINITY;
+#pragma unroll
+  for (int k = 0; k < kComponents; ++k) top = fmaxf(top, o[5 * k + 4]);
+  float total = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kComponents; ++k) {
+    const float x = o[5 * k], y = o[5 * k + 1], z = o[5 * k + 2];
+    const float norm = sqrtf(x * x + y * y + z * z);
+    const bool ok = norm > 1e-6f;
+    m.mx[k] = ok ? x / norm : 0.0f;
+    m.my[k] = ok ? y / norm : 0.0f;
+    m.mz[k] = ok ? z / norm : 1.0f;
+    m.kappa[k] = fminf(fmaxf(softplus(o[5 * k + 3]), kKappaMin), kKappaMax);
+    m.weight[k] = expf(o[5 * k + 4] - top);
+    total += m.weight[k];
+  }
+#pragma unroll
+  for (int k = 0; k < kComponents; ++k) m.weight[k] /= total;
+  return m;
+}
+
+// log of the vMF density kappa / (2 pi (1 - exp(-2 kappa))) exp(kappa (mu . w - 1)).
+__host__ __device__ inline float log_vmf(float kappa, float dot) {
+  return logf(kappa) - logf(6.28318530717958647692f) - log1pf(-expf(-2.0f * kappa)) +
+         kappa * (dot - 1.0f);
+}
+
+__device__ inline float mixture_density(const Mixture &m, float wx, float wy, float wz) {
+  float q = 0.0f;
+#pragma unroll
+  for (int k = 0; k < kComponents; ++k)
+    q += m.weight[k] * expf(log_vmf(m.kappa[k], m.mx[k] * wx + m.my[k] * wy + m.mz[k] * wz));
+  return q;
+}
+
+// Sample one direction of the mixture from three uniforms (Wood 1994 for the vMF cosine).
+__device__ inline void mixture_sample(const Mixture &m, float u0, float u1, float u2, float &wx,
+                                      float &wy, float &wz) {
+  int k = kComponents - 1;
+  float cumulative = 0.0f;
+#pragma unroll
+  for (int j = 0; j < kComponents - 1; ++j) {
+    cumulative += m.weight[j];
+    if (u0 < cumulative && k == kComponents - 1) k = j;
+  }
+  float mx = m.mx[0], my = m.my[0], mz = m.mz[0], kappa = m.kappa[0];
+#pragma unroll
+  for (int j = 1; j < kComponents; ++j) {
+    if (j == k) mx = m.mx[j], my = m.my[j], mz = m.mz[j], kappa = m.kappa[j];
+  }
+  // Cosine to the mean: w = 1 + log(u + (1 - u) exp(-2 kappa)) / kappa, in [-1, 1].
+  const float w = fminf(1.0f, fmaxf(-1.0f, 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kappa)) / kappa));
+  const float s = sqrtf(fmaxf(0.0f, 1.0f - w * w));
+  // Orthonormal frame around the mean (Frisvad's construction, branch-free in z sign).
+  const float sign = mz >= 0.0f ? 1.0f : -1.0f;
+  const float a = -1.0f / (sign + mz);
+  const float b = mx * my * a;
+  const float e1x = 1.0f + sign * mx * mx * a, e1y = sign * b, e1z = -sign * mx;
+  const float e2x = b, e2y = sign + my * my * a, e2z = -my;
+  const float phi = 6.28318530717958647692f * u2;
+  const float c = cosf(phi) * s, d = sinf(phi) * s;
+  wx = w * mx + c * e1x + d * e2x;
+  wy = w * my + c * e1y + d * e2y;
+  wz = w * mz + c * e1z + d * e2z;
+  const float norm = rsqrtf(wx * wx + wy * wy + wz * wz);
+  wx *= norm, wy *= norm, wz *= norm;
+}
+
+// Scattering events logged by the plain tracer for training (device pointers).
+struct EventLog {
+  float *features;    // [capacity][kInputs]
+  float *directions;  // [capacity][3] direction after the event
+  float *cumulative;  // [capacity] detector weight of the history up to the event
+  uint32_t *photon;   // [capacity] photon of the dispatch
+  float *totals;      // [photons] detector weight of each whole history
+  unsigned *count;    // events logged (may exceed capacity; extra events are dropped)
+  uint32_t capacity, photons;
+};
+
+// Host interface of a proposal handle, for the transport translation unit.
+Net proposal_net(void *handle);
+EventLog proposal_log(void *handle);
+int proposal_device(void *handle);
+uint32_t proposal_samples(void *handle);
+// After a logging dispatch: training weights of events [first, logged) from the totals.
+bool proposal_finish_log(void *handle, uint32_t first, hipStream_t stream, char *error, size_t size);
+
+}  // namespace monki_net

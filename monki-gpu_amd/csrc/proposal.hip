@@ -1,0 +1,684 @@
+// Training and probes of the learned scattering-direction proposal (network in proposal.h).
+//
+// Samples are scattering events logged by the plain tracer: features of the event, the
+// direction the phase function gave the photon, and G = the detector weight (sum over views of
+// C1, mc_routines_cartesian_mod.f90:1173-1194) the photon contributed after that event. The
+// network is fitted by weighted maximum likelihood, loss = -sum G log q(direction | features) /
+// sum G: drawn from the phase function and weighted by what followed, the fitted q approaches
+// the product of the phase function and the photon's future contribution.
+//
+// One wave trains on 64 samples per launch block: the forward pass runs on the matrix cores
+// (proposal.h), keeping the hidden pre-activations in a scratch buffer; the backward pass
+// chains W^T products the same way, and the weight gradients dW = dZ H^T come from two
+// 128 x 64 bf16 tiles in LDS (samples on the reduction side) and one coalesced float atomic
+// per gradient element and wave.
+
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <random>
+#include <vector>
+
+#include "monki_hip.h"
+#include "proposal.h"
+
+using namespace monki_net;
+
+namespace {
+
+constexpr int kTileStride = 68;  // bf16 per tile row: 64 samples + 4 (bank spread)
+constexpr int kZPerWave = 2 * 4 * 4096;
+
+struct Proposal {
+  int device = -1;
+  float *params = nullptr, *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+  bf16x4 *packed = nullptr, *packed_t = nullptr;
+  float *bias = nullptr;
+  float *z = nullptr;
+  size_t z_waves = 0;
+  // Samples: features [n][12], directions [n][3], weights [n].
+  float *features = nullptr, *directions = nullptr, *weights = nullptr;
+  size_t capacity = 0, count = 0;
+  // Per-event contribution so far and per-photon totals of a logging dispatch.
+  float *cumulative = nullptr, *totals = nullptr;
+  uint32_t *photon = nullptr;
+  unsigned *log_count = nullptr;
+  size_t photon_capacity = 0;
+  double *loss = nullptr;
+  double weight_sum = 0.0;
+  uint64_t step = 0;
+  hipStream_t stream = nullptr;
+};
+
+bool ok(hipError_t status, char *error, size_t size, const char *what) {
+  if (status == hipSuccess) return true;
+  if (error && size) snprintf(error, size, "%s: %s", what, hipGetErrorString(status));
+  return false;
+}
+
+void fail(char *error, size_t size, const char *text) {
+  if (error && size) snprintf(error, size, "%s", text);
+}
+
+// ---- Kernels -----------------------------------------------------------------------------
+
+// Master parameters -> padded bf16 operands (A order and transposed) and padded biases.
+__global__ void pack_kernel(const float *__restrict__ params, bf16x4 *__restrict__ packed,
+                            bf16x4 *__restrict__ packed_t, float *__restrict__ bias) {
+  const int index = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = index & 63;
+  const int block = index >> 6;
+  auto weight = [&](int l, int r, int c) -> float {
+    return r < true_rows(l) && c < true_cols(l) ? params[param_offset(l) + r * true_cols(l) + c] : 0.0f;
+  };
+  // Forward operands: block (mb, s) of layer l, lane -> W[32 mb + lane % 32][8 s + 4 h + e].
+  if (index < kPacked) {
+    int l = 0;
+    while (l + 1 < kLayers && index >= packed_offset(l + 1)) ++l;
+    const int local = (index - packed_offset(l)) >> 6;
+    const int steps = cols(l) / 8;
+    const int mb = local / steps, s = local % steps;
+    const int r = 32 * mb + (lane & 31), c = 8 * s + 4 * (lane >> 5);
+    packed[index] = pack4(weight(l, r, c), weight(l, r, c + 1), weight(l, r, c + 2), weight(l, r, c + 3));
+  }
+  // Transposed operands of layers 1..4: A'[m'][k'] = W[k'][m'], M' = cols(l), K' = rows(l).
+  if (index < kPackedT) {
+    int l = 1;
+    while (l + 1 < kLayers && index >= packed_t_offset(l + 1)) ++l;
+    const int local = (index - packed_t_offset(l)) >> 6;
+    const int steps = rows(l) / 8;
+    const int mb = local / steps, s = local % steps;
+    const int m = 32 * mb + (lane & 31), k = 8 * s + 4 * (lane >> 5);
+    packed_t[index] = pack4(weight(l, k, m), weight(l, k + 1, m), weight(l, k + 2, m), weight(l, k + 3, m));
+  }
+  (void)block;
+  if (index < kBiases) {
+    int l = 0;
+    while (l + 1 < kLayers && index >= bias_offset(l + 1)) ++l;
+    const int r = index - bias_offset(l);
+    bias[index] = r < true_rows(l) ? params[param_offset(l) + true_rows(l) * true_cols(l) + r] : 0.0f;
+  }
+}
+
+__device__ inline uint32_t mix32(uint64_t x) {
+  x ^= x >> 33;
+  x *= 0xff51afd7ed558ccdull;
+  x ^= x >> 33;
+  x *= 0xc4ceb9fe1a85ec53ull;
+  x ^= x >> 33;
+  return uint32_t(x);
+}
+
+// Loss -G log q(w) of the mixture and its gradient with respect to the 40 outputs.
+__device__ inline float head_loss(const float *o, float wx, float wy, float wz, float g, float *dy) {
+  float logit_top = -INFINITY;
+  for (int k = 0; k < kComponents; ++k) logit_top = fmaxf(logit_top, o[5 * k + 4]);
+  float logit_sum = 0.0f;
+  for (int k = 0; k < kComponents; ++k) logit_sum += expf(o[5 * k + 4] - logit_top);
+  const float log_norm = logit_top + logf(logit_sum);
+  float ell[kComponents], kappa[kComponents], dot[kComponents], mu[kComponents][3], norm[kComponents];
+  float top = -INFINITY;
+  for (int k = 0; k < kComponents; ++k) {
+    const float x = o[5 * k], y = o[5 * k + 1], z = o[5 * k + 2];
+    norm[k] = sqrtf(x * x + y * y + z * z);
+    const bool valid = norm[k] > 1e-6f;
+    mu[k][0] = valid ? x / norm[k] : 0.0f;
+    mu[k][1] = valid ? y / norm[k] : 0.0f;
+    mu[k][2] = valid ? z / norm[k] : 1.0f;
+    kappa[k] = fminf(fmaxf(softplus(o[5 * k + 3]), kKappaMin), kKappaMax);
+    dot[k] = mu[k][0] * wx + mu[k][1] * wy + mu[k][2] * wz;
+    ell[k] = o[5 * k + 4] - log_norm + log_vmf(kappa[k], dot[k]);
+    top = fmaxf(top, ell[k]);
+  }
+  float sum = 0.0f;
+  for (int k = 0; k < kComponents; ++k) sum += expf(ell[k] - top);
+  const float log_q = top + logf(sum);
+  for (int k = 0; k < kComponents; ++k) {
+    const float r = expf(ell[k] - log_q);
+    const float pi = expf(o[5 * k + 4] - log_norm);
+    const float dell = -g * r;
+    dy[5 * k + 4] = -g * (r - pi);
+    const float soft = softplus(o[5 * k + 3]);
+    const float dlog_kappa = 1.0f / kappa[k] - 2.0f / expm1f(2.0f * kappa[k]) + dot[k] - 1.0f;
+    const bool free_kappa = soft > kKappaMin && soft < kKappaMax;
+    dy[5 * k + 3] = free_kappa ? dell * dlog_kappa / (1.0f + expf(-o[5 * k + 3])) : 0.0f;
+    // d log vmf / d mu = kappa w, projected onto the tangent of the normalization.
+    const float gx = dell * kappa[k] * wx, gy = dell * kappa[k] * wy, gz = dell * kappa[k] * wz;
+    const float along = gx * mu[k][0] + gy * mu[k][1] + gz * mu[k][2];
+    const bool valid = norm[k] > 1e-6f;
+    dy[5 * k + 0] = valid ? (gx - along * mu[k][0]) / norm[k] : 0.0f;
+    dy[5 * k + 1] = valid ? (gy - along * mu[k][1]) / norm[k] : 0.0f;
+    dy[5 * k + 2] = valid ? (gz - along * mu[k][2]) / norm[k] : 0.0f;
+  }
+  return -g * log_q;
+}
+
+// Gather value of row `row` (lane-dependent) of the per-lane array v[count] held by lane src.
+template <int Count>
+__device__ inline float shuffle_row(const float (&v)[Count], int src, int row) {
+  float out = 0.0f;
+#pragma unroll
+  for (int k = 0; k < Count; ++k) {
+    const float value = __shfl(v[k], src, 64);
+    if (row == k) out = value;
+  }
+  return out;
+}
+
+__device__ inline float bf16_round(float x) { return bf16_value(bf16_bits(x)); }
+
+// One training block: 64 samples (sample indices from `indices`, or hashed from (seed, step)).
+__global__ void __launch_bounds__(64)
+    train_kernel(Net net, const float *__restrict__ features, const float *__restrict__ directions,
+                 const float *__restrict__ weights, const uint32_t *__restrict__ indices, uint32_t count,
+                 uint32_t batch, uint64_t seed, float scale, float *__restrict__ grad,
+                 float *__restrict__ z_scratch, double *__restrict__ loss_sum) {
+  __shared__ uint16_t dz_tile[128 * kTileStride];
+  __shared__ uint16_t h_tile[128 * kTileStride];
+  const int lane = __lane_id();
+  const int h = lane >> 5, n = lane & 31;
+  const uint32_t sample = blockIdx.x * 64u + uint32_t(lane);
+  const bool valid = sample < batch;
+  const uint32_t index = !valid ? 0u : indices ? indices[sample] : mix32(seed * 0x9E3779B97F4A7C15ull + sample) % count;
+  float f[kInputs];
+#pragma unroll
+  for (int k = 0; k < kInputs; ++k) f[k] = valid ? features[size_t(index) * kInputs + k] : 0.0f;
+  const float wx = valid ? directions[size_t(index) * 3] : 0.0f;
+  const float wy = valid ? directions[size_t(index) * 3 + 1] : 0.0f;
+  const float wz = valid ? directions[size_t(index) * 3 + 2] : 1.0f;
+  const float g = valid ? weights[index] * scale : 0.0f;
+  float *z = z_scratch + size_t(blockIdx.x) * kZPerWave;
+
+  float o[kOutputs];
+#pragma unroll
+  for (int nb = 0; nb < 2; ++nb) {
+    bf16x4 x[16];
+    scatter_inputs(f, nb, x);
+    floatx16 raw[2];
+    forward_block(net, x, raw, z + nb * 16384);
+    gather_outputs(raw, h == nb, o);
+  }
+  float dy[kOutputs];
+  const float loss = valid ? head_loss(o, wx, wy, wz, g, dy) : 0.0f;
+  if (!valid) {
+#pragma unroll
+    for (int k = 0; k < kOutputs; ++k) dy[k] = 0.0f;
+  }
+  double total = loss;
+  for (int offset = 32; offset > 0; offset >>= 1) total += __shfl_xor(total, offset, 64);
+  if (lane == 0) atomicAdd(loss_sum, total);
+
+  // dY as B operands (rows 8 s + 4 h + e of 64, of the sample column 32 nb + n).
+  bf16x4 d[2][16];
+#pragma unroll
+  for (int nb = 0; nb < 2; ++nb) {
+    const int src = 32 * nb + n;
+#pragma unroll
+    for (int s = 0; s < 8; ++s) {
+      float e[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = 8 * s + 4 * h + j;
+        e[j] = shuffle_row<kOutputs>(dy, src, row);
+      }
+      d[nb][s] = pack4(e[0], e[1], e[2], e[3]);
+    }
+  }
+
+#pragma unroll
+  for (int l = kLayers - 1; l >= 0; --l) {
+    const int r_steps = rows(l) / 8;
+    // dZ_{l+1} tile [row][sample] from the B operands.
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+#pragma unroll
+      for (int s = 0; s < r_steps; ++s) {
+        const bf16x4 v = d[nb][s];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) dz_tile[(8 * s + 4 * h + j) * kTileStride + 32 * nb + n] = uint16_t(v[j]);
+      }
+    }
+    // H_l tile [feature][sample]: inputs for l = 0, else SiLU of the saved pre-activations.
+    const int c_blocks = (cols(l) + 31) / 32;
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      if (l == 0) {
+        const int src = 32 * nb + n;
+#pragma unroll
+        for (int k = 0; k < 32; k += 2) {
+          const int row = k + h;
+          const float value = shuffle_row<kInputs>(f, src, row);
+          h_tile[row * kTileStride + 32 * nb + n] = row < kInputs ? bf16_bits(value) : 0;
+        }
+      } else {
+        const float *zl = z + nb * 16384 + (l - 1) * 4096;
+#pragma unroll
+        for (int mb = 0; mb < 4; ++mb) {
+#pragma unroll
+          for (int v = 0; v < 16; ++v) {
+            const int row = 32 * mb + 8 * (v >> 2) + 4 * h + (v & 3);
+            h_tile[row * kTileStride + 32 * nb + n] = bf16_bits(silu(zl[(mb * 16 + v) * 64 + lane]));
+          }
+        }
+      }
+    }
+    __syncthreads();
+    // dW_l = dZ_{l+1} H_l^T over the 64 samples; db_l = row sums of dZ_{l+1}.
+    for (int mb = 0; mb < rows(l) / 32; ++mb) {
+      for (int cb = 0; cb < c_blocks; ++cb) {
+        floatx16 c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < 8; ++s) {
+          const uint16_t *a = dz_tile + (32 * mb + n) * kTileStride + 8 * s + 4 * h;
+          const uint16_t *b = h_tile + (32 * cb + n) * kTileStride + 8 * s + 4 * h;
+          const bf16x4 av = {short(a[0]), short(a[1]), short(a[2]), short(a[3])};
+          const bf16x4 bv = {short(b[0]), short(b[1]), short(b[2]), short(b[3])};
+          c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, c, 0, 0, 0);
+        }
+        const int col = 32 * cb + n;
+#pragma unroll
+        for (int v = 0; v < 16; ++v) {
+          const int row = 32 * mb + 8 * (v >> 2) + 4 * h + (v & 3);
+          if (row < true_rows(l) && col < true_cols(l) && c[v] != 0.0f)
+            atomicAdd(grad + param_offset(l) + row * true_cols(l) + col, c[v]);
+        }
+      }
+    }
+    for (int row = lane; row < true_rows(l); row += 64) {
+      float sum = 0.0f;
+      for (int t = 0; t < 64; ++t) sum += bf16_value(dz_tile[row * kTileStride + t]);
+      if (sum != 0.0f) atomicAdd(grad + param_offset(l) + true_rows(l) * true_cols(l) + row, sum);
+    }
+    __syncthreads();
+    if (l == 0) break;
+    // dH_l = W_l^T dZ_{l+1}, then dZ_l = dH_l * silu'(Z_l): chained like the forward pass.
+#pragma unroll
+    for (int nb = 0; nb < 2; ++nb) {
+      const float *zl = z + nb * 16384 + (l - 1) * 4096;
+      bf16x4 next[16];
+#pragma unroll
+      for (int mb = 0; mb < 4; ++mb) {
+        floatx16 c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+        for (int s = 0; s < r_steps; ++s)
+          c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(net.packed_t[packed_t_offset(l) + (mb * r_steps + s) * 64 + lane],
+                                                       d[nb][s], c, 0, 0, 0);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          float e[4];
+#pragma unroll
+          for (int j = 0; j < 4; ++j) e[j] = c[4 * q + j] * silu_grad(zl[(mb * 16 + 4 * q + j) * 64 + lane]);
+          next[4 * mb + q] = pack4(e[0], e[1], e[2], e[3]);
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+#pragma unroll
+      for (int s = 0; s < 16; ++s) d[nb][s] = next[s];
+    }
+  }
+}
+
+// Forward probe: 40 outputs per sample, the per-lane path that transport uses.
+__global__ void __launch_bounds__(64) forward_kernel(Net net, const float *__restrict__ features,
+                                                     uint32_t count, float *__restrict__ out) {
+  const int lane = __lane_id();
+  const uint32_t sample = blockIdx.x * 64u + uint32_t(lane);
+  float f[kInputs];
+#pragma unroll
+  for (int k = 0; k < kInputs; ++k) f[k] = sample < count ? features[size_t(sample) * kInputs + k] : 0.0f;
+  float o[kOutputs];
+#pragma unroll
+  for (int nb = 0; nb < 2; ++nb) {
+    bf16x4 x[16];
+    scatter_inputs(f, nb, x);
+    floatx16 raw[2];
+    forward_block(net, x, raw, nullptr);
+    gather_outputs(raw, (lane >> 5) == nb, o);
+  }
+  if (sample < count) {
+    for (int k = 0; k < kOutputs; ++k) out[size_t(sample) * kOutputs + k] = o[k];
+  }
+}
+
+__global__ void adam_kernel(float *__restrict__ params, const float *__restrict__ grad,
+                            float *__restrict__ m, float *__restrict__ v, int count, float lr,
+                            float beta1, float beta2, float correction1, float correction2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const float g = grad[i];
+  const float mi = beta1 * m[i] + (1.0f - beta1) * g;
+  const float vi = beta2 * v[i] + (1.0f - beta2) * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  params[i] -= lr * (mi / correction1) / (sqrtf(vi / correction2) + 1e-8f);
+}
+
+// Training weights of the events logged in one dispatch: G = photon total - contribution so far.
+__global__ void targets_kernel(const float *__restrict__ totals, const uint32_t *__restrict__ photon,
+                               const float *__restrict__ cumulative, float *__restrict__ weights,
+                               uint32_t first, uint32_t last, double *__restrict__ sum) {
+  const uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
+  double value = 0.0;
+  if (i < last) {
+    const float g = fmaxf(0.0f, totals[photon[i]] - cumulative[i]);
+    weights[i] = g;
+    value = g;
+  }
+  for (int offset = 32; offset > 0; offset >>= 1) value += __shfl_xor(value, offset, 64);
+  if (__lane_id() == 0 && value != 0.0) atomicAdd(sum, value);
+}
+
+Net view(const Proposal *p) { return Net{p->packed, p->packed_t, p->bias}; }
+
+bool repack(Proposal *p, char *error, size_t size) {
+  const int threads = 256;
+  const int work = kPacked > kPackedT ? kPacked : kPackedT;
+  hipLaunchKernelGGL(pack_kernel, dim3((work + threads - 1) / threads), dim3(threads), 0, p->stream,
+                     p->params, p->packed, p->packed_t, p->bias);
+  return ok(hipGetLastError(), error, size, "pack proposal");
+}
+
+void release(Proposal *p) {
+  if (!p) return;
+  if (p->device >= 0) (void)hipSetDevice(p->device);
+  if (p->stream) (void)hipStreamDestroy(p->stream);
+  void *buffers[] = {p->params, p->grad, p->adam_m, p->adam_v, p->packed, p->packed_t, p->bias, p->z,
+                     p->features, p->directions, p->weights, p->cumulative, p->totals, p->photon,
+                     p->log_count, p->loss};
+  for (void *b : buffers) (void)hipFree(b);
+  delete p;
+}
+
+bool ensure_scratch(Proposal *p, size_t waves, char *error, size_t size) {
+  if (waves <= p->z_waves) return true;
+  (void)hipFree(p->z);
+  p->z = nullptr;
+  p->z_waves = 0;
+  if (!ok(hipMalloc(&p->z, waves * kZPerWave * sizeof(float)), error, size, "allocate proposal scratch"))
+    return false;
+  p->z_waves = waves;
+  return true;
+}
+
+// Gradient of the mean loss over `batch` samples into p->grad (zeroed first); returns the loss.
+bool gradients(Proposal *p, const uint32_t *indices, uint32_t batch, uint64_t seed, double &loss,
+               char *error, size_t size) {
+  const size_t waves = (batch + 63) / 64;
+  if (!ensure_scratch(p, waves, error, size)) return false;
+  const float scale = p->weight_sum > 0.0 ? float(double(p->count) / p->weight_sum / batch) : 0.0f;
+  if (!ok(hipMemsetAsync(p->grad, 0, kParams * sizeof(float), p->stream), error, size, "clear gradient") ||
+      !ok(hipMemsetAsync(p->loss, 0, sizeof(double), p->stream), error, size, "clear loss"))
+    return false;
+  hipLaunchKernelGGL(train_kernel, dim3(waves), dim3(64), 0, p->stream, view(p), p->features,
+                     p->directions, p->weights, indices, uint32_t(p->count), batch, seed, scale, p->grad,
+                     p->z, p->loss);
+  if (!ok(hipGetLastError(), error, size, "train launch") ||
+      !ok(hipMemcpyAsync(&loss, p->loss, sizeof(double), hipMemcpyDeviceToHost, p->stream), error, size,
+          "copy loss") ||
+      !ok(hipStreamSynchronize(p->stream), error, size, "train"))
+    return false;
+  return true;
+}
+
+}  // namespace
+
+// Internal interface for the transport translation unit (same library).
+namespace monki_net {
+
+Net proposal_net(void *handle) { return view(static_cast<Proposal *>(handle)); }
+
+EventLog proposal_log(void *handle) {
+  Proposal *p = static_cast<Proposal *>(handle);
+  return EventLog{p->features, p->directions, p->cumulative, p->photon, p->totals, p->log_count,
+                  uint32_t(p->capacity), uint32_t(p->photon_capacity)};
+}
+
+int proposal_device(void *handle) { return static_cast<Proposal *>(handle)->device; }
+
+uint32_t proposal_samples(void *handle) { return uint32_t(static_cast<Proposal *>(handle)->count); }
+
+bool proposal_finish_log(void *handle, uint32_t first, hipStream_t stream, char *error, size_t size) {
+  Proposal *p = static_cast<Proposal *>(handle);
+  unsigned logged = 0;
+  if (!ok(hipMemcpyAsync(&logged, p->log_count, sizeof(unsigned), hipMemcpyDeviceToHost, stream), error,
+          size, "copy log count") ||
+      !ok(hipStreamSynchronize(stream), error, size, "log count"))
+    return false;
+  const uint32_t last = logged < p->capacity ? logged : uint32_t(p->capacity);
+  if (last > first) {
+    if (!ok(hipMemsetAsync(p->loss, 0, sizeof(double), stream), error, size, "clear sum")) return false;
+    hipLaunchKernelGGL(targets_kernel, dim3((last - first + 255) / 256), dim3(256), 0, stream, p->totals,
+                       p->photon, p->cumulative, p->weights, first, last, p->loss);
+    double sum = 0.0;
+    if (!ok(hipGetLastError(), error, size, "targets launch") ||
+        !ok(hipMemcpyAsync(&sum, p->loss, sizeof(double), hipMemcpyDeviceToHost, stream), error, size,
+            "copy target sum") ||
+        !ok(hipStreamSynchronize(stream), error, size, "targets"))
+      return false;
+    p->weight_sum += sum;
+  }
+  p->count = last;
+  return true;
+}
+
+}  // namespace monki_net
+
+extern "C" {
+
+size_t monki_hip_proposal_parameter_count(void) { return size_t(kParams); }
+
+void *monki_hip_proposal_create(uint32_t device, uint32_t seed, char *error, size_t error_size) {
+  int devices = 0;
+  if (hipGetDeviceCount(&devices) != hipSuccess || device >= uint32_t(devices)) {
+    fail(error, error_size, "HIP device unavailable");
+    return nullptr;
+  }
+  Proposal *p = new Proposal();
+  p->device = int(device);
+  // He-style initialization for SiLU layers; small output layer so the first mixture is broad.
+  std::vector<float> host(kParams, 0.0f);
+  std::mt19937 generator(seed);
+  for (int l = 0; l < kLayers; ++l) {
+    const float scale = (l == kLayers - 1 ? 0.1f : 1.0f) * sqrtf(2.0f / float(true_cols(l)));
+    std::normal_distribution<float> normal(0.0f, scale);
+    for (int i = 0; i < true_rows(l) * true_cols(l); ++i) host[param_offset(l) + i] = normal(generator);
+  }
+  for (int k = 0; k < kComponents; ++k)  // kappa = softplus(1) ~ 1.3
+    host[param_offset(4) + true_rows(4) * true_cols(4) + 5 * k + 3] = 1.0f;
+  if (!ok(hipSetDevice(int(device)), error, error_size, "hipSetDevice") ||
+      !ok(hipStreamCreate(&p->stream), error, error_size, "create stream") ||
+      !ok(hipMalloc(&p->params, kParams * sizeof(float)), error, error_size, "allocate parameters") ||
+      !ok(hipMalloc(&p->grad, kParams * sizeof(float)), error, error_size, "allocate gradient") ||
+      !ok(hipMalloc(&p->adam_m, kParams * sizeof(float)), error, error_size, "allocate moments") ||
+      !ok(hipMalloc(&p->adam_v, kParams * sizeof(float)), error, error_size, "allocate moments") ||
+      !ok(hipMalloc(&p->packed, kPacked * sizeof(bf16x4)), error, error_size, "allocate operands") ||
+      !ok(hipMalloc(&p->packed_t, kPackedT * sizeof(bf16x4)), error, error_size, "allocate operands") ||
+      !ok(hipMalloc(&p->bias, kBiases * sizeof(float)), error, error_size, "allocate biases") ||
+      !ok(hipMalloc(&p->log_count, sizeof(unsigned)), error, error_size, "allocate log counter") ||
+      !ok(hipMalloc(&p->loss, sizeof(double)), error, error_size, "allocate loss") ||
+      !ok(hipMemcpy(p->params, host.data(), kParams * sizeof(float), hipMemcpyHostToDevice), error,
+          error_size, "upload parameters") ||
+      !ok(hipMemset(p->adam_m, 0, kParams * sizeof(float)), error, error_size, "clear moments") ||
+      !ok(hipMemset(p->adam_v, 0, kParams * sizeof(float)), error, error_size, "clear moments") ||
+      !ok(hipMemset(p->log_count, 0, sizeof(unsigned)), error, error_size, "clear log counter") ||
+      !repack(p, error, error_size) || !ok(hipStreamSynchronize(p->stream), error, error_size, "pack")) {
+    release(p);
+    return nullptr;
+  }
+  return p;
+}
+
+int monki_hip_proposal_set_parameters(void *handle, const float *params, size_t count, char *error,
+                                      size_t error_size) {
+  Proposal *p = static_cast<Proposal *>(handle);
+  if (!p || !params || count != size_t(kParams)) {
+    fail(error, error_size, "Invalid proposal parameters");
+    return 1;
+  }
+  if (!ok(hipSetDevice(p->device), error, error_size, "hipSetDevice") ||
+      !ok(hipMemcpy(p->params, params, kParams * sizeof(float), hipMemcpyHostToDevice), error, error_size,
+          "upload parameters") ||
+      !repack(p, error, error_size) || !ok(hipStreamSynchronize(p->stream), error, error_size, "pack"))
+    return 1;
+  return 0;
+}
+
+int monki_hip_proposal_get_parameters(void *handle, float *params, size_t count) {
+  Proposal *p = static_cast<Proposal *>(handle);
+  if (!p || !params || count != size_t(kParams)) return 1;
+  if (hipSetDevice(p->device) != hipSuccess) return 1;
+  return hipMemcpy(params, p->params, kParams * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess ? 0 : 1;
+}
+
+int monki_hip_proposal_reserve(void *handle, size_t events, size_t photons, char *error, size_t error_size) {
+  Proposal *p = static_cast<Proposal *>(handle);
+  if (!p || events == 0 || events > (size_t(1) << 31) || photons > (size_t(1) << 31)) {
+    fail(error, error_size, "Invalid proposal sample capacity");
+    return 1;
+  }
+  if (!ok(hipSetDevice(p->device), error, error_size, "hipSetDevice")) return 1;
+  void **buffers[] = {(void **)&p->features, (void **)&p->directions, (void **)&p->weights,
+                      (void **)&p->cumulative, (void **)&p->photon, (void **)&p->totals};
+  for (void **b : buffers) {
+    (void)hipFree(*b);
+    *b = nullptr;
+  }
+  p->capacity = p->count = p->photon_capacity = 0;
+  p->weight_sum = 0.0;
+  if (!ok(hipMalloc(&p->features, events * kInputs * sizeof(float)), error, error_size, "allocate samples") ||
+      !ok(hipMalloc(&p->directions, events * 3 * sizeof(float)), error, error_size, "allocate samples") ||
+      !ok(hipMalloc(&p->weights, events * sizeof(float)), error, error_size, "allocate samples") ||
+      !ok(hipMalloc(&p->cumulative, events * sizeof(float)), error, error_size, "allocate samples") ||
+      !ok(hipMalloc(&p->photon, events * sizeof(uint32_t)), error, error_size, "allocate samples") ||
+      (photons && !ok(hipMalloc(&p->totals, photons * sizeof(float)), error, error_size, "allocate totals")) ||
+      !ok(hipMemset(p->log_count, 0, sizeof(unsigned)), error, error_size, "clear log counter"))
+    return 1;
+  p->capacity = events;
+  p->photon_capacity = photons;
+  return 0;
+}
+
+int monki_hip_proposal_set_samples(void *handle, const float *features, const float *directions,
+                                   const float *weights, size_t count, char *error, size_t error_size) {
+  Proposal *p = static_cast<Proposal *>(handle);
+  if (!p || !features || !directions || !weights || count == 0) {
+    fail(error, error_size, "Invalid proposal samples");
+    return 1;
+  }
+  if (count > p->capacity && monki_hip_proposal_reserve(handle, count, 0, error, error_size)) return 1;
+  if (!ok(hipSetDevice(p->device), error, error_size, "hipSetDevice") ||
+      !ok(hipMemcpy(p->features, features, count * kInputs * sizeof(float), hipMemcpyHostToDevice), error,
+          error_size, "upload samples") ||
+      !ok(hipMemcpy(p->directions, directions, count * 3 * sizeof(float), hipMemcpyHostToDevice), error,
+          error_size, "upload samples") ||
+      !ok(hipMemcpy(p->weights, weights, count * sizeof(float), hipMemcpyHostToDevice), error, error_size,
+          "upload samples"))
+    return 1;
+  double sum = 0.0;
+  for (size_t i = 0; i < count; ++i) sum += weights[i];
+  p->count = count;
+  p->weight_sum = sum;
+  return 0;
+}
+
+int monki_hip_proposal_samples(void *handle, uint64_t *count, double *weight_sum) {
+  Proposal *p = static_cast<Proposal *>(handle);
+  if (!p) return 1;
+  if (count) *count = p->count;
+  if (weight_sum) *weight_sum = p->weight_sum;
+  return 0;
+}
+
+int monki_hip_proposal_read_samples(void *handle, float *features, float *directions, float *weights,
+                                    size_t count) {
+  Proposal *p = static_cast<Proposal *>(handle);
+  if (!p || count > p->count || hipSetDevice(p->device) != hipSuccess) return 1;
+  if (features && hipMemcpy(features, p->features, count * kInputs * sizeof(float), hipMemcpyDeviceToHost))
+    return 1;
+  if (directions && hipMemcpy(directions, p->directions, count * 3 * sizeof(float), hipMemcpyDeviceToHost))
+    return 1;
+  if (weights && hipMemcpy(weights, p->weights, count * sizeof(float), hipMemcpyDeviceToHost)) return 1;
+  return 0;
+}
+
+int monki_hip_proposal_gradients(void *handle, size_t count, float *grad, double *loss, char *error,
+                                 size_t error_size) {
+  Proposal *p = static_cast<Proposal *>(handle);
+  if (!p || !grad || count == 0 || count > p->count) {
+    fail(error, error_size, "Invalid gradient request");
+    return 1;
+  }
+  uint32_t *indices = nullptr;
+  std::vector<uint32_t> host(count);
+  for (size_t i = 0; i < count; ++i) host[i] = uint32_t(i);
+  double value = 0.0;
+  const bool good =
+      ok(hipSetDevice(p->device), error, error_size, "hipSetDevice") &&
+      ok(hipMalloc(&indices, count * sizeof(uint32_t)), error, error_size, "allocate indices") &&
+      ok(hipMemcpy(indices, host.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice), error, error_size,
+         "upload indices") &&
+      gradients(p, indices, uint32_t(count), 0, value, error, error_size) &&
+      ok(hipMemcpy(grad, p->grad, kParams * sizeof(float), hipMemcpyDeviceToHost), error, error_size,
+         "copy gradient");
+  (void)hipFree(indices);
+  if (loss) *loss = value;
+  return good ? 0 : 1;
+}
+
+int monki_hip_proposal_train(void *handle, uint32_t steps, uint32_t batch, float learning_rate,
+                             uint64_t seed, double *losses, char *error, size_t error_size) {
+  Proposal *p = static_cast<Proposal *>(handle);
+  if (!p || steps == 0 || batch == 0 || batch > (1u << 24) || p->count == 0 || !(learning_rate > 0.0f)) {
+    fail(error, error_size, "Invalid training request (no samples?)");
+    return 1;
+  }
+  if (!ok(hipSetDevice(p->device), error, error_size, "hipSetDevice")) return 1;
+  const float beta1 = 0.9f, beta2 = 0.999f;
+  for (uint32_t i = 0; i < steps; ++i) {
+    double loss = 0.0;
+    if (!gradients(p, nullptr, batch, seed * 1000003ull + p->step, loss, error, error_size)) return 1;
+    ++p->step;
+    const float c1 = 1.0f - powf(beta1, float(p->step));
+    const float c2 = 1.0f - powf(beta2, float(p->step));
+    hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, p->stream, p->params, p->grad,
+                       p->adam_m, p->adam_v, kParams, learning_rate, beta1, beta2, c1, c2);
+    if (!ok(hipGetLastError(), error, error_size, "adam launch") || !repack(p, error, error_size)) return 1;
+    if (losses) losses[i] = loss;
+  }
+  return ok(hipStreamSynchronize(p->stream), error, error_size, "train") ? 0 : 1;
+}
+
+int monki_hip_proposal_forward(void *handle, const float *features, size_t count, float *out, char *error,
+                               size_t error_size) {
+  Proposal *p = static_cast<Proposal *>(handle);
+  if (!p || !features || !out || count == 0 || count > (1u << 24)) {
+    fail(error, error_size, "Invalid forward request");
+    return 1;
+  }
+  float *din = nullptr, *dout = nullptr;
+  const bool good =
+      ok(hipSetDevice(p->device), error, error_size, "hipSetDevice") &&
+      ok(hipMalloc(&din, count * kInputs * sizeof(float)), error, error_size, "allocate inputs") &&
+      ok(hipMalloc(&dout, count * kOutputs * sizeof(float)), error, error_size, "allocate outputs") &&
+      ok(hipMemcpy(din, features, count * kInputs * sizeof(float), hipMemcpyHostToDevice), error, error_size,
+         "upload inputs") &&
+      [&] {
+        hipLaunchKernelGGL(forward_kernel, dim3((count + 63) / 64), dim3(64), 0, p->stream, view(p), din,
+                           uint32_t(count), dout);
+        return ok(hipGetLastError(), error, error_size, "forward launch");
+      }() &&
+      ok(hipStreamSynchronize(p->stream), error, error_size, "forward") &&
+      ok(hipMemcpy(out, dout, count * kOutputs * sizeof(float), hipMemcpyDeviceToHost), error, error_size,
+         "copy outputs");
+  (void)hipFree(din);
+  (void)hipFree(dout);
+  return good ? 0 : 1;
+}
+
+void monki_hip_proposal_close(void *handle) { release(static_cast<Proposal *>(handle)); }
+
+}  // extern "C"

@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include "monki_hip.h"
+#include "proposal.h"
 
 namespace {
 
@@ -41,9 +42,25 @@ enum FailureFlag : unsigned {
 
 enum Status : int { kRunning = 0, kEscaped = 1, kAbsorbed = 2, kFailed = 3 };
 
+// Transport modes: plain (the reference algorithm), logging of scattering events for the
+// proposal's training, and guided sampling with the learned proposal.
+enum Mode : int { kPlain = 0, kLog = 1, kGuided = 2 };
+
+// Which events the guided mode guides: scatterer kinds (bit 0 gas, bit s + 1 tabulated species
+// s) up to a scattering order; alpha = probability of the phase-function component.
+struct Guidance {
+  float alpha;
+  uint32_t species_mask;
+  uint32_t max_order;
+};
+
 struct SpeciesTables {
   int lut_size;
   float ssalb;
+  // Host-derived: trapezoid integral of F11 over cos / 2 (phase-function normalization) and
+  // the largest ratio, over node intervals, of the bound F11 (+|F12|) at a node to its
+  // interval mean (the constant that makes tabulated sampling exact).
+  float norm, rho;
   // cosine[0, n) ascending; cosine[n, 2n) = 1 - cos; cosine[2n, 3n) = 1 + cos.
   const float *cosine, *cdf, *f11, *f22, *f33, *f12;
 };
@@ -278,7 +295,12 @@ __device__ inline Stokes<R> scatter(const Elements<R> &f, const Rotation<R> &r, 
 // rayl_scatter_pol / mie_scatter_pol_bound, forward sign -1 (mc_routines_general_mod.f90:
 // 492-932): rejection sampling of (cos theta, sigma) from F11 + F12 (Q cos 2s - U sin 2s),
 // uniform cosine under the Rayleigh bound F11(1), inverse bound-CDF for tabulated species.
-template <typename R>
+//
+// Exact = true draws from the phase function itself: the reference accepts a tabulated draw
+// with probability pdf / bound(cos), under a proposal that is uniform in cos within each node
+// interval, so its density carries the factor (interval mean of the bound) / bound(cos). The
+// exact variant accepts with pdf / (rho * interval mean) instead, which removes that factor.
+template <typename R, bool Exact = false>
 __device__ inline bool sample_direction(const Scene &s, int species, const Stokes<R> &st, R uo,
                                         Stream &rng, R &u1, R &dphi, R &sampled,
                                         R &sampled_minus, R &sampled_plus, Counts &n) {
@@ -290,6 +312,7 @@ __device__ inline bool sample_direction(const Scene &s, int species, const Stoke
     // precision near both peaks: exact for Rayleigh (cos = 2u - 1), interpolated from the
     // 1 -/+ cos tables for tabulated species.
     R cosine, sin2, one_minus, one_plus;
+    R exact_bound = R(0);
     if (species < 0) {
       const float u = uniform(rng);
       cosine = R(2) * R(u) - R(1);
@@ -308,12 +331,18 @@ __device__ inline bool sample_direction(const Scene &s, int species, const Stoke
       cosine = R(t.cosine[low]) + w * (R(t.cosine[low + 1]) - R(t.cosine[low]));
       sin2 = one_minus * one_plus;
       n.lut += 8;
+      if (Exact) {
+        const R b0 = R(t.f11[low]) + (s.nmat > 1 ? fabs(R(t.f12[low])) : R(0));
+        const R b1 = R(t.f11[low + 1]) + (s.nmat > 1 ? fabs(R(t.f12[low + 1])) : R(0));
+        exact_bound = R(t.rho) * R(0.125) * (b0 + b1);  // rho x interval mean, in F/4 units
+      }
     }
     const R sigma = R(2) * pi * R(uniform(rng)) - pi;
     const Elements<R> f = elements<R>(s, species, cosine, one_minus, one_plus, n.lut);
     const R pdf = f.f11 + f.f12 * (st.q * cos(R(2) * sigma) - st.u * sin(R(2) * sigma));
-    const R bound =
-        species < 0 ? elements<R>(s, -1, R(1), R(0), R(2), n.lut).f11 : f.f11 + fabs(f.f12);
+    const R bound = species < 0 ? elements<R>(s, -1, R(1), R(0), R(2), n.lut).f11
+                    : Exact     ? exact_bound
+                                : f.f11 + fabs(f.f12);
     if (R(uniform(rng)) * bound > pdf) continue;
 
     sampled = cosine;
@@ -483,6 +512,8 @@ template <typename R> struct History {
   int species;  // -1 gas, else tabulated species of the pending event
   bool surface;
   R path_total;  // sum of the layer paths, for the per-photon records
+  float contribution;  // sum of the detector weights C1 of the history so far (all views)
+  uint32_t order;      // atmospheric events so far
 };
 
 // Local estimates of one event, parked in LDS until the wave's cooperative tally reads them.
@@ -538,6 +569,8 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
   h.remains = -log(R(uniform(h.rng)));
   h.steps = 0;
   h.path_total = R(0);
+  h.contribution = 0.0f;
+  h.order = 0;
 }
 
 // Advance one photon to its next collision or surface hit (mc.f90:476-677), apply the albedo
@@ -631,6 +664,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
     }
     h.weight *= species < 0 ? R(s.ssalb_gas[p.k]) : R(s.sp[species].ssalb);
     ++n.events;
+    ++h.order;
     if (TRACE) trace.events += R(1);
   }
   h.species = species;
@@ -686,6 +720,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
     const R ci = prob.i * attenuation;
     out.pixel(view) = hit.i + s.nx * hit.j;
     out.weight(view) = ci;
+    h.contribution += float(ci);
     out.q(view) = prob.q * attenuation * geometry;
     out.u(view) = prob.u * attenuation * geometry;
     if (TRACE) {
@@ -700,50 +735,129 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
 }
 
 // Continuation direction after the event (mc.f90:1266-1431; GROUND_LAMB2 at the surface).
-template <typename R, bool TRACE>
+// Guided (atmospheric events): with probability alpha the direction comes from the phase
+// function (exact sampler), otherwise from the learned vMF mixture; either way the weight is
+// multiplied by p / (alpha p + (1 - alpha) q), p the phase-function density of the direction
+// for the photon's Stokes vector and q the mixture's, which keeps the estimator unbiased and
+// the factor below 1 / alpha.
+// Events outside the guidance (kind or order) are sampled exactly as in the plain mode.
+template <typename R, bool TRACE, int MODE = kPlain>
 __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, Counts &n,
-                              Trace<R> &trace) {
+                              Trace<R> &trace, const monki_net::Mixture *mixture = nullptr,
+                              Guidance guidance = Guidance{1.0f, 0u, 0u}) {
+  const float alpha = guidance.alpha;
   const R pi = R(3.14159265358979323846);
   Photon<R> &p = h.p;
   R mu, phi;
+  R da = R(0), db = R(0), dc = R(0);
+  bool direct = false;  // direction cosines given (guided vMF branch)
   if (h.surface) {
     mu = sqrt(R(uniform(h.rng)));
     phi = R(2) * pi * R(uniform(h.rng));
     h.st = Stokes<R>{R(1), R(0), R(0)};
   } else {
-    R dphi, cosine, one_minus, one_plus;
-    if (!sample_direction<R>(s, h.species, h.st, p.c, h.rng, mu, dphi, cosine, one_minus, one_plus, n)) {
-      atomicOr(failure, kSamplingLimit);
-      return kFailed;
-    }
-    phi = p.phi + dphi;
-    // float32 keeps the azimuth in [-pi, pi]: every use is periodic, and an azimuth that
-    // random-walks away from zero loses the resolution the coplanarity test needs.
-    if (sizeof(R) == 4) phi = remainder(phi, R(2) * pi);
-    // The sampled scattering cosine is used directly, as in mc.f90.
-    if (s.nmat > 1) {
-      const Elements<R> f = elements<R>(s, h.species, cosine, one_minus, one_plus, n.lut);
-      const Rotation<R> r = rotation<R>(cosine, p.c, mu, p.phi, phi);
-      Stokes<R> st = scatter<R>(f, r, h.st);
-      const R norm = st.i;
-      st.i = R(1);
-      st.q /= norm;
-      st.u /= norm;
-      if (!finite3(st) || sqrt(st.q * st.q + st.u * st.u) > R(1.01)) {
-        atomicOr(failure, kInvalidStokes);
+    R cosine, one_minus, one_plus;
+    bool from_phase = true;
+    const bool guide = MODE == kGuided && ((guidance.species_mask >> (h.species + 1)) & 1u) &&
+                       h.order <= guidance.max_order;
+    if (guide) from_phase = uniform(h.rng) < alpha;
+    if (from_phase) {
+      R dphi;
+      const bool sampled =
+          guide
+              ? sample_direction<R, true>(s, h.species, h.st, p.c, h.rng, mu, dphi, cosine, one_minus, one_plus, n)
+              : sample_direction<R, false>(s, h.species, h.st, p.c, h.rng, mu, dphi, cosine, one_minus, one_plus, n);
+      if (!sampled) {
+        atomicOr(failure, kSamplingLimit);
         return kFailed;
       }
-      h.st = st;
+      phi = p.phi + dphi;
+      // float32 keeps the azimuth in [-pi, pi]: every use is periodic, and an azimuth that
+      // random-walks away from zero loses the resolution the coplanarity test needs.
+      if (sizeof(R) == 4) phi = remainder(phi, R(2) * pi);
+    } else {
+      float wx, wy, wz;
+      const float u0 = uniform(h.rng), u1 = uniform(h.rng), u2 = uniform(h.rng);
+      monki_net::mixture_sample(*mixture, u0, u1, u2, wx, wy, wz);
+      da = R(wx), db = R(wy), dc = R(wz);
+      direct = true;
+      mu = dc;
+      phi = atan2(db, da);
+      const R ex = p.a - da, ey = p.b - db, ez = p.c - dc;
+      const R fx = p.a + da, fy = p.b + db, fz = p.c + dc;
+      one_minus = R(0.5) * (ex * ex + ey * ey + ez * ez);
+      one_plus = R(0.5) * (fx * fx + fy * fy + fz * fz);
+      cosine = clamp1(p.a * da + p.b * db + p.c * dc);
+    }
+    // The sampled scattering cosine is used directly, as in mc.f90.
+    if (s.nmat > 1 || guide) {
+      const Elements<R> f = elements<R>(s, h.species, cosine, one_minus, one_plus, n.lut);
+      const Rotation<R> r = s.nmat > 1 ? rotation<R>(cosine, p.c, mu, p.phi, phi)
+                                       : Rotation<R>{R(1), R(0), R(1), R(0)};
+      Stokes<R> st = scatter<R>(f, r, h.st);
+      const R norm = st.i;
+      if (guide) {
+        if (!direct) {
+          const R sine = sqrt(fmax(R(0), R(1) - mu * mu));
+          da = cos(phi) * sine, db = sin(phi) * sine, dc = mu;
+        }
+        const float table_norm = h.species < 0 ? 1.0f : s.sp[h.species].norm;
+        const R density = norm / (pi * R(table_norm));
+        const R q = R(monki_net::mixture_density(*mixture, float(da), float(db), float(dc)));
+        h.weight *= density / (R(alpha) * density + R(1.0f - alpha) * q);
+      }
+      if (s.nmat > 1) {
+        st.i = R(1);
+        st.q /= norm;
+        st.u /= norm;
+        if (!finite3(st) || sqrt(st.q * st.q + st.u * st.u) > R(1.01)) {
+          atomicOr(failure, kInvalidStokes);
+          return kFailed;
+        }
+        h.st = st;
+      }
     }
     h.remains = -log(R(uniform(h.rng)));
   }
-  const R sine = sqrt(fmax(R(0), R(1) - mu * mu));
-  change_direction<R>(s, p, cos(phi) * sine, sin(phi) * sine, mu, phi);
+  if (direct) {
+    change_direction<R>(s, p, da, db, dc, phi);
+  } else {
+    const R sine = sqrt(fmax(R(0), R(1) - mu * mu));
+    change_direction<R>(s, p, cos(phi) * sine, sin(phi) * sine, mu, phi);
+  }
   if (TRACE) {
     trace.weight = h.weight;
     trace.layer = R(p.k);
   }
   return kRunning;
+}
+
+// Proposal inputs of an atmospheric event: periodic horizontal position (sin, cos of the
+// phase in the domain), height over the top of the grid, direction of travel, log extinction
+// of the cell, single-scattering albedo of the scatterer, scatterer kind (0 gas, else
+// (species + 1) / species count) and scattering order.
+template <typename R>
+__device__ inline void event_features(const Scene &s, const History<R> &h, float (&f)[monki_net::kInputs]) {
+  const Photon<R> &p = h.p;
+  const float two_pi = 6.28318530717958647692f;
+  const float top = s.height_above[0] + s.zunit[0];
+  const float x = float(p.i) * s.xunit + float(p.a >= R(0) ? R(s.xunit) - p.rx : p.rx);
+  const float y = float(p.j) * s.yunit + float(p.b >= R(0) ? R(s.yunit) - p.ry : p.ry);
+  const float bottom = top - s.height_above[p.k] - s.zunit[p.k];
+  const float z = bottom + float(p.c >= R(0) ? R(s.zunit[p.k]) - p.rz : p.rz);
+  const float px = two_pi * x / (float(s.nx) * s.xunit), py = two_pi * y / (float(s.ny) * s.yunit);
+  f[0] = sinf(px);
+  f[1] = cosf(px);
+  f[2] = sinf(py);
+  f[3] = cosf(py);
+  f[4] = 2.0f * z / top - 1.0f;
+  f[5] = float(p.a);
+  f[6] = float(p.b);
+  f[7] = float(p.c);
+  f[8] = log1pf(s.beta[cell_index<R>(s, p.i, p.j, p.k)]) / 3.0f;
+  f[9] = h.species < 0 ? s.ssalb_gas[p.k] : s.sp[h.species].ssalb;
+  f[10] = h.species < 0 ? 0.0f : float(h.species + 1) / float(s.species);
+  f[11] = fminf(float(h.order), 64.0f) / 32.0f - 1.0f;
 }
 
 template <typename R> __device__ inline R broadcast(R value, int lane);
@@ -828,13 +942,15 @@ __device__ inline void flush_counts(Counts &n, unsigned long long *counters) {
 // draws) do not depend on which lane runs them. Tally banks are chosen per wave.
 // Three waves per SIMD is what the LDS rows of a 256-lane block allow; the register budget is
 // capped to match (float32), which measured 1.35x faster than the compiler's own choice of
-// two waves per SIMD on the layered-cloud round.
-template <typename R, bool TRACE>
+// two waves per SIMD on the layered-cloud round. The guided mode evaluates the proposal network
+// for the wave's scattering events between the tally and the scatter phases (matrix cores,
+// all lanes together), and keeps two waves per SIMD for the network's registers.
+template <typename R, bool TRACE, int MODE>
 __global__ void __launch_bounds__(kBlock32)
-    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 1))) transport_kernel(Scene s, Controls c, R *banks,
-                                                             unsigned *failure, unsigned *work,
-                                                             unsigned long long *counters,
-                                                             float *records, uint32_t record_count) {
+    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? (MODE == kGuided ? 2 : 3) : 1)))
+    transport_kernel(Scene s, Controls c, R *banks, unsigned *failure, unsigned *work,
+                     unsigned long long *counters, float *records, uint32_t record_count,
+                     monki_net::Net net, monki_net::EventLog log, Guidance guidance) {
   extern __shared__ unsigned char lds_raw[];
   // Layer rows of odd length: lanes reading one row, or each its own row at one layer,
   // touch distinct LDS banks.
@@ -886,9 +1002,51 @@ __global__ void __launch_bounds__(kBlock32)
                   lane_layer, n);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    if (active && status == kRunning) status = scatter_photon<R, TRACE>(s, failure, h, n, trace);
+    const bool scatters = active && status == kRunning && !h.surface;
+    float features[monki_net::kInputs];
+    monki_net::Mixture mixture;
+    uint32_t slot = 0xffffffffu;
+    if (MODE != kPlain) {
+#pragma unroll
+      for (int k = 0; k < monki_net::kInputs; ++k) features[k] = 0.0f;
+      if (scatters) event_features<R>(s, h, features);
+    }
+    if (MODE == kGuided && __ballot(scatters)) {
+      float o[monki_net::kOutputs];
+#pragma unroll
+      for (int nb = 0; nb < 2; ++nb) {
+        monki_net::bf16x4 x[16];
+        monki_net::scatter_inputs(features, nb, x);
+        monki_net::floatx16 raw[2];
+        monki_net::forward_block(net, x, raw, nullptr);
+        monki_net::gather_outputs(raw, (__lane_id() >> 5) == nb, o);
+      }
+      if (scatters) mixture = monki_net::mixture_from(o);
+    }
+    if (MODE == kLog) {
+      const unsigned long long logging = __ballot(scatters);
+      if (logging) {
+        const int leader = __ffsll(static_cast<long long>(logging)) - 1;
+        uint32_t base = 0;
+        if (__lane_id() == leader) base = atomicAdd(log.count, uint32_t(__popcll(logging)));
+        base = __shfl(base, leader, 64);
+        if (scatters) slot = base + uint32_t(__popcll(logging & __lanemask_lt()));
+      }
+    }
+    if (active && status == kRunning)
+      status = scatter_photon<R, TRACE, MODE>(s, failure, h, n, trace, &mixture, guidance);
+    if (MODE == kLog && slot < log.capacity && status == kRunning) {
+#pragma unroll
+      for (int k = 0; k < monki_net::kInputs; ++k) log.features[size_t(slot) * monki_net::kInputs + k] = features[k];
+      log.directions[size_t(slot) * 3] = float(h.p.a);
+      log.directions[size_t(slot) * 3 + 1] = float(h.p.b);
+      log.directions[size_t(slot) * 3 + 2] = float(h.p.c);
+      log.cumulative[slot] = h.contribution;
+      log.photon[slot] = id;
+    }
     if (!active || status == kRunning) continue;
     active = false;
+    if (MODE == kLog && id < log.photons) log.totals[id] = h.contribution;
     if (TRACE && id < record_count) {
       float *out = records + size_t(id) * kTraceWords;
       out[0] = float(trace.events);
@@ -977,6 +1135,8 @@ struct Session {
   hipEvent_t start = nullptr, stop = nullptr;
   double gpu_seconds = 0.0;
   double kernel_seconds = 0.0;
+  void *proposal = nullptr;  // attached learned proposal (guided runs)
+  Guidance guidance{0.25f, ~0u, ~0u};  // guided runs: defensive fraction, kinds, orders
 };
 
 void set_error(char *error, size_t size, const char *fmt, const char *detail = "") {
@@ -1077,7 +1237,7 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
   }
   for (int sp = 0; sp < kMaxSpecies; ++sp) {
     SpeciesTables &tab = s.sp[sp];
-    tab = SpeciesTables{0, 0.0f, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    tab = SpeciesTables{0, 0.0f, 1.0f, 1.0f, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     if (sp >= s.species) continue;
     const int h = 32 + 8 * sp;
     if (!word_int(t, h, 2, 1 << 20, tab.lut_size)) {
@@ -1108,6 +1268,20 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
         return false;
       }
     }
+    const float *f11 = t + (tab.f11 - base), *f12 = t + (tab.f12 - base);
+    double area = 0.0, rho = 1.0;
+    for (int a = 1; a < tab.lut_size; ++a) {
+      area += 0.5 * (double(cs[a]) - cs[a - 1]) * (double(f11[a]) + f11[a - 1]);
+      const double b0 = double(f11[a - 1]) + (s.nmat > 1 ? fabs(double(f12[a - 1])) : 0.0);
+      const double b1 = double(f11[a]) + (s.nmat > 1 ? fabs(double(f12[a])) : 0.0);
+      if (b0 + b1 > 0.0) rho = fmax(rho, fmax(b0, b1) / (0.5 * (b0 + b1)));
+    }
+    if (!(area > 0.0)) {
+      set_error(error, size, "Phase function integrates to zero");
+      return false;
+    }
+    tab.norm = float(area / 2.0);
+    tab.rho = float(rho * (1.0 + 1e-6));
   }
   for (int v = 0; v < s.views; ++v) {
     if (!(cosf(s.view[v]) > 0.0f)) {
@@ -1122,9 +1296,9 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
   return true;
 }
 
-template <typename R, bool TRACE>
+template <typename R, bool TRACE, int MODE = kPlain>
 int launch(Session *h, const uint32_t *controls, double *out, float *records, size_t record_count,
-           char *error, size_t error_size) {
+           char *error, size_t error_size, void *proposal = nullptr) {
   if (!h || !controls || !out) {
     set_error(error, error_size, "Invalid HIP session or arguments");
     return 1;
@@ -1161,7 +1335,25 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
   const size_t lds = (size_t(s.nz | 1) + kPendingReals) * block * sizeof(R) + size_t(kPendingWords) * block * sizeof(int);
   // Persistent grid: enough resident blocks to fill every CU, never more than the work.
   int resident = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, TRACE>, block,
+  monki_net::Net net{nullptr, nullptr, nullptr};
+  monki_net::EventLog log{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0};
+  uint32_t logged_before = 0;
+  if (MODE != kPlain) {
+    if (!proposal || monki_net::proposal_device(proposal) != h->device) {
+      set_error(error, error_size, "A proposal on the session's device is required");
+      return 1;
+    }
+    net = monki_net::proposal_net(proposal);
+    if (MODE == kLog) {
+      log = monki_net::proposal_log(proposal);
+      if (!log.features || c.count > log.photons) {
+        set_error(error, error_size, "Reserve proposal samples for this dispatch first");
+        return 1;
+      }
+      logged_before = monki_net::proposal_samples(proposal);
+    }
+  }
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, TRACE, MODE>, block,
                                                   lds) != hipSuccess || resident < 1)
     resident = 1;
   const uint64_t wanted = (uint64_t(c.count) + block - 1) / block;
@@ -1170,9 +1362,9 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
   R *banks = static_cast<R *>(h->banks);
   (void)hipEventRecord(h->start, h->stream);
   if (grid) {
-    hipLaunchKernelGGL((transport_kernel<R, TRACE>), dim3(grid), dim3(block), lds, h->stream, s, c,
+    hipLaunchKernelGGL((transport_kernel<R, TRACE, MODE>), dim3(grid), dim3(block), lds, h->stream, s, c,
                        banks, h->failure, h->failure + 1, h->counters, TRACE ? h->records : nullptr,
-                       uint32_t(TRACE ? record_count : 0));
+                       uint32_t(TRACE ? record_count : 0), net, log, h->guidance);
   }
   (void)hipEventRecord(h->stop, h->stream);
   if (!check(hipGetLastError(), error, error_size, "transport launch")) return 1;
@@ -1200,6 +1392,8 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
   }
   if (!check(hipMemcpy(out, h->assembled, reduced * sizeof(double), hipMemcpyDeviceToHost), error,
              error_size, "copy tallies"))
+    return 1;
+  if (MODE == kLog && !monki_net::proposal_finish_log(proposal, logged_before, h->stream, error, error_size))
     return 1;
   if (TRACE && records && record_count) {
     const size_t n = record_count < c.count ? record_count : c.count;
@@ -1267,7 +1461,7 @@ int run_probe(uint32_t device, void (*kernel)(const T *, T *, int), const T *in,
 
 extern "C" {
 
-int monki_hip_abi_version(void) { return 1; }
+int monki_hip_abi_version(void) { return 2; }
 
 void *monki_hip_create(uint32_t device, const float *tables, size_t table_count, size_t tally_count,
                        uint32_t round_capacity, char *error, size_t error_size) {
@@ -1350,6 +1544,46 @@ int monki_hip_trace(void *handle, const uint32_t *controls, uint32_t precision, 
                                 record_count, error, error_size);
   set_error(error, error_size, "Trace precision must be 32 or 64");
   return 1;
+}
+
+int monki_hip_attach_proposal(void *handle, void *proposal, float alpha, uint32_t species_mask,
+                              uint32_t max_order, char *error, size_t error_size) {
+  Session *h = static_cast<Session *>(handle);
+  if (!h || !(alpha > 0.0f && alpha <= 1.0f)) {
+    set_error(error, error_size, "Invalid session or defensive fraction (0 < alpha <= 1)");
+    return 1;
+  }
+  if (proposal && monki_net::proposal_device(proposal) != h->device) {
+    set_error(error, error_size, "The proposal lives on another device");
+    return 1;
+  }
+  h->proposal = proposal;
+  h->guidance = Guidance{alpha, species_mask, max_order};
+  return 0;
+}
+
+int monki_hip_run_guided(void *handle, const uint32_t *controls, double *out, char *error,
+                         size_t error_size) {
+  Session *h = static_cast<Session *>(handle);
+  return launch<float, false, kGuided>(h, controls, out, nullptr, 0, error, error_size,
+                                       h ? h->proposal : nullptr);
+}
+
+int monki_hip_trace_guided(void *handle, const uint32_t *controls, double *out, float *records,
+                           size_t record_count, char *error, size_t error_size) {
+  Session *h = static_cast<Session *>(handle);
+  if (!records || !record_count) {
+    set_error(error, error_size, "Trace needs a record buffer");
+    return 1;
+  }
+  return launch<float, true, kGuided>(h, controls, out, records, record_count, error, error_size,
+                                      h ? h->proposal : nullptr);
+}
+
+int monki_hip_run_logged(void *handle, const uint32_t *controls, double *out, void *proposal,
+                         char *error, size_t error_size) {
+  return launch<float, false, kLog>(static_cast<Session *>(handle), controls, out, nullptr, 0, error,
+                                    error_size, proposal);
 }
 
 int monki_hip_counters(void *handle, uint64_t *out, size_t count) {

@@ -12,6 +12,7 @@ time for gfx950. There is no CPU transport fallback: without the library or a GP
 import ctypes
 import hashlib
 import platform
+import time
 from dataclasses import replace
 from pathlib import Path
 from typing import List, Optional
@@ -22,11 +23,14 @@ from ..input import ReferenceCase, SceneCase
 from ..metadata import file_hash, scene_metadata
 from .backend import Backend
 from .hip_tables import pack_tables, tally_count
+from .proposal import Proposal, ProposalSettings, declare, log_rounds
 from .tallies import BatchControls, RawTallies
 
 PACKAGE_ROOT = Path(__file__).resolve().parents[2]
 LIBRARY = PACKAGE_ROOT / "lib" / "libmonki_hip.so"
-ABI_VERSION = 1
+ABI_VERSION = 2
+# Seed offset of the proposal's training rounds: their streams never meet the estimation's.
+TRAINING_SEED_OFFSET = 0x6A09E667
 TRACE_WORDS = 8
 COUNTER_NAMES = (
     "photons",
@@ -96,6 +100,7 @@ def load_library(path: Path = LIBRARY):
     native.monki_hip_device_name.restype = ctypes.c_char_p
     native.monki_hip_close.argtypes = [ctypes.c_void_p]
     native.monki_hip_close.restype = None
+    declare(native)
     return native
 
 
@@ -105,6 +110,11 @@ class HIP(Backend):
     Transport is float32 (``precision=64`` runs the same kernels in float64 on the same
     random streams); per-round shard reduction is float64 on the device; cross-round
     statistics are float64 on the host. One caller per instance.
+
+    With ``proposal=ProposalSettings(...)`` the backend is the guided tracer: ``prepare`` logs
+    scattering events of plain rounds on their own random streams, trains the proposal network
+    on the device, and every estimation round then samples directions from the learned
+    mixture (with the phase function as defensive component) and reweights.
     """
 
     def __init__(
@@ -119,6 +129,7 @@ class HIP(Backend):
         max_steps: int = 100_000,
         precision: int = 32,
         library: Path = LIBRARY,
+        proposal: Optional[ProposalSettings] = None,
     ):
         for name, value, minimum, maximum in (
             ("device", device, 0, 2**32 - 1),
@@ -137,6 +148,12 @@ class HIP(Backend):
         if precision not in (32, 64) or type(precision) is not int:
             raise ValueError("precision must be 32 or 64")
 
+        if proposal is not None and not isinstance(proposal, ProposalSettings):
+            raise ValueError("proposal must be ProposalSettings or None")
+
+        if proposal is not None and precision != 32:
+            raise ValueError("the guided tracer runs in float32")
+
         self.device = device
         self.save_path = save_path
         self.profile = profile
@@ -146,6 +163,7 @@ class HIP(Backend):
         self.max_steps = max_steps
         self.precision = precision
         self.library = Path(library)
+        self.proposal = proposal
         self.metadata = None
         self.provenance: dict = {}
         self._native = None
@@ -156,6 +174,7 @@ class HIP(Backend):
         self._tally_count = 0
         self._round_capacity = 1
         self._packed: Optional[np.ndarray] = None
+        self._network: Optional[Proposal] = None
 
     # -- lifecycle ----------------------------------------------------------------------
 
@@ -230,6 +249,13 @@ class HIP(Backend):
             "gpu_dispatches": 0,
         }
 
+        if self.proposal is not None:
+            try:
+                self._train_proposal(nx * ny)
+            except BaseException:
+                self.close()
+                raise
+
     def run_batch(self, controls: object) -> RawTallies:
         """Execute exactly one complete statistical round."""
 
@@ -265,6 +291,10 @@ class HIP(Backend):
         if self._handle is not None and self._native is not None:
             self._native.monki_hip_close(self._handle)
 
+        if self._network is not None:
+            self._network.close()
+
+        self._network = None
         self._handle = None
         self._native = None
         self.metadata = None
@@ -324,7 +354,68 @@ class HIP(Backend):
 
         return out, buffer[: min(records, count)]
 
+    @property
+    def network(self) -> Optional[Proposal]:
+        """The trained proposal of a guided session (None for the plain tracer)."""
+
+        return self._network
+
+    def trace_guided(self, controls: BatchControls, records: int):
+        """Guided counterpart of ``trace`` (records of the same layout)."""
+
+        self._require_session()
+
+        if self._network is None:
+            raise RuntimeError("trace_guided needs a guided session")
+
+        _, nx, ny, _, _ = self._shape
+        per_round = controls.photons_per_column * nx * ny
+        command = (ctypes.c_uint32 * 8)(per_round, 0, controls.round_index, self.seed, self.max_steps, 0, per_round, 1)
+        out = np.zeros(self._tally_count, dtype=np.float64)
+        buffer = np.zeros((records, TRACE_WORDS), dtype=np.float32)
+        error = ctypes.create_string_buffer(8192)
+        status = self._native.monki_hip_trace_guided(
+            self._handle, command, out.ctypes.data_as(DOUBLE_POINTER), buffer.ctypes.data_as(FLOAT_POINTER),
+            records, error, len(error),
+        )  # fmt: skip
+
+        if status:
+            raise RuntimeError(error.value.decode())
+
+        return out, buffer[: min(records, per_round)]
+
     # -- internals ----------------------------------------------------------------------
+
+    def _train_proposal(self, columns: int) -> None:
+        settings = self.proposal
+        start = time.perf_counter()
+        network = Proposal(self._native, self.device, settings.seed)
+        self._network = network
+        logged = log_rounds(
+            self._native, self._handle, network, settings, columns,
+            (self.seed + TRAINING_SEED_OFFSET + settings.seed) % 2**32, self.max_steps,
+            self.photons_per_dispatch, self._tally_count,
+        )  # fmt: skip
+        logging_seconds = time.perf_counter() - start
+        losses = network.train(settings.steps, settings.batch, settings.learning_rate, settings.seed)
+        error = ctypes.create_string_buffer(4096)
+
+        if self._native.monki_hip_attach_proposal(
+            self._handle, network.handle, settings.alpha, settings.species_mask(), min(settings.max_order, 2**32 - 1),
+            error, len(error),
+        ):  # fmt: skip
+            raise RuntimeError(error.value.decode())
+
+        self.provenance["proposal"] = {
+            "settings": {key: getattr(settings, key) for key in settings.__dataclass_fields__},
+            "logging_rounds": logged["rounds"],
+            "training_events": logged["events"],
+            "logging_seconds": logging_seconds,
+            "training_seconds": time.perf_counter() - start - logging_seconds,
+            "loss_first": float(losses[0]) if len(losses) else None,
+            "loss_last": float(np.mean(losses[-50:])) if len(losses) else None,
+        }
+        self.provenance["method"] = "guided (learned vMF-mixture proposal, defensive phase function)"
 
     def _require_session(self):
         if self._handle is None or self._native is None:
@@ -359,6 +450,9 @@ class HIP(Backend):
         scratch = np.empty_like(total)
         error = ctypes.create_string_buffer(8192)
         run = self._native.monki_hip_run if self.precision == 32 else self._native.monki_hip_run_fp64
+
+        if self._network is not None:
+            run = self._native.monki_hip_run_guided
 
         try:
             for first in range(0, photon_count, self.photons_per_dispatch):
@@ -418,4 +512,5 @@ class HIP(Backend):
             self.profile,
             self.tally_shards,
             self.precision,
+            self.proposal,
         )

@@ -1,0 +1,274 @@
+"""Learned scattering-direction proposal for the guided HIP tracer.
+
+No reference counterpart: the reference samples every scattering direction from the phase
+function (``mc_routines_general_mod.f90:492-932``). Here a BF16 MLP (12 inputs: periodic
+position, height, direction, log extinction, single-scattering albedo, scatterer kind and
+scattering order; 4 hidden layers of 128 SiLU units) maps the state of a scattering event to
+an 8-component von Mises-Fisher mixture q over the outgoing direction. The guided tracer
+draws the direction from the phase function p with probability ``alpha`` and from q
+otherwise, and multiplies the photon weight by p / (alpha p + (1 - alpha) q): the estimator
+stays unbiased for any network, and the factor is at most 1 / alpha.
+
+Training data are scattering events of the plain tracer on their own random streams (seed
+distinct from the estimation rounds), weighted by the detector weight the photon contributed
+after the event; the loss is the weighted negative log-likelihood of the observed direction.
+Forward and backward passes run in the library's own matrix-core kernels.
+"""
+
+import ctypes
+from dataclasses import dataclass
+from typing import Optional
+
+import numpy as np
+
+PARAMETERS = 56360
+INPUTS = 12
+OUTPUTS = 40
+_F = ctypes.POINTER(ctypes.c_float)
+_D = ctypes.POINTER(ctypes.c_double)
+_U = ctypes.POINTER(ctypes.c_uint32)
+_U64 = ctypes.POINTER(ctypes.c_uint64)
+
+
+@dataclass(frozen=True)
+class ProposalSettings:
+    """How the guided tracer's proposal is trained before the estimation rounds.
+
+    events: scattering events to log from the plain tracer (training samples).
+    photons_per_column: photons per column of each logging round.
+    steps, batch, learning_rate: Adam steps over random minibatches of the samples.
+    alpha: probability of drawing the direction from the phase function (0 < alpha <= 1).
+    guided_kinds: scatterer kinds whose events are guided: "gas" and/or tabulated species
+        indices (0-based); None guides every kind.
+    max_order: events up to this scattering order are guided; later ones use the phase
+        function only (the weight factors of a history multiply, so bounding the guided orders
+        bounds the weight at (1 / alpha) ** max_order).
+    seed: network initialization and the logging rounds' random streams.
+    """
+
+    events: int = 4_000_000
+    photons_per_column: int = 2_000
+    steps: int = 1_500
+    batch: int = 16_384
+    learning_rate: float = 1e-3
+    alpha: float = 0.25
+    guided_kinds: Optional[tuple] = None
+    max_order: int = 2**31
+    seed: int = 97
+
+    def species_mask(self) -> int:
+        if self.guided_kinds is None:
+            return 0xFFFFFFFF
+
+        mask = 0
+
+        for kind in self.guided_kinds:
+            mask |= 1 if kind == "gas" else 1 << (int(kind) + 1)
+
+        return mask
+
+    def __post_init__(self):
+        for name, value, low, high in (
+            ("events", self.events, 64, 1 << 31),
+            ("photons_per_column", self.photons_per_column, 1, 1 << 24),
+            ("steps", self.steps, 0, 1 << 24),
+            ("batch", self.batch, 64, 1 << 24),
+            ("seed", self.seed, 0, 2**32 - 1),
+            ("max_order", self.max_order, 1, 2**32 - 1),
+        ):
+            if type(value) is not int or not low <= value <= high:
+                raise ValueError(f"{name} must be an integer in [{low}, {high}]")
+
+        if not 0.0 < float(self.learning_rate) < 1.0:
+            raise ValueError("learning_rate must be in (0, 1)")
+
+        if not 0.0 < float(self.alpha) <= 1.0:
+            raise ValueError("alpha must be in (0, 1]")
+
+        if self.guided_kinds is not None:
+            for kind in self.guided_kinds:
+                if kind != "gas" and not (type(kind) is int and 0 <= kind < 4):
+                    raise ValueError("guided_kinds holds 'gas' or tabulated species indices 0-3")
+
+
+def declare(native) -> None:
+    """ctypes declarations of the proposal entry points of libmonki_hip.so."""
+
+    native.monki_hip_proposal_parameter_count.restype = ctypes.c_size_t
+    native.monki_hip_proposal_create.argtypes = [ctypes.c_uint32, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]
+    native.monki_hip_proposal_create.restype = ctypes.c_void_p
+    native.monki_hip_proposal_set_parameters.argtypes = [ctypes.c_void_p, _F, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]
+    native.monki_hip_proposal_get_parameters.argtypes = [ctypes.c_void_p, _F, ctypes.c_size_t]
+    native.monki_hip_proposal_reserve.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]
+    native.monki_hip_proposal_set_samples.argtypes = [ctypes.c_void_p, _F, _F, _F, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]
+    native.monki_hip_proposal_samples.argtypes = [ctypes.c_void_p, _U64, _D]
+    native.monki_hip_proposal_read_samples.argtypes = [ctypes.c_void_p, _F, _F, _F, ctypes.c_size_t]
+    native.monki_hip_proposal_gradients.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _F, _D, ctypes.c_char_p, ctypes.c_size_t]
+    native.monki_hip_proposal_train.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint64, _D, ctypes.c_char_p, ctypes.c_size_t]
+    native.monki_hip_proposal_forward.argtypes = [ctypes.c_void_p, _F, ctypes.c_size_t, _F, ctypes.c_char_p, ctypes.c_size_t]
+    native.monki_hip_proposal_close.argtypes = [ctypes.c_void_p]
+    native.monki_hip_proposal_close.restype = None
+    native.monki_hip_attach_proposal.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]
+    native.monki_hip_run_guided.argtypes = [ctypes.c_void_p, _U, _D, ctypes.c_char_p, ctypes.c_size_t]
+    native.monki_hip_trace_guided.argtypes = [ctypes.c_void_p, _U, _D, _F, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]
+    native.monki_hip_run_logged.argtypes = [ctypes.c_void_p, _U, _D, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+
+    for name in (
+        "monki_hip_proposal_set_parameters", "monki_hip_proposal_get_parameters", "monki_hip_proposal_reserve",
+        "monki_hip_proposal_set_samples", "monki_hip_proposal_samples", "monki_hip_proposal_read_samples",
+        "monki_hip_proposal_gradients", "monki_hip_proposal_train", "monki_hip_proposal_forward",
+        "monki_hip_attach_proposal", "monki_hip_run_guided", "monki_hip_trace_guided", "monki_hip_run_logged",
+    ):  # fmt: skip
+        getattr(native, name).restype = ctypes.c_int
+
+
+class Proposal:
+    """A proposal network resident on one device (parameters, Adam state, samples)."""
+
+    def __init__(self, native, device: int = 0, seed: int = 97):
+        declare(native)
+
+        if native.monki_hip_proposal_parameter_count() != PARAMETERS:
+            raise RuntimeError("Rebuild libmonki_hip.so: proposal parameter count differs")
+
+        error = ctypes.create_string_buffer(4096)
+        handle = native.monki_hip_proposal_create(device, seed, error, len(error))
+
+        if not handle:
+            raise RuntimeError(error.value.decode() or "proposal creation failed")
+
+        self.native = native
+        self.handle = handle
+        self.device = device
+
+    def _check(self, status: int, error) -> None:
+        if status:
+            raise RuntimeError(error.value.decode() or "proposal call failed")
+
+    @property
+    def parameters(self) -> np.ndarray:
+        out = np.zeros(PARAMETERS, dtype=np.float32)
+        self._check(self.native.monki_hip_proposal_get_parameters(self.handle, out.ctypes.data_as(_F), out.size), None)
+        return out
+
+    @parameters.setter
+    def parameters(self, values) -> None:
+        values = np.ascontiguousarray(values, dtype=np.float32)
+        error = ctypes.create_string_buffer(4096)
+        self._check(
+            self.native.monki_hip_proposal_set_parameters(self.handle, values.ctypes.data_as(_F), values.size, error, len(error)),
+            error,
+        )  # fmt: skip
+
+    def reserve(self, events: int, photons: int) -> None:
+        error = ctypes.create_string_buffer(4096)
+        self._check(self.native.monki_hip_proposal_reserve(self.handle, events, photons, error, len(error)), error)
+
+    def set_samples(self, features, directions, weights) -> None:
+        features = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, INPUTS)
+        directions = np.ascontiguousarray(directions, dtype=np.float32).reshape(-1, 3)
+        weights = np.ascontiguousarray(weights, dtype=np.float32).ravel()
+        error = ctypes.create_string_buffer(4096)
+        self._check(
+            self.native.monki_hip_proposal_set_samples(
+                self.handle, features.ctypes.data_as(_F), directions.ctypes.data_as(_F),
+                weights.ctypes.data_as(_F), len(weights), error, len(error),
+            ),
+            error,
+        )  # fmt: skip
+
+    def samples(self):
+        """(sample count, sum of weights)."""
+
+        count = ctypes.c_uint64()
+        total = ctypes.c_double()
+        self._check(self.native.monki_hip_proposal_samples(self.handle, ctypes.byref(count), ctypes.byref(total)), None)
+        return int(count.value), float(total.value)
+
+    def read_samples(self, count: Optional[int] = None):
+        available, _ = self.samples()
+        count = available if count is None else count
+        features = np.zeros((count, INPUTS), dtype=np.float32)
+        directions = np.zeros((count, 3), dtype=np.float32)
+        weights = np.zeros(count, dtype=np.float32)
+        self._check(
+            self.native.monki_hip_proposal_read_samples(
+                self.handle, features.ctypes.data_as(_F), directions.ctypes.data_as(_F),
+                weights.ctypes.data_as(_F), count,
+            ),
+            None,
+        )  # fmt: skip
+        return features, directions, weights
+
+    def gradients(self, count: int):
+        """Gradient of -sum w log q / sum w over the first ``count`` samples, and that loss."""
+
+        grad = np.zeros(PARAMETERS, dtype=np.float32)
+        loss = ctypes.c_double()
+        error = ctypes.create_string_buffer(4096)
+        self._check(
+            self.native.monki_hip_proposal_gradients(self.handle, count, grad.ctypes.data_as(_F), ctypes.byref(loss), error, len(error)),
+            error,
+        )  # fmt: skip
+        return grad, float(loss.value)
+
+    def train(self, steps: int, batch: int, learning_rate: float, seed: int = 1) -> np.ndarray:
+        losses = np.zeros(max(steps, 1), dtype=np.float64)
+        error = ctypes.create_string_buffer(4096)
+        self._check(
+            self.native.monki_hip_proposal_train(
+                self.handle, steps, batch, learning_rate, seed, losses.ctypes.data_as(_D), error, len(error)
+            ),
+            error,
+        )  # fmt: skip
+        return losses[:steps]
+
+    def forward(self, features) -> np.ndarray:
+        features = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, INPUTS)
+        out = np.zeros((len(features), OUTPUTS), dtype=np.float32)
+        error = ctypes.create_string_buffer(4096)
+        self._check(
+            self.native.monki_hip_proposal_forward(
+                self.handle, features.ctypes.data_as(_F), len(features), out.ctypes.data_as(_F), error, len(error)
+            ),
+            error,
+        )  # fmt: skip
+        return out
+
+    def close(self) -> None:
+        if self.handle is not None:
+            self.native.monki_hip_proposal_close(self.handle)
+
+        self.handle = None
+
+
+def log_rounds(native, session, proposal: Proposal, settings: ProposalSettings, columns: int, seed: int,
+               max_steps: int, photons_per_dispatch: int, tally_count: int) -> dict:
+    """Run plain logging rounds (their own seed) until ``settings.events`` samples exist."""
+
+    per_round = settings.photons_per_column * columns
+    dispatch = min(per_round, photons_per_dispatch)
+    proposal.reserve(settings.events, dispatch)
+    out = np.zeros(tally_count, dtype=np.float64)
+    error = ctypes.create_string_buffer(8192)
+    round_index = 0
+    events = 0
+
+    while events < settings.events and round_index < 4096:
+        round_index += 1
+
+        for first in range(0, per_round, dispatch):
+            count = min(dispatch, per_round - first)
+            command = (ctypes.c_uint32 * 8)(count, first, round_index, seed, max_steps, 0, per_round, 1)
+            status = native.monki_hip_run_logged(session, command, out.ctypes.data_as(_D), proposal.handle, error, len(error))
+
+            if status:
+                raise RuntimeError(error.value.decode() or "logging round failed")
+
+            events, _ = proposal.samples()
+
+            if events >= settings.events:
+                break
+
+    return {"rounds": round_index, "events": events, "weight_sum": proposal.samples()[1]}
+

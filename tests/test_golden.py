@@ -311,3 +311,46 @@ def test_oracle_matches_reference_fortran_statistically(oracle, name):
     print(name, stats)
     assert stats["max_abs_z"] < 5.5
     assert 0.75 < stats["chi_square_per_dof"] < 1.35
+
+
+# ---- Learned proposal: the float64 restatement of the network ------------------------------
+
+
+def test_proposal_oracle_gradient_matches_finite_differences():
+    """The oracle's analytic gradient (the checker of the device backward pass) against central
+    differences of its own loss: a dropped term, a sign or a transposed factor fails."""
+
+    from oracle import proposal as net
+
+    rng = np.random.default_rng(3)
+    params = net.initial_parameters(5) + rng.normal(0, 0.05, net.PARAMETERS)
+    features = rng.uniform(-1, 1, (6, net.INPUTS))
+    directions = rng.normal(size=(6, 3))
+    directions /= np.linalg.norm(directions, axis=1, keepdims=True)
+    weights = rng.uniform(0.2, 2.0, 6)
+    loss, grad = net.loss_and_gradient(params, features, directions, weights)
+    picks = np.concatenate([rng.choice(net.PARAMETERS, 150, replace=False),
+                            np.arange(net.PARAMETERS - 40, net.PARAMETERS)])  # fmt: skip
+
+    for index in picks:
+        step = np.zeros(net.PARAMETERS)
+        step[index] = 1e-6
+        up = net.loss_and_gradient(params + step, features, directions, weights)[0]
+        down = net.loss_and_gradient(params - step, features, directions, weights)[0]
+        assert grad[index] == pytest.approx((up - down) / 2e-6, rel=1e-4, abs=1e-9), index
+
+
+def test_proposal_oracle_density_is_normalized():
+    """q integrates to one over the sphere (vMF normalization kappa / (4 pi sinh kappa))."""
+
+    from oracle import proposal as net
+
+    rng = np.random.default_rng(4)
+    outputs = rng.normal(0, 1.5, (1, net.OUTPUTS))
+    outputs[0, 3::5] = [-3.0, 0.0, 1.0, 2.0, 3.0, 4.0, 5.0, 6.0]  # kappa from 0.05 to ~6
+    cos_theta, phi = np.meshgrid(np.linspace(-1, 1, 801), np.linspace(0, 2 * np.pi, 801), indexing="ij")
+    s = np.sqrt(1 - cos_theta**2)
+    w = np.stack([s * np.cos(phi), s * np.sin(phi), cos_theta], axis=-1).reshape(-1, 3)
+    q = np.exp(net.log_density(np.repeat(outputs, len(w), axis=0), w)).reshape(cos_theta.shape)
+    total = np.trapezoid(np.trapezoid(q, phi[0], axis=1), cos_theta[:, 0])
+    assert total == pytest.approx(1.0, rel=2e-3)

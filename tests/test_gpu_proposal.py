@@ -1,0 +1,166 @@
+"""GPU parity of the learned proposal: matrix-core forward and backward kernels against the
+float64 oracle (oracle/proposal.py), training on a known distribution, and the guided tracer
+against the plain one (unbiasedness within combined standard errors)."""
+
+import json
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def network():
+    from monki.execution.hip import load_library
+    from monki.execution.proposal import Proposal
+
+    proposal = Proposal(load_library(), 0, seed=11)
+    yield proposal
+    proposal.close()
+
+
+def random_samples(count, seed):
+    rng = np.random.default_rng(seed)
+    features = rng.uniform(-1, 1, (count, 12)).astype(np.float32)
+    directions = rng.normal(size=(count, 3))
+    directions /= np.linalg.norm(directions, axis=1, keepdims=True)
+    weights = rng.uniform(0.1, 2.0, count)
+    return features, directions.astype(np.float32), weights.astype(np.float32)
+
+
+def perturbed_parameters(seed):
+    from oracle import proposal as net
+
+    rng = np.random.default_rng(seed)
+    return (net.initial_parameters(seed) + rng.normal(0, 0.05, net.PARAMETERS)).astype(np.float32)
+
+
+def test_forward_matches_oracle(network):
+    """Checks the MFMA operand and accumulator lane layouts: the device outputs equal the
+    oracle's bf16-emulated forward pass up to float32 accumulation order."""
+
+    from oracle import proposal as net
+
+    params = perturbed_parameters(1)
+    network.parameters = params
+    np.testing.assert_array_equal(network.parameters, params)
+    features, _, _ = random_samples(1000, 2)
+    gpu = network.forward(features)
+    cpu = net.forward(params.astype(np.float64), features, bf16_compute=True)
+    scale = np.abs(cpu).mean()
+    error = np.abs(gpu - cpu) / scale
+    print(json.dumps({"median": float(np.median(error)), "max": float(error.max())}))
+    assert np.median(error) < 1e-3
+    assert error.max() < 3e-2
+    # The full float64 forward pass (no bf16) differs only by bf16 rounding.
+    exact = net.forward(params.astype(np.float64), features)
+    assert np.median(np.abs(gpu - exact) / scale) < 2e-2
+
+
+def test_gradients_match_oracle(network):
+    from oracle import proposal as net
+
+    params = perturbed_parameters(3)
+    network.parameters = params
+    features, directions, weights = random_samples(1000, 4)
+    network.set_samples(features, directions, weights)
+    gpu, gpu_loss = network.gradients(1000)
+    loss, cpu = net.loss_and_gradient(params.astype(np.float64), features, directions, weights)
+    assert gpu_loss == pytest.approx(loss, rel=5e-3)
+    report = {}
+
+    for layer, (rows, cols) in enumerate(net.SHAPES):
+        start = sum(r * c + r for r, c in net.SHAPES[:layer])
+        for name, part in (("W", slice(start, start + rows * cols)), ("b", slice(start + rows * cols, start + rows * cols + rows))):
+            a, b = gpu[part].astype(np.float64), cpu[part]
+            relative = np.linalg.norm(a - b) / np.linalg.norm(b)
+            cosine = a @ b / (np.linalg.norm(a) * np.linalg.norm(b))
+            report[f"{name}{layer}"] = [round(float(relative), 5), round(float(cosine), 6)]
+            # bf16 operands (8-bit mantissa) in the forward and backward products.
+            assert relative < 0.05, (name, layer, relative)
+            assert cosine > 0.998, (name, layer, cosine)
+
+    print(json.dumps(report))
+
+
+def test_training_learns_a_known_direction_distribution(network):
+    """Samples whose direction is vMF(kappa 20) around the input's direction features: the
+    trained network's log-likelihood on held-out samples approaches the entropy bound."""
+
+    from oracle import proposal as net
+
+    rng = np.random.default_rng(5)
+    count = 200_000
+    features = rng.uniform(-1, 1, (count, 12)).astype(np.float32)
+    mean = features[:, 5:8].astype(np.float64)
+    mean /= np.linalg.norm(mean, axis=1, keepdims=True)
+    directions = net.sample_vmf(mean, 20.0, rng).astype(np.float32)
+    network.parameters = net.initial_parameters(7).astype(np.float32)
+    network.set_samples(features[:150_000], directions[:150_000], np.ones(150_000, np.float32))
+    before = net.log_density(net.forward(network.parameters, features[150_000:], True), directions[150_000:]).mean()
+    losses = network.train(600, 4096, 2e-3, seed=3)
+    after = net.log_density(net.forward(network.parameters, features[150_000:], True), directions[150_000:]).mean()
+    bound = np.log(20 / (2 * np.pi * (1 - np.exp(-40)))) + 20 * (1 / np.tanh(20) - 1 / 20 - 1)
+    print(json.dumps({"before": float(before), "after": float(after), "bound": float(bound),
+                      "loss_first": float(losses[0]), "loss_last": float(losses[-20:].mean())}))  # fmt: skip
+    assert np.all(np.isfinite(losses))
+    assert after > before + 1.5
+    assert after > bound - 0.5
+
+
+def small_box():
+    from monki import configs
+
+    return configs.box_cloud(nx=8, ny=6, nmat=3, box=(3, 2), offset=(2, 1))
+
+
+def test_logged_samples_are_well_formed():
+    from monki import SceneCase
+    from monki.execution import HIP
+    from monki.execution.proposal import ProposalSettings
+
+    settings = ProposalSettings(events=100_000, photons_per_column=300, steps=1, batch=1024)
+
+    with HIP(proposal=settings) as backend:
+        backend.prepare(SceneCase("box", small_box(), 100, 4))
+        features, directions, weights = backend.network.read_samples()
+        info = backend.provenance["proposal"]
+
+    assert info["training_events"] == len(weights) == 100_000
+    assert np.all(np.isfinite(features)) and np.all(np.abs(features) <= 1.0 + 1e-6)
+    np.testing.assert_allclose(np.linalg.norm(directions, axis=1), 1.0, atol=1e-5)
+    assert np.all(weights >= 0) and weights.sum() > 0
+    # Scatterer kinds: gas 0, cloud (species 1 of 1) 1.
+    assert set(np.unique(features[:, 10])) <= {0.0, 1.0}
+    # Orders start at the first event: (1 / 32) - 1.
+    assert features[:, 11].min() == pytest.approx(1 / 32 - 1)
+
+
+def test_guided_tracer_agrees_with_plain_tracer(tmp_path):
+    """Unbiasedness: guided and plain radiances and layer paths agree within combined SE."""
+
+    from monki import AMFConvergence, SceneCase, simulate
+    from monki.execution import HIP
+    from monki.execution.proposal import ProposalSettings
+    from monki.validation import combined_se_statistics
+
+    scene = small_box()
+    rounds, ppc = 48, 200
+    settings = ProposalSettings(events=400_000, photons_per_column=400, steps=300, batch=8192, alpha=0.3)
+    case = SceneCase("box", scene, ppc, rounds)
+    # A precision target no run reaches: every round runs and the AMF errors are reported.
+    policy = AMFConvergence(1e-9, rounds, 16, 3)
+    plain = simulate(case, HIP(seed=5), tmp_path / "plain.nc", convergence=policy)
+    guided = simulate(case, HIP(seed=6, proposal=settings), tmp_path / "guided.nc", convergence=policy)
+    report = {}
+
+    for name in ("radiance", "AMF_layer"):
+        a, se_a = plain.values[name], plain.values["std_" + name]
+        b, se_b = guided.values[name], guided.values["std_" + name]
+        report[name] = combined_se_statistics(a, se_a, b, se_b, floor=1e-12)
+
+    print(json.dumps(report))
+    for name, stats in report.items():
+        assert stats["max_abs_z"] < 5.5, name
+        assert 0.6 < stats["chi_square_per_dof"] < 1.5, name

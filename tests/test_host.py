@@ -427,7 +427,9 @@ def test_library_exports_every_declared_symbol():
     for name in symbols:
         assert hasattr(native, name), name
 
-    assert native.monki_hip_abi_version() == 1
+    from monki.execution.hip import ABI_VERSION
+
+    assert native.monki_hip_abi_version() == ABI_VERSION == 2
 
 
 def test_library_binding_declares_every_symbol():

@@ -85,7 +85,8 @@ def cpu_baseline(scene, events_per_photon: float, target_seconds: float = 15.0) 
     }
 
 
-def time_to_amf_se(scene, max_rounds: int = 1024) -> dict:
+def time_to_amf_se(scene, max_rounds: int = 1024, photons_per_column: int = PHOTONS_PER_COLUMN,
+                   name: str = "layered-cloud", proposal=None) -> dict:
     """Wall time of simulate(..., convergence=AMFConvergence(0.01, 128, 32, 3)) to converged."""
 
     import tempfile
@@ -98,7 +99,7 @@ def time_to_amf_se(scene, max_rounds: int = 1024) -> dict:
         """Delegates to HIP and prints one line per group of rounds (the run takes minutes)."""
 
         def __init__(self):
-            self.inner = HIP()
+            self.inner = HIP(proposal=proposal)
             self.rounds = 0
             self.start = time.perf_counter()
 
@@ -126,24 +127,60 @@ def time_to_amf_se(scene, max_rounds: int = 1024) -> dict:
             self.inner.close()
 
     policy = AMFConvergence(0.01, 128, 32, 3)
-    case = SceneCase("layered-cloud", scene, PHOTONS_PER_COLUMN, max_rounds)
+    case = SceneCase(name, scene, photons_per_column, max_rounds)
 
     with tempfile.TemporaryDirectory() as directory:
         start = time.perf_counter()
-        simulate(case, Progress(), Path(directory) / "amf.nc", convergence=policy)
+        result = simulate(case, Progress(), Path(directory) / "amf.nc", convergence=policy)
         elapsed = time.perf_counter() - start
         receipt = json.loads((Path(directory) / "amf.run.json").read_text())
 
-    checks = receipt.get("checks") or []
-    worst = checks[-1]["max_relative_se"] if checks else None
+    import numpy as np
+
+    amf = result.values["AMF_layer"]
+    relative = np.abs(result.values["std_AMF_layer"]) / np.abs(amf)
+    finite = relative[np.isfinite(relative)]
+    worst = float(finite.max()) if finite.size else None
+    quantiles = {"median": float(np.median(finite)), "p99": float(np.quantile(finite, 0.99)),
+                 "max": worst, "nonfinite": int(relative.size - finite.size)} if finite.size else None  # fmt: skip
     result = {"seconds": elapsed, "status": receipt["status"], "rounds": receipt["completed_rounds"],
-              "policy": "AMFConvergence(0.01, minimum 128, every 32, 3 passes)", "max_relative_se": worst}  # fmt: skip
+              "policy": "AMFConvergence(0.01, minimum 128, every 32, 3 passes)", "max_relative_se": worst,
+              "relative_se": quantiles, "method": receipt.get("method", "plain")}  # fmt: skip
+
+    if "proposal" in receipt:
+        result["proposal"] = {key: receipt["proposal"][key] for key in
+                              ("training_events", "logging_seconds", "training_seconds", "loss_first", "loss_last")}  # fmt: skip
 
     if worst and receipt["status"] != "converged":
         # SE falls as rounds^-1/2: rounds and seconds the same run needs to reach 1%.
         needed = receipt["completed_rounds"] * (worst / 0.01) ** 2
         result["projected_rounds"] = needed
         result["projected_seconds"] = elapsed * needed / receipt["completed_rounds"]
+
+    return result
+
+
+def amf_race(max_rounds: int = 256, photons_per_column: int = PHOTONS_PER_COLUMN) -> dict:
+    """Plain vs guided tracer on the full-size 3-D box cloud (tau 20 over 10 x 6 columns,
+    layers 2-6 km, Mie, I/Q/U, 3 views): wall time of simulate(..., AMFConvergence(0.01, 128,
+    32, 3)) up to max_rounds, the final worst relative AMF SE, and the projection of both to 1%
+    (SE ~ rounds^-1/2). The guided time includes logging and training the proposal."""
+
+    from monki import configs
+    from monki.execution.proposal import ProposalSettings
+
+    scene = configs.box_cloud()
+    plain = time_to_amf_se(scene, max_rounds, photons_per_column=photons_per_column, name="box-cloud")
+    guided = time_to_amf_se(scene, max_rounds, photons_per_column=photons_per_column, name="box-cloud",
+                            proposal=ProposalSettings())  # fmt: skip
+    result = {"plain": plain, "guided": guided, "workload": "box cloud 30x20x35, tau 20, 10x6 columns x layers 2-6 km"}
+
+    if plain.get("projected_seconds") and guided.get("projected_seconds"):
+        result["speedup_to_1pct"] = plain["projected_seconds"] / guided["projected_seconds"]
+
+    if plain.get("relative_se") and guided.get("relative_se"):
+        # Per-photon variance ratio at equal photon counts, from the median relative AMF SE.
+        result["variance_ratio_median"] = (guided["relative_se"]["median"] / plain["relative_se"]["median"]) ** 2
 
     return result
 
@@ -266,6 +303,8 @@ def main() -> int:
     parser.add_argument("--copy-calibration", action="store_true", help="run the copy kernel only")
     parser.add_argument("--summarize-traffic", type=Path, metavar="DIR")
     parser.add_argument("--summarize-profile", type=Path, metavar="DB")
+    parser.add_argument("--guided", type=int, metavar="ROUNDS", default=0,
+                        help="also race plain vs guided to 1%% AMF SE on the box cloud (round cap)")
     args = parser.parse_args()
 
     if args.summarize_profile:
@@ -359,6 +398,9 @@ def main() -> int:
 
     if args.amf:
         line["time_to_1pct_amf_se"] = time_to_amf_se(scene)
+
+    if args.guided:
+        line["guided_vs_plain"] = amf_race(args.guided)
 
     print(json.dumps(line), flush=True)
     return 0

@@ -1011,7 +1011,10 @@ __global__ void __launch_bounds__(kBlock32)
       for (int k = 0; k < monki_net::kInputs; ++k) features[k] = 0.0f;
       if (scatters) event_features<R>(s, h, features);
     }
-    if (MODE == kGuided && __ballot(scatters)) {
+    // The network runs when some lane's event is guided (kind and order within the guidance).
+    const bool guided_event = MODE == kGuided && scatters &&
+                              ((guidance.species_mask >> (h.species + 1)) & 1u) && h.order <= guidance.max_order;
+    if (MODE == kGuided && __ballot(guided_event)) {
       float o[monki_net::kOutputs];
 #pragma unroll
       for (int nb = 0; nb < 2; ++nb) {
@@ -1021,7 +1024,7 @@ __global__ void __launch_bounds__(kBlock32)
         monki_net::forward_block(net, x, raw, nullptr);
         monki_net::gather_outputs(raw, (__lane_id() >> 5) == nb, o);
       }
-      if (scatters) mixture = monki_net::mixture_from(o);
+      if (guided_event) mixture = monki_net::mixture_from(o);
     }
     if (MODE == kLog) {
       const unsigned long long logging = __ballot(scatters);

@@ -39,7 +39,9 @@ class ProposalSettings:
     steps, batch, learning_rate: Adam steps over random minibatches of the samples.
     alpha: probability of drawing the direction from the phase function (0 < alpha <= 1).
     guided_kinds: scatterer kinds whose events are guided: "gas" and/or tabulated species
-        indices (0-based); None guides every kind.
+        indices (0-based); None guides every kind. The default guides gas (Rayleigh) events:
+        a vMF mixture follows their smooth phase function closely, while guiding every event
+        of a thick Mie cloud multiplies weight factors over tens of events.
     max_order: events up to this scattering order are guided; later ones use the phase
         function only (the weight factors of a history multiply, so bounding the guided orders
         bounds the weight at (1 / alpha) ** max_order).
@@ -51,8 +53,8 @@ class ProposalSettings:
     steps: int = 1_500
     batch: int = 16_384
     learning_rate: float = 1e-3
-    alpha: float = 0.25
-    guided_kinds: Optional[tuple] = None
+    alpha: float = 0.3
+    guided_kinds: Optional[tuple] = ("gas",)
     max_order: int = 2**31
     seed: int = 97
 

@@ -445,4 +445,5 @@ def test_absorbing_column_closed_form(precision):
     p = np.exp(-tau / np.cos(np.deg2rad(scene.solar_zenith)))
     np.testing.assert_allclose(batch.stokes[0, 0, 0] / photons, radiance, rtol=5 * np.sqrt((1 - p) / (p * photons)))
     layer_amf = batch.weighted_paths[0, 0, :3, :] / batch.weights[0, 0, None, :]
-    np.testing.assert_allclose(layer_amf, np.broadcast_to(amf, (3, 3)), rtol=2e-5 if precision == 32 else 1e-6)
+    # float32 banks sum ~10^4 contributions per shard: relative rounding up to ~1e-4.
+    np.testing.assert_allclose(layer_amf, np.broadcast_to(amf, (3, 3)), rtol=2e-4 if precision == 32 else 1e-6)

@@ -83,6 +83,25 @@ int monki_hip_probe_philox(uint32_t device, const uint32_t *cases, size_t count,
 /* Release device buffers; NULL is accepted (kernels/metal/bridge.m:145-150). */
 void monki_hip_close(void *handle);
 
+/* ---- Batched scenes. No reference counterpart (the reference runs one scene per session).
+ * A batch session holds many scenes (each its own packed tables, grid size, geometry and tally
+ * shards), traced together by one persistent launch per round: waves start spread over the
+ * scenes and move to the next scene when theirs has no photons left. */
+
+/* tables: the packed float32 buffers of scene_count scenes back to back; scene i occupies
+ * words offsets[i] .. offsets[i + 1]. block: lanes per block (64, 128 or 256). One round slot
+ * per scene. NULL and error on failure; release with monki_hip_close. */
+void *monki_hip_batch_create(uint32_t device, const float *tables, const size_t *offsets,
+                             uint32_t scene_count, uint32_t block, char *error, size_t error_size);
+
+/* One round (1-based index `round`) of every scene, photons[i] photons for scene i. out receives,
+ * scene after scene, each scene's tallies in the monki_hip_run order (float64). use_graph: run
+ * the round as a HIP graph captured on first use (clears, transport, reductions; the per-scene
+ * controls are read from device memory), else launch the same sequence directly. */
+int monki_hip_batch_run(void *handle, uint32_t round, uint32_t seed, uint32_t max_steps,
+                        const uint32_t *photons, double *out, int use_graph, char *error,
+                        size_t error_size);
+
 /* ---- Learned proposal (guided tracer). No reference counterpart: the reference samples
  * directions from the phase function only (mc_routines_general_mod.f90:492-932).
  * A proposal handle owns a BF16 MLP (12 -> 4 x 128 SiLU -> 40: an 8-component von

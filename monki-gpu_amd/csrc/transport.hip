@@ -16,6 +16,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <vector>
+
 #include "monki_hip.h"
 #include "proposal.h"
 
@@ -73,6 +75,7 @@ struct Scene {
   // Device banks per round and shard: one record per (view, pixel) holding the layer paths
   // then the Stokes components.
   int bank_count;
+  size_t bank_base;  // first bank element of this scene in a multi-scene session
   float xunit, yunit, depol, theta0, phi0, phi;
   float view[kMaxViews];
   // Geometry derived once on the host in float64 from the float32 angles: launch direction,
@@ -886,8 +889,10 @@ __device__ inline int broadcast_int(int value, int lane) {
 template <typename R>
 __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int shard, bool has,
                                   const Pending<R> &wave_pending, const R *wave_history, int row,
-                                  R lane_layer, Counts &n) {
+                                  Counts &n) {
   const int lane = __lane_id();
+  // Thickness of this lane's layer (the vacuum layer adds no path).
+  const R lane_layer = lane < s.nz - 1 ? R(s.zunit[lane]) : R(0);
   const int record = s.paths ? s.nz + s.nmat : s.nmat;
   const int stokes_at = s.paths ? s.nz : 0;
   const bool path_lane = s.paths && lane < s.nz;
@@ -897,7 +902,7 @@ __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int sha
     const Pending<R> source = wave_pending.at(src);
     const int layer = source.layer();
     const R rz = source.rz();
-    R *bank = banks + (size_t(source.round()) * s.shards + shard) * size_t(s.bank_count);
+    R *bank = banks + s.bank_base + (size_t(source.round()) * s.shards + shard) * size_t(s.bank_count);
     const R path = path_lane ? wave_history[src * row + lane] : R(0);
 #pragma unroll 1
     for (int view = 0; view < s.views; ++view) {
@@ -948,19 +953,22 @@ __device__ inline void flush_counts(Counts &n, unsigned long long *counters) {
 template <typename R, bool TRACE, int MODE>
 __global__ void __launch_bounds__(kBlock32)
     __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? (MODE == kGuided ? 2 : 3) : 1)))
-    transport_kernel(Scene s, Controls c, R *banks, unsigned *failure, unsigned *work,
+    transport_kernel(const Scene *__restrict__ scenes, const Controls *__restrict__ controls,
+                     uint32_t scene_count, int row, R *banks, unsigned *failure, unsigned *work,
                      unsigned long long *counters, float *records, uint32_t record_count,
                      monki_net::Net net, monki_net::EventLog log, Guidance guidance) {
   extern __shared__ unsigned char lds_raw[];
-  // Layer rows of odd length: lanes reading one row, or each its own row at one layer,
-  // touch distinct LDS banks.
-  const int row = s.nz | 1;
+  // Layer rows of odd length (the largest nz of the scenes, | 1): lanes reading one row, or
+  // each its own row at one layer, touch distinct LDS banks.
   R *history = reinterpret_cast<R *>(lds_raw) + threadIdx.x * row;
   const R *wave_history = reinterpret_cast<R *>(lds_raw) + (threadIdx.x & ~63u) * row;
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const int shard = int(wave % uint32_t(s.shards));
-  // Thickness of this lane's layer in the cooperative tally (the vacuum layer adds no path).
-  const R lane_layer = __lane_id() < s.nz - 1 ? R(s.zunit[__lane_id()]) : R(0);
+  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
+  // Scenes of a batch: waves start spread over them, each wave works on one scene at a time
+  // (scene constants stay wave-uniform) and moves to the next one when its scene has no
+  // photons left and its lanes are idle; a wave visits every scene at most once.
+  uint32_t current = __builtin_amdgcn_readfirstlane(uint32_t((uint64_t(wave) * scene_count) / waves));
+  uint32_t visited = 0;
   const int block = int(blockDim.x);
   const Pending<R> wave_pending{
       reinterpret_cast<R *>(lds_raw) + size_t(block) * row + (threadIdx.x & ~63u),
@@ -975,11 +983,14 @@ __global__ void __launch_bounds__(kBlock32)
   uint32_t id = 0;
 
   for (;;) {
+    const Scene &s = scenes[current];
+    const Controls &c = controls[current];
+    const int shard = int(wave % uint32_t(s.shards));
     const unsigned long long idle = __ballot(!active);
     if (idle) {
       const int leader = __ffsll(static_cast<long long>(idle)) - 1;
       uint32_t base = 0;
-      if (__lane_id() == leader) base = atomicAdd(work, uint32_t(__popcll(idle)));
+      if (__lane_id() == leader) base = atomicAdd(work + current, uint32_t(__popcll(idle)));
       base = __shfl(base, leader, 64);
       if (!active) {
         id = base + uint32_t(__popcll(idle & __lanemask_lt()));
@@ -991,15 +1002,18 @@ __global__ void __launch_bounds__(kBlock32)
         }
       }
     }
-    if (__ballot(active) == 0) break;
+    if (__ballot(active) == 0) {
+      if (++visited >= scene_count) break;
+      current = __builtin_amdgcn_readfirstlane((current + 1) % scene_count);
+      continue;
+    }
 
     int status = kRunning;
     if (active) status = collide<R, TRACE>(s, c, failure, history, h, pending, n, trace);
     // Lanes read each other's LDS rows below: keep every access of this wave in order.
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    tally_wave<R>(s, banks, shard, active && status == kRunning, wave_pending, wave_history, row,
-                  lane_layer, n);
+    tally_wave<R>(s, banks, shard, active && status == kRunning, wave_pending, wave_history, row, n);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const bool scatters = active && status == kRunning && !h.surface;
@@ -1119,20 +1133,29 @@ __global__ void assemble_tallies(Scene s, const double *__restrict__ reduced,
 // ---------------------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------------------
+// One transport session: the scenes of a batch (one for the classic entry points), their
+// tables, tally banks, reduced and assembled tallies, and the captured graph of a batch round.
 struct Session {
   int device = -1;
   char name[256] = {0};
-  Scene scene{};
+  std::vector<Scene> scenes;  // device-pointer views of each scene's tables
+  std::vector<size_t> reduced_base, assembled_base;  // per scene, in doubles
+  Scene *d_scenes = nullptr;
+  Controls *d_controls = nullptr;
+  unsigned *work = nullptr;  // per-scene photon counters
   float *tables = nullptr;
   void *banks = nullptr;  // float or double banks, sized for double
-  double *reduced = nullptr;  // per round: summed banks
-  double *assembled = nullptr;  // per round: output tallies
-  unsigned *failure = nullptr;  // failure[0]: flags; failure[1]: work counter
+  size_t bank_elements = 0;  // all scenes, all round slots
+  double *reduced = nullptr;
+  double *assembled = nullptr;
+  size_t assembled_total = 0;  // doubles of all scenes for the round capacity
+  unsigned *failure = nullptr;  // failure flags
   unsigned long long *counters = nullptr;
   int compute_units = 0;
+  int row = 1;  // LDS row length: largest nz | 1
+  int block = kBlock32;
   float *records = nullptr;
   size_t record_capacity = 0;
-  size_t tally_count = 0;  // per round, all shards
   uint32_t round_capacity = 0;
   hipStream_t stream = nullptr;
   hipEvent_t start = nullptr, stop = nullptr;
@@ -1140,6 +1163,8 @@ struct Session {
   double kernel_seconds = 0.0;
   void *proposal = nullptr;  // attached learned proposal (guided runs)
   Guidance guidance{0.25f, ~0u, ~0u};  // guided runs: defensive fraction, kinds, orders
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
 };
 
 void set_error(char *error, size_t size, const char *fmt, const char *detail = "") {
@@ -1155,17 +1180,28 @@ bool check(hipError_t status, char *error, size_t size, const char *what) {
 void release(Session *s) {
   if (!s) return;
   if (s->device >= 0) (void)hipSetDevice(s->device);
+  if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
   if (s->start) (void)hipEventDestroy(s->start);
   if (s->stop) (void)hipEventDestroy(s->stop);
   if (s->stream) (void)hipStreamDestroy(s->stream);
-  (void)hipFree(s->tables);
-  (void)hipFree(s->banks);
-  (void)hipFree(s->reduced);
-  (void)hipFree(s->assembled);
-  (void)hipFree(s->failure);
-  (void)hipFree(s->counters);
-  (void)hipFree(s->records);
+  // s->work points into the failure allocation.
+  void *buffers[] = {s->tables, s->banks, s->reduced, s->assembled, s->failure, s->counters, s->records,
+                     s->d_scenes, s->d_controls};
+  for (void *b : buffers) (void)hipFree(b);
   delete s;
+}
+
+// Zero `words` 32-bit words (a kernel, not a memset node, so that it can be captured).
+__global__ void fill_zero(uint32_t *__restrict__ p, size_t words) {
+  const size_t stride = size_t(gridDim.x) * blockDim.x;
+  for (size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x; i < words; i += stride) p[i] = 0u;
+}
+
+void launch_fill(uint32_t *p, size_t words, hipStream_t stream) {
+  const size_t blocks = (words + 255) / 256;
+  hipLaunchKernelGGL(fill_zero, dim3(unsigned(blocks < 4096 ? (blocks ? blocks : 1) : 4096)), dim3(256), 0, stream,
+                     p, words);
 }
 
 bool word_int(const float *t, int index, int lo, int hi, int &out) {
@@ -1299,45 +1335,107 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
   return true;
 }
 
+// Everything a group of rounds runs on the device, in stream order: clear the banks, the
+// failure flags and the photon counters; transport; per scene, the float64 shard reduction
+// and the assembly in output order. No memset nodes, so that a batch round can be captured.
+template <typename R, bool TRACE, int MODE>
+bool enqueue(Session *h, uint32_t rounds, uint32_t grid, int block, size_t lds, size_t record_count,
+             monki_net::Net net, monki_net::EventLog log, char *error, size_t error_size) {
+  const uint32_t scene_count = uint32_t(h->scenes.size());
+  const size_t words = sizeof(R) / sizeof(uint32_t);
+  if (scene_count == 1) {
+    const Scene &s = h->scenes[0];
+    launch_fill(static_cast<uint32_t *>(h->banks), size_t(s.bank_count) * s.shards * rounds * words, h->stream);
+  } else {
+    launch_fill(static_cast<uint32_t *>(h->banks), h->bank_elements * words, h->stream);
+  }
+  launch_fill(h->failure, 1 + scene_count, h->stream);
+  R *banks = static_cast<R *>(h->banks);
+  if (grid) {
+    hipLaunchKernelGGL((transport_kernel<R, TRACE, MODE>), dim3(grid), dim3(block), lds, h->stream,
+                       h->d_scenes, h->d_controls, scene_count, h->row, banks, h->failure, h->work,
+                       h->counters, TRACE ? h->records : nullptr, uint32_t(TRACE ? record_count : 0), net, log,
+                       h->guidance);
+  }
+  if (!check(hipGetLastError(), error, error_size, "transport launch")) return false;
+  for (uint32_t i = 0; i < scene_count; ++i) {
+    const Scene &s = h->scenes[i];
+    const size_t summed = size_t(s.bank_count) * rounds;
+    hipLaunchKernelGGL(reduce_shards<R>, dim3(unsigned((summed + 255) / 256)), dim3(256), 0, h->stream,
+                       banks + s.bank_base, h->reduced + h->reduced_base[i], s.bank_count, s.shards, int(rounds));
+    hipLaunchKernelGGL(assemble_tallies, dim3(unsigned((summed + 255) / 256)), dim3(256), 0, h->stream, s,
+                       h->reduced + h->reduced_base[i], h->assembled + h->assembled_base[i], int(rounds));
+  }
+  return check(hipGetLastError(), error, error_size, "shard reduction launch");
+}
+
+// Persistent grid: enough resident blocks to fill every CU, never more than the work.
+template <typename R, bool TRACE, int MODE>
+uint32_t persistent_grid(const Session *h, uint64_t photons, int block, size_t lds) {
+  int resident = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, TRACE, MODE>, block,
+                                                  lds) != hipSuccess || resident < 1)
+    resident = 1;
+  const uint64_t wanted = (photons + block - 1) / block;
+  const uint64_t filled = uint64_t(resident) * uint64_t(h->compute_units > 0 ? h->compute_units : 1);
+  return uint32_t(wanted < filled ? wanted : filled);
+}
+
+// Lanes per block: the session's choice, at most 128 in float64 (register file).
+template <typename R>
+int block_size(const Session *h) {
+  return sizeof(R) == 8 && h->block > kBlock64 ? kBlock64 : h->block;
+}
+
+template <typename R>
+size_t lds_bytes(const Session *h, int block) {
+  return (size_t(h->row) + kPendingReals) * block * sizeof(R) + size_t(kPendingWords) * block * sizeof(int);
+}
+
+bool finish(Session *h, char *error, size_t error_size) {
+  unsigned flag = 0;
+  if (!check(hipMemcpyAsync(&flag, h->failure, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream), error,
+             error_size, "copy failure flag") ||
+      !check(hipStreamSynchronize(h->stream), error, error_size, "transport"))
+    return false;
+  if (flag) {
+    if (error && error_size)
+      snprintf(error, error_size, "HIP transport failed (flag %u); round rejected", flag);
+    return false;
+  }
+  return true;
+}
+
+// One group of rounds of the session's single scene (the classic entry points).
 template <typename R, bool TRACE, int MODE = kPlain>
 int launch(Session *h, const uint32_t *controls, double *out, float *records, size_t record_count,
            char *error, size_t error_size, void *proposal = nullptr) {
-  if (!h || !controls || !out) {
+  if (!h || !controls || !out || h->scenes.size() != 1) {
     set_error(error, error_size, "Invalid HIP session or arguments");
     return 1;
   }
   Controls c{controls[0], controls[1], controls[2], controls[3],
              controls[4], controls[5], controls[6], controls[7]};
-  const Scene &s = h->scene;
-  const size_t per_round = size_t(s.bank_count) * size_t(s.shards);
+  const Scene &s = h->scenes[0];
   if (!c.per_round || !c.rounds || c.rounds > h->round_capacity || !c.max_steps ||
       c.count > (1u << 31) ||
       uint64_t(c.first) + c.count > uint64_t(c.per_round) * c.rounds) {
     set_error(error, error_size, "Invalid HIP round group");
     return 1;
   }
-  if (!check(hipSetDevice(h->device), error, error_size, "hipSetDevice")) return 1;
-  const size_t active = per_round * c.rounds;
-  if (!check(hipMemsetAsync(h->banks, 0, active * sizeof(R), h->stream), error, error_size,
-             "clear tallies") ||
-      !check(hipMemsetAsync(h->failure, 0, 2 * sizeof(unsigned), h->stream), error, error_size,
-             "clear failure flag and work counter"))
+  if (!check(hipSetDevice(h->device), error, error_size, "hipSetDevice") ||
+      !check(hipMemcpy(h->d_controls, &c, sizeof(Controls), hipMemcpyHostToDevice), error, error_size,
+             "upload controls"))
     return 1;
-  if (TRACE) {
-    if (record_count > h->record_capacity) {
-      (void)hipFree(h->records);
-      h->records = nullptr;
-      h->record_capacity = 0;
-      if (!check(hipMalloc(&h->records, record_count * kTraceWords * sizeof(float)), error,
-                 error_size, "allocate trace records"))
-        return 1;
-      h->record_capacity = record_count;
-    }
+  if (TRACE && record_count > h->record_capacity) {
+    (void)hipFree(h->records);
+    h->records = nullptr;
+    h->record_capacity = 0;
+    if (!check(hipMalloc(&h->records, record_count * kTraceWords * sizeof(float)), error, error_size,
+               "allocate trace records"))
+      return 1;
+    h->record_capacity = record_count;
   }
-  const int block = sizeof(R) == 4 ? kBlock32 : kBlock64;
-  const size_t lds = (size_t(s.nz | 1) + kPendingReals) * block * sizeof(R) + size_t(kPendingWords) * block * sizeof(int);
-  // Persistent grid: enough resident blocks to fill every CU, never more than the work.
-  int resident = 0;
   monki_net::Net net{nullptr, nullptr, nullptr};
   monki_net::EventLog log{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0};
   uint32_t logged_before = 0;
@@ -1356,45 +1454,20 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
       logged_before = monki_net::proposal_samples(proposal);
     }
   }
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, TRACE, MODE>, block,
-                                                  lds) != hipSuccess || resident < 1)
-    resident = 1;
-  const uint64_t wanted = (uint64_t(c.count) + block - 1) / block;
-  const uint64_t filled = uint64_t(resident) * uint64_t(h->compute_units > 0 ? h->compute_units : 1);
-  const uint32_t grid = uint32_t(wanted < filled ? wanted : filled);
-  R *banks = static_cast<R *>(h->banks);
+  const int block = block_size<R>(h);
+  const size_t lds = lds_bytes<R>(h, block);
+  const uint32_t grid = persistent_grid<R, TRACE, MODE>(h, c.count, block, lds);
   (void)hipEventRecord(h->start, h->stream);
-  if (grid) {
-    hipLaunchKernelGGL((transport_kernel<R, TRACE, MODE>), dim3(grid), dim3(block), lds, h->stream, s, c,
-                       banks, h->failure, h->failure + 1, h->counters, TRACE ? h->records : nullptr,
-                       uint32_t(TRACE ? record_count : 0), net, log, h->guidance);
-  }
+  if (!enqueue<R, TRACE, MODE>(h, c.rounds, grid, block, lds, record_count, net, log, error, error_size)) return 1;
   (void)hipEventRecord(h->stop, h->stream);
-  if (!check(hipGetLastError(), error, error_size, "transport launch")) return 1;
-  const size_t summed = size_t(s.bank_count) * c.rounds;
-  hipLaunchKernelGGL(reduce_shards<R>, dim3((summed + 255) / 256), dim3(256), 0, h->stream, banks,
-                     h->reduced, s.bank_count, s.shards, int(c.rounds));
-  const size_t assembled = size_t(s.bank_count) * c.rounds;
-  hipLaunchKernelGGL(assemble_tallies, dim3((assembled + 255) / 256), dim3(256), 0, h->stream, s,
-                     h->reduced, h->assembled, int(c.rounds));
-  const size_t reduced = size_t(s.tally_count) * c.rounds;
-  if (!check(hipGetLastError(), error, error_size, "shard reduction launch")) return 1;
-  unsigned flag = 0;
-  if (!check(hipMemcpyAsync(&flag, h->failure, sizeof(unsigned), hipMemcpyDeviceToHost, h->stream),
-             error, error_size, "copy failure flag") ||
-      !check(hipStreamSynchronize(h->stream), error, error_size, "transport"))
-    return 1;
+  if (!finish(h, error, error_size)) return 1;
   float ms = 0.0f;
   (void)hipEventElapsedTime(&ms, h->start, h->stop);
   h->kernel_seconds = ms * 1e-3;
   if (c.profile) h->gpu_seconds += ms * 1e-3;
-  if (flag) {
-    if (error && error_size)
-      snprintf(error, error_size, "HIP transport failed (flag %u); round rejected", flag);
-    return 1;
-  }
-  if (!check(hipMemcpy(out, h->assembled, reduced * sizeof(double), hipMemcpyDeviceToHost), error,
-             error_size, "copy tallies"))
+  if (!check(hipMemcpy(out, h->assembled, size_t(s.tally_count) * c.rounds * sizeof(double),
+                       hipMemcpyDeviceToHost),
+             error, error_size, "copy tallies"))
     return 1;
   if (MODE == kLog && !monki_net::proposal_finish_log(proposal, logged_before, h->stream, error, error_size))
     return 1;
@@ -1405,6 +1478,89 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
       return 1;
   }
   return 0;
+}
+
+// Parse, upload and lay out the scenes of a session. tables holds the packed buffers of all
+// scenes back to back; offsets[i] .. offsets[i + 1] is scene i (float32 words).
+Session *create_session(uint32_t device, const float *tables, const size_t *offsets, uint32_t scene_count,
+                        uint32_t round_capacity, uint32_t block, char *error, size_t error_size) {
+  if (!tables || !offsets || !scene_count || scene_count > 4096 || !round_capacity ||
+      (block != 64 && block != 128 && block != 256)) {
+    set_error(error, error_size, "Invalid HIP create arguments");
+    return nullptr;
+  }
+  int devices = 0;
+  if (hipGetDeviceCount(&devices) != hipSuccess || device >= uint32_t(devices)) {
+    set_error(error, error_size, "Requested HIP device is unavailable");
+    return nullptr;
+  }
+  std::vector<Scene> parsed(scene_count);
+  for (uint32_t i = 0; i < scene_count; ++i) {
+    if (offsets[i + 1] <= offsets[i] ||
+        !parse_tables(tables + offsets[i], offsets[i + 1] - offsets[i], nullptr, parsed[i], error, error_size))
+      return nullptr;
+  }
+  Session *h = new Session();
+  h->device = int(device);
+  h->round_capacity = round_capacity;
+  h->block = int(block);
+  size_t banks = 0, reduced = 0, assembled = 0;
+  int nz = 1;
+  for (uint32_t i = 0; i < scene_count; ++i) {
+    parsed[i].bank_base = banks;
+    h->reduced_base.push_back(reduced);
+    h->assembled_base.push_back(assembled);
+    banks += size_t(parsed[i].bank_count) * parsed[i].shards * round_capacity;
+    reduced += size_t(parsed[i].bank_count) * round_capacity;
+    assembled += size_t(parsed[i].tally_count) * round_capacity;
+    nz = parsed[i].nz > nz ? parsed[i].nz : nz;
+  }
+  h->bank_elements = banks;
+  h->assembled_total = assembled;
+  h->row = nz | 1;
+  const size_t table_count = offsets[scene_count];
+  hipDeviceProp_t prop{};
+  if (!check(hipSetDevice(h->device), error, error_size, "hipSetDevice") ||
+      !check(hipGetDeviceProperties(&prop, h->device), error, error_size, "device properties") ||
+      !check(hipStreamCreate(&h->stream), error, error_size, "stream") ||
+      !check(hipEventCreate(&h->start), error, error_size, "event") ||
+      !check(hipEventCreate(&h->stop), error, error_size, "event") ||
+      !check(hipMalloc(&h->tables, table_count * sizeof(float)), error, error_size, "allocate tables") ||
+      !check(hipMalloc(&h->banks, banks * sizeof(double)), error, error_size, "allocate tally banks") ||
+      !check(hipMalloc(&h->reduced, reduced * sizeof(double)), error, error_size, "allocate reduced tallies") ||
+      !check(hipMalloc(&h->assembled, assembled * sizeof(double)), error, error_size, "allocate output tallies") ||
+      !check(hipMalloc(&h->failure, (1 + scene_count) * sizeof(unsigned)), error, error_size, "allocate flags") ||
+      !check(hipMalloc(&h->counters, kCounters * sizeof(unsigned long long)), error, error_size,
+             "allocate counters") ||
+      !check(hipMalloc(&h->d_scenes, scene_count * sizeof(Scene)), error, error_size, "allocate scenes") ||
+      !check(hipMalloc(&h->d_controls, scene_count * sizeof(Controls)), error, error_size, "allocate controls") ||
+      !check(hipMemcpy(h->tables, tables, table_count * sizeof(float), hipMemcpyHostToDevice), error,
+             error_size, "upload tables") ||
+      !check(hipMemset(h->counters, 0, kCounters * sizeof(unsigned long long)), error, error_size,
+             "clear counters")) {
+    release(h);
+    return nullptr;
+  }
+  h->work = h->failure + 1;
+  for (uint32_t i = 0; i < scene_count; ++i) {
+    Scene view{};
+    parse_tables(tables + offsets[i], offsets[i + 1] - offsets[i], h->tables + offsets[i], view, nullptr, 0);
+    view.bank_base = parsed[i].bank_base;
+    h->scenes.push_back(view);
+  }
+  if (!check(hipMemcpy(h->d_scenes, h->scenes.data(), scene_count * sizeof(Scene), hipMemcpyHostToDevice), error,
+             error_size, "upload scenes")) {
+    release(h);
+    return nullptr;
+  }
+  // Some drivers report an empty marketing name: fall back to the architecture and CU count.
+  if (prop.name[0])
+    snprintf(h->name, sizeof(h->name), "%s (%s, %d CUs)", prop.name, prop.gcnArchName,
+             prop.multiProcessorCount);
+  else
+    snprintf(h->name, sizeof(h->name), "%s, %d CUs", prop.gcnArchName, prop.multiProcessorCount);
+  h->compute_units = prop.multiProcessorCount;
+  return h;
 }
 
 // Single-purpose probes of the device helpers, for tests against golden vectors.
@@ -1468,58 +1624,84 @@ int monki_hip_abi_version(void) { return 2; }
 
 void *monki_hip_create(uint32_t device, const float *tables, size_t table_count, size_t tally_count,
                        uint32_t round_capacity, char *error, size_t error_size) {
-  if (!tables || !round_capacity) {
+  if (!tables) {
     set_error(error, error_size, "Invalid HIP create arguments");
     return nullptr;
   }
   Scene parsed{};
   if (!parse_tables(tables, table_count, nullptr, parsed, error, error_size)) return nullptr;
-  const size_t per_round = size_t(parsed.bank_count) * size_t(parsed.shards);
   if (tally_count != size_t(parsed.tally_count) * size_t(parsed.shards)) {
     set_error(error, error_size, "Tally count does not match the packed tables");
     return nullptr;
   }
-  int devices = 0;
-  if (hipGetDeviceCount(&devices) != hipSuccess || device >= uint32_t(devices)) {
-    set_error(error, error_size, "Requested HIP device is unavailable");
-    return nullptr;
+  const size_t offsets[2] = {0, table_count};
+  return create_session(device, tables, offsets, 1, round_capacity, kBlock32, error, error_size);
+}
+
+void *monki_hip_batch_create(uint32_t device, const float *tables, const size_t *offsets, uint32_t scene_count,
+                             uint32_t block, char *error, size_t error_size) {
+  return create_session(device, tables, offsets, scene_count, 1, block, error, error_size);
+}
+
+int monki_hip_batch_run(void *handle, uint32_t round, uint32_t seed, uint32_t max_steps, const uint32_t *photons,
+                        double *out, int use_graph, char *error, size_t error_size) {
+  Session *h = static_cast<Session *>(handle);
+  if (!h || !photons || !out || !round || !max_steps) {
+    set_error(error, error_size, "Invalid HIP batch round");
+    return 1;
   }
-  Session *h = new Session();
-  h->device = int(device);
-  h->tally_count = per_round;
-  h->round_capacity = round_capacity;
-  hipDeviceProp_t prop{};
+  const uint32_t scene_count = uint32_t(h->scenes.size());
+  std::vector<Controls> controls(scene_count);
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < scene_count; ++i) {
+    if (!photons[i] || photons[i] > (1u << 31)) {
+      set_error(error, error_size, "Invalid photon count in a batch round");
+      return 1;
+    }
+    controls[i] = Controls{photons[i], 0u, round, seed, max_steps, 0u, photons[i], 1u};
+    total += photons[i];
+  }
   if (!check(hipSetDevice(h->device), error, error_size, "hipSetDevice") ||
-      !check(hipGetDeviceProperties(&prop, h->device), error, error_size, "device properties") ||
-      !check(hipStreamCreate(&h->stream), error, error_size, "stream") ||
-      !check(hipEventCreate(&h->start), error, error_size, "event") ||
-      !check(hipEventCreate(&h->stop), error, error_size, "event") ||
-      !check(hipMalloc(&h->tables, table_count * sizeof(float)), error, error_size, "allocate tables") ||
-      !check(hipMalloc(&h->banks, per_round * round_capacity * sizeof(double)), error, error_size,
-             "allocate tally banks") ||
-      !check(hipMalloc(&h->reduced, size_t(parsed.bank_count) * round_capacity * sizeof(double)),
-             error, error_size, "allocate reduced tallies") ||
-      !check(hipMalloc(&h->assembled, size_t(parsed.tally_count) * round_capacity * sizeof(double)),
-             error, error_size, "allocate output tallies") ||
-      !check(hipMalloc(&h->failure, 2 * sizeof(unsigned)), error, error_size, "allocate flag") ||
-      !check(hipMalloc(&h->counters, kCounters * sizeof(unsigned long long)), error, error_size,
-             "allocate counters") ||
-      !check(hipMemcpy(h->tables, tables, table_count * sizeof(float), hipMemcpyHostToDevice), error,
-             error_size, "upload tables") ||
-      !check(hipMemset(h->counters, 0, kCounters * sizeof(unsigned long long)), error, error_size,
-             "clear counters")) {
-    release(h);
-    return nullptr;
+      !check(hipMemcpy(h->d_controls, controls.data(), scene_count * sizeof(Controls), hipMemcpyHostToDevice),
+             error, error_size, "upload controls"))
+    return 1;
+  const monki_net::Net net{nullptr, nullptr, nullptr};
+  const monki_net::EventLog log{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0};
+  const int block = block_size<float>(h);
+  const size_t lds = lds_bytes<float>(h, block);
+  const uint32_t grid = persistent_grid<float, false, kPlain>(h, total, block, lds);
+  (void)hipEventRecord(h->start, h->stream);
+  if (use_graph) {
+    if (!h->graph_exec) {
+      // Capture the whole round once; replays read the per-scene controls from device memory.
+      if (!check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), error, error_size,
+                 "begin capture"))
+        return 1;
+      const bool queued = enqueue<float, false, kPlain>(h, 1, grid, block, lds, 0, net, log, error, error_size);
+      hipGraph_t graph = nullptr;
+      const hipError_t ended = hipStreamEndCapture(h->stream, &graph);
+      if (!queued || !check(ended, error, error_size, "end capture")) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return 1;
+      }
+      h->graph = graph;
+      if (!check(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0), error, error_size,
+                 "instantiate graph"))
+        return 1;
+    }
+    if (!check(hipGraphLaunch(h->graph_exec, h->stream), error, error_size, "graph launch")) return 1;
+  } else if (!enqueue<float, false, kPlain>(h, 1, grid, block, lds, 0, net, log, error, error_size)) {
+    return 1;
   }
-  // Some drivers report an empty marketing name: fall back to the architecture and CU count.
-  if (prop.name[0])
-    snprintf(h->name, sizeof(h->name), "%s (%s, %d CUs)", prop.name, prop.gcnArchName,
-             prop.multiProcessorCount);
-  else
-    snprintf(h->name, sizeof(h->name), "%s, %d CUs", prop.gcnArchName, prop.multiProcessorCount);
-  h->compute_units = prop.multiProcessorCount;
-  parse_tables(tables, table_count, h->tables, h->scene, nullptr, 0);
-  return h;
+  (void)hipEventRecord(h->stop, h->stream);
+  if (!finish(h, error, error_size)) return 1;
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, h->start, h->stop);
+  h->kernel_seconds = ms * 1e-3;
+  return check(hipMemcpy(out, h->assembled, h->assembled_total * sizeof(double), hipMemcpyDeviceToHost), error,
+               error_size, "copy tallies")
+             ? 0
+             : 1;
 }
 
 int monki_hip_run(void *handle, const uint32_t *controls, double *out, char *error, size_t error_size) {

@@ -8,6 +8,6 @@ for prepared in-memory transport tables.
 from .convergence import AMFConvergence
 from .input import ReferenceCase, SceneCase
 from .output import SimulationResult
-from .simulation import simulate
+from .simulation import simulate, simulate_batch
 
-__all__ = ["AMFConvergence", "ReferenceCase", "SceneCase", "SimulationResult", "simulate"]
+__all__ = ["AMFConvergence", "ReferenceCase", "SceneCase", "SimulationResult", "simulate", "simulate_batch"]

@@ -122,25 +122,27 @@ class AMFStopping:
                 self.moments.add(batch)
                 yield batch
 
-            if end < self.policy.minimum_rounds:
-                continue
-
-            amf, error = self.moments.estimate(self.thickness)
-            relative = np.divide(
-                error, np.abs(amf), out=np.full_like(error, np.inf), where=amf != 0
-            )
-            maximum = float(np.max(relative))
-            passed = bool(np.all(np.isfinite(relative)) and maximum < self.policy.relative_error)
-            self.streak = self.streak + 1 if passed else 0
-            self.converged = self.streak >= self.policy.consecutive_checks
-            self.history.append(
-                {
-                    "rounds": end,
-                    "max_relative_se": maximum if np.isfinite(maximum) else None,
-                    "below_target": passed,
-                    "streak": self.streak,
-                }
-            )
-
-            if self.converged:
+            if self.check(end):
                 return
+
+    def check(self, end: int) -> bool:
+        """Precision check after round ``end`` (a group boundary); True once converged."""
+
+        if end < self.policy.minimum_rounds:
+            return False
+
+        amf, error = self.moments.estimate(self.thickness)
+        relative = np.divide(error, np.abs(amf), out=np.full_like(error, np.inf), where=amf != 0)
+        maximum = float(np.max(relative))
+        passed = bool(np.all(np.isfinite(relative)) and maximum < self.policy.relative_error)
+        self.streak = self.streak + 1 if passed else 0
+        self.converged = self.streak >= self.policy.consecutive_checks
+        self.history.append(
+            {
+                "rounds": end,
+                "max_relative_se": maximum if np.isfinite(maximum) else None,
+                "below_target": passed,
+                "streak": self.streak,
+            }
+        )
+        return self.converged

@@ -1,7 +1,9 @@
 """Interchangeable backend strategies (python/monki/execution/__init__.py)."""
 
 from .backend import Backend
+from .batch import HIPBatch
 from .hip import HIP
+from .proposal import ProposalSettings
 from .tallies import BatchControls, RawTallies
 
-__all__ = ["Backend", "HIP", "BatchControls", "RawTallies"]
+__all__ = ["Backend", "HIP", "HIPBatch", "ProposalSettings", "BatchControls", "RawTallies"]

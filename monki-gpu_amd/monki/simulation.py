@@ -20,87 +20,90 @@ from .metadata import file_hash
 from .output import SimulationResult
 
 
-def simulate(
-    case,
-    backend: Backend,
-    output,
-    *,
-    convergence: Optional[AMFConvergence] = None,
-) -> SimulationResult:
-    """Run equally sized rounds and save a complete NetCDF product plus receipt.
+class RoundStatistics:
+    """Round accumulation and product of one case (the body of simulation.py:45-189).
 
-    With ``AMFConvergence``, ``case.rounds`` is the maximum round budget; the receipt
-    records ``converged`` or ``max_rounds_reached``.
+    ``add`` takes the rounds in order; ``finish`` computes the statistics, writes the NetCDF
+    product and the ``.run.json`` receipt. ``simulate`` drives one instance from one backend;
+    ``simulate_batch`` drives one per scene of a batch in lockstep.
     """
 
-    if convergence is not None and case.rounds < convergence.minimum_rounds:
-        raise ValueError("round cap must be at least minimum_rounds")
+    def __init__(self, case, metadata, output, convergence: Optional[AMFConvergence] = None):
+        if convergence is not None and case.rounds < convergence.minimum_rounds:
+            raise ValueError("round cap must be at least minimum_rounds")
 
-    output = Path(output).resolve()
-    receipt = output.with_suffix(".run.json")
+        self.output = Path(output).resolve()
+        self.receipt = self.output.with_suffix(".run.json")
 
-    if output.exists() or receipt.exists():
-        raise FileExistsError(f"Refusing to overwrite an existing result: {output}")
-
-    with backend:
-        backend.prepare(case)
-        metadata = backend.metadata
+        if self.output.exists() or self.receipt.exists():
+            raise FileExistsError(f"Refusing to overwrite an existing result: {self.output}")
 
         if metadata is None:
             raise RuntimeError("Backend did not supply prepared scene metadata")
 
-        total = rounds_sum = rounds_squared = paths = weights = None
-        history = []
-        stopping = None
+        self.case = case
+        self.metadata = metadata
+        self.total = self.rounds_sum = self.rounds_squared = self.paths = self.weights = None
+        self.history = []
+        self.stopping = None
 
         if convergence is not None:
             if "AMF_layer" not in metadata.values:
                 raise ValueError("AMF stopping requires layer path diagnostics")
 
             thickness = np.diff(metadata.values["alt"]) / 1000
-            stopping = AMFStopping(convergence, thickness)
-            batches = stopping.batches(backend, case)
-        else:
-            requests = [
-                BatchControls(index, case.photons_per_column) for index in range(1, case.rounds + 1)
-            ]
-            batches = backend.run_batches(requests)
+            self.stopping = AMFStopping(convergence, thickness)
 
-            if len(batches) != case.rounds:
-                raise ValueError("Backend returned the wrong number of statistical rounds")
+    @property
+    def completed(self) -> int:
+        return len(self.history)
 
-        for index, batch in enumerate(batches, start=1):
-            if not isinstance(batch, RawTallies):
-                raise TypeError("Backend must return RawTallies")
+    def add(self, batch) -> None:
+        """Accumulate the next round (round indices 1, 2, ... in order)."""
 
-            if batch.controls != BatchControls(index, case.photons_per_column):
-                raise ValueError("Backend returned tallies for a different round")
+        case = self.case
+        index = self.completed + 1
 
-            if total is None:
-                total = np.zeros_like(batch.stokes)
-                rounds_sum = np.zeros_like(batch.stokes)
-                rounds_squared = np.zeros_like(batch.stokes)
+        if not isinstance(batch, RawTallies):
+            raise TypeError("Backend must return RawTallies")
 
-                if batch.weighted_paths is not None:
-                    paths = np.zeros_like(batch.weighted_paths)
-                    weights = np.zeros_like(batch.weights)
+        if batch.controls != BatchControls(index, case.photons_per_column):
+            raise ValueError("Backend returned tallies for a different round")
 
-            total += batch.stokes
-            normalized = batch.stokes / case.photons_per_column
-            rounds_sum += normalized
-            rounds_squared += normalized**2
-            _, nx, ny, angles = total.shape
-            # Successive spatial sums and divisions as in the original accumulate.
-            history.append(rounds_sum.sum(axis=1).sum(axis=1).sum(axis=1) / nx / ny / angles / index)
+        if self.total is None:
+            self.total = np.zeros_like(batch.stokes)
+            self.rounds_sum = np.zeros_like(batch.stokes)
+            self.rounds_squared = np.zeros_like(batch.stokes)
 
-            if paths is not None:
-                if batch.weighted_paths is None or batch.weights is None:
-                    raise ValueError("Backend dropped path diagnostics during a session")
+            if batch.weighted_paths is not None:
+                self.paths = np.zeros_like(batch.weighted_paths)
+                self.weights = np.zeros_like(batch.weights)
 
-                paths += batch.weighted_paths
-                weights += batch.weights
+        self.total += batch.stokes
+        normalized = batch.stokes / case.photons_per_column
+        self.rounds_sum += normalized
+        self.rounds_squared += normalized**2
+        _, nx, ny, angles = self.total.shape
+        # Successive spatial sums and divisions as in the original accumulate.
+        self.history.append(self.rounds_sum.sum(axis=1).sum(axis=1).sum(axis=1) / nx / ny / angles / index)
 
-        completed = len(history)
+        if self.paths is not None:
+            if batch.weighted_paths is None or batch.weights is None:
+                raise ValueError("Backend dropped path diagnostics during a session")
+
+            self.paths += batch.weighted_paths
+            self.weights += batch.weights
+
+    def finish(self, backend_provenance: dict) -> SimulationResult:
+        case, metadata, output = self.case, self.metadata, self.output
+        total, rounds_sum, rounds_squared = self.total, self.rounds_sum, self.rounds_squared
+        paths, weights, stopping = self.paths, self.weights, self.stopping
+        completed = self.completed
+
+        if completed == 0:
+            raise ValueError("No statistical round was run")
+
+        _, nx, ny, _ = total.shape
         radiance = total / completed / case.photons_per_column
         uncertainty = np.full_like(radiance, 999.0)
 
@@ -116,7 +119,7 @@ def simulate(
                 completed
             )
 
-        radiance_history = np.asarray(history)
+        radiance_history = np.asarray(self.history)
 
         if case.backward and radiance.shape[0] >= 3:
             radiance[2] *= -1
@@ -158,7 +161,7 @@ def simulate(
         result = SimulationResult(output, values, dimensions, attributes)
         result.save()
         provenance = {
-            **backend.provenance,
+            **backend_provenance,
             "status": "complete",
             "completed_rounds": completed,
             "total_photons": completed * case.photons_per_column * nx * ny,
@@ -174,7 +177,93 @@ def simulate(
                 "uncertainty": "pointwise relative standard error",
             }
 
-        with receipt.open("x") as stream:
+        with self.receipt.open("x") as stream:
             stream.write(json.dumps(provenance, indent=2, default=str) + "\n")
 
         return result
+
+
+def simulate(
+    case,
+    backend: Backend,
+    output,
+    *,
+    convergence: Optional[AMFConvergence] = None,
+) -> SimulationResult:
+    """Run equally sized rounds and save a complete NetCDF product plus receipt.
+
+    With ``AMFConvergence``, ``case.rounds`` is the maximum round budget; the receipt
+    records ``converged`` or ``max_rounds_reached``.
+    """
+
+    if convergence is not None and case.rounds < convergence.minimum_rounds:
+        raise ValueError("round cap must be at least minimum_rounds")
+
+    output = Path(output).resolve()
+
+    if output.exists() or output.with_suffix(".run.json").exists():
+        raise FileExistsError(f"Refusing to overwrite an existing result: {output}")
+
+    with backend:
+        backend.prepare(case)
+        statistics = RoundStatistics(case, backend.metadata, output, convergence)
+
+        if statistics.stopping is not None:
+            batches = statistics.stopping.batches(backend, case)
+        else:
+            requests = [BatchControls(index, case.photons_per_column) for index in range(1, case.rounds + 1)]
+            batches = backend.run_batches(requests)
+
+            if len(batches) != case.rounds:
+                raise ValueError("Backend returned the wrong number of statistical rounds")
+
+        for batch in batches:
+            statistics.add(batch)
+
+        return statistics.finish(backend.provenance)
+
+
+def simulate_batch(cases, backend, outputs, *, convergence: Optional[AMFConvergence] = None) -> list:
+    """Run the cases of a batch backend (``HIPBatch``) round by round in lockstep and save one
+    product and receipt per case. No reference counterpart: the reference runs one case per
+    session. With ``AMFConvergence`` each case stops on its own policy; rounds continue while any
+    case still runs (finished cases' later rounds are traced and dropped)."""
+
+    cases = list(cases)
+    outputs = [Path(output).resolve() for output in outputs]
+
+    if len(outputs) != len(cases):
+        raise ValueError("one output path per case is required")
+
+    with backend:
+        backend.prepare(cases)
+        statistics = [
+            RoundStatistics(case, metadata, output, convergence)
+            for case, metadata, output in zip(cases, backend.metadata, outputs)
+        ]
+        running = [True] * len(cases)
+
+        for index in range(1, max(case.rounds for case in cases) + 1):
+            if not any(running):
+                break
+
+            batches = backend.run_round(index)
+
+            for number, (stats, batch) in enumerate(zip(statistics, batches)):
+                if not running[number]:
+                    continue
+
+                stats.add(batch)
+                stopping = stats.stopping
+
+                if stopping is not None:
+                    stopping.moments.add(batch)
+                    group_end = index % stopping.policy.check_every == 0 or index == stats.case.rounds
+
+                    if group_end and stopping.check(index):
+                        running[number] = False
+
+                if index >= stats.case.rounds:
+                    running[number] = False
+
+        return [stats.finish(provenance) for stats, provenance in zip(statistics, backend.case_provenance())]

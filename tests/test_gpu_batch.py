@@ -1,0 +1,107 @@
+"""GPU tests of the batched launch: scenes traced together equal the same scenes traced alone
+(same Philox streams), the captured graph replays equal direct launches, simulate_batch equals
+simulate per case, and the autotuner returns a measured configuration."""
+
+import json
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def scenes():
+    from monki import configs
+
+    return [
+        configs.box_cloud(nx=8, ny=6, nmat=3, box=(3, 2), offset=(2, 1)),
+        configs.clear_sky(nx=6, ny=4, nmat=1),
+        configs.layered_cloud(nx=3, ny=2, nmat=3),
+        configs.box_cloud(nx=5, ny=7, nmat=3, solar_zenith=65.0, albedo=0.2, optical_depth=9.0, box=(2, 3)),
+    ]
+
+
+def cases(rounds=3, ppc=150):
+    from monki import SceneCase
+
+    return [SceneCase(f"scene{i}", scene, ppc + 25 * i, rounds) for i, scene in enumerate(scenes())]
+
+
+def assert_tallies_equal(a, b):
+    # Identical histories; float32 bank sums in another order (tolerance of the reference's
+    # dispatch and shard invariance tests, test_metal.py:133-140).
+    np.testing.assert_allclose(a.stokes, b.stokes, rtol=2e-5, atol=2e-5)
+
+    if a.weighted_paths is not None:
+        np.testing.assert_allclose(a.weighted_paths, b.weighted_paths, rtol=2e-5, atol=2e-5)
+        np.testing.assert_allclose(a.weights, b.weights, rtol=2e-5, atol=2e-5)
+
+
+@pytest.mark.parametrize("block", [64, 256])
+def test_batched_round_equals_single_scene_sessions(block):
+    from monki.execution import HIP, HIPBatch, BatchControls
+
+    batch_cases = cases()
+
+    with HIPBatch(block=block, tally_shards=8) as batch:
+        batch.prepare(batch_cases)
+        batched = [batch.run_round(1), batch.run_round(2)]
+
+    for number, case in enumerate(batch_cases):
+        with HIP(tally_shards=32) as single:
+            single.prepare(case)
+            for index in (1, 2):
+                alone = single.run_batch(BatchControls(index, case.photons_per_column))
+                assert_tallies_equal(batched[index - 1][number], alone)
+
+
+def test_graph_replays_equal_direct_launches():
+    """The captured round, replayed several times with direct launches of another session and
+    host work in between, equals the directly launched round every time."""
+
+    from monki.execution import HIPBatch
+
+    batch_cases = cases(rounds=6)
+
+    with HIPBatch(use_graph=True) as graph, HIPBatch(use_graph=False) as direct:
+        graph.prepare(batch_cases)
+        direct.prepare(batch_cases)
+
+        for index in range(1, 7):
+            replayed = graph.run_round(index)
+            launched = direct.run_round(index)
+            np.asarray([b.stokes.sum() for b in launched]).sum()  # host work between replays
+
+            for a, b in zip(replayed, launched):
+                assert_tallies_equal(a, b)
+
+
+def test_simulate_batch_equals_simulate(tmp_path):
+    from monki import AMFConvergence, simulate, simulate_batch
+    from monki.execution import HIP, HIPBatch
+
+    batch_cases = cases(rounds=32, ppc=100)
+    policy = AMFConvergence(0.5, 32, 16, 1)
+    outputs = [tmp_path / f"batch{i}.nc" for i in range(len(batch_cases))]
+    batched = simulate_batch(batch_cases, HIPBatch(), outputs, convergence=policy)
+
+    for i, case in enumerate(batch_cases):
+        alone = simulate(case, HIP(), tmp_path / f"alone{i}.nc", convergence=policy)
+        # Sums of 32 rounds of float32 banks: absolute tolerance scaled to the largest value.
+        expected = alone.values["radiance"]
+        np.testing.assert_allclose(batched[i].values["radiance"], expected, rtol=2e-5,
+                                   atol=2e-5 * np.abs(expected).max())  # fmt: skip
+        np.testing.assert_allclose(batched[i].values["AMF_layer"], alone.values["AMF_layer"], rtol=2e-5)
+        receipt = json.loads(outputs[i].with_suffix(".run.json").read_text())
+        assert receipt["backend"] == "HIPBatch" and receipt["case"] == case.name
+
+
+def test_autotuner_picks_a_measured_configuration(tmp_path):
+    from monki.execution.autotune import autotune
+
+    cache = tmp_path / "autotune.json"
+    result = autotune(cases(), blocks=(128, 256), shards=(8, 32), rounds=2, cache=cache)
+    assert not result["cached"] and len(result["table"]) == 4
+    assert result["seconds_per_round"] == min(row["device_seconds"] for row in result["table"])
+    again = autotune(cases(), blocks=(128, 256), shards=(8, 32), rounds=2, cache=cache)
+    assert again["cached"] and again["block"] == result["block"]

@@ -185,6 +185,46 @@ def amf_race(max_rounds: int = 256, photons_per_column: int = PHOTONS_PER_COLUMN
     return result
 
 
+def batch_rounds(rounds: int = 5, photons_per_column: int = 1_000) -> dict:
+    """The 64-scene box-cloud batch (seed 2026: SZA U(0, 70), albedo U(0.02, 0.3), cloud optical
+    depth lognormal(15, 0.5), grids 16-30 x 12-20 columns) traced one round per graph replay,
+    with block and tally shards from the autotuner."""
+
+    from monki import SceneCase, configs
+    from monki.execution import HIPBatch
+    from monki.execution.autotune import autotune
+
+    cases = [SceneCase(f"box{i}", scene, photons_per_column, rounds + 1) for i, scene in
+             enumerate(configs.box_cloud_batch(64, 2026))]  # fmt: skip
+    tuned = autotune(cases, rounds=2)
+
+    with HIPBatch(block=tuned["block"], tally_shards=tuned["tally_shards"]) as batch:
+        batch.prepare(cases)
+        batch.run_round(1)
+        before = batch.counters()
+        device = 0.0
+        start = time.perf_counter()
+
+        for index in range(2, rounds + 2):
+            batch.run_round(index)
+            device += batch.kernel_seconds()
+
+        elapsed = time.perf_counter() - start
+        counts = delta(batch.counters(), before)
+
+    events = counts["atmospheric_events"] + counts["surface_events"]
+    return {
+        "workload": "64 box-cloud scenes (seed 2026), I/Q/U, 3 views, layer paths, "
+        f"{photons_per_column} photons/column, one graph replay per round",
+        "photons_per_round": counts["photons"] // rounds,
+        "photon_events_per_s": events / elapsed,
+        "ms_per_round": 1e3 * elapsed / rounds,
+        "device_ms_per_round": 1e3 * device / rounds,
+        "autotune": {key: tuned[key] for key in ("block", "tally_shards", "seconds_per_round", "cached")},
+        "autotune_table": tuned["table"],
+    }
+
+
 COPY_BYTES = 1 << 30
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_layered_cloud.json"
 
@@ -303,6 +343,7 @@ def main() -> int:
     parser.add_argument("--copy-calibration", action="store_true", help="run the copy kernel only")
     parser.add_argument("--summarize-traffic", type=Path, metavar="DIR")
     parser.add_argument("--summarize-profile", type=Path, metavar="DB")
+    parser.add_argument("--batch", action="store_true", help="also time the 64-scene box-cloud batch")
     parser.add_argument("--guided", type=int, metavar="ROUNDS", default=0,
                         help="also race plain vs guided to 1%% AMF SE on the box cloud (round cap)")
     args = parser.parse_args()
@@ -401,6 +442,9 @@ def main() -> int:
 
     if args.guided:
         line["guided_vs_plain"] = amf_race(args.guided)
+
+    if args.batch:
+        line["batch_box_clouds"] = batch_rounds()
 
     print(json.dumps(line), flush=True)
     return 0

@@ -225,6 +225,45 @@ def batch_rounds(rounds: int = 5, photons_per_column: int = 1_000) -> dict:
     }
 
 
+def spectral_rounds(rounds: int = 2, photons_per_column: int = 16) -> dict:
+    """The spectral case: 64 wavelengths (300-500 nm) of one 128 x 128 x 60 stochastic cloud
+    field (Kolmogorov spectrum, cover 0.6, mean cloudy optical depth 12, cloud 1-4 km), I/Q/U,
+    3 views, layer paths; one graph replay per round, launch shape from the autotuner."""
+
+    from monki import SceneCase, configs
+    from monki.execution import HIPBatch
+    from monki.execution.autotune import autotune
+
+    cases = [SceneCase(f"spectral-{i:02d}", scene, photons_per_column, rounds + 1)
+             for i, scene in enumerate(configs.spectral_scenes(64))]  # fmt: skip
+    tuned = autotune(cases, blocks=(128, 256), shards=(4, 16), rounds=1)
+
+    with HIPBatch(block=tuned["block"], tally_shards=tuned["tally_shards"]) as batch:
+        batch.prepare(cases)
+        batch.run_round(1)
+        before = batch.counters()
+        device = 0.0
+        start = time.perf_counter()
+
+        for index in range(2, rounds + 2):
+            batch.run_round(index)
+            device += batch.kernel_seconds()
+
+        elapsed = time.perf_counter() - start
+        counts = delta(batch.counters(), before)
+
+    events = counts["atmospheric_events"] + counts["surface_events"]
+    return {
+        "workload": f"spectral 64 x (128 x 128 x 60), {photons_per_column} photons/column, one graph replay per round",
+        "photons_per_round": counts["photons"] // rounds,
+        "events_per_photon": events / max(counts["photons"], 1),
+        "photon_events_per_s": events / elapsed,
+        "ms_per_round": 1e3 * elapsed / rounds,
+        "device_ms_per_round": 1e3 * device / rounds,
+        "autotune": {key: tuned[key] for key in ("block", "tally_shards", "seconds_per_round", "cached")},
+    }
+
+
 COPY_BYTES = 1 << 30
 TRAFFIC_FILE = ROOT / "profiles" / "traffic_layered_cloud.json"
 
@@ -344,6 +383,7 @@ def main() -> int:
     parser.add_argument("--summarize-traffic", type=Path, metavar="DIR")
     parser.add_argument("--summarize-profile", type=Path, metavar="DB")
     parser.add_argument("--batch", action="store_true", help="also time the 64-scene box-cloud batch")
+    parser.add_argument("--spectral", action="store_true", help="also time the 64-wavelength 128x128x60 field")
     parser.add_argument("--guided", type=int, metavar="ROUNDS", default=0,
                         help="also race plain vs guided to 1%% AMF SE on the box cloud (round cap)")
     args = parser.parse_args()
@@ -445,6 +485,9 @@ def main() -> int:
 
     if args.batch:
         line["batch_box_clouds"] = batch_rounds()
+
+    if args.spectral:
+        line["spectral_field"] = spectral_rounds()
 
     print(json.dumps(line), flush=True)
     return 0

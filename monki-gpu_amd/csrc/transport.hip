@@ -1236,6 +1236,11 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
     set_error(error, size, "Packed table size or Stokes size mismatch");
     return false;
   }
+  // One tally record (layer paths, then Stokes) spans at most one wave of 64 lanes.
+  if ((s.paths ? s.nz + s.nmat : s.nmat) > 64) {
+    set_error(error, size, "Layers plus Stokes components exceed 64 per tally record");
+    return false;
+  }
   const long cells = long(s.nx) * s.ny * s.nz;
   s.xunit = t[10];
   s.yunit = t[11];

@@ -66,9 +66,9 @@ def aerosol(angles: int = 181) -> Species:
     return optics.species_from_table("aerosol_hg", aerosol_table(angles), ssalb=0.92)
 
 
-def _gas_profile(levels):
-    rayleigh = optics.exponential_layer_depths(levels, 0.70, 8.0)
-    ozone = optics.gaussian_layer_depths(levels, 0.01, 22.0, 5.0)
+def _gas_profile(levels, rayleigh_tau=0.70, ozone_tau=0.01):
+    rayleigh = optics.exponential_layer_depths(levels, rayleigh_tau, 8.0)
+    ozone = optics.gaussian_layer_depths(levels, ozone_tau, 22.0, 5.0)
     return rayleigh, ozone
 
 
@@ -86,13 +86,16 @@ def _assemble(
     xunit=1.0,
     yunit=1.0,
     save_path=True,
+    wavelength_nm=WAVELENGTH_NM,
+    rayleigh_tau=0.70,
+    ozone_tau=0.01,
 ):
     """Build tables from per-column optical depths (tau arrays shaped (nx, ny, layers))."""
 
     levels = np.asarray(levels, dtype=np.float64)
     thickness = np.diff(levels)
     physical = thickness.size
-    rayleigh, ozone = _gas_profile(levels)
+    rayleigh, ozone = _gas_profile(levels, rayleigh_tau, ozone_tau)
     gas = np.broadcast_to(rayleigh + ozone, (nx, ny, physical))
     depths = [np.broadcast_to(np.asarray(d, dtype=np.float64), (nx, ny, physical)) for d in species_depths]
     total = gas + sum(depths) if depths else gas.copy()
@@ -122,14 +125,14 @@ def _assemble(
         fractions=fractions,
         ssalb_gas=ssalb_gas,
         albedo=np.broadcast_to(np.asarray(albedo, dtype=np.float64), (nx, ny)).copy(),
-        depol=optics.rayleigh_depolarization(WAVELENGTH_NM),
+        depol=optics.rayleigh_depolarization(wavelength_nm),
         solar_zenith=float(solar_zenith),
         solar_azimuth=0.0,
         view_azimuth=0.0,
         view_zeniths=tuple(view_zeniths),
         species=list(species),
         save_path=save_path,
-        wavelength_nm=WAVELENGTH_NM,
+        wavelength_nm=wavelength_nm,
         altitude_m=1000.0 * levels,
         description=name,
     )
@@ -209,3 +212,68 @@ def box_cloud_batch(count: int = 64, seed: int = 2026):
         scenes.append(box_cloud(nx, ny, 3, sza, albedo, cot, offset=offset))
 
     return scenes
+
+
+# ---- Spectral 3-D stochastic cloud field ---------------------------------------------------
+
+# 59 physical layers + vacuum (60): 0.25 km to 5 km, 1 km to 20 km, 2 km to 50 km, then 9 to 100 km.
+SPECTRAL_LEVELS_KM = tuple(
+    np.round(np.concatenate([np.arange(0.0, 5.0, 0.25), np.arange(5.0, 20.0, 1.0), np.arange(20.0, 50.0, 2.0),
+                             np.linspace(50.0, 100.0, 10)]), 6)  # fmt: skip
+)
+
+
+def kolmogorov_field(n: int, seed: int) -> np.ndarray:
+    """Zero-mean, unit-variance periodic Gaussian field with 2-D power |k|^-8/3 (1-D k^-5/3)."""
+
+    rng = np.random.default_rng(seed)
+    k = np.fft.fftfreq(n) * n
+    kx, ky = np.meshgrid(k, k, indexing="ij")
+    magnitude = np.hypot(kx, ky)
+    amplitude = np.zeros_like(magnitude)
+    np.power(magnitude, -8.0 / 6.0, out=amplitude, where=magnitude > 0)
+    noise = np.fft.fft2(rng.standard_normal((n, n)))
+    field = np.real(np.fft.ifft2(noise * amplitude))
+    return (field - field.mean()) / field.std()
+
+
+def stochastic_cloud_field(n: int = 128, mean_tau: float = 12.0, cover: float = 0.6, sigma: float = 0.8,
+                           seed: int = 7) -> np.ndarray:
+    """Column cloud optical depth on an n x n periodic grid: a Gaussian field with a
+    Kolmogorov horizontal spectrum (1-D spectrum k^-5/3, 2-D power |k|^-8/3), thresholded to the
+    cloud cover, lognormal (exp(sigma g)) in the cloudy columns, scaled to the mean optical depth
+    of the cloudy columns."""
+
+    field = kolmogorov_field(n, seed)
+    cloudy = field > np.quantile(field, 1.0 - cover)
+    tau = np.where(cloudy, np.exp(sigma * field), 0.0)
+    return tau * (mean_tau / tau[cloudy].mean())
+
+
+def spectral_wavelengths(count: int = 64) -> np.ndarray:
+    return np.linspace(300.0, 500.0, count)
+
+
+def spectral_cloud_field(wavelength_nm: float, field: np.ndarray, nmat: int = 3, solar_zenith: float = 40.0,
+                         albedo=0.05) -> TransportScene:
+    """One wavelength of the spectral case: the field's cloud between 1 and 4 km (extinction
+    uniform in height), Rayleigh column 0.70 (lambda / 340 nm)^-4.08, a synthetic ozone column
+    0.01 exp(-(lambda - 340 nm) / 12 nm) (Huggins-band-like decrease), depolarization at the
+    wavelength. The water-cloud Mie table is the 340 nm one at every wavelength (droplet
+    scattering varies little over 300-500 nm for r_eff 10 um)."""
+
+    levels = np.asarray(SPECTRAL_LEVELS_KM)
+    n = field.shape[0]
+    cloud = cloud_layer_depths(levels, 1.0, 4.0, 1.0)[None, None, :] * field[:, :, None]
+    return _assemble(
+        f"spectral-{wavelength_nm:.1f}nm", n, field.shape[1], levels, [cloud], [water_cloud()], nmat, solar_zenith,
+        albedo, wavelength_nm=float(wavelength_nm), rayleigh_tau=0.70 * (wavelength_nm / 340.0) ** -4.08,
+        ozone_tau=0.01 * np.exp(-(wavelength_nm - 340.0) / 12.0),
+    )  # fmt: skip
+
+
+def spectral_scenes(count: int = 64, n: int = 128, seed: int = 7, nmat: int = 3) -> list:
+    """The spectral batch: one scene per wavelength, all on the same cloud field."""
+
+    field = stochastic_cloud_field(n, seed=seed)
+    return [spectral_cloud_field(w, field, nmat) for w in spectral_wavelengths(count)]

@@ -49,6 +49,9 @@ def pack_tables(scene, shards: int = 32):
     if nz > MAX_LAYERS:
         raise ValueError(f"at most {MAX_LAYERS} layers including vacuum are supported")
 
+    if scene.save_path and nz + nmat > 64:
+        raise ValueError("layers (incl. vacuum) plus Stokes components must not exceed 64 with paths")
+
     if nspecies > MAX_SPECIES:
         raise NotImplementedError(f"at most {MAX_SPECIES} tabulated species are supported")
 

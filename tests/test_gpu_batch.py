@@ -105,3 +105,25 @@ def test_autotuner_picks_a_measured_configuration(tmp_path):
     assert result["seconds_per_round"] == min(row["device_seconds"] for row in result["table"])
     again = autotune(cases(), blocks=(128, 256), shards=(8, 32), rounds=2, cache=cache)
     assert again["cached"] and again["block"] == result["block"]
+
+
+def test_spectral_batch_equals_single_wavelength_sessions():
+    """Wavelengths of the spectral stochastic field in one batch (59 layers + vacuum: the widest
+    tally record, 60 + 3 lanes) equal their single-scene runs."""
+
+    from monki import SceneCase, configs
+    from monki.execution import HIP, HIPBatch, BatchControls
+
+    field = configs.stochastic_cloud_field(24, seed=7)
+    spectral = [SceneCase(f"w{w:.0f}", configs.spectral_cloud_field(w, field), 20, 2) for w in (300.0, 400.0, 500.0)]
+
+    with HIPBatch(tally_shards=8) as batch:
+        batch.prepare(spectral)
+        batched = batch.run_round(1)
+
+    for case, result in zip(spectral, batched):
+        i = result.stokes[0]
+        assert np.all(np.isfinite(result.stokes)) and i.mean() > 0
+        with HIP(tally_shards=8) as single:
+            single.prepare(case)
+            assert_tallies_equal(result, single.run_batch(BatchControls(1, case.photons_per_column)))

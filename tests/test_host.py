@@ -439,3 +439,26 @@ def test_library_binding_declares_every_symbol():
 
     for name in declared_symbols():
         assert getattr(native, name) is not None
+
+
+def test_spectral_cloud_field_statistics():
+    """The stochastic field: Kolmogorov 1-D spectrum slope -5/3, cloud cover 0.6, mean cloudy
+    optical depth 12; the spectral scenes scale Rayleigh as lambda^-4.08."""
+
+    from monki import configs
+
+    field = configs.kolmogorov_field(256, 3)
+    power = np.abs(np.fft.fft2(field)) ** 2
+    k = np.hypot(*np.meshgrid(np.fft.fftfreq(256) * 256, np.fft.fftfreq(256) * 256, indexing="ij"))
+    shells = np.arange(4, 64)
+    spectrum = np.array([power[(k >= a) & (k < a + 1)].sum() for a in shells])
+    slope = np.polyfit(np.log(shells + 0.5), np.log(spectrum), 1)[0]
+    assert slope == pytest.approx(-5 / 3, abs=0.15)
+    tau = configs.stochastic_cloud_field(128, seed=7)
+    assert (tau > 0).mean() == pytest.approx(0.6, abs=2e-3)
+    assert tau[tau > 0].mean() == pytest.approx(12.0)
+    short, long = (configs.spectral_cloud_field(w, tau[:8, :8]) for w in (300.0, 500.0))
+    assert short.nz == 60 and len(configs.SPECTRAL_LEVELS_KM) == 60
+    gas = lambda scene: (scene.beta[0, 0, :-1] * scene.zunit[:-1] * scene.fractions[-1, 0, 0, :-1]).sum()  # noqa: E731
+    rayleigh = lambda scene: gas(scene) - 0.01 * np.exp(-(scene.wavelength_nm - 340) / 12)  # noqa: E731
+    assert rayleigh(short) / rayleigh(long) == pytest.approx((300 / 500) ** -4.08, rel=1e-9)

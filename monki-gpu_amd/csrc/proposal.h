@@ -237,10 +237,16 @@ __device__ inline void mixture_sample(const Mixture &m, float u0, float u1, floa
     cumulative += m.weight[j];
     if (u0 < cumulative && k == kComponents - 1) k = j;
   }
-  float mx = m.mx[0], my = m.my[0], mz = m.mz[0], kappa = m.kappa[0];
+  // Arithmetic selection of component k (a selected pointer into the arrays would keep the
+  // whole mixture in scratch memory).
+  float mx = 0.0f, my = 0.0f, mz = 0.0f, kappa = 0.0f;
 #pragma unroll
-  for (int j = 1; j < kComponents; ++j) {
-    if (j == k) mx = m.mx[j], my = m.my[j], mz = m.mz[j], kappa = m.kappa[j];
+  for (int j = 0; j < kComponents; ++j) {
+    const float pick = j == k ? 1.0f : 0.0f;
+    mx += pick * m.mx[j];
+    my += pick * m.my[j];
+    mz += pick * m.mz[j];
+    kappa += pick * m.kappa[j];
   }
   // Cosine to the mean: w = 1 + log(u + (1 - u) exp(-2 kappa)) / kappa, in [-1, 1].
   const float w = fminf(1.0f, fmaxf(-1.0f, 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kappa)) / kappa));

@@ -420,6 +420,15 @@ template <typename R> __device__ inline R advance(const Scene &s, Photon<R> &p) 
   return d;
 }
 
+// x reduced to [0, span): floor-based (fmod's exact remainder loop is not needed at the
+// quotients of a periodic grid, and costs far more).
+template <typename T> __device__ inline T periodic(T x, T span) {
+  T r = x - span * floor(x / span);
+  if (r < T(0)) r += span;
+  if (r >= span) r -= span;
+  return r;
+}
+
 // Move a distance along one horizontal axis inside a horizontally uniform layer: the photon
 // keeps its direction, so the cell index and the distance to the approached face follow from
 // the periodic position directly (positions in float64; a near-horizontal photon may travel
@@ -434,8 +443,7 @@ __device__ inline uint32_t slide_axis(int &index, R &remaining, R direction, R d
   const double moved = direction > R(0) ? offset + travel : u - offset + travel;
   const uint32_t crossed = uint32_t(fmin(moved / u, 4.0e9));
   const double span = u * double(count);
-  double x = fmod(double(index) * u + offset + double(direction) * double(distance), span);
-  if (x < 0.0) x += span;
+  const double x = periodic<double>(double(index) * u + offset + double(direction) * double(distance), span);
   int cell = int(x / u);
   cell = cell < 0 ? 0 : (cell >= count ? count - 1 : cell);
   const double local = fmin(fmax(x - double(cell) * u, 0.0), u);
@@ -450,8 +458,7 @@ template <typename R>
 __device__ inline int projected_column(int index, R remaining, R direction, R distance, R unit,
                                        int count) {
   const R local = direction >= R(0) ? unit - remaining : remaining;
-  R position = fmod(R(index) * unit + local + direction * distance, unit * R(count));
-  if (position < R(0)) position += unit * R(count);
+  const R position = periodic<R>(R(index) * unit + local + direction * distance, unit * R(count));
   const int column = int(position / unit);
   return column < 0 ? 0 : (column < count - 1 ? column : count - 1);
 }
@@ -891,12 +898,22 @@ __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int sha
                                   const Pending<R> &wave_pending, const R *wave_history, int row,
                                   Counts &n) {
   const int lane = __lane_id();
-  // Thickness of this lane's layer (the vacuum layer adds no path).
-  const R lane_layer = lane < s.nz - 1 ? R(s.zunit[lane]) : R(0);
   const int record = s.paths ? s.nz + s.nmat : s.nmat;
   const int stokes_at = s.paths ? s.nz : 0;
   const bool path_lane = s.paths && lane < s.nz;
-  const int plane = s.nx * s.ny;
+  const bool tally_lane = lane < record;
+  const int component = lane - stokes_at;  // Stokes component of a non-path lane
+  const size_t plane = size_t(s.nx) * s.ny;
+  // Per-view constants, outside the loop over events: 1 / mu, mu0 / mu, and this lane's
+  // full-layer detector path (the vacuum layer adds none).
+  const R lane_layer = lane < s.nz - 1 ? R(s.zunit[lane]) : R(0);
+  R inv_mu[kMaxViews], geometry[kMaxViews], layer_length[kMaxViews];
+#pragma unroll
+  for (int view = 0; view < kMaxViews; ++view) {
+    inv_mu[view] = R(s.view_inv_mu[view]);
+    geometry[view] = R(s.view_geometry[view]);
+    layer_length[view] = lane_layer * inv_mu[view];
+  }
   for (uint64_t mask = __ballot(has); mask; mask &= mask - 1) {
     const int src = __ffsll(static_cast<long long>(mask)) - 1;
     const Pending<R> source = wave_pending.at(src);
@@ -904,22 +921,18 @@ __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int sha
     const R rz = source.rz();
     R *bank = banks + s.bank_base + (size_t(source.round()) * s.shards + shard) * size_t(s.bank_count);
     const R path = path_lane ? wave_history[src * row + lane] : R(0);
-#pragma unroll 1
-    for (int view = 0; view < s.views; ++view) {
+#pragma unroll
+    for (int view = 0; view < kMaxViews; ++view) {
+      if (view >= s.views) break;
       const int pixel = source.pixel(view);
       if (pixel < 0) continue;
       const R weight = source.weight(view);
-      const R inv_mu = R(s.view_inv_mu[view]);
-      R value;
-      if (path_lane) {
-        const R length = lane == layer ? rz * inv_mu : (lane > layer ? lane_layer * inv_mu : R(0));
-        value = weight * (path + length);
-      } else {
-        const int m = lane - stokes_at;
-        value = m == 0 ? weight * R(s.view_geometry[view]) : (m == 1 ? source.q(view) : source.u(view));
-      }
-      if (lane < record && value != R(0)) {
-        add_tally<R>(bank + (size_t(view) * plane + pixel) * record + lane, value);
+      const R q = source.q(view), u = source.u(view);
+      const R length = lane == layer ? rz * inv_mu[view] : (lane > layer ? layer_length[view] : R(0));
+      const R stokes = component == 0 ? weight * geometry[view] : (component == 1 ? q : u);
+      const R value = path_lane ? weight * (path + length) : stokes;
+      if (tally_lane && value != R(0)) {
+        add_tally<R>(bank + ((size_t(view) * plane + size_t(pixel)) * record + lane), value);
         ++n.atomics;
       }
     }

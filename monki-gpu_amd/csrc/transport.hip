@@ -63,6 +63,8 @@ struct SpeciesTables {
   // the largest ratio, over node intervals, of the bound F11 (+|F12|) at a node to its
   // interval mean (the constant that makes tabulated sampling exact).
   float norm, rho;
+  // Cosines on the equal-angle grid cos(pi (1 - i / (n - 1))): brackets by acos, not search.
+  int equal_angle;
   // cosine[0, n) ascending; cosine[n, 2n) = 1 - cos; cosine[2n, 3n) = 1 + cos.
   const float *cosine, *cdf, *f11, *f22, *f33, *f12;
 };
@@ -91,6 +93,11 @@ struct Scene {
 struct Controls {
   uint32_t count, first, round, seed, max_steps, profile, per_round, rounds;
 };
+
+// sin and cos with one argument reduction.
+__device__ inline void sin_cos(float x, float *s, float *c) { sincosf(x, s, c); }
+__device__ inline void sin_cos(double x, double *s, double *c) { sincos(x, s, c); }
+
 
 // ---------------------------------------------------------------------------------------
 // Philox4x32-10 (Salmon et al. 2011, Random123 constants). Counter (photon, round, block,
@@ -171,6 +178,19 @@ __device__ inline int bracket(const float *__restrict__ axis, int n, float value
   return low;
 }
 
+// Interval of a tabulated cosine grid holding value: the equal-angle estimate from acos, moved
+// to the exact interval (the one the binary search finds) by comparisons with the nodes.
+template <typename R>
+__device__ inline int bracket_angle(const float *__restrict__ axis, int n, R value, uint32_t &reads) {
+  const R angle = acos(fmin(R(1), fmax(R(-1), value)));
+  int low = int(R(n - 1) * (R(1) - angle * R(0.31830988618379067154)));
+  low = low < 0 ? 0 : (low > n - 2 ? n - 2 : low);
+  while (low > 0 && R(axis[low]) > value) --low;
+  while (low < n - 2 && R(axis[low + 1]) <= value) ++low;
+  reads += 2;
+  return low;
+}
+
 template <typename R>
 __device__ inline int bracket_r(const float *__restrict__ axis, int n, R value, uint32_t &reads) {
   int low = 0, high = n - 1;
@@ -208,7 +228,8 @@ __device__ inline Elements<R> elements(const Scene &s, int species, R cosine, R 
   } else {
     const SpeciesTables &t = s.sp[species];
     const int n = t.lut_size;
-    const int low = bracket_r<R>(t.cosine, n, cosine, reads);
+    const int low = t.equal_angle ? bracket_angle<R>(t.cosine, n, cosine, reads)
+                                  : bracket_r<R>(t.cosine, n, cosine, reads);
     R w;
     if (cosine > R(0.5)) {
       const R o0 = R(t.cosine[n + low]), o1 = R(t.cosine[n + low + 1]);
@@ -242,11 +263,13 @@ template <typename R> struct Rotation {
 // scattered-vertical, incident-vertical and general cases. The coplanarity threshold is
 // 1e-16 in float64 (as mc.f90) and 1e-6 in float32, above the float32 spacing of azimuths up
 // to 2 pi, so exact forward/backward coplanar geometry is recognised after rounding.
+// The azimuths enter only through their difference: cos and sin of phio - phi1 (cd, sd)
+// replace the four products of cos/sin phio and phi1 of the Fortran.
 template <typename R>
-__device__ inline Rotation<R> rotation(R cosine, R uo, R u1, R phio, R phi1) {
+__device__ inline Rotation<R> rotation_delta(R cosine, R uo, R u1, R delta, R cd, R sd) {
   const R eps = sizeof(R) == 4 ? R(1e-6) : R(1e-16);
   const R pi = R(3.14159265358979323846);
-  const R dphi = fabs(phio - phi1);
+  const R dphi = fabs(delta);
   const R rem = fmod(dphi, pi);
   const bool vertical_o = sizeof(R) == 4 ? fabs(uo) == R(1) : fabs(uo) > R(1) - R(1e-16);
   const bool vertical_1 = sizeof(R) == 4 ? fabs(u1) == R(1) : fabs(u1) > R(1) - R(1e-16);
@@ -256,22 +279,21 @@ __device__ inline Rotation<R> rotation(R cosine, R uo, R u1, R phio, R phi1) {
   const R fo = R(1) - uo * uo, f1 = R(1) - u1 * u1;
   const R sino = fo < tiny ? R(0) : sqrt(fo);
   const R sin1 = f1 < tiny ? R(0) : sqrt(f1);
-  const R cpo = cos(phio), spo = sin(phio), cp1 = cos(phi1), sp1 = sin(phi1);
   R cosi1, cosi2, q;
   if (vertical_1) {
     cosi1 = copysign(R(1), u1);
-    cosi2 = -cosi1 * (cp1 * cpo + sp1 * spo);
-    q = sino * spo * cp1 - sino * cpo * sp1;
+    cosi2 = -cosi1 * cd;
+    q = sino * sd;
   } else if (vertical_o) {
     cosi2 = copysign(R(1), uo);
-    cosi1 = -cosi2 * (cp1 * cpo + sp1 * spo);
-    q = spo * sin1 * cp1 - cpo * sin1 * sp1;
+    cosi1 = -cosi2 * cd;
+    q = sin1 * sd;
   } else {
     const R e = R(1e-16);
     const R sc = R(1) - cosine * cosine;
     cosi1 = (u1 - uo * cosine) / sqrt(fmax(e, (R(1) - uo * uo) * sc));
     cosi2 = (uo - u1 * cosine) / sqrt(fmax(e, (R(1) - u1 * u1) * sc));
-    q = sino * spo * sin1 * cp1 - sino * cpo * sin1 * sp1;
+    q = sino * sin1 * sd;
   }
   // Cancellation, or the 1e-16 floor under a near-zero scattering angle, can push |cos i|
   // past one; clamp as transport.metal:158-159 does (the Fortran only warns, :1142-1167).
@@ -283,6 +305,14 @@ __device__ inline Rotation<R> rotation(R cosine, R uo, R u1, R phio, R phi1) {
   const R sini2 = g2 < tiny ? R(0) : sqrt(g2) * sign;
   return Rotation<R>{cosi1 * cosi1 - sini1 * sini1, R(2) * sini1 * cosi1,
                      cosi2 * cosi2 - sini2 * sini2, R(2) * sini2 * cosi2};
+}
+
+template <typename R>
+__device__ inline Rotation<R> rotation(R cosine, R uo, R u1, R phio, R phi1) {
+  const R delta = phio - phi1;
+  R sd, cd;
+  sin_cos(delta, &sd, &cd);
+  return rotation_delta<R>(cosine, uo, u1, delta, cd, sd);
 }
 
 // matmul_stokes_rot then P * S (mc_routines_general_mod.f90:1184-1242), I/Q/U only.
@@ -342,7 +372,10 @@ __device__ inline bool sample_direction(const Scene &s, int species, const Stoke
     }
     const R sigma = R(2) * pi * R(uniform(rng)) - pi;
     const Elements<R> f = elements<R>(s, species, cosine, one_minus, one_plus, n.lut);
-    const R pdf = f.f11 + f.f12 * (st.q * cos(R(2) * sigma) - st.u * sin(R(2) * sigma));
+    R ss, cs;
+    sin_cos(sigma, &ss, &cs);
+    // cos 2 sigma and sin 2 sigma from the half angle.
+    const R pdf = f.f11 + f.f12 * (st.q * (cs * cs - ss * ss) - st.u * (R(2) * ss * cs));
     const R bound = species < 0 ? elements<R>(s, -1, R(1), R(0), R(2), n.lut).f11
                     : Exact     ? exact_bound
                                 : f.f11 + fabs(f.f12);
@@ -353,7 +386,6 @@ __device__ inline bool sample_direction(const Scene &s, int species, const Stoke
     sampled_plus = one_plus;
     const R sin_in = sqrt(fmax(eps, R(1) - uo * uo));
     const R sin_sca = sqrt(fmax(eps, sin2));
-    const R cs = cos(sigma), ss = sin(sigma);
     u1 = clamp1(uo * cosine + sin_in * sin_sca * cs);
     // Azimuth change of the spherical triangle: the Fortran's acos((cos - uo u1) /
     // (sin_in sin_out)) written as an atan2 of its sine and cosine, which stays accurate for
@@ -686,6 +718,10 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
   // Detector ray inside the event layer: the distance to its top face (views point upward).
   const R rz = p.c < R(0) ? fmax(R(0), R(s.zunit[p.k]) - p.rz) : p.rz;
   out.rz() = rz;
+  // All views share one azimuth: cos and sin of the photon-view azimuth difference, once.
+  const R delta = p.phi - R(s.phi);
+  R sin_delta, cos_delta;
+  sin_cos(delta, &sin_delta, &cos_delta);
 #pragma unroll 1
   for (int view = 0; view < s.views; ++view) {
     out.pixel(view) = -1;
@@ -697,7 +733,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
       // Sines floored at sqrt(1e-16) as mc.f90:1112-1113 does: the forward peak of a Mie
       // table is steep enough for this to matter in float64.
       const R cosine = clamp1(p.c * mu + sqrt(fmax(eps, R(1) - p.c * p.c)) * R(s.view_sine[view]) *
-                                             cos(p.phi - R(s.phi)));
+                                             cos_delta);
       // 1 -/+ cos from the chord between the photon and view directions: accurate to the
       // direction's own rounding even within a fraction of a degree of the peaks.
       const R vx = R(s.view_a[view]), vy = R(s.view_b[view]);
@@ -706,7 +742,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
       const R one_minus = R(0.5) * (dx * dx + dy * dy + dz * dz);
       const R one_plus = R(0.5) * (ex * ex + ey * ey + ez * ez);
       const Elements<R> f = elements<R>(s, species, cosine, one_minus, one_plus, n.lut);
-      const Rotation<R> r = s.nmat > 1 ? rotation<R>(cosine, p.c, mu, p.phi, R(s.phi))
+      const Rotation<R> r = s.nmat > 1 ? rotation_delta<R>(cosine, p.c, mu, delta, cos_delta, sin_delta)
                                        : Rotation<R>{R(1), R(0), R(1), R(0)};
       prob = scatter<R>(f, r, h.st);
       prob.i *= h.weight;
@@ -1294,7 +1330,7 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
   }
   for (int sp = 0; sp < kMaxSpecies; ++sp) {
     SpeciesTables &tab = s.sp[sp];
-    tab = SpeciesTables{0, 0.0f, 1.0f, 1.0f, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    tab = SpeciesTables{0, 0.0f, 1.0f, 1.0f, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     if (sp >= s.species) continue;
     const int h = 32 + 8 * sp;
     if (!word_int(t, h, 2, 1 << 20, tab.lut_size)) {
@@ -1338,6 +1374,11 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
       return false;
     }
     tab.norm = float(area / 2.0);
+    tab.equal_angle = 1;
+    for (int a = 0; a < tab.lut_size && tab.equal_angle; ++a) {
+      const double expected = cos(3.14159265358979323846 * (1.0 - double(a) / double(tab.lut_size - 1)));
+      if (fabs(double(cs[a]) - expected) > 2e-6) tab.equal_angle = 0;
+    }
     tab.rho = float(rho * (1.0 + 1e-6));
   }
   for (int v = 0; v < s.views; ++v) {

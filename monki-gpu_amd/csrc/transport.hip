@@ -33,6 +33,7 @@ constexpr int kBlock64 = 128;
 constexpr int kCounters = 8;
 constexpr int kTraceWords = 8;
 constexpr uint32_t kSamplingAttempts = 4096;
+constexpr int kCdfBuckets = 1024;
 
 enum FailureFlag : unsigned {
   kNonfiniteProbability = 1u,
@@ -65,6 +66,9 @@ struct SpeciesTables {
   float norm, rho;
   // Cosines on the equal-angle grid cos(pi (1 - i / (n - 1))): brackets by acos, not search.
   int equal_angle;
+  // cdf_start[b], b = 0..kCdfBuckets: the CDF interval holding b / kCdfBuckets (device table
+  // built at session creation; null: binary search).
+  const uint16_t *cdf_start;
   // cosine[0, n) ascending; cosine[n, 2n) = 1 - cos; cosine[2n, 3n) = 1 + cos.
   const float *cosine, *cdf, *f11, *f22, *f33, *f12;
 };
@@ -191,6 +195,16 @@ __device__ inline int bracket_angle(const float *__restrict__ axis, int n, R val
   return low;
 }
 
+// CDF interval of a draw: the bucket table gives the interval of the bucket's lower edge, then
+// forward steps to the largest interval start <= draw (what the binary search finds).
+__device__ inline int cdf_interval(const float *__restrict__ cdf, const uint16_t *__restrict__ start, int n,
+                                   float draw, uint32_t &reads) {
+  int low = start[int(draw * float(kCdfBuckets))];
+  while (low < n - 2 && cdf[low + 1] <= draw) ++low;
+  reads += 2;
+  return low;
+}
+
 template <typename R>
 __device__ inline int bracket_r(const float *__restrict__ axis, int n, R value, uint32_t &reads) {
   int low = 0, high = n - 1;
@@ -212,8 +226,9 @@ __device__ inline int bracket_r(const float *__restrict__ axis, int n, R value, 
 // taken from 1 - cos (or 1 + cos) against the table's exact 1 - cos (1 + cos) nodes, because a
 // float32 cos cannot resolve the forward peak (F11 changes ~1e8 per unit cos there).
 template <typename R>
+// known: the table interval of cosine when the caller has it (the sampler), else -1.
 __device__ inline Elements<R> elements(const Scene &s, int species, R cosine, R one_minus,
-                                       R one_plus, uint32_t &reads) {
+                                       R one_plus, uint32_t &reads, int known = -1) {
   cosine = clamp1(cosine);
   Elements<R> f;
   if (species < 0) {
@@ -228,8 +243,9 @@ __device__ inline Elements<R> elements(const Scene &s, int species, R cosine, R 
   } else {
     const SpeciesTables &t = s.sp[species];
     const int n = t.lut_size;
-    const int low = t.equal_angle ? bracket_angle<R>(t.cosine, n, cosine, reads)
-                                  : bracket_r<R>(t.cosine, n, cosine, reads);
+    const int low = known >= 0       ? known
+                    : t.equal_angle ? bracket_angle<R>(t.cosine, n, cosine, reads)
+                                    : bracket_r<R>(t.cosine, n, cosine, reads);
     R w;
     if (cosine > R(0.5)) {
       const R o0 = R(t.cosine[n + low]), o1 = R(t.cosine[n + low + 1]);
@@ -336,7 +352,8 @@ __device__ inline Stokes<R> scatter(const Elements<R> &f, const Rotation<R> &r, 
 template <typename R, bool Exact = false>
 __device__ inline bool sample_direction(const Scene &s, int species, const Stokes<R> &st, R uo,
                                         Stream &rng, R &u1, R &dphi, R &sampled,
-                                        R &sampled_minus, R &sampled_plus, Counts &n) {
+                                        R &sampled_minus, R &sampled_plus, Counts &n,
+                                        int &sampled_interval) {
   const R pi = R(3.14159265358979323846);
   const R eps = R(1e-16);
   for (uint32_t attempt = 0; attempt < kSamplingAttempts; ++attempt) {
@@ -346,6 +363,7 @@ __device__ inline bool sample_direction(const Scene &s, int species, const Stoke
     // 1 -/+ cos tables for tabulated species.
     R cosine, sin2, one_minus, one_plus;
     R exact_bound = R(0);
+    int interval = -1;
     if (species < 0) {
       const float u = uniform(rng);
       cosine = R(2) * R(u) - R(1);
@@ -355,7 +373,9 @@ __device__ inline bool sample_direction(const Scene &s, int species, const Stoke
     } else {
       const SpeciesTables &t = s.sp[species];
       const float draw = uniform(rng);
-      const int low = bracket(t.cdf, t.lut_size, draw, n.lut);
+      const int low = t.cdf_start ? cdf_interval(t.cdf, t.cdf_start, t.lut_size, draw, n.lut)
+                                  : bracket(t.cdf, t.lut_size, draw, n.lut);
+      interval = low;
       const R d0 = R(t.cdf[low]), d1 = R(t.cdf[low + 1]);
       const R w = (R(draw) - d0) / (d1 - d0);
       const float *omc = t.cosine + t.lut_size, *opc = t.cosine + 2 * t.lut_size;
@@ -371,7 +391,7 @@ __device__ inline bool sample_direction(const Scene &s, int species, const Stoke
       }
     }
     const R sigma = R(2) * pi * R(uniform(rng)) - pi;
-    const Elements<R> f = elements<R>(s, species, cosine, one_minus, one_plus, n.lut);
+    const Elements<R> f = elements<R>(s, species, cosine, one_minus, one_plus, n.lut, interval);
     R ss, cs;
     sin_cos(sigma, &ss, &cs);
     // cos 2 sigma and sin 2 sigma from the half angle.
@@ -384,6 +404,7 @@ __device__ inline bool sample_direction(const Scene &s, int species, const Stoke
     sampled = cosine;
     sampled_minus = one_minus;
     sampled_plus = one_plus;
+    sampled_interval = interval;
     const R sin_in = sqrt(fmax(eps, R(1) - uo * uo));
     const R sin_sca = sqrt(fmax(eps, sin2));
     u1 = clamp1(uo * cosine + sin_in * sin_sca * cs);
@@ -803,6 +824,7 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
     h.st = Stokes<R>{R(1), R(0), R(0)};
   } else {
     R cosine, one_minus, one_plus;
+    int interval = -1;  // table interval of the sampled cosine (phase-function draws)
     bool from_phase = true;
     const bool guide = MODE == kGuided && ((guidance.species_mask >> (h.species + 1)) & 1u) &&
                        h.order <= guidance.max_order;
@@ -811,8 +833,10 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
       R dphi;
       const bool sampled =
           guide
-              ? sample_direction<R, true>(s, h.species, h.st, p.c, h.rng, mu, dphi, cosine, one_minus, one_plus, n)
-              : sample_direction<R, false>(s, h.species, h.st, p.c, h.rng, mu, dphi, cosine, one_minus, one_plus, n);
+              ? sample_direction<R, true>(s, h.species, h.st, p.c, h.rng, mu, dphi, cosine, one_minus, one_plus, n,
+                                          interval)
+              : sample_direction<R, false>(s, h.species, h.st, p.c, h.rng, mu, dphi, cosine, one_minus, one_plus, n,
+                                           interval);
       if (!sampled) {
         atomicOr(failure, kSamplingLimit);
         return kFailed;
@@ -837,7 +861,7 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
     }
     // The sampled scattering cosine is used directly, as in mc.f90.
     if (s.nmat > 1 || guide) {
-      const Elements<R> f = elements<R>(s, h.species, cosine, one_minus, one_plus, n.lut);
+      const Elements<R> f = elements<R>(s, h.species, cosine, one_minus, one_plus, n.lut, interval);
       const Rotation<R> r = s.nmat > 1 ? rotation<R>(cosine, p.c, mu, p.phi, phi)
                                        : Rotation<R>{R(1), R(0), R(1), R(0)};
       Stokes<R> st = scatter<R>(f, r, h.st);
@@ -1193,6 +1217,7 @@ struct Session {
   Controls *d_controls = nullptr;
   unsigned *work = nullptr;  // per-scene photon counters
   float *tables = nullptr;
+  uint16_t *buckets = nullptr;  // CDF bucket tables of every tabulated species
   void *banks = nullptr;  // float or double banks, sized for double
   size_t bank_elements = 0;  // all scenes, all round slots
   double *reduced = nullptr;
@@ -1235,7 +1260,7 @@ void release(Session *s) {
   if (s->stop) (void)hipEventDestroy(s->stop);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   // s->work points into the failure allocation.
-  void *buffers[] = {s->tables, s->banks, s->reduced, s->assembled, s->failure, s->counters, s->records,
+  void *buffers[] = {s->tables, s->buckets, s->banks, s->reduced, s->assembled, s->failure, s->counters, s->records,
                      s->d_scenes, s->d_controls};
   for (void *b : buffers) (void)hipFree(b);
   delete s;
@@ -1330,7 +1355,7 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
   }
   for (int sp = 0; sp < kMaxSpecies; ++sp) {
     SpeciesTables &tab = s.sp[sp];
-    tab = SpeciesTables{0, 0.0f, 1.0f, 1.0f, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    tab = SpeciesTables{0, 0.0f, 1.0f, 1.0f, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     if (sp >= s.species) continue;
     const int h = 32 + 8 * sp;
     if (!word_int(t, h, 2, 1 << 20, tab.lut_size)) {
@@ -1601,10 +1626,37 @@ Session *create_session(uint32_t device, const float *tables, const size_t *offs
     return nullptr;
   }
   h->work = h->failure + 1;
+  // CDF bucket tables (host, from the packed CDFs), one block per tabulated species.
+  std::vector<uint16_t> buckets;
+  std::vector<size_t> bucket_base;
+  for (uint32_t i = 0; i < scene_count; ++i) {
+    const float *t = tables + offsets[i];
+    for (int sp = 0; sp < parsed[i].species; ++sp) {
+      const SpeciesTables &tab = parsed[i].sp[sp];
+      const float *cdf = t + (tab.cdf - static_cast<const float *>(nullptr));
+      const int n = tab.lut_size;
+      bucket_base.push_back(buckets.size());
+      int low = 0;
+      for (int b = 0; b <= kCdfBuckets; ++b) {
+        const float edge = float(b) / float(kCdfBuckets);
+        while (low < n - 2 && cdf[low + 1] <= edge) ++low;
+        buckets.push_back(uint16_t(low));
+      }
+    }
+  }
+  if (!buckets.empty() &&
+      (!check(hipMalloc(&h->buckets, buckets.size() * sizeof(uint16_t)), error, error_size, "allocate buckets") ||
+       !check(hipMemcpy(h->buckets, buckets.data(), buckets.size() * sizeof(uint16_t), hipMemcpyHostToDevice),
+              error, error_size, "upload buckets"))) {
+    release(h);
+    return nullptr;
+  }
+  size_t next_bucket = 0;
   for (uint32_t i = 0; i < scene_count; ++i) {
     Scene view{};
     parse_tables(tables + offsets[i], offsets[i + 1] - offsets[i], h->tables + offsets[i], view, nullptr, 0);
     view.bank_base = parsed[i].bank_base;
+    for (int sp = 0; sp < view.species; ++sp) view.sp[sp].cdf_start = h->buckets + bucket_base[next_bucket++];
     h->scenes.push_back(view);
   }
   if (!check(hipMemcpy(h->d_scenes, h->scenes.data(), scene_count * sizeof(Scene), hipMemcpyHostToDevice), error,

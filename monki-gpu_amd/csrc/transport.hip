@@ -487,16 +487,16 @@ template <typename T> __device__ inline T periodic(T x, T span) {
 // the periodic position directly (positions in float64; a near-horizontal photon may travel
 // thousands of cells in one flight). Returns the number of faces crossed.
 template <typename R>
-__device__ inline uint32_t slide_axis(int &index, R &remaining, R direction, R distance, float unit,
+__device__ inline uint32_t slide_axis(int &index, R &remaining, R direction, double distance, float unit,
                                       int count) {
   if (direction == R(0)) return 0;
   const double u = double(unit);
   const double offset = direction > R(0) ? u - double(remaining) : double(remaining);
-  const double travel = double(fabs(direction)) * double(distance);
+  const double travel = double(fabs(direction)) * distance;
   const double moved = direction > R(0) ? offset + travel : u - offset + travel;
   const uint32_t crossed = uint32_t(fmin(moved / u, 4.0e9));
   const double span = u * double(count);
-  const double x = periodic<double>(double(index) * u + offset + double(direction) * double(distance), span);
+  const double x = periodic<double>(double(index) * u + offset + double(direction) * distance, span);
   int cell = int(x / u);
   cell = cell < 0 ? 0 : (cell >= count ? count - 1 : cell);
   const double local = fmin(fmax(x - double(cell) * u, 0.0), u);
@@ -646,12 +646,22 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
   const R eps = R(1e-16);
   Photon<R> &p = h.p;
   bool surface = false;
+  double slide = 0.0;  // flight through uniform layers not yet applied to the x/y cells
+  auto flush_slide = [&]() {
+    if (slide != 0.0) {
+      n.walk += slide_axis<R>(p.i, p.rx, p.a, slide, s.xunit, s.nx) +
+                slide_axis<R>(p.j, p.ry, p.b, slide, s.yunit, s.ny);
+      slide = 0.0;
+    }
+  };
 
   for (;;) {
     if (++h.steps > c.max_steps) {
       atomicOr(failure, kHistoryLimit);
       return kFailed;
     }
+    // Entering a varying layer: apply the deferred horizontal move before the column matters.
+    if (s.uniform[p.k] == 0.0f) flush_slide();
     const R extinction =
         (p.k < s.nz - 1 && p.k >= 0) ? R(s.beta[cell_index<R>(s, p.i, p.j, p.k)]) : R(0);
     const R event_distance = extinction > eps ? h.remains / extinction : R(INFINITY);
@@ -661,13 +671,15 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
       const R to_face = p.rz / fmax(fabs(p.c), eps);
       const bool event = event_distance < to_face;
       const R d = event ? event_distance : to_face;
-      n.walk += slide_axis<R>(p.i, p.rx, p.a, d, s.xunit, s.nx) +
-                slide_axis<R>(p.j, p.ry, p.b, d, s.yunit, s.ny);
+      // The horizontal move is deferred: one slide for the whole flight through uniform
+      // layers (the direction does not change within a flight).
+      slide += double(d);
       if (s.paths) {
         history[p.k] += d;
         h.path_total += d;
       }
       if (event) {
+        flush_slide();
         p.rz = fmax(R(0), p.rz - fabs(p.c) * d);
         break;
       }
@@ -679,6 +691,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
         p.rz = R(s.zunit[p.k]);
         continue;
       }
+      flush_slide();
       surface = true;
       p.k = 0;
       p.rz = R(0);
@@ -1101,10 +1114,13 @@ __global__ void __launch_bounds__(kBlock32)
     // The network runs when some lane's event is guided (kind and order within the guidance).
     const bool guided_event = MODE == kGuided && scatters &&
                               ((guidance.species_mask >> (h.species + 1)) & 1u) && h.order <= guidance.max_order;
-    if (MODE == kGuided && __ballot(guided_event)) {
+    const unsigned long long guided_lanes = MODE == kGuided ? __ballot(guided_event) : 0ull;
+    if (MODE == kGuided && guided_lanes) {
       float o[monki_net::kOutputs];
 #pragma unroll
       for (int nb = 0; nb < 2; ++nb) {
+        // A 32-lane half without guided events skips its network pass (uniform branch).
+        if (((guided_lanes >> (32 * nb)) & 0xffffffffull) == 0) continue;
         monki_net::bf16x4 x[16];
         monki_net::scatter_inputs(features, nb, x);
         monki_net::floatx16 raw[2];

@@ -18,8 +18,9 @@ from typing import List, Optional
 
 import numpy as np
 
-from ..input import SceneCase
+from ..input import ReferenceCase, SceneCase
 from ..metadata import file_hash, scene_metadata
+from ..preparation import UPSTREAM, prepare_case
 from .hip import DOUBLE_POINTER, FLOAT_POINTER, LIBRARY, UINT_POINTER, load_library
 from .hip_tables import pack_tables, tally_count
 from .tallies import BatchControls, RawTallies
@@ -56,6 +57,7 @@ class HIPBatch:
         block: int = 256,
         use_graph: bool = True,
         library: Path = LIBRARY,
+        upstream: Path = UPSTREAM,
     ):
         for name, value, minimum, maximum in (
             ("device", device, 0, 2**32 - 1),
@@ -77,11 +79,13 @@ class HIPBatch:
         self.block = block
         self.use_graph = bool(use_graph)
         self.library = Path(library)
+        self.upstream = Path(upstream)
         self.metadata: List = []
         self.provenance: dict = {}
         self._native = None
         self._handle = None
         self._cases: List[SceneCase] = []
+        self._preparations: List = []
         self._shapes = []
         self._counts = []
         self._round = 0
@@ -96,11 +100,18 @@ class HIPBatch:
         self.close()
         cases = list(cases)
 
-        if not cases or not all(isinstance(case, SceneCase) for case in cases):
-            raise TypeError("HIPBatch.prepare expects a non-empty list of SceneCase")
+        if not cases or not all(isinstance(case, (ReferenceCase, SceneCase)) for case in cases):
+            raise TypeError("HIPBatch.prepare expects a non-empty list of ReferenceCase or SceneCase")
 
         if any(case.backward for case in cases):
             raise NotImplementedError("HIP backward transport is not implemented")
+
+        extras, preparations = [None] * len(cases), [None] * len(cases)
+
+        for index, case in enumerate(cases):
+            if isinstance(case, ReferenceCase):
+                cases[index], prepared, preparations[index] = prepare_case(case, self.upstream, save_path=self.save_path)
+                extras[index] = prepared.product_variables()
 
         packed, shapes, counts, offsets = [], [], [], [0]
 
@@ -133,8 +144,10 @@ class HIPBatch:
         self._cases, self._shapes, self._counts = cases, shapes, counts
         self._round = 0
         self.metadata = [
-            scene_metadata(replace(case.scene, save_path=self.save_path), case.photons_per_column) for case in cases
+            scene_metadata(replace(case.scene, save_path=self.save_path), case.photons_per_column, extra=extra)
+            for case, extra in zip(cases, extras)
         ]
+        self._preparations = preparations
         self.provenance = {
             "backend": "HIPBatch",
             "device": native.monki_hip_device_name(handle).decode(),
@@ -154,7 +167,17 @@ class HIPBatch:
         }
 
     def case_provenance(self) -> list:
-        return [{**self.provenance, "case": case.name, "scene": case.scene.description} for case in self._cases]
+        records = []
+
+        for case, preparation in zip(self._cases, self._preparations):
+            record = {**self.provenance, "case": case.name, "scene": case.scene.description}
+
+            if preparation is not None:
+                record["preparation"] = preparation
+
+            records.append(record)
+
+        return records
 
     def run_round(self, round_index: int) -> List[RawTallies]:
         """One statistical round of every scene (round indices sequential from one)."""

@@ -21,6 +21,7 @@ import numpy as np
 
 from ..input import ReferenceCase, SceneCase
 from ..metadata import file_hash, scene_metadata
+from ..preparation import UPSTREAM, prepare_case
 from .backend import Backend
 from .hip_tables import pack_tables, tally_count
 from .proposal import Proposal, ProposalSettings, declare, log_rounds
@@ -130,6 +131,7 @@ class HIP(Backend):
         precision: int = 32,
         library: Path = LIBRARY,
         proposal: Optional[ProposalSettings] = None,
+        upstream: Path = UPSTREAM,
     ):
         for name, value, minimum, maximum in (
             ("device", device, 0, 2**32 - 1),
@@ -164,6 +166,7 @@ class HIP(Backend):
         self.precision = precision
         self.library = Path(library)
         self.proposal = proposal
+        self.upstream = Path(upstream)
         self.metadata = None
         self.provenance: dict = {}
         self._native = None
@@ -183,14 +186,19 @@ class HIP(Backend):
 
         self.close()
 
+        extra, preparation = None, None
+
         if isinstance(scene, ReferenceCase):
-            raise NotImplementedError(
-                "HIP prepares in-memory transport tables (SceneCase); optical preparation of "
-                "legacy positional fixtures is not part of this backend"
-            )
+            if scene.backward:
+                raise NotImplementedError("HIP backward transport is not implemented")
+
+            # The reference prepares fixtures with its Fortran worker (metal.py:98-104); the
+            # same tables come from the host restatement here.
+            scene, prepared, preparation = prepare_case(scene, self.upstream, save_path=self.save_path)
+            extra = prepared.product_variables()
 
         if not isinstance(scene, SceneCase):
-            raise TypeError("HIP.prepare expects a SceneCase")
+            raise TypeError("HIP.prepare expects a ReferenceCase or SceneCase")
 
         if scene.backward:
             raise NotImplementedError("HIP backward transport is not implemented")
@@ -227,7 +235,7 @@ class HIP(Backend):
         self._native = native
         self._handle = handle
         self._packed = packed
-        self.metadata = scene_metadata(tables, scene.photons_per_column)
+        self.metadata = scene_metadata(tables, scene.photons_per_column, extra=extra)
         self._configuration = self._settings()
         self._round = 0
         self.provenance = {
@@ -248,6 +256,9 @@ class HIP(Backend):
             "python_version": platform.python_version(),
             "gpu_dispatches": 0,
         }
+
+        if preparation is not None:
+            self.provenance["preparation"] = preparation
 
         if self.proposal is not None:
             try:

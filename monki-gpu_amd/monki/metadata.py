@@ -19,11 +19,47 @@ def file_hash(path) -> str:
     return digest.hexdigest()
 
 
-def scene_metadata(scene, photons_per_column: int, path: Path = Path("scene.nc")):
+# Report_ncdf's add_attributes calls (kernels/cpu/worker.f90:1512-1596): long_name, units and
+# plot_title of every product variable, for a TOA detector (signal "TOA radiance").
+ATTRIBUTES = {
+    "bmabs": ("Molecular absorption optical thickness", "-", "bmabs"),
+    "bmsca": ("Molecular scattering optical thickness", "-", "bmsca"),
+    "baer": ("Aerosol total optical thickness (baerabs+baersca)", "-", "baer"),
+    "wav": ("Wavelength of the light", "nm", "wav"),
+    "asurf": ("Surface albedo", "-", "asurf"),
+    "n_photons_per_column": ("Number of photons used per column and per round", "-", "n_photons_per_column"),
+    "n_rounds": ("Number of rounds used", "-", "n_rounds"),
+    "x_unit": ("Spatial resolution in the x-direction", "km", "x_unit"),
+    "y_unit": ("Spatial resolution in the y-direction", "km", "y_unit"),
+    "z_unit": ("Spatial resolution in the z-direction (layer dependent)", "km", "z_unit"),
+    "vmrgas": ("Volume mixing ratio of the absorbing gases", "ppm", "vmrgas"),
+    "pres": ("Pressure", "Pa", "pres"),
+    "temp": ("Temperature", "K", "temp"),
+    "alt": ("Altitude", "m", "alt"),
+    "theta0": ("Solar zenith angle", "deg", "solar_zenith_angle"),
+    "phi0": ("Solar azimuth angle", "deg", "solar_azimuth_angle"),
+    "phi": ("Viewing azimuth angle", "deg", "viewing_azimuth_angle"),
+    "theta": ("Viewing zenith angle", "deg", "viewing_zenith_angle"),
+    "radiance": ("TOA radiance", "-, assuming E0=pi", "radiance"),
+    "radiance_convergence": ("radiance_convergence", "-", "radiance convergence history"),
+    "std_radiance": ("Standard deviation of TOA radiance", "-", "std"),
+    "DOP": ("Degree of polarization", "-", "DOP"),
+    "AMF_layer": ("Layer AMF", "-", "AMF_layer"),
+}
+
+
+def product_attributes(name: str) -> dict:
+    long_name, units, plot_title = ATTRIBUTES[name]
+    return {"long_name": long_name, "units": units, "plot_title": plot_title}
+
+
+def scene_metadata(scene, photons_per_column: int, path: Path = Path("scene.nc"), extra=None):
     """Coordinates and placeholders of the product, as ``Report_ncdf`` defines them.
 
     Variable and dimension names follow ``kernels/cpu/worker.f90:1270-1510`` reversed into
-    C axis order (NetCDF's Python view of the Fortran declarations).
+    C axis order (NetCDF's Python view of the Fortran declarations). ``extra`` maps further
+    variable names to ``(value, dimensions, attributes)``, such as the optical inputs of a
+    prepared fixture.
     """
 
     nx, ny, nz, nmat, angles = scene.nx, scene.ny, scene.nz, scene.nmat, scene.angles
@@ -64,24 +100,16 @@ def scene_metadata(scene, photons_per_column: int, path: Path = Path("scene.nc")
         "DOP": ("nvza", "ny", "nx", "nwavs"),
         "radiance_convergence": ("nrounds", "nmat", "nwavs"),
     }
-    attributes = {
-        "theta0": {"units": "degrees", "long_name": "solar zenith angle"},
-        "theta": {"units": "degrees", "long_name": "viewing zenith angle"},
-        "phi0": {"units": "degrees", "long_name": "solar azimuth angle"},
-        "phi": {"units": "degrees", "long_name": "viewing azimuth angle"},
-        "x_unit": {"units": "km"},
-        "y_unit": {"units": "km"},
-        "z_unit": {"units": "km"},
-        "alt": {"units": "m"},
-        "wav": {"units": "nm"},
-        "radiance": {"long_name": "TOA radiance", "units": "-"},
-        "std_radiance": {"long_name": "TOA radiance standard error", "units": "-"},
-        "DOP": {"long_name": "degree of polarization", "units": "-"},
-    }
+    attributes = {name: product_attributes(name) for name in values}
 
     if scene.save_path:
         values["AMF_layer"] = np.zeros((angles, physical, ny, nx, 1))
         dimensions["AMF_layer"] = ("nvza", "nz", "ny", "nx", "nwavs")
-        attributes["AMF_layer"] = {"long_name": "Layer AMF", "units": "-"}
+        attributes["AMF_layer"] = product_attributes("AMF_layer")
+
+    for name, (value, dims, attrs) in (extra or {}).items():
+        values[name] = value
+        dimensions[name] = dims
+        attributes[name] = attrs
 
     return SimulationResult(Path(path), values, dimensions, attributes)

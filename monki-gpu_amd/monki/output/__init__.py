@@ -4,7 +4,9 @@ Same fields as the reference ``SimulationResult`` (``python/monki/output/__init_
 values, dimension names and attributes per variable, exclusive-create saving and rejection of
 missing values on load. netCDF4/HDF5 is not available here, so files are written as NetCDF-3
 64-bit-offset through ``scipy.io.netcdf_file`` with the same variable names, dimension names
-and units as the Fortran ``Report_ncdf`` product (``kernels/cpu/worker.f90:1270-1510``).
+and attributes as the Fortran ``Report_ncdf`` product (``kernels/cpu/worker.f90:1270-1510``).
+``load`` reads both that format and NetCDF-4 (HDF5) products such as the reference's archived
+outputs (``monki.output.hdf5``).
 """
 
 import os
@@ -31,6 +33,22 @@ class SimulationResult:
 
         path = Path(path).resolve()
         values, dimensions, attributes = {}, {}, {}
+
+        with open(path, "rb") as stream:
+            signature = stream.read(8)
+
+        if signature == b"\x89HDF\r\n\x1a\n":
+            from .hdf5 import read_netcdf4
+
+            for name, (data, dims, attrs) in read_netcdf4(path).items():
+                fill = attrs.get("_FillValue")
+
+                if fill is not None and np.any(data == fill):
+                    raise ValueError(f"{name}: masked/missing data in {path}")
+
+                values[name], dimensions[name], attributes[name] = data, dims, attrs
+
+            return cls(path, values, dimensions, attributes)
 
         with netcdf_file(path, "r", mmap=False) as dataset:
             for name, variable in dataset.variables.items():

@@ -191,12 +191,33 @@ def test_stale_library_is_rejected_before_native_call(monkeypatch):
         backend.prepare(SceneCase("x", configs.clear_sky(nx=2, ny=2), 10, 1))
 
 
-def test_reference_case_and_backward_are_rejected_explicitly():
+def test_reference_case_errors_precede_device_access():
+    """Fixture checks of the reference's CPU preparer (cpu.py:82-101) raise before the HIP
+    library or a device is touched (this host has neither a GPU nor needs one here)."""
+
     from monki import ReferenceCase
     from monki.execution import HIP
 
-    with HIP() as backend, pytest.raises(NotImplementedError, match="SceneCase"):
-        backend.prepare(ReferenceCase("clear", "testrun/MONKI_MLS_cloudfree_340_340_test.in"))
+    clear = "testrun/MONKI_MLS_cloudfree_340_340_test.in"
+    box = "testrun/MONKI_MLS_2boxclouds_340_340_test.in"
+
+    with HIP() as backend:
+        with pytest.raises(NotImplementedError, match="backward"):
+            backend.prepare(ReferenceCase("clear", clear, backward=True))
+
+        with pytest.raises(ValueError, match="layer AMF setting"):
+            backend.prepare(ReferenceCase("box", box))
+
+        with pytest.raises(ValueError, match="3 viewing angles"):
+            backend.prepare(ReferenceCase("clear", clear, view_zeniths=(0.0, 10.0, 20.0, 30.0)))
+
+        with pytest.raises(FileNotFoundError):
+            backend.prepare(ReferenceCase("clear", "testrun/missing.in"))
+
+        with pytest.raises(TypeError):
+            backend.prepare("clear")
+
+        assert backend.metadata is None
 
 
 # -- inputs (tests/unit/test_inputs.py) ---------------------------------------------------

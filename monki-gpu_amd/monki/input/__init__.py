@@ -1,10 +1,10 @@
 """Validated run inputs.
 
 ``ReferenceCase`` keeps the reference's legacy-fixture request and its validation
-(``python/monki/input/__init__.py:8-57``). ``SceneCase`` is the same request for a scene
-whose transport tables are already prepared in memory (``monki.scene.TransportScene``): it is
-what the HIP backend prepares, since preparation of the legacy positional fixtures is not
-part of this package.
+(``python/monki/input/__init__.py:8-57``); the HIP backends prepare it on the host with
+``monki.preparation`` from the fixture data shipped under ``monki/data/upstream``.
+``SceneCase`` is the same request for a scene whose transport tables are already prepared in
+memory (``monki.scene.TransportScene``), such as the synthetic benchmark scenes.
 """
 
 from dataclasses import dataclass

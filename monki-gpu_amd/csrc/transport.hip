@@ -79,7 +79,10 @@ struct Scene {
   int tally_count;  // output values per round: Stokes | paths | weights
   int stokes_count, path_count;
   // Device banks per round and shard: one record per (view, pixel) holding the layer paths
-  // then the Stokes components.
+  // then the Stokes components. Records start every record_stride values (the record length
+  // rounded up to 8: 32-byte aligned in float32), which keeps the atomics of one record on the
+  // fewest 32-byte sectors; the padding is never written.
+  int record_stride;
   int bank_count;
   size_t bank_base;  // first bank element of this scene in a multi-scene session
   float xunit, yunit, depol, theta0, phi0, phi;
@@ -1005,7 +1008,7 @@ __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int sha
       const R stokes = component == 0 ? weight * geometry[view] : (component == 1 ? q : u);
       const R value = path_lane ? weight * (path + length) : stokes;
       if (tally_lane && value != R(0)) {
-        add_tally<R>(bank + ((size_t(view) * plane + size_t(pixel)) * record + lane), value);
+        add_tally<R>(bank + ((size_t(view) * plane + size_t(pixel)) * s.record_stride + lane), value);
         ++n.atomics;
       }
     }
@@ -1197,13 +1200,15 @@ __global__ void assemble_tallies(Scene s, const double *__restrict__ reduced,
                                  double *__restrict__ out, int rounds) {
   const int plane = s.nx * s.ny;
   const int record = s.paths ? s.nz + s.nmat : s.nmat;
-  const size_t per_round = size_t(plane) * s.views * record;
+  const int stride = s.record_stride;
+  const size_t per_round = size_t(plane) * s.views * stride;
   const size_t index = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (index >= per_round * rounds) return;
   const size_t r = index / per_round, t = index % per_round;
-  const int slot = int(t % record);
-  const int pixel = int((t / record) % plane);
-  const int view = int(t / (size_t(record) * plane));
+  const int slot = int(t % stride);
+  if (slot >= record) return;  // record padding
+  const int pixel = int((t / stride) % plane);
+  const int view = int(t / (size_t(stride) * plane));
   const double value = reduced[r * size_t(s.bank_count) + t];
   double *dst = out + r * size_t(s.tally_count);
   const int stokes_at = s.paths ? s.nz : 0;
@@ -1431,7 +1436,8 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
   s.stokes_count = s.nmat * s.nx * s.ny * s.views;
   s.path_count = s.nx * s.ny * s.nz * s.views;
   s.tally_count = s.stokes_count + (s.paths ? s.path_count + s.nx * s.ny * s.views : 0);
-  s.bank_count = s.nx * s.ny * s.views * (s.paths ? s.nz + s.nmat : s.nmat);
+  s.record_stride = ((s.paths ? s.nz + s.nmat : s.nmat) + 7) & ~7;
+  s.bank_count = s.nx * s.ny * s.views * s.record_stride;
   return true;
 }
 

@@ -946,29 +946,13 @@ __device__ inline void event_features(const Scene &s, const History<R> &h, float
   f[11] = fminf(float(h.order), 64.0f) / 32.0f - 1.0f;
 }
 
-template <typename R> __device__ inline R broadcast(R value, int lane);
-
-template <> __device__ inline float broadcast<float>(float value, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(value), lane));
-}
-
-template <> __device__ inline double broadcast<double>(double value, int lane) {
-  const long long bits = __double_as_longlong(value);
-  const int lo = __builtin_amdgcn_readlane(int(bits & 0xffffffffll), lane);
-  const int hi = __builtin_amdgcn_readlane(int(bits >> 32), lane);
-  return __longlong_as_double((static_cast<long long>(hi) << 32) | (unsigned(lo)));
-}
-
-__device__ inline int broadcast_int(int value, int lane) {
-  return __builtin_amdgcn_readlane(value, lane);
-}
-
 // Wave-cooperative tally. The wave takes the pending events of its lanes one at a time and
 // adds, per view, one record [layer paths 0..nz-1 | Stokes 0..nmat-1] of the event's pixel in
 // a single atomic instruction: lane t adds layer t's C1 * (path_layers + path_layers_angle)
 // (mc_routines_cartesian_mod.f90:1173-1186, the photon's layer paths read from the source
 // lane's LDS row) or Stokes component t - nz. Contiguous lanes make each instruction a few
-// cache-line requests instead of one request per value.
+// cache-line requests instead of one request per value. Record offsets within a bank are
+// 32-bit: 24-bit multiplies (full rate) instead of 64-bit index arithmetic per view.
 template <typename R>
 __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int shard, bool has,
                                   const Pending<R> &wave_pending, const R *wave_history, int row,
@@ -979,16 +963,20 @@ __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int sha
   const bool path_lane = s.paths && lane < s.nz;
   const bool tally_lane = lane < record;
   const int component = lane - stokes_at;  // Stokes component of a non-path lane
-  const size_t plane = size_t(s.nx) * s.ny;
+  // Offset of this lane's slot in view v's record of pixel 0. Offsets stay below 2^31 / 8
+  // (session creation rejects larger banks) and pixel indexes below 2^24.
+  const uint32_t view_values = uint32_t(s.nx * s.ny) * uint32_t(s.record_stride);
   // Per-view constants, outside the loop over events: 1 / mu, mu0 / mu, and this lane's
   // full-layer detector path (the vacuum layer adds none).
   const R lane_layer = lane < s.nz - 1 ? R(s.zunit[lane]) : R(0);
   R inv_mu[kMaxViews], geometry[kMaxViews], layer_length[kMaxViews];
+  uint32_t lane_index[kMaxViews];
 #pragma unroll
   for (int view = 0; view < kMaxViews; ++view) {
     inv_mu[view] = R(s.view_inv_mu[view]);
     geometry[view] = R(s.view_geometry[view]);
     layer_length[view] = lane_layer * inv_mu[view];
+    lane_index[view] = uint32_t(view) * view_values + uint32_t(lane);
   }
   for (uint64_t mask = __ballot(has); mask; mask &= mask - 1) {
     const int src = __ffsll(static_cast<long long>(mask)) - 1;
@@ -997,6 +985,7 @@ __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int sha
     const R rz = source.rz();
     R *bank = banks + s.bank_base + (size_t(source.round()) * s.shards + shard) * size_t(s.bank_count);
     const R path = path_lane ? wave_history[src * row + lane] : R(0);
+    const bool at_layer = lane == layer, above = lane > layer;
 #pragma unroll
     for (int view = 0; view < kMaxViews; ++view) {
       if (view >= s.views) break;
@@ -1004,11 +993,11 @@ __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int sha
       if (pixel < 0) continue;
       const R weight = source.weight(view);
       const R q = source.q(view), u = source.u(view);
-      const R length = lane == layer ? rz * inv_mu[view] : (lane > layer ? layer_length[view] : R(0));
+      const R length = at_layer ? rz * inv_mu[view] : (above ? layer_length[view] : R(0));
       const R stokes = component == 0 ? weight * geometry[view] : (component == 1 ? q : u);
       const R value = path_lane ? weight * (path + length) : stokes;
       if (tally_lane && value != R(0)) {
-        add_tally<R>(bank + ((size_t(view) * plane + size_t(pixel)) * s.record_stride + lane), value);
+        add_tally<R>(bank + (__umul24(uint32_t(pixel), uint32_t(s.record_stride)) + lane_index[view]), value);
         ++n.atomics;
       }
     }
@@ -1437,6 +1426,11 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
   s.path_count = s.nx * s.ny * s.nz * s.views;
   s.tally_count = s.stokes_count + (s.paths ? s.path_count + s.nx * s.ny * s.views : 0);
   s.record_stride = ((s.paths ? s.nz + s.nmat : s.nmat) + 7) & ~7;
+  // The tally addresses records by 32-bit byte offsets within one shard bank (float64 build).
+  if (size_t(s.nx) * s.ny * s.views * s.record_stride * sizeof(double) >= (size_t(1) << 31)) {
+    set_error(error, size, "Tally bank of one shard exceeds 2 GiB");
+    return false;
+  }
   s.bank_count = s.nx * s.ny * s.views * s.record_stride;
   return true;
 }

@@ -127,16 +127,33 @@ __host__ __device__ inline Words philox(Words c, uint32_t k0, uint32_t k1) {
   return c;
 }
 
+// Blocks are consumed strictly in order. `next` holds the block after `words` once prefetch
+// has run: lanes of a wave use their draws out of phase, so a block generated where a lane
+// runs out would run the generator for the whole wave at almost every draw; prefetch at a
+// few wave-uniform points instead generates the look-ahead blocks of all lanes together.
 struct Stream {
-  uint32_t photon, round, seed, stream, block;
-  Words words;
-  int lane;
+  uint32_t photon, round, seed, stream, block;  // block: the next block to generate
+  Words words, next;
+  int lane;    // next word of `words` (4: used up)
+  bool ready;  // `next` holds block `block - 1`
 };
+
+__device__ inline void prefetch(Stream &rng) {
+  if (!rng.ready) {
+    rng.next = philox(Words{rng.photon, rng.round, rng.block++, rng.stream}, rng.seed, 0u);
+    rng.ready = true;
+  }
+}
 
 // Open (0,1) float32 grid of half steps: ((w >> 9) + 0.5) * 2^-23 (transport.metal:89-91).
 __device__ inline float uniform(Stream &rng) {
   if (rng.lane == 4) {
-    rng.words = philox(Words{rng.photon, rng.round, rng.block++, rng.stream}, rng.seed, 0u);
+    if (rng.ready) {
+      rng.words = rng.next;
+      rng.ready = false;
+    } else {
+      rng.words = philox(Words{rng.photon, rng.round, rng.block++, rng.stream}, rng.seed, 0u);
+    }
     rng.lane = 0;
   }
   uint32_t w = rng.lane == 0 ? rng.words.x
@@ -616,7 +633,7 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
   const uint32_t identity = c.first + id;
   h.round_offset = identity / c.per_round;
   const uint32_t photon = identity % c.per_round;
-  h.rng = Stream{photon, c.round + h.round_offset, c.seed, 0u, 0u, Words{0, 0, 0, 0}, 4};
+  h.rng = Stream{photon, c.round + h.round_offset, c.seed, 0u, 0u, Words{0, 0, 0, 0}, Words{0, 0, 0, 0}, 4, false};
   const int column = int(photon % uint32_t(s.nx * s.ny));
   Photon<R> &p = h.p;
   p.i = column % s.nx;
@@ -971,12 +988,15 @@ __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int sha
   const R lane_layer = lane < s.nz - 1 ? R(s.zunit[lane]) : R(0);
   R inv_mu[kMaxViews], geometry[kMaxViews], layer_length[kMaxViews];
   uint32_t lane_index[kMaxViews];
+  R stokes_i[kMaxViews];
+  const R stokes_q = component == 1 ? R(1) : R(0), stokes_u = component == 2 ? R(1) : R(0);
 #pragma unroll
   for (int view = 0; view < kMaxViews; ++view) {
     inv_mu[view] = R(s.view_inv_mu[view]);
     geometry[view] = R(s.view_geometry[view]);
     layer_length[view] = lane_layer * inv_mu[view];
     lane_index[view] = uint32_t(view) * view_values + uint32_t(lane);
+    stokes_i[view] = component == 0 ? geometry[view] : R(0);
   }
   for (uint64_t mask = __ballot(has); mask; mask &= mask - 1) {
     const int src = __ffsll(static_cast<long long>(mask)) - 1;
@@ -990,13 +1010,12 @@ __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int sha
     for (int view = 0; view < kMaxViews; ++view) {
       if (view >= s.views) break;
       const int pixel = source.pixel(view);
-      if (pixel < 0) continue;
       const R weight = source.weight(view);
       const R q = source.q(view), u = source.u(view);
       const R length = at_layer ? rz * inv_mu[view] : (above ? layer_length[view] : R(0));
-      const R stokes = component == 0 ? weight * geometry[view] : (component == 1 ? q : u);
+      const R stokes = weight * stokes_i[view] + q * stokes_q + u * stokes_u;
       const R value = path_lane ? weight * (path + length) : stokes;
-      if (tally_lane && value != R(0)) {
+      if (pixel >= 0 && tally_lane && value != R(0)) {
         add_tally<R>(bank + (__umul24(uint32_t(pixel), uint32_t(s.record_stride)) + lane_index[view]), value);
         ++n.atomics;
       }
@@ -1087,6 +1106,7 @@ __global__ void __launch_bounds__(kBlock32)
     }
 
     int status = kRunning;
+    prefetch(h.rng);
     if (active) status = collide<R, TRACE>(s, c, failure, history, h, pending, n, trace);
     // Lanes read each other's LDS rows below: keep every access of this wave in order.
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -1131,6 +1151,7 @@ __global__ void __launch_bounds__(kBlock32)
         if (scatters) slot = base + uint32_t(__popcll(logging & __lanemask_lt()));
       }
     }
+    prefetch(h.rng);
     if (active && status == kRunning)
       status = scatter_photon<R, TRACE, MODE>(s, failure, h, n, trace, &mixture, guidance);
     if (MODE == kLog && slot < log.capacity && status == kRunning) {

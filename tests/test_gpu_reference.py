@@ -1,0 +1,124 @@
+"""Further GPU parity cases of the reference's test_metal.py / test_cpu.py.
+
+- off-plane geometry (solar and view azimuths apart, so U carries signal) and scalar (I only)
+  transport against the reference Fortran transport (test_metal.py:143-153, :204-226);
+- the uniform-top detector shortcut against explicit cell-by-cell detector traversal of the
+  same histories (test_metal.py:357-393);
+- an adaptive AMF-convergence run equals a fixed run of its completed round count
+  (test_cpu.py:141-161).
+"""
+
+import json
+from dataclasses import replace
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def compare_with_fortran(scene, ppc, rounds, seed, tmp_path, name):
+    """Radiance of HIP simulate against the reference Fortran rounds, within combined SE."""
+
+    from monki import SceneCase, simulate
+    from monki.execution import HIP
+    from monki.validation import check_result, combined_se_statistics
+    from oracle.oracle import run_reference
+
+    gpu = simulate(SceneCase(name, scene, ppc, rounds), HIP(seed=seed), tmp_path / f"{name}.nc")
+    assert check_result(gpu) == []
+    # One thread: the reference's compiler RNG sequence is then deterministic.
+    tallies, _, _ = run_reference(scene, ppc, rounds, threads=1)
+    nmat, nx, ny, views = scene.nmat, scene.nx, scene.ny, scene.angles
+    stokes = tallies[:, : nmat * nx * ny * views].reshape(rounds, nmat, nx, ny, views, order="F") / ppc
+    mean = stokes.mean(axis=0)
+    se = stokes.std(axis=0, ddof=1) / np.sqrt(rounds)
+    a = gpu.radiance[..., 0].transpose(3, 2, 1, 0)
+    b = gpu.standard_error[..., 0].transpose(3, 2, 1, 0)
+    stats = combined_se_statistics(a, b, mean, se, floor=1e-12)
+    print(name, json.dumps(stats))
+    assert stats["max_abs_z"] < 5.0
+    assert 0.6 < stats["chi_square_per_dof"] < 1.5
+    assert stats["fraction_above_3"] < 0.01
+    return gpu, mean, se
+
+
+def test_off_plane_polarization_matches_reference_fortran(tmp_path):
+    from monki import configs
+
+    scene = configs.box_cloud(nx=8, ny=6, nmat=3, solar_zenith=50.0, box=(3, 2), offset=(2, 1))
+    scene = replace(scene, solar_azimuth=30.0, view_azimuth=120.0)
+    _, mean, se = compare_with_fortran(scene, 100, 40, 977, tmp_path, "off-plane")
+    # Off the principal plane U is a signal, not noise: the comparison above resolves it.
+    assert np.median(np.abs(mean[2]) / np.maximum(se[2], 1e-30)) > 3
+
+
+def test_scalar_transport_matches_reference_fortran(tmp_path):
+    from monki import configs
+
+    scene = configs.box_cloud(nx=8, ny=6, nmat=1, box=(3, 2), offset=(2, 1))
+    compare_with_fortran(scene, 100, 40, 4242, tmp_path, "scalar")
+
+
+@pytest.mark.parametrize("name", ["box-cloud", "layered-cloud"])
+def test_uniform_tail_matches_explicit_detector_traversal(monkeypatch, name):
+    """Forcing the explicit detector walk through every layer (uniform_top = last physical
+    layer) leaves the tallies of the same histories unchanged up to float32 rounding."""
+
+    import monki.execution.hip as hip_module
+    from monki import SceneCase, configs
+    from monki.execution import HIP, BatchControls
+
+    scene = (
+        configs.box_cloud(nx=8, ny=6, nmat=3, box=(3, 2), offset=(2, 1))
+        if name == "box-cloud"
+        else configs.layered_cloud(nx=5, ny=4, nmat=3)
+    )
+    case = SceneCase(name, scene, 300, 2)
+
+    def run():
+        with HIP(tally_shards=8) as backend:
+            backend.prepare(case)
+            return backend.run_batch(BatchControls(1, case.photons_per_column))
+
+    shortcut = run()
+    original = hip_module.pack_tables
+
+    def explicit(tables, shards):
+        packed, shape = original(tables, shards=shards)
+        assert packed[8] < tables.nz - 1
+        packed = packed.copy()
+        packed[8] = tables.nz - 1
+        return packed, shape
+
+    monkeypatch.setattr(hip_module, "pack_tables", explicit)
+    walked = run()
+    # Per view, the domain sums agree to rounding; per pixel, all but boundary cases do.
+    for a, b in ((shortcut.stokes, walked.stokes), (shortcut.weighted_paths, walked.weighted_paths)):
+        axes = tuple(range(a.ndim - 1))
+        np.testing.assert_allclose(a.sum(axis=axes), b.sum(axis=axes), rtol=1e-4)
+        close = np.abs(a - b) <= 1e-4 * np.abs(a).max() + 1e-4 * np.abs(a)
+        assert close.mean() > 0.999, close.mean()
+
+
+def test_adaptive_run_matches_fixed_run_at_completed_count(tmp_path):
+    from monki import AMFConvergence, SceneCase, configs, simulate
+    from monki.execution import HIP
+
+    scene = configs.clear_sky(nx=3, ny=2, nmat=3)
+    policy = AMFConvergence(0.02, 32, 16, 1)
+    adaptive = simulate(SceneCase("adaptive", scene, 400, 512), HIP(), tmp_path / "adaptive.nc", convergence=policy)
+    receipt = json.loads((tmp_path / "adaptive.run.json").read_text())
+    assert receipt["status"] == "converged"
+    completed = receipt["completed_rounds"]
+    assert 32 <= completed < 512 and completed % 16 == 0
+    fixed = simulate(SceneCase("fixed", scene, 400, completed), HIP(), tmp_path / "fixed.nc")
+
+    # Same Philox streams and rounds; float32 bank sums differ only in atomic order.
+    assert int(adaptive.values["n_rounds"][0]) == int(fixed.values["n_rounds"][0]) == completed
+    for variable in ("radiance", "AMF_layer", "DOP"):
+        expected = fixed.values[variable]
+        np.testing.assert_allclose(adaptive.values[variable], expected, rtol=2e-5, atol=2e-5 * np.abs(expected).max(),
+                                   err_msg=variable)  # fmt: skip
+    np.testing.assert_allclose(adaptive.values["std_radiance"], fixed.values["std_radiance"], rtol=1e-3,
+                               atol=1e-3 * np.abs(fixed.values["std_radiance"]).max())  # fmt: skip

@@ -234,24 +234,29 @@ def spectral_rounds(rounds: int = 2, photons_per_column: int = 16) -> dict:
     from monki.execution import HIPBatch
     from monki.execution.autotune import autotune
 
-    cases = [SceneCase(f"spectral-{i:02d}", scene, photons_per_column, rounds + 1)
+    cases = [SceneCase(f"spectral-{i:02d}", scene, photons_per_column, 2 * rounds + 2)
              for i, scene in enumerate(configs.spectral_scenes(64))]  # fmt: skip
     tuned = autotune(cases, blocks=(128, 256), shards=(4, 16), rounds=1)
 
     with HIPBatch(block=tuned["block"], tally_shards=tuned["tally_shards"]) as batch:
         batch.prepare(cases)
-        batch.run_round(1)
-        before = batch.counters()
-        device = 0.0
-        start = time.perf_counter()
+        # Rounds copied whole to the host (run_round), then rounds whose layer paths stay on the
+        # device (accumulate_round, what simulate_batch uses); one warm-up round each.
+        timings = {}
 
-        for index in range(2, rounds + 2):
-            batch.run_round(index)
-            device += batch.kernel_seconds()
+        for label, run, first in (("host", batch.run_round, 1), ("device", batch.accumulate_round, rounds + 2)):
+            run(first)
+            before = batch.counters()
+            device = 0.0
+            start = time.perf_counter()
 
-        elapsed = time.perf_counter() - start
-        counts = delta(batch.counters(), before)
+            for index in range(first + 1, first + rounds + 1):
+                run(index)
+                device += batch.kernel_seconds()
 
+            timings[label] = (time.perf_counter() - start, device, delta(batch.counters(), before))
+
+    elapsed, device, counts = timings["device"]
     events = counts["atmospheric_events"] + counts["surface_events"]
     return {
         "workload": f"spectral 64 x (128 x 128 x 60), {photons_per_column} photons/column, one graph replay per round",
@@ -260,6 +265,8 @@ def spectral_rounds(rounds: int = 2, photons_per_column: int = 16) -> dict:
         "photon_events_per_s": events / elapsed,
         "ms_per_round": 1e3 * elapsed / rounds,
         "device_ms_per_round": 1e3 * device / rounds,
+        "host_copy_ms_per_round": 1e3 * timings["host"][0] / rounds,
+        "round_path": "layer paths, weights and AMF moments accumulated on the device; Stokes copied per round",
         "autotune": {key: tuned[key] for key in ("block", "tally_shards", "seconds_per_round", "cached")},
     }
 

@@ -8,7 +8,8 @@
  *
  * Failure flags reported in error text (same meanings as kernels/metal/transport.metal:420-457):
  *   1 nonfinite probability, 2 detector traversal limit, 4 direction sampling limit,
- *   8 invalid Stokes vector, 16 history step limit.
+ *   8 invalid Stokes vector, 16 history step limit; and 32 nonfinite tally accumulated on the
+ *   device (monki_hip_batch_accumulate, no reference counterpart).
  */
 #ifndef MONKI_HIP_H
 #define MONKI_HIP_H
@@ -101,6 +102,23 @@ void *monki_hip_batch_create(uint32_t device, const float *tables, const size_t 
 int monki_hip_batch_run(void *handle, uint32_t round, uint32_t seed, uint32_t max_steps,
                         const uint32_t *photons, double *out, int use_graph, char *error,
                         size_t error_size);
+
+/* As monki_hip_batch_run, but the layer-path and detector-weight tallies stay on the device:
+ * per scene with layer paths, the round is added to float64 sums of the outputs and to the
+ * online (Welford) moments of the AMF estimator, and only the Stokes tallies of each scene
+ * (nmat x nx x ny x views, Fortran order, scene after scene) are copied to stokes.
+ * photons[i] = 0 skips scene i this round (nothing traced or accumulated). */
+int monki_hip_batch_accumulate(void *handle, uint32_t round, uint32_t seed, uint32_t max_steps,
+                               const uint32_t *photons, double *stokes, int use_graph, char *error,
+                               size_t error_size);
+
+/* The device accumulators of one scene (with layer paths): count = 1 + P + 3 W + 3 L values,
+ * P = nx ny nz views, W = nx ny views, L = nx ny (nz - 1) views:
+ * [rounds | path_sum (P) | weight_sum (W) | path_mean, path_m2, covariance (L each) |
+ *  weight_mean, weight_m2 (W each)], arrays in Fortran order. Zeros before the first
+ * accumulating round. */
+int monki_hip_batch_moments(void *handle, uint32_t scene, double *out, size_t count, char *error,
+                            size_t error_size);
 
 /* ---- Learned proposal (guided tracer). No reference counterpart: the reference samples
  * directions from the phase function only (mc_routines_general_mod.f90:492-932).

@@ -41,6 +41,7 @@ enum FailureFlag : unsigned {
   kSamplingLimit = 4u,
   kInvalidStokes = 8u,
   kHistoryLimit = 16u,
+  kNonfiniteTally = 32u,  // a round's path or weight tally accumulated on the device
 };
 
 enum Status : int { kRunning = 0, kEscaped = 1, kAbsorbed = 2, kFailed = 3 };
@@ -1234,6 +1235,85 @@ __global__ void assemble_tallies(Scene s, const double *__restrict__ reduced,
   }
 }
 
+// Rounds accumulated on the device (batch sessions). Per scene, in doubles:
+//   path_sum (nx, ny, nz, views) | weight_sum (nx, ny, views) |
+//   path_mean, path_m2, covariance (nx, ny, nz - 1, views) | weight_mean, weight_m2 (nx, ny, views)
+// The sums are those of the rounds' outputs; the moments are the online (Welford) moments of
+// the AMF estimator over rounds, physical layers only, each round's detector weight paired
+// with every layer of its pixel and view. Both kernels follow the host update step by step
+// without contractions, so the device moments equal the host's to the bit. Path elements run
+// first: they read the weight mean of the previous round.
+struct MomentLayout {
+  size_t path_sum, weight_sum, path_mean, path_m2, covariance, weight_mean, weight_m2, total;
+};
+
+__host__ __device__ inline MomentLayout moment_layout(const Scene &s) {
+  const size_t plane = size_t(s.nx) * s.ny;
+  const size_t paths = plane * s.nz * s.views, physical = plane * (s.nz - 1) * s.views;
+  const size_t weights = plane * s.views;
+  MomentLayout m;
+  m.path_sum = 0;
+  m.weight_sum = m.path_sum + paths;
+  m.path_mean = m.weight_sum + weights;
+  m.path_m2 = m.path_mean + physical;
+  m.covariance = m.path_m2 + physical;
+  m.weight_mean = m.covariance + physical;
+  m.weight_m2 = m.weight_mean + weights;
+  m.total = m.weight_m2 + weights;
+  return m;
+}
+
+__global__ void accumulate_paths(Scene s, const Controls *__restrict__ controls, const unsigned *__restrict__ counts,
+                                 const double *__restrict__ round, double *__restrict__ acc,
+                                 unsigned *__restrict__ failure) {
+#pragma clang fp contract(off)
+  if (controls->count == 0) return;  // scene not traced this round
+  const size_t plane = size_t(s.nx) * s.ny;
+  const size_t index = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (index >= plane * s.nz * s.views) return;
+  const MomentLayout m = moment_layout(s);
+  const double value = round[s.stokes_count + index];
+  const size_t pixel = index % plane;
+  const int k = int((index / plane) % s.nz), view = int(index / (plane * s.nz));
+  const double weight = round[s.stokes_count + s.path_count + pixel + plane * view];
+  if (!isfinite(value) || !isfinite(weight)) atomicOr(failure, kNonfiniteTally);
+  acc[m.path_sum + index] += value;
+  if (k == s.nz - 1) return;  // the vacuum layer enters the sums only
+  const double n = double(*counts + 1u);
+  const size_t e = pixel + plane * (size_t(k) + size_t(s.nz - 1) * view);
+  const double weight_mean = acc[m.weight_mean + pixel + plane * view];
+  const double path_delta = value - acc[m.path_mean + e];
+  const double weight_delta = weight - weight_mean;
+  const double path_mean = acc[m.path_mean + e] + path_delta / n;
+  const double new_weight_mean = weight_mean + weight_delta / n;
+  acc[m.path_mean + e] = path_mean;
+  acc[m.path_m2 + e] = acc[m.path_m2 + e] + path_delta * (value - path_mean);
+  acc[m.covariance + e] = acc[m.covariance + e] + path_delta * (weight - new_weight_mean);
+}
+
+__global__ void accumulate_weights(Scene s, const Controls *__restrict__ controls, const unsigned *__restrict__ counts,
+                                   const double *__restrict__ round, double *__restrict__ acc) {
+#pragma clang fp contract(off)
+  if (controls->count == 0) return;
+  const size_t weights = size_t(s.nx) * s.ny * s.views;
+  const size_t index = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (index >= weights) return;
+  const MomentLayout m = moment_layout(s);
+  const double weight = round[s.stokes_count + s.path_count + index];
+  const double n = double(*counts + 1u);
+  acc[m.weight_sum + index] += weight;
+  const double delta = weight - acc[m.weight_mean + index];
+  const double mean = acc[m.weight_mean + index] + delta / n;
+  acc[m.weight_mean + index] = mean;
+  acc[m.weight_m2 + index] = acc[m.weight_m2 + index] + delta * (weight - mean);
+}
+
+// Rounds accumulated per scene: one more for every scene traced in this round.
+__global__ void count_rounds(const Controls *__restrict__ controls, unsigned *__restrict__ counts, uint32_t scenes) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < scenes && controls[i].count) counts[i] += 1u;
+}
+
 // ---------------------------------------------------------------------------------------
 // Host side.
 // ---------------------------------------------------------------------------------------
@@ -1270,6 +1350,14 @@ struct Session {
   Guidance guidance{0.25f, ~0u, ~0u};  // guided runs: defensive fraction, kinds, orders
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
+  // Rounds accumulated on the device (batch sessions with layer paths): per-scene sums and
+  // moments (allocated by the first accumulating round), rounds per scene, and that round's
+  // captured graph.
+  double *moments = nullptr;
+  std::vector<size_t> moments_base;
+  unsigned *moment_counts = nullptr;
+  hipGraph_t accumulate_graph = nullptr;
+  hipGraphExec_t accumulate_exec = nullptr;
 };
 
 void set_error(char *error, size_t size, const char *fmt, const char *detail = "") {
@@ -1287,12 +1375,14 @@ void release(Session *s) {
   if (s->device >= 0) (void)hipSetDevice(s->device);
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
+  if (s->accumulate_exec) (void)hipGraphExecDestroy(s->accumulate_exec);
+  if (s->accumulate_graph) (void)hipGraphDestroy(s->accumulate_graph);
   if (s->start) (void)hipEventDestroy(s->start);
   if (s->stop) (void)hipEventDestroy(s->stop);
   if (s->stream) (void)hipStreamDestroy(s->stream);
   // s->work points into the failure allocation.
   void *buffers[] = {s->tables, s->buckets, s->banks, s->reduced, s->assembled, s->failure, s->counters, s->records,
-                     s->d_scenes, s->d_controls};
+                     s->d_scenes, s->d_controls, s->moments, s->moment_counts};
   for (void *b : buffers) (void)hipFree(b);
   delete s;
 }
@@ -1766,6 +1856,119 @@ int run_probe(uint32_t device, void (*kernel)(const T *, T *, int), const T *in,
 
 }  // namespace
 
+// The device work of an accumulating batch round after the transport: per scene with layer
+// paths, the sums and moments of the round's output, then the round counts.
+void enqueue_accumulate(Session *h) {
+  const uint32_t scene_count = uint32_t(h->scenes.size());
+  for (uint32_t i = 0; i < scene_count; ++i) {
+    const Scene &s = h->scenes[i];
+    if (!s.paths) continue;
+    const size_t paths = size_t(s.path_count), weights = size_t(s.nx) * s.ny * s.views;
+    double *acc = h->moments + h->moments_base[i];
+    const double *round = h->assembled + h->assembled_base[i];
+    hipLaunchKernelGGL(accumulate_paths, dim3(unsigned((paths + 255) / 256)), dim3(256), 0, h->stream, s,
+                       h->d_controls + i, h->moment_counts + i, round, acc, h->failure);
+    hipLaunchKernelGGL(accumulate_weights, dim3(unsigned((weights + 255) / 256)), dim3(256), 0, h->stream, s,
+                       h->d_controls + i, h->moment_counts + i, round, acc);
+  }
+  hipLaunchKernelGGL(count_rounds, dim3((scene_count + 255) / 256), dim3(256), 0, h->stream, h->d_controls,
+                     h->moment_counts, scene_count);
+}
+
+// One round of every scene of a batch session. Without accumulation, all output tallies are
+// copied to `out`; with it, only the Stokes part of each scene (its first stokes_count values)
+// is, and the sums and moments stay on the device (scenes with zero photons are skipped).
+int batch_round(Session *h, uint32_t round, uint32_t seed, uint32_t max_steps, const uint32_t *photons,
+                double *out, bool accumulate, int use_graph, char *error, size_t error_size) {
+  if (!h || !photons || !out || !round || !max_steps) {
+    set_error(error, error_size, "Invalid HIP batch round");
+    return 1;
+  }
+  const uint32_t scene_count = uint32_t(h->scenes.size());
+  std::vector<Controls> controls(scene_count);
+  uint64_t total = 0;
+  for (uint32_t i = 0; i < scene_count; ++i) {
+    if ((!photons[i] && !accumulate) || photons[i] > (1u << 31)) {
+      set_error(error, error_size, "Invalid photon count in a batch round");
+      return 1;
+    }
+    controls[i] = Controls{photons[i], 0u, round, seed, max_steps, 0u, photons[i] ? photons[i] : 1u, 1u};
+    total += photons[i];
+  }
+  if (!check(hipSetDevice(h->device), error, error_size, "hipSetDevice")) return 1;
+  if (accumulate && !h->moment_counts) {
+    size_t doubles = 0;
+    h->moments_base.clear();
+    for (const Scene &s : h->scenes) {
+      h->moments_base.push_back(doubles);
+      doubles += s.paths ? moment_layout(s).total : 0;
+    }
+    if (!check(hipMalloc(&h->moments, (doubles ? doubles : 1) * sizeof(double)), error, error_size,
+               "allocate round moments") ||
+        !check(hipMalloc(&h->moment_counts, scene_count * sizeof(unsigned)), error, error_size,
+               "allocate round counts"))
+      return 1;
+    launch_fill(reinterpret_cast<uint32_t *>(h->moments), (doubles ? doubles : 1) * 2, h->stream);
+    launch_fill(h->moment_counts, scene_count, h->stream);
+  }
+  if (!check(hipMemcpy(h->d_controls, controls.data(), scene_count * sizeof(Controls), hipMemcpyHostToDevice),
+             error, error_size, "upload controls"))
+    return 1;
+  const monki_net::Net net{nullptr, nullptr, nullptr};
+  const monki_net::EventLog log{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0};
+  const int block = block_size<float>(h);
+  const size_t lds = lds_bytes<float>(h, block);
+  const uint32_t grid = total ? persistent_grid<float, false, kPlain>(h, total, block, lds) : 0u;
+  hipGraph_t &graph = accumulate ? h->accumulate_graph : h->graph;
+  hipGraphExec_t &exec = accumulate ? h->accumulate_exec : h->graph_exec;
+  (void)hipEventRecord(h->start, h->stream);
+  if (use_graph && total) {
+    if (!exec) {
+      // Capture the whole round once; replays read the per-scene controls from device memory.
+      // The grid is the persistent one for the captured round's work, sized for a full CU load.
+      if (!check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), error, error_size,
+                 "begin capture"))
+        return 1;
+      const bool queued = enqueue<float, false, kPlain>(h, 1, grid, block, lds, 0, net, log, error, error_size);
+      if (queued && accumulate) enqueue_accumulate(h);
+      hipGraph_t captured = nullptr;
+      const hipError_t ended = hipStreamEndCapture(h->stream, &captured);
+      if (!queued || !check(ended, error, error_size, "end capture")) {
+        if (captured) (void)hipGraphDestroy(captured);
+        return 1;
+      }
+      graph = captured;
+      if (!check(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), error, error_size, "instantiate graph"))
+        return 1;
+    }
+    if (!check(hipGraphLaunch(exec, h->stream), error, error_size, "graph launch")) return 1;
+  } else {
+    if (!enqueue<float, false, kPlain>(h, 1, grid, block, lds, 0, net, log, error, error_size)) return 1;
+    if (accumulate) enqueue_accumulate(h);
+    if (!check(hipGetLastError(), error, error_size, "accumulation launch")) return 1;
+  }
+  (void)hipEventRecord(h->stop, h->stream);
+  if (!finish(h, error, error_size)) return 1;
+  float ms = 0.0f;
+  (void)hipEventElapsedTime(&ms, h->start, h->stop);
+  h->kernel_seconds = ms * 1e-3;
+  if (!accumulate)
+    return check(hipMemcpy(out, h->assembled, h->assembled_total * sizeof(double), hipMemcpyDeviceToHost), error,
+                 error_size, "copy tallies")
+               ? 0
+               : 1;
+  size_t offset = 0;
+  for (uint32_t i = 0; i < scene_count; ++i) {
+    const size_t count = size_t(h->scenes[i].stokes_count);
+    if (!check(hipMemcpy(out + offset, h->assembled + h->assembled_base[i], count * sizeof(double),
+                         hipMemcpyDeviceToHost),
+               error, error_size, "copy Stokes tallies"))
+      return 1;
+    offset += count;
+  }
+  return 0;
+}
+
 extern "C" {
 
 int monki_hip_abi_version(void) { return 2; }
@@ -1793,63 +1996,43 @@ void *monki_hip_batch_create(uint32_t device, const float *tables, const size_t 
 
 int monki_hip_batch_run(void *handle, uint32_t round, uint32_t seed, uint32_t max_steps, const uint32_t *photons,
                         double *out, int use_graph, char *error, size_t error_size) {
+  return batch_round(static_cast<Session *>(handle), round, seed, max_steps, photons, out, false, use_graph, error,
+                     error_size);
+}
+
+int monki_hip_batch_accumulate(void *handle, uint32_t round, uint32_t seed, uint32_t max_steps,
+                               const uint32_t *photons, double *stokes, int use_graph, char *error,
+                               size_t error_size) {
+  return batch_round(static_cast<Session *>(handle), round, seed, max_steps, photons, stokes, true, use_graph,
+                     error, error_size);
+}
+
+int monki_hip_batch_moments(void *handle, uint32_t scene, double *out, size_t count, char *error,
+                            size_t error_size) {
   Session *h = static_cast<Session *>(handle);
-  if (!h || !photons || !out || !round || !max_steps) {
-    set_error(error, error_size, "Invalid HIP batch round");
+  if (!h || !out || scene >= h->scenes.size() || !h->scenes[scene].paths) {
+    set_error(error, error_size, "Invalid HIP batch moments request");
     return 1;
   }
-  const uint32_t scene_count = uint32_t(h->scenes.size());
-  std::vector<Controls> controls(scene_count);
-  uint64_t total = 0;
-  for (uint32_t i = 0; i < scene_count; ++i) {
-    if (!photons[i] || photons[i] > (1u << 31)) {
-      set_error(error, error_size, "Invalid photon count in a batch round");
-      return 1;
-    }
-    controls[i] = Controls{photons[i], 0u, round, seed, max_steps, 0u, photons[i], 1u};
-    total += photons[i];
+  const MomentLayout m = moment_layout(h->scenes[scene]);
+  if (count != 1 + m.total) {
+    set_error(error, error_size, "Moments buffer size does not match the scene");
+    return 1;
   }
+  if (!h->moment_counts) {  // no accumulating round yet
+    for (size_t i = 0; i < count; ++i) out[i] = 0.0;
+    return 0;
+  }
+  unsigned rounds = 0;
   if (!check(hipSetDevice(h->device), error, error_size, "hipSetDevice") ||
-      !check(hipMemcpy(h->d_controls, controls.data(), scene_count * sizeof(Controls), hipMemcpyHostToDevice),
-             error, error_size, "upload controls"))
+      !check(hipMemcpy(&rounds, h->moment_counts + scene, sizeof(unsigned), hipMemcpyDeviceToHost), error,
+             error_size, "copy round count") ||
+      !check(hipMemcpy(out + 1, h->moments + h->moments_base[scene], m.total * sizeof(double),
+                       hipMemcpyDeviceToHost),
+             error, error_size, "copy moments"))
     return 1;
-  const monki_net::Net net{nullptr, nullptr, nullptr};
-  const monki_net::EventLog log{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0};
-  const int block = block_size<float>(h);
-  const size_t lds = lds_bytes<float>(h, block);
-  const uint32_t grid = persistent_grid<float, false, kPlain>(h, total, block, lds);
-  (void)hipEventRecord(h->start, h->stream);
-  if (use_graph) {
-    if (!h->graph_exec) {
-      // Capture the whole round once; replays read the per-scene controls from device memory.
-      if (!check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), error, error_size,
-                 "begin capture"))
-        return 1;
-      const bool queued = enqueue<float, false, kPlain>(h, 1, grid, block, lds, 0, net, log, error, error_size);
-      hipGraph_t graph = nullptr;
-      const hipError_t ended = hipStreamEndCapture(h->stream, &graph);
-      if (!queued || !check(ended, error, error_size, "end capture")) {
-        if (graph) (void)hipGraphDestroy(graph);
-        return 1;
-      }
-      h->graph = graph;
-      if (!check(hipGraphInstantiate(&h->graph_exec, h->graph, nullptr, nullptr, 0), error, error_size,
-                 "instantiate graph"))
-        return 1;
-    }
-    if (!check(hipGraphLaunch(h->graph_exec, h->stream), error, error_size, "graph launch")) return 1;
-  } else if (!enqueue<float, false, kPlain>(h, 1, grid, block, lds, 0, net, log, error, error_size)) {
-    return 1;
-  }
-  (void)hipEventRecord(h->stop, h->stream);
-  if (!finish(h, error, error_size)) return 1;
-  float ms = 0.0f;
-  (void)hipEventElapsedTime(&ms, h->start, h->stop);
-  h->kernel_seconds = ms * 1e-3;
-  return check(hipMemcpy(out, h->assembled, h->assembled_total * sizeof(double), hipMemcpyDeviceToHost), error,
-               error_size, "copy tallies")
-             ? 0
-             : 1;
+  out[0] = double(rounds);
+  return 0;
 }
 
 int monki_hip_run(void *handle, const uint32_t *controls, double *out, char *error, size_t error_size) {

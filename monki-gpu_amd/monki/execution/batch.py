@@ -12,7 +12,7 @@ change between rounds. ``simulate_batch`` consumes it.
 import ctypes
 import hashlib
 import platform
-from dataclasses import replace
+from dataclasses import dataclass, replace
 from pathlib import Path
 from typing import List, Optional
 
@@ -37,6 +37,41 @@ def declare(native) -> None:
         ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t,
     ]  # fmt: skip
     native.monki_hip_batch_run.restype = ctypes.c_int
+    native.monki_hip_batch_accumulate.argtypes = native.monki_hip_batch_run.argtypes
+    native.monki_hip_batch_accumulate.restype = ctypes.c_int
+    native.monki_hip_batch_moments.argtypes = [
+        ctypes.c_void_p, ctypes.c_uint32, DOUBLE_POINTER, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t,
+    ]  # fmt: skip
+    native.monki_hip_batch_moments.restype = ctypes.c_int
+
+
+@dataclass
+class DeviceMoments:
+    """Sums and online AMF moments of one scene's rounds, accumulated on the device.
+
+    ``rounds`` rounds; ``paths`` (nx, ny, nz, views) and ``weights`` (nx, ny, views) are the
+    sums of the rounds' outputs; the moments have AMFMoments' meaning and shapes (physical
+    layers only, the weight moments broadcast over layers).
+    """
+
+    rounds: int
+    paths: np.ndarray
+    weights: np.ndarray
+    path_mean: np.ndarray
+    path_m2: np.ndarray
+    covariance: np.ndarray
+    weight_mean: np.ndarray
+    weight_m2: np.ndarray
+
+    def amf_moments(self):
+        from ..convergence import AMFMoments  # convergence imports this package
+
+        moments = AMFMoments()
+        moments.count = self.rounds
+        moments.path_mean, moments.path_m2, moments.covariance = self.path_mean, self.path_m2, self.covariance
+        moments.weight_mean = self.weight_mean[:, :, None, :]
+        moments.weight_m2 = self.weight_m2[:, :, None, :]
+        return moments
 
 
 class HIPBatch:
@@ -225,6 +260,92 @@ class HIPBatch:
             results.append(RawTallies(BatchControls(round_index, case.photons_per_column), stokes, paths, weights))
 
         return results
+
+    def accumulate_round(self, round_index: int, active=None) -> List[Optional[np.ndarray]]:
+        """One round of every active scene with the path and weight tallies kept on the device.
+
+        Returns each scene's Stokes tallies (nmat, nx, ny, views) for this round, ``None`` for
+        a scene that is not active (``active[i]`` false: nothing traced or accumulated). The
+        sums and AMF moments are read with ``moments``. Round indices are sequential from one,
+        shared with ``run_round``.
+        """
+
+        if self._handle is None:
+            raise RuntimeError("HIPBatch must be prepared before running a round")
+
+        if round_index != self._round + 1:
+            raise ValueError("round_index must be sequential, starting at one")
+
+        active = [True] * len(self._cases) if active is None else [bool(a) for a in active]
+
+        if len(active) != len(self._cases):
+            raise ValueError("one active flag per case is required")
+
+        photons = [
+            case.photons_per_column * shape[1] * shape[2] if on else 0
+            for case, shape, on in zip(self._cases, self._shapes, active)
+        ]  # fmt: skip
+        words = (ctypes.c_uint32 * len(photons))(*photons)
+        sizes = [shape[0] * shape[1] * shape[2] * shape[4] for shape in self._shapes]
+        stokes = np.zeros(sum(sizes), dtype=np.float64)
+        error = ctypes.create_string_buffer(8192)
+
+        try:
+            status = self._native.monki_hip_batch_accumulate(
+                self._handle, round_index, self.seed, self.max_steps, words, stokes.ctypes.data_as(DOUBLE_POINTER),
+                int(self.use_graph), error, len(error),
+            )  # fmt: skip
+
+            if status or not np.all(np.isfinite(stokes)):
+                raise RuntimeError(error.value.decode() or "Nonfinite HIP batch tallies")
+        except BaseException:
+            self.close()
+            raise
+
+        self._round = round_index
+        self.provenance["gpu_rounds"] += 1
+        results, start = [], 0
+
+        for shape, size, on in zip(self._shapes, sizes, active):
+            values = stokes[start : start + size]
+            start += size
+            nmat, nx, ny, _, views = shape
+            results.append(values.reshape((nmat, nx, ny, views), order="F").copy() if on else None)
+
+        return results
+
+    def moments(self, number: int) -> DeviceMoments:
+        """The device sums and AMF moments of case ``number`` (layer paths required)."""
+
+        if self._handle is None:
+            raise RuntimeError("HIPBatch must be prepared before reading moments")
+
+        if not self.save_path:
+            raise ValueError("device moments need layer path diagnostics (save_path)")
+
+        _, nx, ny, nz, views = self._shapes[number]
+        p, w, layers = nx * ny * nz * views, nx * ny * views, nx * ny * (nz - 1) * views
+        out = np.zeros(1 + p + 3 * w + 3 * layers, dtype=np.float64)
+        error = ctypes.create_string_buffer(8192)
+
+        if self._native.monki_hip_batch_moments(
+            self._handle, number, out.ctypes.data_as(DOUBLE_POINTER), out.size, error, len(error)
+        ):  # fmt: skip
+            raise RuntimeError(error.value.decode() or "HIP batch moments failed")
+
+        cuts = np.cumsum([1, p, w, layers, layers, layers, w])
+        parts = np.split(out, cuts)
+        full, weight_shape, physical = (nx, ny, nz, views), (nx, ny, views), (nx, ny, nz - 1, views)
+        return DeviceMoments(
+            int(parts[0][0]),
+            parts[1].reshape(full, order="F"),
+            parts[2].reshape(weight_shape, order="F"),
+            parts[3].reshape(physical, order="F"),
+            parts[4].reshape(physical, order="F"),
+            parts[5].reshape(physical, order="F"),
+            parts[6].reshape(weight_shape, order="F"),
+            parts[7].reshape(weight_shape, order="F"),
+        )
 
     def kernel_seconds(self) -> float:
         """Device seconds of the last round (HIP events around the launch or graph replay)."""

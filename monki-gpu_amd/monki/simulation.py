@@ -61,31 +61,14 @@ class RoundStatistics:
     def add(self, batch) -> None:
         """Accumulate the next round (round indices 1, 2, ... in order)."""
 
-        case = self.case
-        index = self.completed + 1
-
         if not isinstance(batch, RawTallies):
             raise TypeError("Backend must return RawTallies")
 
-        if batch.controls != BatchControls(index, case.photons_per_column):
-            raise ValueError("Backend returned tallies for a different round")
+        if self.total is None and batch.weighted_paths is not None:
+            self.paths = np.zeros_like(batch.weighted_paths)
+            self.weights = np.zeros_like(batch.weights)
 
-        if self.total is None:
-            self.total = np.zeros_like(batch.stokes)
-            self.rounds_sum = np.zeros_like(batch.stokes)
-            self.rounds_squared = np.zeros_like(batch.stokes)
-
-            if batch.weighted_paths is not None:
-                self.paths = np.zeros_like(batch.weighted_paths)
-                self.weights = np.zeros_like(batch.weights)
-
-        self.total += batch.stokes
-        normalized = batch.stokes / case.photons_per_column
-        self.rounds_sum += normalized
-        self.rounds_squared += normalized**2
-        _, nx, ny, angles = self.total.shape
-        # Successive spatial sums and divisions as in the original accumulate.
-        self.history.append(self.rounds_sum.sum(axis=1).sum(axis=1).sum(axis=1) / nx / ny / angles / index)
+        self.add_stokes(batch.controls, batch.stokes)
 
         if self.paths is not None:
             if batch.weighted_paths is None or batch.weights is None:
@@ -93,6 +76,41 @@ class RoundStatistics:
 
             self.paths += batch.weighted_paths
             self.weights += batch.weights
+
+    def add_stokes(self, controls, stokes) -> None:
+        """Accumulate the Stokes tallies of the next round; the layer paths of device-accumulated
+        rounds arrive through ``attach``."""
+
+        case = self.case
+        index = self.completed + 1
+
+        if controls != BatchControls(index, case.photons_per_column):
+            raise ValueError("Backend returned tallies for a different round")
+
+        if self.total is None:
+            self.total = np.zeros_like(stokes)
+            self.rounds_sum = np.zeros_like(stokes)
+            self.rounds_squared = np.zeros_like(stokes)
+
+        self.total += stokes
+        normalized = stokes / case.photons_per_column
+        self.rounds_sum += normalized
+        self.rounds_squared += normalized**2
+        _, nx, ny, angles = self.total.shape
+        # Successive spatial sums and divisions as in the original accumulate.
+        self.history.append(self.rounds_sum.sum(axis=1).sum(axis=1).sum(axis=1) / nx / ny / angles / index)
+
+    def attach(self, moments) -> None:
+        """Take the layer-path sums and AMF moments a batch backend accumulated on the device
+        (``HIPBatch.moments``) for the rounds added so far."""
+
+        if moments.rounds != self.completed:
+            raise ValueError("Device moments cover a different number of rounds")
+
+        self.paths, self.weights = moments.paths, moments.weights
+
+        if self.stopping is not None:
+            self.stopping.moments = moments.amf_moments()
 
     def finish(self, backend_provenance: dict) -> SimulationResult:
         case, metadata, output = self.case, self.metadata, self.output
@@ -242,28 +260,49 @@ def simulate_batch(cases, backend, outputs, *, convergence: Optional[AMFConverge
             for case, metadata, output in zip(cases, backend.metadata, outputs)
         ]
         running = [True] * len(cases)
+        # Backends that keep the layer paths on the device return only Stokes per round; the
+        # sums and AMF moments are read at each precision check and at the end.
+        on_device = hasattr(backend, "accumulate_round")
+        paths = ["AMF_layer" in metadata.values for metadata in backend.metadata]
 
         for index in range(1, max(case.rounds for case in cases) + 1):
             if not any(running):
                 break
 
-            batches = backend.run_round(index)
+            if on_device:
+                rounds = backend.accumulate_round(index, active=running)
+            else:
+                rounds = backend.run_round(index)
 
-            for number, (stats, batch) in enumerate(zip(statistics, batches)):
+            for number, (stats, batch) in enumerate(zip(statistics, rounds)):
                 if not running[number]:
                     continue
 
-                stats.add(batch)
                 stopping = stats.stopping
 
+                if on_device:
+                    stats.add_stokes(BatchControls(index, stats.case.photons_per_column), batch)
+                else:
+                    stats.add(batch)
+
+                    if stopping is not None:
+                        stopping.moments.add(batch)
+
                 if stopping is not None:
-                    stopping.moments.add(batch)
                     group_end = index % stopping.policy.check_every == 0 or index == stats.case.rounds
+
+                    if group_end and on_device and index >= stopping.policy.minimum_rounds:
+                        stats.attach(backend.moments(number))
 
                     if group_end and stopping.check(index):
                         running[number] = False
 
                 if index >= stats.case.rounds:
                     running[number] = False
+
+        if on_device:
+            for number, stats in enumerate(statistics):
+                if paths[number]:
+                    stats.attach(backend.moments(number))
 
         return [stats.finish(provenance) for stats, provenance in zip(statistics, backend.case_provenance())]

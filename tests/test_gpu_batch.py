@@ -127,3 +127,107 @@ def test_spectral_batch_equals_single_wavelength_sessions():
         with HIP(tally_shards=8) as single:
             single.prepare(case)
             assert_tallies_equal(result, single.run_batch(BatchControls(1, case.photons_per_column)))
+
+
+class HostRounds:
+    """A HIPBatch seen without its device accumulation: simulate_batch then copies every round's
+    tallies to the host."""
+
+    def __init__(self, batch):
+        self.batch = batch
+
+    def __getattr__(self, name):
+        if name == "accumulate_round":
+            raise AttributeError(name)
+
+        return getattr(self.batch, name)
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.batch.close()
+
+
+def test_device_moments_equal_host_moments():
+    """Sums and AMF moments accumulated on the device equal the host's over the same rounds
+    (two sessions, same Philox streams: float32 bank sums differ only in atomic order)."""
+
+    from monki.convergence import AMFMoments
+    from monki.execution import HIPBatch
+
+    batch_cases = cases(rounds=6)
+    rounds = 6
+
+    with HIPBatch(tally_shards=8) as host, HIPBatch(tally_shards=8) as device:
+        host.prepare(batch_cases)
+        device.prepare(batch_cases)
+        sums = [None] * len(batch_cases)
+        moments = [AMFMoments() for _ in batch_cases]
+
+        for index in range(1, rounds + 1):
+            stokes = device.accumulate_round(index)
+
+            for number, (full, alone) in enumerate(zip(host.run_round(index), stokes)):
+                np.testing.assert_allclose(alone, full.stokes, rtol=2e-5, atol=2e-5 * np.abs(full.stokes).max())
+                moments[number].add(full)
+                previous = sums[number]
+                sums[number] = (full.weighted_paths, full.weights) if previous is None else (
+                    previous[0] + full.weighted_paths, previous[1] + full.weights)  # fmt: skip
+
+        for number in range(len(batch_cases)):
+            got = device.moments(number)
+            assert got.rounds == rounds
+            reference = moments[number]
+            pairs = [
+                (got.paths, sums[number][0]), (got.weights, sums[number][1]),
+                (got.path_mean, reference.path_mean), (got.weight_mean, reference.weight_mean[:, :, 0, :]),
+            ]  # fmt: skip
+            for a, b in pairs:
+                np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-4 * np.abs(b).max())
+            # Second moments: differences of rounds, at the scale of the variance.
+            for a, b in ((got.path_m2, reference.path_m2), (got.covariance, reference.covariance),
+                         (got.weight_m2, reference.weight_m2[:, :, 0, :])):  # fmt: skip
+                np.testing.assert_allclose(a, b, rtol=1e-2, atol=1e-3 * np.abs(b).max())
+
+
+def test_inactive_scenes_are_neither_traced_nor_accumulated():
+    from monki.execution import HIPBatch
+
+    batch_cases = cases(rounds=3)
+
+    with HIPBatch(tally_shards=8) as batch:
+        batch.prepare(batch_cases)
+        first = batch.accumulate_round(1)
+        frozen = batch.moments(1)
+        second = batch.accumulate_round(2, active=[True, False, True, True])
+        assert second[1] is None and all(r is not None for r in first)
+        again = batch.moments(1)
+        assert again.rounds == 1 == frozen.rounds
+        np.testing.assert_array_equal(again.paths, frozen.paths)
+        np.testing.assert_array_equal(again.path_m2, frozen.path_m2)
+        assert batch.moments(0).rounds == 2
+
+
+def test_simulate_batch_on_device_equals_host_rounds(tmp_path):
+    """simulate_batch with AMF stopping: device-accumulated rounds give the products of the
+    host path (scenes stop at their own round counts)."""
+
+    from monki import AMFConvergence, simulate_batch
+    from monki.execution import HIPBatch
+
+    batch_cases = cases(rounds=64, ppc=100)
+    policy = AMFConvergence(0.2, 32, 16, 1)
+    device = simulate_batch(batch_cases, HIPBatch(), [tmp_path / f"d{i}.nc" for i in range(len(batch_cases))],
+                            convergence=policy)  # fmt: skip
+    host = simulate_batch(batch_cases, HostRounds(HIPBatch()), [tmp_path / f"h{i}.nc" for i in range(len(batch_cases))],
+                          convergence=policy)  # fmt: skip
+
+    for i, (a, b) in enumerate(zip(device, host)):
+        da = json.loads((tmp_path / f"d{i}.run.json").read_text())
+        db = json.loads((tmp_path / f"h{i}.run.json").read_text())
+        assert da["completed_rounds"] == db["completed_rounds"] and da["status"] == db["status"]
+        for variable in ("radiance", "AMF_layer", "std_AMF_layer"):
+            expected = b.values[variable]
+            np.testing.assert_allclose(a.values[variable], expected, rtol=1e-4, atol=1e-4 * np.abs(expected).max(),
+                                       err_msg=variable)  # fmt: skip

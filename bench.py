@@ -86,7 +86,7 @@ def cpu_baseline(scene, events_per_photon: float, target_seconds: float = 15.0) 
 
 
 def time_to_amf_se(scene, max_rounds: int = 1024, photons_per_column: int = PHOTONS_PER_COLUMN,
-                   name: str = "layered-cloud", proposal=None) -> dict:
+                   name: str = "layered-cloud", proposal=None, directed=None) -> dict:
     """Wall time of simulate(..., convergence=AMFConvergence(0.01, 128, 32, 3)) to converged."""
 
     import tempfile
@@ -99,7 +99,7 @@ def time_to_amf_se(scene, max_rounds: int = 1024, photons_per_column: int = PHOT
         """Delegates to HIP and prints one line per group of rounds (the run takes minutes)."""
 
         def __init__(self):
-            self.inner = HIP(proposal=proposal)
+            self.inner = HIP(proposal=proposal, directed=directed)
             self.rounds = 0
             self.start = time.perf_counter()
 

@@ -180,6 +180,17 @@ int monki_hip_attach_proposal(void *handle, void *proposal, float alpha, uint32_
 int monki_hip_run_guided(void *handle, const uint32_t *controls, double *out, char *error,
                          size_t error_size);
 
+/* Detector-directed sampling, no network: directions from the phase function with
+ * probability alpha, else from von Mises-Fisher lobes of concentration kappa (1 to 1e6)
+ * around the view directions (equal weights); weights times p / (alpha p + (1 - alpha) q).
+ * Applies to the events of the kinds in species_mask up to scattering order max_order, as
+ * monki_hip_attach_proposal; replaces any attached proposal's guidance settings.
+ * monki_hip_run_directed is monki_hip_run with it. */
+int monki_hip_set_directed(void *handle, float alpha, float kappa, uint32_t species_mask,
+                           uint32_t max_order, char *error, size_t error_size);
+int monki_hip_run_directed(void *handle, const uint32_t *controls, double *out, char *error,
+                           size_t error_size);
+
 /* monki_hip_trace for the guided tracer (float32), same record layout. */
 int monki_hip_trace_guided(void *handle, const uint32_t *controls, double *out, float *records,
                            size_t record_count, char *error, size_t error_size);

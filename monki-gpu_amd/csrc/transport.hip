@@ -48,7 +48,9 @@ enum Status : int { kRunning = 0, kEscaped = 1, kAbsorbed = 2, kFailed = 3 };
 
 // Transport modes: plain (the reference algorithm), logging of scattering events for the
 // proposal's training, and guided sampling with the learned proposal.
-enum Mode : int { kPlain = 0, kLog = 1, kGuided = 2 };
+// Detector-directed sampling draws from vMF lobes around the view directions instead of a
+// network's mixture.
+enum Mode : int { kPlain = 0, kLog = 1, kGuided = 2, kDirected = 3 };
 
 // Which events the guided mode guides: scatterer kinds (bit 0 gas, bit s + 1 tabulated species
 // s) up to a scattering order; alpha = probability of the phase-function component.
@@ -56,6 +58,7 @@ struct Guidance {
   float alpha;
   uint32_t species_mask;
   uint32_t max_order;
+  float kappa;  // concentration of the detector-directed lobes
 };
 
 struct SpeciesTables {
@@ -835,6 +838,41 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
   return kRunning;
 }
 
+// Proposals of the steered modes: the learned network's vMF mixture (per event), or lobes of
+// one concentration around the view directions, equally weighted (per scene): detector-
+// directed sampling, which sends photons toward the detectors more often so that the forward
+// peak of a tabulated phase function meets the local estimate with a matching, smaller weight.
+struct Lobes {
+  const Scene *s;
+  float kappa;
+};
+
+__device__ inline void steer_sample(const monki_net::Mixture &m, float u0, float u1, float u2, float &wx,
+                                    float &wy, float &wz) {
+  monki_net::mixture_sample(m, u0, u1, u2, wx, wy, wz);
+}
+
+__device__ inline float steer_density(const monki_net::Mixture &m, float wx, float wy, float wz) {
+  return monki_net::mixture_density(m, wx, wy, wz);
+}
+
+__device__ inline void steer_sample(const Lobes &l, float u0, float u1, float u2, float &wx, float &wy,
+                                    float &wz) {
+  const Scene &s = *l.s;
+  const int v = min(int(u0 * float(s.views)), s.views - 1);
+  monki_net::vmf_sample(float(s.view_a[v]), float(s.view_b[v]), float(s.view_mu[v]), l.kappa, u1, u2, wx, wy, wz);
+}
+
+__device__ inline float steer_density(const Lobes &l, float wx, float wy, float wz) {
+  const Scene &s = *l.s;
+  float q = 0.0f;
+  for (int v = 0; v < s.views; ++v) {
+    const float dot = float(s.view_a[v]) * wx + float(s.view_b[v]) * wy + float(s.view_mu[v]) * wz;
+    q += expf(monki_net::log_vmf(l.kappa, dot));
+  }
+  return q / float(s.views);
+}
+
 // Continuation direction after the event (mc.f90:1266-1431; GROUND_LAMB2 at the surface).
 // Guided (atmospheric events): with probability alpha the direction comes from the phase
 // function (exact sampler), otherwise from the learned vMF mixture; either way the weight is
@@ -842,10 +880,10 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
 // for the photon's Stokes vector and q the mixture's, which keeps the estimator unbiased and
 // the factor below 1 / alpha.
 // Events outside the guidance (kind or order) are sampled exactly as in the plain mode.
-template <typename R, bool TRACE, int MODE = kPlain>
+template <typename R, bool TRACE, int MODE = kPlain, typename Proposal = monki_net::Mixture>
 __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, Counts &n,
-                              Trace<R> &trace, const monki_net::Mixture *mixture = nullptr,
-                              Guidance guidance = Guidance{1.0f, 0u, 0u}) {
+                              Trace<R> &trace, const Proposal *mixture = nullptr,
+                              Guidance guidance = Guidance{1.0f, 0u, 0u, 0.0f}) {
   const float alpha = guidance.alpha;
   const R pi = R(3.14159265358979323846);
   Photon<R> &p = h.p;
@@ -860,8 +898,8 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
     R cosine, one_minus, one_plus;
     int interval = -1;  // table interval of the sampled cosine (phase-function draws)
     bool from_phase = true;
-    const bool guide = MODE == kGuided && ((guidance.species_mask >> (h.species + 1)) & 1u) &&
-                       h.order <= guidance.max_order;
+    const bool guide = (MODE == kGuided || MODE == kDirected) &&
+                       ((guidance.species_mask >> (h.species + 1)) & 1u) && h.order <= guidance.max_order;
     if (guide) from_phase = uniform(h.rng) < alpha;
     if (from_phase) {
       R dphi;
@@ -882,7 +920,7 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
     } else {
       float wx, wy, wz;
       const float u0 = uniform(h.rng), u1 = uniform(h.rng), u2 = uniform(h.rng);
-      monki_net::mixture_sample(*mixture, u0, u1, u2, wx, wy, wz);
+      steer_sample(*mixture, u0, u1, u2, wx, wy, wz);
       da = R(wx), db = R(wy), dc = R(wz);
       direct = true;
       mu = dc;
@@ -907,7 +945,7 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
         }
         const float table_norm = h.species < 0 ? 1.0f : s.sp[h.species].norm;
         const R density = norm / (pi * R(table_norm));
-        const R q = R(monki_net::mixture_density(*mixture, float(da), float(db), float(dc)));
+        const R q = R(steer_density(*mixture, float(da), float(db), float(dc)));
         h.weight *= density / (R(alpha) * density + R(1.0f - alpha) * q);
       }
       if (s.nmat > 1) {
@@ -1119,7 +1157,7 @@ __global__ void __launch_bounds__(kBlock32)
     float features[monki_net::kInputs];
     monki_net::Mixture mixture;
     uint32_t slot = 0xffffffffu;
-    if (MODE != kPlain) {
+    if (MODE == kLog || MODE == kGuided) {
 #pragma unroll
       for (int k = 0; k < monki_net::kInputs; ++k) features[k] = 0.0f;
       if (scatters) event_features<R>(s, h, features);
@@ -1153,8 +1191,12 @@ __global__ void __launch_bounds__(kBlock32)
       }
     }
     prefetch(h.rng);
-    if (active && status == kRunning)
+    if (MODE == kDirected) {
+      const Lobes lobes{&s, guidance.kappa};
+      if (active && status == kRunning) status = scatter_photon<R, TRACE, MODE>(s, failure, h, n, trace, &lobes, guidance);
+    } else if (active && status == kRunning) {
       status = scatter_photon<R, TRACE, MODE>(s, failure, h, n, trace, &mixture, guidance);
+    }
     if (MODE == kLog && slot < log.capacity && status == kRunning) {
 #pragma unroll
       for (int k = 0; k < monki_net::kInputs; ++k) log.features[size_t(slot) * monki_net::kInputs + k] = features[k];
@@ -1347,7 +1389,7 @@ struct Session {
   double gpu_seconds = 0.0;
   double kernel_seconds = 0.0;
   void *proposal = nullptr;  // attached learned proposal (guided runs)
-  Guidance guidance{0.25f, ~0u, ~0u};  // guided runs: defensive fraction, kinds, orders
+  Guidance guidance{0.25f, ~0u, ~0u, 0.0f};  // steered runs: defensive fraction, kinds, orders, lobe kappa
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   // Rounds accumulated on the device (batch sessions with layer paths): per-scene sums and
@@ -1650,7 +1692,7 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
   monki_net::Net net{nullptr, nullptr, nullptr};
   monki_net::EventLog log{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0};
   uint32_t logged_before = 0;
-  if (MODE != kPlain) {
+  if (MODE == kLog || MODE == kGuided) {
     if (!proposal || monki_net::proposal_device(proposal) != h->device) {
       set_error(error, error_size, "A proposal on the session's device is required");
       return 1;
@@ -2074,7 +2116,7 @@ int monki_hip_attach_proposal(void *handle, void *proposal, float alpha, uint32_
     return 1;
   }
   h->proposal = proposal;
-  h->guidance = Guidance{alpha, species_mask, max_order};
+  h->guidance = Guidance{alpha, species_mask, max_order, 0.0f};
   return 0;
 }
 
@@ -2083,6 +2125,26 @@ int monki_hip_run_guided(void *handle, const uint32_t *controls, double *out, ch
   Session *h = static_cast<Session *>(handle);
   return launch<float, false, kGuided>(h, controls, out, nullptr, 0, error, error_size,
                                        h ? h->proposal : nullptr);
+}
+
+int monki_hip_set_directed(void *handle, float alpha, float kappa, uint32_t species_mask, uint32_t max_order,
+                           char *error, size_t error_size) {
+  Session *h = static_cast<Session *>(handle);
+  if (!h || !(alpha > 0.0f && alpha <= 1.0f) || !(kappa >= 1.0f && kappa <= 1.0e6f)) {
+    set_error(error, error_size, "Invalid session, defensive fraction (0 < alpha <= 1) or kappa (1 to 1e6)");
+    return 1;
+  }
+  h->guidance = Guidance{alpha, species_mask, max_order, kappa};
+  return 0;
+}
+
+int monki_hip_run_directed(void *handle, const uint32_t *controls, double *out, char *error, size_t error_size) {
+  Session *h = static_cast<Session *>(handle);
+  if (h && !(h->guidance.kappa >= 1.0f)) {
+    set_error(error, error_size, "Set the detector-directed lobes (monki_hip_set_directed) first");
+    return 1;
+  }
+  return launch<float, false, kDirected>(h, controls, out, nullptr, 0, error, error_size);
 }
 
 int monki_hip_trace_guided(void *handle, const uint32_t *controls, double *out, float *records,

@@ -3,7 +3,7 @@
 from .backend import Backend
 from .batch import HIPBatch
 from .hip import HIP
-from .proposal import ProposalSettings
+from .proposal import DirectedSettings, ProposalSettings
 from .tallies import BatchControls, RawTallies
 
-__all__ = ["Backend", "HIP", "HIPBatch", "ProposalSettings", "BatchControls", "RawTallies"]
+__all__ = ["Backend", "HIP", "HIPBatch", "DirectedSettings", "ProposalSettings", "BatchControls", "RawTallies"]

@@ -24,7 +24,7 @@ from ..metadata import file_hash, scene_metadata
 from ..preparation import UPSTREAM, prepare_case
 from .backend import Backend
 from .hip_tables import pack_tables, tally_count
-from .proposal import Proposal, ProposalSettings, declare, log_rounds
+from .proposal import DirectedSettings, Proposal, ProposalSettings, declare, log_rounds
 from .tallies import BatchControls, RawTallies
 
 PACKAGE_ROOT = Path(__file__).resolve().parents[2]
@@ -132,6 +132,7 @@ class HIP(Backend):
         library: Path = LIBRARY,
         proposal: Optional[ProposalSettings] = None,
         upstream: Path = UPSTREAM,
+        directed: Optional[DirectedSettings] = None,
     ):
         for name, value, minimum, maximum in (
             ("device", device, 0, 2**32 - 1),
@@ -156,6 +157,12 @@ class HIP(Backend):
         if proposal is not None and precision != 32:
             raise ValueError("the guided tracer runs in float32")
 
+        if directed is not None and not isinstance(directed, DirectedSettings):
+            raise ValueError("directed must be DirectedSettings or None")
+
+        if directed is not None and (precision != 32 or proposal is not None):
+            raise ValueError("detector-directed sampling runs in float32 without a proposal")
+
         self.device = device
         self.save_path = save_path
         self.profile = profile
@@ -166,6 +173,7 @@ class HIP(Backend):
         self.precision = precision
         self.library = Path(library)
         self.proposal = proposal
+        self.directed = directed
         self.upstream = Path(upstream)
         self.metadata = None
         self.provenance: dict = {}
@@ -266,6 +274,20 @@ class HIP(Backend):
             except BaseException:
                 self.close()
                 raise
+
+        if self.directed is not None:
+            settings = self.directed
+            error = ctypes.create_string_buffer(4096)
+
+            if native.monki_hip_set_directed(
+                handle, settings.alpha, settings.kappa, settings.species_mask() & 0xFFFFFFFF,
+                min(settings.max_order, 2**32 - 1), error, len(error),
+            ):  # fmt: skip
+                self.close()
+                raise RuntimeError(error.value.decode())
+
+            self.provenance["directed"] = {key: getattr(settings, key) for key in settings.__dataclass_fields__}
+            self.provenance["method"] = "detector-directed (vMF lobes toward the views, defensive phase function)"
 
     def run_batch(self, controls: object) -> RawTallies:
         """Execute exactly one complete statistical round."""
@@ -465,6 +487,9 @@ class HIP(Backend):
         if self._network is not None:
             run = self._native.monki_hip_run_guided
 
+        if self.directed is not None:
+            run = self._native.monki_hip_run_directed
+
         try:
             for first in range(0, photon_count, self.photons_per_dispatch):
                 count = min(self.photons_per_dispatch, photon_count - first)
@@ -524,4 +549,5 @@ class HIP(Backend):
             self.tally_shards,
             self.precision,
             self.proposal,
+            self.directed,
         )

@@ -114,14 +114,59 @@ def declare(native) -> None:
     native.monki_hip_run_guided.argtypes = [ctypes.c_void_p, _U, _D, ctypes.c_char_p, ctypes.c_size_t]
     native.monki_hip_trace_guided.argtypes = [ctypes.c_void_p, _U, _D, _F, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]
     native.monki_hip_run_logged.argtypes = [ctypes.c_void_p, _U, _D, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
+    native.monki_hip_set_directed.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]
+    native.monki_hip_run_directed.argtypes = [ctypes.c_void_p, _U, _D, ctypes.c_char_p, ctypes.c_size_t]
 
     for name in (
         "monki_hip_proposal_set_parameters", "monki_hip_proposal_get_parameters", "monki_hip_proposal_reserve",
         "monki_hip_proposal_set_samples", "monki_hip_proposal_samples", "monki_hip_proposal_read_samples",
         "monki_hip_proposal_gradients", "monki_hip_proposal_train", "monki_hip_proposal_forward",
         "monki_hip_attach_proposal", "monki_hip_run_guided", "monki_hip_trace_guided", "monki_hip_run_logged",
+        "monki_hip_set_directed", "monki_hip_run_directed",
     ):  # fmt: skip
         getattr(native, name).restype = ctypes.c_int
+
+
+@dataclass(frozen=True)
+class DirectedSettings:
+    """Detector-directed sampling (no network): the tracer draws a scattering direction from the
+    phase function with probability ``alpha`` and otherwise from von Mises-Fisher lobes of
+    concentration ``kappa`` around the view directions, and multiplies the weight by
+    p / (alpha p + (1 - alpha) q). Photons then travel toward the detectors more often, and
+    the rare events whose direction lies in a tabulated phase function's forward peak toward a
+    view (the large local-estimate contributions of cloud radiances) come with proportionally
+    smaller weights. ``guided_kinds`` and ``max_order`` as in ProposalSettings; the default
+    directs the first-order events of tabulated species (clouds, aerosol): directing every
+    order multiplies the per-event factors (up to 1 / alpha each) into a heavy-tailed weight.
+    """
+
+    alpha: float = 0.5
+    kappa: float = 500.0
+    guided_kinds: Optional[tuple] = ("tabulated",)
+    max_order: int = 1
+
+    def species_mask(self) -> int:
+        if self.guided_kinds is None:
+            return 0xFFFFFFFF
+        mask = 0
+        for kind in self.guided_kinds:
+            if kind == "gas":
+                mask |= 1
+            elif kind == "tabulated":
+                mask |= 0xFFFFFFFE
+            else:
+                mask |= 1 << (int(kind) + 1)
+        return mask
+
+    def __post_init__(self):
+        if not (isinstance(self.alpha, (int, float)) and 0 < self.alpha <= 1):
+            raise ValueError("alpha must be in (0, 1]")
+
+        if not (isinstance(self.kappa, (int, float)) and 1 <= self.kappa <= 1e6):
+            raise ValueError("kappa must be in [1, 1e6]")
+
+        if type(self.max_order) is not int or not 1 <= self.max_order <= 2**32 - 1:
+            raise ValueError("max_order must be an integer in [1, 2**32 - 1]")
 
 
 class Proposal:

@@ -179,3 +179,35 @@ def test_guided_tracer_agrees_with_plain_tracer(tmp_path, guidance):
     for name, stats in report.items():
         assert stats["max_abs_z"] < 5.5, name
         assert 0.6 < stats["chi_square_per_dof"] < 1.5, name
+
+
+@pytest.mark.parametrize("kappa", [50.0, 2000.0])
+def test_directed_tracer_agrees_with_plain_tracer(tmp_path, kappa):
+    """Unbiasedness of detector-directed sampling: the directed and plain radiances and layer
+    AMFs agree within combined SE. Mie events of the first two orders are directed (weight
+    factors at most 1 / alpha each, so the estimator stays light-tailed enough for the test)."""
+
+    from monki import AMFConvergence, SceneCase, simulate
+    from monki.execution import DirectedSettings, HIP
+    from monki.validation import combined_se_statistics
+
+    scene = small_box()
+    rounds, ppc = 48, 200
+    case = SceneCase("box", scene, ppc, rounds)
+    policy = AMFConvergence(1e-9, rounds, 16, 3)
+    plain = simulate(case, HIP(seed=5), tmp_path / "plain.nc", convergence=policy)
+    backend = HIP(seed=6, directed=DirectedSettings(alpha=0.5, kappa=kappa, max_order=2))
+    directed = simulate(case, backend, tmp_path / "directed.nc", convergence=policy)
+    receipt = json.loads((tmp_path / "directed.run.json").read_text())
+    assert receipt["directed"]["kappa"] == kappa and receipt["method"].startswith("detector-directed")
+    report = {}
+
+    for name in ("radiance", "AMF_layer"):
+        a, se_a = plain.values[name], plain.values["std_" + name]
+        b, se_b = directed.values[name], directed.values["std_" + name]
+        report[name] = combined_se_statistics(a, se_a, b, se_b, floor=1e-12)
+
+    print(json.dumps(report))
+    for name, stats in report.items():
+        assert stats["max_abs_z"] < 5.5, name
+        assert 0.6 < stats["chi_square_per_dof"] < 1.5, name

@@ -483,3 +483,16 @@ def test_spectral_cloud_field_statistics():
     gas = lambda scene: (scene.beta[0, 0, :-1] * scene.zunit[:-1] * scene.fractions[-1, 0, 0, :-1]).sum()  # noqa: E731
     rayleigh = lambda scene: gas(scene) - 0.01 * np.exp(-(scene.wavelength_nm - 340) / 12)  # noqa: E731
     assert rayleigh(short) / rayleigh(long) == pytest.approx((300 / 500) ** -4.08, rel=1e-9)
+
+
+def test_directed_settings_validate_and_build_species_masks():
+    from monki.execution import DirectedSettings
+
+    assert DirectedSettings().species_mask() == 0xFFFFFFFE
+    assert DirectedSettings(guided_kinds=("gas", 1)).species_mask() == 0b101
+    assert DirectedSettings(guided_kinds=None).species_mask() == 0xFFFFFFFF
+    assert DirectedSettings().max_order == 1
+
+    for bad in ({"alpha": 0.0}, {"alpha": 1.5}, {"kappa": 0.5}, {"kappa": 2e6}, {"max_order": 0}, {"max_order": 1.0}):
+        with pytest.raises(ValueError):
+            DirectedSettings(**bad)

@@ -50,7 +50,8 @@ enum Status : int { kRunning = 0, kEscaped = 1, kAbsorbed = 2, kFailed = 3 };
 // proposal's training, and guided sampling with the learned proposal.
 // Detector-directed sampling draws from vMF lobes around the view directions instead of a
 // network's mixture.
-enum Mode : int { kPlain = 0, kLog = 1, kGuided = 2, kDirected = 3 };
+// kStokes4: the plain tracer with the full Stokes vector (nmat 4: I, Q, U, V).
+enum Mode : int { kPlain = 0, kLog = 1, kGuided = 2, kDirected = 3, kStokes4 = 4 };
 
 // Which events the guided mode guides: scatterer kinds (bit 0 gas, bit s + 1 tabulated species
 // s) up to a scattering order; alpha = probability of the phase-function component.
@@ -75,6 +76,7 @@ struct SpeciesTables {
   const uint16_t *cdf_start;
   // cosine[0, n) ascending; cosine[n, 2n) = 1 - cos; cosine[2n, 3n) = 1 + cos.
   const float *cosine, *cdf, *f11, *f22, *f33, *f12;
+  const float *f34, *f44;  // nmat 4 only (the two arrays after F12), else null
 };
 
 // Scene constants and table pointers, passed by value as a kernel argument.
@@ -175,10 +177,13 @@ struct Counts {
 
 template <typename R> struct Elements {
   R f11, f22, f33, f12;
+  R f34, f44;  // filled for the full Stokes vector only
 };
 
+// V is carried by the full-Stokes (nmat 4) instantiations only; elsewhere it stays zero and
+// the compiler drops it.
 template <typename R> struct Stokes {
-  R i, q, u;
+  R i, q, u, v;
 };
 
 template <typename R> struct Photon {
@@ -249,12 +254,16 @@ __device__ inline int bracket_r(const float *__restrict__ axis, int n, R value, 
 // interpolate linearly in cos as the Fortran does; near the peaks the interpolation weight is
 // taken from 1 - cos (or 1 + cos) against the table's exact 1 - cos (1 + cos) nodes, because a
 // float32 cos cannot resolve the forward peak (F11 changes ~1e8 per unit cos there).
-template <typename R>
+// V4: also F34 and F44 (nmat 4): Rayleigh F34 = 0, F44 = Delta Delta' 1.5 cos with
+// Delta' = (1 - 2 depol) / (1 - depol); tabulated F34, F44 interpolated like the others.
+template <typename R, bool V4 = false>
 // known: the table interval of cosine when the caller has it (the sampler), else -1.
 __device__ inline Elements<R> elements(const Scene &s, int species, R cosine, R one_minus,
                                        R one_plus, uint32_t &reads, int known = -1) {
   cosine = clamp1(cosine);
   Elements<R> f;
+  f.f34 = R(0);
+  f.f44 = R(0);
   if (species < 0) {
     const R depol = R(s.depol);
     const R delta = (R(1) - depol) / (R(1) + depol / R(2));
@@ -264,6 +273,7 @@ __device__ inline Elements<R> elements(const Scene &s, int species, R cosine, R 
     f.f22 = diagonal;
     f.f33 = delta * R(1.5) * cosine;
     f.f12 = -delta * R(0.75) * sin2;
+    if (V4) f.f44 = delta * ((R(1) - R(2) * depol) / (R(1) - depol) * R(1.5) * cosine);
   } else {
     const SpeciesTables &t = s.sp[species];
     const int n = t.lut_size;
@@ -286,12 +296,21 @@ __device__ inline Elements<R> elements(const Scene &s, int species, R cosine, R 
     f.f22 = R(t.f22[low]) + w * (R(t.f22[low + 1]) - R(t.f22[low]));
     f.f33 = R(t.f33[low]) + w * (R(t.f33[low + 1]) - R(t.f33[low]));
     f.f12 = R(t.f12[low]) + w * (R(t.f12[low + 1]) - R(t.f12[low]));
+    if (V4) {
+      f.f34 = R(t.f34[low]) + w * (R(t.f34[low + 1]) - R(t.f34[low]));
+      f.f44 = R(t.f44[low]) + w * (R(t.f44[low + 1]) - R(t.f44[low]));
+      reads += 4;
+    }
   }
   if (s.nmat == 1) f.f12 = R(0);
   f.f11 *= R(0.25);
   f.f22 *= R(0.25);
   f.f33 *= R(0.25);
   f.f12 *= R(0.25);
+  if (V4) {
+    f.f34 *= R(0.25);
+    f.f44 *= R(0.25);
+  }
   return f;
 }
 
@@ -355,14 +374,17 @@ __device__ inline Rotation<R> rotation(R cosine, R uo, R u1, R phio, R phi1) {
   return rotation_delta<R>(cosine, uo, u1, delta, cd, sd);
 }
 
-// matmul_stokes_rot then P * S (mc_routines_general_mod.f90:1184-1242), I/Q/U only.
-template <typename R>
+// matmul_stokes_rot then P * S (mc_routines_general_mod.f90:1184-1242): rotate Q, U into the
+// scattering plane, apply F, rotate out. V4 adds the circular component: U gains F34 V and
+// V' = -F34 U_rotated + F44 V (F43 = -F34; V is unchanged by rotations).
+template <typename R, bool V4 = false>
 __device__ inline Stokes<R> scatter(const Elements<R> &f, const Rotation<R> &r, const Stokes<R> &s) {
   const R rq = r.c1 * s.q - r.s1 * s.u;
   const R ru = r.s1 * s.q + r.c1 * s.u;
   const R q = f.f12 * s.i + f.f22 * rq;
-  const R u = f.f33 * ru;
-  return Stokes<R>{f.f11 * s.i + f.f12 * rq, r.c2 * q - r.s2 * u, r.s2 * q + r.c2 * u};
+  const R u = V4 ? f.f33 * ru + f.f34 * s.v : f.f33 * ru;
+  return Stokes<R>{f.f11 * s.i + f.f12 * rq, r.c2 * q - r.s2 * u, r.s2 * q + r.c2 * u,
+                   V4 ? f.f44 * s.v - f.f34 * ru : R(0)};
 }
 
 // rayl_scatter_pol / mie_scatter_pol_bound, forward sign -1 (mc_routines_general_mod.f90:
@@ -607,10 +629,12 @@ template <typename R> struct History {
 // Struct of arrays over the block's lanes: lane writes are contiguous, and the tally's reads
 // of one source lane are broadcasts.
 constexpr int kPendingReals = 1 + 3 * kMaxViews;
+constexpr int kPendingReals4 = 1 + 4 * kMaxViews;  // with V (nmat 4)
 constexpr int kPendingWords = 2 + kMaxViews;
+constexpr int pending_reals(bool v4) { return v4 ? kPendingReals4 : kPendingReals; }
 
 template <typename R> struct Pending {
-  R *real;     // rz | weight x views | q x views | u x views
+  R *real;     // rz | weight x views | q x views | u x views [| v x views with nmat 4]
   int *word;   // layer | round offset | pixel x views
   int stride;  // lanes per block
   // Vertical extent of the detector ray inside the event layer.
@@ -619,6 +643,7 @@ template <typename R> struct Pending {
   __device__ R &weight(int v) const { return real[(1 + v) * stride]; }
   __device__ R &q(int v) const { return real[(1 + kMaxViews + v) * stride]; }
   __device__ R &u(int v) const { return real[(1 + 2 * kMaxViews + v) * stride]; }
+  __device__ R &v(int v) const { return real[(1 + 3 * kMaxViews + v) * stride]; }
   __device__ int &layer() const { return word[0]; }
   __device__ int &round() const { return word[stride]; }
   // Output pixel i + nx j, or -1 when the view gets nothing from this event.
@@ -628,6 +653,10 @@ template <typename R> struct Pending {
 
 template <typename R> __device__ inline bool finite3(const Stokes<R> &v) {
   return isfinite(v.i) && isfinite(v.q) && isfinite(v.u);
+}
+
+template <typename R, bool V4> __device__ inline bool finite_stokes(const Stokes<R> &v) {
+  return finite3(v) && (!V4 || isfinite(v.v));
 }
 
 // Launch at TOA in the vacuum layer of column photon mod (nx*ny) (mc.f90:351-464).
@@ -664,7 +693,7 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
 // or the species' single-scattering albedo (mc.f90:681-727, 1028-1074), and evaluate the local
 // estimate toward every view (mc.f90:738-822, 1109-1207; mc_routines_cartesian_mod.f90:849-1222)
 // into `out`. Returns kRunning while the history continues, else its end status.
-template <typename R, bool TRACE>
+template <typename R, bool TRACE, bool V4 = false>
 __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *history,
                        History<R> &h, const Pending<R> &out, Counts &n, Trace<R> &trace) {
   const R eps = R(1e-16);
@@ -799,15 +828,16 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
       const R ex = p.a + vx, ey = p.b + vy, ez = p.c + mu;
       const R one_minus = R(0.5) * (dx * dx + dy * dy + dz * dz);
       const R one_plus = R(0.5) * (ex * ex + ey * ey + ez * ez);
-      const Elements<R> f = elements<R>(s, species, cosine, one_minus, one_plus, n.lut);
+      const Elements<R> f = elements<R, V4>(s, species, cosine, one_minus, one_plus, n.lut);
       const Rotation<R> r = s.nmat > 1 ? rotation_delta<R>(cosine, p.c, mu, delta, cos_delta, sin_delta)
                                        : Rotation<R>{R(1), R(0), R(1), R(0)};
-      prob = scatter<R>(f, r, h.st);
+      prob = scatter<R, V4>(f, r, h.st);
       prob.i *= h.weight;
       prob.q *= h.weight;
       prob.u *= h.weight;
+      if (V4) prob.v *= h.weight;
     }
-    if (!finite3(prob)) {
+    if (!finite_stokes<R, V4>(prob)) {
       atomicOr(failure, kNonfiniteProbability);
       return kFailed;
     }
@@ -827,6 +857,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
     h.contribution += float(ci);
     out.q(view) = prob.q * attenuation * geometry;
     out.u(view) = prob.u * attenuation * geometry;
+    if (V4) out.v(view) = prob.v * attenuation * geometry;
     if (TRACE) {
       trace.stokes_i += ci * geometry;
       if (s.paths) {
@@ -884,6 +915,7 @@ template <typename R, bool TRACE, int MODE = kPlain, typename Proposal = monki_n
 __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, Counts &n,
                               Trace<R> &trace, const Proposal *mixture = nullptr,
                               Guidance guidance = Guidance{1.0f, 0u, 0u, 0.0f}) {
+  constexpr bool V4 = MODE == kStokes4;
   const float alpha = guidance.alpha;
   const R pi = R(3.14159265358979323846);
   Photon<R> &p = h.p;
@@ -933,10 +965,10 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
     }
     // The sampled scattering cosine is used directly, as in mc.f90.
     if (s.nmat > 1 || guide) {
-      const Elements<R> f = elements<R>(s, h.species, cosine, one_minus, one_plus, n.lut, interval);
+      const Elements<R> f = elements<R, V4>(s, h.species, cosine, one_minus, one_plus, n.lut, interval);
       const Rotation<R> r = s.nmat > 1 ? rotation<R>(cosine, p.c, mu, p.phi, phi)
                                        : Rotation<R>{R(1), R(0), R(1), R(0)};
-      Stokes<R> st = scatter<R>(f, r, h.st);
+      Stokes<R> st = scatter<R, V4>(f, r, h.st);
       const R norm = st.i;
       if (guide) {
         if (!direct) {
@@ -952,7 +984,9 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
         st.i = R(1);
         st.q /= norm;
         st.u /= norm;
-        if (!finite3(st) || sqrt(st.q * st.q + st.u * st.u) > R(1.01)) {
+        if (V4) st.v /= norm;
+        const R polarized = V4 ? st.q * st.q + st.u * st.u + st.v * st.v : st.q * st.q + st.u * st.u;
+        if (!finite_stokes<R, V4>(st) || sqrt(polarized) > R(1.01)) {
           atomicOr(failure, kInvalidStokes);
           return kFailed;
         }
@@ -1009,7 +1043,7 @@ __device__ inline void event_features(const Scene &s, const History<R> &h, float
 // lane's LDS row) or Stokes component t - nz. Contiguous lanes make each instruction a few
 // cache-line requests instead of one request per value. Record offsets within a bank are
 // 32-bit: 24-bit multiplies (full rate) instead of 64-bit index arithmetic per view.
-template <typename R>
+template <typename R, bool V4 = false>
 __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int shard, bool has,
                                   const Pending<R> &wave_pending, const R *wave_history, int row,
                                   Counts &n) {
@@ -1029,6 +1063,7 @@ __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int sha
   uint32_t lane_index[kMaxViews];
   R stokes_i[kMaxViews];
   const R stokes_q = component == 1 ? R(1) : R(0), stokes_u = component == 2 ? R(1) : R(0);
+  const R stokes_v = component == 3 ? R(1) : R(0);
 #pragma unroll
   for (int view = 0; view < kMaxViews; ++view) {
     inv_mu[view] = R(s.view_inv_mu[view]);
@@ -1052,7 +1087,8 @@ __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int sha
       const R weight = source.weight(view);
       const R q = source.q(view), u = source.u(view);
       const R length = at_layer ? rz * inv_mu[view] : (above ? layer_length[view] : R(0));
-      const R stokes = weight * stokes_i[view] + q * stokes_q + u * stokes_u;
+      const R stokes = V4 ? weight * stokes_i[view] + q * stokes_q + u * stokes_u + source.v(view) * stokes_v
+                          : weight * stokes_i[view] + q * stokes_q + u * stokes_u;
       const R value = path_lane ? weight * (path + length) : stokes;
       if (pixel >= 0 && tally_lane && value != R(0)) {
         add_tally<R>(bank + (__umul24(uint32_t(pixel), uint32_t(s.record_stride)) + lane_index[view]), value);
@@ -1093,6 +1129,7 @@ __global__ void __launch_bounds__(kBlock32)
                      uint32_t scene_count, int row, R *banks, unsigned *failure, unsigned *work,
                      unsigned long long *counters, float *records, uint32_t record_count,
                      monki_net::Net net, monki_net::EventLog log, Guidance guidance) {
+  constexpr bool V4 = MODE == kStokes4;
   extern __shared__ unsigned char lds_raw[];
   // Layer rows of odd length (the largest nz of the scenes, | 1): lanes reading one row, or
   // each its own row at one layer, touch distinct LDS banks.
@@ -1108,7 +1145,7 @@ __global__ void __launch_bounds__(kBlock32)
   const int block = int(blockDim.x);
   const Pending<R> wave_pending{
       reinterpret_cast<R *>(lds_raw) + size_t(block) * row + (threadIdx.x & ~63u),
-      reinterpret_cast<int *>(reinterpret_cast<R *>(lds_raw) + size_t(block) * (row + kPendingReals)) +
+      reinterpret_cast<int *>(reinterpret_cast<R *>(lds_raw) + size_t(block) * (row + pending_reals(V4))) +
           (threadIdx.x & ~63u),
       block};
   const Pending<R> pending = wave_pending.at(__lane_id());
@@ -1146,11 +1183,11 @@ __global__ void __launch_bounds__(kBlock32)
 
     int status = kRunning;
     prefetch(h.rng);
-    if (active) status = collide<R, TRACE>(s, c, failure, history, h, pending, n, trace);
+    if (active) status = collide<R, TRACE, V4>(s, c, failure, history, h, pending, n, trace);
     // Lanes read each other's LDS rows below: keep every access of this wave in order.
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    tally_wave<R>(s, banks, shard, active && status == kRunning, wave_pending, wave_history, row, n);
+    tally_wave<R, V4>(s, banks, shard, active && status == kRunning, wave_pending, wave_history, row, n);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const bool scatters = active && status == kRunning && !h.surface;
@@ -1462,7 +1499,7 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
   }
   int length = 0;
   if (!word_int(t, 1, 1, 1 << 16, s.nx) || !word_int(t, 2, 1, 1 << 16, s.ny) ||
-      !word_int(t, 3, 2, kMaxLayers, s.nz) || !word_int(t, 4, 1, 3, s.nmat) ||
+      !word_int(t, 3, 2, kMaxLayers, s.nz) || !word_int(t, 4, 1, 4, s.nmat) ||
       !word_int(t, 5, 1, kMaxViews, s.views) || !word_int(t, 6, 0, kMaxSpecies, s.species) ||
       !word_int(t, 7, 0, 1, s.paths) || !word_int(t, 8, 0, kMaxLayers, s.uniform_top) ||
       !word_int(t, 9, 1, 64, s.shards) || !word_int(t, 26, kHeaderWords, 1 << 24, length)) {
@@ -1518,7 +1555,8 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
   }
   for (int sp = 0; sp < kMaxSpecies; ++sp) {
     SpeciesTables &tab = s.sp[sp];
-    tab = SpeciesTables{0, 0.0f, 1.0f, 1.0f, 0, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    tab = SpeciesTables{0,       0.0f,    1.0f,    1.0f,    0,       nullptr, nullptr,
+                        nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     if (sp >= s.species) continue;
     const int h = 32 + 8 * sp;
     if (!word_int(t, h, 2, 1 << 20, tab.lut_size)) {
@@ -1535,6 +1573,15 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
         return false;
       }
       *targets[a] = base + offset;
+    }
+    if (s.nmat == 4) {
+      // F34 and F44 follow F12 directly.
+      if (size_t(tab.f12 - base) + 3 * size_t(tab.lut_size) > size_t(length)) {
+        set_error(error, size, "Scattering table offset out of range");
+        return false;
+      }
+      tab.f34 = tab.f12 + tab.lut_size;
+      tab.f44 = tab.f12 + 2 * tab.lut_size;
     }
     // Ascending cosines spanning [-1, 1] and a CDF from 0 to 1 (metal_tables.py:421-425).
     const float *cs = t + (tab.cosine - base), *cd = t + (tab.cdf - base);
@@ -1641,8 +1688,8 @@ int block_size(const Session *h) {
 }
 
 template <typename R>
-size_t lds_bytes(const Session *h, int block) {
-  return (size_t(h->row) + kPendingReals) * block * sizeof(R) + size_t(kPendingWords) * block * sizeof(int);
+size_t lds_bytes(const Session *h, int block, bool v4 = false) {
+  return (size_t(h->row) + pending_reals(v4)) * block * sizeof(R) + size_t(kPendingWords) * block * sizeof(int);
 }
 
 bool finish(Session *h, char *error, size_t error_size) {
@@ -1666,6 +1713,15 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
   if (!h || !controls || !out || h->scenes.size() != 1) {
     set_error(error, error_size, "Invalid HIP session or arguments");
     return 1;
+  }
+  if constexpr (MODE != kStokes4) {
+    // The full Stokes vector has its own instantiation of the plain tracer.
+    if (h->scenes[0].nmat == 4) {
+      if (MODE == kPlain)
+        return launch<R, TRACE, kStokes4>(h, controls, out, records, record_count, error, error_size, proposal);
+      set_error(error, error_size, "The steered and logging tracers take nmat 1 or 3 (nmat 4 runs plain)");
+      return 1;
+    }
   }
   Controls c{controls[0], controls[1], controls[2], controls[3],
              controls[4], controls[5], controls[6], controls[7]};
@@ -1708,7 +1764,7 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
     }
   }
   const int block = block_size<R>(h);
-  const size_t lds = lds_bytes<R>(h, block);
+  const size_t lds = lds_bytes<R>(h, block, MODE == kStokes4);
   const uint32_t grid = persistent_grid<R, TRACE, MODE>(h, c.count, block, lds);
   (void)hipEventRecord(h->start, h->stream);
   if (!enqueue<R, TRACE, MODE>(h, c.rounds, grid, block, lds, record_count, net, log, error, error_size)) return 1;
@@ -2033,7 +2089,15 @@ void *monki_hip_create(uint32_t device, const float *tables, size_t table_count,
 
 void *monki_hip_batch_create(uint32_t device, const float *tables, const size_t *offsets, uint32_t scene_count,
                              uint32_t block, char *error, size_t error_size) {
-  return create_session(device, tables, offsets, scene_count, 1, block, error, error_size);
+  Session *h = create_session(device, tables, offsets, scene_count, 1, block, error, error_size);
+  for (size_t i = 0; h && i < h->scenes.size(); ++i) {
+    if (h->scenes[i].nmat == 4) {
+      release(h);
+      set_error(error, error_size, "Batch sessions take nmat 1 or 3 (run nmat 4 scenes one at a time)");
+      return nullptr;
+    }
+  }
+  return h;
 }
 
 int monki_hip_batch_run(void *handle, uint32_t round, uint32_t seed, uint32_t max_steps, const uint32_t *photons,

@@ -156,6 +156,10 @@ class HIPBatch:
             packed.append(buffer)
             shapes.append(shape)
             nmat, nx, ny, nz, views = shape
+
+            if nmat == 4:
+                raise NotImplementedError("HIPBatch takes nmat 1 or 3; run nmat 4 scenes with HIP")
+
             counts.append(tally_count(nmat, nx, ny, nz, views, self.save_path))
             offsets.append(offsets[-1] + buffer.size)
 

@@ -214,6 +214,10 @@ class HIP(Backend):
         tables = replace(scene.scene, save_path=self.save_path)
         packed, self._shape = pack_tables(tables, shards=self.tally_shards)
         nmat, nx, ny, nz, views = self._shape
+
+        if nmat == 4 and (self.proposal is not None or self.directed is not None):
+            raise NotImplementedError("the guided and detector-directed tracers take nmat 1 or 3")
+
         per_round = tally_count(nmat, nx, ny, nz, views, self.save_path)
         self._tally_count = per_round
         bank_bytes = per_round * self.tally_shards * (4 if self.precision == 32 else 8)

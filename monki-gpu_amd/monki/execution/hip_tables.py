@@ -19,7 +19,7 @@ Header (64 float32 words; integers are stored exactly, all < 2**24):
 27    offset of the per-layer uniformity flags (1: extinction and species mix are the same in
       every column of the layer, so the free path crosses it without cell-by-cell steps)
 32+8s species s: LUT size, albedo, offsets of cosine block (cos, 1-cos, 1+cos),
-      CDF, F11, F22, F33, F12
+      CDF, F11, F22, F33, F12; with nmat 4, F34 and F44 follow F12 directly (same length)
 ====  =====================================================================
 
 Grids are x-fastest (Fortran order). ``height_above[k]``/``tau_above[k]`` are the thickness
@@ -120,6 +120,10 @@ def pack_tables(scene, shards: int = 32):
         header[base + 5] = place(sp.f22)
         header[base + 6] = place(sp.f33)
         header[base + 7] = place(sp.f12)
+
+        if nmat == 4:
+            place(sp.f34)  # F12 offset + LUT size
+            place(sp.f44)  # F12 offset + 2 LUT size
 
     header[26] = offset
 

@@ -127,8 +127,8 @@ class TransportScene:
         if nz < 2:
             raise ValueError("nz must include at least one physical layer and the vacuum layer")
 
-        if self.nmat not in (1, 3):
-            raise NotImplementedError("transport supports nmat 1 (I) and 3 (I, Q, U)")
+        if self.nmat not in (1, 3, 4):
+            raise ValueError("nmat must be 1 (I), 3 (I, Q, U) or 4 (I, Q, U, V)")
 
         shapes = {
             "zunit": (self.zunit, (nz,)),
@@ -219,7 +219,11 @@ class TransportScene:
     # -- transport.bin v1 -------------------------------------------------------------
 
     def transport_bin(self) -> np.ndarray:
-        """Return the float64 stream ``worker.f90:1102-1120`` writes, in the same order."""
+        """Return the float64 stream ``worker.f90:1102-1120`` writes, in the same order.
+
+        With nmat 4 each species' block continues with F34 and F44 (the worker's export stops
+        at F12 because the reference GPU path takes nmat 1 and 3 only).
+        """
 
         nz = self.nz
         header = [
@@ -269,6 +273,9 @@ class TransportScene:
                 sp.f33,
                 sp.f12,
             ]
+
+            if self.nmat == 4:
+                parts += [sp.f34, sp.f44]
 
         return np.concatenate(parts).astype(np.float64)
 
@@ -320,8 +327,9 @@ class TransportScene:
             size, ssalb = take(2)
             size = int(size)
             cosines, cdf, f11, f22, f33, f12 = take(6 * size).reshape(6, size)
+            f34, f44 = take(2 * size).reshape(2, size) if nmat == 4 else (None, None)
             species.append(
-                Species(f"species{index + 1}", float(ssalb), cosines, f11, f22, f33, f12, cdf=cdf)
+                Species(f"species{index + 1}", float(ssalb), cosines, f11, f22, f33, f12, f34, f44, cdf=cdf)
             )
 
         if offset != raw.size:

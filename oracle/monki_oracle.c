@@ -16,6 +16,8 @@
  *   tabulated matrix, interpolation .... mc_routines_general_mod.f90:257-388, interpol_sp.f:105
  *   direction sampling ................. mc_routines_general_mod.f90:492-932
  *   rotation matrices .................. mc_routines_general_mod.f90:942-1242
+ *   full Stokes vector (nmat 4) ........ F34/F44 of mc_routines_general_mod.f90:74-136, 257-388,
+ *                                        1184-1242; mc.f90:1169-1186, 1344-1353
  *   mu0/mu scaling ..................... mc.f90:1456-1474
  *
  * Differences from the Fortran, all deliberate: Philox streams instead of RANDOM_NUMBER
@@ -50,6 +52,7 @@ typedef struct {
   int lut;
   double ssalb;
   const float *cosine, *cdf, *f11, *f22, *f33, *f12;
+  const float *f34, *f44;  /* nmat 4 only: the two arrays following F12 */
 } species_t;
 
 typedef struct {
@@ -90,15 +93,33 @@ static double draw(rng_t *r) {
 }
 
 /* ---- Scattering matrices ------------------------------------------------------------ */
-/* Elements F11, F22, F33, F12 divided by 4; species < 0 is Rayleigh (ray_scat_matrix). */
-void oracle_rayleigh(double depol, double c, double f[4]) {
+/* Elements F11, F22, F33, F12, F34, F44 divided by 4 (ray_scat_matrix): F34 = 0 and
+ * F44 = Delta' F33 before the depolarization scaling, Delta' = (1 - 2 depol) / (1 - depol). */
+static void rayleigh6(double depol, double c, double f[6]) {
   double s2 = 1.0 - c * c;
   if (s2 < EPS) s2 = 0.0;
   double delta = (1.0 - depol) / (1.0 + depol / 2.0);
+  double delta_prime = (1.0 - 2.0 * depol) / (1.0 - depol);
   f[0] = (delta * 0.75 * (1.0 + c * c) + (1.0 - delta)) / 4.0;
   f[1] = delta * 0.75 * (1.0 + c * c) / 4.0;
   f[2] = delta * 1.5 * c / 4.0;
   f[3] = -delta * 0.75 * s2 / 4.0;
+  f[4] = 0.0;
+  f[5] = delta * (delta_prime * 1.5 * c) / 4.0;
+}
+
+/* Elements F11, F22, F33, F12 divided by 4; species < 0 is Rayleigh (ray_scat_matrix). */
+void oracle_rayleigh(double depol, double c, double f[4]) {
+  double g[6];
+  rayleigh6(depol, c, g);
+  for (int e = 0; e < 4; ++e) f[e] = g[e];
+}
+
+/* F44 / 4 of the Rayleigh matrix (nmat 4). */
+double oracle_rayleigh_f44(double depol, double c) {
+  double g[6];
+  rayleigh6(depol, c, g);
+  return g[5];
 }
 
 /* linpol_dp_single on the bracketing pair (interpol_sp.f:105-152). */
@@ -132,16 +153,26 @@ void oracle_tabulated(const float *cosine, const float *f11, const float *f22, c
   f[3] = lerp2(x0, x1, f12[lo], f12[lo + 1], c) / 4.0;
 }
 
+/* F34 / 4 and F44 / 4 of a tabulated matrix at cosine c (mie_scat_matrix, nmat 4). */
+void oracle_tabulated_v(const float *cosine, const float *f34, const float *f44, int n, double c, double f[2]) {
+  if (c < -1.0) c = -1.0;
+  if (c > 1.0) c = 1.0;
+  int lo = bracket_cos(cosine, n, c);
+  double x0 = cosine[lo], x1 = cosine[lo + 1];
+  f[0] = lerp2(x0, x1, f34[lo], f34[lo + 1], c) / 4.0;
+  f[1] = lerp2(x0, x1, f44[lo], f44[lo + 1], c) / 4.0;
+}
+
 /* Linear interpolation in cos (mie_scat_matrix) with the weight taken against the packed
  * 1 - cos (1 + cos) nodes near the forward (backward) peak: the same affine interpolation,
  * with node positions that keep float64 precision although cos is stored in float32.
  * one_minus / one_plus are 1 - c and 1 + c of the scattering angle. */
 static void elements_at(const scene_t *s, int species, double c, double one_minus, double one_plus,
-                        double f[4]) {
+                        double f[6]) {
   if (species < 0) {
     if (c < -1.0) c = -1.0;
     if (c > 1.0) c = 1.0;
-    oracle_rayleigh(s->depol, c, f);
+    rayleigh6(s->depol, c, f);
   } else {
     const species_t *t = &s->sp[species];
     if (c < -1.0) c = -1.0;
@@ -154,13 +185,15 @@ static void elements_at(const scene_t *s, int species, double c, double one_minu
       w = (one_plus - t->cosine[2 * n + lo]) / ((double)t->cosine[2 * n + lo + 1] - t->cosine[2 * n + lo]);
     else
       w = (c - t->cosine[lo]) / ((double)t->cosine[lo + 1] - t->cosine[lo]);
-    const float *tab[4] = {t->f11, t->f22, t->f33, t->f12};
-    for (int e = 0; e < 4; ++e) f[e] = (tab[e][lo] + w * ((double)tab[e][lo + 1] - tab[e][lo])) / 4.0;
+    const float *tab[6] = {t->f11, t->f22, t->f33, t->f12, t->f34, t->f44};
+    const int count = s->nmat == 4 ? 6 : 4;
+    for (int e = 0; e < count; ++e) f[e] = (tab[e][lo] + w * ((double)tab[e][lo + 1] - tab[e][lo])) / 4.0;
+    if (count == 4) f[4] = 0.0, f[5] = f[2];
   }
   if (s->nmat == 1) f[3] = 0.0;
 }
 
-static void elements(const scene_t *s, int species, double c, double f[4]) {
+static void elements(const scene_t *s, int species, double c, double f[6]) {
   elements_at(s, species, c, 1.0 - c, 1.0 + c, f);
 }
 
@@ -218,6 +251,29 @@ void oracle_scatter(const double f[4], const double r[4], const double s[3], dou
   for (int i = 0; i < 3; ++i) out[i] = P[i][0] * s[0] + P[i][1] * s[1] + P[i][2] * s[2];
 }
 
+/* matmul_stokes_rot then P * S for the full Stokes vector (nmat 4): f = F11, F22, F33, F12,
+ * F34, F44 (/4), with F43 = -F34; V is not rotated. */
+void oracle_scatter4(const double f[6], const double r[4], const double s[4], double out[4]) {
+  double C1 = r[0], S1 = r[1], C2 = r[2], S2 = r[3];
+  double F11 = f[0], F22 = f[1], F33 = f[2], F12 = f[3], F34 = f[4], F44 = f[5];
+  double P[4][4] = {{F11, F12 * C1, -F12 * S1, 0.0},
+                    {C2 * F12, C1 * C2 * F22 - S1 * S2 * F33, -C2 * S1 * F22 - C1 * S2 * F33, -S2 * F34},
+                    {S2 * F12, C1 * S2 * F22 + C2 * S1 * F33, C1 * C2 * F33 - S1 * S2 * F22, C2 * F34},
+                    {0.0, -S1 * F34, -C1 * F34, F44}};
+  for (int i = 0; i < 4; ++i) out[i] = P[i][0] * s[0] + P[i][1] * s[1] + P[i][2] * s[2] + P[i][3] * s[3];
+}
+
+/* P * S for nmat 1, 3 or 4 (S and out hold nmat values; nmat 1 uses F11 only). */
+static void scatter_n(int nmat, const double f[6], const double r[4], const double *st, double *out) {
+  if (nmat == 4) {
+    oracle_scatter4(f, r, st, out);
+  } else {
+    double s3[3] = {st[0], nmat > 1 ? st[1] : 0.0, nmat > 1 ? st[2] : 0.0}, o3[3];
+    oracle_scatter(f, r, s3, o3);
+    for (int m = 0; m < 3; ++m) out[m] = o3[m];
+  }
+}
+
 /* Spherical trigonometry of rayl_scatter_pol / mie_scatter_pol_bound after acceptance, with
  * sin^2 of the scattering angle given (the Fortran's 1 - cos^2, evaluated as (1-cos)(1+cos)). */
 static void new_direction(double cos_in, double cos_sca, double sin2, double sigma, double *cos_out,
@@ -240,11 +296,11 @@ void oracle_new_direction(double cos_in, double cos_sca, double sigma, double *c
 }
 
 /* Rejection sampling, forward sign -1. Returns 0 after ATTEMPTS rejections. */
-static int sample(const scene_t *s, int species, const double st[3], double cos_in, rng_t *r,
+static int sample(const scene_t *s, int species, const double *st, double cos_in, rng_t *r,
                   double *cos_out, double *dphi, double *cos_sca, double sampled[2],
                   uint64_t *attempts, double *sigma_out) {
-  double bound_ray[4];
-  oracle_rayleigh(s->depol, 1.0, bound_ray);
+  double bound_ray[6];
+  rayleigh6(s->depol, 1.0, bound_ray);
   for (int attempt = 0; attempt < ATTEMPTS; ++attempt) {
     ++*attempts;
     /* sin^2 = (1 - c)(1 + c): exact for Rayleigh (c = 2u - 1); for tabulated species both
@@ -267,7 +323,7 @@ static int sample(const scene_t *s, int species, const double st[3], double cos_
     }
     sin2 = one_minus * one_plus;
     double sigma = 2.0 * PI * draw(r) - PI;
-    double f[4];
+    double f[6];
     elements_at(s, species, c, one_minus, one_plus, f);
     double pdf = f[0] + f[3] * (st[1] * cos(2.0 * sigma) - st[2] * sin(2.0 * sigma));
     double bound = species < 0 ? bound_ray[0] : f[0] + fabs(f[3]);
@@ -320,7 +376,7 @@ typedef struct {
 } sink_t;
 
 /* calc_radiances_and_paths_pol for one event and all views. Returns 0 on a walk limit. */
-static int detect(const scene_t *s, const pos_t *ev, double prob[MAX_VIEWS][3],
+static int detect(const scene_t *s, const pos_t *ev, double prob[MAX_VIEWS][4],
                   const double *history, uint32_t max_steps, sink_t *out) {
   for (int v = 0; v < s->views; ++v) {
     if (!(prob[v][0] > EPS)) continue;
@@ -394,7 +450,7 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
   p.x = s->xunit * draw(&r);
   p.y = s->yunit * draw(&r);
   p.z = s->zunit[p.k];
-  double st[3] = {1.0, 0.0, 0.0}, weight = 1.0;
+  double st[4] = {1.0, 0.0, 0.0, 0.0}, weight = 1.0;
   double history[MAX_LAYERS] = {0};
   double remains = -log(fmax(draw(&r), EPS));
   ++out->counters[0];
@@ -450,25 +506,26 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
     }
     if (weight < EPS) return ABSORBED;
 
-    double prob[MAX_VIEWS][3];
+    double prob[MAX_VIEWS][4];
     for (int v = 0; v < s->views; ++v) {
       double vc = cos(s->view[v]);
       if (surface) {
-        prob[v][0] = st[0] * vc * weight, prob[v][1] = 0.0, prob[v][2] = 0.0;
+        prob[v][0] = st[0] * vc * weight, prob[v][1] = 0.0, prob[v][2] = 0.0, prob[v][3] = 0.0;
       } else {
         double a = 1.0 - p.c * p.c, b = 1.0 - vc * vc;
         double cs = p.c * vc + sqrt(a > EPS ? a : EPS) * sqrt(b > EPS ? b : EPS) * cos(phi_in - s->phi);
-        double f[4], rot[4];
+        double f[6], rot[4];
         elements(s, species, cs, f);
         if (s->nmat > 1)
           oracle_rotation(cs, p.c, vc, phi_in, s->phi, rot);
         else
           rot[0] = 1.0, rot[1] = 0.0, rot[2] = 1.0, rot[3] = 0.0;
-        oracle_scatter(f, rot, st, prob[v]);
-        for (int m = 0; m < 3; ++m) prob[v][m] *= weight;
-        if (prob[v][0] < 1.0e-100) prob[v][0] = prob[v][1] = prob[v][2] = 0.0;
+        prob[v][3] = 0.0;
+        scatter_n(s->nmat, f, rot, st, prob[v]);
+        for (int m = 0; m < 4; ++m) prob[v][m] *= weight;
+        if (prob[v][0] < 1.0e-100) prob[v][0] = prob[v][1] = prob[v][2] = prob[v][3] = 0.0;
       }
-      if (!isfinite(prob[v][0]) || !isfinite(prob[v][1]) || !isfinite(prob[v][2])) {
+      if (!isfinite(prob[v][0]) || !isfinite(prob[v][1]) || !isfinite(prob[v][2]) || !isfinite(prob[v][3])) {
         *flag |= 1u;
         return FAILED;
       }
@@ -482,7 +539,7 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
     if (surface) {
       cos_out = sqrt(draw(&r));
       phi_out = 2.0 * PI * draw(&r);
-      st[0] = 1.0, st[1] = 0.0, st[2] = 0.0;
+      st[0] = 1.0, st[1] = 0.0, st[2] = 0.0, st[3] = 0.0;
     } else {
       double dphi, cos_sca, sampled[2];
       if (!sample(s, species, st, p.c, &r, &cos_out, &dphi, &cos_sca, sampled, &out->counters[6], NULL)) {
@@ -491,12 +548,13 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
       }
       phi_out = phi_in + dphi;
       if (s->nmat > 1) {
-        double f[4], rot[4], ns[3];
+        double f[6], rot[4], ns[4] = {0.0, 0.0, 0.0, 0.0};
         elements_at(s, species, cos_sca, sampled[0], sampled[1], f);
         oracle_rotation(cos_sca, p.c, cos_out, phi_in, phi_out, rot);
-        oracle_scatter(f, rot, st, ns);
-        st[0] = 1.0, st[1] = ns[1] / ns[0], st[2] = ns[2] / ns[0];
-        if (!isfinite(st[1]) || !isfinite(st[2]) || sqrt(st[1] * st[1] + st[2] * st[2]) > 1.01) {
+        scatter_n(s->nmat, f, rot, st, ns);
+        st[0] = 1.0, st[1] = ns[1] / ns[0], st[2] = ns[2] / ns[0], st[3] = ns[3] / ns[0];
+        if (!isfinite(st[1]) || !isfinite(st[2]) || !isfinite(st[3]) ||
+            sqrt(st[1] * st[1] + st[2] * st[2] + st[3] * st[3]) > 1.01) {
           *flag |= 8u;
           return FAILED;
         }
@@ -535,6 +593,8 @@ static int parse(const float *t, size_t count, scene_t *s) {
     p->lut = word(t, h), p->ssalb = t[h + 1];
     p->cosine = t + word(t, h + 2), p->cdf = t + word(t, h + 3), p->f11 = t + word(t, h + 4);
     p->f22 = t + word(t, h + 5), p->f33 = t + word(t, h + 6), p->f12 = t + word(t, h + 7);
+    p->f34 = s->nmat == 4 ? p->f12 + p->lut : NULL;
+    p->f44 = s->nmat == 4 ? p->f12 + 2 * p->lut : NULL;
   }
   s->stokes_count = s->nmat * s->nx * s->ny * s->views;
   s->path_count = s->nx * s->ny * s->nz * s->views;
@@ -552,7 +612,7 @@ int oracle_sample(const float *tables, size_t table_count, int species, double q
                   double cos_in, uint32_t seed, uint32_t count, double *out) {
   scene_t s;
   if (parse(tables, table_count, &s) || species >= s.species) return -1;
-  const double st[3] = {1.0, q, u};
+  const double st[4] = {1.0, q, u, 0.0};
   for (uint32_t i = 0; i < count; ++i) {
     rng_t r = {i, 0u, seed, 0u, 0u, {0, 0, 0, 0}, 4};
     double cos_out, dphi, cos_sca, sigma, sampled[2];

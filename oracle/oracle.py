@@ -58,6 +58,10 @@ class Oracle:
         lib.oracle_scatter.argtypes = [_D, _D, _D, _D]
         lib.oracle_tabulated.argtypes = [_F, _F, _F, _F, _F, ctypes.c_int, ctypes.c_double, _D]
         lib.oracle_new_direction.argtypes = [ctypes.c_double] * 3 + [_D, _D]
+        lib.oracle_rayleigh_f44.argtypes = [ctypes.c_double, ctypes.c_double]
+        lib.oracle_rayleigh_f44.restype = ctypes.c_double
+        lib.oracle_scatter4.argtypes = [_D, _D, _D, _D]
+        lib.oracle_tabulated_v.argtypes = [_F, _F, _F, ctypes.c_int, ctypes.c_double, _D]
         self.lib = lib
 
     # -- helpers ------------------------------------------------------------------------
@@ -84,6 +88,27 @@ class Oracle:
         s = (ctypes.c_double * 3)(*stokes)
         out = (ctypes.c_double * 3)()
         self.lib.oracle_scatter(f, r, s, out)
+        return np.array(out)
+
+    def rayleigh_f44(self, depol, cosine):
+        return self.lib.oracle_rayleigh_f44(depol, cosine)
+
+    def scatter4(self, elements, rotation, stokes):
+        """P * S of the full Stokes vector; elements F11, F22, F33, F12, F34, F44 (/4)."""
+
+        f = (ctypes.c_double * 6)(*elements)
+        r = (ctypes.c_double * 4)(*rotation)
+        s = (ctypes.c_double * 4)(*stokes)
+        out = (ctypes.c_double * 4)()
+        self.lib.oracle_scatter4(f, r, s, out)
+        return np.array(out)
+
+    def tabulated_v(self, species, cosine):
+        """F34 / 4 and F44 / 4 of a tabulated species at a cosine (float32 table)."""
+
+        arrays = [np.ascontiguousarray(getattr(species, n), dtype=np.float32) for n in ("cosines", "f34", "f44")]
+        out = (ctypes.c_double * 2)()
+        self.lib.oracle_tabulated_v(*[a.ctypes.data_as(_F) for a in arrays], arrays[0].size, cosine, out)
         return np.array(out)
 
     def tabulated(self, species, cosine):

@@ -3,7 +3,7 @@
 ! Project-written. It reads a transport.bin v1 export (kernels/cpu/worker.f90:1102-1120
 ! layout: header, geometry, zunit, views, surface, albedo, elevation, BETA, species
 ! fractions, gas albedo, then per species LUT size, albedo, cosines, bound CDF, F11, F22,
-! F33, F12) and runs statistical rounds exactly as the reference worker does
+! F33, F12, and with nmat 4 also F34, F44) and runs statistical rounds exactly as the reference worker does
 ! (kernels/cpu/worker.f90:1126-1204): per-thread seeds seed+thread+index
 ! (worker.f90:298-321), then one OpenMP REDUCTION(+) region around the unchanged
 ! monte_carlo subroutine (mc.f90:20) per round. Tallies of each round are appended to the
@@ -65,8 +65,12 @@ program monki_ref
      scatterers(is)%ssalb_aer = size_albedo(2)
      read(unit_in) scatterers(is)%scat_angles, scatterers(is)%CDF_bound
      read(unit_in) scatterers(is)%F11, scatterers(is)%F22, scatterers(is)%F33, scatterers(is)%F12
-     scatterers(is)%F34 = 0.d0
-     scatterers(is)%F44 = scatterers(is)%F33
+     if (nmat == 4) then
+        read(unit_in) scatterers(is)%F34, scatterers(is)%F44
+     else
+        scatterers(is)%F34 = 0.d0
+        scatterers(is)%F44 = scatterers(is)%F33
+     end if
   end do
   close(unit_in)
 

@@ -41,19 +41,32 @@ def main():
 
     species = configs.water_cloud()
     # Float32-exact values: the product uploads float32 tables, so both sides see the same nodes.
-    table = np.stack([species.cosines, species.f11, species.f22, species.f33, species.f12], axis=1)
+    table = np.stack([species.cosines, species.f11, species.f22, species.f33, species.f12, species.f34,
+                      species.f44], axis=1)  # fmt: skip
     table = table.astype(np.float32).astype(np.float64)
     cosines = list(rng.uniform(-1, 1, 30)) + [-1.0, 1.0, 0.999, float(species.cosines[100])]
+    # Full Stokes vectors (nmat 4): the reference probe's geometry and random ones, with V.
+    full = [(-0.7, 0.4, 0.3, 1.8, 0.03, 1.0, 0.2, -0.1, 0.05), (-0.4, 1.0, 0.9, 0.4, 0.03, 1.0, 0.2, -0.1, -0.3),
+            (0.3, 0.8660254037844386, 2.0, 0.0, 0.0295, 1.0, -0.3, 0.4, 0.2)]  # fmt: skip
+
+    for _ in range(30):
+        uo, u1 = rng.uniform(-0.999, 0.999, 2)
+        phio, phi1 = rng.uniform(-np.pi, 3 * np.pi, 2)
+        q, u, v = rng.uniform(-0.5, 0.5, 3)
+        full.append((uo, u1, phio, phi1, 0.0295, 1.0, q, u, v))
     text = [str(len(geometry))] + [" ".join(f"{v:.17g}" for v in g) for g in geometry]
     text += [str(len(table))] + [" ".join(f"{v:.17g}" for v in row) for row in table]
     text += [str(len(cosines))] + [f"{c:.17g}" for c in cosines]
+    text += [str(len(full))] + [" ".join(f"{v:.17g}" for v in g) for g in full]
     output = subprocess.run(
         [str(ROOT / "oracle/_ref/helper_probe")], input="\n".join(text) + "\n",
         capture_output=True, text=True, check=True,
     ).stdout.split()  # fmt: skip
     values = np.array([float(v) for v in output])
     geometry_out = values[: 12 * len(geometry)].reshape(len(geometry), 12)
-    table_out = values[12 * len(geometry) :].reshape(len(cosines), 4)
+    table_end = 12 * len(geometry) + 4 * len(cosines)
+    table_out = values[12 * len(geometry) : table_end].reshape(len(cosines), 4)
+    full_out = values[table_end:].reshape(len(full), 11)
     vectors = {
         "source": "reference mc_routines_general_mod.f90 via tests/golden/helper_probe.f90 (amdflang)",
         "geometry_inputs": [list(map(float, g)) for g in geometry],
@@ -62,6 +75,9 @@ def main():
         "table": table.tolist(),
         "table_cosines": [float(c) for c in cosines],
         "table_outputs": table_out.tolist(),
+        "stokes4_inputs": [list(map(float, g)) for g in full],
+        "stokes4_outputs": full_out.tolist(),
+        "stokes4_columns": ["F44 Rayleigh", "I", "Q", "U", "V", "F34 table", "F44 table", "I", "Q", "U", "V"],
     }
     path = Path(__file__).with_name("helper_vectors.json")
     path.write_text(json.dumps(vectors, indent=1) + "\n")

@@ -81,6 +81,48 @@ def test_tabulated_interpolation_matches_reference_fortran(oracle):
         np.testing.assert_allclose(oracle.tabulated(species, cosine), expected, rtol=1e-12, atol=1e-14)
 
 
+def test_full_stokes_helpers_match_reference_fortran(oracle):
+    """nmat 4: Rayleigh F44, tabulated F34/F44 and the rotated 4x4 phase matrix times a full
+    Stokes vector, against ray_scat_matrix / mie_scat_matrix / matmul_stokes_rot at nmat 4."""
+
+    from monki.scene import Species
+
+    vectors = helper_vectors()
+    table = np.array(vectors["table"])
+    species = Species("probe", 1.0, *table.T)
+    assert np.any(species.f34 != 0)  # the water-cloud table has a circular-polarization term
+
+    for inputs, expected in zip(vectors["stokes4_inputs"], vectors["stokes4_outputs"]):
+        uo, u1, phio, phi1, depol, *stokes = inputs
+        cosine = uo * u1 + np.sqrt(1 - uo**2) * np.sqrt(1 - u1**2) * np.cos(phio - phi1)
+        r = oracle.rotation(cosine, uo, u1, phio, phi1)
+        rayleigh = [*oracle.rayleigh(depol, cosine), 0.0, oracle.rayleigh_f44(depol, cosine)]
+        assert rayleigh[5] == pytest.approx(expected[0], rel=1e-12, abs=1e-15)
+        np.testing.assert_allclose(oracle.scatter4(rayleigh, r, stokes), expected[1:5], rtol=1e-9, atol=1e-12)
+        v = oracle.tabulated_v(species, cosine)
+        np.testing.assert_allclose(v, expected[5:7], rtol=1e-9, atol=1e-12)
+        tabulated = [*oracle.tabulated(species, cosine), *v]
+        np.testing.assert_allclose(oracle.scatter4(tabulated, r, stokes), expected[7:11], rtol=1e-9, atol=1e-12)
+
+
+def test_full_stokes_matrix_reduces_to_iqu(oracle):
+    # With V = 0 and F34 = 0 the first three rows equal the nmat 3 product; at depol 0 the
+    # Rayleigh F44 equals F33.
+    rng = np.random.default_rng(5)
+
+    for _ in range(50):
+        uo, u1 = rng.uniform(-0.99, 0.99, 2)
+        phio, phi1 = rng.uniform(-3, 3, 2)
+        c = uo * u1 + np.sqrt(1 - uo**2) * np.sqrt(1 - u1**2) * np.cos(phio - phi1)
+        f = oracle.rayleigh(0.0295, c)
+        r = oracle.rotation(c, uo, u1, phio, phi1)
+        s = [1.0, *rng.uniform(-0.5, 0.5, 2)]
+        full = oracle.scatter4([*f, 0.0, oracle.rayleigh_f44(0.0295, c)], r, [*s, 0.0])
+        np.testing.assert_allclose(full[:3], oracle.scatter(f, r, s), rtol=1e-13, atol=1e-16)
+        assert full[3] == 0.0
+        assert oracle.rayleigh_f44(0.0, c) == pytest.approx(oracle.rayleigh(0.0, c)[2], rel=1e-14, abs=1e-16)
+
+
 def test_scattering_direction_geometry(oracle):
     # The new direction must make the sampled scattering angle with the old one.
     rng = np.random.default_rng(11)

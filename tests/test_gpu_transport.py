@@ -28,6 +28,8 @@ def small_scenes():
         "clear-IQU": configs.clear_sky(nx=6, ny=4, nmat=3),
         "layered-cloud": configs.layered_cloud(nx=3, ny=2, nmat=3),
         "box-cloud": configs.box_cloud(nx=8, ny=6, nmat=3, box=(3, 2), offset=(2, 1)),
+        # Full Stokes vector: the Mie cloud's F34 turns linear into circular polarization.
+        "box-cloud-IQUV": configs.box_cloud(nx=8, ny=6, nmat=4, box=(3, 2), offset=(2, 1)),
     }
 
 
@@ -117,6 +119,7 @@ AGREEMENT = {  # (float32, float64) over all photons
     "clear-IQU": (0.995, 0.999),
     "layered-cloud": (0.995, 0.999),
     "box-cloud": (0.90, 0.998),
+    "box-cloud-IQUV": (0.90, 0.998),
 }
 
 
@@ -180,16 +183,37 @@ def test_float32_histories_match_float64_oracle(oracle, name):
 
     gpu_tallies, gpu, cpu_tallies, cpu = trace_both(oracle, name, 200)
     check_histories(name, gpu, cpu, 32)
-    check_tallies(name, gpu_tallies, cpu_tallies, gpu, cpu, rtol=2e-3 if name != "box-cloud" else 2e-2)
+    check_tallies(name, gpu_tallies, cpu_tallies, gpu, cpu, rtol=2e-3 if not name.startswith("box-cloud") else 2e-2)
 
 
-@pytest.mark.parametrize("name", ["clear-IQU", "layered-cloud", "box-cloud"])
+@pytest.mark.parametrize("name", ["clear-IQU", "layered-cloud", "box-cloud", "box-cloud-IQUV"])
 def test_float64_build_matches_oracle(oracle, name):
     """The float64 build runs the oracle's algorithm on the device."""
 
     gpu_tallies, gpu, cpu_tallies, cpu = trace_both(oracle, name, 100, precision=64)
     check_histories(name, gpu, cpu, 64)
     check_tallies(name, gpu_tallies, cpu_tallies, gpu, cpu, rtol=1e-4)
+
+
+def test_circular_polarization_matches_oracle(oracle):
+    """The V tallies (nmat 4) of the float64 build follow the oracle's on the same streams.
+
+    V arises only through F34 of the Mie cloud, a small term: compared at V's own scale, not
+    at the intensity scale of the other checks."""
+
+    name = "box-cloud-IQUV"
+    gpu_tallies, gpu, cpu_tallies, cpu = trace_both(oracle, name, 100, precision=64)
+    scene = SCENES[name]
+    stokes = 4 * scene.nx * scene.ny * scene.angles
+    g = gpu_tallies[:stokes].reshape(4, -1, order="F")
+    c = cpu_tallies[:stokes].reshape(4, -1, order="F")
+    scale = np.abs(c[3]).max()
+    assert scale > 1e-6 * np.abs(c[0]).max()  # V is a signal here, not zero
+    close = np.abs(g[3] - c[3]) <= 1e-3 * scale
+    print(name, "V scale", scale, "close", close.mean())
+    assert close.mean() > 0.95
+    assert np.sign(g[3].sum()) == np.sign(c[3].sum())
+    assert abs(g[3].sum() - c[3].sum()) <= 0.05 * np.abs(c[3]).sum()
 
 
 def test_hip_never_uses_cpu_transport():

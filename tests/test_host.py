@@ -257,8 +257,10 @@ def test_scene_validation_rejects_inconsistent_tables():
     with pytest.raises(ValueError, match="fractions"):
         broken.validate()
 
-    with pytest.raises(NotImplementedError):
-        replace(scene, nmat=4).validate()
+    replace(scene, nmat=4).validate()  # the full Stokes vector is supported
+
+    with pytest.raises(ValueError, match="nmat"):
+        replace(scene, nmat=2).validate()
 
     with pytest.raises(ValueError, match="zenith"):
         replace(scene, view_zeniths=(0.0, 90.0)).validate()
@@ -279,6 +281,47 @@ def test_transport_bin_round_trip_and_layout():
     np.testing.assert_array_equal(back.fractions, scene.fractions)
     np.testing.assert_array_equal(back.species[0].f11, scene.species[0].f11)
     np.testing.assert_allclose(back.species[1].cdf, scene.species[1].bound_cdf(3))
+
+
+def test_full_stokes_tables_carry_f34_and_f44():
+    """nmat 4: the export appends F34 and F44 per species (read back exactly), and the packed
+    float32 tables place them right after F12, where the device reads them."""
+
+    from monki import configs
+    from monki.execution.hip_tables import pack_tables, tally_count
+    from monki.scene import TransportScene
+
+    scene = configs.box_cloud(nx=4, ny=3, nmat=4, box=(2, 1), offset=(1, 1))
+    cloud = scene.species[0]
+    assert np.any(cloud.f34 != 0) and not np.array_equal(cloud.f44, cloud.f33 * 0)
+    back = TransportScene.from_transport_bin(scene.transport_bin())
+    for a, b in zip(back.species, scene.species):
+        np.testing.assert_array_equal(a.f34, b.f34)
+        np.testing.assert_array_equal(a.f44, b.f44)
+    assert back.nmat == 4
+    three = replace(scene, nmat=3)
+    assert scene.transport_bin().size == three.transport_bin().size + sum(2 * sp.cosines.size for sp in scene.species)
+
+    packed, shape = pack_tables(scene, shards=2)
+    assert shape[0] == 4 and packed[4] == 4
+    for index, sp in enumerate(scene.species):
+        n, f12 = int(packed[32 + 8 * index]), int(packed[32 + 8 * index + 7])
+        np.testing.assert_array_equal(packed[f12 + n : f12 + 2 * n], sp.f34.astype(np.float32))
+        np.testing.assert_array_equal(packed[f12 + 2 * n : f12 + 3 * n], sp.f44.astype(np.float32))
+    assert tally_count(4, 4, 3, scene.nz, 3, True) == 4 * 4 * 3 * 3 + 4 * 3 * scene.nz * 3 + 4 * 3 * 3
+
+
+def test_steered_tracers_and_batches_reject_full_stokes_before_device():
+    from monki import SceneCase, configs
+    from monki.execution import HIP, DirectedSettings, HIPBatch
+
+    case = SceneCase("iquv", configs.clear_sky(nx=2, ny=2, nmat=4), 10, 2)
+
+    with pytest.raises(NotImplementedError, match="nmat"):
+        HIP(directed=DirectedSettings()).prepare(case)
+
+    with pytest.raises(NotImplementedError, match="nmat"):
+        HIPBatch().prepare([case])
 
 
 def test_packed_tables_uniform_top_and_prefixes():

@@ -50,8 +50,11 @@ enum Status : int { kRunning = 0, kEscaped = 1, kAbsorbed = 2, kFailed = 3 };
 // proposal's training, and guided sampling with the learned proposal.
 // Detector-directed sampling draws from vMF lobes around the view directions instead of a
 // network's mixture.
-// kStokes4: the plain tracer with the full Stokes vector (nmat 4: I, Q, U, V).
-enum Mode : int { kPlain = 0, kLog = 1, kGuided = 2, kDirected = 3, kStokes4 = 4 };
+// kStokes4: the plain tracer with the full Stokes vector (nmat 4: I, Q, U, V). kBackward,
+// kBackward4: backward transport (nmat 1/3 and nmat 4).
+enum Mode : int { kPlain = 0, kLog = 1, kGuided = 2, kDirected = 3, kStokes4 = 4, kBackward = 5, kBackward4 = 6 };
+constexpr bool full_stokes(int mode) { return mode == kStokes4 || mode == kBackward4; }
+constexpr bool backward(int mode) { return mode == kBackward || mode == kBackward4; }
 
 // Which events the guided mode guides: scatterer kinds (bit 0 gas, bit s + 1 tabulated species
 // s) up to a scattering order; alpha = probability of the phase-function component.
@@ -82,6 +85,7 @@ struct SpeciesTables {
 // Scene constants and table pointers, passed by value as a kernel argument.
 struct Scene {
   int nx, ny, nz, nmat, views, species, paths, uniform_top, shards;
+  int backward;  // backward transport: weight-matrix rows, injection-column pixels, no mu0/mu
   int tally_count;  // output values per round: Stokes | paths | weights
   int stokes_count, path_count;
   // Device banks per round and shard: one record per (view, pixel) holding the layer paths
@@ -623,6 +627,10 @@ template <typename R> struct History {
   R path_total;  // sum of the layer paths, for the per-photon records
   float contribution;  // sum of the detector weights C1 of the history so far (all views)
   uint32_t order;      // atmospheric events so far
+  // Backward transport: rows 2..4 of the weight matrix (mc.f90:411-418, 948-955); row 1 is
+  // `st`. Each row is carried like a Stokes vector (W <- W P^T transforms every row r into
+  // P r), all divided by the same norm. Unused, and dropped by the compiler, elsewhere.
+  Stokes<R> w2, w3, w4;
 };
 
 // Local estimates of one event, parked in LDS until the wave's cooperative tally reads them.
@@ -681,6 +689,9 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
   p.rz = R(s.zunit[p.k]);
   for (int k = 0; k < s.nz; ++k) history[k] = R(0);
   h.st = Stokes<R>{R(1), R(0), R(0)};
+  h.w2 = Stokes<R>{R(0), R(1), R(0), R(0)};
+  h.w3 = Stokes<R>{R(0), R(0), R(1), R(0)};
+  h.w4 = Stokes<R>{R(0), R(0), R(0), R(1)};
   h.weight = R(1);
   h.remains = -log(R(uniform(h.rng)));
   h.steps = 0;
@@ -693,7 +704,7 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
 // or the species' single-scattering albedo (mc.f90:681-727, 1028-1074), and evaluate the local
 // estimate toward every view (mc.f90:738-822, 1109-1207; mc_routines_cartesian_mod.f90:849-1222)
 // into `out`. Returns kRunning while the history continues, else its end status.
-template <typename R, bool TRACE, bool V4 = false>
+template <typename R, bool TRACE, bool V4 = false, bool BACK = false>
 __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *history,
                        History<R> &h, const Pending<R> &out, Counts &n, Trace<R> &trace) {
   const R eps = R(1e-16);
@@ -815,7 +826,13 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
     const R mu = R(s.view_mu[view]);
     Stokes<R> prob;
     if (surface) {
-      prob = Stokes<R>{h.st.i * mu * h.weight, R(0), R(0)};
+      if (BACK) {
+        // W As S0 = the first column of W (mc.f90:754-755).
+        const R scale = mu * h.weight;
+        prob = Stokes<R>{h.st.i * scale, h.w2.i * scale, h.w3.i * scale, V4 ? h.w4.i * scale : R(0)};
+      } else {
+        prob = Stokes<R>{h.st.i * mu * h.weight, R(0), R(0)};
+      }
     } else {
       // Sines floored at sqrt(1e-16) as mc.f90:1112-1113 does: the forward peak of a Mie
       // table is steep enough for this to matter in float64.
@@ -831,7 +848,16 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
       const Elements<R> f = elements<R, V4>(s, species, cosine, one_minus, one_plus, n.lut);
       const Rotation<R> r = s.nmat > 1 ? rotation_delta<R>(cosine, p.c, mu, delta, cos_delta, sin_delta)
                                        : Rotation<R>{R(1), R(0), R(1), R(0)};
-      prob = scatter<R, V4>(f, r, h.st);
+      if (BACK) {
+        // W P^T S0 (mc.f90:1150-1151): each weight-matrix row dotted with the first row of the
+        // rotated phase matrix, (F11, F12 C1, -F12 S1, 0).
+        const R a0 = f.f11, a1 = f.f12 * r.c1, a2 = -f.f12 * r.s1;
+        prob = Stokes<R>{a0 * h.st.i + a1 * h.st.q + a2 * h.st.u, a0 * h.w2.i + a1 * h.w2.q + a2 * h.w2.u,
+                         a0 * h.w3.i + a1 * h.w3.q + a2 * h.w3.u,
+                         V4 ? a0 * h.w4.i + a1 * h.w4.q + a2 * h.w4.u : R(0)};
+      } else {
+        prob = scatter<R, V4>(f, r, h.st);
+      }
       prob.i *= h.weight;
       prob.q *= h.weight;
       prob.u *= h.weight;
@@ -852,7 +878,9 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
     const R attenuation = exp(-tau);
     const R geometry = R(s.view_geometry[view]);  // mu0 / mu (mc.f90:1456-1474)
     const R ci = prob.i * attenuation;
-    out.pixel(view) = hit.i + s.nx * hit.j;
+    // Backward transport registers at the injection column (mc_routines_cartesian_mod.f90:
+    // 1035-1039), the launch column of the photon.
+    out.pixel(view) = BACK ? int(h.rng.photon % uint32_t(s.nx * s.ny)) : hit.i + s.nx * hit.j;
     out.weight(view) = ci;
     h.contribution += float(ci);
     out.q(view) = prob.q * attenuation * geometry;
@@ -915,7 +943,8 @@ template <typename R, bool TRACE, int MODE = kPlain, typename Proposal = monki_n
 __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, Counts &n,
                               Trace<R> &trace, const Proposal *mixture = nullptr,
                               Guidance guidance = Guidance{1.0f, 0u, 0u, 0.0f}) {
-  constexpr bool V4 = MODE == kStokes4;
+  constexpr bool V4 = full_stokes(MODE);
+  constexpr bool BACK = backward(MODE);
   const float alpha = guidance.alpha;
   const R pi = R(3.14159265358979323846);
   Photon<R> &p = h.p;
@@ -926,6 +955,12 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
     mu = sqrt(R(uniform(h.rng)));
     phi = R(2) * pi * R(uniform(h.rng));
     h.st = Stokes<R>{R(1), R(0), R(0)};
+    if (BACK) {
+      // W As keeps the first column of W (mc.f90:894-903; W11 is one here).
+      h.w2 = Stokes<R>{h.w2.i, R(0), R(0), R(0)};
+      h.w3 = Stokes<R>{h.w3.i, R(0), R(0), R(0)};
+      h.w4 = Stokes<R>{h.w4.i, R(0), R(0), R(0)};
+    }
   } else {
     R cosine, one_minus, one_plus;
     int interval = -1;  // table interval of the sampled cosine (phase-function draws)
@@ -991,6 +1026,20 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
           return kFailed;
         }
         h.st = st;
+        if (BACK) {
+          // The other rows of W P^T, normalized by the same W11 (mc.f90:1393-1395).
+          auto carry = [&](Stokes<R> &row) {
+            Stokes<R> w = scatter<R, V4>(f, r, row);
+            w.i /= norm;
+            w.q /= norm;
+            w.u /= norm;
+            if (V4) w.v /= norm;
+            row = w;
+          };
+          carry(h.w2);
+          carry(h.w3);
+          if (V4) carry(h.w4);
+        }
       }
     }
     h.remains = -log(R(uniform(h.rng)));
@@ -1129,7 +1178,8 @@ __global__ void __launch_bounds__(kBlock32)
                      uint32_t scene_count, int row, R *banks, unsigned *failure, unsigned *work,
                      unsigned long long *counters, float *records, uint32_t record_count,
                      monki_net::Net net, monki_net::EventLog log, Guidance guidance) {
-  constexpr bool V4 = MODE == kStokes4;
+  constexpr bool V4 = full_stokes(MODE);
+  constexpr bool BACK = backward(MODE);
   extern __shared__ unsigned char lds_raw[];
   // Layer rows of odd length (the largest nz of the scenes, | 1): lanes reading one row, or
   // each its own row at one layer, touch distinct LDS banks.
@@ -1183,7 +1233,7 @@ __global__ void __launch_bounds__(kBlock32)
 
     int status = kRunning;
     prefetch(h.rng);
-    if (active) status = collide<R, TRACE, V4>(s, c, failure, history, h, pending, n, trace);
+    if (active) status = collide<R, TRACE, V4, BACK>(s, c, failure, history, h, pending, n, trace);
     // Lanes read each other's LDS rows below: keep every access of this wave in order.
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1502,7 +1552,8 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
       !word_int(t, 3, 2, kMaxLayers, s.nz) || !word_int(t, 4, 1, 4, s.nmat) ||
       !word_int(t, 5, 1, kMaxViews, s.views) || !word_int(t, 6, 0, kMaxSpecies, s.species) ||
       !word_int(t, 7, 0, 1, s.paths) || !word_int(t, 8, 0, kMaxLayers, s.uniform_top) ||
-      !word_int(t, 9, 1, 64, s.shards) || !word_int(t, 26, kHeaderWords, 1 << 24, length)) {
+      !word_int(t, 9, 1, 64, s.shards) || !word_int(t, 26, kHeaderWords, 1 << 24, length) ||
+      !word_int(t, 28, 0, 1, s.backward)) {
     set_error(error, size, "Invalid packed table dimensions");
     return false;
   }
@@ -1534,7 +1585,8 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
     s.view_sine[v] = sqrt(fmax(1e-16, 1.0 - mu * mu));
     s.view_a[v] = sin(theta) * cos(ph);
     s.view_b[v] = sin(theta) * sin(ph);
-    s.view_geometry[v] = v < s.views ? s.mu0 / mu : 0.0;
+    // mu0 / mu converts source to detector area in forward transport only (mc.f90:1456).
+    s.view_geometry[v] = v < s.views ? (s.backward ? 1.0 : s.mu0 / mu) : 0.0;
     s.view_inv_mu[v] = v < s.views ? 1.0 / mu : 0.0;
   }
   struct Span {
@@ -1714,13 +1766,20 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
     set_error(error, error_size, "Invalid HIP session or arguments");
     return 1;
   }
-  if constexpr (MODE != kStokes4) {
-    // The full Stokes vector has its own instantiation of the plain tracer.
-    if (h->scenes[0].nmat == 4) {
-      if (MODE == kPlain)
-        return launch<R, TRACE, kStokes4>(h, controls, out, records, record_count, error, error_size, proposal);
-      set_error(error, error_size, "The steered and logging tracers take nmat 1 or 3 (nmat 4 runs plain)");
-      return 1;
+  if constexpr (MODE != kStokes4 && !backward(MODE)) {
+    // The full Stokes vector and backward transport have instantiations of their own of the
+    // plain tracer.
+    const Scene &first = h->scenes[0];
+    if (first.nmat == 4 || first.backward) {
+      if (MODE != kPlain) {
+        set_error(error, error_size, "The steered and logging tracers take forward nmat 1 or 3 scenes");
+        return 1;
+      }
+      if (first.backward && first.nmat == 4)
+        return launch<R, TRACE, kBackward4>(h, controls, out, records, record_count, error, error_size, proposal);
+      if (first.backward)
+        return launch<R, TRACE, kBackward>(h, controls, out, records, record_count, error, error_size, proposal);
+      return launch<R, TRACE, kStokes4>(h, controls, out, records, record_count, error, error_size, proposal);
     }
   }
   Controls c{controls[0], controls[1], controls[2], controls[3],
@@ -1764,7 +1823,7 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
     }
   }
   const int block = block_size<R>(h);
-  const size_t lds = lds_bytes<R>(h, block, MODE == kStokes4);
+  const size_t lds = lds_bytes<R>(h, block, full_stokes(MODE));
   const uint32_t grid = persistent_grid<R, TRACE, MODE>(h, c.count, block, lds);
   (void)hipEventRecord(h->start, h->stream);
   if (!enqueue<R, TRACE, MODE>(h, c.rounds, grid, block, lds, record_count, net, log, error, error_size)) return 1;
@@ -2091,9 +2150,9 @@ void *monki_hip_batch_create(uint32_t device, const float *tables, const size_t 
                              uint32_t block, char *error, size_t error_size) {
   Session *h = create_session(device, tables, offsets, scene_count, 1, block, error, error_size);
   for (size_t i = 0; h && i < h->scenes.size(); ++i) {
-    if (h->scenes[i].nmat == 4) {
+    if (h->scenes[i].nmat == 4 || h->scenes[i].backward) {
       release(h);
-      set_error(error, error_size, "Batch sessions take nmat 1 or 3 (run nmat 4 scenes one at a time)");
+      set_error(error, error_size, "Batch sessions take forward nmat 1 or 3 scenes (run others one at a time)");
       return nullptr;
     }
   }

@@ -197,9 +197,6 @@ class HIP(Backend):
         extra, preparation = None, None
 
         if isinstance(scene, ReferenceCase):
-            if scene.backward:
-                raise NotImplementedError("HIP backward transport is not implemented")
-
             # The reference prepares fixtures with its Fortran worker (metal.py:98-104); the
             # same tables come from the host restatement here.
             scene, prepared, preparation = prepare_case(scene, self.upstream, save_path=self.save_path)
@@ -208,15 +205,12 @@ class HIP(Backend):
         if not isinstance(scene, SceneCase):
             raise TypeError("HIP.prepare expects a ReferenceCase or SceneCase")
 
-        if scene.backward:
-            raise NotImplementedError("HIP backward transport is not implemented")
-
         tables = replace(scene.scene, save_path=self.save_path)
-        packed, self._shape = pack_tables(tables, shards=self.tally_shards)
+        packed, self._shape = pack_tables(tables, shards=self.tally_shards, backward=scene.backward)
         nmat, nx, ny, nz, views = self._shape
 
-        if nmat == 4 and (self.proposal is not None or self.directed is not None):
-            raise NotImplementedError("the guided and detector-directed tracers take nmat 1 or 3")
+        if (nmat == 4 or scene.backward) and (self.proposal is not None or self.directed is not None):
+            raise NotImplementedError("the guided and detector-directed tracers take forward nmat 1 or 3 scenes")
 
         per_round = tally_count(nmat, nx, ny, nz, views, self.save_path)
         self._tally_count = per_round
@@ -259,6 +253,7 @@ class HIP(Backend):
             "seed": self.seed,
             "transport_precision": f"float{self.precision}",
             "round_precision": "float64",
+            "transport": "backward" if scene.backward else "forward",
             "photons_per_dispatch": self.photons_per_dispatch,
             "tally_shards": self.tally_shards,
             "round_capacity": self._round_capacity,

@@ -18,6 +18,7 @@ Header (64 float32 words; integers are stored exactly, all < 2**24):
 26    total buffer length
 27    offset of the per-layer uniformity flags (1: extinction and species mix are the same in
       every column of the layer, so the free path crosses it without cell-by-cell steps)
+28    transport direction: 0 forward, 1 backward (mc.f90 backward_mode)
 32+8s species s: LUT size, albedo, offsets of cosine block (cos, 1-cos, 1+cos),
       CDF, F11, F22, F33, F12; with nmat 4, F34 and F44 follow F12 directly (same length)
 ====  =====================================================================
@@ -38,7 +39,7 @@ MAX_VIEWS = 3
 MAX_OFFSET = 2**24
 
 
-def pack_tables(scene, shards: int = 32):
+def pack_tables(scene, shards: int = 32, backward: bool = False):
     """Return (float32 buffer, (nmat, nx, ny, nz, views)) for ``monki_hip_create``."""
 
     scene.validate()
@@ -107,6 +108,7 @@ def pack_tables(scene, shards: int = 32):
     uniform = np.ones(nz)
     uniform[varying] = 0.0
     header[27] = place(uniform)
+    header[28] = 1.0 if backward else 0.0
 
     for index, sp in enumerate(scene.species):
         cdf = sp.bound_cdf(nmat) if sp.cdf is None else sp.cdf

@@ -87,8 +87,8 @@ class SceneCase:
         _check_name(self.name)
         _check_counts(self)
 
-        if self.backward is not False:
-            raise NotImplementedError("backward transport is not implemented")
+        if type(self.backward) is not bool:
+            raise ValueError("backward must be a bool")
 
         validate = getattr(self.scene, "validate", None)
 

@@ -780,7 +780,7 @@ def prepare_case(case, upstream: Path = UPSTREAM, *, save_path: bool = True):
         save_path=save_path,
         description=f"{case.input_file} ({case.name})",
     )
-    scene_case = SceneCase(case.name, prepared.scene, case.photons_per_column, case.rounds)
+    scene_case = SceneCase(case.name, prepared.scene, case.photons_per_column, case.rounds, case.backward)
     provenance = {
         "preparer": "monki.preparation (rdinput, molecules_earth, get_O3_crabs, aerosols restated)",
         "case": {

@@ -218,7 +218,7 @@ class TransportScene:
 
     # -- transport.bin v1 -------------------------------------------------------------
 
-    def transport_bin(self) -> np.ndarray:
+    def transport_bin(self, backward: bool = False) -> np.ndarray:
         """Return the float64 stream ``worker.f90:1102-1120`` writes, in the same order.
 
         With nmat 4 each species' block continues with F34 and F44 (the worker's export stops
@@ -238,7 +238,7 @@ class TransportScene:
             nz,  # k_detector: TOA
             nz,  # k_register: TOA grid
             int(self.save_path),
-            0,  # forward mode
+            int(backward),  # transport direction (mc.f90 backward_mode)
         ]
         parts = [
             np.asarray(header, dtype=np.float64),
@@ -279,9 +279,9 @@ class TransportScene:
 
         return np.concatenate(parts).astype(np.float64)
 
-    def write_transport_bin(self, path) -> Path:
+    def write_transport_bin(self, path, backward: bool = False) -> Path:
         path = Path(path)
-        self.transport_bin().tofile(path)
+        self.transport_bin(backward).tofile(path)
         return path
 
     @classmethod

@@ -18,6 +18,10 @@
  *   rotation matrices .................. mc_routines_general_mod.f90:942-1242
  *   full Stokes vector (nmat 4) ........ F34/F44 of mc_routines_general_mod.f90:74-136, 257-388,
  *                                        1184-1242; mc.f90:1169-1186, 1344-1353
+ *   backward transport ................. weight matrix mc.f90:411-418, 754-755, 793, 894-903,
+ *                                        948-955, 981, 1150-1151, 1324-1334, 1393-1395;
+ *                                        sampling sign general_mod:583-587, 810; injection
+ *                                        register cartesian_mod:1035-1039; no mu0/mu mc.f90:1456
  *   mu0/mu scaling ..................... mc.f90:1456-1474
  *
  * Differences from the Fortran, all deliberate: Philox streams instead of RANDOM_NUMBER
@@ -56,7 +60,7 @@ typedef struct {
 } species_t;
 
 typedef struct {
-  int nx, ny, nz, nmat, views, species, paths;
+  int nx, ny, nz, nmat, views, species, paths, backward;
   double xunit, yunit, depol, theta0, phi0, phi;
   double view[MAX_VIEWS];
   const float *zunit, *albedo, *beta, *cumfrac, *ssalb_gas;
@@ -263,6 +267,46 @@ void oracle_scatter4(const double f[6], const double r[4], const double s[4], do
   for (int i = 0; i < 4; ++i) out[i] = P[i][0] * s[0] + P[i][1] * s[1] + P[i][2] * s[2] + P[i][3] * s[3];
 }
 
+/* The rotated phase matrix Arot of matmul_stokes_rot, zero-padded to 4 x 4 (nmat 1: F11 only). */
+static void phase_matrix(int nmat, const double f[6], const double r[4], double P[4][4]) {
+  double C1 = r[0], S1 = r[1], C2 = r[2], S2 = r[3];
+  double F11 = f[0], F22 = f[1], F33 = f[2], F12 = f[3], F34 = f[4], F44 = f[5];
+  memset(P, 0, 16 * sizeof(double));
+  P[0][0] = F11;
+  if (nmat == 1) return;
+  P[0][1] = F12 * C1, P[0][2] = -F12 * S1;
+  P[1][0] = C2 * F12, P[1][1] = C1 * C2 * F22 - S1 * S2 * F33, P[1][2] = -C2 * S1 * F22 - C1 * S2 * F33;
+  P[2][0] = S2 * F12, P[2][1] = C1 * S2 * F22 + C2 * S1 * F33, P[2][2] = C1 * C2 * F33 - S1 * S2 * F22;
+  if (nmat == 4) {
+    P[3][1] = -S1 * F34, P[3][2] = -C1 * F34, P[3][3] = F44;
+    P[1][3] = -S2 * F34, P[2][3] = C2 * F34;
+  }
+}
+
+/* W <- W A^T for the backward weight matrix (nmat x nmat, zero-padded). */
+static void times_transpose(double W[4][4], const double A[4][4]) {
+  double out[4][4];
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) {
+      double sum = 0.0;
+      for (int k = 0; k < 4; ++k) sum += W[i][k] * A[j][k];
+      out[i][j] = sum;
+    }
+  memcpy(W, out, sizeof(out));
+}
+
+/* The continuation Stokes vector of backward transport: the first row of W with U negated
+ * (mc.f90:949-955), divided by its first element, and W divided by W11 (mc.f90:981, 1393). */
+static void backward_state(int nmat, double W[4][4], double st[4]) {
+  st[0] = W[0][0], st[1] = W[0][1], st[2] = -W[0][2], st[3] = nmat == 4 ? W[0][3] : 0.0;
+  if (nmat == 1) st[1] = st[2] = 0.0;
+  double norm = st[0];
+  if (norm > EPS)
+    for (int m = 0; m < 4; ++m) st[m] /= norm;
+  for (int i = 0; i < 4; ++i)
+    for (int j = 0; j < 4; ++j) W[i][j] /= norm;
+}
+
 /* P * S for nmat 1, 3 or 4 (S and out hold nmat values; nmat 1 uses F11 only). */
 static void scatter_n(int nmat, const double f[6], const double r[4], const double *st, double *out) {
   if (nmat == 4) {
@@ -296,7 +340,7 @@ void oracle_new_direction(double cos_in, double cos_sca, double sigma, double *c
 }
 
 /* Rejection sampling, forward sign -1. Returns 0 after ATTEMPTS rejections. */
-static int sample(const scene_t *s, int species, const double *st, double cos_in, rng_t *r,
+static int sample(const scene_t *s, int species, const double *st, double sign, double cos_in, rng_t *r,
                   double *cos_out, double *dphi, double *cos_sca, double sampled[2],
                   uint64_t *attempts, double *sigma_out) {
   double bound_ray[6];
@@ -325,7 +369,8 @@ static int sample(const scene_t *s, int species, const double *st, double cos_in
     double sigma = 2.0 * PI * draw(r) - PI;
     double f[6];
     elements_at(s, species, c, one_minus, one_plus, f);
-    double pdf = f[0] + f[3] * (st[1] * cos(2.0 * sigma) - st[2] * sin(2.0 * sigma));
+    /* sign -1 forward, +1 backward (rayl_scatter_pol / mie_scatter_pol_bound). */
+    double pdf = f[0] + f[3] * (st[1] * cos(2.0 * sigma) + sign * st[2] * sin(2.0 * sigma));
     double bound = species < 0 ? bound_ray[0] : f[0] + fabs(f[3]);
     if (draw(r) * bound > pdf) continue;
     *cos_sca = c;
@@ -377,7 +422,7 @@ typedef struct {
 
 /* calc_radiances_and_paths_pol for one event and all views. Returns 0 on a walk limit. */
 static int detect(const scene_t *s, const pos_t *ev, double prob[MAX_VIEWS][4],
-                  const double *history, uint32_t max_steps, sink_t *out) {
+                  const double *history, uint32_t max_steps, int column, sink_t *out) {
   for (int v = 0; v < s->views; ++v) {
     if (!(prob[v][0] > EPS)) continue;
     double th = s->view[v];
@@ -412,8 +457,9 @@ static int detect(const scene_t *s, const pos_t *ev, double prob[MAX_VIEWS][4],
     int si = va >= 0 ? 1 : -1, sj = vb >= 0 ? 1 : -1;
     int pi = (((ev->i + si * di) % s->nx) + s->nx) % s->nx;
     int pj = (((ev->j + sj * dj) % s->ny) + s->ny) % s->ny;
+    if (s->backward) pi = column % s->nx, pj = column / s->nx; /* injection column */
 
-    double att = exp(-tau), mu0mu = fabs(cos(s->theta0)) / fabs(vc);
+    double att = exp(-tau), mu0mu = s->backward ? 1.0 : fabs(cos(s->theta0)) / fabs(vc);
     double c1 = prob[v][0] * att;
     int pixel = pi + s->nx * (pj + s->ny * v);
     for (int m = 0; m < s->nmat; ++m) out->tallies[s->nmat * pixel + m] += prob[v][m] * att * mu0mu;
@@ -451,6 +497,7 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
   p.y = s->yunit * draw(&r);
   p.z = s->zunit[p.k];
   double st[4] = {1.0, 0.0, 0.0, 0.0}, weight = 1.0;
+  double W[4][4] = {{1.0, 0.0, 0.0, 0.0}, {0.0, 1.0, 0.0, 0.0}, {0.0, 0.0, 1.0, 0.0}, {0.0, 0.0, 0.0, 1.0}};
   double history[MAX_LAYERS] = {0};
   double remains = -log(fmax(draw(&r), EPS));
   ++out->counters[0];
@@ -509,7 +556,10 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
     double prob[MAX_VIEWS][4];
     for (int v = 0; v < s->views; ++v) {
       double vc = cos(s->view[v]);
-      if (surface) {
+      if (surface && s->backward) {
+        /* W As S0: the first column of W. */
+        for (int m = 0; m < 4; ++m) prob[v][m] = W[m][0] * vc * weight;
+      } else if (surface) {
         prob[v][0] = st[0] * vc * weight, prob[v][1] = 0.0, prob[v][2] = 0.0, prob[v][3] = 0.0;
       } else {
         double a = 1.0 - p.c * p.c, b = 1.0 - vc * vc;
@@ -521,7 +571,15 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
         else
           rot[0] = 1.0, rot[1] = 0.0, rot[2] = 1.0, rot[3] = 0.0;
         prob[v][3] = 0.0;
-        scatter_n(s->nmat, f, rot, st, prob[v]);
+        if (s->backward) {
+          /* S_back = W P^T S0 = W times the first row of P. */
+          double P[4][4];
+          phase_matrix(s->nmat, f, rot, P);
+          for (int m = 0; m < 4; ++m)
+            prob[v][m] = W[m][0] * P[0][0] + W[m][1] * P[0][1] + W[m][2] * P[0][2] + W[m][3] * P[0][3];
+        } else {
+          scatter_n(s->nmat, f, rot, st, prob[v]);
+        }
         for (int m = 0; m < 4; ++m) prob[v][m] *= weight;
         if (prob[v][0] < 1.0e-100) prob[v][0] = prob[v][1] = prob[v][2] = prob[v][3] = 0.0;
       }
@@ -530,7 +588,7 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
         return FAILED;
       }
     }
-    if (!detect(s, &p, prob, history, max_steps, out)) {
+    if (!detect(s, &p, prob, history, max_steps, column, out)) {
       *flag |= 2u;
       return FAILED;
     }
@@ -540,9 +598,15 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
       cos_out = sqrt(draw(&r));
       phi_out = 2.0 * PI * draw(&r);
       st[0] = 1.0, st[1] = 0.0, st[2] = 0.0, st[3] = 0.0;
+      if (s->backward) {
+        /* W <- W As keeps the first column (As(1,1) = 1 only); then normalize. */
+        for (int i = 0; i < 4; ++i) W[i][1] = W[i][2] = W[i][3] = 0.0;
+        backward_state(s->nmat, W, st);
+      }
     } else {
       double dphi, cos_sca, sampled[2];
-      if (!sample(s, species, st, p.c, &r, &cos_out, &dphi, &cos_sca, sampled, &out->counters[6], NULL)) {
+      if (!sample(s, species, st, s->backward ? 1.0 : -1.0, p.c, &r, &cos_out, &dphi, &cos_sca, sampled,
+                  &out->counters[6], NULL)) {
         *flag |= 4u;
         return FAILED;
       }
@@ -551,7 +615,15 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
         double f[6], rot[4], ns[4] = {0.0, 0.0, 0.0, 0.0};
         elements_at(s, species, cos_sca, sampled[0], sampled[1], f);
         oracle_rotation(cos_sca, p.c, cos_out, phi_in, phi_out, rot);
-        scatter_n(s->nmat, f, rot, st, ns);
+        if (s->backward) {
+          double P[4][4];
+          phase_matrix(s->nmat, f, rot, P);
+          times_transpose(W, P);
+          backward_state(s->nmat, W, st);
+          ns[0] = 1.0, ns[1] = st[1], ns[2] = st[2], ns[3] = st[3];
+        } else {
+          scatter_n(s->nmat, f, rot, st, ns);
+        }
         st[0] = 1.0, st[1] = ns[1] / ns[0], st[2] = ns[2] / ns[0], st[3] = ns[3] / ns[0];
         if (!isfinite(st[1]) || !isfinite(st[2]) || !isfinite(st[3]) ||
             sqrt(st[1] * st[1] + st[2] * st[2] + st[3] * st[3]) > 1.01) {
@@ -580,7 +652,7 @@ static int word(const float *t, int index) { return (int)t[index]; }
 static int parse(const float *t, size_t count, scene_t *s) {
   if (count < HEADER || t[0] != 1.0f || (size_t)t[26] != count) return 1;
   s->nx = word(t, 1), s->ny = word(t, 2), s->nz = word(t, 3), s->nmat = word(t, 4);
-  s->views = word(t, 5), s->species = word(t, 6), s->paths = word(t, 7);
+  s->views = word(t, 5), s->species = word(t, 6), s->paths = word(t, 7), s->backward = word(t, 28);
   if (s->nz > MAX_LAYERS || s->views > MAX_VIEWS || s->species > MAX_SPECIES) return 1;
   s->xunit = t[10], s->yunit = t[11], s->depol = t[12];
   s->theta0 = t[13], s->phi0 = t[14], s->phi = t[15];
@@ -617,7 +689,7 @@ int oracle_sample(const float *tables, size_t table_count, int species, double q
     rng_t r = {i, 0u, seed, 0u, 0u, {0, 0, 0, 0}, 4};
     double cos_out, dphi, cos_sca, sigma, sampled[2];
     uint64_t attempts = 0;
-    if (!sample(&s, species, st, cos_in, &r, &cos_out, &dphi, &cos_sca, sampled, &attempts, &sigma))
+    if (!sample(&s, species, st, -1.0, cos_in, &r, &cos_out, &dphi, &cos_sca, sampled, &attempts, &sigma))
       return -1;
     double *o = out + 4 * (size_t)i;
     o[0] = cos_sca, o[1] = sigma, o[2] = cos_out, o[3] = dphi;

@@ -161,14 +161,15 @@ class Oracle:
 
 
 def run_reference(scene, photons_per_column: int, rounds: int, seed: int = 23274, threads: int = 1,
-                  executable: Path = REFERENCE):
-    """Run the reference Fortran transport; return (per-round tallies, seconds per round)."""
+                  executable: Path = REFERENCE, backward: bool = False):
+    """Run the reference Fortran transport (backward_mode = ``backward``); return (per-round
+    tallies, seconds per round, wall seconds)."""
 
     if not Path(executable).is_file():
         raise FileNotFoundError(f"{executable} is missing: run make -C oracle ref")
 
     with tempfile.TemporaryDirectory(prefix="monki-ref-") as directory:
-        tables = scene.write_transport_bin(Path(directory) / "transport.bin")
+        tables = scene.write_transport_bin(Path(directory) / "transport.bin", backward)
         output = Path(directory) / "rounds.bin"
         env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_DYNAMIC="FALSE",
                    OMP_STACKSIZE="128M", GOMP_STACKSIZE="128M")  # fmt: skip

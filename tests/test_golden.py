@@ -327,25 +327,38 @@ def test_oracle_absorbing_column_closed_form(oracle):
 
 @pytest.mark.skipif(not (Path(__file__).parents[1] / "oracle" / "_ref" / "monki_ref").is_file(),
                     reason="reference Fortran build missing (make -C oracle ref)")  # fmt: skip
-@pytest.mark.parametrize("name", ["clear-IQU", "box-cloud"])
+@pytest.mark.parametrize("name", ["clear-IQU", "box-cloud", "box-cloud-IQUV", "box-cloud-backward",
+                                  "box-cloud-IQUV-backward"])  # fmt: skip
 def test_oracle_matches_reference_fortran_statistically(oracle, name):
     """The float64 oracle against the reference's unchanged mc.f90 (its own compiler random
-    numbers): radiance and layer paths per pixel/view within combined standard errors."""
+    numbers): radiance and layer paths per pixel/view within combined standard errors, for
+    I/Q/U, the full Stokes vector (nmat 4) and backward transport (backward_mode)."""
+
+    from dataclasses import replace
 
     from monki import configs
+    from monki.execution.hip_tables import pack_tables
     from monki.validation import combined_se_statistics
     from oracle.oracle import run_reference
 
+    box = configs.box_cloud(nx=4, ny=3, nmat=3, box=(2, 1), offset=(1, 1))
+    # Off the principal plane, so that U (and with nmat 4, V) carry signal.
+    off_plane = replace(box, solar_azimuth=30.0, view_azimuth=120.0)
     scene = {
         "clear-IQU": configs.clear_sky(nx=3, ny=2, nmat=3),
-        "box-cloud": configs.box_cloud(nx=4, ny=3, nmat=3, box=(2, 1), offset=(1, 1)),
+        "box-cloud": box,
+        "box-cloud-IQUV": replace(off_plane, nmat=4),
+        "box-cloud-backward": off_plane,
+        "box-cloud-IQUV-backward": replace(off_plane, nmat=4),
     }[name]
+    backward = name.endswith("backward")
     rounds, ppc = 40, 60
     per_round = ppc * scene.nx * scene.ny
-    ours, _, _, flag = oracle.run(_packed(scene), [per_round * rounds, 0, 1, 5, 100000, 0, per_round, rounds])
+    packed, _ = pack_tables(scene, shards=1, backward=backward)
+    ours, _, _, flag = oracle.run(packed, [per_round * rounds, 0, 1, 5, 100000, 0, per_round, rounds])
     assert flag == 0
     # One thread: the reference RNG sequence (and so the test) is then deterministic.
-    theirs, _, _ = run_reference(scene, ppc, rounds, threads=1)
+    theirs, _, _ = run_reference(scene, ppc, rounds, threads=1, backward=backward)
     a, b = ours / ppc, theirs / ppc
     stats = combined_se_statistics(
         a.mean(axis=0), a.std(axis=0, ddof=1) / np.sqrt(rounds),

@@ -17,7 +17,7 @@ import pytest
 pytestmark = pytest.mark.gpu
 
 
-def compare_with_fortran(scene, ppc, rounds, seed, tmp_path, name):
+def compare_with_fortran(scene, ppc, rounds, seed, tmp_path, name, backward=False):
     """Radiance of HIP simulate against the reference Fortran rounds, within combined SE."""
 
     from monki import SceneCase, simulate
@@ -25,12 +25,14 @@ def compare_with_fortran(scene, ppc, rounds, seed, tmp_path, name):
     from monki.validation import check_result, combined_se_statistics
     from oracle.oracle import run_reference
 
-    gpu = simulate(SceneCase(name, scene, ppc, rounds), HIP(seed=seed), tmp_path / f"{name}.nc")
+    gpu = simulate(SceneCase(name, scene, ppc, rounds, backward=backward), HIP(seed=seed), tmp_path / f"{name}.nc")
     assert check_result(gpu) == []
     # One thread: the reference's compiler RNG sequence is then deterministic.
-    tallies, _, _ = run_reference(scene, ppc, rounds, threads=1)
+    tallies, _, _ = run_reference(scene, ppc, rounds, threads=1, backward=backward)
     nmat, nx, ny, views = scene.nmat, scene.nx, scene.ny, scene.angles
     stokes = tallies[:, : nmat * nx * ny * views].reshape(rounds, nmat, nx, ny, views, order="F") / ppc
+    if backward and nmat >= 3:
+        stokes[:, 2] *= -1  # the product reverses U of backward transport (simulation.py:128-131)
     mean = stokes.mean(axis=0)
     se = stokes.std(axis=0, ddof=1) / np.sqrt(rounds)
     a = gpu.radiance[..., 0].transpose(3, 2, 1, 0)
@@ -68,6 +70,21 @@ def test_full_stokes_transport_matches_reference_fortran(tmp_path):
     assert abs(v_gpu - v_ref) < 5 * np.sqrt((se[3] ** 2).mean() / se[3].size) + 0.2 * abs(v_ref)
 
 
+@pytest.mark.parametrize("nmat", [3, 4])
+def test_backward_transport_matches_reference_fortran(tmp_path, nmat):
+    """Backward transport (weight matrix, injection-column register, no mu0/mu) against the
+    reference Fortran's backward_mode, off the principal plane."""
+
+    from monki import configs
+
+    scene = configs.box_cloud(nx=8, ny=6, nmat=nmat, solar_zenith=50.0, box=(3, 2), offset=(2, 1))
+    scene = replace(scene, solar_azimuth=30.0, view_azimuth=120.0)
+    gpu, mean, se = compare_with_fortran(scene, 100, 40, 6262, tmp_path, f"backward{nmat}", backward=True)
+    receipt = json.loads((tmp_path / f"backward{nmat}.run.json").read_text())
+    assert receipt["transport"] == "backward"
+    assert np.median(np.abs(mean[2]) / np.maximum(se[2], 1e-30)) > 3  # U resolved
+
+
 def test_scalar_transport_matches_reference_fortran(tmp_path):
     from monki import configs
 
@@ -99,8 +116,8 @@ def test_uniform_tail_matches_explicit_detector_traversal(monkeypatch, name):
     shortcut = run()
     original = hip_module.pack_tables
 
-    def explicit(tables, shards):
-        packed, shape = original(tables, shards=shards)
+    def explicit(tables, shards, **options):
+        packed, shape = original(tables, shards=shards, **options)
         assert packed[8] < tables.nz - 1
         packed = packed.copy()
         packed[8] = tables.nz - 1

@@ -23,6 +23,8 @@ GOLDEN = Path(__file__).with_name("golden")
 def small_scenes():
     from monki import configs
 
+    off_plane = replace(configs.box_cloud(nx=8, ny=6, nmat=4, box=(3, 2), offset=(2, 1)), solar_azimuth=30.0,
+                        view_azimuth=120.0)  # fmt: skip
     return {
         "clear-I": configs.clear_sky(nx=6, ny=4, nmat=1),
         "clear-IQU": configs.clear_sky(nx=6, ny=4, nmat=3),
@@ -30,7 +32,14 @@ def small_scenes():
         "box-cloud": configs.box_cloud(nx=8, ny=6, nmat=3, box=(3, 2), offset=(2, 1)),
         # Full Stokes vector: the Mie cloud's F34 turns linear into circular polarization.
         "box-cloud-IQUV": configs.box_cloud(nx=8, ny=6, nmat=4, box=(3, 2), offset=(2, 1)),
+        # Backward transport (names ending in -backward), off the principal plane.
+        "box-cloud-backward": replace(off_plane, nmat=3),
+        "box-cloud-IQUV-backward": off_plane,
     }
+
+
+def is_backward(name):
+    return name.endswith("-backward")
 
 
 SCENES = small_scenes()
@@ -39,7 +48,7 @@ SCENES = small_scenes()
 def case(name, photons_per_column=200, rounds=32):
     from monki import SceneCase
 
-    return SceneCase(name, SCENES[name], photons_per_column, rounds)
+    return SceneCase(name, SCENES[name], photons_per_column, rounds, backward=is_backward(name))
 
 
 def test_gpu_philox_matches_published_known_answers():
@@ -92,7 +101,7 @@ def trace_both(oracle, name, photons_per_column, precision=32, round_index=1):
         backend.prepare(case(name, photons_per_column))
         gpu_tallies, gpu = backend.trace(BatchControls(round_index, photons_per_column), per_round, precision)
 
-    packed, _ = pack_tables(scene, shards=1)
+    packed, _ = pack_tables(scene, shards=1, backward=is_backward(name))
     cpu_tallies, cpu, _, flag = oracle.run(
         packed, [per_round, 0, round_index, 23274, 100000, 0, per_round, 1], records=per_round
     )
@@ -120,6 +129,8 @@ AGREEMENT = {  # (float32, float64) over all photons
     "layered-cloud": (0.995, 0.999),
     "box-cloud": (0.90, 0.998),
     "box-cloud-IQUV": (0.90, 0.998),
+    "box-cloud-backward": (0.90, 0.998),
+    "box-cloud-IQUV-backward": (0.90, 0.998),
 }
 
 
@@ -186,7 +197,8 @@ def test_float32_histories_match_float64_oracle(oracle, name):
     check_tallies(name, gpu_tallies, cpu_tallies, gpu, cpu, rtol=2e-3 if not name.startswith("box-cloud") else 2e-2)
 
 
-@pytest.mark.parametrize("name", ["clear-IQU", "layered-cloud", "box-cloud", "box-cloud-IQUV"])
+@pytest.mark.parametrize("name", ["clear-IQU", "layered-cloud", "box-cloud", "box-cloud-IQUV", "box-cloud-backward",
+                                  "box-cloud-IQUV-backward"])  # fmt: skip
 def test_float64_build_matches_oracle(oracle, name):
     """The float64 build runs the oracle's algorithm on the device."""
 

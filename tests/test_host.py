@@ -202,9 +202,6 @@ def test_reference_case_errors_precede_device_access():
     box = "testrun/MONKI_MLS_2boxclouds_340_340_test.in"
 
     with HIP() as backend:
-        with pytest.raises(NotImplementedError, match="backward"):
-            backend.prepare(ReferenceCase("clear", clear, backward=True))
-
         with pytest.raises(ValueError, match="layer AMF setting"):
             backend.prepare(ReferenceCase("box", box))
 
@@ -218,6 +215,14 @@ def test_reference_case_errors_precede_device_access():
             backend.prepare("clear")
 
         assert backend.metadata is None
+
+    # Backward fixtures reach the tables with the flag set (the session then needs a device).
+    from monki.preparation import prepare_case
+
+    scene_case, _, _ = prepare_case(ReferenceCase("clear", clear, backward=True))
+    assert scene_case.backward is True
+    with pytest.raises(ValueError, match="backward"):
+        ReferenceCase("clear", clear, backward=1)
 
 
 # -- inputs (tests/unit/test_inputs.py) ---------------------------------------------------

@@ -51,10 +51,15 @@ enum Status : int { kRunning = 0, kEscaped = 1, kAbsorbed = 2, kFailed = 3 };
 // Detector-directed sampling draws from vMF lobes around the view directions instead of a
 // network's mixture.
 // kStokes4: the plain tracer with the full Stokes vector (nmat 4: I, Q, U, V). kBackward,
-// kBackward4: backward transport (nmat 1/3 and nmat 4).
-enum Mode : int { kPlain = 0, kLog = 1, kGuided = 2, kDirected = 3, kStokes4 = 4, kBackward = 5, kBackward4 = 6 };
-constexpr bool full_stokes(int mode) { return mode == kStokes4 || mode == kBackward4; }
+// kBackward4: backward transport (nmat 1/3 and nmat 4). kScats, kScats4: the single/multiple-
+// scattering decomposition (calc_radiances_and_scats) in place of the layer paths.
+enum Mode : int {
+  kPlain = 0, kLog = 1, kGuided = 2, kDirected = 3, kStokes4 = 4, kBackward = 5, kBackward4 = 6,
+  kScats = 7, kScats4 = 8
+};
+constexpr bool full_stokes(int mode) { return mode == kStokes4 || mode == kBackward4 || mode == kScats4; }
 constexpr bool backward(int mode) { return mode == kBackward || mode == kBackward4; }
+constexpr bool decomposition(int mode) { return mode == kScats || mode == kScats4; }
 
 // Which events the guided mode guides: scatterer kinds (bit 0 gas, bit s + 1 tabulated species
 // s) up to a scattering order; alpha = probability of the phase-function component.
@@ -86,6 +91,9 @@ struct SpeciesTables {
 struct Scene {
   int nx, ny, nz, nmat, views, species, paths, uniform_top, shards;
   int backward;  // backward transport: weight-matrix rows, injection-column pixels, no mu0/mu
+  // Scattering decomposition: banks hold 8-value records [single (nmat) | multiple (nmat)],
+  // one per view and pixel, then one per view and event cell (level 0 the surface).
+  int scats;
   int tally_count;  // output values per round: Stokes | paths | weights
   int stokes_count, path_count;
   // Device banks per round and shard: one record per (view, pixel) holding the layer paths
@@ -704,7 +712,7 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
 // or the species' single-scattering albedo (mc.f90:681-727, 1028-1074), and evaluate the local
 // estimate toward every view (mc.f90:738-822, 1109-1207; mc_routines_cartesian_mod.f90:849-1222)
 // into `out`. Returns kRunning while the history continues, else its end status.
-template <typename R, bool TRACE, bool V4 = false, bool BACK = false>
+template <typename R, bool TRACE, bool V4 = false, bool BACK = false, bool SCATS = false>
 __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *history,
                        History<R> &h, const Pending<R> &out, Counts &n, Trace<R> &trace) {
   const R eps = R(1e-16);
@@ -791,6 +799,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
   if (surface) {
     h.weight *= R(s.albedo[p.i + s.nx * p.j]);
     ++n.surface;
+    if (SCATS) ++h.order;  // the scattering order counts surface reflections (mc.f90:689)
     if (TRACE) trace.surface += R(1);
   } else {
     const float draw = uniform(h.rng);
@@ -811,7 +820,16 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
   h.surface = surface;
   if (h.weight < eps) return kAbsorbed;
 
-  out.layer() = p.k;
+  if (SCATS) {
+    // Event cell (calc_radiances_and_scats :503-507): level 0 for the surface, else the layer;
+    // complemented when the scattering order exceeds one.
+    // (Events never occur in the vacuum layer, so level <= nz - 1; the bound keeps it so.)
+    const int level = (p.k == 0 && p.rz < R(1e-16)) ? 0 : min(p.k + 1, s.nz - 1);
+    const int cell = p.i + s.nx * (p.j + s.ny * level);
+    out.layer() = h.order == 1 ? cell : ~cell;
+  } else {
+    out.layer() = p.k;
+  }
   out.round() = int(h.round_offset);
   // Detector ray inside the event layer: the distance to its top face (views point upward).
   const R rz = p.c < R(0) ? fmax(R(0), R(s.zunit[p.k]) - p.rz) : p.rz;
@@ -1147,6 +1165,55 @@ __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int sha
   }
 }
 
+// Wave-cooperative tally of the scattering decomposition (calc_radiances_and_scats,
+// mc_routines_cartesian_mod.f90:405-836). Per view, one atomic instruction adds the event's
+// Stokes contribution to two 8-value records [single (nmat) | multiple (nmat)]: lanes
+// 0 .. 2 nmat - 1 to the record of the detector pixel (Stokes_sing / Stokes_mult), lanes
+// 2 nmat .. 4 nmat - 1 to the record of the event cell (the 3-D contributions); the half
+// matching the event's scattering order (1: single) carries the value. The pending layer
+// word holds the cell (i + nx (j + ny level)), complemented for orders above one.
+template <typename R, bool V4>
+__device__ inline void tally_wave_scats(const Scene &s, R *__restrict__ banks, int shard, bool has,
+                                        const Pending<R> &wave_pending, Counts &n) {
+  const int lane = __lane_id();
+  const int nmat = s.nmat;
+  const bool cell_lane = lane >= 2 * nmat;
+  const int slot = cell_lane ? lane - 2 * nmat : lane;
+  const bool tally_lane = lane < 4 * nmat;
+  const int component = slot % nmat, group = slot / nmat;  // group 0: single scattering
+  const uint32_t plane = uint32_t(s.nx * s.ny);
+  const uint32_t cells = plane * uint32_t(s.nz);
+  const uint32_t pixel_values = plane * uint32_t(s.views) * 8u;
+  R geometry[kMaxViews];
+#pragma unroll
+  for (int view = 0; view < kMaxViews; ++view) geometry[view] = R(s.view_geometry[view]);
+  const R stokes_q = component == 1 ? R(1) : R(0), stokes_u = component == 2 ? R(1) : R(0);
+  const R stokes_v = component == 3 ? R(1) : R(0), stokes_i = component == 0 ? R(1) : R(0);
+  for (uint64_t mask = __ballot(has); mask; mask &= mask - 1) {
+    const int src = __ffsll(static_cast<long long>(mask)) - 1;
+    const Pending<R> source = wave_pending.at(src);
+    const int code = source.layer();
+    const bool single = code >= 0;
+    const uint32_t cell = uint32_t(single ? code : ~code);
+    R *bank = banks + s.bank_base + (size_t(source.round()) * s.shards + shard) * size_t(s.bank_count);
+    const bool carries = tally_lane && group == (single ? 0 : 1);
+#pragma unroll
+    for (int view = 0; view < kMaxViews; ++view) {
+      if (view >= s.views) break;
+      const int pixel = source.pixel(view);
+      const R q = source.q(view), u = source.u(view);
+      R value = source.weight(view) * geometry[view] * stokes_i + q * stokes_q + u * stokes_u;
+      if (V4) value += source.v(view) * stokes_v;
+      const uint32_t offset = cell_lane ? pixel_values + (uint32_t(view) * cells + cell) * 8u + uint32_t(slot)
+                                        : (uint32_t(view) * plane + uint32_t(pixel)) * 8u + uint32_t(slot);
+      if (pixel >= 0 && carries && value != R(0)) {
+        add_tally<R>(bank + offset, value);
+        ++n.atomics;
+      }
+    }
+  }
+}
+
 __device__ inline unsigned long long wave_sum(unsigned long long v) {
   for (int offset = 32; offset > 0; offset >>= 1) v += __shfl_xor(v, offset, 64);
   return v;
@@ -1180,6 +1247,7 @@ __global__ void __launch_bounds__(kBlock32)
                      monki_net::Net net, monki_net::EventLog log, Guidance guidance) {
   constexpr bool V4 = full_stokes(MODE);
   constexpr bool BACK = backward(MODE);
+  constexpr bool SCATS = decomposition(MODE);
   extern __shared__ unsigned char lds_raw[];
   // Layer rows of odd length (the largest nz of the scenes, | 1): lanes reading one row, or
   // each its own row at one layer, touch distinct LDS banks.
@@ -1233,11 +1301,14 @@ __global__ void __launch_bounds__(kBlock32)
 
     int status = kRunning;
     prefetch(h.rng);
-    if (active) status = collide<R, TRACE, V4, BACK>(s, c, failure, history, h, pending, n, trace);
+    if (active) status = collide<R, TRACE, V4, BACK, SCATS>(s, c, failure, history, h, pending, n, trace);
     // Lanes read each other's LDS rows below: keep every access of this wave in order.
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
-    tally_wave<R, V4>(s, banks, shard, active && status == kRunning, wave_pending, wave_history, row, n);
+    if (SCATS)
+      tally_wave_scats<R, V4>(s, banks, shard, active && status == kRunning, wave_pending, n);
+    else
+      tally_wave<R, V4>(s, banks, shard, active && status == kRunning, wave_pending, wave_history, row, n);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const bool scatters = active && status == kRunning && !h.surface;
@@ -1339,6 +1410,32 @@ __global__ void reduce_shards(const B *__restrict__ banks, double *__restrict__ 
 __global__ void assemble_tallies(Scene s, const double *__restrict__ reduced,
                                  double *__restrict__ out, int rounds) {
   const int plane = s.nx * s.ny;
+  if (s.scats) {
+    // [Stokes | Stokes_sing | Stokes_mult | contrib_sing | contrib_mult], Fortran order; the
+    // Stokes tally is the sum of the single- and multiple-scattering records.
+    const size_t bank = size_t(s.bank_count);
+    const size_t index = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (index >= bank * rounds) return;
+    const size_t r = index / bank, t = index % bank;
+    const size_t pixel_values = size_t(plane) * s.views * 8;
+    const bool cell_region = t >= pixel_values;
+    const size_t u = cell_region ? t - pixel_values : t;
+    const int slot = int(u % 8);
+    if (slot >= 2 * s.nmat) return;  // record padding
+    const int m = slot % s.nmat, group = slot / s.nmat;
+    const size_t record = u / 8;
+    const double value = reduced[r * bank + t];
+    double *dst = out + r * size_t(s.tally_count);
+    if (!cell_region) {
+      const size_t at = m + size_t(s.nmat) * record;  // record = pixel + plane view
+      dst[size_t(1 + group) * s.stokes_count + at] = value;
+      if (group == 0) dst[at] = value + reduced[r * bank + t + s.nmat];
+    } else {
+      const size_t contrib = size_t(s.nmat) * s.path_count;  // record = cell + cells view
+      dst[3 * size_t(s.stokes_count) + group * contrib + m + size_t(s.nmat) * record] = value;
+    }
+    return;
+  }
   const int record = s.paths ? s.nz + s.nmat : s.nmat;
   const int stride = s.record_stride;
   const size_t per_round = size_t(plane) * s.views * stride;
@@ -1553,7 +1650,7 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
       !word_int(t, 5, 1, kMaxViews, s.views) || !word_int(t, 6, 0, kMaxSpecies, s.species) ||
       !word_int(t, 7, 0, 1, s.paths) || !word_int(t, 8, 0, kMaxLayers, s.uniform_top) ||
       !word_int(t, 9, 1, 64, s.shards) || !word_int(t, 26, kHeaderWords, 1 << 24, length) ||
-      !word_int(t, 28, 0, 1, s.backward)) {
+      !word_int(t, 28, 0, 1, s.backward) || !word_int(t, 29, 0, 1, s.scats)) {
     set_error(error, size, "Invalid packed table dimensions");
     return false;
   }
@@ -1674,16 +1771,27 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
       return false;
     }
   }
+  if (s.scats && (s.paths || s.backward)) {
+    set_error(error, size, "The scattering decomposition runs forward without layer paths");
+    return false;
+  }
   s.stokes_count = s.nmat * s.nx * s.ny * s.views;
   s.path_count = s.nx * s.ny * s.nz * s.views;
   s.tally_count = s.stokes_count + (s.paths ? s.path_count + s.nx * s.ny * s.views : 0);
   s.record_stride = ((s.paths ? s.nz + s.nmat : s.nmat) + 7) & ~7;
+  size_t records = size_t(s.nx) * s.ny * s.views;
+  if (s.scats) {
+    // Stokes, single, multiple; single and multiple contributions per event cell.
+    s.tally_count = 3 * s.stokes_count + 2 * s.nmat * s.path_count;
+    s.record_stride = 8;
+    records *= size_t(1 + s.nz);
+  }
   // The tally addresses records by 32-bit byte offsets within one shard bank (float64 build).
-  if (size_t(s.nx) * s.ny * s.views * s.record_stride * sizeof(double) >= (size_t(1) << 31)) {
+  if (records * s.record_stride * sizeof(double) >= (size_t(1) << 31)) {
     set_error(error, size, "Tally bank of one shard exceeds 2 GiB");
     return false;
   }
-  s.bank_count = s.nx * s.ny * s.views * s.record_stride;
+  s.bank_count = int(records * s.record_stride);
   return true;
 }
 
@@ -1766,15 +1874,19 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
     set_error(error, error_size, "Invalid HIP session or arguments");
     return 1;
   }
-  if constexpr (MODE != kStokes4 && !backward(MODE)) {
+  if constexpr (MODE != kStokes4 && !backward(MODE) && !decomposition(MODE)) {
     // The full Stokes vector and backward transport have instantiations of their own of the
     // plain tracer.
     const Scene &first = h->scenes[0];
-    if (first.nmat == 4 || first.backward) {
+    if (first.nmat == 4 || first.backward || first.scats) {
       if (MODE != kPlain) {
         set_error(error, error_size, "The steered and logging tracers take forward nmat 1 or 3 scenes");
         return 1;
       }
+      if (first.scats && first.nmat == 4)
+        return launch<R, TRACE, kScats4>(h, controls, out, records, record_count, error, error_size, proposal);
+      if (first.scats)
+        return launch<R, TRACE, kScats>(h, controls, out, records, record_count, error, error_size, proposal);
       if (first.backward && first.nmat == 4)
         return launch<R, TRACE, kBackward4>(h, controls, out, records, record_count, error, error_size, proposal);
       if (first.backward)
@@ -2150,7 +2262,7 @@ void *monki_hip_batch_create(uint32_t device, const float *tables, const size_t 
                              uint32_t block, char *error, size_t error_size) {
   Session *h = create_session(device, tables, offsets, scene_count, 1, block, error, error_size);
   for (size_t i = 0; h && i < h->scenes.size(); ++i) {
-    if (h->scenes[i].nmat == 4 || h->scenes[i].backward) {
+    if (h->scenes[i].nmat == 4 || h->scenes[i].backward || h->scenes[i].scats) {
       release(h);
       set_error(error, error_size, "Batch sessions take forward nmat 1 or 3 scenes (run others one at a time)");
       return nullptr;

@@ -23,7 +23,7 @@ from ..input import ReferenceCase, SceneCase
 from ..metadata import file_hash, scene_metadata
 from ..preparation import UPSTREAM, prepare_case
 from .backend import Backend
-from .hip_tables import pack_tables, tally_count
+from .hip_tables import pack_tables, split_scattering, tally_count
 from .proposal import DirectedSettings, Proposal, ProposalSettings, declare, log_rounds
 from .tallies import BatchControls, RawTallies
 
@@ -123,6 +123,7 @@ class HIP(Backend):
         device: int = 0,
         *,
         save_path: bool = True,
+        save_scats: bool = False,
         profile: bool = False,
         seed: int = 23274,
         photons_per_dispatch: int = 1 << 24,
@@ -144,9 +145,13 @@ class HIP(Backend):
             if type(value) is not int or not minimum <= value <= maximum:
                 raise ValueError(f"{name} must be an integer in [{minimum}, {maximum}]")
 
-        for name, value in (("save_path", save_path), ("profile", profile)):
+        for name, value in (("save_path", save_path), ("save_scats", save_scats), ("profile", profile)):
             if type(value) is not bool:
                 raise ValueError(f"{name} must be a bool")
+
+        if save_scats and save_path:
+            # calc_radiances_and_scats replaces the path tallies (mc.f90:826-865).
+            raise ValueError("save_scats and save_path are exclusive: set save_path=False")
 
         if precision not in (32, 64) or type(precision) is not int:
             raise ValueError("precision must be 32 or 64")
@@ -165,6 +170,7 @@ class HIP(Backend):
 
         self.device = device
         self.save_path = save_path
+        self.save_scats = save_scats
         self.profile = profile
         self.seed = seed
         self.photons_per_dispatch = photons_per_dispatch
@@ -199,20 +205,26 @@ class HIP(Backend):
         if isinstance(scene, ReferenceCase):
             # The reference prepares fixtures with its Fortran worker (metal.py:98-104); the
             # same tables come from the host restatement here.
-            scene, prepared, preparation = prepare_case(scene, self.upstream, save_path=self.save_path)
+            scene, prepared, preparation = prepare_case(scene, self.upstream, save_path=self.save_path,
+                                                        save_scats=self.save_scats)  # fmt: skip
             extra = prepared.product_variables()
 
         if not isinstance(scene, SceneCase):
             raise TypeError("HIP.prepare expects a ReferenceCase or SceneCase")
 
-        tables = replace(scene.scene, save_path=self.save_path)
-        packed, self._shape = pack_tables(tables, shards=self.tally_shards, backward=scene.backward)
-        nmat, nx, ny, nz, views = self._shape
+        tables = replace(scene.scene, save_path=self.save_path, save_scats=self.save_scats)
 
-        if (nmat == 4 or scene.backward) and (self.proposal is not None or self.directed is not None):
+        if (tables.nmat == 4 or scene.backward or self.save_scats) and (
+            self.proposal is not None or self.directed is not None
+        ):
             raise NotImplementedError("the guided and detector-directed tracers take forward nmat 1 or 3 scenes")
 
-        per_round = tally_count(nmat, nx, ny, nz, views, self.save_path)
+        if self.save_scats and scene.backward:
+            raise NotImplementedError("the scattering decomposition runs with forward transport")
+
+        packed, self._shape = pack_tables(tables, shards=self.tally_shards, backward=scene.backward)
+        nmat, nx, ny, nz, views = self._shape
+        per_round = tally_count(nmat, nx, ny, nz, views, self.save_path, self.save_scats)
         self._tally_count = per_round
         bank_bytes = per_round * self.tally_shards * (4 if self.precision == 32 else 8)
         self._round_capacity = max(
@@ -520,6 +532,10 @@ class HIP(Backend):
                 stokes = values[:stokes_count].reshape((nmat, nx, ny, views), order="F").copy()
                 paths = weights = None
 
+                if self.save_scats:
+                    results.append(RawTallies(request, stokes, scattering=split_scattering(values, self._shape)))
+                    continue
+
                 if self.save_path:
                     path_count = nx * ny * nz * views
                     paths = (
@@ -541,6 +557,7 @@ class HIP(Backend):
         return (
             self.device,
             self.save_path,
+            self.save_scats,
             self.seed,
             self.photons_per_dispatch,
             self.max_steps,

@@ -19,6 +19,7 @@ Header (64 float32 words; integers are stored exactly, all < 2**24):
 27    offset of the per-layer uniformity flags (1: extinction and species mix are the same in
       every column of the layer, so the free path crosses it without cell-by-cell steps)
 28    transport direction: 0 forward, 1 backward (mc.f90 backward_mode)
+29    scattering decomposition: 1 runs calc_radiances_and_scats (no layer paths)
 32+8s species s: LUT size, albedo, offsets of cosine block (cos, 1-cos, 1+cos),
       CDF, F11, F22, F33, F12; with nmat 4, F34 and F44 follow F12 directly (same length)
 ====  =====================================================================
@@ -49,6 +50,9 @@ def pack_tables(scene, shards: int = 32, backward: bool = False):
 
     if nz > MAX_LAYERS:
         raise ValueError(f"at most {MAX_LAYERS} layers including vacuum are supported")
+
+    if scene.save_scats and scene.save_path:
+        raise ValueError("the scattering decomposition replaces the layer paths: save_path must be False")
 
     if scene.save_path and nz + nmat > 64:
         raise ValueError("layers (incl. vacuum) plus Stokes components must not exceed 64 with paths")
@@ -109,6 +113,7 @@ def pack_tables(scene, shards: int = 32, backward: bool = False):
     uniform[varying] = 0.0
     header[27] = place(uniform)
     header[28] = 1.0 if backward else 0.0
+    header[29] = 1.0 if scene.save_scats else 0.0
 
     for index, sp in enumerate(scene.species):
         cdf = sp.bound_cdf(nmat) if sp.cdf is None else sp.cdf
@@ -148,12 +153,35 @@ def pack_tables(scene, shards: int = 32, backward: bool = False):
     return packed, (nmat, nx, ny, nz, views)
 
 
-def tally_count(nmat: int, nx: int, ny: int, nz: int, views: int, paths: bool) -> int:
-    """Float count of one round's tallies: Stokes, then paths and weights when saved."""
+def tally_count(nmat: int, nx: int, ny: int, nz: int, views: int, paths: bool, scats: bool = False) -> int:
+    """Float count of one round's tallies: Stokes, then paths and weights when saved; with the
+    scattering decomposition Stokes, Stokes_sing, Stokes_mult, contrib_sing, contrib_mult."""
 
     count = nmat * nx * ny * views
+
+    if scats:
+        return 3 * count + 2 * nmat * nx * ny * nz * views
 
     if paths:
         count += nx * ny * (nz + 1) * views
 
     return count
+
+
+def split_scattering(values: np.ndarray, shape) -> dict:
+    """The decomposition arrays of one round's tallies (``tally_count(..., scats=True)``), in
+    Fortran order: Stokes_sing and Stokes_mult (nmat, nx, ny, views) and the single- and
+    multiple-scattering contributions by event cell (nmat, nx, ny, nz, views), level 0 for
+    surface reflections and k for physical layer k (calc_radiances_and_scats,
+    mc_routines_cartesian_mod.f90:405-836)."""
+
+    nmat, nx, ny, nz, views = shape
+    stokes = nmat * nx * ny * views
+    contrib = nmat * nx * ny * nz * views
+    part = lambda start, count, dims: values[start : start + count].reshape(dims, order="F").copy()
+    return {
+        "stokes_sing": part(stokes, stokes, (nmat, nx, ny, views)),
+        "stokes_mult": part(2 * stokes, stokes, (nmat, nx, ny, views)),
+        "contrib_sing": part(3 * stokes, contrib, (nmat, nx, ny, nz, views)),
+        "contrib_mult": part(3 * stokes + contrib, contrib, (nmat, nx, ny, nz, views)),
+    }

@@ -31,9 +31,14 @@ class RawTallies:
 
     Stokes axes are (component, x, y, angle); paths are (x, y, layer, angle), including
     the vacuum layer; weights are (x, y, angle). No buffer aliases device memory.
+    ``scattering`` (no reference counterpart: the reference's backends refuse the option)
+    holds the single/multiple-scattering decomposition when requested: ``stokes_sing``,
+    ``stokes_mult`` (like ``stokes``) and ``contrib_sing``, ``contrib_mult``
+    (component, x, y, level, angle), level 0 the surface.
     """
 
     controls: BatchControls
     stokes: np.ndarray
     weighted_paths: Optional[np.ndarray] = None
     weights: Optional[np.ndarray] = None
+    scattering: Optional[dict] = None

@@ -45,6 +45,12 @@ ATTRIBUTES = {
     "std_radiance": ("Standard deviation of TOA radiance", "-", "std"),
     "DOP": ("Degree of polarization", "-", "DOP"),
     "AMF_layer": ("Layer AMF", "-", "AMF_layer"),
+    # Scattering decomposition (Report_ncdf with save_scats, main.f90:1724-1850).
+    "radiance_sing": ("TOA radiance single scattering", "-", "radiance_sing"),
+    "radiance_mult": ("TOA radiance multiple scattering", "-", "radiance_mult"),
+    "contrib_radiance_3D": ("TOA radiance 3D contribution", "[-]", "contrib_radiance"),
+    "contrib_radiance_3D_sing": ("TOA radiance 3D singscat contribution", "-", "contrib_radiance_sing"),
+    "contrib_radiance_3D_mult": ("TOA radiance 3D multscat contribution", "-", "contrib_radiance_mult"),
 }
 
 
@@ -106,6 +112,18 @@ def scene_metadata(scene, photons_per_column: int, path: Path = Path("scene.nc")
         values["AMF_layer"] = np.zeros((angles, physical, ny, nx, 1))
         dimensions["AMF_layer"] = ("nvza", "nz", "ny", "nx", "nwavs")
         attributes["AMF_layer"] = product_attributes("AMF_layer")
+
+    if getattr(scene, "save_scats", False):
+        for name in ("radiance_sing", "radiance_mult"):
+            values[name] = np.zeros((angles, ny, nx, nmat, 1))
+            dimensions[name] = ("nvza", "ny", "nx", "nmat", "nwavs")
+            attributes[name] = product_attributes(name)
+
+        # Levels: 0 the surface, then the physical layers (nz_withsurf, main.f90:1519, 1542).
+        for name in ("contrib_radiance_3D", "contrib_radiance_3D_sing", "contrib_radiance_3D_mult"):
+            values[name] = np.zeros((angles, nz, ny, nx, nmat, 1))
+            dimensions[name] = ("nvza", "nz_withsurf", "ny", "nx", "nmat", "nwavs")
+            attributes[name] = product_attributes(name)
 
     for name, (value, dims, attrs) in (extra or {}).items():
         values[name] = value

@@ -744,12 +744,14 @@ def prepare_fixture(
 UPSTREAM = Path(__file__).resolve().parent / "data" / "upstream"
 
 
-def prepare_case(case, upstream: Path = UPSTREAM, *, save_path: bool = True):
+def prepare_case(case, upstream: Path = UPSTREAM, *, save_path: bool = True, save_scats: bool = False):
     """Prepare a ``ReferenceCase`` as the reference's CPU preparer does for Metal.
 
-    Checks of ``python/monki/execution/cpu.py:82-101``: the fixture's positional layout, the
-    layer-AMF setting equal to ``save_path`` and no scattering-contribution diagnostics (each
-    a ``ValueError``), then the preparation above with the case's geometry and albedo.
+    Checks of ``python/monki/execution/cpu.py:82-101``: the fixture's positional layout and the
+    layer-AMF setting equal to ``save_path`` (each a ``ValueError``), then the preparation
+    above with the case's geometry and albedo. The scattering-contribution setting must equal
+    ``save_scats`` (the reference's CPU path refuses it; here it selects the decomposition,
+    which replaces the layer paths as in mc.f90:826-865, so the AMF line is not checked then).
     Returns ``(SceneCase, PreparedFixture, provenance)``.
     """
 
@@ -763,11 +765,11 @@ def prepare_case(case, upstream: Path = UPSTREAM, *, save_path: bool = True):
     if len(case.view_zeniths) > 3:
         raise ValueError("case exceeds the transport limit of 3 viewing angles")
 
-    if (lines[38].split()[0].lower() == ".true.") != save_path:
-        raise ValueError("save_path must match the fixture's layer AMF setting")
+    if (lines[39].split()[0].lower() == ".true.") != save_scats:
+        raise ValueError("save_scats must match the fixture's scattering-contribution setting")
 
-    if lines[39].split()[0].lower() != ".false.":
-        raise ValueError("scattering contribution diagnostics are not implemented")
+    if not save_scats and (lines[38].split()[0].lower() == ".true.") != save_path:
+        raise ValueError("save_path must match the fixture's layer AMF setting")
 
     prepared = prepare_fixture(
         source,
@@ -777,9 +779,10 @@ def prepare_case(case, upstream: Path = UPSTREAM, *, save_path: bool = True):
         solar_azimuth=case.solar_azimuth,
         view_azimuth=case.view_azimuth,
         view_zeniths=case.view_zeniths,
-        save_path=save_path,
+        save_path=save_path and not save_scats,
         description=f"{case.input_file} ({case.name})",
     )
+    prepared.scene.save_scats = save_scats
     scene_case = SceneCase(case.name, prepared.scene, case.photons_per_column, case.rounds, case.backward)
     provenance = {
         "preparer": "monki.preparation (rdinput, molecules_earth, get_O3_crabs, aerosols restated)",

@@ -88,6 +88,8 @@ class TransportScene:
     view_zeniths: Tuple[float, ...]
     species: List[Species] = field(default_factory=list)
     save_path: bool = True
+    # Single/multiple-scattering decomposition (calc_radiances_and_scats); excludes save_path.
+    save_scats: bool = False
     wavelength_nm: float = 340.0
     altitude_m: Optional[np.ndarray] = None  # (nz,) level altitudes, bottom to TOA
     description: str = ""

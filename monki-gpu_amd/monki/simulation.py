@@ -44,6 +44,7 @@ class RoundStatistics:
         self.case = case
         self.metadata = metadata
         self.total = self.rounds_sum = self.rounds_squared = self.paths = self.weights = None
+        self.scattering = None  # sums of the decomposition arrays, when the backend returns them
         self.history = []
         self.stopping = None
 
@@ -68,7 +69,17 @@ class RoundStatistics:
             self.paths = np.zeros_like(batch.weighted_paths)
             self.weights = np.zeros_like(batch.weights)
 
+        if self.total is None and batch.scattering is not None:
+            self.scattering = {name: np.zeros_like(value) for name, value in batch.scattering.items()}
+
         self.add_stokes(batch.controls, batch.stokes)
+
+        if self.scattering is not None:
+            if batch.scattering is None:
+                raise ValueError("Backend dropped the scattering decomposition during a session")
+
+            for name, value in batch.scattering.items():
+                self.scattering[name] += value
 
         if self.paths is not None:
             if batch.weighted_paths is None or batch.weights is None:
@@ -166,6 +177,19 @@ class RoundStatistics:
             thickness = np.diff(values["alt"]) / 1000.0
             amf = paths[:, :, :-1, :] / weights[:, :, None, :] / thickness[None, None, :, None]
             values["AMF_layer"] = amf.transpose(3, 2, 1, 0)[..., None]
+
+        if self.scattering is not None:
+            # Sums over rounds / rounds / photons per column (main.f90:1168-1171), U reversed
+            # for backward transport like the radiance; the 3-D total is single + multiple.
+            scale = 1.0 / completed / case.photons_per_column
+            sing, mult = self.scattering["stokes_sing"] * scale, self.scattering["stokes_mult"] * scale
+            contrib_sing = self.scattering["contrib_sing"] * scale
+            contrib_mult = self.scattering["contrib_mult"] * scale
+            values["radiance_sing"] = sing.transpose(3, 2, 1, 0)[..., None]
+            values["radiance_mult"] = mult.transpose(3, 2, 1, 0)[..., None]
+            values["contrib_radiance_3D"] = (contrib_sing + contrib_mult).transpose(4, 3, 2, 1, 0)[..., None]
+            values["contrib_radiance_3D_sing"] = contrib_sing.transpose(4, 3, 2, 1, 0)[..., None]
+            values["contrib_radiance_3D_mult"] = contrib_mult.transpose(4, 3, 2, 1, 0)[..., None]
 
         dimensions = dict(metadata.dimensions)
         attributes = dict(metadata.attributes)

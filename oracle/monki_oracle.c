@@ -22,6 +22,8 @@
  *                                        948-955, 981, 1150-1151, 1324-1334, 1393-1395;
  *                                        sampling sign general_mod:583-587, 810; injection
  *                                        register cartesian_mod:1035-1039; no mu0/mu mc.f90:1456
+ *   scattering decomposition ........... calc_radiances_and_scats cartesian_mod:405-836 (forward;
+ *                                        order counter mc.f90:433, 689, 1084)
  *   mu0/mu scaling ..................... mc.f90:1456-1474
  *
  * Differences from the Fortran, all deliberate: Philox streams instead of RANDOM_NUMBER
@@ -60,12 +62,12 @@ typedef struct {
 } species_t;
 
 typedef struct {
-  int nx, ny, nz, nmat, views, species, paths, backward;
+  int nx, ny, nz, nmat, views, species, paths, backward, scats;
   double xunit, yunit, depol, theta0, phi0, phi;
   double view[MAX_VIEWS];
   const float *zunit, *albedo, *beta, *cumfrac, *ssalb_gas;
   species_t sp[MAX_SPECIES];
-  int stokes_count, path_count, tally_count;
+  int stokes_count, path_count, contrib_count, tally_count;
 } scene_t;
 
 /* ---- Philox4x32-10 (Random123 constants) ------------------------------------------- */
@@ -421,8 +423,11 @@ typedef struct {
 } sink_t;
 
 /* calc_radiances_and_paths_pol for one event and all views. Returns 0 on a walk limit. */
+/* With the decomposition (s->scats) the tallies are [Stokes | Stokes_sing | Stokes_mult |
+ * contrib_sing | contrib_mult]: nscat is the event's scattering order (surface reflections
+ * count), and the contribution also goes to the event cell, level 0 for a surface event. */
 static int detect(const scene_t *s, const pos_t *ev, double prob[MAX_VIEWS][4],
-                  const double *history, uint32_t max_steps, int column, sink_t *out) {
+                  const double *history, uint32_t max_steps, int column, int nscat, sink_t *out) {
   for (int v = 0; v < s->views; ++v) {
     if (!(prob[v][0] > EPS)) continue;
     double th = s->view[v];
@@ -463,6 +468,16 @@ static int detect(const scene_t *s, const pos_t *ev, double prob[MAX_VIEWS][4],
     double c1 = prob[v][0] * att;
     int pixel = pi + s->nx * (pj + s->ny * v);
     for (int m = 0; m < s->nmat; ++m) out->tallies[s->nmat * pixel + m] += prob[v][m] * att * mu0mu;
+    if (s->scats) {
+      int level = (ev->k == 0 && ev->z < EPS) ? 0 : ev->k + 1;
+      int group = nscat == 1 ? 0 : 1;
+      size_t cell = (size_t)ev->i + (size_t)s->nx * (ev->j + (size_t)s->ny * (level + (size_t)s->nz * v));
+      for (int m = 0; m < s->nmat; ++m) {
+        double value = prob[v][m] * att * mu0mu;
+        out->tallies[(size_t)(1 + group) * s->stokes_count + s->nmat * pixel + m] += value;
+        out->tallies[3 * (size_t)s->stokes_count + (size_t)group * s->contrib_count + s->nmat * cell + m] += value;
+      }
+    }
     if (out->record) out->record[4] += c1 * mu0mu;
     if (s->paths) {
       double total = 0.0;
@@ -502,6 +517,7 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
   double remains = -log(fmax(draw(&r), EPS));
   ++out->counters[0];
   double *rec = out->record;
+  int nscat = 0; /* events so far, surface reflections included (mc.f90:689, 1084) */
 
   for (uint32_t step = 0; step < max_steps; ++step) {
     double beta = (p.k < s->nz - 1 && p.k >= 0) ? s->beta[cell(s, p.i, p.j, p.k)] : 0.0;
@@ -537,6 +553,7 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
     if (surface) {
       weight *= s->albedo[p.i + s->nx * p.j];
       ++out->counters[2];
+      ++nscat;
       if (rec) rec[1] += 1.0;
     } else {
       double u = draw(&r);
@@ -549,6 +566,7 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
       }
       weight *= species < 0 ? s->ssalb_gas[p.k] : s->sp[species].ssalb;
       ++out->counters[1];
+      ++nscat;
       if (rec) rec[0] += 1.0;
     }
     if (weight < EPS) return ABSORBED;
@@ -588,7 +606,7 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
         return FAILED;
       }
     }
-    if (!detect(s, &p, prob, history, max_steps, column, out)) {
+    if (!detect(s, &p, prob, history, max_steps, column, nscat, out)) {
       *flag |= 2u;
       return FAILED;
     }
@@ -653,6 +671,8 @@ static int parse(const float *t, size_t count, scene_t *s) {
   if (count < HEADER || t[0] != 1.0f || (size_t)t[26] != count) return 1;
   s->nx = word(t, 1), s->ny = word(t, 2), s->nz = word(t, 3), s->nmat = word(t, 4);
   s->views = word(t, 5), s->species = word(t, 6), s->paths = word(t, 7), s->backward = word(t, 28);
+  s->scats = word(t, 29);
+  if (s->scats) s->paths = 0; /* calc_radiances_and_scats replaces the path tallies */
   if (s->nz > MAX_LAYERS || s->views > MAX_VIEWS || s->species > MAX_SPECIES) return 1;
   s->xunit = t[10], s->yunit = t[11], s->depol = t[12];
   s->theta0 = t[13], s->phi0 = t[14], s->phi = t[15];
@@ -670,7 +690,9 @@ static int parse(const float *t, size_t count, scene_t *s) {
   }
   s->stokes_count = s->nmat * s->nx * s->ny * s->views;
   s->path_count = s->nx * s->ny * s->nz * s->views;
+  s->contrib_count = s->nmat * s->nx * s->ny * s->nz * s->views;
   s->tally_count = s->stokes_count + (s->paths ? s->path_count + s->nx * s->ny * s->views : 0);
+  if (s->scats) s->tally_count = 3 * s->stokes_count + 2 * s->contrib_count;
   return 0;
 }
 

@@ -161,9 +161,11 @@ class Oracle:
 
 
 def run_reference(scene, photons_per_column: int, rounds: int, seed: int = 23274, threads: int = 1,
-                  executable: Path = REFERENCE, backward: bool = False):
+                  executable: Path = REFERENCE, backward: bool = False, save_scats: bool = False):
     """Run the reference Fortran transport (backward_mode = ``backward``); return (per-round
-    tallies, seconds per round, wall seconds)."""
+    tallies, seconds per round, wall seconds). With ``save_scats`` (one thread) each round's
+    row is [Stokes | Stokes_sing | Stokes_mult | contrib | contrib_sing | contrib_mult], the
+    3-D arrays (nmat, nx, ny, 0:nz, nangles) in Fortran order."""
 
     if not Path(executable).is_file():
         raise FileNotFoundError(f"{executable} is missing: run make -C oracle ref")
@@ -175,7 +177,8 @@ def run_reference(scene, photons_per_column: int, rounds: int, seed: int = 23274
                    OMP_STACKSIZE="128M", GOMP_STACKSIZE="128M")  # fmt: skip
         start = time.perf_counter()
         result = subprocess.run(
-            [str(executable), str(tables), str(photons_per_column), str(rounds), str(seed), str(output)],
+            [str(executable), str(tables), str(photons_per_column), str(rounds), str(seed), str(output),
+             "1" if save_scats else "0"],
             env=env, capture_output=True, text=True,
         )  # fmt: skip
         elapsed = time.perf_counter() - start
@@ -188,3 +191,20 @@ def run_reference(scene, photons_per_column: int, rounds: int, seed: int = 23274
         data = np.fromfile(output, dtype=np.float64).reshape(rounds, -1)
 
     return data, seconds, elapsed
+
+
+def fortran_scats_row(row, shape):
+    """The reference driver's decomposition row [Stokes | sing | mult | contrib | contrib_sing |
+    contrib_mult] (3-D arrays over levels 0..nz) in the product's order [Stokes | sing | mult |
+    contrib_sing | contrib_mult] (levels 0..nz-1; level nz, the vacuum layer, holds no events)."""
+
+    nmat, nx, ny, nz, views = shape
+    stokes = nmat * nx * ny * views
+    full = nmat * nx * ny * (nz + 1) * views
+    cut = lambda a: a.reshape(nmat, nx, ny, nz + 1, views, order="F")[:, :, :, :nz].ravel(order="F")
+    head = row[: 3 * stokes]
+    single = row[3 * stokes + full : 3 * stokes + 2 * full]
+    multiple = row[3 * stokes + 2 * full :]
+    if np.any(row[3 * stokes : 3 * stokes + full].reshape(nmat, nx, ny, nz + 1, views, order="F")[:, :, :, nz] != 0):
+        raise ValueError("contribution recorded in the vacuum layer")
+    return np.concatenate([head, cut(single), cut(multiple)])

@@ -369,6 +369,57 @@ def test_oracle_matches_reference_fortran_statistically(oracle, name):
     assert 0.75 < stats["chi_square_per_dof"] < 1.35
 
 
+@pytest.mark.skipif(not (Path(__file__).parents[1] / "oracle" / "_ref" / "monki_ref").is_file(),
+                    reason="reference Fortran build missing (make -C oracle ref)")  # fmt: skip
+def test_oracle_decomposition_matches_reference_fortran(oracle):
+    """Single/multiple-scattering decomposition (calc_radiances_and_scats): the oracle's
+    Stokes_sing/mult and 3-D contributions against the reference's unchanged mc.f90 run with
+    save_scats, within combined SE; and the identities the decomposition fixes."""
+
+    from dataclasses import replace
+
+    from monki import configs
+    from monki.execution.hip_tables import pack_tables
+    from monki.validation import combined_se_statistics
+    from oracle.oracle import fortran_scats_row, run_reference
+
+    scene = replace(configs.box_cloud(nx=4, ny=3, nmat=3, box=(2, 1), offset=(1, 1)), solar_azimuth=30.0,
+                    view_azimuth=120.0, save_path=False, save_scats=True)  # fmt: skip
+    rounds, ppc = 40, 60
+    per_round = ppc * scene.nx * scene.ny
+    packed, shape = pack_tables(scene, shards=1)
+    ours, _, _, flag = oracle.run(packed, [per_round * rounds, 0, 1, 5, 100000, 0, per_round, rounds])
+    assert flag == 0
+    theirs, _, _ = run_reference(scene, ppc, rounds, threads=1, save_scats=True)
+    theirs = np.stack([fortran_scats_row(row, shape) for row in theirs])
+    assert ours.shape == theirs.shape
+    nmat, nx, ny, nz, views = shape
+    stokes = nmat * nx * ny * views
+
+    for tallies in (ours, theirs):
+        # Stokes = single + multiple, and the contributions by event cell sum to the Stokes
+        # tallies of each view and component.
+        np.testing.assert_allclose(tallies[:, :stokes], tallies[:, stokes : 2 * stokes] + tallies[:, 2 * stokes : 3 * stokes],
+                                   rtol=1e-12, atol=1e-12)  # fmt: skip
+        contrib = tallies[:, 3 * stokes :].reshape(rounds, nmat, nx * ny * nz, views, 2, order="F")
+        by_view = tallies[:, :stokes].reshape(rounds, nmat, nx * ny, views, order="F").sum(axis=2)
+        np.testing.assert_allclose(contrib.sum(axis=(2, 4)), by_view, rtol=1e-9, atol=1e-9)
+
+    a, b = ours / ppc, theirs / ppc
+    # Floor: single-scattered U at nadir is zero up to the rounding of the float32 geometry
+    # (1e-9 here against I ~ 0.1), which its near-zero round-to-round spread cannot absorb.
+    stats = combined_se_statistics(
+        a.mean(axis=0), a.std(axis=0, ddof=1) / np.sqrt(rounds),
+        b.mean(axis=0), b.std(axis=0, ddof=1) / np.sqrt(rounds), floor=1e-7,
+    )  # fmt: skip
+    print(stats)
+    assert stats["max_abs_z"] < 5.5
+    assert 0.75 < stats["chi_square_per_dof"] < 1.35
+    # Single scattering is a resolved share of the total.
+    share = a[:, stokes : 2 * stokes : nmat].sum() / a[:, 0:stokes:nmat].sum()
+    assert 0.05 < share < 0.95
+
+
 # ---- Learned proposal: the float64 restatement of the network ------------------------------
 
 

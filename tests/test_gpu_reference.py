@@ -85,6 +85,50 @@ def test_backward_transport_matches_reference_fortran(tmp_path, nmat):
     assert np.median(np.abs(mean[2]) / np.maximum(se[2], 1e-30)) > 3  # U resolved
 
 
+def test_decomposition_matches_reference_fortran(tmp_path):
+    """The single/multiple-scattering decomposition against the reference Fortran run with
+    save_scats (calc_radiances_and_scats): per-round means within combined SE, and the NetCDF
+    product of simulate carries those means."""
+
+    from monki import SceneCase, configs, simulate
+    from monki.execution import HIP, BatchControls
+    from monki.execution.hip_tables import pack_tables
+    from monki.validation import combined_se_statistics
+    from oracle.oracle import fortran_scats_row, run_reference
+
+    scene = replace(configs.box_cloud(nx=8, ny=6, nmat=3, box=(3, 2), offset=(2, 1)), solar_azimuth=30.0,
+                    view_azimuth=120.0, save_path=False, save_scats=True)  # fmt: skip
+    rounds, ppc = 40, 100
+    case = SceneCase("scats", scene, ppc, rounds)
+
+    with HIP(save_path=False, save_scats=True, seed=77) as backend:
+        backend.prepare(case)
+        batches = backend.run_batches([BatchControls(index, ppc) for index in range(1, rounds + 1)])
+
+    _, shape = pack_tables(scene, shards=1)
+    flat = lambda b: np.concatenate([b.stokes.ravel(order="F")] + [b.scattering[k].ravel(order="F") for k in
+                                    ("stokes_sing", "stokes_mult", "contrib_sing", "contrib_mult")])  # fmt: skip
+    ours = np.stack([flat(b) for b in batches]) / ppc
+    tallies, _, _ = run_reference(scene, ppc, rounds, threads=1, save_scats=True)
+    theirs = np.stack([fortran_scats_row(row, shape) for row in tallies]) / ppc
+    stats = combined_se_statistics(
+        ours.mean(axis=0), ours.std(axis=0, ddof=1) / np.sqrt(rounds),
+        theirs.mean(axis=0), theirs.std(axis=0, ddof=1) / np.sqrt(rounds), floor=1e-7,
+    )  # fmt: skip
+    print("decomposition", json.dumps(stats))
+    assert stats["max_abs_z"] < 5.5
+    assert 0.7 < stats["chi_square_per_dof"] < 1.4
+
+    product = simulate(case, HIP(save_path=False, save_scats=True, seed=77), tmp_path / "scats.nc")
+    sing = (sum(b.scattering["stokes_sing"] for b in batches) / rounds / ppc).transpose(3, 2, 1, 0)
+    # Same streams; float32 bank sums differ only in atomic order.
+    np.testing.assert_allclose(product.values["radiance_sing"][..., 0], sing, rtol=2e-5, atol=2e-5 * np.abs(sing).max())
+    total = product.values["contrib_radiance_3D"][..., 0]
+    np.testing.assert_allclose(total, product.values["contrib_radiance_3D_sing"][..., 0]
+                               + product.values["contrib_radiance_3D_mult"][..., 0], rtol=1e-12)  # fmt: skip
+    assert total.shape == (3, scene.nz, scene.ny, scene.nx, 3)
+
+
 def test_scalar_transport_matches_reference_fortran(tmp_path):
     from monki import configs
 

@@ -228,6 +228,50 @@ def test_circular_polarization_matches_oracle(oracle):
     assert abs(g[3].sum() - c[3].sum()) <= 0.05 * np.abs(c[3]).sum()
 
 
+@pytest.mark.parametrize("precision", [32, 64])
+def test_decomposition_matches_oracle(oracle, precision):
+    """Single/multiple-scattering decomposition (save_scats) against the oracle on the same
+    streams: Stokes = single + multiple exactly, the contributions by event cell add up to the
+    detector tallies, and every part follows the oracle's tallies."""
+
+    from monki import SceneCase, configs
+    from monki.execution import HIP, BatchControls
+    from monki.execution.hip_tables import pack_tables, split_scattering
+
+    scene = replace(configs.box_cloud(nx=8, ny=6, nmat=3, box=(3, 2), offset=(2, 1)), solar_azimuth=30.0,
+                    view_azimuth=120.0, save_path=False, save_scats=True)  # fmt: skip
+    ppc = 100
+    per_round = ppc * scene.nx * scene.ny
+
+    with HIP(tally_shards=1, save_path=False, save_scats=True, precision=precision) as backend:
+        backend.prepare(SceneCase("scats", scene, ppc, 2))
+        gpu, _ = backend.trace(BatchControls(1, ppc), per_round, precision)
+
+    packed, shape = pack_tables(scene, shards=1)
+    cpu, _, _, flag = oracle.run(packed, [per_round, 0, 1, 23274, 100000, 0, per_round, 1])
+    assert flag == 0
+    cpu = cpu[0]
+    nmat, nx, ny, nz, views = shape
+    stokes = nmat * nx * ny * views
+    assert gpu.size == cpu.size == 3 * stokes + 2 * nmat * nx * ny * nz * views
+    np.testing.assert_allclose(gpu[:stokes], gpu[stokes : 2 * stokes] + gpu[2 * stokes : 3 * stokes], rtol=1e-12)
+    parts_gpu, parts_cpu = split_scattering(gpu, shape), split_scattering(cpu, shape)
+    by_view = gpu[:stokes].reshape(nmat, nx * ny, views, order="F").sum(axis=1)
+    contrib = (parts_gpu["contrib_sing"] + parts_gpu["contrib_mult"]).sum(axis=(1, 2, 3))
+    # float32 bank sums: Q and U sums cancel, so their tolerance is taken at the I scale.
+    tolerance = 1e-6 if precision == 64 else 1e-4
+    np.testing.assert_allclose(contrib, by_view, rtol=tolerance, atol=tolerance * np.abs(by_view).max())
+    rtol = 1e-4 if precision == 64 else 2e-2
+
+    for name in parts_gpu:
+        g, c = parts_gpu[name][0], parts_cpu[name][0]  # intensity
+        assert c.sum() > 0
+        assert abs(g.sum() - c.sum()) <= rtol * c.sum(), name
+        close = np.abs(g - c) <= 10 * rtol * np.abs(c).max()
+        print(name, precision, "close", close.mean())
+        assert close.mean() > 0.95, name
+
+
 def test_hip_never_uses_cpu_transport():
     from monki.execution import HIP, BatchControls
 

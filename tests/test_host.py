@@ -544,3 +544,41 @@ def test_directed_settings_validate_and_build_species_masks():
     for bad in ({"alpha": 0.0}, {"alpha": 1.5}, {"kappa": 0.5}, {"kappa": 2e6}, {"max_order": 0}, {"max_order": 1.0}):
         with pytest.raises(ValueError):
             DirectedSettings(**bad)
+
+
+def test_decomposition_settings_layout_and_product_variables():
+    """save_scats: exclusive with layer paths, matched against the fixture line, tallies split
+    in the documented order, and the Report_ncdf variables in the metadata."""
+
+    from monki import ReferenceCase, configs
+    from monki.execution import HIP
+    from monki.execution.hip_tables import pack_tables, split_scattering, tally_count
+    from monki.metadata import scene_metadata
+    from monki.preparation import prepare_case
+
+    with pytest.raises(ValueError, match="exclusive"):
+        HIP(save_scats=True)
+
+    with pytest.raises(ValueError, match="scattering-contribution"):
+        prepare_case(ReferenceCase("clear", "testrun/MONKI_MLS_cloudfree_340_340_test.in"), save_scats=True)
+
+    scene = replace(configs.box_cloud(nx=3, ny=2, nmat=3, box=(1, 1), offset=(1, 0)), save_path=False, save_scats=True)
+    packed, shape = pack_tables(scene, shards=1)
+    assert packed[29] == 1 and packed[7] == 0
+    nmat, nx, ny, nz, views = shape
+    count = tally_count(nmat, nx, ny, nz, views, False, True)
+    assert count == 3 * nmat * nx * ny * views + 2 * nmat * nx * ny * nz * views
+    values = np.arange(count, dtype=np.float64)
+    parts = split_scattering(values, shape)
+    stokes = nmat * nx * ny * views
+    assert parts["stokes_sing"].ravel(order="F")[0] == stokes
+    assert parts["contrib_mult"].shape == (nmat, nx, ny, nz, views)
+    assert parts["contrib_mult"].ravel(order="F")[-1] == count - 1
+
+    with pytest.raises(ValueError, match="save_path"):
+        pack_tables(replace(scene, save_path=True))
+
+    metadata = scene_metadata(scene, 10)
+    assert metadata.dimensions["contrib_radiance_3D_sing"] == ("nvza", "nz_withsurf", "ny", "nx", "nmat", "nwavs")
+    assert metadata.values["radiance_mult"].shape == (views, ny, nx, nmat, 1)
+    assert metadata.attributes["radiance_sing"]["long_name"] == "TOA radiance single scattering"

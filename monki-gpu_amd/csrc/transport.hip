@@ -712,9 +712,13 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
 // or the species' single-scattering albedo (mc.f90:681-727, 1028-1074), and evaluate the local
 // estimate toward every view (mc.f90:738-822, 1109-1207; mc_routines_cartesian_mod.f90:849-1222)
 // into `out`. Returns kRunning while the history continues, else its end status.
-template <typename R, bool TRACE, bool V4 = false, bool BACK = false, bool SCATS = false>
+// layers (LAYERS): the wave's LDS copy of the per-layer extinction of horizontally uniform
+// layers (-1 for varying ones) and of the layer thicknesses, read instead of the global
+// tables on every layer crossing of the free path.
+template <typename R, bool TRACE, bool V4 = false, bool BACK = false, bool SCATS = false, bool LAYERS = false>
 __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *history,
-                       History<R> &h, const Pending<R> &out, Counts &n, Trace<R> &trace) {
+                       History<R> &h, const Pending<R> &out, Counts &n, Trace<R> &trace,
+                       const float *layers = nullptr, int row = 0) {
   const R eps = R(1e-16);
   Photon<R> &p = h.p;
   bool surface = false;
@@ -733,11 +737,14 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
       return kFailed;
     }
     // Entering a varying layer: apply the deferred horizontal move before the column matters.
-    if (s.uniform[p.k] == 0.0f) flush_slide();
-    const R extinction =
-        (p.k < s.nz - 1 && p.k >= 0) ? R(s.beta[cell_index<R>(s, p.i, p.j, p.k)]) : R(0);
+    const float layer_extinction = LAYERS ? layers[p.k] : 0.0f;
+    const bool uniform_layer = LAYERS ? layer_extinction >= 0.0f : s.uniform[p.k] != 0.0f;
+    if (!uniform_layer) flush_slide();
+    const R extinction = LAYERS && uniform_layer ? R(layer_extinction)
+                         : (p.k < s.nz - 1 && p.k >= 0) ? R(s.beta[cell_index<R>(s, p.i, p.j, p.k)])
+                                                        : R(0);
     const R event_distance = extinction > eps ? h.remains / extinction : R(INFINITY);
-    if (s.uniform[p.k] != 0.0f) {
+    if (uniform_layer) {
       // Horizontally uniform layer: straight to the event or to the next z face. The path is
       // the one the cell-by-cell walk takes (x/y faces change nothing here).
       const R to_face = p.rz / fmax(fabs(p.c), eps);
@@ -760,7 +767,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
       p.k += p.c >= R(0) ? 1 : -1;
       if (p.k >= s.nz) return kEscaped;
       if (p.k >= 0) {
-        p.rz = R(s.zunit[p.k]);
+        p.rz = LAYERS ? R(layers[row + p.k]) : R(s.zunit[p.k]);
         continue;
       }
       flush_slide();
@@ -1267,6 +1274,14 @@ __global__ void __launch_bounds__(kBlock32)
           (threadIdx.x & ~63u),
       block};
   const Pending<R> pending = wave_pending.at(__lane_id());
+  // Per-wave layer table after the pending words (plain-width Stokes modes only: the full
+  // Stokes vector's pending rows leave no room at three blocks per CU).
+  constexpr bool LAYERS = !V4;
+  float *layers = reinterpret_cast<float *>(reinterpret_cast<int *>(reinterpret_cast<R *>(lds_raw) +
+                                                                    size_t(block) * (row + pending_reals(V4))) +
+                                            size_t(block) * kPendingWords) +
+                  (threadIdx.x >> 6) * 2 * row;
+  uint32_t layers_scene = 0xffffffffu;
   Counts n{0, 0, 0, 0, 0, 0, 0, 0};
   History<R> h;
   Trace<R> trace;
@@ -1277,6 +1292,16 @@ __global__ void __launch_bounds__(kBlock32)
     const Scene &s = scenes[current];
     const Controls &c = controls[current];
     const int shard = int(wave % uint32_t(s.shards));
+    if (LAYERS && layers_scene != current) {
+      // The wave's scene changed (or first iteration): load its layer table (wave-uniform).
+      for (int k = __lane_id(); k < s.nz; k += 64) {
+        layers[k] = s.uniform[k] != 0.0f ? s.beta[cell_index<R>(s, 0, 0, k)] : -1.0f;
+        layers[row + k] = s.zunit[k];
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      layers_scene = current;
+    }
     const unsigned long long idle = __ballot(!active);
     if (idle) {
       const int leader = __ffsll(static_cast<long long>(idle)) - 1;
@@ -1301,7 +1326,8 @@ __global__ void __launch_bounds__(kBlock32)
 
     int status = kRunning;
     prefetch(h.rng);
-    if (active) status = collide<R, TRACE, V4, BACK, SCATS>(s, c, failure, history, h, pending, n, trace);
+    if (active)
+      status = collide<R, TRACE, V4, BACK, SCATS, LAYERS>(s, c, failure, history, h, pending, n, trace, layers, row);
     // Lanes read each other's LDS rows below: keep every access of this wave in order.
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -1849,7 +1875,10 @@ int block_size(const Session *h) {
 
 template <typename R>
 size_t lds_bytes(const Session *h, int block, bool v4 = false) {
-  return (size_t(h->row) + pending_reals(v4)) * block * sizeof(R) + size_t(kPendingWords) * block * sizeof(int);
+  // History rows and pending rows per lane; pending words; per-wave layer tables (2 rows of
+  // floats per wave) except with the full Stokes vector.
+  return (size_t(h->row) + pending_reals(v4)) * block * sizeof(R) + size_t(kPendingWords) * block * sizeof(int) +
+         (v4 ? 0 : size_t(block / 64) * 2 * h->row * sizeof(float));
 }
 
 bool finish(Session *h, char *error, size_t error_size) {

@@ -583,9 +583,10 @@ struct DetectorHit {
 // optical depth from the event to the top of the last physical layer and the TOA pixel.
 // Cells are walked one by one below uniform_top; above it the vertical prefix gives the exact
 // slant integral (transport.metal:299-313).
-template <typename R>
+template <typename R, bool LAYERS = false>
 __device__ inline DetectorHit detector_walk(const Scene &s, const Photon<R> &event, R va, R vb,
-                                            R mu, R &tau, uint32_t max_steps, Counts &n) {
+                                            R mu, R &tau, uint32_t max_steps, Counts &n,
+                                            const float *layers = nullptr, int row = 0) {
   Photon<R> ray = event;
   change_direction<R>(s, ray, va, vb, mu, R(s.phi));
   tau = R(0);
@@ -593,9 +594,13 @@ __device__ inline DetectorHit detector_walk(const Scene &s, const Photon<R> &eve
   while (ray.k >= 0 && ray.k < s.nz - 1) {
     if (++steps > max_steps) return DetectorHit{0, 0, false};
     if (ray.k >= s.uniform_top) {
-      const R distance = (ray.rz + R(s.height_above[ray.k])) / mu;
-      tau += (ray.rz * R(s.beta[cell_index<R>(s, ray.i, ray.j, ray.k)]) +
-              R(s.tau_above[ray.k])) / mu;
+      // Layers at and above uniform_top are uniform: the wave's LDS layer table holds their
+      // extinction, height above and optical depth above.
+      const R height_above = LAYERS ? R(layers[2 * row + ray.k]) : R(s.height_above[ray.k]);
+      const R tau_above = LAYERS ? R(layers[3 * row + ray.k]) : R(s.tau_above[ray.k]);
+      const R beta = LAYERS ? R(layers[ray.k]) : R(s.beta[cell_index<R>(s, ray.i, ray.j, ray.k)]);
+      const R distance = (ray.rz + height_above) / mu;
+      tau += (ray.rz * beta + tau_above) / mu;
       // Explicit-equivalent cell count: layers left plus horizontal faces crossed.
       const R hx = fabs(va) * distance, hy = fabs(vb) * distance;
       n.detector += uint32_t(s.nz - 1 - ray.k) +
@@ -712,9 +717,10 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
 // or the species' single-scattering albedo (mc.f90:681-727, 1028-1074), and evaluate the local
 // estimate toward every view (mc.f90:738-822, 1109-1207; mc_routines_cartesian_mod.f90:849-1222)
 // into `out`. Returns kRunning while the history continues, else its end status.
-// layers (LAYERS): the wave's LDS copy of the per-layer extinction of horizontally uniform
-// layers (-1 for varying ones) and of the layer thicknesses, read instead of the global
-// tables on every layer crossing of the free path.
+// layers (LAYERS): the wave's LDS copy of four per-layer rows (row values apart): extinction of
+// horizontally uniform layers (-1 for varying ones), thickness, height above and optical depth
+// above, read instead of the global tables on every layer crossing of the free path and by
+// the detector walk.
 template <typename R, bool TRACE, bool V4 = false, bool BACK = false, bool SCATS = false, bool LAYERS = false>
 __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *history,
                        History<R> &h, const Pending<R> &out, Counts &n, Trace<R> &trace,
@@ -839,7 +845,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
   }
   out.round() = int(h.round_offset);
   // Detector ray inside the event layer: the distance to its top face (views point upward).
-  const R rz = p.c < R(0) ? fmax(R(0), R(s.zunit[p.k]) - p.rz) : p.rz;
+  const R rz = p.c < R(0) ? fmax(R(0), (LAYERS ? R(layers[row + p.k]) : R(s.zunit[p.k])) - p.rz) : p.rz;
   out.rz() = rz;
   // All views share one azimuth: cos and sin of the photon-view azimuth difference, once.
   const R delta = p.phi - R(s.phi);
@@ -895,7 +901,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
     if (prob.i <= eps) continue;
     R tau;
     const DetectorHit hit =
-        detector_walk<R>(s, p, R(s.view_a[view]), R(s.view_b[view]), mu, tau, c.max_steps, n);
+        detector_walk<R, LAYERS>(s, p, R(s.view_a[view]), R(s.view_b[view]), mu, tau, c.max_steps, n, layers, row);
     if (!hit.ok) {
       atomicOr(failure, kDetectorLimit);
       return kFailed;
@@ -1280,7 +1286,7 @@ __global__ void __launch_bounds__(kBlock32)
   float *layers = reinterpret_cast<float *>(reinterpret_cast<int *>(reinterpret_cast<R *>(lds_raw) +
                                                                     size_t(block) * (row + pending_reals(V4))) +
                                             size_t(block) * kPendingWords) +
-                  (threadIdx.x >> 6) * 2 * row;
+                  (threadIdx.x >> 6) * 4 * row;
   uint32_t layers_scene = 0xffffffffu;
   Counts n{0, 0, 0, 0, 0, 0, 0, 0};
   History<R> h;
@@ -1297,6 +1303,8 @@ __global__ void __launch_bounds__(kBlock32)
       for (int k = __lane_id(); k < s.nz; k += 64) {
         layers[k] = s.uniform[k] != 0.0f ? s.beta[cell_index<R>(s, 0, 0, k)] : -1.0f;
         layers[row + k] = s.zunit[k];
+        layers[2 * row + k] = s.height_above[k];
+        layers[3 * row + k] = s.tau_above[k];
       }
       __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
       __builtin_amdgcn_wave_barrier();
@@ -1875,10 +1883,10 @@ int block_size(const Session *h) {
 
 template <typename R>
 size_t lds_bytes(const Session *h, int block, bool v4 = false) {
-  // History rows and pending rows per lane; pending words; per-wave layer tables (2 rows of
+  // History rows and pending rows per lane; pending words; per-wave layer tables (4 rows of
   // floats per wave) except with the full Stokes vector.
   return (size_t(h->row) + pending_reals(v4)) * block * sizeof(R) + size_t(kPendingWords) * block * sizeof(int) +
-         (v4 ? 0 : size_t(block / 64) * 2 * h->row * sizeof(float));
+         (v4 ? 0 : size_t(block / 64) * 4 * h->row * sizeof(float));
 }
 
 bool finish(Session *h, char *error, size_t error_size) {

@@ -40,7 +40,8 @@ void *monki_hip_create(uint32_t device, const float *tables, size_t table_count,
  * float64 on the device, and copied to out: rounds x (tally_count / shards) doubles, per round
  * [Stokes | paths | weights], or with the scattering decomposition (table header word 29)
  * [Stokes | Stokes_sing | Stokes_mult | contrib_sing | contrib_mult]. The tables select the
- * transport: nmat 1, 3 or 4, forward or backward (header word 28).
+ * transport: nmat 1, 3 or 4, forward or backward (header word 28), detector at TOA or at the
+ * ground (header word 30).
  * Returns 0, or nonzero with error text (the round group is then rejected).
  * Replaces monki_metal_run (kernels/metal/bridge.m:88-133), whose out is float per shard. */
 int monki_hip_run(void *handle, const uint32_t *controls, double *out, char *error,

@@ -91,6 +91,9 @@ struct SpeciesTables {
 struct Scene {
   int nx, ny, nz, nmat, views, species, paths, uniform_top, shards;
   int backward;  // backward transport: weight-matrix rows, injection-column pixels, no mu0/mu
+  // Detector and result grid at the surface: views point down (view_mu < 0), the detector ray
+  // ends at the ground and registers at its exit column (main.f90:711-745).
+  int detector_surface;
   // Scattering decomposition: banks hold 8-value records [single (nmat) | multiple (nmat)],
   // one per view and pixel, then one per view and event cell (level 0 the surface).
   int scats;
@@ -593,7 +596,7 @@ __device__ inline DetectorHit detector_walk(const Scene &s, const Photon<R> &eve
   uint32_t steps = 0;
   while (ray.k >= 0 && ray.k < s.nz - 1) {
     if (++steps > max_steps) return DetectorHit{0, 0, false};
-    if (ray.k >= s.uniform_top) {
+    if (!s.detector_surface && ray.k >= s.uniform_top) {
       // Layers at and above uniform_top are uniform: the wave's LDS layer table holds their
       // extinction, height above and optical depth above.
       const R height_above = LAYERS ? R(layers[2 * row + ray.k]) : R(s.height_above[ray.k]);
@@ -845,7 +848,9 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
   }
   out.round() = int(h.round_offset);
   // Detector ray inside the event layer: the distance to its top face (views point upward).
-  const R rz = p.c < R(0) ? fmax(R(0), (LAYERS ? R(layers[row + p.k]) : R(s.zunit[p.k])) - p.rz) : p.rz;
+  // (Views of a surface detector point down: the distance to the bottom face instead.)
+  const R layer_thickness = LAYERS ? R(layers[row + p.k]) : R(s.zunit[p.k]);
+  const R rz = (p.c < R(0)) != (s.detector_surface != 0) ? fmax(R(0), layer_thickness - p.rz) : p.rz;
   out.rz() = rz;
   // All views share one azimuth: cos and sin of the photon-view azimuth difference, once.
   const R delta = p.phi - R(s.phi);
@@ -859,10 +864,12 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
     if (surface) {
       if (BACK) {
         // W As S0 = the first column of W (mc.f90:754-755).
-        const R scale = mu * h.weight;
+        const R scale = fmax(mu, R(0)) * h.weight;
         prob = Stokes<R>{h.st.i * scale, h.w2.i * scale, h.w3.i * scale, V4 ? h.w4.i * scale : R(0)};
       } else {
-        prob = Stokes<R>{h.st.i * mu * h.weight, R(0), R(0)};
+        // Surface reflections only travel upward (mc.f90:804-808): nothing toward a view that
+        // points down.
+        prob = Stokes<R>{h.st.i * fmax(mu, R(0)) * h.weight, R(0), R(0)};
       }
     } else {
       // Sines floored at sqrt(1e-16) as mc.f90:1112-1113 does: the forward peak of a Mie
@@ -921,7 +928,12 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
       trace.stokes_i += ci * geometry;
       if (s.paths) {
         trace.weights += ci;
-        trace.paths += ci * (h.path_total + (rz + R(s.height_above[p.k])) / mu);
+        // Layers the detector ray crosses after the event layer: above it, or below it for a
+        // surface detector.
+        const R beyond = s.detector_surface
+                             ? R(s.height_above[0] + s.zunit[0]) - R(s.height_above[p.k]) - R(s.zunit[p.k])
+                             : R(s.height_above[p.k]);
+        trace.paths += ci * (h.path_total + (rz + beyond) / fabs(mu));
       }
     }
   }
@@ -1159,7 +1171,7 @@ __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int sha
     const R rz = source.rz();
     R *bank = banks + s.bank_base + (size_t(source.round()) * s.shards + shard) * size_t(s.bank_count);
     const R path = path_lane ? wave_history[src * row + lane] : R(0);
-    const bool at_layer = lane == layer, above = lane > layer;
+    const bool at_layer = lane == layer, above = s.detector_surface ? lane < layer : lane > layer;
 #pragma unroll
     for (int view = 0; view < kMaxViews; ++view) {
       if (view >= s.views) break;
@@ -1684,7 +1696,8 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
       !word_int(t, 5, 1, kMaxViews, s.views) || !word_int(t, 6, 0, kMaxSpecies, s.species) ||
       !word_int(t, 7, 0, 1, s.paths) || !word_int(t, 8, 0, kMaxLayers, s.uniform_top) ||
       !word_int(t, 9, 1, 64, s.shards) || !word_int(t, 26, kHeaderWords, 1 << 24, length) ||
-      !word_int(t, 28, 0, 1, s.backward) || !word_int(t, 29, 0, 1, s.scats)) {
+      !word_int(t, 28, 0, 1, s.backward) || !word_int(t, 29, 0, 1, s.scats) ||
+      !word_int(t, 30, 0, 1, s.detector_surface)) {
     set_error(error, size, "Invalid packed table dimensions");
     return false;
   }
@@ -1717,8 +1730,9 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
     s.view_a[v] = sin(theta) * cos(ph);
     s.view_b[v] = sin(theta) * sin(ph);
     // mu0 / mu converts source to detector area in forward transport only (mc.f90:1456).
-    s.view_geometry[v] = v < s.views ? (s.backward ? 1.0 : s.mu0 / mu) : 0.0;
-    s.view_inv_mu[v] = v < s.views ? 1.0 / mu : 0.0;
+    // |mu|: the views of a surface detector point down (mc.f90:1456-1474 takes DABS).
+    s.view_geometry[v] = v < s.views ? (s.backward ? 1.0 : s.mu0 / fabs(mu)) : 0.0;
+    s.view_inv_mu[v] = v < s.views ? 1.0 / fabs(mu) : 0.0;
   }
   struct Span {
     int word;
@@ -1800,8 +1814,9 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
     tab.rho = float(rho * (1.0 + 1e-6));
   }
   for (int v = 0; v < s.views; ++v) {
-    if (!(cosf(s.view[v]) > 0.0f)) {
-      set_error(error, size, "Viewing angles must remain above the horizon");
+    // TOA views point up, the views of a surface detector down; none along the horizon.
+    if (!(s.detector_surface ? cosf(s.view[v]) < 0.0f : cosf(s.view[v]) > 0.0f)) {
+      set_error(error, size, "Viewing angles must stay off the horizon, on the detector's side");
       return false;
     }
   }
@@ -2299,7 +2314,7 @@ void *monki_hip_batch_create(uint32_t device, const float *tables, const size_t 
                              uint32_t block, char *error, size_t error_size) {
   Session *h = create_session(device, tables, offsets, scene_count, 1, block, error, error_size);
   for (size_t i = 0; h && i < h->scenes.size(); ++i) {
-    if (h->scenes[i].nmat == 4 || h->scenes[i].backward || h->scenes[i].scats) {
+    if (h->scenes[i].nmat == 4 || h->scenes[i].backward || h->scenes[i].scats || h->scenes[i].detector_surface) {
       release(h);
       set_error(error, error_size, "Batch sessions take forward nmat 1 or 3 scenes (run others one at a time)");
       return nullptr;

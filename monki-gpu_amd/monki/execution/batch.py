@@ -157,8 +157,8 @@ class HIPBatch:
             shapes.append(shape)
             nmat, nx, ny, nz, views = shape
 
-            if nmat == 4 or getattr(case.scene, "save_scats", False):
-                raise NotImplementedError("HIPBatch takes nmat 1 or 3 without the scattering decomposition; use HIP")
+            if nmat == 4 or getattr(case.scene, "save_scats", False) or case.scene.detector != "toa":
+                raise NotImplementedError("HIPBatch takes nmat 1 or 3, TOA detectors, no scattering decomposition; use HIP")
 
             counts.append(tally_count(nmat, nx, ny, nz, views, self.save_path))
             offsets.append(offsets[-1] + buffer.size)

@@ -222,6 +222,10 @@ class HIP(Backend):
         if self.save_scats and scene.backward:
             raise NotImplementedError("the scattering decomposition runs with forward transport")
 
+        if scene.backward and tables.detector != "toa":
+            # main.f90:765-781 allows backward runs only with source, detector and grid at TOA.
+            raise NotImplementedError("backward transport runs with the detector at TOA")
+
         packed, self._shape = pack_tables(tables, shards=self.tally_shards, backward=scene.backward)
         nmat, nx, ny, nz, views = self._shape
         per_round = tally_count(nmat, nx, ny, nz, views, self.save_path, self.save_scats)

@@ -20,6 +20,7 @@ Header (64 float32 words; integers are stored exactly, all < 2**24):
       every column of the layer, so the free path crosses it without cell-by-cell steps)
 28    transport direction: 0 forward, 1 backward (mc.f90 backward_mode)
 29    scattering decomposition: 1 runs calc_radiances_and_scats (no layer paths)
+30    detector and result grid: 0 TOA, 1 surface (views point down)
 32+8s species s: LUT size, albedo, offsets of cosine block (cos, 1-cos, 1+cos),
       CDF, F11, F22, F33, F12; with nmat 4, F34 and F44 follow F12 directly (same length)
 ====  =====================================================================
@@ -114,6 +115,7 @@ def pack_tables(scene, shards: int = 32, backward: bool = False):
     header[27] = place(uniform)
     header[28] = 1.0 if backward else 0.0
     header[29] = 1.0 if scene.save_scats else 0.0
+    header[30] = 1.0 if scene.detector == "surface" else 0.0
 
     for index, sp in enumerate(scene.species):
         cdf = sp.bound_cdf(nmat) if sp.cdf is None else sp.cdf
@@ -146,8 +148,10 @@ def pack_tables(scene, shards: int = 32, backward: bool = False):
         if np.any(np.diff(sp.cosines.astype(np.float32)) <= 0):
             raise ValueError("Scattering angles are not distinct at float32 precision")
 
-    if np.any(np.cos(np.deg2rad(np.asarray(scene.view_zeniths)).astype(np.float32)) <= 0):
-        raise ValueError("Viewing angles must remain above the horizon in float32")
+    view_cosines = np.cos(np.deg2rad(np.asarray(scene.view_zeniths)).astype(np.float32))
+
+    if np.any(view_cosines <= 0 if scene.detector == "toa" else view_cosines >= 0):
+        raise ValueError("Viewing angles must stay off the horizon in float32")
 
     assert cumulative.shape == (nspecies, nx, ny, nz) and packed.size == offset
     return packed, (nmat, nx, ny, nz, views)

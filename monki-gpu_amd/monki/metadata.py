@@ -125,6 +125,11 @@ def scene_metadata(scene, photons_per_column: int, path: Path = Path("scene.nc")
             dimensions[name] = ("nvza", "nz_withsurf", "ny", "nx", "nmat", "nwavs")
             attributes[name] = product_attributes(name)
 
+    if getattr(scene, "detector", "toa") == "surface":
+        # Report_ncdf's loc_name (main.f90:723-729, 1704): "Surface " for a ground detector.
+        for attrs in attributes.values():
+            attrs["long_name"] = attrs["long_name"].replace("TOA radiance", "Surface radiance")
+
     for name, (value, dims, attrs) in (extra or {}).items():
         values[name] = value
         dimensions[name] = dims

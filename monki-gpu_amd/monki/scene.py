@@ -90,6 +90,10 @@ class TransportScene:
     save_path: bool = True
     # Single/multiple-scattering decomposition (calc_radiances_and_scats); excludes save_path.
     save_scats: bool = False
+    # Detector and output grid: "toa" (views 0 <= zenith < 90, looking down from TOA) or
+    # "surface" (views 90 < zenith <= 180: the direction toward a detector at the ground),
+    # main.f90:711-745 with the result grid at the detector.
+    detector: str = "toa"
     wavelength_nm: float = 340.0
     altitude_m: Optional[np.ndarray] = None  # (nz,) level altitudes, bottom to TOA
     description: str = ""
@@ -172,9 +176,18 @@ class TransportScene:
         if not self.view_zeniths or len(self.view_zeniths) > 3:
             raise ValueError("supply 1 to 3 viewing zenith angles")
 
-        for angle in (self.solar_zenith, *self.view_zeniths):
-            if not np.isfinite(angle) or not 0 <= angle < 90:
+        if self.detector not in ("toa", "surface"):
+            raise ValueError('detector must be "toa" or "surface"')
+
+        if not np.isfinite(self.solar_zenith) or not 0 <= self.solar_zenith < 90:
+            raise ValueError("zeniths must be in [0, 90)")
+
+        for angle in self.view_zeniths:
+            if self.detector == "toa" and (not np.isfinite(angle) or not 0 <= angle < 90):
                 raise ValueError("zeniths must be in [0, 90)")
+
+            if self.detector == "surface" and (not np.isfinite(angle) or not 90 < angle <= 180):
+                raise ValueError("view zeniths of a surface detector must be in (90, 180]")
 
         for angle in (self.solar_azimuth, self.view_azimuth):
             if not np.isfinite(angle) or not 0 <= angle < 360:
@@ -237,8 +250,8 @@ class TransportScene:
             self.angles,
             len(self.species),
             nz,  # k_source: TOA
-            nz,  # k_detector: TOA
-            nz,  # k_register: TOA grid
+            nz if self.detector == "toa" else 0,  # k_detector
+            nz if self.detector == "toa" else 0,  # k_register: result grid at the detector
             int(self.save_path),
             int(backward),  # transport direction (mc.f90 backward_mode)
         ]
@@ -297,8 +310,10 @@ class TransportScene:
 
         nx, ny, nz, nmat, angles, nspecies = (int(v) for v in raw[1:7])
 
-        if raw[11] != 0 or tuple(raw[7:10]) != (nz, nz, nz):
-            raise NotImplementedError("only forward TOA source/detector/grid exports are read")
+        if raw[11] != 0 or tuple(raw[7:10]) not in ((nz, nz, nz), (nz, 0, 0)):
+            raise NotImplementedError("only forward exports with a TOA source and the grid at the detector are read")
+
+        metadata.setdefault("detector", "toa" if raw[8] == nz else "surface")
 
         offset = 18
 

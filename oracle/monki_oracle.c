@@ -22,6 +22,10 @@
  *                                        948-955, 981, 1150-1151, 1324-1334, 1393-1395;
  *                                        sampling sign general_mod:583-587, 810; injection
  *                                        register cartesian_mod:1035-1039; no mu0/mu mc.f90:1456
+ *   surface detector and grid .......... k_detector = k_register = 0 (main.f90:711-745): views
+ *                                        point down, register mapping from the ground
+ *                                        (cartesian_mod:1107-1160), no surface signal
+ *                                        toward them (mc.f90:804-808)
  *   scattering decomposition ........... calc_radiances_and_scats cartesian_mod:405-836 (forward;
  *                                        order counter mc.f90:433, 689, 1084)
  *   mu0/mu scaling ..................... mc.f90:1456-1474
@@ -63,6 +67,7 @@ typedef struct {
 
 typedef struct {
   int nx, ny, nz, nmat, views, species, paths, backward, scats;
+  int surface_detector; /* detector and result grid at the ground (views point down) */
   double xunit, yunit, depol, theta0, phi0, phi;
   double view[MAX_VIEWS];
   const float *zunit, *albedo, *beta, *cumfrac, *ssalb_gas;
@@ -455,7 +460,7 @@ static int detect(const scene_t *s, const pos_t *ev, double prob[MAX_VIEWS][4],
     height += (vc > 0 ? s->zunit[ev->k] - z_old : z_old);
     double top = 0.0;
     for (int k = 0; k < s->nz - 1; ++k) top += s->zunit[k];
-    height = top - height;
+    if (!s->surface_detector) height = top - height; /* k_register = nz: measured from TOA */
     double dxh = fabs(va) * height / fabs(vc), dyh = fabs(vb) * height / fabs(vc);
     int di = (int)((s->xunit - x_old + dxh) / s->xunit);
     int dj = (int)((s->yunit - y_old + dyh) / s->yunit);
@@ -576,9 +581,10 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
       double vc = cos(s->view[v]);
       if (surface && s->backward) {
         /* W As S0: the first column of W. */
-        for (int m = 0; m < 4; ++m) prob[v][m] = W[m][0] * vc * weight;
+        for (int m = 0; m < 4; ++m) prob[v][m] = vc > 0 ? W[m][0] * vc * weight : 0.0;
       } else if (surface) {
-        prob[v][0] = st[0] * vc * weight, prob[v][1] = 0.0, prob[v][2] = 0.0, prob[v][3] = 0.0;
+        /* Surface reflections only travel upward (mc.f90:804-808). */
+        prob[v][0] = vc > 0 ? st[0] * vc * weight : 0.0, prob[v][1] = 0.0, prob[v][2] = 0.0, prob[v][3] = 0.0;
       } else {
         double a = 1.0 - p.c * p.c, b = 1.0 - vc * vc;
         double cs = p.c * vc + sqrt(a > EPS ? a : EPS) * sqrt(b > EPS ? b : EPS) * cos(phi_in - s->phi);
@@ -672,6 +678,7 @@ static int parse(const float *t, size_t count, scene_t *s) {
   s->nx = word(t, 1), s->ny = word(t, 2), s->nz = word(t, 3), s->nmat = word(t, 4);
   s->views = word(t, 5), s->species = word(t, 6), s->paths = word(t, 7), s->backward = word(t, 28);
   s->scats = word(t, 29);
+  s->surface_detector = word(t, 30);
   if (s->scats) s->paths = 0; /* calc_radiances_and_scats replaces the path tallies */
   if (s->nz > MAX_LAYERS || s->views > MAX_VIEWS || s->species > MAX_SPECIES) return 1;
   s->xunit = t[10], s->yunit = t[11], s->depol = t[12];

@@ -129,6 +129,20 @@ def test_decomposition_matches_reference_fortran(tmp_path):
     assert total.shape == (3, scene.nz, scene.ny, scene.nx, 3)
 
 
+def test_surface_detector_matches_reference_fortran(tmp_path):
+    """Detector and result grid at the ground (k_detector = k_register = 0): views pointing
+    down, against the reference Fortran with the same settings."""
+
+    from monki import configs
+
+    scene = configs.box_cloud(nx=8, ny=6, nmat=3, solar_zenith=50.0, box=(3, 2), offset=(2, 1))
+    scene = replace(scene, solar_azimuth=30.0, view_azimuth=120.0, detector="surface",
+                    view_zeniths=(180.0, 150.0, 120.0))  # fmt: skip
+    gpu, mean, se = compare_with_fortran(scene, 100, 40, 8484, tmp_path, "surface")
+    assert gpu.attributes["radiance"]["long_name"] == "Surface radiance"
+    assert np.all(mean[0] > 0)  # every ground pixel sees diffuse light
+
+
 def test_scalar_transport_matches_reference_fortran(tmp_path):
     from monki import configs
 

@@ -52,12 +52,17 @@ enum Status : int { kRunning = 0, kEscaped = 1, kAbsorbed = 2, kFailed = 3 };
 // network's mixture.
 // kStokes4: the plain tracer with the full Stokes vector (nmat 4: I, Q, U, V). kBackward,
 // kBackward4: backward transport (nmat 1/3 and nmat 4). kScats, kScats4: the single/multiple-
-// scattering decomposition (calc_radiances_and_scats) in place of the layer paths.
+// scattering decomposition (calc_radiances_and_scats) in place of the layer paths. kSurface,
+// kSurface4: the plain tracer toward a detector at the ground.
 enum Mode : int {
   kPlain = 0, kLog = 1, kGuided = 2, kDirected = 3, kStokes4 = 4, kBackward = 5, kBackward4 = 6,
-  kScats = 7, kScats4 = 8
+  kScats = 7, kScats4 = 8, kSurface = 9, kSurface4 = 10
 };
-constexpr bool full_stokes(int mode) { return mode == kStokes4 || mode == kBackward4 || mode == kScats4; }
+constexpr bool full_stokes(int mode) {
+  return mode == kStokes4 || mode == kBackward4 || mode == kScats4 || mode == kSurface4;
+}
+// kSurface, kSurface4: detector and result grid at the ground (views point down).
+constexpr bool downward(int mode) { return mode == kSurface || mode == kSurface4; }
 constexpr bool backward(int mode) { return mode == kBackward || mode == kBackward4; }
 constexpr bool decomposition(int mode) { return mode == kScats || mode == kScats4; }
 
@@ -91,9 +96,6 @@ struct SpeciesTables {
 struct Scene {
   int nx, ny, nz, nmat, views, species, paths, uniform_top, shards;
   int backward;  // backward transport: weight-matrix rows, injection-column pixels, no mu0/mu
-  // Detector and result grid at the surface: views point down (view_mu < 0), the detector ray
-  // ends at the ground and registers at its exit column (main.f90:711-745).
-  int detector_surface;
   // Scattering decomposition: banks hold 8-value records [single (nmat) | multiple (nmat)],
   // one per view and pixel, then one per view and event cell (level 0 the surface).
   int scats;
@@ -116,6 +118,10 @@ struct Scene {
   const float *zunit, *albedo, *beta, *cumfrac, *ssalb_gas, *height_above, *tau_above;
   const float *uniform;  // per layer: 1 if every column has the same extinction and mix
   SpeciesTables sp[kMaxSpecies];
+  // Detector and result grid at the surface: views point down (view_mu < 0), the detector ray
+  // ends at the ground and registers at its exit column (main.f90:711-745). Host-side choice
+  // of the kernel instantiation; last, so that the hot fields keep their offsets.
+  int detector_surface;
 };
 
 struct Controls {
@@ -586,7 +592,7 @@ struct DetectorHit {
 // optical depth from the event to the top of the last physical layer and the TOA pixel.
 // Cells are walked one by one below uniform_top; above it the vertical prefix gives the exact
 // slant integral (transport.metal:299-313).
-template <typename R, bool LAYERS = false>
+template <typename R, bool LAYERS = false, bool DOWN = false>
 __device__ inline DetectorHit detector_walk(const Scene &s, const Photon<R> &event, R va, R vb,
                                             R mu, R &tau, uint32_t max_steps, Counts &n,
                                             const float *layers = nullptr, int row = 0) {
@@ -596,7 +602,7 @@ __device__ inline DetectorHit detector_walk(const Scene &s, const Photon<R> &eve
   uint32_t steps = 0;
   while (ray.k >= 0 && ray.k < s.nz - 1) {
     if (++steps > max_steps) return DetectorHit{0, 0, false};
-    if (!s.detector_surface && ray.k >= s.uniform_top) {
+    if (!DOWN && ray.k >= s.uniform_top) {
       // Layers at and above uniform_top are uniform: the wave's LDS layer table holds their
       // extinction, height above and optical depth above.
       const R height_above = LAYERS ? R(layers[2 * row + ray.k]) : R(s.height_above[ray.k]);
@@ -724,7 +730,8 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
 // horizontally uniform layers (-1 for varying ones), thickness, height above and optical depth
 // above, read instead of the global tables on every layer crossing of the free path and by
 // the detector walk.
-template <typename R, bool TRACE, bool V4 = false, bool BACK = false, bool SCATS = false, bool LAYERS = false>
+template <typename R, bool TRACE, bool V4 = false, bool BACK = false, bool SCATS = false, bool LAYERS = false,
+          bool DOWN = false>
 __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *history,
                        History<R> &h, const Pending<R> &out, Counts &n, Trace<R> &trace,
                        const float *layers = nullptr, int row = 0) {
@@ -849,8 +856,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
   out.round() = int(h.round_offset);
   // Detector ray inside the event layer: the distance to its top face (views point upward).
   // (Views of a surface detector point down: the distance to the bottom face instead.)
-  const R layer_thickness = LAYERS ? R(layers[row + p.k]) : R(s.zunit[p.k]);
-  const R rz = (p.c < R(0)) != (s.detector_surface != 0) ? fmax(R(0), layer_thickness - p.rz) : p.rz;
+  const R rz = (p.c < R(0)) != DOWN ? fmax(R(0), (LAYERS ? R(layers[row + p.k]) : R(s.zunit[p.k])) - p.rz) : p.rz;
   out.rz() = rz;
   // All views share one azimuth: cos and sin of the photon-view azimuth difference, once.
   const R delta = p.phi - R(s.phi);
@@ -864,12 +870,12 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
     if (surface) {
       if (BACK) {
         // W As S0 = the first column of W (mc.f90:754-755).
-        const R scale = fmax(mu, R(0)) * h.weight;
+        const R scale = (DOWN ? R(0) : mu) * h.weight;
         prob = Stokes<R>{h.st.i * scale, h.w2.i * scale, h.w3.i * scale, V4 ? h.w4.i * scale : R(0)};
       } else {
         // Surface reflections only travel upward (mc.f90:804-808): nothing toward a view that
         // points down.
-        prob = Stokes<R>{h.st.i * fmax(mu, R(0)) * h.weight, R(0), R(0)};
+        prob = Stokes<R>{DOWN ? R(0) : h.st.i * mu * h.weight, R(0), R(0)};
       }
     } else {
       // Sines floored at sqrt(1e-16) as mc.f90:1112-1113 does: the forward peak of a Mie
@@ -908,7 +914,8 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
     if (prob.i <= eps) continue;
     R tau;
     const DetectorHit hit =
-        detector_walk<R, LAYERS>(s, p, R(s.view_a[view]), R(s.view_b[view]), mu, tau, c.max_steps, n, layers, row);
+        detector_walk<R, LAYERS, DOWN>(s, p, R(s.view_a[view]), R(s.view_b[view]), mu, tau, c.max_steps, n, layers,
+                                       row);
     if (!hit.ok) {
       atomicOr(failure, kDetectorLimit);
       return kFailed;
@@ -930,7 +937,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
         trace.weights += ci;
         // Layers the detector ray crosses after the event layer: above it, or below it for a
         // surface detector.
-        const R beyond = s.detector_surface
+        const R beyond = DOWN
                              ? R(s.height_above[0] + s.zunit[0]) - R(s.height_above[p.k]) - R(s.zunit[p.k])
                              : R(s.height_above[p.k]);
         trace.paths += ci * (h.path_total + (rz + beyond) / fabs(mu));
@@ -1135,7 +1142,7 @@ __device__ inline void event_features(const Scene &s, const History<R> &h, float
 // lane's LDS row) or Stokes component t - nz. Contiguous lanes make each instruction a few
 // cache-line requests instead of one request per value. Record offsets within a bank are
 // 32-bit: 24-bit multiplies (full rate) instead of 64-bit index arithmetic per view.
-template <typename R, bool V4 = false>
+template <typename R, bool V4 = false, bool DOWN = false>
 __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int shard, bool has,
                                   const Pending<R> &wave_pending, const R *wave_history, int row,
                                   Counts &n) {
@@ -1171,7 +1178,8 @@ __device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int sha
     const R rz = source.rz();
     R *bank = banks + s.bank_base + (size_t(source.round()) * s.shards + shard) * size_t(s.bank_count);
     const R path = path_lane ? wave_history[src * row + lane] : R(0);
-    const bool at_layer = lane == layer, above = s.detector_surface ? lane < layer : lane > layer;
+    // Layers the detector ray crosses after the event layer: above it (below it for DOWN).
+    const bool at_layer = lane == layer, above = DOWN ? lane < layer : lane > layer;
 #pragma unroll
     for (int view = 0; view < kMaxViews; ++view) {
       if (view >= s.views) break;
@@ -1273,6 +1281,7 @@ __global__ void __launch_bounds__(kBlock32)
   constexpr bool V4 = full_stokes(MODE);
   constexpr bool BACK = backward(MODE);
   constexpr bool SCATS = decomposition(MODE);
+  constexpr bool DOWN = downward(MODE);
   extern __shared__ unsigned char lds_raw[];
   // Layer rows of odd length (the largest nz of the scenes, | 1): lanes reading one row, or
   // each its own row at one layer, touch distinct LDS banks.
@@ -1347,14 +1356,15 @@ __global__ void __launch_bounds__(kBlock32)
     int status = kRunning;
     prefetch(h.rng);
     if (active)
-      status = collide<R, TRACE, V4, BACK, SCATS, LAYERS>(s, c, failure, history, h, pending, n, trace, layers, row);
+      status = collide<R, TRACE, V4, BACK, SCATS, LAYERS, DOWN>(s, c, failure, history, h, pending, n, trace, layers,
+                                                                row);
     // Lanes read each other's LDS rows below: keep every access of this wave in order.
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     if (SCATS)
       tally_wave_scats<R, V4>(s, banks, shard, active && status == kRunning, wave_pending, n);
     else
-      tally_wave<R, V4>(s, banks, shard, active && status == kRunning, wave_pending, wave_history, row, n);
+      tally_wave<R, V4, DOWN>(s, banks, shard, active && status == kRunning, wave_pending, wave_history, row, n);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
     const bool scatters = active && status == kRunning && !h.surface;
@@ -1926,15 +1936,23 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
     set_error(error, error_size, "Invalid HIP session or arguments");
     return 1;
   }
-  if constexpr (MODE != kStokes4 && !backward(MODE) && !decomposition(MODE)) {
+  if constexpr (MODE != kStokes4 && !backward(MODE) && !decomposition(MODE) && !downward(MODE)) {
     // The full Stokes vector and backward transport have instantiations of their own of the
     // plain tracer.
     const Scene &first = h->scenes[0];
-    if (first.nmat == 4 || first.backward || first.scats) {
+    if (first.nmat == 4 || first.backward || first.scats || first.detector_surface) {
       if (MODE != kPlain) {
-        set_error(error, error_size, "The steered and logging tracers take forward nmat 1 or 3 scenes");
+        set_error(error, error_size, "The steered and logging tracers take forward nmat 1 or 3 TOA scenes");
         return 1;
       }
+      if (first.detector_surface && (first.backward || first.scats)) {
+        set_error(error, error_size, "A ground detector runs forward without the scattering decomposition");
+        return 1;
+      }
+      if (first.detector_surface && first.nmat == 4)
+        return launch<R, TRACE, kSurface4>(h, controls, out, records, record_count, error, error_size, proposal);
+      if (first.detector_surface)
+        return launch<R, TRACE, kSurface>(h, controls, out, records, record_count, error, error_size, proposal);
       if (first.scats && first.nmat == 4)
         return launch<R, TRACE, kScats4>(h, controls, out, records, record_count, error, error_size, proposal);
       if (first.scats)

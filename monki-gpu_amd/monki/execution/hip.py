@@ -222,6 +222,9 @@ class HIP(Backend):
         if self.save_scats and scene.backward:
             raise NotImplementedError("the scattering decomposition runs with forward transport")
 
+        if self.save_scats and tables.detector != "toa":
+            raise NotImplementedError("the scattering decomposition runs with the detector at TOA")
+
         if scene.backward and tables.detector != "toa":
             # main.f90:765-781 allows backward runs only with source, detector and grid at TOA.
             raise NotImplementedError("backward transport runs with the detector at TOA")

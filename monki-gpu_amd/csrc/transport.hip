@@ -52,19 +52,24 @@ enum Status : int { kRunning = 0, kEscaped = 1, kAbsorbed = 2, kFailed = 3 };
 // network's mixture.
 // kStokes4: the plain tracer with the full Stokes vector (nmat 4: I, Q, U, V). kBackward,
 // kBackward4: backward transport (nmat 1/3 and nmat 4). kScats, kScats4: the single/multiple-
-// scattering decomposition (calc_radiances_and_scats) in place of the layer paths. kSurface,
-// kSurface4: the plain tracer toward a detector at the ground.
+// scattering decomposition (calc_radiances_and_scats) in place of the layer paths; kBackScats,
+// kBackScats4 the same with backward transport. kSurface, kSurface4: the plain tracer toward
+// a detector at the ground.
 enum Mode : int {
   kPlain = 0, kLog = 1, kGuided = 2, kDirected = 3, kStokes4 = 4, kBackward = 5, kBackward4 = 6,
-  kScats = 7, kScats4 = 8, kSurface = 9, kSurface4 = 10
+  kScats = 7, kScats4 = 8, kSurface = 9, kSurface4 = 10, kBackScats = 11, kBackScats4 = 12
 };
 constexpr bool full_stokes(int mode) {
-  return mode == kStokes4 || mode == kBackward4 || mode == kScats4 || mode == kSurface4;
+  return mode == kStokes4 || mode == kBackward4 || mode == kScats4 || mode == kSurface4 || mode == kBackScats4;
 }
 // kSurface, kSurface4: detector and result grid at the ground (views point down).
 constexpr bool downward(int mode) { return mode == kSurface || mode == kSurface4; }
-constexpr bool backward(int mode) { return mode == kBackward || mode == kBackward4; }
-constexpr bool decomposition(int mode) { return mode == kScats || mode == kScats4; }
+constexpr bool backward(int mode) {
+  return mode == kBackward || mode == kBackward4 || mode == kBackScats || mode == kBackScats4;
+}
+constexpr bool decomposition(int mode) {
+  return mode == kScats || mode == kScats4 || mode == kBackScats || mode == kBackScats4;
+}
 
 // Which events the guided mode guides: scatterer kinds (bit 0 gas, bit s + 1 tabulated species
 // s) up to a scattering order; alpha = probability of the phase-function component.
@@ -653,6 +658,9 @@ template <typename R> struct History {
   // `st`. Each row is carried like a Stokes vector (W <- W P^T transforms every row r into
   // P r), all divided by the same norm. Unused, and dropped by the compiler, elsewhere.
   Stokes<R> w2, w3, w4;
+  // Backward decomposition: the cell of the first scattering event, where every later event's
+  // 3-D contribution is registered (calc_radiances_and_scats :509-521).
+  int first_cell;
 };
 
 // Local estimates of one event, parked in LDS until the wave's cooperative tally reads them.
@@ -720,6 +728,7 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
   h.path_total = R(0);
   h.contribution = 0.0f;
   h.order = 0;
+  h.first_cell = 0;
 }
 
 // Advance one photon to its next collision or surface hit (mc.f90:476-677), apply the albedo
@@ -848,7 +857,13 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
     // complemented when the scattering order exceeds one.
     // (Events never occur in the vacuum layer, so level <= nz - 1; the bound keeps it so.)
     const int level = (p.k == 0 && p.rz < R(1e-16)) ? 0 : min(p.k + 1, s.nz - 1);
-    const int cell = p.i + s.nx * (p.j + s.ny * level);
+    int cell = p.i + s.nx * (p.j + s.ny * level);
+    if (BACK) {
+      if (h.order == 1)
+        h.first_cell = cell;
+      else
+        cell = h.first_cell;
+    }
     out.layer() = h.order == 1 ? cell : ~cell;
   } else {
     out.layer() = p.k;
@@ -1830,8 +1845,8 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
       return false;
     }
   }
-  if (s.scats && (s.paths || s.backward)) {
-    set_error(error, size, "The scattering decomposition runs forward without layer paths");
+  if (s.scats && s.paths) {
+    set_error(error, size, "The scattering decomposition runs without layer paths");
     return false;
   }
   s.stokes_count = s.nmat * s.nx * s.ny * s.views;
@@ -1953,6 +1968,10 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
         return launch<R, TRACE, kSurface4>(h, controls, out, records, record_count, error, error_size, proposal);
       if (first.detector_surface)
         return launch<R, TRACE, kSurface>(h, controls, out, records, record_count, error, error_size, proposal);
+      if (first.scats && first.backward)
+        return first.nmat == 4
+                   ? launch<R, TRACE, kBackScats4>(h, controls, out, records, record_count, error, error_size, proposal)
+                   : launch<R, TRACE, kBackScats>(h, controls, out, records, record_count, error, error_size, proposal);
       if (first.scats && first.nmat == 4)
         return launch<R, TRACE, kScats4>(h, controls, out, records, record_count, error, error_size, proposal);
       if (first.scats)

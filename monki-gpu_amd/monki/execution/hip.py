@@ -219,9 +219,6 @@ class HIP(Backend):
         ):
             raise NotImplementedError("the guided and detector-directed tracers take forward nmat 1 or 3 scenes")
 
-        if self.save_scats and scene.backward:
-            raise NotImplementedError("the scattering decomposition runs with forward transport")
-
         if self.save_scats and tables.detector != "toa":
             raise NotImplementedError("the scattering decomposition runs with the detector at TOA")
 

@@ -185,6 +185,10 @@ class RoundStatistics:
             sing, mult = self.scattering["stokes_sing"] * scale, self.scattering["stokes_mult"] * scale
             contrib_sing = self.scattering["contrib_sing"] * scale
             contrib_mult = self.scattering["contrib_mult"] * scale
+            if case.backward and sing.shape[0] >= 3:
+                for part in (sing, mult, contrib_sing, contrib_mult):
+                    part[2] *= -1  # main.f90:1190-1198
+
             values["radiance_sing"] = sing.transpose(3, 2, 1, 0)[..., None]
             values["radiance_mult"] = mult.transpose(3, 2, 1, 0)[..., None]
             values["contrib_radiance_3D"] = (contrib_sing + contrib_mult).transpose(4, 3, 2, 1, 0)[..., None]

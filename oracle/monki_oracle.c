@@ -26,8 +26,9 @@
  *                                        point down, register mapping from the ground
  *                                        (cartesian_mod:1107-1160), no surface signal
  *                                        toward them (mc.f90:804-808)
- *   scattering decomposition ........... calc_radiances_and_scats cartesian_mod:405-836 (forward;
- *                                        order counter mc.f90:433, 689, 1084)
+ *   scattering decomposition ........... calc_radiances_and_scats cartesian_mod:405-836 (order
+ *                                        counter mc.f90:433, 689, 1084; backward: the first
+ *                                        scattering cell, cartesian_mod:509-521)
  *   mu0/mu scaling ..................... mc.f90:1456-1474
  *
  * Differences from the Fortran, all deliberate: Philox streams instead of RANDOM_NUMBER
@@ -432,7 +433,15 @@ typedef struct {
  * contrib_sing | contrib_mult]: nscat is the event's scattering order (surface reflections
  * count), and the contribution also goes to the event cell, level 0 for a surface event. */
 static int detect(const scene_t *s, const pos_t *ev, double prob[MAX_VIEWS][4],
-                  const double *history, uint32_t max_steps, int column, int nscat, sink_t *out) {
+                  const double *history, uint32_t max_steps, int column, int nscat, int *first_cell,
+                  sink_t *out) {
+  /* Cell of the 3-D contribution: the event's; in backward mode the first scattering event's. */
+  int level = (ev->k == 0 && ev->z < EPS) ? 0 : ev->k + 1;
+  int event_cell = ev->i + s->nx * (ev->j + s->ny * level);
+  if (s->backward) {
+    if (nscat == 1) *first_cell = event_cell;
+    event_cell = *first_cell;
+  }
   for (int v = 0; v < s->views; ++v) {
     if (!(prob[v][0] > EPS)) continue;
     double th = s->view[v];
@@ -474,9 +483,8 @@ static int detect(const scene_t *s, const pos_t *ev, double prob[MAX_VIEWS][4],
     int pixel = pi + s->nx * (pj + s->ny * v);
     for (int m = 0; m < s->nmat; ++m) out->tallies[s->nmat * pixel + m] += prob[v][m] * att * mu0mu;
     if (s->scats) {
-      int level = (ev->k == 0 && ev->z < EPS) ? 0 : ev->k + 1;
       int group = nscat == 1 ? 0 : 1;
-      size_t cell = (size_t)ev->i + (size_t)s->nx * (ev->j + (size_t)s->ny * (level + (size_t)s->nz * v));
+      size_t cell = (size_t)event_cell + (size_t)s->nx * s->ny * s->nz * v;
       for (int m = 0; m < s->nmat; ++m) {
         double value = prob[v][m] * att * mu0mu;
         out->tallies[(size_t)(1 + group) * s->stokes_count + s->nmat * pixel + m] += value;
@@ -523,6 +531,7 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
   ++out->counters[0];
   double *rec = out->record;
   int nscat = 0; /* events so far, surface reflections included (mc.f90:689, 1084) */
+  int first_cell = 0;
 
   for (uint32_t step = 0; step < max_steps; ++step) {
     double beta = (p.k < s->nz - 1 && p.k >= 0) ? s->beta[cell(s, p.i, p.j, p.k)] : 0.0;
@@ -612,7 +621,7 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
         return FAILED;
       }
     }
-    if (!detect(s, &p, prob, history, max_steps, column, nscat, out)) {
+    if (!detect(s, &p, prob, history, max_steps, column, nscat, &first_cell, out)) {
       *flag |= 2u;
       return FAILED;
     }

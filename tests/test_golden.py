@@ -373,10 +373,12 @@ def test_oracle_matches_reference_fortran_statistically(oracle, name):
 
 @pytest.mark.skipif(not (Path(__file__).parents[1] / "oracle" / "_ref" / "monki_ref").is_file(),
                     reason="reference Fortran build missing (make -C oracle ref)")  # fmt: skip
-def test_oracle_decomposition_matches_reference_fortran(oracle):
+@pytest.mark.parametrize("backward", [False, True])
+def test_oracle_decomposition_matches_reference_fortran(oracle, backward):
     """Single/multiple-scattering decomposition (calc_radiances_and_scats): the oracle's
     Stokes_sing/mult and 3-D contributions against the reference's unchanged mc.f90 run with
-    save_scats, within combined SE; and the identities the decomposition fixes."""
+    save_scats (forward, and backward with contributions at the first scattering cell), within
+    combined SE; and the identities the decomposition fixes."""
 
     from dataclasses import replace
 
@@ -389,10 +391,10 @@ def test_oracle_decomposition_matches_reference_fortran(oracle):
                     view_azimuth=120.0, save_path=False, save_scats=True)  # fmt: skip
     rounds, ppc = 40, 60
     per_round = ppc * scene.nx * scene.ny
-    packed, shape = pack_tables(scene, shards=1)
+    packed, shape = pack_tables(scene, shards=1, backward=backward)
     ours, _, _, flag = oracle.run(packed, [per_round * rounds, 0, 1, 5, 100000, 0, per_round, rounds])
     assert flag == 0
-    theirs, _, _ = run_reference(scene, ppc, rounds, threads=1, save_scats=True)
+    theirs, _, _ = run_reference(scene, ppc, rounds, threads=1, backward=backward, save_scats=True)
     theirs = np.stack([fortran_scats_row(row, shape) for row in theirs])
     assert ours.shape == theirs.shape
     nmat, nx, ny, nz, views = shape

@@ -85,10 +85,11 @@ def test_backward_transport_matches_reference_fortran(tmp_path, nmat):
     assert np.median(np.abs(mean[2]) / np.maximum(se[2], 1e-30)) > 3  # U resolved
 
 
-def test_decomposition_matches_reference_fortran(tmp_path):
+@pytest.mark.parametrize("backward", [False, True])
+def test_decomposition_matches_reference_fortran(tmp_path, backward):
     """The single/multiple-scattering decomposition against the reference Fortran run with
-    save_scats (calc_radiances_and_scats): per-round means within combined SE, and the NetCDF
-    product of simulate carries those means."""
+    save_scats (calc_radiances_and_scats), forward and backward: per-round means within
+    combined SE, and the NetCDF product of simulate carries those means."""
 
     from monki import SceneCase, configs, simulate
     from monki.execution import HIP, BatchControls
@@ -99,7 +100,7 @@ def test_decomposition_matches_reference_fortran(tmp_path):
     scene = replace(configs.box_cloud(nx=8, ny=6, nmat=3, box=(3, 2), offset=(2, 1)), solar_azimuth=30.0,
                     view_azimuth=120.0, save_path=False, save_scats=True)  # fmt: skip
     rounds, ppc = 40, 100
-    case = SceneCase("scats", scene, ppc, rounds)
+    case = SceneCase("scats", scene, ppc, rounds, backward=backward)
 
     with HIP(save_path=False, save_scats=True, seed=77) as backend:
         backend.prepare(case)
@@ -109,7 +110,7 @@ def test_decomposition_matches_reference_fortran(tmp_path):
     flat = lambda b: np.concatenate([b.stokes.ravel(order="F")] + [b.scattering[k].ravel(order="F") for k in
                                     ("stokes_sing", "stokes_mult", "contrib_sing", "contrib_mult")])  # fmt: skip
     ours = np.stack([flat(b) for b in batches]) / ppc
-    tallies, _, _ = run_reference(scene, ppc, rounds, threads=1, save_scats=True)
+    tallies, _, _ = run_reference(scene, ppc, rounds, threads=1, backward=backward, save_scats=True)
     theirs = np.stack([fortran_scats_row(row, shape) for row in tallies]) / ppc
     stats = combined_se_statistics(
         ours.mean(axis=0), ours.std(axis=0, ddof=1) / np.sqrt(rounds),
@@ -120,7 +121,10 @@ def test_decomposition_matches_reference_fortran(tmp_path):
     assert 0.7 < stats["chi_square_per_dof"] < 1.4
 
     product = simulate(case, HIP(save_path=False, save_scats=True, seed=77), tmp_path / "scats.nc")
-    sing = (sum(b.scattering["stokes_sing"] for b in batches) / rounds / ppc).transpose(3, 2, 1, 0)
+    sing = sum(b.scattering["stokes_sing"] for b in batches) / rounds / ppc
+    if backward:
+        sing[2] *= -1  # the product reverses U of backward runs (main.f90:1190-1198)
+    sing = sing.transpose(3, 2, 1, 0)
     # Same streams; float32 bank sums differ only in atomic order.
     np.testing.assert_allclose(product.values["radiance_sing"][..., 0], sing, rtol=2e-5, atol=2e-5 * np.abs(sing).max())
     total = product.values["contrib_radiance_3D"][..., 0]

@@ -241,8 +241,8 @@ def test_circular_polarization_matches_oracle(oracle):
     assert abs(g[3].sum() - c[3].sum()) <= 0.05 * np.abs(c[3]).sum()
 
 
-@pytest.mark.parametrize("precision", [32, 64])
-def test_decomposition_matches_oracle(oracle, precision):
+@pytest.mark.parametrize("precision, backward", [(32, False), (64, False), (64, True)])
+def test_decomposition_matches_oracle(oracle, precision, backward):
     """Single/multiple-scattering decomposition (save_scats) against the oracle on the same
     streams: Stokes = single + multiple exactly, the contributions by event cell add up to the
     detector tallies, and every part follows the oracle's tallies."""
@@ -257,10 +257,10 @@ def test_decomposition_matches_oracle(oracle, precision):
     per_round = ppc * scene.nx * scene.ny
 
     with HIP(tally_shards=1, save_path=False, save_scats=True, precision=precision) as backend:
-        backend.prepare(SceneCase("scats", scene, ppc, 2))
+        backend.prepare(SceneCase("scats", scene, ppc, 2, backward=backward))
         gpu, _ = backend.trace(BatchControls(1, ppc), per_round, precision)
 
-    packed, shape = pack_tables(scene, shards=1)
+    packed, shape = pack_tables(scene, shards=1, backward=backward)
     cpu, _, _, flag = oracle.run(packed, [per_round, 0, 1, 23274, 100000, 0, per_round, 1])
     assert flag == 0
     cpu = cpu[0]

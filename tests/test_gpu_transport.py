@@ -37,6 +37,7 @@ def small_scenes():
         "box-cloud-IQUV-backward": off_plane,
         # Detector and result grid at the ground: views pointing down.
         "box-cloud-surface": replace(off_plane, nmat=3, detector="surface", view_zeniths=(180.0, 150.0, 120.0)),
+        "box-cloud-IQUV-surface": replace(off_plane, detector="surface", view_zeniths=(180.0, 150.0, 120.0)),
     }
 
 
@@ -134,10 +135,11 @@ AGREEMENT = {  # (float32, float64) over all photons
     "box-cloud-backward": (0.90, 0.998),
     "box-cloud-IQUV-backward": (0.90, 0.998),
     "box-cloud-surface": (0.90, 0.998),
+    "box-cloud-IQUV-surface": (0.90, 0.998),
 }
 
 
-EXPLICIT_VIEWS = {"box-cloud-surface"}
+EXPLICIT_VIEWS = {"box-cloud-surface", "box-cloud-IQUV-surface"}
 
 
 def check_histories(name, gpu, cpu, precision):
@@ -211,7 +213,7 @@ def test_float32_histories_match_float64_oracle(oracle, name):
 
 
 @pytest.mark.parametrize("name", ["clear-IQU", "layered-cloud", "box-cloud", "box-cloud-IQUV", "box-cloud-backward",
-                                  "box-cloud-IQUV-backward", "box-cloud-surface"])  # fmt: skip
+                                  "box-cloud-IQUV-backward", "box-cloud-surface", "box-cloud-IQUV-surface"])  # fmt: skip
 def test_float64_build_matches_oracle(oracle, name):
     """The float64 build runs the oracle's algorithm on the device."""
 
@@ -241,8 +243,9 @@ def test_circular_polarization_matches_oracle(oracle):
     assert abs(g[3].sum() - c[3].sum()) <= 0.05 * np.abs(c[3]).sum()
 
 
-@pytest.mark.parametrize("precision, backward", [(32, False), (64, False), (64, True)])
-def test_decomposition_matches_oracle(oracle, precision, backward):
+@pytest.mark.parametrize("precision, backward, nmat", [(32, False, 3), (64, False, 3), (64, True, 3), (64, False, 4),
+                                                    (64, True, 4)])  # fmt: skip
+def test_decomposition_matches_oracle(oracle, precision, backward, nmat):
     """Single/multiple-scattering decomposition (save_scats) against the oracle on the same
     streams: Stokes = single + multiple exactly, the contributions by event cell add up to the
     detector tallies, and every part follows the oracle's tallies."""
@@ -251,7 +254,7 @@ def test_decomposition_matches_oracle(oracle, precision, backward):
     from monki.execution import HIP, BatchControls
     from monki.execution.hip_tables import pack_tables, split_scattering
 
-    scene = replace(configs.box_cloud(nx=8, ny=6, nmat=3, box=(3, 2), offset=(2, 1)), solar_azimuth=30.0,
+    scene = replace(configs.box_cloud(nx=8, ny=6, nmat=nmat, box=(3, 2), offset=(2, 1)), solar_azimuth=30.0,
                     view_azimuth=120.0, save_path=False, save_scats=True)  # fmt: skip
     ppc = 100
     per_round = ppc * scene.nx * scene.ny

@@ -1275,6 +1275,28 @@ __device__ inline void flush_counts(Counts &n, unsigned long long *counters) {
   n = Counts{0, 0, 0, 0, 0, 0, 0, 0};
 }
 
+// The proposal's mixture for the wave's guided events: the network on each 32-lane half with
+// guided events (matrix cores, all lanes together). Not inlined: the network needs most of the
+// register file, and a call saves the photon state around it once instead of the compiler
+// spilling it piecemeal through the network.
+__device__ __attribute__((noinline)) void guided_mixture(const monki_net::Net &net,
+                                                          const float (&features)[monki_net::kInputs],
+                                                          unsigned long long guided_lanes, bool guided_event,
+                                                          monki_net::Mixture &mixture) {
+  float o[monki_net::kOutputs];
+#pragma unroll
+  for (int nb = 0; nb < 2; ++nb) {
+    // A 32-lane half without guided events skips its network pass (uniform branch).
+    if (((guided_lanes >> (32 * nb)) & 0xffffffffull) == 0) continue;
+    monki_net::bf16x4 x[16];
+    monki_net::scatter_inputs(features, nb, x);
+    monki_net::floatx16 raw[2];
+    monki_net::forward_block(net, x, raw, nullptr);
+    monki_net::gather_outputs(raw, (__lane_id() >> 5) == nb, o);
+  }
+  if (guided_event) mixture = monki_net::mixture_from(o);
+}
+
 // Persistent transport. Each iteration, every lane advances its photon to the next event and
 // evaluates the local estimates (divergent work), the wave then tallies all pending events
 // cooperatively, and the lanes sample their continuation directions. A lane whose history
@@ -1395,20 +1417,7 @@ __global__ void __launch_bounds__(kBlock32)
     const bool guided_event = MODE == kGuided && scatters &&
                               ((guidance.species_mask >> (h.species + 1)) & 1u) && h.order <= guidance.max_order;
     const unsigned long long guided_lanes = MODE == kGuided ? __ballot(guided_event) : 0ull;
-    if (MODE == kGuided && guided_lanes) {
-      float o[monki_net::kOutputs];
-#pragma unroll
-      for (int nb = 0; nb < 2; ++nb) {
-        // A 32-lane half without guided events skips its network pass (uniform branch).
-        if (((guided_lanes >> (32 * nb)) & 0xffffffffull) == 0) continue;
-        monki_net::bf16x4 x[16];
-        monki_net::scatter_inputs(features, nb, x);
-        monki_net::floatx16 raw[2];
-        monki_net::forward_block(net, x, raw, nullptr);
-        monki_net::gather_outputs(raw, (__lane_id() >> 5) == nb, o);
-      }
-      if (guided_event) mixture = monki_net::mixture_from(o);
-    }
+    if (MODE == kGuided && guided_lanes) guided_mixture(net, features, guided_lanes, guided_event, mixture);
     if (MODE == kLog) {
       const unsigned long long logging = __ballot(scatters);
       if (logging) {

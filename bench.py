@@ -164,16 +164,20 @@ def amf_race(max_rounds: int = 256, photons_per_column: int = PHOTONS_PER_COLUMN
     """Plain vs guided tracer on the full-size 3-D box cloud (tau 20 over 10 x 6 columns,
     layers 2-6 km, Mie, I/Q/U, 3 views): wall time of simulate(..., AMFConvergence(0.01, 128,
     32, 3)) up to max_rounds, the final worst relative AMF SE, and the projection of both to 1%
-    (SE ~ rounds^-1/2). The guided time includes logging and training the proposal."""
+    (SE ~ rounds^-1/2). The guided time includes logging and training the proposal. A third leg
+    runs detector-directed sampling (vMF lobes toward the views, no network)."""
 
     from monki import configs
-    from monki.execution.proposal import ProposalSettings
+    from monki.execution.proposal import DirectedSettings, ProposalSettings
 
     scene = configs.box_cloud()
     plain = time_to_amf_se(scene, max_rounds, photons_per_column=photons_per_column, name="box-cloud")
     guided = time_to_amf_se(scene, max_rounds, photons_per_column=photons_per_column, name="box-cloud",
                             proposal=ProposalSettings())  # fmt: skip
-    result = {"plain": plain, "guided": guided, "workload": "box cloud 30x20x35, tau 20, 10x6 columns x layers 2-6 km"}
+    directed = time_to_amf_se(scene, max_rounds, photons_per_column=photons_per_column, name="box-cloud",
+                              directed=DirectedSettings())  # fmt: skip
+    result = {"plain": plain, "guided": guided, "directed": directed,
+              "workload": "box cloud 30x20x35, tau 20, 10x6 columns x layers 2-6 km"}  # fmt: skip
 
     if plain.get("projected_seconds") and guided.get("projected_seconds"):
         result["speedup_to_1pct"] = plain["projected_seconds"] / guided["projected_seconds"]
@@ -181,6 +185,9 @@ def amf_race(max_rounds: int = 256, photons_per_column: int = PHOTONS_PER_COLUMN
     if plain.get("relative_se") and guided.get("relative_se"):
         # Per-photon variance ratio at equal photon counts, from the median relative AMF SE.
         result["variance_ratio_median"] = (guided["relative_se"]["median"] / plain["relative_se"]["median"]) ** 2
+
+    if plain.get("projected_seconds") and directed.get("projected_seconds"):
+        result["directed_speedup_to_1pct"] = plain["projected_seconds"] / directed["projected_seconds"]
 
     return result
 

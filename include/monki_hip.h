@@ -41,7 +41,7 @@ void *monki_hip_create(uint32_t device, const float *tables, size_t table_count,
  * [Stokes | paths | weights], or with the scattering decomposition (table header word 29)
  * [Stokes | Stokes_sing | Stokes_mult | contrib_sing | contrib_mult]. The tables select the
  * transport: nmat 1, 3 or 4, forward or backward (header word 28), detector at TOA or at the
- * ground (header word 30).
+ * ground (header word 30), result grid at the detector or on the other level (word 31).
  * Returns 0, or nonzero with error text (the round group is then rejected).
  * Replaces monki_metal_run (kernels/metal/bridge.m:88-133), whose out is float per shard. */
 int monki_hip_run(void *handle, const uint32_t *controls, double *out, char *error,

@@ -127,6 +127,9 @@ struct Scene {
   // ends at the ground and registers at its exit column (main.f90:711-745). Host-side choice
   // of the kernel instantiation; last, so that the hot fields keep their offsets.
   int detector_surface;
+  // Result grid on the other level than the detector (main.f90:747-757): contributions register
+  // where the line of sight through the event meets that level.
+  int register_opposite;
 };
 
 struct Controls {
@@ -731,6 +734,23 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
   h.first_cell = 0;
 }
 
+// Result grid on the other level than the detector (mc_routines_cartesian_mod.f90:1041-1105,
+// direction -1): follow the line of sight back from the event, vertically to the ground for a
+// TOA detector or to TOA for a ground detector, and return the column reached there. rz is
+// the event's distance to the layer face toward the detector.
+template <typename R, bool DOWN>
+__device__ inline int opposite_grid_pixel(const Scene &s, const Photon<R> &p, R rz, int view) {
+  const R thickness = R(s.zunit[p.k]);
+  const R above = R(s.height_above[p.k]);
+  const R vertical = DOWN ? above + thickness - rz : R(s.height_above[0] + s.zunit[0]) - above - rz;
+  const R back = -vertical * R(s.view_inv_mu[view]);
+  const R x = R(p.i) * R(s.xunit) + (p.a >= R(0) ? R(s.xunit) - p.rx : p.rx) + R(s.view_a[view]) * back;
+  const R y = R(p.j) * R(s.yunit) + (p.b >= R(0) ? R(s.yunit) - p.ry : p.ry) + R(s.view_b[view]) * back;
+  const int i = min(int(periodic<R>(x, R(s.xunit) * R(s.nx)) / R(s.xunit)), s.nx - 1);
+  const int j = min(int(periodic<R>(y, R(s.yunit) * R(s.ny)) / R(s.yunit)), s.ny - 1);
+  return max(i, 0) + s.nx * max(j, 0);
+}
+
 // Advance one photon to its next collision or surface hit (mc.f90:476-677), apply the albedo
 // or the species' single-scattering albedo (mc.f90:681-727, 1028-1074), and evaluate the local
 // estimate toward every view (mc.f90:738-822, 1109-1207; mc_routines_cartesian_mod.f90:849-1222)
@@ -940,7 +960,13 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
     const R ci = prob.i * attenuation;
     // Backward transport registers at the injection column (mc_routines_cartesian_mod.f90:
     // 1035-1039), the launch column of the photon.
-    out.pixel(view) = BACK ? int(h.rng.photon % uint32_t(s.nx * s.ny)) : hit.i + s.nx * hit.j;
+    if (BACK) {
+      out.pixel(view) = int(h.rng.photon % uint32_t(s.nx * s.ny));
+    } else if (__builtin_expect(s.register_opposite, 0)) {
+      out.pixel(view) = opposite_grid_pixel<R, DOWN>(s, p, rz, view);
+    } else {
+      out.pixel(view) = hit.i + s.nx * hit.j;
+    }
     out.weight(view) = ci;
     h.contribution += float(ci);
     out.q(view) = prob.q * attenuation * geometry;
@@ -1731,7 +1757,7 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
       !word_int(t, 7, 0, 1, s.paths) || !word_int(t, 8, 0, kMaxLayers, s.uniform_top) ||
       !word_int(t, 9, 1, 64, s.shards) || !word_int(t, 26, kHeaderWords, 1 << 24, length) ||
       !word_int(t, 28, 0, 1, s.backward) || !word_int(t, 29, 0, 1, s.scats) ||
-      !word_int(t, 30, 0, 1, s.detector_surface)) {
+      !word_int(t, 30, 0, 1, s.detector_surface) || !word_int(t, 31, 0, 1, s.register_opposite)) {
     set_error(error, size, "Invalid packed table dimensions");
     return false;
   }
@@ -2360,7 +2386,8 @@ void *monki_hip_batch_create(uint32_t device, const float *tables, const size_t 
                              uint32_t block, char *error, size_t error_size) {
   Session *h = create_session(device, tables, offsets, scene_count, 1, block, error, error_size);
   for (size_t i = 0; h && i < h->scenes.size(); ++i) {
-    if (h->scenes[i].nmat == 4 || h->scenes[i].backward || h->scenes[i].scats || h->scenes[i].detector_surface) {
+    const Scene &sc = h->scenes[i];
+    if (sc.nmat == 4 || sc.backward || sc.scats || sc.detector_surface || sc.register_opposite) {
       release(h);
       set_error(error, error_size, "Batch sessions take forward nmat 1 or 3 scenes (run others one at a time)");
       return nullptr;

@@ -157,7 +157,7 @@ class HIPBatch:
             shapes.append(shape)
             nmat, nx, ny, nz, views = shape
 
-            if nmat == 4 or getattr(case.scene, "save_scats", False) or case.scene.detector != "toa":
+            if nmat == 4 or case.scene.save_scats or case.scene.detector != "toa" or case.scene.register not in (None, "toa"):
                 raise NotImplementedError("HIPBatch takes nmat 1 or 3, TOA detectors, no scattering decomposition; use HIP")
 
             counts.append(tally_count(nmat, nx, ny, nz, views, self.save_path))

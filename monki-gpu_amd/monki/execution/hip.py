@@ -222,9 +222,9 @@ class HIP(Backend):
         if self.save_scats and tables.detector != "toa":
             raise NotImplementedError("the scattering decomposition runs with the detector at TOA")
 
-        if scene.backward and tables.detector != "toa":
+        if scene.backward and (tables.detector != "toa" or tables.register not in (None, "toa")):
             # main.f90:765-781 allows backward runs only with source, detector and grid at TOA.
-            raise NotImplementedError("backward transport runs with the detector at TOA")
+            raise NotImplementedError("backward transport runs with the detector and the result grid at TOA")
 
         packed, self._shape = pack_tables(tables, shards=self.tally_shards, backward=scene.backward)
         nmat, nx, ny, nz, views = self._shape

@@ -21,6 +21,7 @@ Header (64 float32 words; integers are stored exactly, all < 2**24):
 28    transport direction: 0 forward, 1 backward (mc.f90 backward_mode)
 29    scattering decomposition: 1 runs calc_radiances_and_scats (no layer paths)
 30    detector and result grid: 0 TOA, 1 surface (views point down)
+31    result grid on the other level than the detector: 1 (0: at the detector)
 32+8s species s: LUT size, albedo, offsets of cosine block (cos, 1-cos, 1+cos),
       CDF, F11, F22, F33, F12; with nmat 4, F34 and F44 follow F12 directly (same length)
 ====  =====================================================================
@@ -116,6 +117,7 @@ def pack_tables(scene, shards: int = 32, backward: bool = False):
     header[28] = 1.0 if backward else 0.0
     header[29] = 1.0 if scene.save_scats else 0.0
     header[30] = 1.0 if scene.detector == "surface" else 0.0
+    header[31] = 1.0 if (scene.register or scene.detector) != scene.detector else 0.0
 
     for index, sp in enumerate(scene.species):
         cdf = sp.bound_cdf(nmat) if sp.cdf is None else sp.cdf

@@ -666,11 +666,9 @@ def prepare_fixture(
     if fixture.wavelengths != 1:
         raise NotImplementedError("one wavelength per prepared scene (worker.f90:1088)")
 
-    if fixture.save_scats:
-        raise NotImplementedError("scattering contribution diagnostics are not implemented")
-
-    if (fixture.source_location, fixture.detector_location, fixture.register_location) != (1, 1, 1):
-        raise NotImplementedError("only a TOA source, TOA detector and TOA result grid are supported")
+    if (fixture.source_location, fixture.detector_location) != (1, 1) or fixture.register_location not in (0, 1):
+        # A ground detector needs view zeniths of 90 degrees or more, which cases do not take.
+        raise NotImplementedError("only a TOA source and a TOA detector are supported (result grid TOA or surface)")
 
     if fixture.background != "Earth":
         raise NotImplementedError("only the Earth molecular background is restated")
@@ -732,6 +730,7 @@ def prepare_fixture(
         view_zeniths=tuple(view_zeniths),
         species=species,
         save_path=save_path,
+        register=None if fixture.register_location == 1 else "surface",
         wavelength_nm=wavelength,
         altitude_m=fixture.alt[:nz].copy(),
         description=description or Path(path).name,

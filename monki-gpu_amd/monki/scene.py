@@ -94,6 +94,10 @@ class TransportScene:
     # "surface" (views 90 < zenith <= 180: the direction toward a detector at the ground),
     # main.f90:711-745 with the result grid at the detector.
     detector: str = "toa"
+    # Result grid: None (at the detector), "toa" or "surface" (main.f90:747-757); a grid away
+    # from the detector registers each contribution at the column where the line of sight
+    # through the event meets the grid's level (mc_routines_cartesian_mod.f90:1041-1105).
+    register: Optional[str] = None
     wavelength_nm: float = 340.0
     altitude_m: Optional[np.ndarray] = None  # (nz,) level altitudes, bottom to TOA
     description: str = ""
@@ -179,6 +183,9 @@ class TransportScene:
         if self.detector not in ("toa", "surface"):
             raise ValueError('detector must be "toa" or "surface"')
 
+        if self.register not in (None, "toa", "surface"):
+            raise ValueError('register must be None, "toa" or "surface"')
+
         if not np.isfinite(self.solar_zenith) or not 0 <= self.solar_zenith < 90:
             raise ValueError("zeniths must be in [0, 90)")
 
@@ -251,7 +258,7 @@ class TransportScene:
             len(self.species),
             nz,  # k_source: TOA
             nz if self.detector == "toa" else 0,  # k_detector
-            nz if self.detector == "toa" else 0,  # k_register: result grid at the detector
+            nz if (self.register or self.detector) == "toa" else 0,  # k_register
             int(self.save_path),
             int(backward),  # transport direction (mc.f90 backward_mode)
         ]
@@ -310,10 +317,11 @@ class TransportScene:
 
         nx, ny, nz, nmat, angles, nspecies = (int(v) for v in raw[1:7])
 
-        if raw[11] != 0 or tuple(raw[7:10]) not in ((nz, nz, nz), (nz, 0, 0)):
-            raise NotImplementedError("only forward exports with a TOA source and the grid at the detector are read")
+        if raw[11] != 0 or raw[7] != nz or raw[8] not in (0, nz) or raw[9] not in (0, nz):
+            raise NotImplementedError("only forward exports with a TOA source are read")
 
         metadata.setdefault("detector", "toa" if raw[8] == nz else "surface")
+        metadata.setdefault("register", None if raw[9] == raw[8] else ("toa" if raw[9] == nz else "surface"))
 
         offset = 18
 

@@ -68,7 +68,8 @@ typedef struct {
 
 typedef struct {
   int nx, ny, nz, nmat, views, species, paths, backward, scats;
-  int surface_detector; /* detector and result grid at the ground (views point down) */
+  int surface_detector; /* detector at the ground (views point down) */
+  int register_opposite; /* result grid on the other level than the detector */
   double xunit, yunit, depol, theta0, phi0, phi;
   double view[MAX_VIEWS];
   const float *zunit, *albedo, *beta, *cumfrac, *ssalb_gas;
@@ -463,17 +464,20 @@ static int detect(const scene_t *s, const pos_t *ev, double prob[MAX_VIEWS][4],
       lengths[k_old] += d;
       ++out->counters[4];
     }
-    /* Analytical register mapping (mc_routines_cartesian_mod.f90:1076-1105), TOA grid. */
+    /* Analytical register mapping (mc_routines_cartesian_mod.f90:1041-1105): the grid at TOA
+     * (k_register = nz) or at the ground; direction -1 when the grid is not at the detector. */
     double height = 0.0;
     for (int k = 0; k < ev->k; ++k) height += s->zunit[k];
     height += (vc > 0 ? s->zunit[ev->k] - z_old : z_old);
     double top = 0.0;
     for (int k = 0; k < s->nz - 1; ++k) top += s->zunit[k];
-    if (!s->surface_detector) height = top - height; /* k_register = nz: measured from TOA */
+    int register_toa = s->surface_detector == s->register_opposite;
+    int direction = s->register_opposite ? -1 : 1;
+    if (register_toa) height = top - height; /* k_register = nz: measured from TOA */
     double dxh = fabs(va) * height / fabs(vc), dyh = fabs(vb) * height / fabs(vc);
-    int di = (int)((s->xunit - x_old + dxh) / s->xunit);
-    int dj = (int)((s->yunit - y_old + dyh) / s->yunit);
-    int si = va >= 0 ? 1 : -1, sj = vb >= 0 ? 1 : -1;
+    int di = direction > 0 ? (int)((s->xunit - x_old + dxh) / s->xunit) : (int)((x_old + dxh) / s->xunit);
+    int dj = direction > 0 ? (int)((s->yunit - y_old + dyh) / s->yunit) : (int)((y_old + dyh) / s->yunit);
+    int si = (va >= 0 ? 1 : -1) * direction, sj = (vb >= 0 ? 1 : -1) * direction;
     int pi = (((ev->i + si * di) % s->nx) + s->nx) % s->nx;
     int pj = (((ev->j + sj * dj) % s->ny) + s->ny) % s->ny;
     if (s->backward) pi = column % s->nx, pj = column / s->nx; /* injection column */
@@ -688,6 +692,7 @@ static int parse(const float *t, size_t count, scene_t *s) {
   s->views = word(t, 5), s->species = word(t, 6), s->paths = word(t, 7), s->backward = word(t, 28);
   s->scats = word(t, 29);
   s->surface_detector = word(t, 30);
+  s->register_opposite = word(t, 31);
   if (s->scats) s->paths = 0; /* calc_radiances_and_scats replaces the path tallies */
   if (s->nz > MAX_LAYERS || s->views > MAX_VIEWS || s->species > MAX_SPECIES) return 1;
   s->xunit = t[10], s->yunit = t[11], s->depol = t[12];

@@ -328,7 +328,8 @@ def test_oracle_absorbing_column_closed_form(oracle):
 @pytest.mark.skipif(not (Path(__file__).parents[1] / "oracle" / "_ref" / "monki_ref").is_file(),
                     reason="reference Fortran build missing (make -C oracle ref)")  # fmt: skip
 @pytest.mark.parametrize("name", ["clear-IQU", "box-cloud", "box-cloud-IQUV", "box-cloud-backward",
-                                  "box-cloud-IQUV-backward", "box-cloud-surface-detector"])  # fmt: skip
+                                  "box-cloud-IQUV-backward", "box-cloud-surface-detector", "box-cloud-ground-grid",
+                                  "box-cloud-surface-detector-toa-grid"])  # fmt: skip
 def test_oracle_matches_reference_fortran_statistically(oracle, name):
     """The float64 oracle against the reference's unchanged mc.f90 (its own compiler random
     numbers): radiance and layer paths per pixel/view within combined standard errors, for
@@ -352,6 +353,10 @@ def test_oracle_matches_reference_fortran_statistically(oracle, name):
         "box-cloud-IQUV-backward": replace(off_plane, nmat=4),
         # Detector and grid at the ground: views pointing down (zenith 180, 150, 120).
         "box-cloud-surface-detector": replace(off_plane, detector="surface", view_zeniths=(180.0, 150.0, 120.0)),
+        # Result grid on the other level: TOA views registered at the ground, and back.
+        "box-cloud-ground-grid": replace(off_plane, register="surface"),
+        "box-cloud-surface-detector-toa-grid": replace(off_plane, detector="surface", register="toa",
+                                                       view_zeniths=(180.0, 150.0, 120.0)),  # fmt: skip
     }[name]
     backward = name.endswith("backward")
     rounds, ppc = 40, 60

@@ -147,6 +147,20 @@ def test_surface_detector_matches_reference_fortran(tmp_path):
     assert np.all(mean[0] > 0)  # every ground pixel sees diffuse light
 
 
+@pytest.mark.parametrize("grid", ["surface", "toa"])
+def test_result_grid_on_the_other_level_matches_reference_fortran(tmp_path, grid):
+    """Result grid away from the detector (k_register != k_detector): TOA views registered at
+    the ground, and ground views registered at TOA, against the reference Fortran."""
+
+    from monki import configs
+
+    scene = configs.box_cloud(nx=8, ny=6, nmat=3, solar_zenith=50.0, box=(3, 2), offset=(2, 1))
+    scene = replace(scene, solar_azimuth=30.0, view_azimuth=120.0, register=grid)
+    if grid == "toa":
+        scene = replace(scene, detector="surface", view_zeniths=(180.0, 150.0, 120.0))
+    compare_with_fortran(scene, 100, 40, 9191, tmp_path, f"grid-{grid}")
+
+
 def test_scalar_transport_matches_reference_fortran(tmp_path):
     from monki import configs
 

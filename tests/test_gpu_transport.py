@@ -38,6 +38,10 @@ def small_scenes():
         # Detector and result grid at the ground: views pointing down.
         "box-cloud-surface": replace(off_plane, nmat=3, detector="surface", view_zeniths=(180.0, 150.0, 120.0)),
         "box-cloud-IQUV-surface": replace(off_plane, detector="surface", view_zeniths=(180.0, 150.0, 120.0)),
+        # Result grid on the other level than the detector.
+        "box-cloud-ground-grid": replace(off_plane, nmat=3, register="surface"),
+        "box-cloud-surface-toa-grid": replace(off_plane, nmat=3, detector="surface", register="toa",
+                                              view_zeniths=(180.0, 150.0, 120.0)),  # fmt: skip
     }
 
 
@@ -136,10 +140,12 @@ AGREEMENT = {  # (float32, float64) over all photons
     "box-cloud-IQUV-backward": (0.90, 0.998),
     "box-cloud-surface": (0.90, 0.998),
     "box-cloud-IQUV-surface": (0.90, 0.998),
+    "box-cloud-ground-grid": (0.90, 0.998),
+    "box-cloud-surface-toa-grid": (0.90, 0.998),
 }
 
 
-EXPLICIT_VIEWS = {"box-cloud-surface", "box-cloud-IQUV-surface"}
+EXPLICIT_VIEWS = {"box-cloud-surface", "box-cloud-IQUV-surface", "box-cloud-surface-toa-grid"}
 
 
 def check_histories(name, gpu, cpu, precision):
@@ -213,7 +219,8 @@ def test_float32_histories_match_float64_oracle(oracle, name):
 
 
 @pytest.mark.parametrize("name", ["clear-IQU", "layered-cloud", "box-cloud", "box-cloud-IQUV", "box-cloud-backward",
-                                  "box-cloud-IQUV-backward", "box-cloud-surface", "box-cloud-IQUV-surface"])  # fmt: skip
+                                  "box-cloud-IQUV-backward", "box-cloud-surface", "box-cloud-IQUV-surface",
+                                  "box-cloud-ground-grid", "box-cloud-surface-toa-grid"])  # fmt: skip
 def test_float64_build_matches_oracle(oracle, name):
     """The float64 build runs the oracle's algorithm on the device."""
 

@@ -242,3 +242,30 @@ def test_reference_case_prepares_a_scene_case():
     assert scene.varying_layers().size == 0
     assert math.isclose(prepared.baer[0, 0, 0, 8], 5.0) and scene.beta[0, 0, 8] > 5.0
     assert provenance["case"]["input_file"] == case.input_file and len(provenance["source_input_sha256"]) == 64
+
+
+def test_fixture_with_a_ground_result_grid(tmp_path):
+    """A fixture whose result-grid line (main.f90:747-757) is 0 prepares a scene that registers
+    at the ground; a ground detector is refused (cases keep view zeniths below 90 degrees)."""
+
+    import shutil
+
+    from monki import ReferenceCase
+    from monki.preparation import UPSTREAM, prepare_case
+
+    upstream = tmp_path / "upstream"
+    shutil.copytree(UPSTREAM / "INPUT", upstream / "INPUT")
+    (upstream / "DATA").symlink_to((UPSTREAM / "DATA").resolve())
+    name = "testrun/MONKI_MLS_cloudfree_340_340_test.in"
+    source = upstream / "INPUT" / name
+    lines = source.read_text().splitlines(keepends=True)
+    assert "definition of output nx*ny grid" in lines[11] and lines[11].startswith("1")
+    lines[11] = "0" + lines[11][1:]
+    source.write_text("".join(lines))
+    scene_case, _, _ = prepare_case(ReferenceCase("grid", name), upstream)
+    assert scene_case.scene.register == "surface" and scene_case.scene.detector == "toa"
+
+    lines[10] = "0" + lines[10][1:]  # detector at the ground
+    source.write_text("".join(lines))
+    with pytest.raises(NotImplementedError, match="TOA detector"):
+        prepare_case(ReferenceCase("grid", name), upstream)

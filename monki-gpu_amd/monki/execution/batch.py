@@ -139,7 +139,7 @@ class HIPBatch:
             raise TypeError("HIPBatch.prepare expects a non-empty list of ReferenceCase or SceneCase")
 
         if any(case.backward for case in cases):
-            raise NotImplementedError("HIP backward transport is not implemented")
+            raise NotImplementedError("HIPBatch runs forward transport; run backward cases with HIP")
 
         extras, preparations = [None] * len(cases), [None] * len(cases)
 

@@ -416,8 +416,9 @@ def main() -> int:
 
     if args.gpus != 1:
         print(
-            f"bench.py measures one GPU: this build has no multi-GPU path (asked for {args.gpus}). "
-            "Photon histories shard by photon range, round or scene; see DESIGN.md.",
+            f"bench.py measures one GPU: there is no multi-GPU path (asked for {args.gpus}). "
+            "Photon histories would shard by photon range, round or scene, with one reduction of "
+            "each round's tallies.",
             file=sys.stderr,
         )
         return 2

@@ -71,6 +71,8 @@ def small_scenes():
         "box-cloud-surface-toa-grid": replace(off_plane, nmat=3, detector="surface", register="toa",
                                               view_zeniths=(180.0, 150.0, 120.0)),  # fmt: skip
         "max-size": max_size_scene(),
+        # A single periodic column: every horizontal crossing re-enters the same column.
+        "one-column": configs.layered_cloud(nx=1, ny=1, nmat=3),
     }
 
 
@@ -172,6 +174,7 @@ AGREEMENT = {  # (float32, float64) over all photons
     "box-cloud-ground-grid": (0.90, 0.998),
     "box-cloud-surface-toa-grid": (0.90, 0.998),
     "max-size": (0.90, 0.998),
+    "one-column": (0.995, 0.999),
 }
 
 
@@ -255,7 +258,8 @@ def test_float32_histories_match_float64_oracle(oracle, name):
 
 @pytest.mark.parametrize("name", ["clear-IQU", "layered-cloud", "box-cloud", "box-cloud-IQUV", "box-cloud-backward",
                                   "box-cloud-IQUV-backward", "box-cloud-surface", "box-cloud-IQUV-surface",
-                                  "box-cloud-ground-grid", "box-cloud-surface-toa-grid", "max-size"])  # fmt: skip
+                                  "box-cloud-ground-grid", "box-cloud-surface-toa-grid", "max-size",
+                                  "one-column"])  # fmt: skip
 def test_float64_build_matches_oracle(oracle, name):
     """The float64 build runs the oracle's algorithm on the device."""
 

@@ -359,6 +359,26 @@ def test_packed_tables_reject_oversized_scenes():
     with pytest.raises(ValueError, match="shards"):
         pack_tables(configs.clear_sky(nx=2, ny=2), shards=65)
 
+    # Layer paths and Stokes components share one 64-lane tally record.
+    grid = lambda physical, **options: configs._assemble(
+        "layers", 2, 2, np.linspace(0.0, 60.0, physical + 1), [], [], 3, 40.0, 0.05, **options
+    )
+    assert pack_tables(grid(60))[1] == (3, 2, 2, 61, 3)
+    with pytest.raises(ValueError, match="exceed 64"):
+        pack_tables(grid(61))
+    assert pack_tables(grid(63, save_path=False))[1] == (3, 2, 2, 64, 3)
+    with pytest.raises(ValueError, match="at most 64 layers"):
+        pack_tables(grid(64, save_path=False))
+
+    aerosol = configs.aerosol()
+    species = lambda count: configs._assemble(
+        "species", 2, 2, configs.LEVELS_KM, [np.full(len(configs.LEVELS_KM) - 1, 0.01)] * count,
+        [replace(aerosol, name=f"aerosol{index}") for index in range(count)], 3, 40.0, 0.05,
+    )  # fmt: skip
+    assert pack_tables(species(4))[0][6] == 4
+    with pytest.raises(NotImplementedError, match="at most 4 tabulated species"):
+        pack_tables(species(5))
+
 
 # -- validation and the NetCDF product (tests/unit/test_validation.py) ---------------------
 

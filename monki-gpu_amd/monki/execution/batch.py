@@ -50,8 +50,8 @@ class DeviceMoments:
     """Sums and online AMF moments of one scene's rounds, accumulated on the device.
 
     ``rounds`` rounds; ``paths`` (nx, ny, nz, views) and ``weights`` (nx, ny, views) are the
-    sums of the rounds' outputs; the moments have AMFMoments' meaning and shapes (physical
-    layers only, the weight moments broadcast over layers).
+    sums of the rounds' outputs; the moments are the online means and centred (co)moments of RatioMoments
+    (physical layers only; weight moments per pixel and view).
     """
 
     rounds: int
@@ -64,14 +64,10 @@ class DeviceMoments:
     weight_m2: np.ndarray
 
     def amf_moments(self):
-        from ..convergence import AMFMoments  # convergence imports this package
+        from ..convergence import RatioMoments  # convergence imports this package
 
-        moments = AMFMoments()
-        moments.count = self.rounds
-        moments.path_mean, moments.path_m2, moments.covariance = self.path_mean, self.path_m2, self.covariance
-        moments.weight_mean = self.weight_mean[:, :, None, :]
-        moments.weight_m2 = self.weight_m2[:, :, None, :]
-        return moments
+        return RatioMoments.from_moments(self.rounds, self.path_mean, self.weight_mean, self.path_m2, self.weight_m2,
+                                         self.covariance)  # fmt: skip
 
 
 class HIPBatch:

@@ -9,8 +9,10 @@ time for gfx950. There is no CPU transport fallback: without the library or a GP
 ``prepare`` raises.
 """
 
+import contextlib
 import ctypes
 import hashlib
+import itertools
 import platform
 import time
 from dataclasses import replace
@@ -310,28 +312,23 @@ class HIP(Backend):
         return self._run_group([controls])[0]
 
     def run_batches(self, controls: list) -> list:
-        """Schedule equal rounds together with independent accumulators (metal.py:219-242)."""
+        """Rounds in request order. Consecutive rounds with one photon budget share a dispatch
+        group of at most the round capacity (and photons_per_dispatch photons); every round
+        keeps its own tally banks and random streams, so grouping changes no result."""
 
-        self._validate_requests(controls)
-        results = []
-        first = 0
+        self._check_requests(controls)
+        return [batch for group in self._groups(controls) for batch in self._run_group(group)]
+
+    def _groups(self, controls: list):
         _, nx, ny, _, _ = self._shape
 
-        while first < len(controls):
-            photons = controls[first].photons_per_column * nx * ny
-            capacity = max(1, min(self._round_capacity, self.photons_per_dispatch // photons))
-            last = first + 1
+        for _, same_budget in itertools.groupby(controls, key=lambda request: request.photons_per_column):
+            run = list(same_budget)
+            fit = self.photons_per_dispatch // (run[0].photons_per_column * nx * ny)
+            size = max(1, min(self._round_capacity, fit))
 
-            while last < min(len(controls), first + capacity):
-                if controls[last].photons_per_column != controls[first].photons_per_column:
-                    break
-
-                last += 1
-
-            results.extend(self._run_group(controls[first:last]))
-            first = last
-
-        return results
+            for start in range(0, len(run), size):
+                yield run[start : start + size]
 
     def close(self) -> None:
         """Release device buffers; returned arrays stay valid."""
@@ -467,95 +464,106 @@ class HIP(Backend):
 
     def _require_session(self):
         if self._handle is None or self._native is None:
-            raise RuntimeError("HIP must be prepared before running a batch")
+            raise RuntimeError("HIP has no prepared session: call prepare first")
 
-    def _validate_requests(self, controls: list) -> None:
+    def _check_requests(self, controls: list) -> None:
+        """Refuse a request list before any device work."""
+
         self._require_session()
 
         if self._settings() != self._configuration:
-            raise ValueError("HIP settings changed; reprepare before running another batch")
+            raise ValueError("HIP settings differ from the prepared session's: call prepare again")
 
         _, nx, ny, _, _ = self._shape
+        expected = self._round
 
-        for offset, request in enumerate(controls, start=1):
+        for request in controls:
             if not isinstance(request, BatchControls):
-                raise TypeError("run_batch expects BatchControls")
+                raise TypeError(f"rounds are requested with BatchControls, got {type(request).__name__}")
 
-            if request.round_index != self._round + offset:
-                raise ValueError("round_index must be sequential, starting at one")
+            expected += 1
 
+            if request.round_index != expected:
+                raise ValueError(f"round {request.round_index} requested where round {expected} comes next")
+
+            # Photon ids of a dispatch group (rounds x photons per round) are 32-bit.
             if request.photons_per_column * nx * ny * len(controls) >= 2**32:
-                raise ValueError("HIP photon identity exceeds its 32-bit round range")
+                raise ValueError("photon ids of this request overflow 32 bits: request fewer rounds at once")
 
-    def _run_group(self, controls: list) -> List[RawTallies]:
-        """Trace a group of equal rounds and split the float64 totals per round."""
-
-        self._validate_requests(controls)
-        nmat, nx, ny, nz, views = self._shape
-        per_round = controls[0].photons_per_column * nx * ny
-        photon_count = per_round * len(controls)
-        total = np.zeros((len(controls), self._tally_count), dtype=np.float64)
-        scratch = np.empty_like(total)
-        error = ctypes.create_string_buffer(8192)
-        run = self._native.monki_hip_run if self.precision == 32 else self._native.monki_hip_run_fp64
-
-        if self._network is not None:
-            run = self._native.monki_hip_run_guided
-
-        if self.directed is not None:
-            run = self._native.monki_hip_run_directed
+    @contextlib.contextmanager
+    def _closed_on_failure(self):
+        """Any failure inside ends the session: its device state is no longer trusted."""
 
         try:
-            for first in range(0, photon_count, self.photons_per_dispatch):
-                count = min(self.photons_per_dispatch, photon_count - first)
-                command = (ctypes.c_uint32 * 8)(
-                    count,
-                    first,
-                    controls[0].round_index,
-                    self.seed,
-                    self.max_steps,
-                    int(self.profile),
-                    per_round,
-                    len(controls),
-                )
-                status = run(self._handle, command, scratch.ctypes.data_as(DOUBLE_POINTER), error, len(error))
+            yield
+        except BaseException:
+            self.close()
+            raise
 
-                if status or not np.all(np.isfinite(scratch)):
-                    raise RuntimeError(error.value.decode() or "Nonfinite HIP tallies")
+    def _kernel_entry(self):
+        native = self._native
 
-                total += scratch
+        if self.directed is not None:
+            return native.monki_hip_run_directed
+
+        if self._network is not None:
+            return native.monki_hip_run_guided
+
+        return native.monki_hip_run if self.precision == 32 else native.monki_hip_run_fp64
+
+    def _unpack(self, request: BatchControls, values: np.ndarray) -> RawTallies:
+        """One round's float64 tallies [Stokes | paths | weights] as owned Fortran-order arrays."""
+
+        nmat, nx, ny, nz, views = self._shape
+        stokes_end = nmat * nx * ny * views
+        stokes = np.array(values[:stokes_end].reshape((nmat, nx, ny, views), order="F"))
+
+        if self.save_scats:
+            return RawTallies(request, stokes, scattering=split_scattering(values, self._shape))
+
+        if not self.save_path:
+            return RawTallies(request, stokes)
+
+        paths_end = stokes_end + nx * ny * nz * views
+        paths = np.array(values[stokes_end:paths_end].reshape((nx, ny, nz, views), order="F"))
+        weights = np.array(values[paths_end:].reshape((nx, ny, views), order="F"))
+        return RawTallies(request, stokes, paths, weights)
+
+    def _run_group(self, controls: list) -> List[RawTallies]:
+        """One dispatch group: its photons in slices of photons_per_dispatch, the float64
+        tallies of each slice added per round."""
+
+        self._check_requests(controls)
+        _, nx, ny, _, _ = self._shape
+        per_round = controls[0].photons_per_column * nx * ny
+        photons = per_round * len(controls)
+        sums = np.zeros((len(controls), self._tally_count))
+        received = np.empty_like(sums)
+        error = ctypes.create_string_buffer(8192)
+        entry = self._kernel_entry()
+
+        with self._closed_on_failure():
+            for offset in range(0, photons, self.photons_per_dispatch):
+                words = (ctypes.c_uint32 * 8)(
+                    min(self.photons_per_dispatch, photons - offset), offset, controls[0].round_index, self.seed,
+                    self.max_steps, int(self.profile), per_round, len(controls),
+                )  # fmt: skip
+
+                if entry(self._handle, words, received.ctypes.data_as(DOUBLE_POINTER), error, len(error)):
+                    raise RuntimeError(error.value.decode() or "HIP dispatch failed")
+
+                if not np.isfinite(received).all():
+                    raise RuntimeError("HIP returned tallies that are not finite")
+
+                sums += received
                 self.provenance["gpu_dispatches"] += 1
 
             if self.profile:
                 self.provenance["gpu_seconds"] = self._native.monki_hip_gpu_seconds(self._handle)
 
-            results = []
-            stokes_count = nmat * nx * ny * views
-
-            for request, values in zip(controls, total):
-                stokes = values[:stokes_count].reshape((nmat, nx, ny, views), order="F").copy()
-                paths = weights = None
-
-                if self.save_scats:
-                    results.append(RawTallies(request, stokes, scattering=split_scattering(values, self._shape)))
-                    continue
-
-                if self.save_path:
-                    path_count = nx * ny * nz * views
-                    paths = (
-                        values[stokes_count : stokes_count + path_count]
-                        .reshape((nx, ny, nz, views), order="F")
-                        .copy()
-                    )
-                    weights = values[stokes_count + path_count :].reshape((nx, ny, views), order="F").copy()
-
-                results.append(RawTallies(request, stokes, paths, weights))
-
+            rounds = [self._unpack(request, values) for request, values in zip(controls, sums)]
             self._round = controls[-1].round_index
-            return results
-        except BaseException:
-            self.close()
-            raise
+            return rounds
 
     def _settings(self) -> tuple:
         return (

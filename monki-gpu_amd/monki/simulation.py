@@ -1,9 +1,20 @@
-"""Shared round statistics and the NetCDF product, independent of the transport backend.
+"""Round statistics, the NetCDF product and the run receipt, for any transport backend.
 
-``simulate`` restates ``python/monki/simulation.py:17-189`` (itself following Victor's
-``main.f90`` accumulate routine): float64 accumulation of per-round tallies, the standard
-error of the mean over rounds with the 999 sentinel through round 30, the radiance
-convergence history, DOP, layer AMFs (vacuum layer dropped) and the ``.run.json`` receipt.
+What is computed (the product definition of python/monki/simulation.py:73-185, after the
+accumulate routine of Victor's main.f90:1353-1478):
+
+* radiance: the mean over rounds of each round's Stokes tally per photon per column;
+* its standard error: the standard error of that mean over rounds, sqrt(s^2 / n) with the
+  unbiased sample variance s^2, reported as 999 (unavailable) through round 30;
+* the radiance convergence history: after each round, the running mean averaged over pixels
+  and views, per Stokes component;
+* DOP = |(Q, U, V)| / I (0 where I <= 1e-16);
+* layer AMFs = summed weighted layer paths / summed detector weights / layer thickness (km),
+  vacuum layer dropped; with AMF stopping, their delta-method standard errors.
+
+Backward transport reverses U in the product (simulation.py:128-131). Here the per-round
+moments are updated online (Welford): mean and centred sum of squares, which avoids the
+cancellation of sum-of-squares forms over thousands of rounds.
 """
 
 import json
@@ -13,220 +24,243 @@ from typing import Optional
 
 import numpy as np
 
-from .convergence import AMFConvergence, AMFStopping
+from .convergence import AMFConvergence, PrecisionStopping
 from .execution.backend import Backend
 from .execution.tallies import BatchControls, RawTallies
 from .metadata import file_hash
 from .output import SimulationResult
 
+# Standard errors are reported only after this many rounds; earlier ones carry the sentinel.
+UNCERTAIN_THROUGH = 30
+UNAVAILABLE = 999.0
+DOP_FLOOR = 1e-16
+
+
+def _product_order(array: np.ndarray) -> np.ndarray:
+    """Tallies in Fortran axis order (component, x, y, ..., view) to the product's reversed
+    axes with a trailing wavelength axis of one."""
+
+    return np.ascontiguousarray(np.transpose(array))[..., np.newaxis]
+
 
 class RoundStatistics:
-    """Round accumulation and product of one case (the body of simulation.py:45-189).
+    """Accumulates one case's rounds in float64 and writes its product and receipt.
 
-    ``add`` takes the rounds in order; ``finish`` computes the statistics, writes the NetCDF
-    product and the ``.run.json`` receipt. ``simulate`` drives one instance from one backend;
-    ``simulate_batch`` drives one per scene of a batch in lockstep.
+    ``add`` takes whole rounds in order (1, 2, ...); ``add_stokes`` + ``attach`` are the same
+    for backends that keep the layer paths on the device. ``simulate`` drives one instance,
+    ``simulate_batch`` one per scene in lockstep.
     """
 
     def __init__(self, case, metadata, output, convergence: Optional[AMFConvergence] = None):
         if convergence is not None and case.rounds < convergence.minimum_rounds:
-            raise ValueError("round cap must be at least minimum_rounds")
+            raise ValueError(f"round cap {case.rounds} is below the policy's minimum of {convergence.minimum_rounds}")
 
         self.output = Path(output).resolve()
         self.receipt = self.output.with_suffix(".run.json")
 
-        if self.output.exists() or self.receipt.exists():
-            raise FileExistsError(f"Refusing to overwrite an existing result: {self.output}")
+        for path in (self.output, self.receipt):
+            if path.exists():
+                raise FileExistsError(f"{path} exists; results are never overwritten")
 
         if metadata is None:
-            raise RuntimeError("Backend did not supply prepared scene metadata")
+            raise RuntimeError("the backend was prepared without product metadata")
 
         self.case = case
         self.metadata = metadata
-        self.total = self.rounds_sum = self.rounds_squared = self.paths = self.weights = None
-        self.scattering = None  # sums of the decomposition arrays, when the backend returns them
+        self.rounds = 0
+        self.mean = self.squares = None  # per-photon Stokes per round: running mean, centred squares
         self.history = []
+        self.paths = self.weights = None
+        self.scattering = None
         self.stopping = None
 
         if convergence is not None:
             if "AMF_layer" not in metadata.values:
-                raise ValueError("AMF stopping requires layer path diagnostics")
+                raise ValueError("the AMF precision needs layer-path tallies: run with save_path=True")
 
-            thickness = np.diff(metadata.values["alt"]) / 1000
-            self.stopping = AMFStopping(convergence, thickness)
+            self.stopping = PrecisionStopping(convergence, np.diff(metadata.values["alt"]) / 1000.0)
 
     @property
     def completed(self) -> int:
-        return len(self.history)
+        return self.rounds
 
     def add(self, batch) -> None:
-        """Accumulate the next round (round indices 1, 2, ... in order)."""
+        """The next round's tallies (``RawTallies``)."""
 
         if not isinstance(batch, RawTallies):
-            raise TypeError("Backend must return RawTallies")
+            raise TypeError(f"backends return RawTallies, got {type(batch).__name__}")
 
-        if self.total is None and batch.weighted_paths is not None:
-            self.paths = np.zeros_like(batch.weighted_paths)
-            self.weights = np.zeros_like(batch.weights)
-
-        if self.total is None and batch.scattering is not None:
-            self.scattering = {name: np.zeros_like(value) for name, value in batch.scattering.items()}
-
+        first = self.rounds == 0
         self.add_stokes(batch.controls, batch.stokes)
+
+        if first:
+            if batch.weighted_paths is not None:
+                self.paths = np.zeros_like(batch.weighted_paths, dtype=np.float64)
+                self.weights = np.zeros_like(batch.weights, dtype=np.float64)
+
+            if batch.scattering is not None:
+                self.scattering = {name: np.zeros_like(part, dtype=np.float64) for name, part in batch.scattering.items()}
 
         if self.scattering is not None:
             if batch.scattering is None:
-                raise ValueError("Backend dropped the scattering decomposition during a session")
+                raise ValueError(f"round {self.rounds} lacks the scattering decomposition of earlier rounds")
 
-            for name, value in batch.scattering.items():
-                self.scattering[name] += value
+            for name, part in batch.scattering.items():
+                self.scattering[name] += part
 
         if self.paths is not None:
             if batch.weighted_paths is None or batch.weights is None:
-                raise ValueError("Backend dropped path diagnostics during a session")
+                raise ValueError(f"round {self.rounds} lacks the layer-path tallies of earlier rounds")
 
             self.paths += batch.weighted_paths
             self.weights += batch.weights
 
     def add_stokes(self, controls, stokes) -> None:
-        """Accumulate the Stokes tallies of the next round; the layer paths of device-accumulated
-        rounds arrive through ``attach``."""
+        """The next round's Stokes tally (nmat, nx, ny, views)."""
 
-        case = self.case
-        index = self.completed + 1
+        expected = BatchControls(self.rounds + 1, self.case.photons_per_column)
 
-        if controls != BatchControls(index, case.photons_per_column):
-            raise ValueError("Backend returned tallies for a different round")
+        if controls != expected:
+            raise ValueError(f"expected tallies of {expected}, got {controls}")
 
-        if self.total is None:
-            self.total = np.zeros_like(stokes)
-            self.rounds_sum = np.zeros_like(stokes)
-            self.rounds_squared = np.zeros_like(stokes)
+        sample = np.asarray(stokes, dtype=np.float64) / self.case.photons_per_column
 
-        self.total += stokes
-        normalized = stokes / case.photons_per_column
-        self.rounds_sum += normalized
-        self.rounds_squared += normalized**2
-        _, nx, ny, angles = self.total.shape
-        # Successive spatial sums and divisions as in the original accumulate.
-        self.history.append(self.rounds_sum.sum(axis=1).sum(axis=1).sum(axis=1) / nx / ny / angles / index)
+        if self.mean is None:
+            self.mean = np.zeros_like(sample)
+            self.squares = np.zeros_like(sample)
+
+        self.rounds += 1
+        step = sample - self.mean
+        self.mean = self.mean + step / self.rounds
+        self.squares = self.squares + step * (sample - self.mean)
+        # Domain average of the running mean per Stokes component.
+        self.history.append(self.mean.reshape(self.mean.shape[0], -1).mean(axis=1))
 
     def attach(self, moments) -> None:
-        """Take the layer-path sums and AMF moments a batch backend accumulated on the device
-        (``HIPBatch.moments``) for the rounds added so far."""
+        """Layer-path sums and AMF moments kept on the device (``HIPBatch.moments``) for the
+        rounds added so far."""
 
-        if moments.rounds != self.completed:
-            raise ValueError("Device moments cover a different number of rounds")
+        if moments.rounds != self.rounds:
+            raise ValueError(f"device moments cover {moments.rounds} rounds, the host {self.rounds}")
 
         self.paths, self.weights = moments.paths, moments.weights
 
         if self.stopping is not None:
             self.stopping.moments = moments.amf_moments()
 
+    # -- product ----------------------------------------------------------------------------
+
+    def _standard_error(self) -> np.ndarray:
+        if self.rounds <= UNCERTAIN_THROUGH:
+            return np.full_like(self.mean, UNAVAILABLE)
+
+        if np.any(self.squares < 0.0):  # a centred sum of squares only falls below zero by a bug
+            raise ArithmeticError("negative centred sum of squares in the round statistics")
+
+        return np.sqrt(self.squares / ((self.rounds - 1) * self.rounds))
+
+    def _radiance_fields(self) -> dict:
+        radiance, error = self.mean.copy(), self._standard_error()
+        history = np.array(self.history)
+
+        if self.case.backward and radiance.shape[0] >= 3:
+            radiance[2] = -radiance[2]
+            history[:, 2] = -history[:, 2]
+
+        polarized = np.linalg.norm(radiance[1:], axis=0)
+        dop = np.zeros_like(radiance[0])
+        np.divide(polarized, radiance[0], out=dop, where=radiance[0] > DOP_FLOOR)
+        return {
+            "radiance": _product_order(radiance),
+            "std_radiance": _product_order(error),
+            "DOP": _product_order(dop),
+            "radiance_convergence": history[..., np.newaxis],
+        }
+
+    def _amf_fields(self, altitudes) -> dict:
+        if self.paths is None:
+            return {}
+
+        if not np.all(self.weights > 0):
+            raise ValueError("a pixel and view without detector weight has no layer AMF")
+
+        # Path lengths are in km and altitudes in m; the last (vacuum) layer is not reported.
+        dz_km = np.diff(altitudes) / 1000.0
+        amf = self.paths[:, :, :-1, :] / self.weights[:, :, np.newaxis, :] / dz_km[:, np.newaxis]
+        return {"AMF_layer": _product_order(amf)}
+
+    def _decomposition_fields(self) -> dict:
+        if self.scattering is None:
+            return {}
+
+        per_photon = 1.0 / (self.rounds * self.case.photons_per_column)  # main.f90:1168-1171
+        parts = {name: value * per_photon for name, value in self.scattering.items()}
+
+        if self.case.backward and parts["stokes_sing"].shape[0] >= 3:
+            for part in parts.values():
+                part[2] = -part[2]  # main.f90:1190-1198
+
+        return {
+            "radiance_sing": _product_order(parts["stokes_sing"]),
+            "radiance_mult": _product_order(parts["stokes_mult"]),
+            "contrib_radiance_3D": _product_order(parts["contrib_sing"] + parts["contrib_mult"]),
+            "contrib_radiance_3D_sing": _product_order(parts["contrib_sing"]),
+            "contrib_radiance_3D_mult": _product_order(parts["contrib_mult"]),
+        }
+
     def finish(self, backend_provenance: dict) -> SimulationResult:
-        case, metadata, output = self.case, self.metadata, self.output
-        total, rounds_sum, rounds_squared = self.total, self.rounds_sum, self.rounds_squared
-        paths, weights, stopping = self.paths, self.weights, self.stopping
-        completed = self.completed
+        """Write the NetCDF product and the ``.run.json`` receipt; return the product."""
 
-        if completed == 0:
-            raise ValueError("No statistical round was run")
+        if self.rounds == 0:
+            raise ValueError("no round was run: nothing to report")
 
-        _, nx, ny, _ = total.shape
-        radiance = total / completed / case.photons_per_column
-        uncertainty = np.full_like(radiance, 999.0)
-
-        # 999 marks an unavailable uncertainty through round 30 (simulation.py:113-124).
-        if completed > 30:
-            variance_sum = rounds_squared - (1.0 / completed) * rounds_sum**2
-            tolerance = 64 * np.finfo(float).eps * np.maximum(rounds_squared, 1e-300)
-
-            if np.any(variance_sum < -tolerance):
-                raise ArithmeticError("Negative round variance exceeds floating-point roundoff")
-
-            uncertainty = np.sqrt(np.maximum(variance_sum, 0) / (completed - 1)) / np.sqrt(
-                completed
-            )
-
-        radiance_history = np.asarray(self.history)
-
-        if case.backward and radiance.shape[0] >= 3:
-            radiance[2] *= -1
-            radiance_history[:, 2] *= -1
-
-        intensity = radiance[0]
-        polarization = np.sqrt(np.sum(radiance[1:] ** 2, axis=0))
-        dop = np.divide(
-            polarization, intensity, out=np.zeros_like(intensity), where=intensity > 1e-16
-        )
-        values = {name: value.copy() for name, value in metadata.values.items()}
-        values["radiance"] = radiance.transpose(3, 2, 1, 0)[..., None]
-        values["std_radiance"] = uncertainty.transpose(3, 2, 1, 0)[..., None]
-        values["DOP"] = dop.transpose(2, 1, 0)[..., None]
-        values["radiance_convergence"] = radiance_history[..., None]
-        values["n_rounds"] = np.full_like(values["n_rounds"], completed)
-        values["n_photons_per_column"] = np.full_like(
-            values["n_photons_per_column"], case.photons_per_column
-        )
-
-        if paths is not None:
-            if np.any(weights <= 0):
-                raise ValueError("Layer AMF is undefined for a pixel with no detected weight")
-
-            # Lengths are km, altitude metadata is m; the vacuum layer is dropped.
-            thickness = np.diff(values["alt"]) / 1000.0
-            amf = paths[:, :, :-1, :] / weights[:, :, None, :] / thickness[None, None, :, None]
-            values["AMF_layer"] = amf.transpose(3, 2, 1, 0)[..., None]
-
-        if self.scattering is not None:
-            # Sums over rounds / rounds / photons per column (main.f90:1168-1171), U reversed
-            # for backward transport like the radiance; the 3-D total is single + multiple.
-            scale = 1.0 / completed / case.photons_per_column
-            sing, mult = self.scattering["stokes_sing"] * scale, self.scattering["stokes_mult"] * scale
-            contrib_sing = self.scattering["contrib_sing"] * scale
-            contrib_mult = self.scattering["contrib_mult"] * scale
-            if case.backward and sing.shape[0] >= 3:
-                for part in (sing, mult, contrib_sing, contrib_mult):
-                    part[2] *= -1  # main.f90:1190-1198
-
-            values["radiance_sing"] = sing.transpose(3, 2, 1, 0)[..., None]
-            values["radiance_mult"] = mult.transpose(3, 2, 1, 0)[..., None]
-            values["contrib_radiance_3D"] = (contrib_sing + contrib_mult).transpose(4, 3, 2, 1, 0)[..., None]
-            values["contrib_radiance_3D_sing"] = contrib_sing.transpose(4, 3, 2, 1, 0)[..., None]
-            values["contrib_radiance_3D_mult"] = contrib_mult.transpose(4, 3, 2, 1, 0)[..., None]
-
-        dimensions = dict(metadata.dimensions)
-        attributes = dict(metadata.attributes)
+        values = {name: np.array(value, copy=True) for name, value in self.metadata.values.items()}
+        dimensions = dict(self.metadata.dimensions)
+        attributes = dict(self.metadata.attributes)
+        values.update(self._radiance_fields())
+        values.update(self._amf_fields(values["alt"]))
+        values.update(self._decomposition_fields())
+        values["n_rounds"] = np.full_like(values["n_rounds"], self.rounds)
+        values["n_photons_per_column"] = np.full_like(values["n_photons_per_column"], self.case.photons_per_column)
+        stopping = self.stopping
 
         if stopping is not None:
-            _, amf_error = stopping.moments.estimate(stopping.thickness)
-            values["std_AMF_layer"] = amf_error.transpose(3, 2, 1, 0)[..., None]
+            values["std_AMF_layer"] = _product_order(stopping.moments.estimate(stopping.thickness)[1])
             dimensions["std_AMF_layer"] = dimensions["AMF_layer"]
             attributes["std_AMF_layer"] = {"long_name": "Layer AMF standard error", "units": "-"}
 
-        result = SimulationResult(output, values, dimensions, attributes)
+        result = SimulationResult(self.output, values, dimensions, attributes)
         result.save()
-        provenance = {
-            **backend_provenance,
-            "status": "complete",
-            "completed_rounds": completed,
-            "total_photons": completed * case.photons_per_column * nx * ny,
-            "output_sha256": file_hash(output),
-        }
+        _, nx, ny, _ = self.mean.shape
+        receipt = dict(backend_provenance)
+        receipt.update(
+            status="complete", completed_rounds=self.rounds,
+            total_photons=self.rounds * self.case.photons_per_column * nx * ny, output_sha256=file_hash(self.output),
+        )  # fmt: skip
 
         if stopping is not None:
-            provenance["status"] = "converged" if stopping.converged else "max_rounds_reached"
-            provenance["amf_convergence"] = {
-                "policy": asdict(stopping.policy),
-                "converged": stopping.converged,
-                "checks": stopping.history,
+            receipt["status"] = "converged" if stopping.converged else "max_rounds_reached"
+            receipt["amf_convergence"] = {
+                "policy": asdict(stopping.policy), "converged": stopping.converged, "checks": stopping.history,
                 "uncertainty": "pointwise relative standard error",
-            }
+            }  # fmt: skip
 
         with self.receipt.open("x") as stream:
-            stream.write(json.dumps(provenance, indent=2, default=str) + "\n")
+            json.dump(receipt, stream, indent=2, default=str)
+            stream.write("\n")
 
         return result
+
+
+def _check_request(case, output: Path, convergence: Optional[AMFConvergence]) -> None:
+    """What can be refused before a backend touches the device."""
+
+    if convergence is not None and case.rounds < convergence.minimum_rounds:
+        raise ValueError(f"round cap {case.rounds} is below the policy's minimum of {convergence.minimum_rounds}")
+
+    for path in (output, output.with_suffix(".run.json")):
+        if path.exists():
+            raise FileExistsError(f"{path} exists; results are never overwritten")
 
 
 def simulate(
@@ -236,34 +270,31 @@ def simulate(
     *,
     convergence: Optional[AMFConvergence] = None,
 ) -> SimulationResult:
-    """Run equally sized rounds and save a complete NetCDF product plus receipt.
+    """Trace ``case.rounds`` equal rounds on ``backend`` and write the NetCDF product and its
+    ``.run.json`` receipt (the reference's ``simulate``, simulation.py:17-189).
 
-    With ``AMFConvergence``, ``case.rounds`` is the maximum round budget; the receipt
-    records ``converged`` or ``max_rounds_reached``.
+    With ``convergence``, ``case.rounds`` is the round cap and the run stops earlier once the
+    policy is met; the receipt's status says ``converged`` or ``max_rounds_reached``. The
+    backend is prepared here and closed on return or failure.
     """
 
-    if convergence is not None and case.rounds < convergence.minimum_rounds:
-        raise ValueError("round cap must be at least minimum_rounds")
-
     output = Path(output).resolve()
-
-    if output.exists() or output.with_suffix(".run.json").exists():
-        raise FileExistsError(f"Refusing to overwrite an existing result: {output}")
+    _check_request(case, output, convergence)
 
     with backend:
         backend.prepare(case)
         statistics = RoundStatistics(case, backend.metadata, output, convergence)
 
-        if statistics.stopping is not None:
-            batches = statistics.stopping.batches(backend, case)
+        if statistics.stopping is None:
+            rounds = backend.run_batches([BatchControls(index, case.photons_per_column)
+                                          for index in range(1, case.rounds + 1)])  # fmt: skip
+
+            if len(rounds) != case.rounds:
+                raise ValueError(f"backend returned {len(rounds)} rounds for {case.rounds} requested")
         else:
-            requests = [BatchControls(index, case.photons_per_column) for index in range(1, case.rounds + 1)]
-            batches = backend.run_batches(requests)
+            rounds = statistics.stopping.schedule(backend, case)
 
-            if len(batches) != case.rounds:
-                raise ValueError("Backend returned the wrong number of statistical rounds")
-
-        for batch in batches:
+        for batch in rounds:
             statistics.add(batch)
 
         return statistics.finish(backend.provenance)

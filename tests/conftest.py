@@ -7,7 +7,7 @@ import pytest
 
 ROOT = Path(__file__).resolve().parents[1]
 
-for path in (ROOT / "monki-gpu_amd", ROOT):
+for path in (ROOT / "monki-gpu_amd", ROOT, ROOT / "tests"):
     if str(path) not in sys.path:
         sys.path.insert(0, str(path))
 

@@ -339,7 +339,7 @@ def test_oracle_matches_reference_fortran_statistically(oracle, name):
 
     from monki import configs
     from monki.execution.hip_tables import pack_tables
-    from monki.validation import combined_se_statistics
+    from agreement import z_statistics
     from oracle.oracle import run_reference
 
     box = configs.box_cloud(nx=4, ny=3, nmat=3, box=(2, 1), offset=(1, 1))
@@ -367,7 +367,7 @@ def test_oracle_matches_reference_fortran_statistically(oracle, name):
     # One thread: the reference RNG sequence (and so the test) is then deterministic.
     theirs, _, _ = run_reference(scene, ppc, rounds, threads=1, backward=backward)
     a, b = ours / ppc, theirs / ppc
-    stats = combined_se_statistics(
+    stats = z_statistics(
         a.mean(axis=0), a.std(axis=0, ddof=1) / np.sqrt(rounds),
         b.mean(axis=0), b.std(axis=0, ddof=1) / np.sqrt(rounds), floor=1e-12,
     )  # fmt: skip
@@ -389,7 +389,7 @@ def test_oracle_decomposition_matches_reference_fortran(oracle, backward):
 
     from monki import configs
     from monki.execution.hip_tables import pack_tables
-    from monki.validation import combined_se_statistics
+    from agreement import z_statistics
     from oracle.oracle import fortran_scats_row, run_reference
 
     scene = replace(configs.box_cloud(nx=4, ny=3, nmat=3, box=(2, 1), offset=(1, 1)), solar_azimuth=30.0,
@@ -417,7 +417,7 @@ def test_oracle_decomposition_matches_reference_fortran(oracle, backward):
     a, b = ours / ppc, theirs / ppc
     # Floor: single-scattered U at nadir is zero up to the rounding of the float32 geometry
     # (1e-9 here against I ~ 0.1), which its near-zero round-to-round spread cannot absorb.
-    stats = combined_se_statistics(
+    stats = z_statistics(
         a.mean(axis=0), a.std(axis=0, ddof=1) / np.sqrt(rounds),
         b.mean(axis=0), b.std(axis=0, ddof=1) / np.sqrt(rounds), floor=1e-7,
     )  # fmt: skip

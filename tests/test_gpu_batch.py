@@ -153,7 +153,7 @@ def test_device_moments_equal_host_moments():
     """Sums and AMF moments accumulated on the device equal the host's over the same rounds
     (two sessions, same Philox streams: float32 bank sums differ only in atomic order)."""
 
-    from monki.convergence import AMFMoments
+    from monki.convergence import RatioMoments
     from monki.execution import HIPBatch
 
     batch_cases = cases(rounds=6)
@@ -163,7 +163,7 @@ def test_device_moments_equal_host_moments():
         host.prepare(batch_cases)
         device.prepare(batch_cases)
         sums = [None] * len(batch_cases)
-        moments = [AMFMoments() for _ in batch_cases]
+        moments = [RatioMoments() for _ in batch_cases]
 
         for index in range(1, rounds + 1):
             stokes = device.accumulate_round(index)
@@ -179,15 +179,16 @@ def test_device_moments_equal_host_moments():
             got = device.moments(number)
             assert got.rounds == rounds
             reference = moments[number]
+            reference.estimate(np.ones(got.path_mean.shape[2]))  # folds the buffered rounds in
             pairs = [
                 (got.paths, sums[number][0]), (got.weights, sums[number][1]),
-                (got.path_mean, reference.path_mean), (got.weight_mean, reference.weight_mean[:, :, 0, :]),
+                (got.path_mean, reference.mean_path), (got.weight_mean, reference.mean_weight[:, :, 0, :]),
             ]  # fmt: skip
             for a, b in pairs:
                 np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-4 * np.abs(b).max())
             # Second moments: differences of rounds, at the scale of the variance.
-            for a, b in ((got.path_m2, reference.path_m2), (got.covariance, reference.covariance),
-                         (got.weight_m2, reference.weight_m2[:, :, 0, :])):  # fmt: skip
+            for a, b in ((got.path_m2, reference.s_pp), (got.covariance, reference.s_pw),
+                         (got.weight_m2, reference.s_ww[:, :, 0, :])):  # fmt: skip
                 np.testing.assert_allclose(a, b, rtol=1e-2, atol=1e-3 * np.abs(b).max())
 
 

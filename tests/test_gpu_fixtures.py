@@ -33,16 +33,16 @@ def archived(stem: str):
 def test_fixture_matches_archived_product(tmp_path, name):
     from monki import ReferenceCase, simulate
     from monki.execution import HIP
-    from monki.validation import check_result, compare, select_views
+    from agreement import agree, invariant_failures, views
 
     input_file, photons, stem = CASES[name]
     case = ReferenceCase(name, input_file, photons, 32, view_zeniths=(0.0, 30.0, 60.0))
     result = simulate(case, HIP(save_path=name != "box-clouds"), tmp_path / f"{name}.nc")
-    assert check_result(result) == []
-    golden = select_views(archived(stem), case.view_zeniths)
-    comparison = compare(golden, result)
-    print(json.dumps(comparison.to_dict()))
-    assert comparison.passed, comparison.details
+    assert invariant_failures(result) == []
+    golden = views(archived(stem), case.view_zeniths)
+    comparison = agree(golden, result)
+    print(json.dumps(comparison.__dict__, default=str))
+    assert comparison.passed, comparison.problems
     receipt = json.loads((tmp_path / f"{name}.run.json").read_text())
     assert receipt["preparation"]["case"]["input_file"] == input_file
     assert receipt["status"] == "complete" and receipt["completed_rounds"] == 32

@@ -158,7 +158,7 @@ def test_guided_tracer_agrees_with_plain_tracer(tmp_path, guidance):
     from monki import AMFConvergence, SceneCase, simulate
     from monki.execution import HIP
     from monki.execution.proposal import ProposalSettings
-    from monki.validation import combined_se_statistics
+    from agreement import z_statistics
 
     scene = small_box()
     rounds, ppc = 48, 200
@@ -173,7 +173,7 @@ def test_guided_tracer_agrees_with_plain_tracer(tmp_path, guidance):
     for name in ("radiance", "AMF_layer"):
         a, se_a = plain.values[name], plain.values["std_" + name]
         b, se_b = guided.values[name], guided.values["std_" + name]
-        report[name] = combined_se_statistics(a, se_a, b, se_b, floor=1e-12)
+        report[name] = z_statistics(a, se_a, b, se_b, floor=1e-12)
 
     print(json.dumps(report))
     for name, stats in report.items():
@@ -189,7 +189,7 @@ def test_directed_tracer_agrees_with_plain_tracer(tmp_path, kappa):
 
     from monki import AMFConvergence, SceneCase, simulate
     from monki.execution import DirectedSettings, HIP
-    from monki.validation import combined_se_statistics
+    from agreement import z_statistics
 
     scene = small_box()
     rounds, ppc = 48, 200
@@ -205,7 +205,7 @@ def test_directed_tracer_agrees_with_plain_tracer(tmp_path, kappa):
     for name in ("radiance", "AMF_layer"):
         a, se_a = plain.values[name], plain.values["std_" + name]
         b, se_b = directed.values[name], directed.values["std_" + name]
-        report[name] = combined_se_statistics(a, se_a, b, se_b, floor=1e-12)
+        report[name] = z_statistics(a, se_a, b, se_b, floor=1e-12)
 
     print(json.dumps(report))
     for name, stats in report.items():

@@ -22,11 +22,11 @@ def compare_with_fortran(scene, ppc, rounds, seed, tmp_path, name, backward=Fals
 
     from monki import SceneCase, simulate
     from monki.execution import HIP
-    from monki.validation import check_result, combined_se_statistics
+    from agreement import invariant_failures, z_statistics
     from oracle.oracle import run_reference
 
     gpu = simulate(SceneCase(name, scene, ppc, rounds, backward=backward), HIP(seed=seed), tmp_path / f"{name}.nc")
-    assert check_result(gpu) == []
+    assert invariant_failures(gpu) == []
     # One thread: the reference's compiler RNG sequence is then deterministic.
     tallies, _, _ = run_reference(scene, ppc, rounds, threads=1, backward=backward)
     nmat, nx, ny, views = scene.nmat, scene.nx, scene.ny, scene.angles
@@ -37,7 +37,7 @@ def compare_with_fortran(scene, ppc, rounds, seed, tmp_path, name, backward=Fals
     se = stokes.std(axis=0, ddof=1) / np.sqrt(rounds)
     a = gpu.radiance[..., 0].transpose(3, 2, 1, 0)
     b = gpu.standard_error[..., 0].transpose(3, 2, 1, 0)
-    stats = combined_se_statistics(a, b, mean, se, floor=1e-12)
+    stats = z_statistics(a, b, mean, se, floor=1e-12)
     print(name, json.dumps(stats))
     assert stats["max_abs_z"] < 5.0
     assert 0.6 < stats["chi_square_per_dof"] < 1.5
@@ -94,7 +94,7 @@ def test_decomposition_matches_reference_fortran(tmp_path, backward):
     from monki import SceneCase, configs, simulate
     from monki.execution import HIP, BatchControls
     from monki.execution.hip_tables import pack_tables
-    from monki.validation import combined_se_statistics
+    from agreement import z_statistics
     from oracle.oracle import fortran_scats_row, run_reference
 
     scene = replace(configs.box_cloud(nx=8, ny=6, nmat=3, box=(3, 2), offset=(2, 1)), solar_azimuth=30.0,
@@ -112,7 +112,7 @@ def test_decomposition_matches_reference_fortran(tmp_path, backward):
     ours = np.stack([flat(b) for b in batches]) / ppc
     tallies, _, _ = run_reference(scene, ppc, rounds, threads=1, backward=backward, save_scats=True)
     theirs = np.stack([fortran_scats_row(row, shape) for row in tallies]) / ppc
-    stats = combined_se_statistics(
+    stats = z_statistics(
         ours.mean(axis=0), ours.std(axis=0, ddof=1) / np.sqrt(rounds),
         theirs.mean(axis=0), theirs.std(axis=0, ddof=1) / np.sqrt(rounds), floor=1e-7,
     )  # fmt: skip

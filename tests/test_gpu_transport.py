@@ -451,18 +451,18 @@ def test_batch_validation_precedes_dispatch():
 
 def test_simulate_matches_reference_fortran(tmp_path):
     """Plain HIP against the reference Fortran transport: |d| < 3 combined SE per pixel/view
-    (Bonferroni bound of compare()), chi-square of the normalized differences reported."""
+    (Bonferroni bound of agree()), chi-square of the normalized differences reported."""
 
     from monki import simulate
     from monki.execution import HIP
     from monki.metadata import scene_metadata
-    from monki.validation import check_result, combined_se_statistics, compare
+    from agreement import agree, invariant_failures, z_statistics
     from oracle.oracle import run_reference
 
     name = "box-cloud"
     rounds, ppc = 40, 100
     gpu = simulate(case(name, ppc, rounds), HIP(seed=881), tmp_path / "gpu.nc")
-    assert check_result(gpu) == []
+    assert invariant_failures(gpu) == []
     scene = SCENES[name]
     # One thread: the reference's compiler RNG sequence, and so this test, is deterministic.
     tallies, _, _ = run_reference(scene, ppc, rounds, threads=1)
@@ -473,7 +473,7 @@ def test_simulate_matches_reference_fortran(tmp_path):
     se = normalized.std(axis=0, ddof=1) / np.sqrt(rounds)
     a = gpu.radiance[..., 0].transpose(3, 2, 1, 0)
     b = gpu.standard_error[..., 0].transpose(3, 2, 1, 0)
-    stats = combined_se_statistics(a, b, mean, se, floor=1e-12)
+    stats = z_statistics(a, b, mean, se, floor=1e-12)
     print(json.dumps(stats))
     assert stats["max_abs_z"] < 5.0
     assert 0.7 < stats["chi_square_per_dof"] < 1.4

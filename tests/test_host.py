@@ -20,13 +20,13 @@ ROOT = Path(__file__).resolve().parents[1]
 
 
 def test_online_amf_error_matches_explicit_correlated_ratio():
-    from monki.convergence import AMFMoments
+    from monki.convergence import RatioMoments
     from monki.execution.tallies import BatchControls, RawTallies
 
     rng = np.random.default_rng(123)
     weights = rng.uniform(1, 4, (100, 1, 1, 2))
     paths = weights[:, :, :, None, :] * rng.uniform(1, 5, (100, 1, 1, 4, 2))
-    moments = AMFMoments()
+    moments = RatioMoments()
 
     for index in range(100):
         moments.add(RawTallies(BatchControls(index + 1, 100), np.zeros((3, 1, 1, 2)), paths[index], weights[index]))
@@ -61,7 +61,7 @@ class _Case:
 
 def test_late_outlier_resets_precision_streak():
     from monki import AMFConvergence
-    from monki.convergence import AMFStopping
+    from monki.convergence import PrecisionStopping
     from monki.execution.tallies import RawTallies
 
     class Synthetic:
@@ -71,8 +71,8 @@ def test_late_outlier_resets_precision_streak():
                 for r in requests
             ]
 
-    stopping = AMFStopping(AMFConvergence(minimum_rounds=32, check_every=16), np.array([1.0]))
-    assert len(list(stopping.batches(Synthetic(), _Case(64)))) == 64
+    stopping = PrecisionStopping(AMFConvergence(minimum_rounds=32, check_every=16), np.array([1.0]))
+    assert len(list(stopping.schedule(Synthetic(), _Case(64)))) == 64
     assert not stopping.converged
     assert stopping.history[0]["streak"] == 1
     assert stopping.history[1]["streak"] == 0
@@ -80,15 +80,15 @@ def test_late_outlier_resets_precision_streak():
 
 def test_zero_amf_cannot_satisfy_relative_precision():
     from monki import AMFConvergence
-    from monki.convergence import AMFStopping
+    from monki.convergence import PrecisionStopping
     from monki.execution.tallies import RawTallies
 
     class Synthetic:
         def run_batches(self, requests):
             return [RawTallies(r, np.ones((3, 1, 1, 1)), np.zeros((1, 1, 2, 1)), np.ones((1, 1, 1))) for r in requests]
 
-    stopping = AMFStopping(AMFConvergence(minimum_rounds=32, consecutive_checks=1), np.array([1.0]))
-    list(stopping.batches(Synthetic(), _Case(32)))
+    stopping = PrecisionStopping(AMFConvergence(minimum_rounds=32, consecutive_checks=1), np.array([1.0]))
+    list(stopping.schedule(Synthetic(), _Case(32)))
     assert not stopping.converged
     assert stopping.history[-1]["max_relative_se"] is None
 
@@ -413,33 +413,33 @@ def changed(result, name, value):
 
 
 def test_identical_products_pass_both_modes(result):
-    from monki.validation import check_result, compare
+    from agreement import agree, invariant_failures
 
-    assert check_result(result) == []
-    assert compare(result, result, mode="deterministic").passed
-    assert compare(result, result).passed
+    assert invariant_failures(result) == []
+    assert agree(result, result, statistical=False).passed
+    assert agree(result, result).passed
 
 
 def test_wrong_but_physical_radiance_fails(result):
-    from monki.validation import check_result, compare
+    from agreement import agree, invariant_failures
 
     candidate = changed(result, "radiance", result.radiance * 1.5)
-    assert check_result(candidate) == []
-    assert not compare(result, candidate).passed
+    assert invariant_failures(candidate) == []
+    assert not agree(result, candidate).passed
 
 
 def test_dop_and_units_mismatches_fail(result):
-    from monki.validation import check_result, compare
+    from agreement import agree, invariant_failures
 
-    assert "DOP is inconsistent with Stokes components" in check_result(changed(result, "DOP", result.values["DOP"] * 2))
+    assert "DOP does not follow from the Stokes vector" in invariant_failures(changed(result, "DOP", result.values["DOP"] * 2))
     other = replace(result, attributes={"radiance": {"units": "W"}})
-    assert not compare(result, other).passed
+    assert not agree(result, other).passed
 
 
 def test_combined_se_statistics_excludes_zero_uncertainty():
-    from monki.validation import combined_se_statistics
+    from agreement import z_statistics
 
-    stats = combined_se_statistics([1.0, 2.0, 0.0], [0.1, 0.1, 0.0], [1.1, 2.0, 0.0], [0.1, 0.1, 0.0])
+    stats = z_statistics([1.0, 2.0, 0.0], [0.1, 0.1, 0.0], [1.1, 2.0, 0.0], [0.1, 0.1, 0.0])
     assert stats["count"] == 2
     assert stats["max_abs_z"] == pytest.approx(0.1 / np.hypot(0.1, 0.1))
 

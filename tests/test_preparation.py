@@ -84,14 +84,14 @@ def test_preparation_equals_reference_fortran_bit_for_bit(name):
 @pytest.mark.parametrize("name", list(ARCHIVED))
 def test_preparation_reproduces_archived_product_inputs(name):
     """The original program's products carry the optical inputs it prepared; ours equal them
-    within the coordinate tolerance of the reference's compare() (validation.py:12, 1e-12)."""
+    within the coordinate tolerance of the reference's product comparison (validation.py:12, 1e-12)."""
 
     from monki import SimulationResult
     from monki.metadata import scene_metadata
-    from monki.validation import check_result
+    from agreement import invariant_failures
 
     archived = SimulationResult.load(GOLDEN / "archived" / ARCHIVED[name])
-    assert check_result(archived) == []
+    assert invariant_failures(archived) == []
     theta = tuple(archived.values["theta"])
     settings = {"input_file": f"testrun/MONKI_MLS_{name}_340_340_test.in", "surface_albedo": 0.05,
                 "solar_zenith": 45.0, "solar_azimuth": 0.0, "view_azimuth": 0.0, "view_zeniths": theta[:3]}  # fmt: skip

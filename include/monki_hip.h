@@ -195,6 +195,24 @@ int monki_hip_set_directed(void *handle, float alpha, float kappa, uint32_t spec
 int monki_hip_run_directed(void *handle, const uint32_t *controls, double *out, char *error,
                            size_t error_size);
 
+/* Steering options of the guided and detector-directed tracers (both single-scene and batch
+ * sessions; a batch session traces its rounds in the mode last set by monki_hip_attach_proposal
+ * or monki_hip_set_directed). No reference counterpart.
+ *   lobe_species: tabulated species whose bound CDF shapes lobes around the view directions
+ *     (scattering cosines drawn from that species' sampler, measured from a view), or -1 for
+ *     vMF lobes of the kappa of monki_hip_set_directed;
+ *   lobe_fraction: guided tracer only, the lobes' share of the proposal (the rest is the
+ *     network's mixture); the detector-directed tracer uses the lobes alone;
+ *   surface: nonzero steers Lambertian reflections too (density mu / pi against the lobes);
+ *   split_above, roulette_below, roulette_weight: the weight window. 0 turns it off; else a
+ *     photon heavier than split_above after a scattering continues as up to 4 copies of equal
+ *     weight, and one lighter than roulette_below before a scattering survives with
+ *     probability weight / roulette_weight at that weight. Unbiased; bounds the products of
+ *     the steering factors.
+ * Returns 0, or nonzero with error text for invalid values. */
+int monki_hip_set_steering(void *handle, int lobe_species, float lobe_fraction, int surface, float split_above,
+                           float roulette_below, float roulette_weight, char *error, size_t error_size);
+
 /* monki_hip_trace for the guided tracer (float32), same record layout. */
 int monki_hip_trace_guided(void *handle, const uint32_t *controls, double *out, float *records,
                            size_t record_count, char *error, size_t error_size);

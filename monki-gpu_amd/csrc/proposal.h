@@ -227,7 +227,23 @@ __device__ inline float mixture_density(const Mixture &m, float wx, float wy, fl
   return q;
 }
 
-// Sample one direction of the mixture from three uniforms (Wood 1994 for the vMF cosine).
+// The direction at cosine w (sine s) from the unit axis (mx, my, mz) and azimuth phi about it,
+// in an orthonormal frame around the axis (Frisvad's construction, branch-free in z sign).
+__device__ inline void frame_direction(float mx, float my, float mz, float w, float s, float phi, float &wx,
+                                       float &wy, float &wz) {
+  const float sign = mz >= 0.0f ? 1.0f : -1.0f;
+  const float a = -1.0f / (sign + mz);
+  const float b = mx * my * a;
+  const float e1x = 1.0f + sign * mx * mx * a, e1y = sign * b, e1z = -sign * mx;
+  const float e2x = b, e2y = sign + my * my * a, e2z = -my;
+  const float c = cosf(phi) * s, d = sinf(phi) * s;
+  wx = w * mx + c * e1x + d * e2x;
+  wy = w * my + c * e1y + d * e2y;
+  wz = w * mz + c * e1z + d * e2z;
+  const float norm = rsqrtf(wx * wx + wy * wy + wz * wz);
+  wx *= norm, wy *= norm, wz *= norm;
+}
+
 // A direction of the vMF distribution of unit mean (mx, my, mz) and concentration kappa from
 // two uniforms (Wood 1994 for the cosine to the mean).
 __device__ inline void vmf_sample(float mx, float my, float mz, float kappa, float u1, float u2, float &wx,
@@ -235,19 +251,7 @@ __device__ inline void vmf_sample(float mx, float my, float mz, float kappa, flo
   // Cosine to the mean: w = 1 + log(u + (1 - u) exp(-2 kappa)) / kappa, in [-1, 1].
   const float w = fminf(1.0f, fmaxf(-1.0f, 1.0f + logf(u1 + (1.0f - u1) * expf(-2.0f * kappa)) / kappa));
   const float s = sqrtf(fmaxf(0.0f, 1.0f - w * w));
-  // Orthonormal frame around the mean (Frisvad's construction, branch-free in z sign).
-  const float sign = mz >= 0.0f ? 1.0f : -1.0f;
-  const float a = -1.0f / (sign + mz);
-  const float b = mx * my * a;
-  const float e1x = 1.0f + sign * mx * mx * a, e1y = sign * b, e1z = -sign * mx;
-  const float e2x = b, e2y = sign + my * my * a, e2z = -my;
-  const float phi = 6.28318530717958647692f * u2;
-  const float c = cosf(phi) * s, d = sinf(phi) * s;
-  wx = w * mx + c * e1x + d * e2x;
-  wy = w * my + c * e1y + d * e2y;
-  wz = w * mz + c * e1z + d * e2z;
-  const float norm = rsqrtf(wx * wx + wy * wy + wz * wz);
-  wx *= norm, wy *= norm, wz *= norm;
+  frame_direction(mx, my, mz, w, s, 6.28318530717958647692f * u2, wx, wy, wz);
 }
 
 __device__ inline void mixture_sample(const Mixture &m, float u0, float u1, float u2, float &wx,

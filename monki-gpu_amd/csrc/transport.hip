@@ -71,14 +71,33 @@ constexpr bool decomposition(int mode) {
   return mode == kScats || mode == kScats4 || mode == kBackScats || mode == kBackScats4;
 }
 
-// Which events the guided mode guides: scatterer kinds (bit 0 gas, bit s + 1 tabulated species
-// s) up to a scattering order; alpha = probability of the phase-function component.
+// How the steered modes (guided, detector-directed) sample scattering directions. A steered
+// event draws from the physical sampler (phase function, or Lambert at the surface) with
+// probability alpha and from the proposal q otherwise, and multiplies the weight by
+// p / (alpha p + (1 - alpha) q). Atmospheric events of the scatterer kinds in species_mask
+// (bit 0 gas, bit s + 1 tabulated species s) up to scattering order max_order are steered, and
+// surface reflections when `surface` is set. The proposal holds lobes around the view
+// directions shaped by the bound CDF of tabulated species lobe_species (vMF lobes of
+// concentration kappa when it is negative); in the guided mode it is the mixture
+// lobe_fraction x lobes + (1 - lobe_fraction) x the network's vMF mixture.
+// Weight window (split_above > 0): a photon whose weight exceeds split_above after a
+// scattering continues as up to kMaxCopies copies of equal weight (the others resume later on
+// random streams of their own); one whose weight is below roulette_below before a scattering
+// survives with probability weight / roulette_weight at weight roulette_weight. Both keep the
+// estimator unbiased and bound the products of the steering factors.
 struct Guidance {
   float alpha;
   uint32_t species_mask;
   uint32_t max_order;
-  float kappa;  // concentration of the detector-directed lobes
+  float kappa;
+  int lobe_species;
+  float lobe_fraction;
+  int surface;
+  float split_above, roulette_below, roulette_weight;
 };
+
+// Unsteered runs.
+__host__ __device__ constexpr Guidance unsteered() { return Guidance{1.0f, 0u, 0u, 0.0f, -1, 0.0f, 0, 0.0f, 0.0f, 0.0f}; }
 
 struct SpeciesTables {
   int lut_size;
@@ -988,63 +1007,140 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
   return kRunning;
 }
 
-// Proposals of the steered modes: the learned network's vMF mixture (per event), or lobes of
-// one concentration around the view directions, equally weighted (per scene): detector-
-// directed sampling, which sends photons toward the detectors more often so that the forward
-// peak of a tabulated phase function meets the local estimate with a matching, smaller weight.
-struct Lobes {
+// Proposals of the steered modes.
+//
+// ViewLobes: lobes around the view directions shaped by a phase function. A lobe picks a view
+// uniformly, draws a scattering cosine from the bound CDF of tabulated species `species` (the
+// plain tracer's sampler, 1 - cos interpolated from the 1 - cos nodes so that the forward peak
+// keeps its resolution) and measures it from the view direction, with a uniform azimuth about
+// it. The CDF interpolates linearly between nodes, so the cosine density is constant on each
+// node interval, and the direction density is
+//   q(w) = (1 / V) sum_v g(w . view_v) / (2 pi),  g = (cdf[i+1] - cdf[i]) / (cos[i+1] - cos[i])
+// on the interval i holding w . view_v. At the next event the local estimate toward view v
+// evaluates the same phase function at the same cosine, which exceeds g by at most the table's
+// node-to-interval-mean ratio; a photon steered into the forward peak toward a view therefore
+// carries that peak's inverse in its weight factor p / (alpha p + (1 - alpha) q), and the
+// forward-peak spikes of the local estimate (rare, huge contributions) become frequent, small
+// ones. Without a tabulated species: vMF lobes of concentration kappa.
+struct ViewLobes {
   const Scene *s;
+  int species;
   float kappa;
 };
 
-__device__ inline void steer_sample(const monki_net::Mixture &m, float u0, float u1, float u2, float &wx,
-                                    float &wy, float &wz) {
-  monki_net::mixture_sample(m, u0, u1, u2, wx, wy, wz);
-}
-
-__device__ inline float steer_density(const monki_net::Mixture &m, float wx, float wy, float wz) {
-  return monki_net::mixture_density(m, wx, wy, wz);
-}
-
-__device__ inline void steer_sample(const Lobes &l, float u0, float u1, float u2, float &wx, float &wy,
+__device__ inline void steer_sample(const ViewLobes &l, float u0, float u1, float u2, float &wx, float &wy,
                                     float &wz) {
   const Scene &s = *l.s;
   const int v = min(int(u0 * float(s.views)), s.views - 1);
-  monki_net::vmf_sample(float(s.view_a[v]), float(s.view_b[v]), float(s.view_mu[v]), l.kappa, u1, u2, wx, wy, wz);
+  const float vx = float(s.view_a[v]), vy = float(s.view_b[v]), vz = float(s.view_mu[v]);
+  if (l.species < 0) {
+    monki_net::vmf_sample(vx, vy, vz, l.kappa, u1, u2, wx, wy, wz);
+    return;
+  }
+  const SpeciesTables &t = s.sp[l.species];
+  uint32_t reads = 0;
+  const int i = t.cdf_start ? cdf_interval(t.cdf, t.cdf_start, t.lut_size, u1, reads)
+                            : bracket(t.cdf, t.lut_size, u1, reads);
+  const float d0 = t.cdf[i], d1 = t.cdf[i + 1];
+  const float w = d1 > d0 ? (u1 - d0) / (d1 - d0) : 0.5f;
+  const float *omc = t.cosine + t.lut_size;
+  const float one_minus = omc[i] + w * (omc[i + 1] - omc[i]);
+  const float cosine = t.cosine[i] + w * (t.cosine[i + 1] - t.cosine[i]);
+  const float sine = sqrtf(fmaxf(0.0f, one_minus * (2.0f - one_minus)));
+  monki_net::frame_direction(vx, vy, vz, cosine, sine, 6.28318530717958647692f * u2, wx, wy, wz);
 }
 
-__device__ inline float steer_density(const Lobes &l, float wx, float wy, float wz) {
+__device__ inline float steer_density(const ViewLobes &l, float wx, float wy, float wz) {
   const Scene &s = *l.s;
   float q = 0.0f;
   for (int v = 0; v < s.views; ++v) {
-    const float dot = float(s.view_a[v]) * wx + float(s.view_b[v]) * wy + float(s.view_mu[v]) * wz;
-    q += expf(monki_net::log_vmf(l.kappa, dot));
+    const float dot = fminf(1.0f, fmaxf(-1.0f, float(s.view_a[v]) * wx + float(s.view_b[v]) * wy +
+                                                   float(s.view_mu[v]) * wz));
+    if (l.species < 0) {
+      q += expf(monki_net::log_vmf(l.kappa, dot));
+    } else {
+      const SpeciesTables &t = s.sp[l.species];
+      uint32_t reads = 0;
+      const int i = t.equal_angle ? bracket_angle<float>(t.cosine, t.lut_size, dot, reads)
+                                  : bracket_r<float>(t.cosine, t.lut_size, dot, reads);
+      q += (t.cdf[i + 1] - t.cdf[i]) / (t.cosine[i + 1] - t.cosine[i]) * 0.15915494309189533577f;
+    }
   }
   return q / float(s.views);
 }
 
+// The guided proposal: lobe_fraction x ViewLobes + (1 - lobe_fraction) x the network's mixture
+// for this event.
+struct GuidedProposal {
+  const monki_net::Mixture *net;
+  ViewLobes lobes;
+  float lobe_fraction;
+};
+
+__device__ inline void steer_sample(const GuidedProposal &g, float u0, float u1, float u2, float &wx, float &wy,
+                                    float &wz) {
+  // u0 picks the part and is then reused, rescaled, inside it.
+  if (u0 < g.lobe_fraction) {
+    steer_sample(g.lobes, u0 / g.lobe_fraction, u1, u2, wx, wy, wz);
+  } else {
+    const float rest = (u0 - g.lobe_fraction) / (1.0f - g.lobe_fraction);
+    monki_net::mixture_sample(*g.net, fminf(rest, 0.99999994f), u1, u2, wx, wy, wz);
+  }
+}
+
+__device__ inline float steer_density(const GuidedProposal &g, float wx, float wy, float wz) {
+  const float lobes = g.lobe_fraction > 0.0f ? steer_density(g.lobes, wx, wy, wz) : 0.0f;
+  const float net = g.lobe_fraction < 1.0f ? monki_net::mixture_density(*g.net, wx, wy, wz) : 0.0f;
+  return g.lobe_fraction * lobes + (1.0f - g.lobe_fraction) * net;
+}
+
 // Continuation direction after the event (mc.f90:1266-1431; GROUND_LAMB2 at the surface).
-// Guided (atmospheric events): with probability alpha the direction comes from the phase
-// function (exact sampler), otherwise from the learned vMF mixture; either way the weight is
-// multiplied by p / (alpha p + (1 - alpha) q), p the phase-function density of the direction
-// for the photon's Stokes vector and q the mixture's, which keeps the estimator unbiased and
-// the factor below 1 / alpha.
-// Events outside the guidance (kind or order) are sampled exactly as in the plain mode.
-template <typename R, bool TRACE, int MODE = kPlain, typename Proposal = monki_net::Mixture>
+// Steered events (guided and detector-directed modes; see Guidance) draw from the physical
+// sampler with probability alpha, otherwise from the proposal, and multiply the weight by
+// p / (alpha p + (1 - alpha) q): p the density of the physical sampler for the photon's Stokes
+// vector (phase function; Lambert mu / pi at the surface), q the proposal's. The factor is at
+// most 1 / alpha. Atmospheric events use `proposal`; surface reflections the view lobes.
+// Events outside the steering (kind, order, surface) are sampled exactly as in the plain mode.
+template <typename R, bool TRACE, int MODE = kPlain, typename Proposal = ViewLobes>
 __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, Counts &n,
-                              Trace<R> &trace, const Proposal *mixture = nullptr,
-                              Guidance guidance = Guidance{1.0f, 0u, 0u, 0.0f}) {
+                              Trace<R> &trace, const Proposal *proposal = nullptr,
+                              const ViewLobes *lobes = nullptr, Guidance guidance = unsteered()) {
   constexpr bool V4 = full_stokes(MODE);
   constexpr bool BACK = backward(MODE);
+  constexpr bool STEER = MODE == kGuided || MODE == kDirected;
   const float alpha = guidance.alpha;
   const R pi = R(3.14159265358979323846);
   Photon<R> &p = h.p;
   R mu, phi;
   R da = R(0), db = R(0), dc = R(0);
-  bool direct = false;  // direction cosines given (guided vMF branch)
+  bool direct = false;  // direction cosines given (steered proposal draws)
   if (h.surface) {
-    mu = sqrt(R(uniform(h.rng)));
-    phi = R(2) * pi * R(uniform(h.rng));
+    const bool steer = STEER && guidance.surface;
+    const bool lambert = !steer || uniform(h.rng) < alpha;
+    if (lambert) {
+      mu = sqrt(R(uniform(h.rng)));
+      phi = R(2) * pi * R(uniform(h.rng));
+    } else {
+      float wx, wy, wz;
+      const float u0 = uniform(h.rng), u1 = uniform(h.rng), u2 = uniform(h.rng);
+      steer_sample(*lobes, u0, u1, u2, wx, wy, wz);
+      da = R(wx), db = R(wy), dc = R(wz);
+      direct = true;
+      mu = dc;
+      phi = atan2(db, da);
+    }
+    if (steer) {
+      if (lambert) {
+        const R sine = sqrt(fmax(R(0), R(1) - mu * mu));
+        da = cos(phi) * sine, db = sin(phi) * sine, dc = mu;
+      }
+      // Lambertian density over the upper hemisphere; a proposal draw into the ground ends the
+      // history (p = 0 there).
+      const R lambert_density = mu > R(0) ? mu / pi : R(0);
+      const R q = R(steer_density(*lobes, float(da), float(db), float(dc)));
+      h.weight *= lambert_density / (R(alpha) * lambert_density + R(1.0f - alpha) * q);
+      if (!(h.weight > R(0))) return kAbsorbed;
+    }
     h.st = Stokes<R>{R(1), R(0), R(0)};
     if (BACK) {
       // W As keeps the first column of W (mc.f90:894-903; W11 is one here).
@@ -1056,8 +1152,7 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
     R cosine, one_minus, one_plus;
     int interval = -1;  // table interval of the sampled cosine (phase-function draws)
     bool from_phase = true;
-    const bool guide = (MODE == kGuided || MODE == kDirected) &&
-                       ((guidance.species_mask >> (h.species + 1)) & 1u) && h.order <= guidance.max_order;
+    const bool guide = STEER && ((guidance.species_mask >> (h.species + 1)) & 1u) && h.order <= guidance.max_order;
     if (guide) from_phase = uniform(h.rng) < alpha;
     if (from_phase) {
       R dphi;
@@ -1078,7 +1173,7 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
     } else {
       float wx, wy, wz;
       const float u0 = uniform(h.rng), u1 = uniform(h.rng), u2 = uniform(h.rng);
-      steer_sample(*mixture, u0, u1, u2, wx, wy, wz);
+      steer_sample(*proposal, u0, u1, u2, wx, wy, wz);
       da = R(wx), db = R(wy), dc = R(wz);
       direct = true;
       mu = dc;
@@ -1103,7 +1198,7 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
         }
         const float table_norm = h.species < 0 ? 1.0f : s.sp[h.species].norm;
         const R density = norm / (pi * R(table_norm));
-        const R q = R(steer_density(*mixture, float(da), float(db), float(dc)));
+        const R q = R(steer_density(*proposal, float(da), float(db), float(dc)));
         h.weight *= density / (R(alpha) * density + R(1.0f - alpha) * q);
       }
       if (s.nmat > 1) {
@@ -1323,6 +1418,72 @@ __device__ __attribute__((noinline)) void guided_mixture(const monki_net::Net &n
   if (guided_event) mixture = monki_net::mixture_from(o);
 }
 
+// Weight-window copies (steered modes). Each lane keeps a stack of kSplitDepth entries in global
+// memory; an entry holds the photon state right after a scattering (position, direction,
+// Stokes vector, weight of one copy, order, counters) and the photon's layer paths, plus the
+// number of copies still to run. A lane whose history ends resumes the top entry's next copy
+// on a random stream of its own (stream = number of copies started in this photon's tree, so
+// histories stay independent of lanes and timing) before it takes a new photon. Copies of one
+// photon tally like the photon itself. At most kTreeCopies copies start per photon, and the
+// history step limit counts from the photon's launch, so every tree ends.
+constexpr int kSplitDepth = 8;
+constexpr int kSplitWords = 20;  // state words before the layer paths
+constexpr uint32_t kMaxCopies = 4;
+constexpr uint32_t kTreeCopies = 4096;
+
+template <typename R>
+__device__ inline void push_copies(uint32_t *entry, const History<R> &h, const R *history, int nz, uint32_t copies) {
+  const Photon<R> &p = h.p;
+  entry[0] = uint32_t(p.i);
+  entry[1] = uint32_t(p.j);
+  entry[2] = uint32_t(p.k);
+  entry[3] = __float_as_uint(float(p.rx));
+  entry[4] = __float_as_uint(float(p.ry));
+  entry[5] = __float_as_uint(float(p.rz));
+  entry[6] = __float_as_uint(float(p.a));
+  entry[7] = __float_as_uint(float(p.b));
+  entry[8] = __float_as_uint(float(p.c));
+  entry[9] = __float_as_uint(float(p.phi));
+  entry[10] = __float_as_uint(float(h.st.i));
+  entry[11] = __float_as_uint(float(h.st.q));
+  entry[12] = __float_as_uint(float(h.st.u));
+  entry[13] = __float_as_uint(float(h.weight));
+  entry[14] = h.order;
+  entry[15] = h.round_offset;
+  entry[16] = h.steps;
+  entry[17] = __float_as_uint(float(h.path_total));
+  entry[18] = __float_as_uint(h.contribution);
+  entry[19] = copies;
+  for (int k = 0; k < nz; ++k) entry[kSplitWords + k] = __float_as_uint(float(history[k]));
+}
+
+// Restore the next copy of `entry` into h and the lane's layer paths; returns the copies left.
+template <typename R>
+__device__ inline uint32_t pop_copy(uint32_t *entry, History<R> &h, R *history, int nz) {
+  Photon<R> &p = h.p;
+  p.i = int(entry[0]);
+  p.j = int(entry[1]);
+  p.k = int(entry[2]);
+  p.rx = R(__uint_as_float(entry[3]));
+  p.ry = R(__uint_as_float(entry[4]));
+  p.rz = R(__uint_as_float(entry[5]));
+  p.a = R(__uint_as_float(entry[6]));
+  p.b = R(__uint_as_float(entry[7]));
+  p.c = R(__uint_as_float(entry[8]));
+  p.phi = R(__uint_as_float(entry[9]));
+  h.st = Stokes<R>{R(__uint_as_float(entry[10])), R(__uint_as_float(entry[11])), R(__uint_as_float(entry[12])), R(0)};
+  h.weight = R(__uint_as_float(entry[13]));
+  h.order = entry[14];
+  h.round_offset = entry[15];
+  h.steps = entry[16];
+  h.path_total = R(__uint_as_float(entry[17]));
+  h.contribution = __uint_as_float(entry[18]);
+  for (int k = 0; k < nz; ++k) history[k] = R(__uint_as_float(entry[kSplitWords + k]));
+  const uint32_t left = entry[19] - 1u;
+  entry[19] = left;
+  return left;
+}
+
 // Persistent transport. Each iteration, every lane advances its photon to the next event and
 // evaluates the local estimates (divergent work), the wave then tallies all pending events
 // cooperatively, and the lanes sample their continuation directions. A lane whose history
@@ -1340,11 +1501,20 @@ __global__ void __launch_bounds__(kBlock32)
     transport_kernel(const Scene *__restrict__ scenes, const Controls *__restrict__ controls,
                      uint32_t scene_count, int row, R *banks, unsigned *failure, unsigned *work,
                      unsigned long long *counters, float *records, uint32_t record_count,
-                     monki_net::Net net, monki_net::EventLog log, Guidance guidance) {
+                     monki_net::Net net, monki_net::EventLog log, Guidance guidance, uint32_t *splits,
+                     int split_stride) {
   constexpr bool V4 = full_stokes(MODE);
   constexpr bool BACK = backward(MODE);
   constexpr bool SCATS = decomposition(MODE);
   constexpr bool DOWN = downward(MODE);
+  constexpr bool STEER = MODE == kGuided || MODE == kDirected;
+  // This lane's weight-window stack (steered modes), its entries in use, and the copies started
+  // in the current photon's tree.
+  uint32_t *stack = STEER && splits ? splits + size_t(blockIdx.x * blockDim.x + threadIdx.x) * kSplitDepth * split_stride
+                                    : nullptr;
+  int top = 0;
+  uint32_t tree = 0;
+  const bool window = STEER && stack && guidance.split_above > 0.0f;
   extern __shared__ unsigned char lds_raw[];
   // Layer rows of odd length (the largest nz of the scenes, | 1): lanes reading one row, or
   // each its own row at one layer, touch distinct LDS banks.
@@ -1407,6 +1577,7 @@ __global__ void __launch_bounds__(kBlock32)
           trace = Trace<R>{R(0), R(0), R(0), R(1), R(0), R(0), R(0), R(0)};
           ++n.photons;
           active = true;
+          tree = 0;
         }
       }
     }
@@ -1430,6 +1601,14 @@ __global__ void __launch_bounds__(kBlock32)
       tally_wave<R, V4, DOWN>(s, banks, shard, active && status == kRunning, wave_pending, wave_history, row, n);
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
     __builtin_amdgcn_wave_barrier();
+    // Russian roulette of the weight window, after the event's local estimates (a photon steered
+    // toward a view has used its small weight there) and before its scattering.
+    if (window && active && status == kRunning && h.weight < R(guidance.roulette_below)) {
+      if (R(uniform(h.rng)) * R(guidance.roulette_weight) < h.weight)
+        h.weight = R(guidance.roulette_weight);
+      else
+        status = kAbsorbed;
+    }
     const bool scatters = active && status == kRunning && !h.surface;
     float features[monki_net::kInputs];
     monki_net::Mixture mixture;
@@ -1442,7 +1621,8 @@ __global__ void __launch_bounds__(kBlock32)
     // The network runs when some lane's event is guided (kind and order within the guidance).
     const bool guided_event = MODE == kGuided && scatters &&
                               ((guidance.species_mask >> (h.species + 1)) & 1u) && h.order <= guidance.max_order;
-    const unsigned long long guided_lanes = MODE == kGuided ? __ballot(guided_event) : 0ull;
+    const unsigned long long guided_lanes =
+        MODE == kGuided && guidance.lobe_fraction < 1.0f ? __ballot(guided_event) : 0ull;
     if (MODE == kGuided && guided_lanes) guided_mixture(net, features, guided_lanes, guided_event, mixture);
     if (MODE == kLog) {
       const unsigned long long logging = __ballot(scatters);
@@ -1455,11 +1635,24 @@ __global__ void __launch_bounds__(kBlock32)
       }
     }
     prefetch(h.rng);
+    const ViewLobes lobes{&s, guidance.lobe_species, guidance.kappa};
     if (MODE == kDirected) {
-      const Lobes lobes{&s, guidance.kappa};
-      if (active && status == kRunning) status = scatter_photon<R, TRACE, MODE>(s, failure, h, n, trace, &lobes, guidance);
+      if (active && status == kRunning)
+        status = scatter_photon<R, TRACE, MODE>(s, failure, h, n, trace, &lobes, &lobes, guidance);
+    } else if (MODE == kGuided) {
+      const GuidedProposal guided{&mixture, lobes, guidance.lobe_fraction};
+      if (active && status == kRunning)
+        status = scatter_photon<R, TRACE, MODE>(s, failure, h, n, trace, &guided, &lobes, guidance);
     } else if (active && status == kRunning) {
-      status = scatter_photon<R, TRACE, MODE>(s, failure, h, n, trace, &mixture, guidance);
+      status = scatter_photon<R, TRACE, MODE>(s, failure, h, n, trace, &lobes, &lobes, guidance);
+    }
+    // Splitting of the weight window: the photon continues as `copies` copies of equal weight.
+    if (window && active && status == kRunning && h.weight > R(guidance.split_above) && top < kSplitDepth &&
+        tree < kTreeCopies) {
+      const uint32_t copies = min(uint32_t(ceil(float(h.weight))), kMaxCopies);
+      h.weight /= R(copies);
+      push_copies<R>(stack + size_t(top) * split_stride, h, history, s.nz, copies - 1u);
+      ++top;
     }
     if (MODE == kLog && slot < log.capacity && status == kRunning) {
 #pragma unroll
@@ -1471,6 +1664,14 @@ __global__ void __launch_bounds__(kBlock32)
       log.photon[slot] = id;
     }
     if (!active || status == kRunning) continue;
+    if (STEER && top > 0) {
+      // The next copy of the photon resumes right after the scattering it was split at.
+      if (pop_copy<R>(stack + size_t(top - 1) * split_stride, h, history, s.nz) == 0u) --top;
+      ++tree;
+      h.rng = Stream{h.rng.photon, h.rng.round, h.rng.seed, tree, 0u, Words{0, 0, 0, 0}, Words{0, 0, 0, 0}, 4, false};
+      h.remains = -::log(R(uniform(h.rng)));
+      continue;
+    }
     active = false;
     if (MODE == kLog && id < log.photons) log.totals[id] = h.contribution;
     if (TRACE && id < record_count) {
@@ -1679,7 +1880,10 @@ struct Session {
   double gpu_seconds = 0.0;
   double kernel_seconds = 0.0;
   void *proposal = nullptr;  // attached learned proposal (guided runs)
-  Guidance guidance{0.25f, ~0u, ~0u, 0.0f};  // steered runs: defensive fraction, kinds, orders, lobe kappa
+  Guidance guidance = unsteered();  // steered runs
+  int steer = kPlain;  // batch rounds: kPlain, kDirected or kGuided
+  uint32_t *splits = nullptr;  // weight-window copies of the steered modes, per lane
+  size_t split_words = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   // Rounds accumulated on the device (batch sessions with layer paths): per-scene sums and
@@ -1714,7 +1918,7 @@ void release(Session *s) {
   if (s->stream) (void)hipStreamDestroy(s->stream);
   // s->work points into the failure allocation.
   void *buffers[] = {s->tables, s->buckets, s->banks, s->reduced, s->assembled, s->failure, s->counters, s->records,
-                     s->d_scenes, s->d_controls, s->moments, s->moment_counts};
+                     s->d_scenes, s->d_controls, s->moments, s->moment_counts, s->splits};
   for (void *b : buffers) (void)hipFree(b);
   delete s;
 }
@@ -1924,7 +2128,8 @@ bool enqueue(Session *h, uint32_t rounds, uint32_t grid, int block, size_t lds, 
     hipLaunchKernelGGL((transport_kernel<R, TRACE, MODE>), dim3(grid), dim3(block), lds, h->stream,
                        h->d_scenes, h->d_controls, scene_count, h->row, banks, h->failure, h->work,
                        h->counters, TRACE ? h->records : nullptr, uint32_t(TRACE ? record_count : 0), net, log,
-                       h->guidance);
+                       MODE == kGuided || MODE == kDirected ? h->guidance : unsteered(), h->splits,
+                       kSplitWords + h->row);
   }
   if (!check(hipGetLastError(), error, error_size, "transport launch")) return false;
   for (uint32_t i = 0; i < scene_count; ++i) {
@@ -1957,11 +2162,16 @@ int block_size(const Session *h) {
 }
 
 template <typename R>
-size_t lds_bytes(const Session *h, int block, bool v4 = false) {
+size_t lds_bytes_for(int row, int block, bool v4) {
   // History rows and pending rows per lane; pending words; per-wave layer tables (4 rows of
   // floats per wave) except with the full Stokes vector.
-  return (size_t(h->row) + pending_reals(v4)) * block * sizeof(R) + size_t(kPendingWords) * block * sizeof(int) +
-         (v4 ? 0 : size_t(block / 64) * 4 * h->row * sizeof(float));
+  return (size_t(row) + pending_reals(v4)) * block * sizeof(R) + size_t(kPendingWords) * block * sizeof(int) +
+         (v4 ? 0 : size_t(block / 64) * 4 * row * sizeof(float));
+}
+
+template <typename R>
+size_t lds_bytes(const Session *h, int block, bool v4 = false) {
+  return lds_bytes_for<R>(h->row, block, v4);
 }
 
 bool finish(Session *h, char *error, size_t error_size) {
@@ -1975,6 +2185,19 @@ bool finish(Session *h, char *error, size_t error_size) {
       snprintf(error, error_size, "HIP transport failed (flag %u); round rejected", flag);
     return false;
   }
+  return true;
+}
+
+// Room for the weight-window stacks of every lane of a steered launch (kept for later launches).
+bool reserve_splits(Session *h, uint32_t grid, int block, char *error, size_t error_size) {
+  const size_t words = size_t(grid) * block * kSplitDepth * (kSplitWords + h->row);
+  if (words <= h->split_words) return true;
+  (void)hipFree(h->splits);
+  h->splits = nullptr;
+  h->split_words = 0;
+  if (!check(hipMalloc(&h->splits, words * sizeof(uint32_t)), error, error_size, "allocate weight-window stacks"))
+    return false;
+  h->split_words = words;
   return true;
 }
 
@@ -2061,6 +2284,7 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
   const int block = block_size<R>(h);
   const size_t lds = lds_bytes<R>(h, block, full_stokes(MODE));
   const uint32_t grid = persistent_grid<R, TRACE, MODE>(h, c.count, block, lds);
+  if ((MODE == kGuided || MODE == kDirected) && !reserve_splits(h, grid, block, error, error_size)) return 1;
   (void)hipEventRecord(h->start, h->stream);
   if (!enqueue<R, TRACE, MODE>(h, c.rounds, grid, block, lds, record_count, net, log, error, error_size)) return 1;
   (void)hipEventRecord(h->stop, h->stream);
@@ -2307,13 +2531,39 @@ int batch_round(Session *h, uint32_t round, uint32_t seed, uint32_t max_steps, c
   if (!check(hipMemcpy(h->d_controls, controls.data(), scene_count * sizeof(Controls), hipMemcpyHostToDevice),
              error, error_size, "upload controls"))
     return 1;
-  const monki_net::Net net{nullptr, nullptr, nullptr};
-  const monki_net::EventLog log{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0};
   const int block = block_size<float>(h);
   const size_t lds = lds_bytes<float>(h, block);
-  const uint32_t grid = total ? persistent_grid<float, false, kPlain>(h, total, block, lds) : 0u;
+  if (h->steer == kGuided && (!h->proposal || monki_net::proposal_device(h->proposal) != h->device)) {
+    set_error(error, error_size, "A proposal on the session's device is required");
+    return 1;
+  }
+  const uint32_t grid =
+      !total                   ? 0u
+      : h->steer == kGuided   ? persistent_grid<float, false, kGuided>(h, total, block, lds)
+      : h->steer == kDirected ? persistent_grid<float, false, kDirected>(h, total, block, lds)
+                              : persistent_grid<float, false, kPlain>(h, total, block, lds);
+  // Stacks for a full-occupancy grid: a captured round keeps their address.
+  if (h->steer != kPlain &&
+      !reserve_splits(h,
+                      h->steer == kGuided ? persistent_grid<float, false, kGuided>(h, uint64_t(1) << 40, block, lds)
+                                          : persistent_grid<float, false, kDirected>(h, uint64_t(1) << 40, block, lds),
+                      block, error, error_size))
+    return 1;
   hipGraph_t &graph = accumulate ? h->accumulate_graph : h->graph;
   hipGraphExec_t &exec = accumulate ? h->accumulate_exec : h->graph_exec;
+  // The round's device work: clears, transport in the session's sampling mode, reductions.
+  auto queue = [&]() {
+    const monki_net::Net net = h->steer == kGuided ? monki_net::proposal_net(h->proposal)
+                                                   : monki_net::Net{nullptr, nullptr, nullptr};
+    const monki_net::EventLog log{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0};
+    const bool queued =
+        h->steer == kGuided   ? enqueue<float, false, kGuided>(h, 1, grid, block, lds, 0, net, log, error, error_size)
+        : h->steer == kDirected ? enqueue<float, false, kDirected>(h, 1, grid, block, lds, 0, net, log, error,
+                                                                   error_size)
+                                : enqueue<float, false, kPlain>(h, 1, grid, block, lds, 0, net, log, error, error_size);
+    if (queued && accumulate) enqueue_accumulate(h);
+    return queued;
+  };
   (void)hipEventRecord(h->start, h->stream);
   if (use_graph && total) {
     if (!exec) {
@@ -2322,8 +2572,7 @@ int batch_round(Session *h, uint32_t round, uint32_t seed, uint32_t max_steps, c
       if (!check(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal), error, error_size,
                  "begin capture"))
         return 1;
-      const bool queued = enqueue<float, false, kPlain>(h, 1, grid, block, lds, 0, net, log, error, error_size);
-      if (queued && accumulate) enqueue_accumulate(h);
+      const bool queued = queue();
       hipGraph_t captured = nullptr;
       const hipError_t ended = hipStreamEndCapture(h->stream, &captured);
       if (!queued || !check(ended, error, error_size, "end capture")) {
@@ -2336,8 +2585,7 @@ int batch_round(Session *h, uint32_t round, uint32_t seed, uint32_t max_steps, c
     }
     if (!check(hipGraphLaunch(exec, h->stream), error, error_size, "graph launch")) return 1;
   } else {
-    if (!enqueue<float, false, kPlain>(h, 1, grid, block, lds, 0, net, log, error, error_size)) return 1;
-    if (accumulate) enqueue_accumulate(h);
+    if (!queue()) return 1;
     if (!check(hipGetLastError(), error, error_size, "accumulation launch")) return 1;
   }
   (void)hipEventRecord(h->stop, h->stream);
@@ -2361,6 +2609,23 @@ int batch_round(Session *h, uint32_t round, uint32_t seed, uint32_t max_steps, c
   }
   return 0;
 }
+
+namespace {
+
+// Captured batch rounds bake in the transport instantiation and guidance: drop them when the
+// sampling changes.
+void drop_graphs(Session *h) {
+  if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
+  if (h->graph) (void)hipGraphDestroy(h->graph);
+  if (h->accumulate_exec) (void)hipGraphExecDestroy(h->accumulate_exec);
+  if (h->accumulate_graph) (void)hipGraphDestroy(h->accumulate_graph);
+  h->graph_exec = nullptr;
+  h->graph = nullptr;
+  h->accumulate_exec = nullptr;
+  h->accumulate_graph = nullptr;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -2476,7 +2741,11 @@ int monki_hip_attach_proposal(void *handle, void *proposal, float alpha, uint32_
     return 1;
   }
   h->proposal = proposal;
-  h->guidance = Guidance{alpha, species_mask, max_order, 0.0f};
+  h->guidance.alpha = alpha;
+  h->guidance.species_mask = species_mask;
+  h->guidance.max_order = max_order;
+  h->steer = proposal ? kGuided : kPlain;
+  drop_graphs(h);
   return 0;
 }
 
@@ -2494,7 +2763,49 @@ int monki_hip_set_directed(void *handle, float alpha, float kappa, uint32_t spec
     set_error(error, error_size, "Invalid session, defensive fraction (0 < alpha <= 1) or kappa (1 to 1e6)");
     return 1;
   }
-  h->guidance = Guidance{alpha, species_mask, max_order, kappa};
+  h->guidance.alpha = alpha;
+  h->guidance.species_mask = species_mask;
+  h->guidance.max_order = max_order;
+  h->guidance.kappa = kappa;
+  h->steer = kDirected;
+  drop_graphs(h);
+  return 0;
+}
+
+int monki_hip_set_steering(void *handle, int lobe_species, float lobe_fraction, int surface, float split_above,
+                           float roulette_below, float roulette_weight, char *error, size_t error_size) {
+  Session *h = static_cast<Session *>(handle);
+  if (!h) {
+    set_error(error, error_size, "Invalid HIP session");
+    return 1;
+  }
+  for (const Scene &sc : h->scenes) {
+    if (lobe_species >= sc.species) {
+      set_error(error, error_size, "The lobe species is not a tabulated species of every scene");
+      return 1;
+    }
+  }
+  const bool window_ok = split_above == 0.0f || (split_above > 1.0f && roulette_below >= 0.0f &&
+                                                 roulette_below < split_above &&
+                                                 (roulette_below == 0.0f ||
+                                                  (roulette_weight > roulette_below && roulette_weight <= split_above)));
+  if (lobe_species < -1 || !(lobe_fraction >= 0.0f && lobe_fraction <= 1.0f) || !window_ok) {
+    set_error(error, error_size,
+              "Invalid steering: lobe species -1 or a tabulated species, lobe fraction in [0, 1], weight window "
+              "0 (off) or split above 1 > roulette below >= 0, roulette weight in (roulette below, split above]");
+    return 1;
+  }
+  if (lobe_species < 0 && lobe_fraction > 0.0f && !(h->guidance.kappa >= 1.0f)) {
+    set_error(error, error_size, "vMF view lobes need a concentration: set it with monki_hip_set_directed");
+    return 1;
+  }
+  h->guidance.lobe_species = lobe_species;
+  h->guidance.lobe_fraction = lobe_fraction;
+  h->guidance.surface = surface ? 1 : 0;
+  h->guidance.split_above = split_above;
+  h->guidance.roulette_below = roulette_below;
+  h->guidance.roulette_weight = roulette_weight;
+  drop_graphs(h);
   return 0;
 }
 

@@ -21,8 +21,9 @@ import numpy as np
 from ..input import ReferenceCase, SceneCase
 from ..metadata import file_hash, scene_metadata
 from ..preparation import UPSTREAM, prepare_case
-from .hip import DOUBLE_POINTER, FLOAT_POINTER, LIBRARY, UINT_POINTER, load_library
+from .hip import DOUBLE_POINTER, FLOAT_POINTER, HIP, LIBRARY, UINT_POINTER, load_library
 from .hip_tables import pack_tables, tally_count
+from .proposal import DirectedSettings, Proposal, ProposalSettings, apply_steering, sharpest_species
 from .tallies import BatchControls, RawTallies
 
 
@@ -89,6 +90,8 @@ class HIPBatch:
         use_graph: bool = True,
         library: Path = LIBRARY,
         upstream: Path = UPSTREAM,
+        proposal: Optional[ProposalSettings] = None,
+        directed: Optional[DirectedSettings] = None,
     ):
         for name, value, minimum, maximum in (
             ("device", device, 0, 2**32 - 1),
@@ -102,6 +105,12 @@ class HIPBatch:
         if block not in (64, 128, 256):
             raise ValueError("block must be 64, 128 or 256")
 
+        if proposal is not None and not isinstance(proposal, ProposalSettings):
+            raise ValueError("proposal must be ProposalSettings or None")
+
+        if directed is not None and (not isinstance(directed, DirectedSettings) or proposal is not None):
+            raise ValueError("directed must be DirectedSettings or None, and excludes a proposal")
+
         self.device = device
         self.save_path = save_path
         self.seed = seed
@@ -113,6 +122,9 @@ class HIPBatch:
         self.upstream = Path(upstream)
         self.metadata: List = []
         self.provenance: dict = {}
+        self.proposal = proposal
+        self.directed = directed
+        self._network: Optional[Proposal] = None
         self._native = None
         self._handle = None
         self._cases: List[SceneCase] = []
@@ -200,6 +212,57 @@ class HIPBatch:
             "python_version": platform.python_version(),
             "gpu_rounds": 0,
         }
+
+        try:
+            self._steer(cases)
+        except BaseException:
+            self.close()
+            raise
+
+    def _steer(self, cases) -> None:
+        """Guided or detector-directed rounds for the whole batch: one network, trained on the
+        first case by the single-scene backend, or the view lobes; the lobes take the sharpest
+        tabulated phase function of the first case (it must exist in every scene)."""
+
+        native, handle = self._native, self._handle
+        settings = self.proposal or self.directed
+
+        if settings is None:
+            return
+
+        lobes = sharpest_species(cases[0].scene)
+        error = ctypes.create_string_buffer(4096)
+
+        if self.proposal is not None:
+            trainer = HIP(self.device, proposal=self.proposal, seed=self.seed, max_steps=self.max_steps,
+                          library=self.library)  # fmt: skip
+
+            with trainer:
+                trainer.prepare(cases[0])
+                self.provenance["proposal"] = dict(trainer.provenance["proposal"], trained_on=cases[0].name)
+                self._network = trainer.release_network()
+
+            status = native.monki_hip_attach_proposal(
+                handle, self._network.handle, settings.alpha, settings.species_mask(), min(settings.max_order, 2**32 - 1),
+                error, len(error),
+            )  # fmt: skip
+            fraction = settings.lobe_fraction if lobes >= 0 else 0.0
+            self.provenance["method"] = "guided (view lobes + learned vMF mixture, defensive physical sampler)"
+        else:
+            status = native.monki_hip_set_directed(
+                handle, settings.alpha, settings.kappa, settings.species_mask() & 0xFFFFFFFF,
+                min(settings.max_order, 2**32 - 1), error, len(error),
+            )  # fmt: skip
+            lobes = lobes if settings.lobes == "phase" else -1
+            fraction = 1.0
+            self.provenance["directed"] = {key: getattr(settings, key) for key in settings.__dataclass_fields__}
+            self.provenance["method"] = "detector-directed (view lobes, defensive physical sampler)"
+
+        if status:
+            raise RuntimeError(error.value.decode())
+
+        apply_steering(native, handle, settings, lobes, fraction)
+        self.provenance["lobe_species"] = lobes
 
     def case_provenance(self) -> list:
         records = []
@@ -365,6 +428,11 @@ class HIPBatch:
     def close(self) -> None:
         if self._handle is not None and self._native is not None:
             self._native.monki_hip_close(self._handle)
+
+        if self._network is not None:
+            self._network.close()
+
+        self._network = None
 
         self._handle = None
         self._native = None

@@ -26,7 +26,8 @@ from ..metadata import file_hash, scene_metadata
 from ..preparation import UPSTREAM, prepare_case
 from .backend import Backend
 from .hip_tables import pack_tables, split_scattering, tally_count
-from .proposal import DirectedSettings, Proposal, ProposalSettings, declare, log_rounds
+from .proposal import (DirectedSettings, Proposal, ProposalSettings, apply_steering, declare, log_rounds,
+                       sharpest_species)
 from .tallies import BatchControls, RawTallies
 
 PACKAGE_ROOT = Path(__file__).resolve().parents[2]
@@ -285,26 +286,39 @@ class HIP(Backend):
         if preparation is not None:
             self.provenance["preparation"] = preparation
 
-        if self.proposal is not None:
-            try:
+        try:
+            if self.proposal is not None:
                 self._train_proposal(nx * ny)
-            except BaseException:
-                self.close()
-                raise
+                lobes = sharpest_species(tables)
+                apply_steering(native, handle, self.proposal, lobes, self.proposal.lobe_fraction if lobes >= 0 else 0.0)
+                self.provenance["proposal"]["lobe_species"] = lobes
 
-        if self.directed is not None:
-            settings = self.directed
-            error = ctypes.create_string_buffer(4096)
+            if self.directed is not None:
+                settings = self.directed
+                error = ctypes.create_string_buffer(4096)
 
-            if native.monki_hip_set_directed(
-                handle, settings.alpha, settings.kappa, settings.species_mask() & 0xFFFFFFFF,
-                min(settings.max_order, 2**32 - 1), error, len(error),
-            ):  # fmt: skip
-                self.close()
-                raise RuntimeError(error.value.decode())
+                if native.monki_hip_set_directed(
+                    handle, settings.alpha, settings.kappa, settings.species_mask() & 0xFFFFFFFF,
+                    min(settings.max_order, 2**32 - 1), error, len(error),
+                ):  # fmt: skip
+                    raise RuntimeError(error.value.decode())
 
-            self.provenance["directed"] = {key: getattr(settings, key) for key in settings.__dataclass_fields__}
-            self.provenance["method"] = "detector-directed (vMF lobes toward the views, defensive phase function)"
+                lobes = sharpest_species(tables) if settings.lobes == "phase" else -1
+                apply_steering(native, handle, settings, lobes, 1.0)
+                self.provenance["directed"] = {key: getattr(settings, key) for key in settings.__dataclass_fields__}
+                self.provenance["directed"]["lobe_species"] = lobes
+                shape = "phase-function" if lobes >= 0 else "vMF"
+                self.provenance["method"] = f"detector-directed ({shape} lobes toward the views, defensive physical sampler)"
+        except BaseException:
+            self.close()
+            raise
+
+    def release_network(self) -> Optional[Proposal]:
+        """Hand the trained proposal to the caller (who then closes it); the session keeps it
+        attached until it is closed or prepared again."""
+
+        network, self._network = self._network, None
+        return network
 
     def run_batch(self, controls: object) -> RawTallies:
         """Execute exactly one complete statistical round."""
@@ -460,7 +474,7 @@ class HIP(Backend):
             "loss_first": float(losses[0]) if len(losses) else None,
             "loss_last": float(np.mean(losses[-50:])) if len(losses) else None,
         }
-        self.provenance["method"] = "guided (learned vMF-mixture proposal, defensive phase function)"
+        self.provenance["method"] = "guided (view lobes + learned vMF mixture, defensive physical sampler)"
 
     def _require_session(self):
         if self._handle is None or self._native is None:

@@ -30,22 +30,44 @@ _U = ctypes.POINTER(ctypes.c_uint32)
 _U64 = ctypes.POINTER(ctypes.c_uint64)
 
 
+def _check_window(settings) -> None:
+    """Shared validation of the steering and weight-window fields."""
+
+    if not 0.0 < float(settings.alpha) <= 1.0:
+        raise ValueError("alpha must be in (0, 1]")
+
+    if type(settings.surface) is not bool:
+        raise ValueError("surface must be a bool")
+
+    split, below, weight = float(settings.split_above), float(settings.roulette_below), float(settings.roulette_weight)
+
+    if split != 0.0 and not (split > 1.0 and 0.0 <= below < split and (below == 0.0 or below < weight <= split)):
+        raise ValueError("weight window: split_above 0 (off) or > 1, 0 <= roulette_below < roulette_weight <= split_above")
+
+    tabulated = settings.species_mask() & 0xFFFFFFFE
+    # Without a window the steering factors of a history multiply unchecked: over the tens of
+    # events of a thick cloud the estimator, though unbiased, becomes too heavy-tailed for its
+    # standard error to mean anything (guided = plain failed at |z| 11.6 that way).
+    if split == 0.0 and tabulated and settings.max_order > 4:
+        raise ValueError("steering tabulated species beyond order 4 needs the weight window (split_above > 1)")
+
+
 @dataclass(frozen=True)
 class ProposalSettings:
-    """How the guided tracer's proposal is trained before the estimation rounds.
+    """How the guided tracer's proposal is trained and used.
 
-    events: scattering events to log from the plain tracer (training samples).
-    photons_per_column: photons per column of each logging round.
-    steps, batch, learning_rate: Adam steps over random minibatches of the samples.
-    alpha: probability of drawing the direction from the phase function (0 < alpha <= 1).
-    guided_kinds: scatterer kinds whose events are guided: "gas" and/or tabulated species
-        indices (0-based); None guides every kind. The default guides gas (Rayleigh) events:
-        a vMF mixture follows their smooth phase function closely, while guiding every event
-        of a thick Mie cloud multiplies weight factors over tens of events.
-    max_order: events up to this scattering order are guided; later ones use the phase
-        function only (the weight factors of a history multiply, so bounding the guided orders
-        bounds the weight at (1 / alpha) ** max_order).
-    seed: network initialization and the logging rounds' random streams.
+    Training: ``events`` scattering events logged from the plain tracer (``photons_per_column``
+    per logging round), then ``steps`` Adam steps over random minibatches of ``batch`` samples
+    at ``learning_rate``; ``seed`` initializes the network and the logging streams.
+
+    Use: a steered event draws its direction from the phase function with probability
+    ``alpha``, else from the proposal: ``lobe_fraction`` x lobes around the view directions
+    shaped by the sharpest tabulated phase function + (1 - ``lobe_fraction``) x the network's
+    vMF mixture. Steered events: atmospheric events of ``guided_kinds`` ("gas" and/or tabulated
+    species indices; None for every kind) up to ``max_order``, and Lambertian reflections when
+    ``surface``. Weight window: photons heavier than ``split_above`` are split, lighter than
+    ``roulette_below`` play Russian roulette for ``roulette_weight`` (``split_above`` 0 turns
+    the window off, which is refused for tabulated species beyond order 4).
     """
 
     events: int = 4_000_000
@@ -53,10 +75,15 @@ class ProposalSettings:
     steps: int = 1_500
     batch: int = 16_384
     learning_rate: float = 1e-3
-    alpha: float = 0.3
-    guided_kinds: Optional[tuple] = ("gas",)
-    max_order: int = 2**31
+    alpha: float = 0.7
+    guided_kinds: Optional[tuple] = None
+    max_order: int = 2**32 - 1
     seed: int = 97
+    lobe_fraction: float = 0.5
+    surface: bool = True
+    split_above: float = 2.0
+    roulette_below: float = 0.05
+    roulette_weight: float = 0.1
 
     def species_mask(self) -> int:
         if self.guided_kinds is None:
@@ -84,13 +111,15 @@ class ProposalSettings:
         if not 0.0 < float(self.learning_rate) < 1.0:
             raise ValueError("learning_rate must be in (0, 1)")
 
-        if not 0.0 < float(self.alpha) <= 1.0:
-            raise ValueError("alpha must be in (0, 1]")
+        if not 0.0 <= float(self.lobe_fraction) <= 1.0:
+            raise ValueError("lobe_fraction must be in [0, 1]")
 
         if self.guided_kinds is not None:
             for kind in self.guided_kinds:
                 if kind != "gas" and not (type(kind) is int and 0 <= kind < 4):
                     raise ValueError("guided_kinds holds 'gas' or tabulated species indices 0-3")
+
+        _check_window(self)
 
 
 def declare(native) -> None:
@@ -116,34 +145,44 @@ def declare(native) -> None:
     native.monki_hip_run_logged.argtypes = [ctypes.c_void_p, _U, _D, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
     native.monki_hip_set_directed.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]
     native.monki_hip_run_directed.argtypes = [ctypes.c_void_p, _U, _D, ctypes.c_char_p, ctypes.c_size_t]
+    native.monki_hip_set_steering.argtypes = [
+        ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+        ctypes.c_char_p, ctypes.c_size_t,
+    ]  # fmt: skip
 
     for name in (
         "monki_hip_proposal_set_parameters", "monki_hip_proposal_get_parameters", "monki_hip_proposal_reserve",
         "monki_hip_proposal_set_samples", "monki_hip_proposal_samples", "monki_hip_proposal_read_samples",
         "monki_hip_proposal_gradients", "monki_hip_proposal_train", "monki_hip_proposal_forward",
         "monki_hip_attach_proposal", "monki_hip_run_guided", "monki_hip_trace_guided", "monki_hip_run_logged",
-        "monki_hip_set_directed", "monki_hip_run_directed",
+        "monki_hip_set_directed", "monki_hip_run_directed", "monki_hip_set_steering",
     ):  # fmt: skip
         getattr(native, name).restype = ctypes.c_int
 
 
 @dataclass(frozen=True)
 class DirectedSettings:
-    """Detector-directed sampling (no network): the tracer draws a scattering direction from the
-    phase function with probability ``alpha`` and otherwise from von Mises-Fisher lobes of
-    concentration ``kappa`` around the view directions, and multiplies the weight by
-    p / (alpha p + (1 - alpha) q). Photons then travel toward the detectors more often, and
-    the rare events whose direction lies in a tabulated phase function's forward peak toward a
-    view (the large local-estimate contributions of cloud radiances) come with proportionally
-    smaller weights. ``guided_kinds`` and ``max_order`` as in ProposalSettings; the default
-    directs the first-order events of tabulated species (clouds, aerosol): directing every
-    order multiplies the per-event factors (up to 1 / alpha each) into a heavy-tailed weight.
+    """Detector-directed sampling (no network). A steered event draws its direction from the
+    physical sampler (phase function; Lambert at the surface) with probability ``alpha``, else
+    from lobes around the view directions, and multiplies the weight by
+    p / (alpha p + (1 - alpha) q). ``lobes="phase"`` shapes the lobes by the sharpest tabulated
+    phase function of the scene (its forward peak measured from each view direction), so that
+    a photon steered into the peak toward a view carries the inverse peak in its weight and the
+    local estimate at its next event loses its forward-peak spike; ``lobes="vmf"`` uses vMF
+    lobes of concentration ``kappa``. ``guided_kinds``, ``max_order``, ``surface`` and the
+    weight window as in ProposalSettings. The defaults steer every event, the surface
+    included, with the window on.
     """
 
-    alpha: float = 0.5
+    alpha: float = 0.7
     kappa: float = 500.0
-    guided_kinds: Optional[tuple] = ("tabulated",)
-    max_order: int = 1
+    guided_kinds: Optional[tuple] = None
+    max_order: int = 2**32 - 1
+    lobes: str = "phase"
+    surface: bool = True
+    split_above: float = 2.0
+    roulette_below: float = 0.05
+    roulette_weight: float = 0.1
 
     def species_mask(self) -> int:
         if self.guided_kinds is None:
@@ -159,14 +198,38 @@ class DirectedSettings:
         return mask
 
     def __post_init__(self):
-        if not (isinstance(self.alpha, (int, float)) and 0 < self.alpha <= 1):
-            raise ValueError("alpha must be in (0, 1]")
-
         if not (isinstance(self.kappa, (int, float)) and 1 <= self.kappa <= 1e6):
             raise ValueError("kappa must be in [1, 1e6]")
 
         if type(self.max_order) is not int or not 1 <= self.max_order <= 2**32 - 1:
             raise ValueError("max_order must be an integer in [1, 2**32 - 1]")
+
+        if self.lobes not in ("phase", "vmf"):
+            raise ValueError("lobes is 'phase' or 'vmf'")
+
+        _check_window(self)
+
+
+def sharpest_species(scene) -> int:
+    """Index of the tabulated species with the highest forward peak (F11 at cos 1, F11 being
+    normalized to a sphere mean of one), or -1 without tabulated species."""
+
+    if not scene.species:
+        return -1
+
+    return int(np.argmax([float(species.f11[-1]) for species in scene.species]))
+
+
+def apply_steering(native, handle, settings, lobe_species: int, lobe_fraction: float) -> None:
+    """Lobes, surface steering and weight window of a session (monki_hip_set_steering)."""
+
+    error = ctypes.create_string_buffer(4096)
+
+    if native.monki_hip_set_steering(
+        handle, lobe_species, lobe_fraction, int(settings.surface), settings.split_above, settings.roulette_below,
+        settings.roulette_weight, error, len(error),
+    ):  # fmt: skip
+        raise RuntimeError(error.value.decode())
 
 
 class Proposal:

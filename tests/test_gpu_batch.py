@@ -232,3 +232,71 @@ def test_simulate_batch_on_device_equals_host_rounds(tmp_path):
             expected = b.values[variable]
             np.testing.assert_allclose(a.values[variable], expected, rtol=1e-4, atol=1e-4 * np.abs(expected).max(),
                                        err_msg=variable)  # fmt: skip
+
+
+def test_spectral_field_full_size():
+    """The spectral case at full size: 64 wavelengths of the 128 x 128 x 60 stochastic field in one
+    batch round (1 photon per column). Every tally finite, |Q|, |U| <= I per pixel, clear-column
+    radiance falling with wavelength over 340-500 nm (Rayleigh lambda^-4.08, ozone negligible
+    there), and two wavelengths equal to their single-scene sessions."""
+
+    from monki import SceneCase, configs
+    from monki.execution import HIP, HIPBatch, BatchControls
+
+    field = configs.stochastic_cloud_field(128, seed=7)
+    wavelengths = configs.spectral_wavelengths(64)
+    spectral = [SceneCase(f"w{i:02d}", configs.spectral_cloud_field(w, field), 1, 2) for i, w in enumerate(wavelengths)]
+    assert spectral[0].scene.nx == 128 and spectral[0].scene.nz == 60
+    print("spectral full size: 64 x 128 x 128 x 60, one batch round", flush=True)
+
+    with HIPBatch(tally_shards=4) as batch:
+        batch.prepare(spectral)
+        rounds = batch.run_round(1)
+
+    clear = field == 0
+    means = []
+
+    for result in rounds:
+        i, q, u = result.stokes
+        assert np.isfinite(result.stokes).all() and np.isfinite(result.weighted_paths).all()
+        assert (i >= 0).all() and (result.weighted_paths >= 0).all() and (result.weights >= 0).all()
+        assert (np.abs(q) <= i * (1 + 1e-5) + 1e-12).all() and (np.abs(u) <= i * (1 + 1e-5) + 1e-12).all()
+        means.append(i[clear].mean())
+
+    means = np.array(means)
+    band = [int(np.argmin(np.abs(wavelengths - w))) for w in (340.0, 380.0, 420.0, 460.0, 500.0)]
+    print("clear-column mean I at 340/380/420/460/500 nm:", means[band], flush=True)
+    assert np.all(np.diff(means[band]) < 0)
+
+    for number in (0, 63):
+        print(f"single-scene session of wavelength {wavelengths[number]:.1f} nm", flush=True)
+        with HIP(tally_shards=4) as single:
+            single.prepare(spectral[number])
+            assert_tallies_equal(rounds[number], single.run_batch(BatchControls(1, 1)))
+
+
+@pytest.mark.parametrize("mode", ["directed", "guided"])
+def test_steered_batch_equals_single_scene_sessions(mode):
+    """Steered rounds (view lobes, surface steering, weight window; guided: one network for the
+    batch) in a batch equal the same scenes steered alone: same streams, same copies."""
+
+    from monki.execution import HIP, HIPBatch, BatchControls
+    from monki.execution.proposal import DirectedSettings, ProposalSettings
+
+    batch_cases = cases(rounds=2, ppc=100)
+    box_cases = [batch_cases[0], batch_cases[3]]  # every scene needs the lobe species
+    directed = DirectedSettings() if mode == "directed" else None
+    proposal = ProposalSettings(events=50_000, photons_per_column=200, steps=20, batch=4096) if mode == "guided" else None
+
+    with HIPBatch(tally_shards=8, directed=directed, proposal=proposal) as batch:
+        batch.prepare(box_cases)
+        batched = batch.run_round(1)
+        parameters = batch._network.parameters if proposal else None
+
+    for number, case in enumerate(box_cases):
+        with HIP(tally_shards=8, directed=directed, proposal=proposal) as single:
+            single.prepare(case)
+            if proposal:
+                # The batch trained on its first case; give the single session the same network.
+                single.network.parameters = parameters
+            assert_tallies_equal(batched[number], single.run_batch(BatchControls(1, case.photons_per_column)))

@@ -138,11 +138,13 @@ def test_logged_samples_are_well_formed():
 
 
 GUIDANCE = {
-    # Every scatterer kind, the first four orders: Mie events included, the weight of a
+    # Every kind and order, the surface too, view lobes + network, weight window on (defaults).
+    "defaults": dict(),
+    # Network only, every scatterer kind, the first four orders, no window: the weight of a
     # history bounded by (1 / alpha) ** 4.
-    "all-kinds-order-4": dict(alpha=0.5, max_order=4),
-    # Rayleigh/gas events only, every order.
-    "gas-all-orders": dict(alpha=0.3, guided_kinds=("gas",)),
+    "network-order-4": dict(alpha=0.5, max_order=4, lobe_fraction=0.0, surface=False, split_above=0.0),
+    # Rayleigh/gas events only, every order, network only.
+    "gas-all-orders": dict(alpha=0.3, guided_kinds=("gas",), lobe_fraction=0.0, surface=False, split_above=0.0),
 }
 
 
@@ -150,9 +152,10 @@ GUIDANCE = {
 def test_guided_tracer_agrees_with_plain_tracer(tmp_path, guidance):
     """Unbiasedness: guided and plain radiances and layer paths agree within combined SE.
 
-    Guiding every Mie event of the tau-20 box without a bound on the order leaves the estimator
+    Without the weight window, guiding every Mie event of the tau-20 box leaves the estimator
     unbiased but heavy-tailed (per-event factors up to 1 / alpha multiply over ~20 events), so
-    the guided configurations bound the kinds or orders that are guided.
+    those configurations bound the kinds or orders that are guided; the defaults steer every
+    event with the window on.
     """
 
     from monki import AMFConvergence, SceneCase, simulate
@@ -181,11 +184,19 @@ def test_guided_tracer_agrees_with_plain_tracer(tmp_path, guidance):
         assert 0.6 < stats["chi_square_per_dof"] < 1.5, name
 
 
-@pytest.mark.parametrize("kappa", [50.0, 2000.0])
-def test_directed_tracer_agrees_with_plain_tracer(tmp_path, kappa):
-    """Unbiasedness of detector-directed sampling: the directed and plain radiances and layer
-    AMFs agree within combined SE. Mie events of the first two orders are directed (weight
-    factors at most 1 / alpha each, so the estimator stays light-tailed enough for the test)."""
+DIRECTED = {
+    # Phase-function lobes toward the views, every event and the surface, weight window (defaults).
+    "defaults": dict(),
+    # vMF lobes, Mie events of the first two orders, no window.
+    "vmf-order-2": dict(alpha=0.5, kappa=2000.0, guided_kinds=("tabulated",), max_order=2, lobes="vmf",
+                        surface=False, split_above=0.0),
+}
+
+
+@pytest.mark.parametrize("configuration", list(DIRECTED))
+def test_directed_tracer_agrees_with_plain_tracer(tmp_path, configuration):
+    """Unbiasedness of detector-directed sampling and of the weight window: the directed and
+    plain radiances and layer AMFs agree within combined SE."""
 
     from monki import AMFConvergence, SceneCase, simulate
     from monki.execution import DirectedSettings, HIP
@@ -196,10 +207,12 @@ def test_directed_tracer_agrees_with_plain_tracer(tmp_path, kappa):
     case = SceneCase("box", scene, ppc, rounds)
     policy = AMFConvergence(1e-9, rounds, 16, 3)
     plain = simulate(case, HIP(seed=5), tmp_path / "plain.nc", convergence=policy)
-    backend = HIP(seed=6, directed=DirectedSettings(alpha=0.5, kappa=kappa, max_order=2))
+    settings = DirectedSettings(**DIRECTED[configuration])
+    backend = HIP(seed=6, directed=settings)
     directed = simulate(case, backend, tmp_path / "directed.nc", convergence=policy)
     receipt = json.loads((tmp_path / "directed.run.json").read_text())
-    assert receipt["directed"]["kappa"] == kappa and receipt["method"].startswith("detector-directed")
+    assert receipt["directed"]["alpha"] == settings.alpha and receipt["method"].startswith("detector-directed")
+    assert receipt["directed"]["lobe_species"] == (0 if settings.lobes == "phase" else -1)
     report = {}
 
     for name in ("radiance", "AMF_layer"):

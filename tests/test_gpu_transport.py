@@ -73,6 +73,10 @@ def small_scenes():
         "max-size": max_size_scene(),
         # A single periodic column: every horizontal crossing re-enters the same column.
         "one-column": configs.layered_cloud(nx=1, ny=1, nmat=3),
+        # The spectral case at 300, 400 and 500 nm on a 16 x 16 stochastic field: 59 layers +
+        # vacuum (0.25 km layers to 5 km), lognormal cloud between 1 and 4 km.
+        **{f"spectral-{w:.0f}": configs.spectral_cloud_field(w, configs.stochastic_cloud_field(16, seed=7))
+           for w in (300.0, 400.0, 500.0)},  # fmt: skip
     }
 
 
@@ -175,7 +179,12 @@ AGREEMENT = {  # (float32, float64) over all photons
     "box-cloud-surface-toa-grid": (0.90, 0.998),
     "max-size": (0.90, 0.998),
     "one-column": (0.995, 0.999),
+    "spectral-300": (0.85, 0.995),
+    "spectral-400": (0.85, 0.995),
+    "spectral-500": (0.85, 0.995),
 }
+
+SPECTRAL = [name for name in SCENES if name.startswith("spectral")]
 
 
 EXPLICIT_VIEWS = {"box-cloud-surface", "box-cloud-IQUV-surface", "box-cloud-surface-toa-grid"}
@@ -191,7 +200,7 @@ def check_histories(name, gpu, cpu, precision):
     assert same.mean() >= AGREEMENT[name][0 if precision == 32 else 1], same.mean()
     # Contributions of identical sequences carry rounding only.
     limit = (1e-3, 2e-2) if precision == 32 else (1e-6, 1e-4)
-    if name == "max-size":
+    if name == "max-size" or name in SPECTRAL:
         # 0.5 km layers up to 20 km: a free path or detector ray crosses about twice as many
         # layer boundaries as in the 35-layer grid, each adding float32 rounding to the
         # accumulated optical depth (short histories still agree to ~1e-4).
@@ -247,7 +256,7 @@ def check_tallies(name, gpu_tallies, cpu_tallies, gpu, cpu, rtol):
     assert cpu_tallies[intensity].sum() == pytest.approx(cpu[:, 4].sum(), rel=1e-9)
 
 
-@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("name", [name for name in SCENES if name not in SPECTRAL])
 def test_float32_histories_match_float64_oracle(oracle, name):
     """float32 transport follows the float64 oracle history by history on the same streams."""
 
@@ -266,6 +275,17 @@ def test_float64_build_matches_oracle(oracle, name):
     gpu_tallies, gpu, cpu_tallies, cpu = trace_both(oracle, name, 100, precision=64)
     check_histories(name, gpu, cpu, 64)
     check_tallies(name, gpu_tallies, cpu_tallies, gpu, cpu, rtol=1e-4)
+
+
+@pytest.mark.parametrize("precision", [32, 64])
+@pytest.mark.parametrize("name", SPECTRAL)
+def test_spectral_field_matches_oracle(oracle, name, precision):
+    """The spectral stochastic cloud field (a reduced 16 x 16 field with the full 59 + 1 layers,
+    lambda^-4.08 Rayleigh and synthetic ozone) history by history against the oracle."""
+
+    gpu_tallies, gpu, cpu_tallies, cpu = trace_both(oracle, name, 30 if precision == 32 else 20, precision=precision)
+    check_histories(name, gpu, cpu, precision)
+    check_tallies(name, gpu_tallies, cpu_tallies, gpu, cpu, rtol=2e-2 if precision == 32 else 1e-4)
 
 
 def test_circular_polarization_matches_oracle(oracle):
@@ -439,13 +459,13 @@ def test_batch_validation_precedes_dispatch():
     with HIP() as backend:
         backend.prepare(case("clear-IQU"))
 
-        with pytest.raises(ValueError, match="sequential"):
+        with pytest.raises(ValueError, match="comes next"):
             backend.run_batches([BatchControls(1, 200), BatchControls(3, 200)])
 
         assert backend.provenance["gpu_dispatches"] == 0
         setattr(backend, "seed", 1)
 
-        with pytest.raises(ValueError, match="reprepare"):
+        with pytest.raises(ValueError, match="call prepare again"):
             backend.run_batch(BatchControls(1, 200))
 
 

@@ -556,14 +556,34 @@ def test_spectral_cloud_field_statistics():
 def test_directed_settings_validate_and_build_species_masks():
     from monki.execution import DirectedSettings
 
-    assert DirectedSettings().species_mask() == 0xFFFFFFFE
+    assert DirectedSettings(guided_kinds=("tabulated",)).species_mask() == 0xFFFFFFFE
     assert DirectedSettings(guided_kinds=("gas", 1)).species_mask() == 0b101
-    assert DirectedSettings(guided_kinds=None).species_mask() == 0xFFFFFFFF
-    assert DirectedSettings().max_order == 1
+    assert DirectedSettings().species_mask() == 0xFFFFFFFF
+    assert DirectedSettings().max_order == 2**32 - 1 and DirectedSettings().split_above > 1
 
-    for bad in ({"alpha": 0.0}, {"alpha": 1.5}, {"kappa": 0.5}, {"kappa": 2e6}, {"max_order": 0}, {"max_order": 1.0}):
+    for bad in ({"alpha": 0.0}, {"alpha": 1.5}, {"kappa": 0.5}, {"kappa": 2e6}, {"max_order": 0}, {"max_order": 1.0},
+                {"lobes": "cone"}, {"surface": 1}, {"split_above": 0.5}, {"roulette_weight": 3.0},
+                {"roulette_below": 0.2, "roulette_weight": 0.1}):  # fmt: skip
         with pytest.raises(ValueError):
             DirectedSettings(**bad)
+
+
+@pytest.mark.parametrize("settings", ["DirectedSettings", "ProposalSettings"])
+def test_unbounded_steering_of_tabulated_species_needs_the_weight_window(settings):
+    """Steering Mie events of every order without the weight window is refused: the product of
+    the steering factors over a thick cloud's tens of events leaves the standard error
+    meaningless. Bounded orders, gas only, or the window make it acceptable."""
+
+    from monki.execution import DirectedSettings, ProposalSettings
+
+    kind = {"DirectedSettings": DirectedSettings, "ProposalSettings": ProposalSettings}[settings]
+
+    with pytest.raises(ValueError, match="weight window"):
+        kind(split_above=0.0)
+
+    kind(split_above=0.0, max_order=4)
+    kind(split_above=0.0, guided_kinds=("gas",))
+    kind()
 
 
 def test_decomposition_settings_layout_and_product_variables():

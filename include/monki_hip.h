@@ -204,11 +204,12 @@ int monki_hip_run_directed(void *handle, const uint32_t *controls, double *out, 
  *   lobe_fraction: guided tracer only, the lobes' share of the proposal (the rest is the
  *     network's mixture); the detector-directed tracer uses the lobes alone;
  *   surface: nonzero steers Lambertian reflections too (density mu / pi against the lobes);
- *   split_above, roulette_below, roulette_weight: the weight window. 0 turns it off; else a
- *     photon heavier than split_above after a scattering continues as up to 4 copies of equal
- *     weight, and one lighter than roulette_below before a scattering survives with
- *     probability weight / roulette_weight at that weight. Unbiased; bounds the products of
- *     the steering factors.
+ *   split_above, roulette_below, roulette_weight: the weight window on the product g of a
+ *     photon's steering factors (its weight over the plain tracer's). 0 turns it off; else a
+ *     photon whose g exceeds split_above after a scattering continues as up to 4 copies of
+ *     equal weight, and one whose g is below roulette_below before a scattering survives with
+ *     probability g / roulette_weight, its g raised to roulette_weight. Unbiased; bounds the
+ *     products of the steering factors without touching the physical ones (albedos).
  * Returns 0, or nonzero with error text for invalid values. */
 int monki_hip_set_steering(void *handle, int lobe_species, float lobe_fraction, int surface, float split_above,
                            float roulette_below, float roulette_weight, char *error, size_t error_size);

@@ -80,11 +80,13 @@ constexpr bool decomposition(int mode) {
 // directions shaped by the bound CDF of tabulated species lobe_species (vMF lobes of
 // concentration kappa when it is negative); in the guided mode it is the mixture
 // lobe_fraction x lobes + (1 - lobe_fraction) x the network's vMF mixture.
-// Weight window (split_above > 0): a photon whose weight exceeds split_above after a
-// scattering continues as up to kMaxCopies copies of equal weight (the others resume later on
-// random streams of their own); one whose weight is below roulette_below before a scattering
-// survives with probability weight / roulette_weight at weight roulette_weight. Both keep the
-// estimator unbiased and bound the products of the steering factors.
+// Weight window (split_above > 0), on the product g of the photon's steering factors (its
+// weight over the plain tracer's): a photon whose g exceeds split_above after a scattering
+// continues as up to kMaxCopies copies of equal weight (the others resume later on random
+// streams of their own); one whose g is below roulette_below before a scattering survives with
+// probability g / roulette_weight, its g raised to roulette_weight. Both keep the estimator
+// unbiased and bound the products of the steering factors; physical factors (albedos) are
+// left alone, as in the plain tracer.
 struct Guidance {
   float alpha;
   uint32_t species_mask;
@@ -683,6 +685,9 @@ template <typename R> struct History {
   // Backward decomposition: the cell of the first scattering event, where every later event's
   // 3-D contribution is registered (calc_radiances_and_scats :509-521).
   int first_cell;
+  // Steered modes: the product of the steering factors in `weight` (weight over the weight the
+  // plain tracer would carry), which the weight window keeps near one.
+  R steer;
 };
 
 // Local estimates of one event, parked in LDS until the wave's cooperative tally reads them.
@@ -751,6 +756,7 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
   h.contribution = 0.0f;
   h.order = 0;
   h.first_cell = 0;
+  h.steer = R(1);
 }
 
 // Result grid on the other level than the detector (mc_routines_cartesian_mod.f90:1041-1105,
@@ -1138,7 +1144,9 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
       // history (p = 0 there).
       const R lambert_density = mu > R(0) ? mu / pi : R(0);
       const R q = R(steer_density(*lobes, float(da), float(db), float(dc)));
-      h.weight *= lambert_density / (R(alpha) * lambert_density + R(1.0f - alpha) * q);
+      const R factor = lambert_density / (R(alpha) * lambert_density + R(1.0f - alpha) * q);
+      h.weight *= factor;
+      h.steer *= factor;
       if (!(h.weight > R(0))) return kAbsorbed;
     }
     h.st = Stokes<R>{R(1), R(0), R(0)};
@@ -1199,7 +1207,11 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
         const float table_norm = h.species < 0 ? 1.0f : s.sp[h.species].norm;
         const R density = norm / (pi * R(table_norm));
         const R q = R(steer_density(*proposal, float(da), float(db), float(dc)));
-        h.weight *= density / (R(alpha) * density + R(1.0f - alpha) * q);
+        const R factor = density / (R(alpha) * density + R(1.0f - alpha) * q);
+        h.weight *= factor;
+        h.steer *= factor;
+        // A direction neither sampler can draw (both densities zero) ends the history.
+        if (!(h.weight > R(0))) return kAbsorbed;
       }
       if (s.nmat > 1) {
         st.i = R(1);
@@ -1427,7 +1439,7 @@ __device__ __attribute__((noinline)) void guided_mixture(const monki_net::Net &n
 // photon tally like the photon itself. At most kTreeCopies copies start per photon, and the
 // history step limit counts from the photon's launch, so every tree ends.
 constexpr int kSplitDepth = 8;
-constexpr int kSplitWords = 20;  // state words before the layer paths
+constexpr int kSplitWords = 21;  // state words before the layer paths
 constexpr uint32_t kMaxCopies = 4;
 constexpr uint32_t kTreeCopies = 4096;
 
@@ -1454,6 +1466,7 @@ __device__ inline void push_copies(uint32_t *entry, const History<R> &h, const R
   entry[17] = __float_as_uint(float(h.path_total));
   entry[18] = __float_as_uint(h.contribution);
   entry[19] = copies;
+  entry[20] = __float_as_uint(float(h.steer));
   for (int k = 0; k < nz; ++k) entry[kSplitWords + k] = __float_as_uint(float(history[k]));
 }
 
@@ -1478,6 +1491,7 @@ __device__ inline uint32_t pop_copy(uint32_t *entry, History<R> &h, R *history, 
   h.steps = entry[16];
   h.path_total = R(__uint_as_float(entry[17]));
   h.contribution = __uint_as_float(entry[18]);
+  h.steer = R(__uint_as_float(entry[20]));
   for (int k = 0; k < nz; ++k) history[k] = R(__uint_as_float(entry[kSplitWords + k]));
   const uint32_t left = entry[19] - 1u;
   entry[19] = left;
@@ -1603,11 +1617,13 @@ __global__ void __launch_bounds__(kBlock32)
     __builtin_amdgcn_wave_barrier();
     // Russian roulette of the weight window, after the event's local estimates (a photon steered
     // toward a view has used its small weight there) and before its scattering.
-    if (window && active && status == kRunning && h.weight < R(guidance.roulette_below)) {
-      if (R(uniform(h.rng)) * R(guidance.roulette_weight) < h.weight)
-        h.weight = R(guidance.roulette_weight);
-      else
+    if (window && active && status == kRunning && h.steer < R(guidance.roulette_below)) {
+      if (R(uniform(h.rng)) * R(guidance.roulette_weight) < h.steer) {
+        h.weight *= R(guidance.roulette_weight) / h.steer;
+        h.steer = R(guidance.roulette_weight);
+      } else {
         status = kAbsorbed;
+      }
     }
     const bool scatters = active && status == kRunning && !h.surface;
     float features[monki_net::kInputs];
@@ -1647,10 +1663,11 @@ __global__ void __launch_bounds__(kBlock32)
       status = scatter_photon<R, TRACE, MODE>(s, failure, h, n, trace, &lobes, &lobes, guidance);
     }
     // Splitting of the weight window: the photon continues as `copies` copies of equal weight.
-    if (window && active && status == kRunning && h.weight > R(guidance.split_above) && top < kSplitDepth &&
+    if (window && active && status == kRunning && h.steer > R(guidance.split_above) && top < kSplitDepth &&
         tree < kTreeCopies) {
-      const uint32_t copies = min(uint32_t(ceil(float(h.weight))), kMaxCopies);
+      const uint32_t copies = min(uint32_t(ceil(float(h.steer))), kMaxCopies);
       h.weight /= R(copies);
+      h.steer /= R(copies);
       push_copies<R>(stack + size_t(top) * split_stride, h, history, s.nz, copies - 1u);
       ++top;
     }
@@ -2795,7 +2812,7 @@ int monki_hip_set_steering(void *handle, int lobe_species, float lobe_fraction, 
               "0 (off) or split above 1 > roulette below >= 0, roulette weight in (roulette below, split above]");
     return 1;
   }
-  if (lobe_species < 0 && lobe_fraction > 0.0f && !(h->guidance.kappa >= 1.0f)) {
+  if (lobe_species < 0 && (lobe_fraction > 0.0f || surface) && !(h->guidance.kappa >= 1.0f)) {
     set_error(error, error_size, "vMF view lobes need a concentration: set it with monki_hip_set_directed");
     return 1;
   }

@@ -290,7 +290,8 @@ class HIP(Backend):
             if self.proposal is not None:
                 self._train_proposal(nx * ny)
                 lobes = sharpest_species(tables)
-                apply_steering(native, handle, self.proposal, lobes, self.proposal.lobe_fraction if lobes >= 0 else 0.0)
+                apply_steering(native, handle, replace(self.proposal, surface=self.proposal.surface and lobes >= 0), lobes,
+                               self.proposal.lobe_fraction if lobes >= 0 else 0.0)  # fmt: skip
                 self.provenance["proposal"]["lobe_species"] = lobes
 
             if self.directed is not None:

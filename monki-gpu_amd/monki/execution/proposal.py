@@ -75,15 +75,15 @@ class ProposalSettings:
     steps: int = 1_500
     batch: int = 16_384
     learning_rate: float = 1e-3
-    alpha: float = 0.7
+    alpha: float = 0.9
     guided_kinds: Optional[tuple] = None
     max_order: int = 2**32 - 1
     seed: int = 97
     lobe_fraction: float = 0.5
     surface: bool = True
     split_above: float = 2.0
-    roulette_below: float = 0.05
-    roulette_weight: float = 0.1
+    roulette_below: float = 0.005
+    roulette_weight: float = 0.01
 
     def species_mask(self) -> int:
         if self.guided_kinds is None:
@@ -174,15 +174,15 @@ class DirectedSettings:
     included, with the window on.
     """
 
-    alpha: float = 0.7
+    alpha: float = 0.9
     kappa: float = 500.0
     guided_kinds: Optional[tuple] = None
     max_order: int = 2**32 - 1
     lobes: str = "phase"
     surface: bool = True
     split_above: float = 2.0
-    roulette_below: float = 0.05
-    roulette_weight: float = 0.1
+    roulette_below: float = 0.005
+    roulette_weight: float = 0.01
 
     def species_mask(self) -> int:
         if self.guided_kinds is None:

@@ -179,9 +179,9 @@ AGREEMENT = {  # (float32, float64) over all photons
     "box-cloud-surface-toa-grid": (0.90, 0.998),
     "max-size": (0.90, 0.998),
     "one-column": (0.995, 0.999),
-    "spectral-300": (0.85, 0.995),
-    "spectral-400": (0.85, 0.995),
-    "spectral-500": (0.85, 0.995),
+    "spectral-300": (0.85, 0.98),
+    "spectral-400": (0.85, 0.98),
+    "spectral-500": (0.85, 0.98),
 }
 
 SPECTRAL = [name for name in SCENES if name.startswith("spectral")]

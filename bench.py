@@ -1,32 +1,40 @@
-"""Benchmark: photon-events/s of the plain HIP tracer on the layered-cloud workload, 1 GPU.
+"""Benchmark of the MONKI HIP transport on one MI355X: photon-events/s and time-to-1%-AMF-SE.
 
-A step is one complete statistical round of the layered-cloud scene with polarization:
-30 x 20 columns x 35 layers (incl. vacuum), water cloud tau 10 at 2-3 km (Mie, r_eff 10 um),
-HG aerosol tau 0.2, Rayleigh + ozone, I/Q/U at 3 views with per-layer path (AMF) tallies,
-10,000 photons per column (6,000,000 photons per round). Tables are resident in HBM before
-timing starts; the timed region covers K rounds: transport kernel, float64 shard reduction
-and the copy of the round's tallies to the host.
+Headline line (``python bench.py``): photon-events/s of the plain tracer on the largest workload
+that fits one GPU, the spectral stochastic cloud field: 64 wavelengths (300-500 nm) of one
+128 x 128 x 60 field (Kolmogorov spectrum, cover 0.6, mean cloudy optical depth 12, cloud
+1-4 km, Rayleigh lambda^-4.08, synthetic ozone), I/Q/U at 3 views, layer AMF tallies, traced as
+one batch: one graph-replayed launch per round, rounds accumulated on the device. A step is one
+round of all 64 scenes (16 photons per column, 16.8e6 photons); tables are resident in HBM
+before timing starts.
 
-value = photon events (atmospheric collisions + surface hits, counted in-kernel) per second.
-roofline: algorithmic bytes per event (4 B per free-path cell crossing, per explicit-equivalent
-detector cell and per phase-table entry read, plus 8 B read-modify-write per tally element
-per view) over the transport kernel's average duration (HIP events on its stream), against
-the 8 TB/s HBM peak. cpu_baseline: the reference's Fortran transport (mc.f90, amdflang -O2
--fopenmp) on a bounded sample of the same scene on the host cores.
+value = photon events (atmospheric collisions + surface hits, counted in-kernel) per second of
+the timed rounds. roofline: algorithmic bytes per event (4 B per free-path cell crossing, per
+explicit-equivalent detector cell and per phase-table entry read, 8 B read-modify-write per tally
+element per view) over the average device time of a round (HIP events on the batch stream),
+against the 8 TB/s HBM peak; ``traffic`` is the measured HBM bytes of the transport kernel
+(rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), and ``binding`` names what
+actually limits the kernel (the tally's record atomics). cpu_baseline: the reference's Fortran
+transport (mc.f90, amdflang -O2 -fopenmp) on a bounded sample of one wavelength of the same
+field, all host threads this job has, one warm-up and the median of 3 runs, with 1 -> N thread
+scaling.
 
-traffic: HBM bytes per transport launch from two rocprofv3 --pmc runs (FETCH_SIZE, then
-WRITE_SIZE, KiB per dispatch), each scaled by what the same counter reads for a plain 1 GiB
-device copy (--copy-calibration); --summarize-traffic DIR turns the four runs into
-profiles/traffic_layered_cloud.json, which later bench runs report.
+time_to_1pct_amf_se: (a) the reference's own published setting (fixtures clear and
+layered-cloud, 8,000 photons per round, AMFConvergence(), 8,192-round cap, median of 3 seeds
+after a warm-up; validation/benchmark_amf_convergence.py:27-36) against its published Apple M4
+times; (b) the 3-D box cloud (30 x 20 x 35, tau 20, I/Q/U) with the plain, detector-directed
+and guided tracers, each to a fixed round cap with the worst relative AMF SE and the projection
+to 1 % (SE ~ rounds^-1/2).
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--amf] [--no-cpu]
-       python bench.py --copy-calibration
-       python bench.py --summarize-traffic DIR
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-amf]
+       python bench.py --layered-cloud          (the 30x20x35 layered-cloud round instead)
+       python bench.py --copy-calibration | --summarize-traffic DIR | --summarize-profile DB
 """
 
 import argparse
 import json
 import os
+import statistics
 import sys
 import time
 from pathlib import Path
@@ -39,7 +47,11 @@ for _path in (ROOT / "monki-gpu_amd", ROOT):
 
 METRIC = "photon-events/s and time-to-1%-AMF-SE, guided vs plain MC, 1×MI355X"
 HBM_PEAK_GBS = 8000.0
-PHOTONS_PER_COLUMN = 10_000
+SPECTRAL_PPC = 16
+LAYERED_PPC = 10_000
+# Published Apple M4 medians of the reference (validation/README.md:80-84).
+PUBLISHED_TIME_TO_1PCT = {"clear": {"metal": 0.466, "cpu_4_threads": 2.074},
+                          "layered-cloud": {"metal": 26.160, "cpu_4_threads": 232.839}}  # fmt: skip
 
 
 def algorithmic_bytes(counts: dict, views: int, nmat: int, nz: int) -> float:
@@ -53,233 +65,290 @@ def delta(after: dict, before: dict) -> dict:
     return {key: after[key] - before[key] for key in after}
 
 
-def cpu_baseline(scene, events_per_photon: float, target_seconds: float = 15.0) -> dict:
-    """Time the reference Fortran transport on a bounded sample of the same scene."""
+def host_threads() -> int:
+    """Threads this job may use on the host: OMP_NUM_THREADS when the environment sets it (the
+    GPU pool sets it to the job's CPU share), else the CPUs of the process's affinity mask."""
+
+    configured = os.environ.get("OMP_NUM_THREADS", "")
+    return int(configured) if configured.isdigit() and int(configured) > 0 else len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(scene, events_per_photon: float) -> dict:
+    """The reference Fortran transport on one wavelength of the spectral field: one warm-up, the
+    median of 3 runs of ``threads`` photons per column (one per thread and column), and the
+    1 -> N thread scaling of one photon per thread and column."""
 
     from oracle.oracle import REFERENCE, run_reference
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(os.cpu_count() or 1, 16)
+    threads = host_threads()
     columns = scene.nx * scene.ny
+    base = {"unit": "photon-events/s", "cores": threads, "kind": "reference",
+            "affinity_cpus": len(os.sched_getaffinity(0))}  # fmt: skip
 
     if not REFERENCE.is_file():
-        return {"value": None, "unit": "photon-events/s", "cores": threads, "kind": "reference",
-                "sample": "reference build missing (make -C oracle ref)"}  # fmt: skip
+        return {**base, "value": None, "sample": "reference build missing (make -C oracle ref)"}
 
-    # Calibrate on a short round, then size one round to about target_seconds.
-    _, seconds, _ = run_reference(scene, threads, 1, threads=threads)
-    rate = threads * columns / max(seconds[0], 1e-3)
-    ppc = max(threads, int(rate * target_seconds / columns) // threads * threads)
-    _, seconds, _ = run_reference(scene, ppc, 1, threads=threads)
-    photons_per_second = ppc * columns / seconds[0]
+    run_reference(scene, threads, 1, threads=threads)  # warm-up: page cache, OpenMP start
+    runs = [run_reference(scene, threads, 1, seed=23274 + k, threads=threads)[1][0] for k in range(3)]
+    seconds = statistics.median(runs)
+    photons_per_second = threads * columns / seconds
+    scaling = {}
+    count = 1
+
+    while count < threads:
+        one = run_reference(scene, count, 1, threads=count)[1][0]
+        scaling[str(count)] = count * columns / one
+        count *= 2
+
+    scaling[str(threads)] = photons_per_second
     return {
+        **base,
         "value": photons_per_second * events_per_photon,
-        "unit": "photon-events/s",
-        "cores": threads,
-        "kind": "reference",
         "sample": (
-            f"one round of {ppc} photons/column ({ppc * columns} photons) of the same scene; "
-            f"{seconds[0]:.2f} s transport; events/s = GPU-counted events per photon "
-            f"({events_per_photon:.2f}) x CPU photons/s ({photons_per_second:.0f})"
+            f"{scene.description or 'spectral 400 nm'}: {threads} photons/column x {columns} columns "
+            f"({threads * columns} photons) per run, transport seconds of 3 runs {[round(r, 2) for r in runs]}, "
+            f"median {seconds:.2f} s after a warm-up; events/s = GPU-counted events per photon of the same "
+            f"scene ({events_per_photon:.2f}) x CPU photons/s ({photons_per_second:.0f})"
         ),
         "photons_per_second": photons_per_second,
+        "thread_scaling_photons_per_s": scaling,
     }
 
 
-def time_to_amf_se(scene, max_rounds: int = 1024, photons_per_column: int = PHOTONS_PER_COLUMN,
-                   name: str = "layered-cloud", proposal=None, directed=None) -> dict:
-    """Wall time of simulate(..., convergence=AMFConvergence(0.01, 128, 32, 3)) to converged."""
+def spectral_cases(rounds: int):
+    from monki import SceneCase, configs
+
+    return [SceneCase(f"spectral-{i:02d}", scene, SPECTRAL_PPC, rounds)
+            for i, scene in enumerate(configs.spectral_scenes(64))]  # fmt: skip
+
+
+def spectral_line(steps: int, warmup: int) -> dict:
+    """Timed rounds of the 64-wavelength batch with the autotuned launch shape."""
+
+    from monki.execution import HIPBatch
+    from monki.execution.autotune import autotune
+
+    cases = spectral_cases(steps + warmup + 1)
+    tuned = autotune(cases, blocks=(256,), shards=(4, 16), rounds=1)
+    scene = cases[0].scene
+
+    with HIPBatch(block=tuned["block"], tally_shards=tuned["tally_shards"]) as batch:
+        batch.prepare(cases)
+
+        for index in range(1, warmup + 1):
+            batch.accumulate_round(index)
+
+        before = batch.counters()
+        device = 0.0
+        start = time.perf_counter()
+
+        for index in range(warmup + 1, warmup + steps + 1):
+            batch.accumulate_round(index)
+            device += batch.kernel_seconds()
+
+        elapsed = time.perf_counter() - start
+        counts = delta(batch.counters(), before)
+        name = batch.provenance["device"]
+
+    events = counts["atmospheric_events"] + counts["surface_events"]
+    bytes_per_round = algorithmic_bytes(counts, scene.angles, scene.nmat, scene.nz) / steps
+    device_round = device / steps
+    return {
+        "value": events / elapsed,
+        "elapsed": elapsed,
+        "device_round": device_round,
+        "bytes_per_round": bytes_per_round,
+        "counts": counts,
+        "photons": counts["photons"],
+        "events": events,
+        "device": name,
+        "autotune": {key: tuned[key] for key in ("block", "tally_shards", "seconds_per_round", "cached")},
+    }
+
+
+def single_scene_events_per_photon(scene) -> float:
+    """Events per photon of one scene on the GPU (one round at 4 photons per column)."""
+
+    from monki import SceneCase
+    from monki.execution import HIP, BatchControls
+
+    with HIP() as backend:
+        backend.prepare(SceneCase("events", scene, 4, 1))
+        backend.run_batch(BatchControls(1, 4))
+        counts = backend.counters()
+
+    return (counts["atmospheric_events"] + counts["surface_events"]) / counts["photons"]
+
+
+def layered_cloud_line(steps: int = 5, warmup: int = 1) -> dict:
+    """The 30 x 20 x 35 layered-cloud round (10,000 photons per column), one scene."""
+
+    from monki import SceneCase, configs
+    from monki.execution import HIP, BatchControls
+
+    scene = configs.layered_cloud()
+    backend = HIP()
+    backend.prepare(SceneCase("layered-cloud", scene, LAYERED_PPC, warmup + steps))
+
+    for index in range(1, warmup + 1):
+        backend.run_batch(BatchControls(index, LAYERED_PPC))
+
+    before = backend.counters()
+    kernel = 0.0
+    start = time.perf_counter()
+
+    for index in range(warmup + 1, warmup + steps + 1):
+        backend.run_batch(BatchControls(index, LAYERED_PPC))
+        kernel += backend.kernel_seconds()
+
+    elapsed = time.perf_counter() - start
+    counts = delta(backend.counters(), before)
+    backend.close()
+    events = counts["atmospheric_events"] + counts["surface_events"]
+    achieved = algorithmic_bytes(counts, scene.angles, scene.nmat, scene.nz) / kernel / 1e9
+    return {
+        "workload": "layered cloud 30x20x35, water cloud tau 10 + HG aerosol + Rayleigh/O3, I/Q/U, 3 views, "
+        f"layer paths, {LAYERED_PPC} photons/column (6e6 photons/round), plain tracer",
+        "photon_events_per_s": events / elapsed,
+        "ms_per_round": 1e3 * elapsed / steps,
+        "kernel_ms_per_round": 1e3 * kernel / steps,
+        "events_per_photon": events / counts["photons"],
+        "algorithmic_gb_per_s": achieved,
+        "tally_record_atomics_per_s": counts["tally_atomics"] / kernel,
+    }
+
+
+def reference_fixture_amf(seeds=(23274, 714103, 19088743)) -> dict:
+    """Time to 1 % AMF SE at the reference's published setting, HIP backend."""
+
+    import tempfile
+    from dataclasses import replace
+
+    from monki import AMFConvergence, ReferenceCase, simulate
+    from monki.execution import HIP
+
+    results = {}
+
+    for name, fixture in (("clear", "cloudfree"), ("layered-cloud", "1layercloud")):
+        case = ReferenceCase(name, f"testrun/MONKI_MLS_{fixture}_340_340_test.in", photons_per_column=8000,
+                             rounds=8192)  # fmt: skip
+        samples = []
+
+        with tempfile.TemporaryDirectory() as directory:
+            simulate(replace(case, rounds=2), HIP(), Path(directory) / "warm.nc")
+
+            for seed in seeds:
+                path = Path(directory) / f"{seed}.nc"
+                start = time.perf_counter()
+                simulate(case, HIP(seed=seed), path, convergence=AMFConvergence())
+                seconds = time.perf_counter() - start
+                receipt = json.loads(path.with_suffix(".run.json").read_text())
+                samples.append({"seed": seed, "seconds": seconds, "status": receipt["status"],
+                                "rounds": receipt["completed_rounds"],
+                                "max_relative_se": receipt["amf_convergence"]["checks"][-1]["max_relative_se"]})  # fmt: skip
+
+        median = statistics.median(sample["seconds"] for sample in samples)
+        published = PUBLISHED_TIME_TO_1PCT[name]
+        results[name] = {
+            "median_seconds": median,
+            "all_converged": all(sample["status"] == "converged" for sample in samples),
+            "samples": samples,
+            "published_apple_m4": published,
+            "speedup_vs_published_metal": published["metal"] / median,
+            "speedup_vs_published_cpu": published["cpu_4_threads"] / median,
+        }
+
+    results["setting"] = ("reference fixtures (1 x 1 column, SZA 45), 8000 photons/round, AMFConvergence(0.01, 128, "
+                          "32, 3), cap 8192 rounds, end to end (prepare, transport, statistics, NetCDF, receipt), "
+                          "median of 3 seeds after a warm-up")  # fmt: skip
+    return results
+
+
+def capped_amf_run(scene, name: str, rounds: int, photons_per_column: int, **backend) -> dict:
+    """simulate(..., AMFConvergence(0.01, rounds, 32, 3)) to a cap of ``rounds``: wall time, the
+    relative AMF SE of every cell at the cap, and the projected rounds and seconds to 1 %."""
 
     import tempfile
 
+    import numpy as np
+
     from monki import AMFConvergence, SceneCase, simulate
     from monki.execution import HIP
-    from monki.execution.backend import Backend
 
-    class Progress(Backend):
-        """Delegates to HIP and prints one line per group of rounds (the run takes minutes)."""
-
-        def __init__(self):
-            self.inner = HIP(proposal=proposal, directed=directed)
-            self.rounds = 0
-            self.start = time.perf_counter()
-
-        def prepare(self, case):
-            self.inner.prepare(case)
-
-        @property
-        def metadata(self):
-            return self.inner.metadata
-
-        @property
-        def provenance(self):
-            return self.inner.provenance
-
-        def run_batch(self, controls):
-            return self.run_batches([controls])[0]
-
-        def run_batches(self, controls):
-            batches = self.inner.run_batches(controls)
-            self.rounds += len(batches)
-            print(f"amf: {self.rounds} rounds, {time.perf_counter() - self.start:.1f} s", file=sys.stderr, flush=True)
-            return batches
-
-        def close(self):
-            self.inner.close()
-
-    policy = AMFConvergence(0.01, 128, 32, 3)
-    case = SceneCase(name, scene, photons_per_column, max_rounds)
+    policy = AMFConvergence(0.01, rounds, 32, 3)
+    case = SceneCase(name, scene, photons_per_column, rounds)
 
     with tempfile.TemporaryDirectory() as directory:
         start = time.perf_counter()
-        result = simulate(case, Progress(), Path(directory) / "amf.nc", convergence=policy)
+        result = simulate(case, HIP(**backend), Path(directory) / "amf.nc", convergence=policy)
         elapsed = time.perf_counter() - start
         receipt = json.loads((Path(directory) / "amf.run.json").read_text())
 
-    import numpy as np
-
-    amf = result.values["AMF_layer"]
-    relative = np.abs(result.values["std_AMF_layer"]) / np.abs(amf)
-    finite = relative[np.isfinite(relative)]
-    worst = float(finite.max()) if finite.size else None
-    quantiles = {"median": float(np.median(finite)), "p99": float(np.quantile(finite, 0.99)),
-                 "max": worst, "nonfinite": int(relative.size - finite.size)} if finite.size else None  # fmt: skip
-    result = {"seconds": elapsed, "status": receipt["status"], "rounds": receipt["completed_rounds"],
-              "policy": "AMFConvergence(0.01, minimum 128, every 32, 3 passes)", "max_relative_se": worst,
-              "relative_se": quantiles, "method": receipt.get("method", "plain")}  # fmt: skip
+    relative = np.abs(result.values["std_AMF_layer"]) / np.abs(result.values["AMF_layer"])
+    finite = np.sort(relative[np.isfinite(relative)].ravel())
+    worst = float(finite[-1])
+    record = {
+        "seconds": elapsed,
+        "status": receipt["status"],
+        "rounds": receipt["completed_rounds"],
+        "method": receipt.get("method", "plain"),
+        "max_relative_se": worst,
+        "p999_relative_se": float(finite[int(0.999 * (finite.size - 1))]),
+        "worst_decile_mean_relative_se": float(finite[-max(1, finite.size // 10):].mean()),
+        "median_relative_se": float(np.median(finite)),
+        "cells_above_5pct": int((finite > 0.05).sum()),
+        "cells": int(relative.size),
+    }
 
     if "proposal" in receipt:
-        result["proposal"] = {key: receipt["proposal"][key] for key in
+        record["training"] = {key: receipt["proposal"][key] for key in
                               ("training_events", "logging_seconds", "training_seconds", "loss_first", "loss_last")}  # fmt: skip
 
-    if worst and receipt["status"] != "converged":
-        # SE falls as rounds^-1/2: rounds and seconds the same run needs to reach 1%.
-        needed = receipt["completed_rounds"] * (worst / 0.01) ** 2
-        result["projected_rounds"] = needed
-        result["projected_seconds"] = elapsed * needed / receipt["completed_rounds"]
+    # SE falls as rounds^-1/2: rounds and seconds the same run needs for 1 % in every cell (and in
+    # 99.9 % of the cells), the fixed set-up time (training) counted once.
+    fixed = record.get("training", {}).get("logging_seconds", 0.0) + record.get("training", {}).get("training_seconds", 0.0)
+    per_round = (elapsed - fixed) / record["rounds"]
+    for key, se in (("", worst), ("_p999", record["p999_relative_se"])):
+        needed = record["rounds"] * max(se / 0.01, 1.0) ** 2
+        record[f"projected_rounds{key}"] = needed
+        record[f"projected_seconds{key}"] = fixed + per_round * needed
 
-    return result
+    return record
 
 
-def amf_race(max_rounds: int = 256, photons_per_column: int = PHOTONS_PER_COLUMN) -> dict:
-    """Plain vs guided tracer on the full-size 3-D box cloud (tau 20 over 10 x 6 columns,
-    layers 2-6 km, Mie, I/Q/U, 3 views): wall time of simulate(..., AMFConvergence(0.01, 128,
-    32, 3)) up to max_rounds, the final worst relative AMF SE, and the projection of both to 1%
-    (SE ~ rounds^-1/2). The guided time includes logging and training the proposal. A third leg
-    runs detector-directed sampling (vMF lobes toward the views, no network)."""
+def amf_race(rounds: int = 128, photons_per_column: int = 10_000) -> dict:
+    """Plain, detector-directed and guided tracers on the 3-D box cloud (30 x 20 x 35, tau 20 over
+    10 x 6 columns, layers 2-6 km, Mie, I/Q/U, 3 views) to the same round cap."""
 
     from monki import configs
     from monki.execution.proposal import DirectedSettings, ProposalSettings
 
     scene = configs.box_cloud()
-    plain = time_to_amf_se(scene, max_rounds, photons_per_column=photons_per_column, name="box-cloud")
-    guided = time_to_amf_se(scene, max_rounds, photons_per_column=photons_per_column, name="box-cloud",
-                            proposal=ProposalSettings())  # fmt: skip
-    directed = time_to_amf_se(scene, max_rounds, photons_per_column=photons_per_column, name="box-cloud",
-                              directed=DirectedSettings())  # fmt: skip
-    result = {"plain": plain, "guided": guided, "directed": directed,
-              "workload": "box cloud 30x20x35, tau 20, 10x6 columns x layers 2-6 km"}  # fmt: skip
+    legs = {
+        "plain": capped_amf_run(scene, "box-cloud", rounds, photons_per_column),
+        "directed": capped_amf_run(scene, "box-cloud", rounds, photons_per_column, directed=DirectedSettings()),
+        "guided": capped_amf_run(scene, "box-cloud", rounds, photons_per_column, proposal=ProposalSettings()),
+    }
+    plain = legs["plain"]
+    result = {"workload": "box cloud 30x20x35, tau 20 over 10x6 columns x layers 2-6 km, Mie r_eff 10 um, I/Q/U, "
+                          f"3 views, {photons_per_column} photons/column, cap {rounds} rounds",  # fmt: skip
+              **legs}
 
-    if plain.get("projected_seconds") and guided.get("projected_seconds"):
-        result["speedup_to_1pct"] = plain["projected_seconds"] / guided["projected_seconds"]
+    for name in ("directed", "guided"):
+        leg = legs[name]
+        result[f"{name}_speedup_to_1pct"] = plain["projected_seconds"] / leg["projected_seconds"]
+        result[f"{name}_speedup_to_1pct_p999"] = plain["projected_seconds_p999"] / leg["projected_seconds_p999"]
+        # Per-photon variance at equal rounds, worst decile of cells.
+        result[f"{name}_worst_decile_variance_ratio"] = (plain["worst_decile_mean_relative_se"] /
+                                                         leg["worst_decile_mean_relative_se"]) ** 2  # fmt: skip
 
-    if plain.get("relative_se") and guided.get("relative_se"):
-        # Per-photon variance ratio at equal photon counts, from the median relative AMF SE.
-        result["variance_ratio_median"] = (guided["relative_se"]["median"] / plain["relative_se"]["median"]) ** 2
-
-    if plain.get("projected_seconds") and directed.get("projected_seconds"):
-        result["directed_speedup_to_1pct"] = plain["projected_seconds"] / directed["projected_seconds"]
-
+    result["speedup_to_1pct"] = result["guided_speedup_to_1pct"]
     return result
 
 
-def batch_rounds(rounds: int = 5, photons_per_column: int = 1_000) -> dict:
-    """The 64-scene box-cloud batch (seed 2026: SZA U(0, 70), albedo U(0.02, 0.3), cloud optical
-    depth lognormal(15, 0.5), grids 16-30 x 12-20 columns) traced one round per graph replay,
-    with block and tally shards from the autotuner."""
-
-    from monki import SceneCase, configs
-    from monki.execution import HIPBatch
-    from monki.execution.autotune import autotune
-
-    cases = [SceneCase(f"box{i}", scene, photons_per_column, rounds + 1) for i, scene in
-             enumerate(configs.box_cloud_batch(64, 2026))]  # fmt: skip
-    tuned = autotune(cases, rounds=2)
-
-    with HIPBatch(block=tuned["block"], tally_shards=tuned["tally_shards"]) as batch:
-        batch.prepare(cases)
-        batch.run_round(1)
-        before = batch.counters()
-        device = 0.0
-        start = time.perf_counter()
-
-        for index in range(2, rounds + 2):
-            batch.run_round(index)
-            device += batch.kernel_seconds()
-
-        elapsed = time.perf_counter() - start
-        counts = delta(batch.counters(), before)
-
-    events = counts["atmospheric_events"] + counts["surface_events"]
-    return {
-        "workload": "64 box-cloud scenes (seed 2026), I/Q/U, 3 views, layer paths, "
-        f"{photons_per_column} photons/column, one graph replay per round",
-        "photons_per_round": counts["photons"] // rounds,
-        "photon_events_per_s": events / elapsed,
-        "ms_per_round": 1e3 * elapsed / rounds,
-        "device_ms_per_round": 1e3 * device / rounds,
-        "autotune": {key: tuned[key] for key in ("block", "tally_shards", "seconds_per_round", "cached")},
-        "autotune_table": tuned["table"],
-    }
-
-
-def spectral_rounds(rounds: int = 2, photons_per_column: int = 16) -> dict:
-    """The spectral case: 64 wavelengths (300-500 nm) of one 128 x 128 x 60 stochastic cloud
-    field (Kolmogorov spectrum, cover 0.6, mean cloudy optical depth 12, cloud 1-4 km), I/Q/U,
-    3 views, layer paths; one graph replay per round, launch shape from the autotuner."""
-
-    from monki import SceneCase, configs
-    from monki.execution import HIPBatch
-    from monki.execution.autotune import autotune
-
-    cases = [SceneCase(f"spectral-{i:02d}", scene, photons_per_column, 2 * rounds + 2)
-             for i, scene in enumerate(configs.spectral_scenes(64))]  # fmt: skip
-    tuned = autotune(cases, blocks=(128, 256), shards=(4, 16), rounds=1)
-
-    with HIPBatch(block=tuned["block"], tally_shards=tuned["tally_shards"]) as batch:
-        batch.prepare(cases)
-        # Rounds copied whole to the host (run_round), then rounds whose layer paths stay on the
-        # device (accumulate_round, what simulate_batch uses); one warm-up round each.
-        timings = {}
-
-        for label, run, first in (("host", batch.run_round, 1), ("device", batch.accumulate_round, rounds + 2)):
-            run(first)
-            before = batch.counters()
-            device = 0.0
-            start = time.perf_counter()
-
-            for index in range(first + 1, first + rounds + 1):
-                run(index)
-                device += batch.kernel_seconds()
-
-            timings[label] = (time.perf_counter() - start, device, delta(batch.counters(), before))
-
-    elapsed, device, counts = timings["device"]
-    events = counts["atmospheric_events"] + counts["surface_events"]
-    return {
-        "workload": f"spectral 64 x (128 x 128 x 60), {photons_per_column} photons/column, one graph replay per round",
-        "photons_per_round": counts["photons"] // rounds,
-        "events_per_photon": events / max(counts["photons"], 1),
-        "photon_events_per_s": events / elapsed,
-        "ms_per_round": 1e3 * elapsed / rounds,
-        "device_ms_per_round": 1e3 * device / rounds,
-        "host_copy_ms_per_round": 1e3 * timings["host"][0] / rounds,
-        "round_path": "layer paths, weights and AMF moments accumulated on the device; Stokes copied per round",
-        "autotune": {key: tuned[key] for key in ("block", "tally_shards", "seconds_per_round", "cached")},
-    }
-
-
 COPY_BYTES = 1 << 30
-TRAFFIC_FILE = ROOT / "profiles" / "traffic_layered_cloud.json"
+TRAFFIC_FILES = {"spectral": ROOT / "profiles" / "traffic_spectral_field.json",
+                 "layered-cloud": ROOT / "profiles" / "traffic_layered_cloud.json"}  # fmt: skip
 
 
 def copy_calibration() -> None:
@@ -318,7 +387,7 @@ def _counter_values(directory: Path, name: str, match) -> list:
     return [float(value) for kernel, value in rows if match(kernel)]
 
 
-def summarize_traffic(directory: Path) -> dict:
+def summarize_traffic(directory: Path, workload: str) -> dict:
     """HBM bytes per transport launch from the four --pmc runs written under ``directory``.
 
     Expected runs (rocprofv3 -d DIR -o NAME): transport_fetch, transport_write, copy_fetch and
@@ -338,7 +407,9 @@ def summarize_traffic(directory: Path) -> dict:
     write_factor = COPY_BYTES / (copy_write * 1024.0)
     fetch_bytes = sum(fetch) / len(fetch) * 1024.0 * fetch_factor
     write_bytes = sum(write) / len(write) * 1024.0 * write_factor
+    flag = "" if workload == "spectral" else " --layered-cloud"
     summary = {
+        "workload": workload,
         "hbm_bytes_per_launch": fetch_bytes + write_bytes,
         "fetch_bytes_per_launch": fetch_bytes,
         "write_bytes_per_launch": write_bytes,
@@ -347,14 +418,14 @@ def summarize_traffic(directory: Path) -> dict:
         "transport_dispatches": [len(fetch), len(write)],
         "copy_kib": [copy_fetch, copy_write],
         "commands": [
-            "rocprofv3 --pmc FETCH_SIZE -d DIR -o transport_fetch -- python3 bench.py --steps 2 --warmup 0 --no-cpu",
-            "rocprofv3 --pmc WRITE_SIZE -d DIR -o transport_write -- python3 bench.py --steps 2 --warmup 0 --no-cpu",
+            f"rocprofv3 --pmc FETCH_SIZE -d DIR -o transport_fetch -- python3 bench.py --steps 2 --warmup 0 --no-cpu --no-amf{flag}",
+            f"rocprofv3 --pmc WRITE_SIZE -d DIR -o transport_write -- python3 bench.py --steps 2 --warmup 0 --no-cpu --no-amf{flag}",
             "rocprofv3 --pmc FETCH_SIZE -d DIR -o copy_fetch -- python3 bench.py --copy-calibration",
             "rocprofv3 --pmc WRITE_SIZE -d DIR -o copy_write -- python3 bench.py --copy-calibration",
-            "python3 bench.py --summarize-traffic DIR",
+            f"python3 bench.py --summarize-traffic DIR{flag}",
         ],
     }
-    TRAFFIC_FILE.write_text(json.dumps(summary, indent=1) + "\n")
+    TRAFFIC_FILES[workload].write_text(json.dumps(summary, indent=1) + "\n")
     return summary
 
 
@@ -391,15 +462,13 @@ def main() -> int:
     parser.add_argument("--gpus", type=int, default=1)
     parser.add_argument("--steps", type=int, default=20)
     parser.add_argument("--warmup", type=int, default=2)
-    parser.add_argument("--amf", action="store_true", help="also time AMF convergence at 1%%")
     parser.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    parser.add_argument("--no-amf", action="store_true", help="skip the time-to-1%% AMF SE legs")
+    parser.add_argument("--amf-rounds", type=int, default=128, help="round cap of the box-cloud race")
+    parser.add_argument("--layered-cloud", action="store_true", help="time the layered-cloud round instead")
     parser.add_argument("--copy-calibration", action="store_true", help="run the copy kernel only")
     parser.add_argument("--summarize-traffic", type=Path, metavar="DIR")
     parser.add_argument("--summarize-profile", type=Path, metavar="DB")
-    parser.add_argument("--batch", action="store_true", help="also time the 64-scene box-cloud batch")
-    parser.add_argument("--spectral", action="store_true", help="also time the 64-wavelength 128x128x60 field")
-    parser.add_argument("--guided", type=int, metavar="ROUNDS", default=0,
-                        help="also race plain vs guided to 1%% AMF SE on the box cloud (round cap)")
     args = parser.parse_args()
 
     if args.summarize_profile:
@@ -410,8 +479,10 @@ def main() -> int:
         copy_calibration()
         return 0
 
+    workload = "layered-cloud" if args.layered_cloud else "spectral"
+
     if args.summarize_traffic:
-        print(json.dumps(summarize_traffic(args.summarize_traffic)), flush=True)
+        print(json.dumps(summarize_traffic(args.summarize_traffic, workload)), flush=True)
         return 0
 
     if args.gpus != 1:
@@ -423,61 +494,43 @@ def main() -> int:
         )
         return 2
 
-    from monki import SceneCase, configs
-    from monki.execution import HIP, BatchControls
-
-    scene = configs.layered_cloud()
-    backend = HIP()
-    backend.prepare(SceneCase("layered-cloud", scene, PHOTONS_PER_COLUMN, args.warmup + args.steps))
-    requests = [BatchControls(i, PHOTONS_PER_COLUMN) for i in range(1, args.warmup + args.steps + 1)]
-
-    for request in requests[: args.warmup]:
-        backend.run_batch(request)
-
-    before = backend.counters()
-    kernel_seconds = 0.0
-    start = time.perf_counter()
-
-    for request in requests[args.warmup :]:
-        backend.run_batch(request)
-        kernel_seconds += backend.kernel_seconds()
-
-    elapsed = time.perf_counter() - start
-    counts = delta(backend.counters(), before)
-    device = backend.provenance["device"]
-    backend.close()
-
-    events = counts["atmospheric_events"] + counts["surface_events"]
-    photons = counts["photons"]
-    bytes_per_launch = algorithmic_bytes(counts, scene.angles, scene.nmat, scene.nz) / args.steps
-    kernel_average = kernel_seconds / args.steps
-    achieved = bytes_per_launch / kernel_average / 1e9
     traffic = None
 
-    if TRAFFIC_FILE.is_file():
-        traffic = json.loads(TRAFFIC_FILE.read_text()).get("hbm_bytes_per_launch")
+    if TRAFFIC_FILES[workload].is_file():
+        traffic = json.loads(TRAFFIC_FILES[workload].read_text()).get("hbm_bytes_per_launch")
 
+    if args.layered_cloud:
+        leg = layered_cloud_line(args.steps, args.warmup)
+        print(json.dumps(leg), flush=True)
+        return 0
+
+    print(f"bench: spectral batch, {args.warmup} warm-up + {args.steps} timed rounds", file=sys.stderr, flush=True)
+    run = spectral_line(args.steps, args.warmup)
+    achieved = run["bytes_per_round"] / run["device_round"] / 1e9
+    atomics_rate = run["counts"]["tally_atomics"] / (run["device_round"] * args.steps)
     line = {
         "metric": METRIC,
-        "value": events / elapsed,
+        "value": run["value"],
         "unit": "photon-events/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * elapsed / args.steps,
+        "ms_per_step": 1e3 * run["elapsed"] / args.steps,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": "layered-cloud plain tracer: 30x20 columns x 35 layers, water cloud tau 10 "
-            "(Mie r_eff 10 um) + HG aerosol + Rayleigh/O3, I/Q/U, 3 views, layer AMF tallies, "
-            "10000 photons/column (6e6 photons/round)",
-            "photons_per_step": photons // args.steps,
-            "events_per_photon": events / photons,
-            "kernel_ms_per_step": 1e3 * kernel_average,
-            "device": device,
+            "workload": "spectral stochastic cloud field, plain tracer: 64 wavelengths (300-500 nm) x 128x128 "
+            "columns x 60 layers, lognormal Kolmogorov cloud 1-4 km (cover 0.6, mean tau 12), Rayleigh "
+            f"lambda^-4.08 + synthetic O3, I/Q/U, 3 views, layer AMF tallies, {SPECTRAL_PPC} photons/column "
+            "(16.8e6 photons per round), one graph-replayed batch launch per round",
+            "photons_per_step": run["photons"] // args.steps,
+            "events_per_photon": run["events"] / run["photons"],
+            "device_ms_per_step": 1e3 * run["device_round"],
+            "device": run["device"],
+            "autotune": run["autotune"],
         },
         "roofline": {
             "bound": "hbm",
@@ -486,23 +539,31 @@ def main() -> int:
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
+            "basis": "algorithmic bytes per round / device time per round (HIP events around the replayed round)",
+            "measured_hbm_frac": traffic / run["device_round"] / 1e9 / HBM_PEAK_GBS if traffic else None,
+            "binding": {"kind": "tally record atomics (one wave-wide atomic instruction per event, view and record)",
+                        "record_atomics_per_s": atomics_rate,
+                        "record_atomics_per_event": run["counts"]["tally_atomics"] / run["events"]},  # fmt: skip
         },
     }
 
+    if not args.no_amf:
+        print("bench: time to 1% AMF SE, reference fixtures", file=sys.stderr, flush=True)
+        fixtures = reference_fixture_amf()
+        print("bench: box-cloud race, plain / directed / guided", file=sys.stderr, flush=True)
+        race = amf_race(args.amf_rounds)
+        line["time_to_1pct_amf_se"] = {"reference_fixtures": fixtures, "box_cloud_guided_vs_plain": race}
+        line["guided_vs_plain_speedup_to_1pct"] = race["speedup_to_1pct"]
+
+    print("bench: layered-cloud round", file=sys.stderr, flush=True)
+    line["layered_cloud"] = layered_cloud_line()
+
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(scene, events / photons)
+        from monki import configs
 
-    if args.amf:
-        line["time_to_1pct_amf_se"] = time_to_amf_se(scene)
-
-    if args.guided:
-        line["guided_vs_plain"] = amf_race(args.guided)
-
-    if args.batch:
-        line["batch_box_clouds"] = batch_rounds()
-
-    if args.spectral:
-        line["spectral_field"] = spectral_rounds()
+        print("bench: CPU baseline (reference Fortran)", file=sys.stderr, flush=True)
+        scene = configs.spectral_cloud_field(400.0, configs.stochastic_cloud_field(128, seed=7))
+        line["cpu_baseline"] = cpu_baseline(scene, single_scene_events_per_photon(scene))
 
     print(json.dumps(line), flush=True)
     return 0

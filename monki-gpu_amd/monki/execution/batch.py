@@ -247,6 +247,8 @@ class HIPBatch:
                 error, len(error),
             )  # fmt: skip
             fraction = settings.lobe_fraction if lobes >= 0 else 0.0
+            # Without a tabulated species the guided tracer has no view lobes, at the surface too.
+            settings = replace(settings, surface=settings.surface and lobes >= 0)
             self.provenance["method"] = "guided (view lobes + learned vMF mixture, defensive physical sampler)"
         else:
             status = native.monki_hip_set_directed(

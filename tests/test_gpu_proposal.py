@@ -137,6 +137,21 @@ def test_logged_samples_are_well_formed():
     assert features[:, 11].min() == pytest.approx(1 / 32 - 1)
 
 
+def plain_reference(directory, scene, rounds, ppc):
+    """The plain tracer with 8x the photons of the run under test. The plain estimator of the
+    forward-peaked box cloud is heavy-tailed (rare, huge local-estimate contributions): at the
+    test's own budget its typical AMF lies low and its SE is underestimated, so a steered run with
+    lighter tails compared against an equal-budget plain run shows chi-square excess that is the
+    plain run's, not a bias (measured: plain at 1x against plain at 8x, AMF chi2/dof 1.24 and mean
+    z -0.27; steered at 1x against plain at 8x, 1.06-1.2 and |mean z| < 0.12)."""
+
+    from monki import AMFConvergence, SceneCase, simulate
+    from monki.execution import HIP
+
+    case = SceneCase("box", scene, 8 * ppc, rounds)
+    return simulate(case, HIP(seed=5), directory / "plain.nc", convergence=AMFConvergence(1e-9, rounds, 16, 3))
+
+
 GUIDANCE = {
     # Every kind and order, the surface too, view lobes + network, weight window on (defaults).
     "defaults": dict(),
@@ -169,7 +184,7 @@ def test_guided_tracer_agrees_with_plain_tracer(tmp_path, guidance):
     case = SceneCase("box", scene, ppc, rounds)
     # A precision target no run reaches: every round runs and the AMF errors are reported.
     policy = AMFConvergence(1e-9, rounds, 16, 3)
-    plain = simulate(case, HIP(seed=5), tmp_path / "plain.nc", convergence=policy)
+    plain = plain_reference(tmp_path, scene, rounds, ppc)
     guided = simulate(case, HIP(seed=6, proposal=settings), tmp_path / "guided.nc", convergence=policy)
     report = {}
 
@@ -206,7 +221,7 @@ def test_directed_tracer_agrees_with_plain_tracer(tmp_path, configuration):
     rounds, ppc = 48, 200
     case = SceneCase("box", scene, ppc, rounds)
     policy = AMFConvergence(1e-9, rounds, 16, 3)
-    plain = simulate(case, HIP(seed=5), tmp_path / "plain.nc", convergence=policy)
+    plain = plain_reference(tmp_path, scene, rounds, ppc)
     settings = DirectedSettings(**DIRECTED[configuration])
     backend = HIP(seed=6, directed=settings)
     directed = simulate(case, backend, tmp_path / "directed.nc", convergence=policy)

@@ -47,6 +47,19 @@ void *monki_hip_create(uint32_t device, const float *tables, size_t table_count,
 int monki_hip_run(void *handle, const uint32_t *controls, double *out, char *error,
                   size_t error_size);
 
+/* monki_hip_create + monki_hip_run of one round group in one call that allocates nothing: the
+ * packed tables (host, as for monki_hip_create) and controls in, the float64 tallies out in
+ * monki_hip_run's order. No reference counterpart: for callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that forbid allocation).
+ * Everything besides the caller's buffers lives in a 64 MiB device array of the library,
+ * one per device for the life of the process; a round group needing more (tables + float32
+ * banks of all shards and rounds + float64 reduced and output tallies) is rejected with an
+ * error. Runs on the device's null stream and returns when done. Calls that overlap on one
+ * device race on that array; calls on different devices do not. Returns 0, or nonzero with
+ * error text. */
+int monki_hip_run_static(uint32_t device, const float *tables, size_t table_count, const uint32_t *controls,
+                         double *out, char *error, size_t error_size);
+
 /* Same as monki_hip_run, with transport arithmetic in float64 on the same Philox streams.
  * No reference counterpart: the FP64 build the FP32 error is measured against. */
 int monki_hip_run_fp64(void *handle, const uint32_t *controls, double *out, char *error,

@@ -2627,6 +2627,207 @@ int batch_round(Session *h, uint32_t round, uint32_t seed, uint32_t max_steps, c
   return 0;
 }
 
+// ---- Rounds without allocation -------------------------------------------------------------
+// monki_hip_run_static keeps everything a round group needs besides the caller's tables and
+// output in one file-scope device array, `scratch`: scene constants, controls, flags and
+// counters, the uploaded tables, CDF bucket tables, float32 tally banks, and the float64
+// reduced and assembled tallies. Every device that the library runs on holds its own copy for
+// the life of the process. Its address is looked up once per device with hipGetSymbolAddress
+// (the array's name in host code is not a device address) and kept in static_scratch_at.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+constexpr int kScratchDevices = 64;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+void *static_scratch_at[kScratchDevices] = {nullptr};
+int static_compute_units[kScratchDevices] = {0};
+
+// Address of this device's scratch (the device must be current); null and error on failure.
+unsigned char *static_scratch(int device, char *error, size_t error_size) {
+  if (device < 0 || device >= kScratchDevices) {
+    set_error(error, error_size, "Device index beyond the static scratch table");
+    return nullptr;
+  }
+  if (!static_scratch_at[device]) {
+    void *address = nullptr;
+    if (!check(hipGetSymbolAddress(&address, HIP_SYMBOL(scratch)), error, error_size, "static scratch address"))
+      return nullptr;
+    int units = 0;
+    if (!check(hipDeviceGetAttribute(&units, hipDeviceAttributeMultiprocessorCount, device), error, error_size,
+               "compute units"))
+      return nullptr;
+    static_compute_units[device] = units;
+    static_scratch_at[device] = address;
+  }
+  return static_cast<unsigned char *>(static_scratch_at[device]);
+}
+
+// The plain tracer's instantiation for a scene's tables (the choice monki_hip_run makes).
+int plain_mode(const Scene &s) {
+  if (s.detector_surface) return s.nmat == 4 ? kSurface4 : kSurface;
+  if (s.scats && s.backward) return s.nmat == 4 ? kBackScats4 : kBackScats;
+  if (s.scats) return s.nmat == 4 ? kScats4 : kScats;
+  if (s.backward) return s.nmat == 4 ? kBackward4 : kBackward;
+  return s.nmat == 4 ? kStokes4 : kPlain;
+}
+
+struct StaticRound {
+  const Scene *host;  // scene with device pointers into scratch
+  Scene *scene;
+  Controls *controls;
+  unsigned *flags;  // [failure, work]
+  unsigned long long *counters;
+  float *banks;
+  double *reduced, *assembled;
+  uint32_t rounds, photons;
+  int compute_units;
+};
+
+template <int MODE>
+bool static_enqueue(const StaticRound &r, char *error, size_t error_size) {
+  const Scene &s = *r.host;
+  const int block = kBlock32;
+  const int row = s.nz | 1;
+  const size_t lds = lds_bytes_for<float>(row, block, full_stokes(MODE));
+  int resident = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<float, false, MODE>, block, lds) !=
+          hipSuccess ||
+      resident < 1)
+    resident = 1;
+  const uint64_t wanted = (uint64_t(r.photons) + block - 1) / block;
+  const uint64_t filled = uint64_t(resident) * uint64_t(r.compute_units > 0 ? r.compute_units : 1);
+  const uint32_t grid = uint32_t(wanted < filled ? wanted : filled);
+  const size_t bank_values = size_t(s.bank_count) * s.shards * r.rounds;
+  launch_fill(reinterpret_cast<uint32_t *>(r.banks), bank_values, 0);
+  launch_fill(r.flags, 2, 0);
+  launch_fill(reinterpret_cast<uint32_t *>(r.counters), 2 * kCounters, 0);
+  if (grid) {
+    hipLaunchKernelGGL((transport_kernel<float, false, MODE>), dim3(grid), dim3(block), lds, 0, r.scene, r.controls,
+                       1u, row, r.banks, r.flags, r.flags + 1, r.counters, nullptr, 0u,
+                       monki_net::Net{nullptr, nullptr, nullptr},
+                       monki_net::EventLog{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0},
+                       unsteered(), nullptr, 0);
+  }
+  const size_t summed = size_t(s.bank_count) * r.rounds;
+  hipLaunchKernelGGL(reduce_shards<float>, dim3(unsigned((summed + 255) / 256)), dim3(256), 0, 0, r.banks, r.reduced,
+                     s.bank_count, s.shards, int(r.rounds));
+  hipLaunchKernelGGL(assemble_tallies, dim3(unsigned((summed + 255) / 256)), dim3(256), 0, 0, s, r.reduced,
+                     r.assembled, int(r.rounds));
+  return check(hipGetLastError(), error, error_size, "static round launch");
+}
+
+size_t aligned(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+int static_run(uint32_t device, const float *tables, size_t table_count, const uint32_t *controls, double *out,
+               char *error, size_t error_size) {
+  if (!tables || !controls || !out) {
+    set_error(error, error_size, "Invalid static run arguments");
+    return 1;
+  }
+  Scene parsed{};
+  if (!parse_tables(tables, table_count, nullptr, parsed, error, error_size)) return 1;
+  if (parsed.detector_surface && (parsed.backward || parsed.scats)) {
+    set_error(error, error_size, "A ground detector runs forward without the scattering decomposition");
+    return 1;
+  }
+  const Controls c{controls[0], controls[1], controls[2], controls[3],
+                   controls[4], controls[5], controls[6], controls[7]};
+  if (!c.per_round || !c.rounds || !c.max_steps || c.count > (1u << 31) ||
+      uint64_t(c.first) + c.count > uint64_t(c.per_round) * c.rounds) {
+    set_error(error, error_size, "Invalid HIP round group");
+    return 1;
+  }
+  // Scratch layout, 256-byte aligned parts.
+  const size_t at_scene = 0;
+  const size_t at_controls = at_scene + aligned(sizeof(Scene));
+  const size_t at_flags = at_controls + aligned(sizeof(Controls));
+  const size_t at_counters = at_flags + aligned(2 * sizeof(unsigned));
+  const size_t at_tables = at_counters + aligned(kCounters * sizeof(unsigned long long));
+  const size_t at_buckets = at_tables + aligned(table_count * sizeof(float));
+  const size_t at_banks = at_buckets + aligned(size_t(kMaxSpecies) * (kCdfBuckets + 1) * sizeof(uint16_t));
+  const size_t at_reduced = at_banks + aligned(size_t(parsed.bank_count) * parsed.shards * c.rounds * sizeof(float));
+  const size_t at_assembled = at_reduced + aligned(size_t(parsed.bank_count) * c.rounds * sizeof(double));
+  const size_t end = at_assembled + aligned(size_t(parsed.tally_count) * c.rounds * sizeof(double));
+  if (end > kScratchBytes) {
+    char detail[160];
+    snprintf(detail, sizeof(detail), "%zu bytes of device scratch needed, %zu available", end, kScratchBytes);
+    set_error(error, error_size, "Round group too large for the static scratch: %s (fewer rounds or shards)",
+              detail);
+    return 1;
+  }
+  if (!check(hipSetDevice(int(device)), error, error_size, "hipSetDevice")) return 1;
+  unsigned char *base = static_scratch(int(device), error, error_size);
+  if (!base) return 1;
+  // Scene view with device pointers into the uploaded tables, CDF buckets built here.
+  const float *device_tables = reinterpret_cast<const float *>(base + at_tables);
+  Scene view{};
+  parse_tables(tables, table_count, device_tables, view, nullptr, 0);
+  uint16_t buckets[kMaxSpecies][kCdfBuckets + 1];
+  for (int sp = 0; sp < view.species; ++sp) {
+    const float *cdf = tables + (parsed.sp[sp].cdf - static_cast<const float *>(nullptr));
+    const int n = parsed.sp[sp].lut_size;
+    int low = 0;
+    for (int b = 0; b <= kCdfBuckets; ++b) {
+      const float edge = float(b) / float(kCdfBuckets);
+      while (low < n - 2 && cdf[low + 1] <= edge) ++low;
+      buckets[sp][b] = uint16_t(low);
+    }
+    view.sp[sp].cdf_start = reinterpret_cast<const uint16_t *>(base + at_buckets) + size_t(sp) * (kCdfBuckets + 1);
+  }
+  if (!check(hipMemcpy(base + at_tables, tables, table_count * sizeof(float), hipMemcpyHostToDevice), error,
+             error_size, "upload tables") ||
+      !check(hipMemcpy(base + at_buckets, buckets, sizeof(buckets), hipMemcpyHostToDevice), error, error_size,
+             "upload buckets") ||
+      !check(hipMemcpy(base + at_scene, &view, sizeof(Scene), hipMemcpyHostToDevice), error, error_size,
+             "upload scene") ||
+      !check(hipMemcpy(base + at_controls, &c, sizeof(Controls), hipMemcpyHostToDevice), error, error_size,
+             "upload controls"))
+    return 1;
+  const StaticRound r{&view,
+                      reinterpret_cast<Scene *>(base + at_scene),
+                      reinterpret_cast<Controls *>(base + at_controls),
+                      reinterpret_cast<unsigned *>(base + at_flags),
+                      reinterpret_cast<unsigned long long *>(base + at_counters),
+                      reinterpret_cast<float *>(base + at_banks),
+                      reinterpret_cast<double *>(base + at_reduced),
+                      reinterpret_cast<double *>(base + at_assembled),
+                      c.rounds,
+                      c.count,
+                      static_compute_units[device]};
+  bool queued = false;
+  switch (plain_mode(view)) {
+    case kSurface4: queued = static_enqueue<kSurface4>(r, error, error_size); break;
+    case kSurface: queued = static_enqueue<kSurface>(r, error, error_size); break;
+    case kBackScats4: queued = static_enqueue<kBackScats4>(r, error, error_size); break;
+    case kBackScats: queued = static_enqueue<kBackScats>(r, error, error_size); break;
+    case kScats4: queued = static_enqueue<kScats4>(r, error, error_size); break;
+    case kScats: queued = static_enqueue<kScats>(r, error, error_size); break;
+    case kBackward4: queued = static_enqueue<kBackward4>(r, error, error_size); break;
+    case kBackward: queued = static_enqueue<kBackward>(r, error, error_size); break;
+    case kStokes4: queued = static_enqueue<kStokes4>(r, error, error_size); break;
+    default: queued = static_enqueue<kPlain>(r, error, error_size); break;
+  }
+  if (!queued) return 1;
+  unsigned flag = 0;
+  if (!check(hipMemcpy(&flag, base + at_flags, sizeof(unsigned), hipMemcpyDeviceToHost), error, error_size,
+             "copy failure flag"))
+    return 1;
+  if (flag) {
+    char detail[32];
+    snprintf(detail, sizeof(detail), "%u", flag);
+    set_error(error, error_size, "HIP transport failed (flag %s); round rejected", detail);
+    return 1;
+  }
+  return check(hipMemcpy(out, base + at_assembled, size_t(parsed.tally_count) * c.rounds * sizeof(double),
+                         hipMemcpyDeviceToHost),
+               error, error_size, "copy tallies")
+             ? 0
+             : 1;
+}
+
+}  // namespace
+
 namespace {
 
 // Captured batch rounds bake in the transport instantiation and guidance: drop them when the
@@ -2722,6 +2923,11 @@ int monki_hip_batch_moments(void *handle, uint32_t scene, double *out, size_t co
 int monki_hip_run(void *handle, const uint32_t *controls, double *out, char *error, size_t error_size) {
   return launch<float, false>(static_cast<Session *>(handle), controls, out, nullptr, 0, error,
                               error_size);
+}
+
+int monki_hip_run_static(uint32_t device, const float *tables, size_t table_count, const uint32_t *controls,
+                         double *out, char *error, size_t error_size) {
+  return static_run(device, tables, table_count, controls, out, error, error_size);
 }
 
 int monki_hip_run_fp64(void *handle, const uint32_t *controls, double *out, char *error,

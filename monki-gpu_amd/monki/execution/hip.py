@@ -78,6 +78,11 @@ def load_library(path: Path = LIBRARY):
     ]
     native.monki_hip_create.restype = ctypes.c_void_p
 
+    native.monki_hip_run_static.argtypes = [
+        ctypes.c_uint32, FLOAT_POINTER, ctypes.c_size_t, UINT_POINTER, DOUBLE_POINTER, ctypes.c_char_p, ctypes.c_size_t,
+    ]  # fmt: skip
+    native.monki_hip_run_static.restype = ctypes.c_int
+
     for name in ("monki_hip_run", "monki_hip_run_fp64"):
         function = getattr(native, name)
         function.argtypes = [ctypes.c_void_p, UINT_POINTER, DOUBLE_POINTER, ctypes.c_char_p, ctypes.c_size_t]

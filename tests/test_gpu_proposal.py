@@ -238,3 +238,39 @@ def test_directed_tracer_agrees_with_plain_tracer(tmp_path, configuration):
     for name, stats in report.items():
         assert stats["max_abs_z"] < 5.5, name
         assert 0.6 < stats["chi_square_per_dof"] < 1.5, name
+
+
+def _relative_amf_se(result):
+    relative = np.abs(result.values["std_AMF_layer"]) / np.abs(result.values["AMF_layer"])
+    return np.sort(relative[np.isfinite(relative)].ravel())
+
+
+def test_steered_tracers_reduce_the_worst_cells_variance(tmp_path):
+    """What steering gains over the plain tracer at equal photons on the 30 x 20 box cloud (tau 20,
+    Mie, I/Q/U, 3 views): the mean relative AMF SE of the worst decile of (pixel, layer, view)
+    cells, squared, is the per-photon variance. Measured at 64 rounds x 10,000 photons per
+    column: plain 0.038, detector-directed 0.0095-0.013, guided 0.011; the test asks for a
+    variance ratio above 3 at a fifth of that budget."""
+
+    from monki import AMFConvergence, SceneCase, configs, simulate
+    from monki.execution import DirectedSettings, HIP, ProposalSettings
+
+    scene = configs.box_cloud()
+    rounds, ppc = 64, 2000
+    case = SceneCase("box", scene, ppc, rounds)
+    policy = AMFConvergence(1e-9, rounds, 32, 3)
+    legs = {
+        "plain": HIP(seed=5),
+        "directed": HIP(seed=6, directed=DirectedSettings()),
+        "guided": HIP(seed=7, proposal=ProposalSettings(events=1_000_000, steps=500)),
+    }
+    worst = {}
+
+    for name, backend in legs.items():
+        relative = _relative_amf_se(simulate(case, backend, tmp_path / f"{name}.nc", convergence=policy))
+        worst[name] = float(relative[-relative.size // 10 :].mean())
+
+    ratios = {name: (worst["plain"] / worst[name]) ** 2 for name in ("directed", "guided")}
+    print(json.dumps({"worst_decile_mean_relative_se": worst, "variance_ratio": ratios}))
+    assert ratios["directed"] > 3.0
+    assert ratios["guided"] > 3.0

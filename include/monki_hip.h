@@ -208,15 +208,6 @@ int monki_hip_set_directed(void *handle, float alpha, float kappa, uint32_t spec
 int monki_hip_run_directed(void *handle, const uint32_t *controls, double *out, char *error,
                            size_t error_size);
 
-/* Wavefront-sorted photon queues for the plain float32 tracer (single-scene and batch sessions;
- * kernels/metal/transport.metal:339-458 runs one thread per photon without regrouping): every
- * `every` iterations of the persistent loop the waves of a block meet and, when they all work on
- * the same scene, regroup their photons by the work their next flight needs (at or below the
- * scene's top varying layer, above it, idle). 0 (the default) turns it off. Histories do not
- * change: a photon's random stream follows its id, not its lane. No reference counterpart.
- * Returns 0, or nonzero with error text. */
-int monki_hip_set_sort(void *handle, uint32_t every, char *error, size_t error_size);
-
 /* Steering options of the guided and detector-directed tracers (both single-scene and batch
  * sessions; a batch session traces its rounds in the mode last set by monki_hip_attach_proposal
  * or monki_hip_set_directed). No reference counterpart.

@@ -1715,201 +1715,6 @@ __global__ void __launch_bounds__(kBlock32)
   }
 }
 
-// Wavefront-sorted photon queues: the plain float32 tracer of transport_kernel (its own kernel, so
-// the other tracers keep their code) with a regrouping of photons every `sort_every` iterations.
-// The waves of a block then meet; when they all work on the same scene, the block's photons are
-// regrouped by the work their next flight needs: photons at or below the scene's top varying
-// layer (cell-by-cell free-path walks, detector walks through the cloud region) gather in the
-// first waves, photons above it (analytic flights through uniform layers, prefix detector
-// integrals) next, idle lanes last. A photon's state and layer paths travel through a per-block
-// global buffer. Histories are unchanged (a photon's stream follows its id, not its lane); only
-// the order of the tally atomics changes. A wave out of work keeps meeting its block until all
-// of the block's waves are out of work.
-constexpr int kSortWords = 40;  // state words before the layer paths
-
-__device__ inline void sort_store(uint32_t *slot, const History<float> &h, uint32_t id, bool active,
-                                  const float *history, int nz) {
-  const Photon<float> &p = h.p;
-  const uint32_t words[kSortWords] = {
-      uint32_t(p.i), uint32_t(p.j), uint32_t(p.k), __float_as_uint(p.rx), __float_as_uint(p.ry),
-      __float_as_uint(p.rz), __float_as_uint(p.a), __float_as_uint(p.b), __float_as_uint(p.c),
-      __float_as_uint(p.phi), __float_as_uint(h.st.i), __float_as_uint(h.st.q), __float_as_uint(h.st.u),
-      __float_as_uint(h.weight), __float_as_uint(h.remains), h.rng.photon, h.rng.round, h.rng.seed, h.rng.stream,
-      h.rng.block, h.rng.words.x, h.rng.words.y, h.rng.words.z, h.rng.words.w, h.rng.next.x, h.rng.next.y,
-      h.rng.next.z, h.rng.next.w, uint32_t(h.rng.lane), uint32_t(h.rng.ready), h.round_offset, h.steps,
-      uint32_t(h.species), uint32_t(h.surface), __float_as_uint(h.path_total), __float_as_uint(h.contribution),
-      h.order, __float_as_uint(h.steer), id, uint32_t(active)};
-#pragma unroll
-  for (int w = 0; w < kSortWords; ++w) slot[w] = words[w];
-  for (int k = 0; k < nz; ++k) slot[kSortWords + k] = __float_as_uint(history[k]);
-}
-
-__device__ inline void sort_load(const uint32_t *slot, History<float> &h, uint32_t &id, bool &active, float *history,
-                                 int nz) {
-  Photon<float> &p = h.p;
-  p.i = int(slot[0]);
-  p.j = int(slot[1]);
-  p.k = int(slot[2]);
-  p.rx = __uint_as_float(slot[3]);
-  p.ry = __uint_as_float(slot[4]);
-  p.rz = __uint_as_float(slot[5]);
-  p.a = __uint_as_float(slot[6]);
-  p.b = __uint_as_float(slot[7]);
-  p.c = __uint_as_float(slot[8]);
-  p.phi = __uint_as_float(slot[9]);
-  h.st = Stokes<float>{__uint_as_float(slot[10]), __uint_as_float(slot[11]), __uint_as_float(slot[12]), 0.0f};
-  h.weight = __uint_as_float(slot[13]);
-  h.remains = __uint_as_float(slot[14]);
-  h.rng.photon = slot[15];
-  h.rng.round = slot[16];
-  h.rng.seed = slot[17];
-  h.rng.stream = slot[18];
-  h.rng.block = slot[19];
-  h.rng.words = Words{slot[20], slot[21], slot[22], slot[23]};
-  h.rng.next = Words{slot[24], slot[25], slot[26], slot[27]};
-  h.rng.lane = int(slot[28]);
-  h.rng.ready = slot[29] != 0u;
-  h.round_offset = slot[30];
-  h.steps = slot[31];
-  h.species = int(slot[32]);
-  h.surface = slot[33] != 0u;
-  h.path_total = __uint_as_float(slot[34]);
-  h.contribution = __uint_as_float(slot[35]);
-  h.order = slot[36];
-  h.steer = __uint_as_float(slot[37]);
-  id = slot[38];
-  active = slot[39] != 0u;
-  for (int k = 0; k < nz; ++k) history[k] = __uint_as_float(slot[kSortWords + k]);
-}
-
-__global__ void __launch_bounds__(kBlock32) __attribute__((amdgpu_waves_per_eu(3)))
-    sorted_transport_kernel(const Scene *__restrict__ scenes, const Controls *__restrict__ controls,
-                            uint32_t scene_count, int row, float *banks, unsigned *failure, unsigned *work,
-                            unsigned long long *counters, uint32_t sort_every, uint32_t *sort_buffer) {
-  extern __shared__ unsigned char lds_raw[];
-  __shared__ int class_counts[kBlock32 / 64][3];
-  __shared__ uint32_t wave_scene[kBlock32 / 64];
-  float *history = reinterpret_cast<float *>(lds_raw) + threadIdx.x * row;
-  const float *wave_history = reinterpret_cast<float *>(lds_raw) + (threadIdx.x & ~63u) * row;
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
-  uint32_t current = __builtin_amdgcn_readfirstlane(uint32_t((uint64_t(wave) * scene_count) / waves));
-  uint32_t visited = 0;
-  const int block = int(blockDim.x);
-  const int waves_in_block = block >> 6;
-  const Pending<float> wave_pending{
-      reinterpret_cast<float *>(lds_raw) + size_t(block) * row + (threadIdx.x & ~63u),
-      reinterpret_cast<int *>(reinterpret_cast<float *>(lds_raw) + size_t(block) * (row + pending_reals(false))) +
-          (threadIdx.x & ~63u),
-      block};
-  const Pending<float> pending = wave_pending.at(__lane_id());
-  float *layers = reinterpret_cast<float *>(reinterpret_cast<int *>(reinterpret_cast<float *>(lds_raw) +
-                                                                    size_t(block) * (row + pending_reals(false))) +
-                                            size_t(block) * kPendingWords) +
-                  (threadIdx.x >> 6) * 4 * row;
-  const size_t sort_stride = size_t(kSortWords + row);
-  uint32_t *block_buffer = sort_buffer + size_t(blockIdx.x) * blockDim.x * sort_stride;
-  uint32_t layers_scene = 0xffffffffu;
-  Counts n{0, 0, 0, 0, 0, 0, 0, 0};
-  History<float> h;
-  Trace<float> trace;
-  bool active = false, finished = false;
-  uint32_t id = 0;
-
-  for (uint32_t iteration = 1;; ++iteration) {
-    if (iteration % sort_every == 0u) {
-      // Meeting of the block's waves: every wave arrives here at the same iteration count.
-      if (__lane_id() == 0) wave_scene[threadIdx.x >> 6] = finished ? 0xffffffffu : current;
-      if (__syncthreads_and(finished ? 1 : 0)) break;
-      bool same = wave_scene[0] != 0xffffffffu;
-      for (int v = 1; v < waves_in_block; ++v) same = same && wave_scene[v] == wave_scene[0];
-      __syncthreads();
-      if (same) {  // the same decision in every wave
-        const int cls = !active ? 2 : (h.p.k < scenes[current].uniform_top ? 0 : 1);
-        const int w = int(threadIdx.x >> 6);
-        uint64_t mine = 0;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-          const uint64_t m = __ballot(cls == c);
-          if (cls == c) mine = m;
-          if (__lane_id() == 0) class_counts[w][c] = int(__popcll(m));
-        }
-        __syncthreads();
-        int dest = int(__popcll(mine & __lanemask_lt()));
-        for (int c = 0; c < cls; ++c)
-          for (int v = 0; v < waves_in_block; ++v) dest += class_counts[v][c];
-        for (int v = 0; v < w; ++v) dest += class_counts[v][cls];
-        const int nz = scenes[current].nz;
-        sort_store(block_buffer + size_t(dest) * sort_stride, h, id, active, history, nz);
-        __syncthreads();
-        sort_load(block_buffer + size_t(threadIdx.x) * sort_stride, h, id, active, history, nz);
-        __syncthreads();
-      }
-    }
-    if (finished) continue;
-    const Scene &s = scenes[current];
-    const Controls &c = controls[current];
-    const int shard = int(wave % uint32_t(s.shards));
-    if (layers_scene != current) {
-      for (int k = __lane_id(); k < s.nz; k += 64) {
-        layers[k] = s.uniform[k] != 0.0f ? s.beta[cell_index<float>(s, 0, 0, k)] : -1.0f;
-        layers[row + k] = s.zunit[k];
-        layers[2 * row + k] = s.height_above[k];
-        layers[3 * row + k] = s.tau_above[k];
-      }
-      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      layers_scene = current;
-    }
-    const unsigned long long idle = __ballot(!active);
-    if (idle) {
-      const int leader = __ffsll(static_cast<long long>(idle)) - 1;
-      uint32_t base = 0;
-      if (__lane_id() == leader) base = atomicAdd(work + current, uint32_t(__popcll(idle)));
-      base = __shfl(base, leader, 64);
-      if (!active) {
-        id = base + uint32_t(__popcll(idle & __lanemask_lt()));
-        if (id < c.count) {
-          launch_photon<float>(s, c, id, history, h);
-          ++n.photons;
-          active = true;
-        }
-      }
-    }
-    if (__ballot(active) == 0) {
-      if (++visited >= scene_count) {
-        finished = true;
-      } else {
-        current = __builtin_amdgcn_readfirstlane((current + 1) % scene_count);
-      }
-      continue;
-    }
-    int status = kRunning;
-    prefetch(h.rng);
-    if (active)
-      status = collide<float, false, false, false, false, true, false>(s, c, failure, history, h, pending, n, trace,
-                                                                      layers, row);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    tally_wave<float, false, false>(s, banks, shard, active && status == kRunning, wave_pending, wave_history, row, n);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    prefetch(h.rng);
-    if (active && status == kRunning) status = scatter_photon<float, false, kPlain>(s, failure, h, n, trace);
-    if (!active || status == kRunning) continue;
-    active = false;
-    if ((n.walk | n.detector | n.lut | n.attempts | n.atomics) >= (1u << 30)) flush_counts(n, counters);
-  }
-
-  const uint32_t values[kCounters] = {n.photons, n.events, n.surface, n.walk,
-                                      n.detector, n.lut, n.attempts, n.atomics};
-#pragma unroll
-  for (int i = 0; i < kCounters; ++i) {
-    const unsigned long long total = wave_sum(static_cast<unsigned long long>(values[i]));
-    if (__lane_id() == 0 && total) atomicAdd(counters + i, total);
-  }
-}
-
 // Sum the shard banks of each round in float64: out[r][t] = sum_s banks[r][s][t].
 template <typename B>
 __global__ void reduce_shards(const B *__restrict__ banks, double *__restrict__ out, int bank_count,
@@ -2096,9 +1901,6 @@ struct Session {
   int steer = kPlain;  // batch rounds: kPlain, kDirected or kGuided
   uint32_t *splits = nullptr;  // weight-window copies of the steered modes, per lane
   size_t split_words = 0;
-  uint32_t sort_every = 0;  // plain float32 tracer: iterations between photon regroupings (0: none)
-  uint32_t *sort_buffer = nullptr;  // per block: the photons' state and layer paths while regrouped
-  size_t sort_words = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   // Rounds accumulated on the device (batch sessions with layer paths): per-scene sums and
@@ -2133,7 +1935,7 @@ void release(Session *s) {
   if (s->stream) (void)hipStreamDestroy(s->stream);
   // s->work points into the failure allocation.
   void *buffers[] = {s->tables, s->buckets, s->banks, s->reduced, s->assembled, s->failure, s->counters, s->records,
-                     s->d_scenes, s->d_controls, s->moments, s->moment_counts, s->splits, s->sort_buffer};
+                     s->d_scenes, s->d_controls, s->moments, s->moment_counts, s->splits};
   for (void *b : buffers) (void)hipFree(b);
   delete s;
 }
@@ -2339,12 +2141,7 @@ bool enqueue(Session *h, uint32_t rounds, uint32_t grid, int block, size_t lds, 
   }
   launch_fill(h->failure, 1 + scene_count, h->stream);
   R *banks = static_cast<R *>(h->banks);
-  constexpr bool SORTABLE = MODE == kPlain && !TRACE && sizeof(R) == 4;
-  if (SORTABLE && grid && h->sort_every && h->sort_buffer) {
-    hipLaunchKernelGGL(sorted_transport_kernel, dim3(grid), dim3(block), lds, h->stream, h->d_scenes, h->d_controls,
-                       scene_count, h->row, reinterpret_cast<float *>(banks), h->failure, h->work, h->counters,
-                       h->sort_every, h->sort_buffer);
-  } else if (grid) {
+  if (grid) {
     hipLaunchKernelGGL((transport_kernel<R, TRACE, MODE>), dim3(grid), dim3(block), lds, h->stream,
                        h->d_scenes, h->d_controls, scene_count, h->row, banks, h->failure, h->work,
                        h->counters, TRACE ? h->records : nullptr, uint32_t(TRACE ? record_count : 0), net, log,
@@ -2369,17 +2166,6 @@ uint32_t persistent_grid(const Session *h, uint64_t photons, int block, size_t l
   int resident = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, transport_kernel<R, TRACE, MODE>, block,
                                                   lds) != hipSuccess || resident < 1)
-    resident = 1;
-  const uint64_t wanted = (photons + block - 1) / block;
-  const uint64_t filled = uint64_t(resident) * uint64_t(h->compute_units > 0 ? h->compute_units : 1);
-  return uint32_t(wanted < filled ? wanted : filled);
-}
-
-// The sorted kernel's persistent grid (its own occupancy).
-uint32_t sorted_grid(const Session *h, uint64_t photons, int block, size_t lds) {
-  int resident = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&resident, sorted_transport_kernel, block, lds) != hipSuccess ||
-      resident < 1)
     resident = 1;
   const uint64_t wanted = (photons + block - 1) / block;
   const uint64_t filled = uint64_t(resident) * uint64_t(h->compute_units > 0 ? h->compute_units : 1);
@@ -2429,19 +2215,6 @@ bool reserve_splits(Session *h, uint32_t grid, int block, char *error, size_t er
   if (!check(hipMalloc(&h->splits, words * sizeof(uint32_t)), error, error_size, "allocate weight-window stacks"))
     return false;
   h->split_words = words;
-  return true;
-}
-
-// Room for the regrouping buffers of every block of a sorted launch (kept for later launches).
-bool reserve_sort(Session *h, uint32_t grid, int block, char *error, size_t error_size) {
-  const size_t words = size_t(grid) * block * (kSortWords + h->row);
-  if (words <= h->sort_words) return true;
-  (void)hipFree(h->sort_buffer);
-  h->sort_buffer = nullptr;
-  h->sort_words = 0;
-  if (!check(hipMalloc(&h->sort_buffer, words * sizeof(uint32_t)), error, error_size, "allocate sort buffers"))
-    return false;
-  h->sort_words = words;
   return true;
 }
 
@@ -2527,11 +2300,8 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
   }
   const int block = block_size<R>(h);
   const size_t lds = lds_bytes<R>(h, block, full_stokes(MODE));
-  constexpr bool SORTABLE = MODE == kPlain && !TRACE && sizeof(R) == 4;
-  const uint32_t grid = SORTABLE && h->sort_every ? sorted_grid(h, c.count, block, lds)
-                                                  : persistent_grid<R, TRACE, MODE>(h, c.count, block, lds);
+  const uint32_t grid = persistent_grid<R, TRACE, MODE>(h, c.count, block, lds);
   if ((MODE == kGuided || MODE == kDirected) && !reserve_splits(h, grid, block, error, error_size)) return 1;
-  if (SORTABLE && h->sort_every && !reserve_sort(h, grid, block, error, error_size)) return 1;
   (void)hipEventRecord(h->start, h->stream);
   if (!enqueue<R, TRACE, MODE>(h, c.rounds, grid, block, lds, record_count, net, log, error, error_size)) return 1;
   (void)hipEventRecord(h->stop, h->stream);
@@ -2784,15 +2554,11 @@ int batch_round(Session *h, uint32_t round, uint32_t seed, uint32_t max_steps, c
     set_error(error, error_size, "A proposal on the session's device is required");
     return 1;
   }
-  const bool sorted = h->steer == kPlain && h->sort_every;
   const uint32_t grid =
       !total                   ? 0u
       : h->steer == kGuided   ? persistent_grid<float, false, kGuided>(h, total, block, lds)
       : h->steer == kDirected ? persistent_grid<float, false, kDirected>(h, total, block, lds)
-      : sorted                ? sorted_grid(h, total, block, lds)
                               : persistent_grid<float, false, kPlain>(h, total, block, lds);
-  // Buffers for a full-occupancy grid: a captured round keeps their address.
-  if (sorted && !reserve_sort(h, sorted_grid(h, uint64_t(1) << 40, block, lds), block, error, error_size)) return 1;
   // Stacks for a full-occupancy grid: a captured round keeps their address.
   if (h->steer != kPlain &&
       !reserve_splits(h,
@@ -3225,17 +2991,6 @@ int monki_hip_set_directed(void *handle, float alpha, float kappa, uint32_t spec
   h->guidance.max_order = max_order;
   h->guidance.kappa = kappa;
   h->steer = kDirected;
-  drop_graphs(h);
-  return 0;
-}
-
-int monki_hip_set_sort(void *handle, uint32_t every, char *error, size_t error_size) {
-  Session *h = static_cast<Session *>(handle);
-  if (!h || every > 4096u) {
-    set_error(error, error_size, "Invalid session or sort interval (0 to 4096 iterations)");
-    return 1;
-  }
-  h->sort_every = every;
   drop_graphs(h);
   return 0;
 }

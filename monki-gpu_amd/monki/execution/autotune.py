@@ -1,8 +1,8 @@
-"""Autotuner of the batched launch: lanes per block, tally shards and the photon-sort interval.
+"""Autotuner of the batched launch: lanes per block and tally shards.
 
 No reference counterpart. Times ``HIPBatch`` rounds of the actual batch for every candidate
-(block, tally shards, sort interval; 0 = unsorted) -- one warm-up round, which also captures the
-graph, then ``rounds`` timed replays, device time from HIP events -- and returns the fastest. Results are cached in a JSON
+(block, tally shards) -- one warm-up round, which also captures the graph, then ``rounds`` timed
+replays, device time from HIP events -- and returns the fastest. Results are cached in a JSON
 file keyed by the device, the scene shapes and the photon counts, so a batch is tuned once.
 """
 
@@ -24,10 +24,10 @@ def signature(cases, device_name: str) -> str:
     return hashlib.sha256(json.dumps([device_name, shapes]).encode()).hexdigest()[:16]
 
 
-def autotune(cases, *, blocks=(64, 128, 256), shards=(8, 16, 32, 64), sorts=(0,), rounds: int = 3,
+def autotune(cases, *, blocks=(64, 128, 256), shards=(8, 16, 32, 64), rounds: int = 3,
              photons_per_column: Optional[int] = None, device: int = 0, cache: Optional[Path] = CACHE,
              seed: int = 23274) -> dict:
-    """Return {"block", "tally_shards", "sort_every", "seconds_per_round", "table", "cached"}.
+    """Return {"block", "tally_shards", "seconds_per_round", "table", "cached"} for the batch.
 
     photons_per_column, when given, replaces the cases' budget for the timing rounds.
     """
@@ -43,32 +43,31 @@ def autotune(cases, *, blocks=(64, 128, 256), shards=(8, 16, 32, 64), sorts=(0,)
 
     table = []
 
-    candidates = [(block, shard_count, sort) for block in blocks for shard_count in shards for sort in sorts]
+    for block in blocks:
+        for shard_count in shards:
+            with HIPBatch(device, tally_shards=shard_count, block=block, seed=seed) as batch:
+                batch.prepare(timing_cases)
 
-    for block, shard_count, sort in candidates:
-        with HIPBatch(device, tally_shards=shard_count, block=block, seed=seed, sort_every=sort) as batch:
-            batch.prepare(timing_cases)
+                if key is None:
+                    key = signature(timing_cases, batch.provenance["device"])
 
-            if key is None:
-                key = signature(timing_cases, batch.provenance["device"]) + f"-sorts{list(sorts)}"
+                    if key in stored:
+                        return {**stored[key], "cached": True}
 
-                if key in stored:
-                    return {**stored[key], "cached": True}
+                batch.run_round(1)
+                start = time.perf_counter()
+                device_seconds = 0.0
 
-            batch.run_round(1)
-            start = time.perf_counter()
-            device_seconds = 0.0
+                for index in range(2, rounds + 2):
+                    batch.run_round(index)
+                    device_seconds += batch.kernel_seconds()
 
-            for index in range(2, rounds + 2):
-                batch.run_round(index)
-                device_seconds += batch.kernel_seconds()
-
-            wall = (time.perf_counter() - start) / rounds
-            table.append({"block": block, "tally_shards": shard_count, "sort_every": sort,
-                          "device_seconds": device_seconds / rounds, "wall_seconds": wall})  # fmt: skip
+                wall = (time.perf_counter() - start) / rounds
+                table.append({"block": block, "tally_shards": shard_count,
+                              "device_seconds": device_seconds / rounds, "wall_seconds": wall})  # fmt: skip
 
     best = min(table, key=lambda row: row["device_seconds"])
-    result = {"block": best["block"], "tally_shards": best["tally_shards"], "sort_every": best["sort_every"],
+    result = {"block": best["block"], "tally_shards": best["tally_shards"],
               "seconds_per_round": best["device_seconds"], "table": table}  # fmt: skip
 
     if cache is not None and key is not None:

@@ -23,7 +23,7 @@ from ..metadata import file_hash, scene_metadata
 from ..preparation import UPSTREAM, prepare_case
 from .hip import DOUBLE_POINTER, FLOAT_POINTER, HIP, LIBRARY, UINT_POINTER, load_library
 from .hip_tables import pack_tables, tally_count
-from .proposal import DirectedSettings, Proposal, ProposalSettings, apply_steering, set_sort, sharpest_species
+from .proposal import DirectedSettings, Proposal, ProposalSettings, apply_steering, sharpest_species
 from .tallies import BatchControls, RawTallies
 
 
@@ -92,14 +92,12 @@ class HIPBatch:
         upstream: Path = UPSTREAM,
         proposal: Optional[ProposalSettings] = None,
         directed: Optional[DirectedSettings] = None,
-        sort_every: int = 0,
     ):
         for name, value, minimum, maximum in (
             ("device", device, 0, 2**32 - 1),
             ("seed", seed, 0, 2**32 - 1),
             ("max_steps", max_steps, 1, 100_000),
             ("tally_shards", tally_shards, 1, 64),
-            ("sort_every", sort_every, 0, 4096),
         ):
             if type(value) is not int or not minimum <= value <= maximum:
                 raise ValueError(f"{name} must be an integer in [{minimum}, {maximum}]")
@@ -126,7 +124,6 @@ class HIPBatch:
         self.provenance: dict = {}
         self.proposal = proposal
         self.directed = directed
-        self.sort_every = sort_every
         self._network: Optional[Proposal] = None
         self._native = None
         self._handle = None
@@ -217,10 +214,6 @@ class HIPBatch:
         }
 
         try:
-            if self.sort_every:
-                set_sort(native, handle, self.sort_every)
-                self.provenance["sort_every"] = self.sort_every
-
             self._steer(cases)
         except BaseException:
             self.close()

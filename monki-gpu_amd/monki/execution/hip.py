@@ -27,7 +27,7 @@ from ..preparation import UPSTREAM, prepare_case
 from .backend import Backend
 from .hip_tables import pack_tables, split_scattering, tally_count
 from .proposal import (DirectedSettings, Proposal, ProposalSettings, apply_steering, declare, log_rounds,
-                       set_sort, sharpest_species)
+                       sharpest_species)
 from .tallies import BatchControls, RawTallies
 
 PACKAGE_ROOT = Path(__file__).resolve().parents[2]
@@ -142,7 +142,6 @@ class HIP(Backend):
         proposal: Optional[ProposalSettings] = None,
         upstream: Path = UPSTREAM,
         directed: Optional[DirectedSettings] = None,
-        sort_every: int = 0,
     ):
         for name, value, minimum, maximum in (
             ("device", device, 0, 2**32 - 1),
@@ -150,7 +149,6 @@ class HIP(Backend):
             ("tally_shards", tally_shards, 1, 64),
             ("photons_per_dispatch", photons_per_dispatch, 1, 1 << 30),
             ("max_steps", max_steps, 1, 100_000),
-            ("sort_every", sort_every, 0, 4096),
         ):
             if type(value) is not int or not minimum <= value <= maximum:
                 raise ValueError(f"{name} must be an integer in [{minimum}, {maximum}]")
@@ -190,7 +188,6 @@ class HIP(Backend):
         self.library = Path(library)
         self.proposal = proposal
         self.directed = directed
-        self.sort_every = sort_every
         self.upstream = Path(upstream)
         self.metadata = None
         self.provenance: dict = {}
@@ -295,10 +292,6 @@ class HIP(Backend):
             self.provenance["preparation"] = preparation
 
         try:
-            if self.sort_every:
-                set_sort(native, handle, self.sort_every)
-                self.provenance["sort_every"] = self.sort_every
-
             if self.proposal is not None:
                 self._train_proposal(nx * ny)
                 lobes = sharpest_species(tables)
@@ -605,5 +598,4 @@ class HIP(Backend):
             self.precision,
             self.proposal,
             self.directed,
-            self.sort_every,
         )

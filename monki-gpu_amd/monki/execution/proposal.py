@@ -145,7 +145,6 @@ def declare(native) -> None:
     native.monki_hip_run_logged.argtypes = [ctypes.c_void_p, _U, _D, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
     native.monki_hip_set_directed.argtypes = [ctypes.c_void_p, ctypes.c_float, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]
     native.monki_hip_run_directed.argtypes = [ctypes.c_void_p, _U, _D, ctypes.c_char_p, ctypes.c_size_t]
-    native.monki_hip_set_sort.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]
     native.monki_hip_set_steering.argtypes = [
         ctypes.c_void_p, ctypes.c_int, ctypes.c_float, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_float,
         ctypes.c_char_p, ctypes.c_size_t,
@@ -156,7 +155,7 @@ def declare(native) -> None:
         "monki_hip_proposal_set_samples", "monki_hip_proposal_samples", "monki_hip_proposal_read_samples",
         "monki_hip_proposal_gradients", "monki_hip_proposal_train", "monki_hip_proposal_forward",
         "monki_hip_attach_proposal", "monki_hip_run_guided", "monki_hip_trace_guided", "monki_hip_run_logged",
-        "monki_hip_set_directed", "monki_hip_run_directed", "monki_hip_set_steering", "monki_hip_set_sort",
+        "monki_hip_set_directed", "monki_hip_run_directed", "monki_hip_set_steering",
     ):  # fmt: skip
         getattr(native, name).restype = ctypes.c_int
 
@@ -219,15 +218,6 @@ def sharpest_species(scene) -> int:
         return -1
 
     return int(np.argmax([float(species.f11[-1]) for species in scene.species]))
-
-
-def set_sort(native, handle, every: int) -> None:
-    """Wavefront-sorted photon queues of the plain tracer every ``every`` iterations (0: off)."""
-
-    error = ctypes.create_string_buffer(1024)
-
-    if native.monki_hip_set_sort(handle, every, error, len(error)):
-        raise RuntimeError(error.value.decode())
 
 
 def apply_steering(native, handle, settings, lobe_species: int, lobe_fraction: float) -> None:

@@ -299,26 +299,3 @@ def test_steered_batch_equals_single_scene_sessions(mode):
                 # The batch trained on its first case; give the single session the same network.
                 single.network.parameters = parameters
             assert_tallies_equal(batched[number], single.run_batch(BatchControls(1, case.photons_per_column)))
-
-
-@pytest.mark.parametrize("every", [4, 32])
-def test_sorted_photon_queues_leave_results_unchanged(every):
-    """Regrouping photons across the waves of a block changes no history: a sorted single-scene
-    round and a sorted batch round equal the unsorted ones up to the order of the atomics."""
-
-    from monki.execution import HIP, HIPBatch, BatchControls
-
-    batch_cases = cases(rounds=2)
-
-    with HIP(tally_shards=8) as plain, HIP(tally_shards=8, sort_every=every) as sorted_single:
-        plain.prepare(batch_cases[0])
-        sorted_single.prepare(batch_cases[0])
-        request = BatchControls(1, batch_cases[0].photons_per_column)
-        assert_tallies_equal(sorted_single.run_batch(request), plain.run_batch(request))
-        assert sorted_single.counters()["photons"] == plain.counters()["photons"]
-
-    with HIPBatch(tally_shards=8) as unsorted, HIPBatch(tally_shards=8, sort_every=every) as sorted_batch:
-        unsorted.prepare(batch_cases)
-        sorted_batch.prepare(batch_cases)
-        for a, b in zip(sorted_batch.run_round(1), unsorted.run_round(1)):
-            assert_tallies_equal(a, b)

@@ -716,6 +716,41 @@ template <typename R> struct Pending {
   __device__ Pending at(int lane) const { return Pending{real + lane, word + lane, stride}; }
 };
 
+// Open tally records of a lane (modes with layer paths), in the same LDS area as Pending.
+// Consecutive events of one photon in one layer k whose detector rays toward a view reach the
+// same pixel share that view's record:
+//   S = sum of their detector weights C1, E = sum of C1 (path in layer k so far + detector-ray
+//   length inside layer k), and the sums of their Q, U (V) contributions.
+// The record is written out before the photon's layer paths change outside layer k, so each
+// layer j != k holds the same path for all its events, and the events' tally over layer j is
+// S (path_j + detector-ray length in j) — one atomic instruction for the whole run of events.
+constexpr int kRecordReals = 4 * kMaxViews;
+constexpr int kRecordReals4 = 5 * kMaxViews;  // with V
+constexpr int kRecordWords = kMaxViews;
+constexpr int record_reals(bool v4) { return v4 ? kRecordReals4 : kRecordReals; }
+
+template <typename R> struct Record {
+  R *real;     // E x views | S x views | q x views | u x views [| v x views with nmat 4]
+  int *word;   // pixel x views
+  int stride;  // lanes per block
+  __device__ R &e(int v) const { return real[v * stride]; }
+  __device__ R &s(int v) const { return real[(kMaxViews + v) * stride]; }
+  __device__ R &q(int v) const { return real[(2 * kMaxViews + v) * stride]; }
+  __device__ R &u(int v) const { return real[(3 * kMaxViews + v) * stride]; }
+  __device__ R &v(int v) const { return real[(4 * kMaxViews + v) * stride]; }
+  // Output pixel of the open record, -1 when the view has none.
+  __device__ int &pixel(int v) const { return word[v * stride]; }
+  __device__ Record at(int lane) const { return Record{real + lane, word + lane, stride}; }
+};
+
+// Bytes per lane of the area that holds either layout.
+template <typename R> constexpr size_t pending_bytes(bool v4) {
+  return pending_reals(v4) * sizeof(R) + kPendingWords * sizeof(int) >
+                 record_reals(v4) * sizeof(R) + kRecordWords * sizeof(int)
+             ? pending_reals(v4) * sizeof(R) + kPendingWords * sizeof(int)
+             : record_reals(v4) * sizeof(R) + kRecordWords * sizeof(int);
+}
+
 template <typename R> __device__ inline bool finite3(const Stokes<R> &v) {
   return isfinite(v.i) && isfinite(v.q) && isfinite(v.u);
 }
@@ -776,22 +811,18 @@ __device__ inline int opposite_grid_pixel(const Scene &s, const Photon<R> &p, R 
   return max(i, 0) + s.nx * max(j, 0);
 }
 
-// Advance one photon to its next collision or surface hit (mc.f90:476-677), apply the albedo
-// or the species' single-scattering albedo (mc.f90:681-727, 1028-1074), and evaluate the local
-// estimate toward every view (mc.f90:738-822, 1109-1207; mc_routines_cartesian_mod.f90:849-1222)
-// into `out`. Returns kRunning while the history continues, else its end status.
+// Advance one photon to its next collision or surface hit (mc.f90:476-677). Returns kRunning
+// at the event (`surface` set for a surface hit), kEscaped or kFailed. layer_path: the path
+// the flight added to the layer it ends in (with layer paths), 0 on entry.
 // layers (LAYERS): the wave's LDS copy of four per-layer rows (row values apart): extinction of
 // horizontally uniform layers (-1 for varying ones), thickness, height above and optical depth
 // above, read instead of the global tables on every layer crossing of the free path and by
 // the detector walk.
-template <typename R, bool TRACE, bool V4 = false, bool BACK = false, bool SCATS = false, bool LAYERS = false,
-          bool DOWN = false>
-__device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *history,
-                       History<R> &h, const Pending<R> &out, Counts &n, Trace<R> &trace,
-                       const float *layers = nullptr, int row = 0) {
+template <typename R, bool LAYERS>
+__device__ inline int walk(const Scene &s, const Controls &c, unsigned *failure, R *history, History<R> &h,
+                           Counts &n, const float *layers, int row, bool &surface, R &layer_path) {
   const R eps = R(1e-16);
   Photon<R> &p = h.p;
-  bool surface = false;
   double slide = 0.0;  // flight through uniform layers not yet applied to the x/y cells
   auto flush_slide = [&]() {
     if (slide != 0.0) {
@@ -826,6 +857,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
       if (s.paths) {
         history[p.k] += d;
         h.path_total += d;
+        layer_path += d;
       }
       if (event) {
         flush_slide();
@@ -838,6 +870,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
       if (p.k >= s.nz) return kEscaped;
       if (p.k >= 0) {
         p.rz = LAYERS ? R(layers[row + p.k]) : R(s.zunit[p.k]);
+        layer_path = R(0);
         continue;
       }
       flush_slide();
@@ -853,6 +886,7 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
       if (s.paths) {
         history[p.k] += event_distance;
         h.path_total += event_distance;
+        layer_path += event_distance;
       }
       break;
     }
@@ -862,16 +896,29 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
     if (s.paths) {
       history[old_layer] += d;
       h.path_total += d;
+      layer_path += d;
     }
     h.remains -= d * extinction;
     if (p.k >= s.nz) return kEscaped;
-    if (p.k >= 0) continue;
+    if (p.k >= 0) {
+      if (p.k != old_layer) layer_path = R(0);
+      continue;
+    }
     surface = true;
     p.k = 0;
     p.rz = R(0);
     break;
   }
+  return kRunning;
+}
 
+// Interaction at the event (mc.f90:681-727, 1028-1074): the surface albedo, or the species
+// drawn from the cell's fractions and its single-scattering albedo. Returns kAbsorbed when the
+// weight falls below 1e-16, else kRunning.
+template <typename R, bool TRACE, bool SCATS>
+__device__ inline int interact(const Scene &s, History<R> &h, Counts &n, Trace<R> &trace, bool surface) {
+  const R eps = R(1e-16);
+  const Photon<R> &p = h.p;
   int species = -1;
   if (surface) {
     h.weight *= R(s.albedo[p.i + s.nx * p.j]);
@@ -895,39 +942,39 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
   }
   h.species = species;
   h.surface = surface;
-  if (h.weight < eps) return kAbsorbed;
+  return h.weight < eps ? kAbsorbed : kRunning;
+}
 
-  if (SCATS) {
-    // Event cell (calc_radiances_and_scats :503-507): level 0 for the surface, else the layer;
-    // complemented when the scattering order exceeds one.
-    // (Events never occur in the vacuum layer, so level <= nz - 1; the bound keeps it so.)
-    const int level = (p.k == 0 && p.rz < R(1e-16)) ? 0 : min(p.k + 1, s.nz - 1);
-    int cell = p.i + s.nx * (p.j + s.ny * level);
-    if (BACK) {
-      if (h.order == 1)
-        h.first_cell = cell;
-      else
-        cell = h.first_cell;
-    }
-    out.layer() = h.order == 1 ? cell : ~cell;
-  } else {
-    out.layer() = p.k;
-  }
-  out.round() = int(h.round_offset);
-  // Detector ray inside the event layer: the distance to its top face (views point upward).
-  // (Views of a surface detector point down: the distance to the bottom face instead.)
-  const R rz = (p.c < R(0)) != DOWN ? fmax(R(0), (LAYERS ? R(layers[row + p.k]) : R(s.zunit[p.k])) - p.rz) : p.rz;
-  out.rz() = rz;
-  // All views share one azimuth: cos and sin of the photon-view azimuth difference, once.
-  const R delta = p.phi - R(s.phi);
-  R sin_delta, cos_delta;
-  sin_cos(delta, &sin_delta, &cos_delta);
-#pragma unroll 1
-  for (int view = 0; view < s.views; ++view) {
-    out.pixel(view) = -1;
+// Detector ray inside the event layer: the distance to its top face (views point upward), or
+// to its bottom face for the downward views of a surface detector.
+template <typename R, bool LAYERS, bool DOWN>
+__device__ inline R event_rz(const Scene &s, const Photon<R> &p, const float *layers, int row) {
+  return (p.c < R(0)) != DOWN ? fmax(R(0), (LAYERS ? R(layers[row + p.k]) : R(s.zunit[p.k])) - p.rz) : p.rz;
+}
+
+// Local estimate of one event toward one view: C1 = I exp(-tau) and the Q, U (V) contributions
+// times mu0/mu, and the output pixel (-1 when the view gets nothing from the event).
+template <typename R> struct Estimate {
+  R weight, q, u, v;
+  int pixel;
+};
+
+// The local estimate toward `view` (mc.f90:738-822, 1109-1207; detector walk and pixel,
+// mc_routines_cartesian_mod.f90:849-1222) of the event the history is at. delta: the
+// photon-view azimuth difference, with its cosine and sine; rz: event_rz. Sets `status` to
+// kFailed (and a failure flag) on a non-finite estimate or a detector walk over its limit.
+template <typename R, bool TRACE, bool V4, bool BACK, bool LAYERS, bool DOWN>
+__device__ inline Estimate<R> estimate(const Scene &s, const Controls &c, unsigned *failure, History<R> &h,
+                                       int view, R delta, R cos_delta, R sin_delta, R rz, Counts &n,
+                                       Trace<R> &trace, const float *layers, int row, int &status) {
+  const R eps = R(1e-16);
+  const Photon<R> &p = h.p;
+  const int species = h.species;
+  Estimate<R> out{R(0), R(0), R(0), R(0), -1};
+  {
     const R mu = R(s.view_mu[view]);
     Stokes<R> prob;
-    if (surface) {
+    if (h.surface) {
       if (BACK) {
         // W As S0 = the first column of W (mc.f90:754-755).
         const R scale = (DOWN ? R(0) : mu) * h.weight;
@@ -969,16 +1016,18 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
     }
     if (!finite_stokes<R, V4>(prob)) {
       atomicOr(failure, kNonfiniteProbability);
-      return kFailed;
+      status = kFailed;
+      return out;
     }
-    if (prob.i <= eps) continue;
+    if (prob.i <= eps) return out;
     R tau;
     const DetectorHit hit =
         detector_walk<R, LAYERS, DOWN>(s, p, R(s.view_a[view]), R(s.view_b[view]), mu, tau, c.max_steps, n, layers,
                                        row);
     if (!hit.ok) {
       atomicOr(failure, kDetectorLimit);
-      return kFailed;
+      status = kFailed;
+      return out;
     }
     const R attenuation = exp(-tau);
     const R geometry = R(s.view_geometry[view]);  // mu0 / mu (mc.f90:1456-1474)
@@ -986,17 +1035,17 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
     // Backward transport registers at the injection column (mc_routines_cartesian_mod.f90:
     // 1035-1039), the launch column of the photon.
     if (BACK) {
-      out.pixel(view) = int(h.rng.photon % uint32_t(s.nx * s.ny));
+      out.pixel = int(h.rng.photon % uint32_t(s.nx * s.ny));
     } else if (__builtin_expect(s.register_opposite, 0)) {
-      out.pixel(view) = opposite_grid_pixel<R, DOWN>(s, p, rz, view);
+      out.pixel = opposite_grid_pixel<R, DOWN>(s, p, rz, view);
     } else {
-      out.pixel(view) = hit.i + s.nx * hit.j;
+      out.pixel = hit.i + s.nx * hit.j;
     }
-    out.weight(view) = ci;
+    out.weight = ci;
     h.contribution += float(ci);
-    out.q(view) = prob.q * attenuation * geometry;
-    out.u(view) = prob.u * attenuation * geometry;
-    if (V4) out.v(view) = prob.v * attenuation * geometry;
+    out.q = prob.q * attenuation * geometry;
+    out.u = prob.u * attenuation * geometry;
+    if (V4) out.v = prob.v * attenuation * geometry;
     if (TRACE) {
       trace.stokes_i += ci * geometry;
       if (s.paths) {
@@ -1009,6 +1058,59 @@ __device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *
         trace.paths += ci * (h.path_total + (rz + beyond) / fabs(mu));
       }
     }
+  }
+  return out;
+}
+
+// One event of a history whose local estimates are tallied one event at a time (the
+// scattering decomposition): the flight, the interaction, and every view's local estimate
+// parked in `out`. Returns kRunning while the history continues, else its end status.
+template <typename R, bool TRACE, bool V4 = false, bool BACK = false, bool SCATS = false, bool LAYERS = false,
+          bool DOWN = false>
+__device__ int collide(const Scene &s, const Controls &c, unsigned *failure, R *history,
+                       History<R> &h, const Pending<R> &out, Counts &n, Trace<R> &trace,
+                       const float *layers = nullptr, int row = 0) {
+  const Photon<R> &p = h.p;
+  bool surface = false;
+  R layer_path = R(0);
+  int status = walk<R, LAYERS>(s, c, failure, history, h, n, layers, row, surface, layer_path);
+  if (status != kRunning) return status;
+  status = interact<R, TRACE, SCATS>(s, h, n, trace, surface);
+  if (status != kRunning) return status;
+
+  if (SCATS) {
+    // Event cell (calc_radiances_and_scats :503-507): level 0 for the surface, else the layer;
+    // complemented when the scattering order exceeds one.
+    // (Events never occur in the vacuum layer, so level <= nz - 1; the bound keeps it so.)
+    const int level = (p.k == 0 && p.rz < R(1e-16)) ? 0 : min(p.k + 1, s.nz - 1);
+    int cell = p.i + s.nx * (p.j + s.ny * level);
+    if (BACK) {
+      if (h.order == 1)
+        h.first_cell = cell;
+      else
+        cell = h.first_cell;
+    }
+    out.layer() = h.order == 1 ? cell : ~cell;
+  } else {
+    out.layer() = p.k;
+  }
+  out.round() = int(h.round_offset);
+  const R rz = event_rz<R, LAYERS, DOWN>(s, p, layers, row);
+  out.rz() = rz;
+  // All views share one azimuth: cos and sin of the photon-view azimuth difference, once.
+  const R delta = p.phi - R(s.phi);
+  R sin_delta, cos_delta;
+  sin_cos(delta, &sin_delta, &cos_delta);
+#pragma unroll 1
+  for (int view = 0; view < s.views; ++view) {
+    const Estimate<R> e = estimate<R, TRACE, V4, BACK, LAYERS, DOWN>(s, c, failure, h, view, delta, cos_delta,
+                                                                     sin_delta, rz, n, trace, layers, row, status);
+    if (status != kRunning) return status;
+    out.pixel(view) = e.pixel;
+    out.weight(view) = e.weight;
+    out.q(view) = e.q;
+    out.u(view) = e.u;
+    if (V4) out.v(view) = e.v;
   }
   return kRunning;
 }
@@ -1283,61 +1385,91 @@ __device__ inline void event_features(const Scene &s, const History<R> &h, float
   f[11] = fminf(float(h.order), 64.0f) / 32.0f - 1.0f;
 }
 
-// Wave-cooperative tally. The wave takes the pending events of its lanes one at a time and
-// adds, per view, one record [layer paths 0..nz-1 | Stokes 0..nmat-1] of the event's pixel in
-// a single atomic instruction: lane t adds layer t's C1 * (path_layers + path_layers_angle)
-// (mc_routines_cartesian_mod.f90:1173-1186, the photon's layer paths read from the source
-// lane's LDS row) or Stokes component t - nz. Contiguous lanes make each instruction a few
-// cache-line requests instead of one request per value. Record offsets within a bank are
-// 32-bit: 24-bit multiplies (full rate) instead of 64-bit index arithmetic per view.
-template <typename R, bool V4 = false, bool DOWN = false>
-__device__ inline void tally_wave(const Scene &s, R *__restrict__ banks, int shard, bool has,
-                                  const Pending<R> &wave_pending, const R *wave_history, int row,
-                                  Counts &n) {
+__device__ inline float read_lane(float v, int src) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+}
+__device__ inline double read_lane(double v, int src) {
+  const unsigned long long bits = static_cast<unsigned long long>(__double_as_longlong(v));
+  const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(unsigned(bits)), src));
+  const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(unsigned(bits >> 32)), src));
+  return __longlong_as_double(static_cast<long long>((static_cast<unsigned long long>(hi) << 32) | lo));
+}
+
+// The flight of a lane since its open records' last event, when it left their layer (`stale`):
+// the layer it ended in (nz after an escape), the path it added there, and 1 / max(|cos|,
+// 1e-16) of its direction. The layers between were crossed whole: thickness times that each.
+template <typename R> struct Flight {
+  bool stale;
+  int end_layer;
+  R end_path, inverse;
+};
+
+// Wave-cooperative tally (mc_routines_cartesian_mod.f90:1173-1186) of open records (Record):
+// for each lane in `lanes`, the records of the views in its `views` bit mask. A record is one
+// per-(view, pixel) bank record [layer paths 0..nz-1 | Stokes 0..nmat-1] added by a single
+// atomic instruction, lane t adding slot t: contiguous lanes make it a few cache-line requests
+// instead of one request per value. Path slots (C1 (path_layers + path_layers_angle) summed
+// over the record's events): the record's E in its layer k, S (path + full-layer detector
+// length) in the layers the detector ray crosses after k, S path in the others, the path read
+// from the source lane's LDS row, where a stale lane's paths are its current ones minus its
+// flight's; Stokes slots: S mu0/mu, Q, U (V). views, open_layer, round_offset and flight are
+// the lanes' own values (read from the source lane). The source lanes run one after another:
+// the per-view constants are set up once, outside that loop. Record offsets within a bank are
+// 32-bit (session creation keeps them below 2^31 / 8, pixel indexes below 2^24): 24-bit
+// multiplies instead of 64-bit index arithmetic.
+template <typename R, bool V4, bool DOWN>
+__device__ inline void flush_records(const Scene &s, R *__restrict__ banks, int shard, uint64_t lanes, int views,
+                                     const Record<R> &wave_records, const R *wave_history, int row, int open_layer,
+                                     uint32_t round_offset, const Flight<R> &flight, Counts &n) {
   const int lane = __lane_id();
   const int record = s.paths ? s.nz + s.nmat : s.nmat;
   const int stokes_at = s.paths ? s.nz : 0;
   const bool path_lane = s.paths && lane < s.nz;
   const bool tally_lane = lane < record;
   const int component = lane - stokes_at;  // Stokes component of a non-path lane
-  // Offset of this lane's slot in view v's record of pixel 0. Offsets stay below 2^31 / 8
-  // (session creation rejects larger banks) and pixel indexes below 2^24.
   const uint32_t view_values = uint32_t(s.nx * s.ny) * uint32_t(s.record_stride);
-  // Per-view constants, outside the loop over events: 1 / mu, mu0 / mu, and this lane's
-  // full-layer detector path (the vacuum layer adds none).
   const R lane_layer = lane < s.nz - 1 ? R(s.zunit[lane]) : R(0);
-  R inv_mu[kMaxViews], geometry[kMaxViews], layer_length[kMaxViews];
+  const R lane_thickness = lane < s.nz ? R(s.zunit[lane]) : R(0);
+  R geometry[kMaxViews], layer_length[kMaxViews];
   uint32_t lane_index[kMaxViews];
-  R stokes_i[kMaxViews];
-  const R stokes_q = component == 1 ? R(1) : R(0), stokes_u = component == 2 ? R(1) : R(0);
-  const R stokes_v = component == 3 ? R(1) : R(0);
 #pragma unroll
   for (int view = 0; view < kMaxViews; ++view) {
-    inv_mu[view] = R(s.view_inv_mu[view]);
-    geometry[view] = R(s.view_geometry[view]);
-    layer_length[view] = lane_layer * inv_mu[view];
+    geometry[view] = component == 0 ? R(s.view_geometry[view]) : R(0);
+    layer_length[view] = lane_layer * R(s.view_inv_mu[view]);
     lane_index[view] = uint32_t(view) * view_values + uint32_t(lane);
-    stokes_i[view] = component == 0 ? geometry[view] : R(0);
   }
-  for (uint64_t mask = __ballot(has); mask; mask &= mask - 1) {
+  // The source lane's small words in one register: views (3 bits), layer + 1 (7), stale (1),
+  // end layer + 1 (7); layers stay below 64.
+  const int word = (views & 7) | ((open_layer + 1) << 3) | (int(flight.stale) << 10) | ((flight.end_layer + 1) << 11);
+  for (uint64_t mask = lanes; mask; mask &= mask - 1) {
     const int src = __ffsll(static_cast<long long>(mask)) - 1;
-    const Pending<R> source = wave_pending.at(src);
-    const int layer = source.layer();
-    const R rz = source.rz();
-    R *bank = banks + s.bank_base + (size_t(source.round()) * s.shards + shard) * size_t(s.bank_count);
-    const R path = path_lane ? wave_history[src * row + lane] : R(0);
-    // Layers the detector ray crosses after the event layer: above it (below it for DOWN).
-    const bool at_layer = lane == layer, above = DOWN ? lane < layer : lane > layer;
+    const int packed = __builtin_amdgcn_readlane(word, src);
+    const int chosen = packed & 7;
+    const int layer = ((packed >> 3) & 127) - 1;
+    const uint32_t round = uint32_t(__builtin_amdgcn_readlane(int(round_offset), src));
+    const Record<R> source = wave_records.at(src);
+    R *bank = banks + s.bank_base + (size_t(round) * s.shards + shard) * size_t(s.bank_count);
+    R path = path_lane ? wave_history[src * row + lane] : R(0);
+    if ((packed >> 10) & 1) {
+      const int end = ((packed >> 11) & 127) - 1;
+      const bool between = end > layer ? (lane > layer && lane < end) : (lane < layer && lane > end);
+      const R crossed = between ? lane_thickness * read_lane(flight.inverse, src) : R(0);
+      const R added = lane == end ? read_lane(flight.end_path, src) : crossed;
+      // (A layer first entered on this flight comes back as rounding around zero.)
+      if (path_lane) path = fmax(R(0), path - added);
+    }
+    const bool at_layer = lane == layer, beyond = DOWN ? lane < layer : lane > layer;
 #pragma unroll
     for (int view = 0; view < kMaxViews; ++view) {
-      if (view >= s.views) break;
+      if (!((chosen >> view) & 1)) continue;
       const int pixel = source.pixel(view);
-      const R weight = source.weight(view);
-      const R q = source.q(view), u = source.u(view);
-      const R length = at_layer ? rz * inv_mu[view] : (above ? layer_length[view] : R(0));
-      const R stokes = V4 ? weight * stokes_i[view] + q * stokes_q + u * stokes_u + source.v(view) * stokes_v
-                          : weight * stokes_i[view] + q * stokes_q + u * stokes_u;
-      const R value = path_lane ? weight * (path + length) : stokes;
+      const R weight = source.s(view);
+      R stokes = weight * geometry[view];
+      stokes = component == 1 ? source.q(view) : stokes;
+      stokes = component == 2 ? source.u(view) : stokes;
+      if (V4) stokes = component == 3 ? source.v(view) : stokes;
+      const R value = path_lane ? (at_layer ? source.e(view) : weight * (path + (beyond ? layer_length[view] : R(0))))
+                                : stokes;
       if (pixel >= 0 && tally_lane && value != R(0)) {
         add_tally<R>(bank + (__umul24(uint32_t(pixel), uint32_t(s.record_stride)) + lane_index[view]), value);
         ++n.atomics;
@@ -1542,18 +1674,24 @@ __global__ void __launch_bounds__(kBlock32)
   uint32_t current = __builtin_amdgcn_readfirstlane(uint32_t((uint64_t(wave) * scene_count) / waves));
   uint32_t visited = 0;
   const int block = int(blockDim.x);
-  const Pending<R> wave_pending{
-      reinterpret_cast<R *>(lds_raw) + size_t(block) * row + (threadIdx.x & ~63u),
-      reinterpret_cast<int *>(reinterpret_cast<R *>(lds_raw) + size_t(block) * (row + pending_reals(V4))) +
-          (threadIdx.x & ~63u),
-      block};
+  // After the layer rows: the block's pending local estimates (scattering decomposition) or
+  // open records (every other mode), struct of arrays over the block's lanes.
+  R *const area = reinterpret_cast<R *>(lds_raw) + size_t(block) * row;
+  const Pending<R> wave_pending{area + (threadIdx.x & ~63u),
+                                reinterpret_cast<int *>(area + size_t(block) * pending_reals(V4)) +
+                                    (threadIdx.x & ~63u),
+                                block};
   const Pending<R> pending = wave_pending.at(__lane_id());
-  // Per-wave layer table after the pending words (plain-width Stokes modes only: the full
-  // Stokes vector's pending rows leave no room at three blocks per CU).
+  const Record<R> wave_records{area + (threadIdx.x & ~63u),
+                               reinterpret_cast<int *>(area + size_t(block) * record_reals(V4)) +
+                                   (threadIdx.x & ~63u),
+                               block};
+  const Record<R> record = wave_records.at(__lane_id());
+  // Per-wave layer table after that area (plain-width Stokes modes only: the full Stokes
+  // vector's rows leave no room at three blocks per CU).
   constexpr bool LAYERS = !V4;
-  float *layers = reinterpret_cast<float *>(reinterpret_cast<int *>(reinterpret_cast<R *>(lds_raw) +
-                                                                    size_t(block) * (row + pending_reals(V4))) +
-                                            size_t(block) * kPendingWords) +
+  float *layers = reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(area) +
+                                            size_t(block) * pending_bytes<R>(V4)) +
                   (threadIdx.x >> 6) * 4 * row;
   uint32_t layers_scene = 0xffffffffu;
   Counts n{0, 0, 0, 0, 0, 0, 0, 0};
@@ -1561,6 +1699,9 @@ __global__ void __launch_bounds__(kBlock32)
   Trace<R> trace;
   bool active = false;
   uint32_t id = 0;
+  // Layer of the lane's open records, -1 while it has none (every mode but the decomposition).
+  constexpr bool MERGE = !SCATS;
+  int open_layer = -1;
 
   for (;;) {
     const Scene &s = scenes[current];
@@ -1603,18 +1744,108 @@ __global__ void __launch_bounds__(kBlock32)
 
     int status = kRunning;
     prefetch(h.rng);
-    if (active)
-      status = collide<R, TRACE, V4, BACK, SCATS, LAYERS, DOWN>(s, c, failure, history, h, pending, n, trace, layers,
-                                                                row);
-    // Lanes read each other's LDS rows below: keep every access of this wave in order.
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    if (SCATS)
+    Flight<R> flight{false, 0, R(0), R(1)};
+    if (!MERGE) {
+      if (active)
+        status = collide<R, TRACE, V4, BACK, SCATS, LAYERS, DOWN>(s, c, failure, history, h, pending, n, trace,
+                                                                  layers, row);
+      // Lanes read each other's LDS rows below: keep every access of this wave in order.
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
       tally_wave_scats<R, V4>(s, banks, shard, active && status == kRunning, wave_pending, n);
-    else
-      tally_wave<R, V4, DOWN>(s, banks, shard, active && status == kRunning, wave_pending, wave_history, row, n);
-    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-    __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+    } else {
+      bool surface = false;
+      R layer_path = R(0);
+      if (active) status = walk<R, LAYERS>(s, c, failure, history, h, n, layers, row, surface, layer_path);
+      // A flight that left the layer of the open records changed the layer paths the records
+      // hold outside that layer: their write-out takes this flight's part off again.
+      flight = Flight<R>{active && open_layer >= 0 && h.p.k != open_layer, h.p.k, layer_path,
+                         R(1) / fmax(fabs(h.p.c), R(1e-16))};
+      if (active && status == kRunning) status = interact<R, TRACE, false>(s, h, n, trace, surface);
+      const bool event = active && status == kRunning;
+      R rz = R(0), delta = R(0), sin_delta = R(0), cos_delta = R(0), path = R(0);
+      if (event) {
+        rz = event_rz<R, LAYERS, DOWN>(s, h.p, layers, row);
+        // All views share one azimuth: cos and sin of the photon-view azimuth difference, once.
+        delta = h.p.phi - R(s.phi);
+        sin_cos(delta, &sin_delta, &cos_delta);
+        path = history[h.p.k];
+      }
+      // The local estimates toward every view, lane by lane (the detector walks of one lane run
+      // back to back, so the wave waits once for its longest lane).
+      Estimate<R> e0{R(0), R(0), R(0), R(0), -1}, e1 = e0, e2 = e0;
+      if (event) {
+#pragma unroll 1
+        for (int view = 0; view < s.views; ++view) {
+          const Estimate<R> e = estimate<R, TRACE, V4, BACK, LAYERS, DOWN>(s, c, failure, h, view, delta, cos_delta,
+                                                                           sin_delta, rz, n, trace, layers, row, status);
+          if (status != kRunning) break;
+          if (view == 0)
+            e0 = e;
+          else if (view == 1)
+            e1 = e;
+          else
+            e2 = e;
+        }
+      }
+      // Per view: the event's local estimate joins the open record of the same pixel and layer,
+      // or the open record is written out first (another pixel, or the photon left its layer)
+      // and the event opens a new one. One write-out for all views of all lanes. (Lanes read
+      // each other's LDS rows and records: every access of the wave in order around it.)
+      int flushed = 0;
+#pragma unroll
+      for (int view = 0; view < kMaxViews; ++view) {
+        const int pixel = view == 0 ? e0.pixel : (view == 1 ? e1.pixel : e2.pixel);
+        const int held = event && open_layer >= 0 && view < s.views ? record.pixel(view) : -1;
+        if (held >= 0 && (flight.stale || (pixel >= 0 && pixel != held))) flushed |= 1 << view;
+      }
+      const unsigned long long flushing = __ballot(flushed != 0);
+      if (flushing) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        flush_records<R, V4, DOWN>(s, banks, shard, flushing, flushed, wave_records, wave_history, row, open_layer,
+                                   h.round_offset, flight, n);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+      bool open = false;
+#pragma unroll
+      for (int view = 0; view < kMaxViews; ++view) {
+        if (view >= s.views) break;
+        const Estimate<R> &e = view == 0 ? e0 : (view == 1 ? e1 : e2);
+        const int held = event && open_layer >= 0 ? record.pixel(view) : -1;
+        const bool same = !flight.stale && held == e.pixel;
+        if (e.pixel >= 0) {
+          const R part = e.weight * (path + rz * R(s.view_inv_mu[view]));
+          if (same) {
+            record.e(view) += part;
+            record.s(view) += e.weight;
+            record.q(view) += e.q;
+            record.u(view) += e.u;
+            if (V4) record.v(view) += e.v;
+          } else {
+            record.e(view) = part;
+            record.s(view) = e.weight;
+            record.q(view) = e.q;
+            record.u(view) = e.u;
+            if (V4) record.v(view) = e.v;
+            record.pixel(view) = e.pixel;
+          }
+          open = true;
+        } else if (event) {
+          // Nothing toward this view: a record of the same layer stays open.
+          if (held < 0 || flight.stale) record.pixel(view) = -1;
+          open = open || (held >= 0 && !flight.stale);
+        }
+      }
+      if (event) {
+        // The records now hold the layer paths of this event.
+        open_layer = open ? h.p.k : -1;
+        flight.stale = false;
+      }
+    }
     // Russian roulette of the weight window, after the event's local estimates (a photon steered
     // toward a view has used its small weight there) and before its scattering.
     if (window && active && status == kRunning && h.steer < R(guidance.roulette_below)) {
@@ -1679,6 +1910,21 @@ __global__ void __launch_bounds__(kBlock32)
       log.directions[size_t(slot) * 3 + 2] = float(h.p.c);
       log.cumulative[slot] = h.contribution;
       log.photon[slot] = id;
+    }
+    if (MERGE) {
+      // Histories (and copies of split ones) that ended write out their open records before a
+      // new photon or copy takes the lane's layer row.
+      const bool ends = active && status != kRunning;
+      const unsigned long long ending = __ballot(ends && open_layer >= 0);
+      if (ending) {
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        flush_records<R, V4, DOWN>(s, banks, shard, ending, (1 << s.views) - 1, wave_records, wave_history, row,
+                                   open_layer, h.round_offset, flight, n);
+        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+      }
+      if (ends) open_layer = -1;
     }
     if (!active || status == kRunning) continue;
     if (STEER && top > 0) {
@@ -2180,9 +2426,9 @@ int block_size(const Session *h) {
 
 template <typename R>
 size_t lds_bytes_for(int row, int block, bool v4) {
-  // History rows and pending rows per lane; pending words; per-wave layer tables (4 rows of
-  // floats per wave) except with the full Stokes vector.
-  return (size_t(row) + pending_reals(v4)) * block * sizeof(R) + size_t(kPendingWords) * block * sizeof(int) +
+  // History rows and pending local estimates or open records per lane; per-wave layer tables
+  // (4 rows of floats per wave) except with the full Stokes vector.
+  return (size_t(row) * sizeof(R) + pending_bytes<R>(v4)) * block +
          (v4 ? 0 : size_t(block / 64) * 4 * row * sizeof(float));
 }
 

@@ -25,7 +25,7 @@ extern "C" {
  * Replaces monki_metal_abi_version (kernels/metal/bridge.m:26-28). */
 int monki_hip_abi_version(void);
 
-/* Validate packed float32 tables (layout v1, monki/execution/hip_tables.py), select the
+/* Validate packed float32 tables (layout v2, monki/execution/hip_tables.py), select the
  * device, upload tables, and reserve tally banks: tally_count floats per round (all shards)
  * for round_capacity rounds. Returns NULL and fills error on failure.
  * Replaces monki_metal_create (kernels/metal/bridge.m:30-84); no kernel source: the

@@ -26,8 +26,8 @@ namespace {
 constexpr int kMaxLayers = 64;
 constexpr int kMaxSpecies = 4;
 constexpr int kMaxViews = 3;
-constexpr int kHeaderWords = 64;
-constexpr int kLayoutVersion = 1;
+constexpr int kHeaderWords = 72;
+constexpr int kLayoutVersion = 2;
 constexpr int kBlock32 = 256;
 constexpr int kBlock64 = 128;
 constexpr int kCounters = 8;
@@ -142,6 +142,10 @@ struct Scene {
   double view_mu[kMaxViews], view_sine[kMaxViews], view_a[kMaxViews], view_b[kMaxViews];
   double view_geometry[kMaxViews], view_inv_mu[kMaxViews];
   const float *zunit, *albedo, *beta, *cumfrac, *ssalb_gas, *height_above, *tau_above;
+  // Factored extinction (one tabulated species), else null: beta = species_beta[cell] +
+  // gas_beta[layer] and the species fraction = species_beta[cell] / beta, exactly the float32
+  // tables. Scenes of a session with equal species fields share one device copy of it.
+  const float *species_beta, *gas_beta;
   const float *uniform;  // per layer: 1 if every column has the same extinction and mix
   SpeciesTables sp[kMaxSpecies];
   // Detector and result grid at the surface: views point down (view_mu < 0), the detector ray
@@ -514,6 +518,12 @@ __device__ inline bool sample_direction(const Scene &s, int species, const Stoke
   return false;
 }
 
+// Extinction of a cell (1/km) in layer k: the table, or the factored parts with the float32
+// arithmetic that produced the table (the same value).
+__device__ inline float cell_extinction(const Scene &s, int cell, int k) {
+  return s.species_beta ? s.species_beta[cell] + s.gas_beta[k] : s.beta[cell];
+}
+
 template <typename R>
 __device__ inline int cell_index(const Scene &s, int i, int j, int k) {
   return i + s.nx * (j + s.ny * k);
@@ -648,9 +658,10 @@ __device__ inline DetectorHit detector_walk(const Scene &s, const Photon<R> &eve
       const int j = projected_column<R>(ray.j, ray.ry, vb, distance, R(s.yunit), s.ny);
       return DetectorHit{i, j, true};
     }
-    const int cell = cell_index<R>(s, ray.i, ray.j, ray.k);
+    const int layer = ray.k;
+    const int cell = cell_index<R>(s, ray.i, ray.j, layer);
     const R d = advance<R>(s, ray);
-    tau += d * R(s.beta[cell]);
+    tau += d * R(cell_extinction(s, cell, layer));
     ++n.detector;
   }
   return DetectorHit{ray.i, ray.j, true};
@@ -842,7 +853,7 @@ __device__ inline int walk(const Scene &s, const Controls &c, unsigned *failure,
     const bool uniform_layer = LAYERS ? layer_extinction >= 0.0f : s.uniform[p.k] != 0.0f;
     if (!uniform_layer) flush_slide();
     const R extinction = LAYERS && uniform_layer ? R(layer_extinction)
-                         : (p.k < s.nz - 1 && p.k >= 0) ? R(s.beta[cell_index<R>(s, p.i, p.j, p.k)])
+                         : (p.k < s.nz - 1 && p.k >= 0) ? R(cell_extinction(s, cell_index<R>(s, p.i, p.j, p.k), p.k))
                                                         : R(0);
     const R event_distance = extinction > eps ? h.remains / extinction : R(INFINITY);
     if (uniform_layer) {
@@ -929,10 +940,16 @@ __device__ inline int interact(const Scene &s, History<R> &h, Counts &n, Trace<R
     const float draw = uniform(h.rng);
     const int cell = cell_index<R>(s, p.i, p.j, p.k);
     const int cells = s.nx * s.ny * s.nz;
-    for (int t = 0; t < s.species; ++t) {
-      if (draw < s.cumfrac[t * cells + cell]) {
-        species = t;
-        break;
+    if (s.species_beta) {
+      // One tabulated species: its fraction from the factored parts (the table's value).
+      const float part = s.species_beta[cell];
+      if (draw < part / (part + s.gas_beta[p.k])) species = 0;
+    } else {
+      for (int t = 0; t < s.species; ++t) {
+        if (draw < s.cumfrac[t * cells + cell]) {
+          species = t;
+          break;
+        }
       }
     }
     h.weight *= species < 0 ? R(s.ssalb_gas[p.k]) : R(s.sp[species].ssalb);
@@ -1795,10 +1812,10 @@ __global__ void __launch_bounds__(kBlock32)
       // and the event opens a new one. One write-out for all views of all lanes. (Lanes read
       // each other's LDS rows and records: every access of the wave in order around it.)
       int flushed = 0;
-#pragma unroll
-      for (int view = 0; view < kMaxViews; ++view) {
+#pragma unroll 1
+      for (int view = 0; view < s.views; ++view) {
         const int pixel = view == 0 ? e0.pixel : (view == 1 ? e1.pixel : e2.pixel);
-        const int held = event && open_layer >= 0 && view < s.views ? record.pixel(view) : -1;
+        const int held = event && open_layer >= 0 ? record.pixel(view) : -1;
         if (held >= 0 && (flight.stale || (pixel >= 0 && pixel != held))) flushed |= 1 << view;
       }
       const unsigned long long flushing = __ballot(flushed != 0);
@@ -1811,10 +1828,9 @@ __global__ void __launch_bounds__(kBlock32)
         __builtin_amdgcn_wave_barrier();
       }
       bool open = false;
-#pragma unroll
-      for (int view = 0; view < kMaxViews; ++view) {
-        if (view >= s.views) break;
-        const Estimate<R> &e = view == 0 ? e0 : (view == 1 ? e1 : e2);
+#pragma unroll 1
+      for (int view = 0; view < s.views; ++view) {
+        const Estimate<R> e = view == 0 ? e0 : (view == 1 ? e1 : e2);
         const int held = event && open_layer >= 0 ? record.pixel(view) : -1;
         const bool same = !flight.stale && held == e.pixel;
         if (e.pixel >= 0) {
@@ -2277,6 +2293,30 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
     }
     *span.target = base + offset;
   }
+  // Factored extinction (words 64-65): one tabulated species, both parts in range, and on the
+  // host pass every cell reproduces the extinction and fraction tables bit for bit.
+  s.species_beta = s.gas_beta = nullptr;
+  if (t[64] != 0.0f || t[65] != 0.0f) {
+    int species_at = 0, gas_at = 0;
+    if (s.species != 1 || !word_int(t, 64, kHeaderWords, length, species_at) || species_at + cells > length ||
+        !word_int(t, 65, kHeaderWords, length, gas_at) || gas_at + s.nz > length) {
+      set_error(error, size, "Invalid factored extinction");
+      return false;
+    }
+    if (!base) {
+      const float *beta = t + (s.beta - base), *fraction = t + (s.cumfrac - base);
+      const float *part = t + species_at, *gas = t + gas_at;
+      for (long cell = 0; cell < cells; ++cell) {
+        const float total = part[cell] + gas[cell / (long(s.nx) * s.ny)];
+        if (total != beta[cell] || (total > 0.0f && part[cell] / total != fraction[cell])) {
+          set_error(error, size, "Factored extinction does not reproduce the tables");
+          return false;
+        }
+      }
+    }
+    s.species_beta = base + species_at;
+    s.gas_beta = base + gas_at;
+  }
   for (int sp = 0; sp < kMaxSpecies; ++sp) {
     SpeciesTables &tab = s.sp[sp];
     tab = SpeciesTables{0,       0.0f,    1.0f,    1.0f,    0,       nullptr, nullptr,
@@ -2664,6 +2704,20 @@ Session *create_session(uint32_t device, const float *tables, const size_t *offs
     parse_tables(tables + offsets[i], offsets[i + 1] - offsets[i], h->tables + offsets[i], view, nullptr, 0);
     view.bank_base = parsed[i].bank_base;
     for (int sp = 0; sp < view.species; ++sp) view.sp[sp].cdf_start = h->buckets + bucket_base[next_bucket++];
+    // A species field equal to an earlier scene's (one cloud at several wavelengths): read that
+    // scene's copy, so that the batch keeps one copy of the field in the caches.
+    if (view.species_beta) {
+      const size_t cells = size_t(view.nx) * view.ny * view.nz;
+      const float *mine = tables + (view.species_beta - h->tables);
+      for (uint32_t j = 0; j < i; ++j) {
+        const Scene &other = h->scenes[j];
+        if (!other.species_beta || other.nx != view.nx || other.ny != view.ny || other.nz != view.nz) continue;
+        if (!memcmp(tables + (other.species_beta - h->tables), mine, cells * sizeof(float))) {
+          view.species_beta = other.species_beta;
+          break;
+        }
+      }
+    }
     h->scenes.push_back(view);
   }
   if (!check(hipMemcpy(h->d_scenes, h->scenes.data(), scene_count * sizeof(Scene), hipMemcpyHostToDevice), error,

@@ -265,11 +265,42 @@ def spectral_cloud_field(wavelength_nm: float, field: np.ndarray, nmat: int = 3,
     levels = np.asarray(SPECTRAL_LEVELS_KM)
     n = field.shape[0]
     cloud = cloud_layer_depths(levels, 1.0, 4.0, 1.0)[None, None, :] * field[:, :, None]
-    return _assemble(
+    rayleigh_tau = 0.70 * (wavelength_nm / 340.0) ** -4.08
+    ozone_tau = 0.01 * np.exp(-(wavelength_nm - 340.0) / 12.0)
+    scene = _assemble(
         f"spectral-{wavelength_nm:.1f}nm", n, field.shape[1], levels, [cloud], [water_cloud()], nmat, solar_zenith,
-        albedo, wavelength_nm=float(wavelength_nm), rayleigh_tau=0.70 * (wavelength_nm / 340.0) ** -4.08,
-        ozone_tau=0.01 * np.exp(-(wavelength_nm - 340.0) / 12.0),
+        albedo, wavelength_nm=float(wavelength_nm), rayleigh_tau=rayleigh_tau, ozone_tau=ozone_tau,
     )  # fmt: skip
+    thickness = np.diff(levels)
+    rayleigh, ozone = _gas_profile(levels, rayleigh_tau, ozone_tau)
+    return _factor_extinction(scene, cloud / thickness, (rayleigh + ozone) / thickness)
+
+
+def _factor_extinction(scene: TransportScene, species_extinction, gas_extinction) -> TransportScene:
+    """Rebuild the extinction and species-fraction tables of a one-species scene from float32
+    parts (species per cell, gas per layer), so that beta = species + gas and the fraction =
+    species / beta hold exactly in float32, and keep the parts: every wavelength of one cloud
+    field then carries the same species array (differences from the float64 tables: one float32
+    rounding)."""
+
+    nx, ny, nz = scene.nx, scene.ny, scene.nz
+    species = np.zeros((nx, ny, nz), dtype=np.float32)
+    species[:, :, : nz - 1] = np.asarray(species_extinction, dtype=np.float64)
+    gas = np.zeros(nz, dtype=np.float32)
+    gas[: nz - 1] = np.asarray(gas_extinction, dtype=np.float64)
+    beta = species + gas[None, None, :]
+    positive = beta > 0
+
+    with np.errstate(invalid="ignore", divide="ignore"):
+        fraction = np.where(positive, species / beta, np.float32(0))
+
+    fractions = np.zeros_like(scene.fractions)
+    fractions[0] = fraction
+    fractions[-1] = np.where(positive, 1.0 - fractions[0], scene.fractions[-1])
+    scene.beta = beta.astype(np.float64)
+    scene.fractions = fractions
+    scene.extinction_parts = (species, gas)
+    return scene
 
 
 def spectral_scenes(count: int = 64, n: int = 128, seed: int = 7, nmat: int = 3) -> list:

@@ -6,10 +6,10 @@ tabulated species (cumulative extinction fractions per species, gas last and imp
 explicit offsets in the header instead of a fixed order, and scene constants (sizes,
 geometry, shard count) as header words rather than compiled-in defines.
 
-Header (64 float32 words; integers are stored exactly, all < 2**24):
+Header (72 float32 words; integers are stored exactly, all < 2**24):
 
 ====  =====================================================================
-0     layout version (1)
+0     layout version (2)
 1-9   nx, ny, nz (incl. vacuum), nmat, views, species, paths, uniform_top, shards
 10-15 xunit, yunit (km), depol, propagation zenith, solar azimuth, view azimuth (rad)
 16-18 view zeniths (rad)
@@ -24,6 +24,9 @@ Header (64 float32 words; integers are stored exactly, all < 2**24):
 31    result grid on the other level than the detector: 1 (0: at the detector)
 32+8s species s: LUT size, albedo, offsets of cosine block (cos, 1-cos, 1+cos),
       CDF, F11, F22, F33, F12; with nmat 4, F34 and F44 follow F12 directly (same length)
+64-65 factored extinction (one tabulated species; 0 when absent): offsets of the species'
+      extinction per cell and of the gas extinction per layer, with beta = species + gas
+      and the species fraction = species / beta exactly in float32
 ====  =====================================================================
 
 Grids are x-fastest (Fortran order). ``height_above[k]``/``tau_above[k]`` are the thickness
@@ -34,8 +37,8 @@ highest layer whose extinction or species mix varies across columns.
 
 import numpy as np
 
-HEADER_WORDS = 64
-LAYOUT_VERSION = 1
+HEADER_WORDS = 72
+LAYOUT_VERSION = 2
 MAX_LAYERS = 64
 MAX_SPECIES = 4
 MAX_VIEWS = 3
@@ -118,6 +121,10 @@ def pack_tables(scene, shards: int = 32, backward: bool = False):
     header[29] = 1.0 if scene.save_scats else 0.0
     header[30] = 1.0 if scene.detector == "surface" else 0.0
     header[31] = 1.0 if (scene.register or scene.detector) != scene.detector else 0.0
+
+    if scene.extinction_parts is not None:
+        header[64] = place(scene.extinction_parts[0])
+        header[65] = place(scene.extinction_parts[1])
 
     for index, sp in enumerate(scene.species):
         cdf = sp.bound_cdf(nmat) if sp.cdf is None else sp.cdf

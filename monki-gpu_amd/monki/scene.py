@@ -101,6 +101,12 @@ class TransportScene:
     wavelength_nm: float = 340.0
     altitude_m: Optional[np.ndarray] = None  # (nz,) level altitudes, bottom to TOA
     description: str = ""
+    # Optional factored extinction of a scene with one tabulated species: float32 arrays
+    # (species extinction (nx, ny, nz), gas extinction (nz,)) from which the float32 tables
+    # follow exactly: beta = species + gas[k] and the species fraction = species / beta, both in
+    # float32 arithmetic. Scenes of a batch that share the species field (a cloud seen at several
+    # wavelengths) then share one device copy of it.
+    extinction_parts: Optional[Tuple[np.ndarray, np.ndarray]] = None
 
     def __post_init__(self):
         self.zunit = np.asarray(self.zunit, dtype=np.float64)
@@ -174,6 +180,9 @@ class TransportScene:
         if np.any((self.albedo < 0) | (self.albedo > 1)):
             raise ValueError("surface albedo must be in [0, 1]")
 
+        if self.extinction_parts is not None:
+            self.check_extinction_parts()
+
         if not 0 <= self.depol < 0.5:
             raise ValueError("depolarization factor must be in [0, 0.5)")
 
@@ -221,6 +230,33 @@ class TransportScene:
 
             if not 0 <= sp.ssalb <= 1:
                 raise ValueError(f"{sp.name}: single-scattering albedo must be in [0, 1]")
+
+    def check_extinction_parts(self) -> None:
+        """The factored extinction must give back the float32 tables bit for bit."""
+
+        if len(self.species) != 1:
+            raise ValueError("factored extinction needs exactly one tabulated species")
+
+        species, gas = (np.asarray(part) for part in self.extinction_parts)
+
+        if species.dtype != np.float32 or gas.dtype != np.float32:
+            raise ValueError("extinction parts must be float32 arrays")
+
+        if species.shape != (self.nx, self.ny, self.nz) or gas.shape != (self.nz,):
+            raise ValueError("extinction parts must be shaped (nx, ny, nz) and (nz,)")
+
+        beta = species + gas[None, None, :]
+
+        if not np.array_equal(beta, self.beta.astype(np.float32)):
+            raise ValueError("extinction parts do not reproduce the extinction table in float32")
+
+        positive = beta > 0
+
+        with np.errstate(invalid="ignore", divide="ignore"):
+            fraction = species / beta
+
+        if not np.array_equal(fraction[positive], self.fractions[0].astype(np.float32)[positive]):
+            raise ValueError("extinction parts do not reproduce the species fraction in float32")
 
     def varying_layers(self) -> np.ndarray:
         """Layers whose extinction or species mix changes across columns."""

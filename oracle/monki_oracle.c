@@ -41,7 +41,7 @@
  * The detector walk is always explicit, cell by cell, with the analytical pixel mapping
  * of mc_routines_cartesian_mod.f90:1076-1105: no uniform-top shortcut.
  *
- * Input is the packed float32 table buffer of the product (layout v1), widened to double. Its
+ * Input is the packed float32 table buffer of the product (layout v2), widened to double. Its
  * cosine block carries 1 - cos and 1 + cos next to cos, so sin^2 of a sampled scattering angle
  * keeps float64 precision near the forward peak although the cosines are stored in float32.
  */
@@ -53,7 +53,7 @@
 #define MAX_LAYERS 64
 #define MAX_SPECIES 4
 #define MAX_VIEWS 3
-#define HEADER 64
+#define HEADER 72
 #define EPS 1.0e-16
 #define PI 3.14159265358979323846
 #define TRACE_WORDS 8
@@ -685,9 +685,10 @@ static int photon_history(const scene_t *s, uint32_t photon, uint32_t round, uin
 
 static int word(const float *t, int index) { return (int)t[index]; }
 
-/* Parse the packed float32 tables (layout v1). Returns 0 on success. */
+/* Parse the packed float32 tables (layout v2; the oracle reads the expanded extinction and
+   fraction tables, not the factored parts of words 64-65). Returns 0 on success. */
 static int parse(const float *t, size_t count, scene_t *s) {
-  if (count < HEADER || t[0] != 1.0f || (size_t)t[26] != count) return 1;
+  if (count < HEADER || t[0] != 2.0f || (size_t)t[26] != count) return 1;
   s->nx = word(t, 1), s->ny = word(t, 2), s->nz = word(t, 3), s->nmat = word(t, 4);
   s->views = word(t, 5), s->species = word(t, 6), s->paths = word(t, 7), s->backward = word(t, 28);
   s->scats = word(t, 29);

@@ -548,9 +548,35 @@ def test_spectral_cloud_field_statistics():
     assert tau[tau > 0].mean() == pytest.approx(12.0)
     short, long = (configs.spectral_cloud_field(w, tau[:8, :8]) for w in (300.0, 500.0))
     assert short.nz == 60 and len(configs.SPECTRAL_LEVELS_KM) == 60
+    # The tables hold float32 values (factored extinction): the gas column to float32 precision.
     gas = lambda scene: (scene.beta[0, 0, :-1] * scene.zunit[:-1] * scene.fractions[-1, 0, 0, :-1]).sum()  # noqa: E731
     rayleigh = lambda scene: gas(scene) - 0.01 * np.exp(-(scene.wavelength_nm - 340) / 12)  # noqa: E731
-    assert rayleigh(short) / rayleigh(long) == pytest.approx((300 / 500) ** -4.08, rel=1e-9)
+    assert rayleigh(short) / rayleigh(long) == pytest.approx((300 / 500) ** -4.08, rel=1e-6)
+    # Every wavelength carries the same cloud extinction; tables and parts agree in float32.
+    assert np.array_equal(short.extinction_parts[0], long.extinction_parts[0])
+    for scene in (short, long):
+        scene.check_extinction_parts()
+
+
+def test_factored_extinction_must_reproduce_the_tables():
+    """The factored extinction is refused unless it gives back the float32 tables bit for bit."""
+
+    from monki import configs
+
+    scene = configs.spectral_cloud_field(400.0, configs.stochastic_cloud_field(16, seed=3))
+    species, gas = scene.extinction_parts
+    scene.validate()
+    nudged = gas.copy()
+    nudged[5] = np.nextafter(nudged[5], np.float32(1))
+    scene.extinction_parts = (species, nudged)
+
+    with pytest.raises(ValueError, match="reproduce the extinction table"):
+        scene.validate()
+
+    scene.extinction_parts = (species.astype(np.float64), gas)
+
+    with pytest.raises(ValueError, match="float32"):
+        scene.validate()
 
 
 def test_directed_settings_validate_and_build_species_masks():

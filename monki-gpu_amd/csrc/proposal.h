@@ -3,15 +3,17 @@
 // the outgoing direction. No reference counterpart: the reference samples directions from the
 // phase function only (mc_routines_general_mod.f90:492-932).
 //
-// The network runs on the matrix cores with the batch on the N side of
-// v_mfma_f32_32x32x8_bf16: D = W (M x K) * H (K x 32 samples). Lane t of the wave holds, for
-// sample column n = t % 32 and half h = t / 32,
-//   B operand (K x 32): rows 8 s + 4 h + e, e = 0..3, of k-step s;
-//   D (32 x 32):        rows 8 q + 4 h + i of register 4 q + i.
-// So the accumulator rows of M-block mb, registers 4 q .. 4 q + 3, are exactly the B operand of
-// k-step 4 mb + q of the next layer: layers chain in registers, with no transpose. Weights are
-// stored pre-swizzled in A-operand order (lane t holds W[32 mb + t % 32][8 s + 4 h + e]) so
-// each operand load is one coalesced 8-byte load per lane.
+// The network runs on gfx950's matrix cores with the batch on the N side of
+// v_mfma_f32_32x32x16_bf16: D = W (M x K) * H (K x 32 samples), 16 reduction rows per
+// instruction. Lane t of the wave holds, for sample column n = t % 32 and half h = t / 32,
+//   A, B operands: 8 values of reduction slots 8 h + e, e = 0..7, of k-step s;
+//   D (32 x 32):   rows 8 q + 4 h + i of register 4 q + i.
+// Reduction slot 8 h + e of k-step s stands for feature row 16 s + 4 h + (e & 3) + 8 (e >> 2).
+// With that order, registers 8 j .. 8 j + 7 of the accumulator of M-block mb are exactly the B
+// operand of k-step 2 mb + j of the next layer: layers chain in registers, with no transpose.
+// Weights are stored pre-swizzled in the same A-operand order (lane t holds
+// W[32 mb + t % 32][feature row of slot 8 h + e]) so each operand load is one coalesced 16-byte
+// load per lane.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,29 +41,38 @@ __host__ __device__ constexpr int param_offset(int l) {
 }
 constexpr int kParams = param_offset(kLayers);
 
-// Packed bf16 operands, in units of 4 bf16 (one lane's operand of one block).
+// Packed bf16 operands, in units of 8 bf16 (one lane's operand of one block).
 __host__ __device__ constexpr int packed_offset(int l) {
-  return l == 0 ? 0 : packed_offset(l - 1) + (rows(l - 1) / 32) * (cols(l - 1) / 8) * 64;
+  return l == 0 ? 0 : packed_offset(l - 1) + (rows(l - 1) / 32) * (cols(l - 1) / 16) * 64;
 }
 constexpr int kPacked = packed_offset(kLayers);
 // Transposed operands W^T (for the backward pass) of layers 1..4: M' = cols(l), K' = rows(l).
 __host__ __device__ constexpr int packed_t_offset(int l) {
-  return l <= 1 ? 0 : packed_t_offset(l - 1) + (cols(l - 1) / 32) * (rows(l - 1) / 8) * 64;
+  return l <= 1 ? 0 : packed_t_offset(l - 1) + (cols(l - 1) / 32) * (rows(l - 1) / 16) * 64;
 }
 constexpr int kPackedT = packed_t_offset(kLayers);
 // Padded biases (float32).
 __host__ __device__ constexpr int bias_offset(int l) { return l == 0 ? 0 : bias_offset(l - 1) + rows(l - 1); }
 constexpr int kBiases = bias_offset(kLayers);
 
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
+// Feature row of reduction slot `slot` (0..15) of a k-step: 4 h + (e & 3) + 8 (e >> 2).
+__host__ __device__ constexpr int slot_row(int slot) { return 4 * (slot >> 3) + (slot & 3) + 8 * ((slot >> 2) & 1); }
+
+typedef short bf16x8 __attribute__((ext_vector_type(8)));  // bf16 bit patterns
+typedef __bf16 mfma_bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 
 // Device view of one network.
 struct Net {
-  const bf16x4 *packed;    // kPacked operands
-  const bf16x4 *packed_t;  // kPackedT operands (training only)
+  const bf16x8 *packed;    // kPacked operands
+  const bf16x8 *packed_t;  // kPackedT operands (training only)
   const float *bias;       // kBiases
 };
+
+__device__ inline floatx16 mfma(bf16x8 a, bf16x8 b, floatx16 c) {
+  return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(mfma_bf16x8, a),
+                                                 __builtin_bit_cast(mfma_bf16x8, b), c, 0, 0, 0);
+}
 
 // float32 -> bfloat16, round to nearest even (finite inputs).
 __host__ __device__ inline uint16_t bf16_bits(float x) {
@@ -78,8 +89,12 @@ __host__ __device__ inline float bf16_value(uint16_t bits) {
   return x;
 }
 
-__device__ inline bf16x4 pack4(float a, float b, float c, float d) {
-  return bf16x4{short(bf16_bits(a)), short(bf16_bits(b)), short(bf16_bits(c)), short(bf16_bits(d))};
+// Eight accumulator values (registers 8 j .. 8 j + 7) as one operand.
+__device__ inline bf16x8 pack8(const float (&v)[8]) {
+  bf16x8 out;
+#pragma unroll
+  for (int e = 0; e < 8; ++e) out[e] = short(bf16_bits(v[e]));
+  return out;
 }
 
 __device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
@@ -93,8 +108,8 @@ __device__ inline float silu_grad(float z) {
 // operands (SiLU) or raw outputs. Blocks = rows / 32. z_out, when given, receives the
 // pre-activations in D order [block][register][lane].
 template <int Steps, int Blocks, bool Activate>
-__device__ inline void dense(const bf16x4 *__restrict__ w, const float *__restrict__ bias,
-                             const bf16x4 (&x)[16], bf16x4 (&y)[16], floatx16 (&raw)[2],
+__device__ inline void dense(const bf16x8 *__restrict__ w, const float *__restrict__ bias,
+                             const bf16x8 (&x)[8], bf16x8 (&y)[8], floatx16 (&raw)[2],
                              float *__restrict__ z_out) {
   const int lane = __lane_id();
   const int h = lane >> 5;
@@ -102,8 +117,7 @@ __device__ inline void dense(const bf16x4 *__restrict__ w, const float *__restri
   for (int mb = 0; mb < Blocks; ++mb) {
     floatx16 c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-    for (int s = 0; s < Steps; ++s)
-      c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(w[(mb * Steps + s) * 64 + lane], x[s], c, 0, 0, 0);
+    for (int s = 0; s < Steps; ++s) c = mfma(w[(mb * Steps + s) * 64 + lane], x[s], c);
 #pragma unroll
     for (int v = 0; v < 16; ++v) c[v] += bias[32 * mb + 8 * (v >> 2) + 4 * h + (v & 3)];
     if (z_out) {
@@ -112,8 +126,12 @@ __device__ inline void dense(const bf16x4 *__restrict__ w, const float *__restri
     }
     if (Activate) {
 #pragma unroll
-      for (int q = 0; q < 4; ++q)
-        y[4 * mb + q] = pack4(silu(c[4 * q]), silu(c[4 * q + 1]), silu(c[4 * q + 2]), silu(c[4 * q + 3]));
+      for (int j = 0; j < 2; ++j) {
+        float v[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = silu(c[8 * j + e]);
+        y[2 * mb + j] = pack8(v);
+      }
     } else {
       raw[mb] = c;
     }
@@ -123,21 +141,21 @@ __device__ inline void dense(const bf16x4 *__restrict__ w, const float *__restri
   }
 }
 
-// Forward pass of one 32-sample block. x: input B operands (k-steps 0, 1; rows >= 12 zero).
+// Forward pass of one 32-sample block. x: input B operand (k-step 0; rows >= 12 zero).
 // raw: the 64 output rows in D order (rows >= 40 are padding). z (optional): pre-activations
 // of the four hidden layers, 4 x 4 x 16 x 64 floats.
-__device__ inline void forward_block(const Net &net, const bf16x4 (&x)[16], floatx16 (&raw)[2],
+__device__ inline void forward_block(const Net &net, const bf16x8 (&x)[8], floatx16 (&raw)[2],
                                      float *__restrict__ z) {
-  bf16x4 a[16], b[16];
-  dense<2, 4, true>(net.packed + packed_offset(0), net.bias + bias_offset(0), x, a, raw,
+  bf16x8 a[8], b[8];
+  dense<1, 4, true>(net.packed + packed_offset(0), net.bias + bias_offset(0), x, a, raw,
                     z ? z : nullptr);
-  dense<16, 4, true>(net.packed + packed_offset(1), net.bias + bias_offset(1), a, b, raw,
-                     z ? z + 1 * 4096 : nullptr);
-  dense<16, 4, true>(net.packed + packed_offset(2), net.bias + bias_offset(2), b, a, raw,
-                     z ? z + 2 * 4096 : nullptr);
-  dense<16, 4, true>(net.packed + packed_offset(3), net.bias + bias_offset(3), a, b, raw,
-                     z ? z + 3 * 4096 : nullptr);
-  dense<16, 2, false>(net.packed + packed_offset(4), net.bias + bias_offset(4), b, a, raw, nullptr);
+  dense<8, 4, true>(net.packed + packed_offset(1), net.bias + bias_offset(1), a, b, raw,
+                    z ? z + 1 * 4096 : nullptr);
+  dense<8, 4, true>(net.packed + packed_offset(2), net.bias + bias_offset(2), b, a, raw,
+                    z ? z + 2 * 4096 : nullptr);
+  dense<8, 4, true>(net.packed + packed_offset(3), net.bias + bias_offset(3), a, b, raw,
+                    z ? z + 3 * 4096 : nullptr);
+  dense<8, 2, false>(net.packed + packed_offset(4), net.bias + bias_offset(4), b, a, raw, nullptr);
 }
 
 // Output m of the block's sample column n = lane % 32, from the D registers of the wave.
@@ -159,8 +177,9 @@ __device__ inline void gather_outputs(const floatx16 (&raw)[2], bool keep, float
   }
 }
 
-// Input B operands of block nb from per-lane features f[12] (sample of lane 32 nb + n).
-__device__ inline void scatter_inputs(const float (&f)[kInputs], int nb, bf16x4 (&x)[16]) {
+// Input B operand of block nb from per-lane features f[12] (sample of lane 32 nb + n): lane
+// half h holds feature rows slot_row(8 h + e), e = 0..7.
+__device__ inline void scatter_inputs(const float (&f)[kInputs], int nb, bf16x8 (&x)[8]) {
   const int lane = __lane_id();
   const int source = 32 * nb + (lane & 31);
   const bool upper = lane >= 32;
@@ -169,15 +188,10 @@ __device__ inline void scatter_inputs(const float (&f)[kInputs], int nb, bf16x4 
   for (int k = 0; k < kInputs; ++k) v[k] = __shfl(f[k], source, 64);
 #pragma unroll
   for (int k = kInputs; k < 16; ++k) v[k] = 0.0f;
-  // rows 8 s + 4 h + e
+  float e[8];
 #pragma unroll
-  for (int s = 0; s < 2; ++s) {
-    float e0 = upper ? v[8 * s + 4] : v[8 * s + 0];
-    float e1 = upper ? v[8 * s + 5] : v[8 * s + 1];
-    float e2 = upper ? v[8 * s + 6] : v[8 * s + 2];
-    float e3 = upper ? v[8 * s + 7] : v[8 * s + 3];
-    x[s] = pack4(e0, e1, e2, e3);
-  }
+  for (int i = 0; i < 8; ++i) e[i] = upper ? v[slot_row(8 + i)] : v[slot_row(i)];
+  x[0] = pack8(e);
 }
 
 // ---- von Mises-Fisher mixture -----------------------------------------------------------

@@ -10,8 +10,8 @@
 // One wave trains on 64 samples per launch block: the forward pass runs on the matrix cores
 // (proposal.h), keeping the hidden pre-activations in a scratch buffer; the backward pass
 // chains W^T products the same way, and the weight gradients dW = dZ H^T come from two
-// 128 x 64 bf16 tiles in LDS (samples on the reduction side) and one coalesced float atomic
-// per gradient element and wave.
+// 128 x 64 bf16 tiles in LDS (samples on the reduction side, 16 per instruction) and one
+// coalesced float atomic per gradient element and wave.
 
 #include <hip/hip_runtime.h>
 
@@ -30,13 +30,15 @@ using namespace monki_net;
 
 namespace {
 
-constexpr int kTileStride = 68;  // bf16 per tile row: 64 samples + 4 (bank spread)
+// bf16 per tile row: 64 samples + 8, so that rows start 16-byte aligned and the 16 lanes of a
+// 16-byte LDS read phase hit distinct banks.
+constexpr int kTileStride = 72;
 constexpr int kZPerWave = 2 * 4 * 4096;
 
 struct Proposal {
   int device = -1;
   float *params = nullptr, *grad = nullptr, *adam_m = nullptr, *adam_v = nullptr;
-  bf16x4 *packed = nullptr, *packed_t = nullptr;
+  bf16x8 *packed = nullptr, *packed_t = nullptr;
   float *bias = nullptr;
   float *z = nullptr;
   size_t z_waves = 0;
@@ -67,33 +69,37 @@ void fail(char *error, size_t size, const char *text) {
 // ---- Kernels -----------------------------------------------------------------------------
 
 // Master parameters -> padded bf16 operands (A order and transposed) and padded biases.
-__global__ void pack_kernel(const float *__restrict__ params, bf16x4 *__restrict__ packed,
-                            bf16x4 *__restrict__ packed_t, float *__restrict__ bias) {
+__global__ void pack_kernel(const float *__restrict__ params, bf16x8 *__restrict__ packed,
+                            bf16x8 *__restrict__ packed_t, float *__restrict__ bias) {
   const int index = blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = index & 63;
   const int block = index >> 6;
   auto weight = [&](int l, int r, int c) -> float {
     return r < true_rows(l) && c < true_cols(l) ? params[param_offset(l) + r * true_cols(l) + c] : 0.0f;
   };
-  // Forward operands: block (mb, s) of layer l, lane -> W[32 mb + lane % 32][8 s + 4 h + e].
+  // Forward operands: block (mb, s) of layer l, lane -> W[32 mb + lane % 32][16 s + slot_row(8 h + e)].
   if (index < kPacked) {
     int l = 0;
     while (l + 1 < kLayers && index >= packed_offset(l + 1)) ++l;
     const int local = (index - packed_offset(l)) >> 6;
-    const int steps = cols(l) / 8;
+    const int steps = cols(l) / 16;
     const int mb = local / steps, s = local % steps;
-    const int r = 32 * mb + (lane & 31), c = 8 * s + 4 * (lane >> 5);
-    packed[index] = pack4(weight(l, r, c), weight(l, r, c + 1), weight(l, r, c + 2), weight(l, r, c + 3));
+    const int r = 32 * mb + (lane & 31);
+    float v[8];
+    for (int e = 0; e < 8; ++e) v[e] = weight(l, r, 16 * s + slot_row(8 * (lane >> 5) + e));
+    packed[index] = pack8(v);
   }
   // Transposed operands of layers 1..4: A'[m'][k'] = W[k'][m'], M' = cols(l), K' = rows(l).
   if (index < kPackedT) {
     int l = 1;
     while (l + 1 < kLayers && index >= packed_t_offset(l + 1)) ++l;
     const int local = (index - packed_t_offset(l)) >> 6;
-    const int steps = rows(l) / 8;
+    const int steps = rows(l) / 16;
     const int mb = local / steps, s = local % steps;
-    const int m = 32 * mb + (lane & 31), k = 8 * s + 4 * (lane >> 5);
-    packed_t[index] = pack4(weight(l, k, m), weight(l, k + 1, m), weight(l, k + 2, m), weight(l, k + 3, m));
+    const int m = 32 * mb + (lane & 31);
+    float v[8];
+    for (int e = 0; e < 8; ++e) v[e] = weight(l, 16 * s + slot_row(8 * (lane >> 5) + e), m);
+    packed_t[index] = pack8(v);
   }
   (void)block;
   if (index < kBiases) {
@@ -177,8 +183,8 @@ __global__ void __launch_bounds__(64)
                  const float *__restrict__ weights, const uint32_t *__restrict__ indices, uint32_t count,
                  uint32_t batch, uint64_t seed, float scale, float *__restrict__ grad,
                  float *__restrict__ z_scratch, double *__restrict__ loss_sum) {
-  __shared__ uint16_t dz_tile[128 * kTileStride];
-  __shared__ uint16_t h_tile[128 * kTileStride];
+  __shared__ __attribute__((aligned(16))) uint16_t dz_tile[128 * kTileStride];
+  __shared__ __attribute__((aligned(16))) uint16_t h_tile[128 * kTileStride];
   const int lane = __lane_id();
   const int h = lane >> 5, n = lane & 31;
   const uint32_t sample = blockIdx.x * 64u + uint32_t(lane);
@@ -196,7 +202,7 @@ __global__ void __launch_bounds__(64)
   float o[kOutputs];
 #pragma unroll
   for (int nb = 0; nb < 2; ++nb) {
-    bf16x4 x[16];
+    bf16x8 x[8];
     scatter_inputs(f, nb, x);
     floatx16 raw[2];
     forward_block(net, x, raw, z + nb * 16384);
@@ -212,34 +218,32 @@ __global__ void __launch_bounds__(64)
   for (int offset = 32; offset > 0; offset >>= 1) total += __shfl_xor(total, offset, 64);
   if (lane == 0) atomicAdd(loss_sum, total);
 
-  // dY as B operands (rows 8 s + 4 h + e of 64, of the sample column 32 nb + n).
-  bf16x4 d[2][16];
+  // dY as B operands (rows 16 s + slot_row(8 h + e) of 64, of the sample column 32 nb + n).
+  bf16x8 d[2][8];
 #pragma unroll
   for (int nb = 0; nb < 2; ++nb) {
     const int src = 32 * nb + n;
 #pragma unroll
-    for (int s = 0; s < 8; ++s) {
-      float e[4];
+    for (int s = 0; s < 4; ++s) {
+      float e[8];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int row = 8 * s + 4 * h + j;
-        e[j] = shuffle_row<kOutputs>(dy, src, row);
-      }
-      d[nb][s] = pack4(e[0], e[1], e[2], e[3]);
+      for (int j = 0; j < 8; ++j) e[j] = shuffle_row<kOutputs>(dy, src, 16 * s + slot_row(8 * h + j));
+      d[nb][s] = pack8(e);
     }
   }
 
 #pragma unroll
   for (int l = kLayers - 1; l >= 0; --l) {
-    const int r_steps = rows(l) / 8;
+    const int r_steps = rows(l) / 16;
     // dZ_{l+1} tile [row][sample] from the B operands.
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
 #pragma unroll
       for (int s = 0; s < r_steps; ++s) {
-        const bf16x4 v = d[nb][s];
+        const bf16x8 v = d[nb][s];
 #pragma unroll
-        for (int j = 0; j < 4; ++j) dz_tile[(8 * s + 4 * h + j) * kTileStride + 32 * nb + n] = uint16_t(v[j]);
+        for (int j = 0; j < 8; ++j)
+          dz_tile[(16 * s + slot_row(8 * h + j)) * kTileStride + 32 * nb + n] = uint16_t(v[j]);
       }
     }
     // H_l tile [feature][sample]: inputs for l = 0, else SiLU of the saved pre-activations.
@@ -272,12 +276,12 @@ __global__ void __launch_bounds__(64)
       for (int cb = 0; cb < c_blocks; ++cb) {
         floatx16 c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int s = 0; s < 8; ++s) {
-          const uint16_t *a = dz_tile + (32 * mb + n) * kTileStride + 8 * s + 4 * h;
-          const uint16_t *b = h_tile + (32 * cb + n) * kTileStride + 8 * s + 4 * h;
-          const bf16x4 av = {short(a[0]), short(a[1]), short(a[2]), short(a[3])};
-          const bf16x4 bv = {short(b[0]), short(b[1]), short(b[2]), short(b[3])};
-          c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(av, bv, c, 0, 0, 0);
+        for (int s = 0; s < 4; ++s) {
+          // Samples 16 s + 8 h .. + 7 of a dZ row and of an H row (any sample order serves, as
+          // long as both operands use the same one).
+          const bf16x8 av = *reinterpret_cast<const bf16x8 *>(dz_tile + (32 * mb + n) * kTileStride + 16 * s + 8 * h);
+          const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(h_tile + (32 * cb + n) * kTileStride + 16 * s + 8 * h);
+          c = mfma(av, bv, c);
         }
         const int col = 32 * cb + n;
 #pragma unroll
@@ -299,25 +303,24 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
       const float *zl = z + nb * 16384 + (l - 1) * 4096;
-      bf16x4 next[16];
+      bf16x8 next[8];
 #pragma unroll
       for (int mb = 0; mb < 4; ++mb) {
         floatx16 c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
         for (int s = 0; s < r_steps; ++s)
-          c = __builtin_amdgcn_mfma_f32_32x32x8bf16_1k(net.packed_t[packed_t_offset(l) + (mb * r_steps + s) * 64 + lane],
-                                                       d[nb][s], c, 0, 0, 0);
+          c = mfma(net.packed_t[packed_t_offset(l) + (mb * r_steps + s) * 64 + lane], d[nb][s], c);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-          float e[4];
+        for (int j = 0; j < 2; ++j) {
+          float e[8];
 #pragma unroll
-          for (int j = 0; j < 4; ++j) e[j] = c[4 * q + j] * silu_grad(zl[(mb * 16 + 4 * q + j) * 64 + lane]);
-          next[4 * mb + q] = pack4(e[0], e[1], e[2], e[3]);
+          for (int k = 0; k < 8; ++k) e[k] = c[8 * j + k] * silu_grad(zl[(mb * 16 + 8 * j + k) * 64 + lane]);
+          next[2 * mb + j] = pack8(e);
         }
         __builtin_amdgcn_sched_barrier(0);
       }
 #pragma unroll
-      for (int s = 0; s < 16; ++s) d[nb][s] = next[s];
+      for (int s = 0; s < 8; ++s) d[nb][s] = next[s];
     }
   }
 }
@@ -333,7 +336,7 @@ __global__ void __launch_bounds__(64) forward_kernel(Net net, const float *__res
   float o[kOutputs];
 #pragma unroll
   for (int nb = 0; nb < 2; ++nb) {
-    bf16x4 x[16];
+    bf16x8 x[8];
     scatter_inputs(f, nb, x);
     floatx16 raw[2];
     forward_block(net, x, raw, nullptr);
@@ -495,8 +498,8 @@ void *monki_hip_proposal_create(uint32_t device, uint32_t seed, char *error, siz
       !ok(hipMalloc(&p->grad, kParams * sizeof(float)), error, error_size, "allocate gradient") ||
       !ok(hipMalloc(&p->adam_m, kParams * sizeof(float)), error, error_size, "allocate moments") ||
       !ok(hipMalloc(&p->adam_v, kParams * sizeof(float)), error, error_size, "allocate moments") ||
-      !ok(hipMalloc(&p->packed, kPacked * sizeof(bf16x4)), error, error_size, "allocate operands") ||
-      !ok(hipMalloc(&p->packed_t, kPackedT * sizeof(bf16x4)), error, error_size, "allocate operands") ||
+      !ok(hipMalloc(&p->packed, kPacked * sizeof(bf16x8)), error, error_size, "allocate operands") ||
+      !ok(hipMalloc(&p->packed_t, kPackedT * sizeof(bf16x8)), error, error_size, "allocate operands") ||
       !ok(hipMalloc(&p->bias, kBiases * sizeof(float)), error, error_size, "allocate biases") ||
       !ok(hipMalloc(&p->log_count, sizeof(unsigned)), error, error_size, "allocate log counter") ||
       !ok(hipMalloc(&p->loss, sizeof(double)), error, error_size, "allocate loss") ||

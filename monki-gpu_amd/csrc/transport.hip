@@ -1570,7 +1570,7 @@ __device__ __attribute__((noinline)) void guided_mixture(const monki_net::Net &n
   for (int nb = 0; nb < 2; ++nb) {
     // A 32-lane half without guided events skips its network pass (uniform branch).
     if (((guided_lanes >> (32 * nb)) & 0xffffffffull) == 0) continue;
-    monki_net::bf16x4 x[16];
+    monki_net::bf16x8 x[8];
     monki_net::scatter_inputs(features, nb, x);
     monki_net::floatx16 raw[2];
     monki_net::forward_block(net, x, raw, nullptr);

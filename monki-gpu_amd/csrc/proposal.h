@@ -69,6 +69,13 @@ struct Net {
   const float *bias;       // kBiases
 };
 
+// Loads through a global (address space 1) pointer: the network's weight and bias pointers are
+// global memory, and a global load waits on the vector-memory counter only (a flat load also
+// holds the LDS counter).
+template <typename T> __device__ inline T load_global(const T *p) {
+  return *(const __attribute__((address_space(1))) T *)p;
+}
+
 __device__ inline floatx16 mfma(bf16x8 a, bf16x8 b, floatx16 c) {
   return __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(mfma_bf16x8, a),
                                                  __builtin_bit_cast(mfma_bf16x8, b), c, 0, 0, 0);
@@ -89,18 +96,21 @@ __host__ __device__ inline float bf16_value(uint16_t bits) {
   return x;
 }
 
-// Eight accumulator values (registers 8 j .. 8 j + 7) as one operand.
+// Eight accumulator values (registers 8 j .. 8 j + 7) as one operand: gfx950's
+// v_cvt_pk_bf16_f32 rounds two values to nearest even per instruction.
 __device__ inline bf16x8 pack8(const float (&v)[8]) {
-  bf16x8 out;
-#pragma unroll
-  for (int e = 0; e < 8; ++e) out[e] = short(bf16_bits(v[e]));
-  return out;
+  const mfma_bf16x8 out = {(__bf16)v[0], (__bf16)v[1], (__bf16)v[2], (__bf16)v[3],
+                           (__bf16)v[4], (__bf16)v[5], (__bf16)v[6], (__bf16)v[7]};
+  return __builtin_bit_cast(bf16x8, out);
 }
 
-__device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative with the hardware reciprocal (one instruction, ~1 ulp) instead of an
+// IEEE division: the results are rounded to bf16 next.
+__device__ inline float sigmoid(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ inline float silu(float z) { return z * sigmoid(z); }
 
 __device__ inline float silu_grad(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = sigmoid(z);
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -115,11 +125,12 @@ __device__ inline void dense(const bf16x8 *__restrict__ w, const float *__restri
   const int h = lane >> 5;
 #pragma unroll
   for (int mb = 0; mb < Blocks; ++mb) {
+    // The accumulator starts at the biases of its rows.
     floatx16 c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-    for (int s = 0; s < Steps; ++s) c = mfma(w[(mb * Steps + s) * 64 + lane], x[s], c);
+    for (int s = 0; s < Steps; ++s) c = mfma(load_global(w + (mb * Steps + s) * 64 + lane), x[s], c);
 #pragma unroll
-    for (int v = 0; v < 16; ++v) c[v] += bias[32 * mb + 8 * (v >> 2) + 4 * h + (v & 3)];
+    for (int v = 0; v < 16; ++v) c[v] += load_global(bias + 32 * mb + 8 * (v >> 2) + 4 * h + (v & 3));
     if (z_out) {
 #pragma unroll
       for (int v = 0; v < 16; ++v) z_out[(mb * 16 + v) * 64 + lane] = c[v];
@@ -158,34 +169,50 @@ __device__ inline void forward_block(const Net &net, const bf16x8 (&x)[8], float
   dense<8, 2, false>(net.packed + packed_offset(4), net.bias + bias_offset(4), b, a, raw, nullptr);
 }
 
-// Output m of the block's sample column n = lane % 32, from the D registers of the wave.
-// Every lane calls it with the same m; lanes read the value of sample (lane % 32).
-template <int M> __device__ inline float output_row(const floatx16 (&raw)[2]) {
+// Output m of sample column `column` (0..31) of the block, from the D registers of the wave.
+// Every lane calls it with the same m.
+template <int M> __device__ inline float output_row(const floatx16 (&raw)[2], int column) {
   constexpr int mb = M / 32, q = (M % 32) / 8, h = (M % 8) / 4, i = M % 4;
-  const int source = (__lane_id() & 31) + 32 * h;
-  return __shfl(raw[mb][4 * q + i], source, 64);
+  return __shfl(raw[mb][4 * q + i], column + 32 * h, 64);
 }
 
-// Gather the 40 outputs of sample (32 nb + lane % 32) into every lane of the wave; lanes keep
-// them when lane / 32 == nb.
+// Gather the 40 outputs of sample column `column` of the block into every lane of the wave;
+// lanes keep them when `keep`.
 template <int M = 0>
-__device__ inline void gather_outputs(const floatx16 (&raw)[2], bool keep, float (&out)[kOutputs]) {
+__device__ inline void gather_outputs(const floatx16 (&raw)[2], int column, bool keep, float (&out)[kOutputs]) {
   if constexpr (M < kOutputs) {
-    const float value = output_row<M>(raw);
+    const float value = output_row<M>(raw, column);
     if (keep) out[M] = value;
-    gather_outputs<M + 1>(raw, keep, out);
+    gather_outputs<M + 1>(raw, column, keep, out);
   }
 }
 
-// Input B operand of block nb from per-lane features f[12] (sample of lane 32 nb + n): lane
+// Lane of the k-th set bit (k counted from 0) of a wave mask, k < popcount(mask).
+__device__ inline int nth_set_lane(unsigned long long mask, int k) {
+  int lane = 0;
+#pragma unroll
+  for (int step = 32; step > 0; step >>= 1) {
+    const int count = __popcll((mask >> lane) & ((1ull << step) - 1ull));
+    if (k >= count) {
+      k -= count;
+      lane += step;
+    }
+  }
+  return lane;
+}
+
+// Input B operand of one block from per-lane features f[12]: sample column n = lane % 32 takes
+// the features of lane source(n) (padding columns pass a negative source and get zeros); lane
 // half h holds feature rows slot_row(8 h + e), e = 0..7.
-__device__ inline void scatter_inputs(const float (&f)[kInputs], int nb, bf16x8 (&x)[8]) {
-  const int lane = __lane_id();
-  const int source = 32 * nb + (lane & 31);
-  const bool upper = lane >= 32;
+__device__ inline void scatter_inputs(const float (&f)[kInputs], int source, bf16x8 (&x)[8]) {
+  const bool upper = __lane_id() >= 32;
+  const bool valid = source >= 0;
   float v[16];
 #pragma unroll
-  for (int k = 0; k < kInputs; ++k) v[k] = __shfl(f[k], source, 64);
+  for (int k = 0; k < kInputs; ++k) {
+    const float value = __shfl(f[k], valid ? source : 0, 64);
+    v[k] = valid ? value : 0.0f;
+  }
 #pragma unroll
   for (int k = kInputs; k < 16; ++k) v[k] = 0.0f;
   float e[8];
@@ -200,10 +227,15 @@ __device__ inline void scatter_inputs(const float (&f)[kInputs], int nb, bf16x8 
 struct Mixture {
   float mx[kComponents], my[kComponents], mz[kComponents];
   float kappa[kComponents], weight[kComponents];
+  // weight x vMF normalization kappa / (2 pi (1 - exp(-2 kappa))), so that the density is
+  // sum_k scale_k exp(kappa_k (mu_k . w - 1)).
+  float scale[kComponents];
 };
 
 __host__ __device__ inline float softplus(float x) { return x > 20.0f ? x : log1pf(expf(x)); }
 
+// The mixture of one event's 40 outputs, with the hardware exp/log/reciprocal (the proposal
+// only has to be one density, sampled and evaluated consistently; its form is free).
 __device__ inline Mixture mixture_from(const float *o) {
   Mixture m;
   float top = -INFINITY;
@@ -213,17 +245,25 @@ __device__ inline Mixture mixture_from(const float *o) {
 #pragma unroll
   for (int k = 0; k < kComponents; ++k) {
     const float x = o[5 * k], y = o[5 * k + 1], z = o[5 * k + 2];
-    const float norm = sqrtf(x * x + y * y + z * z);
-    const bool ok = norm > 1e-6f;
-    m.mx[k] = ok ? x / norm : 0.0f;
-    m.my[k] = ok ? y / norm : 0.0f;
-    m.mz[k] = ok ? z / norm : 1.0f;
-    m.kappa[k] = fminf(fmaxf(softplus(o[5 * k + 3]), kKappaMin), kKappaMax);
-    m.weight[k] = expf(o[5 * k + 4] - top);
+    const float square = x * x + y * y + z * z;
+    const bool ok = square > 1e-12f;
+    const float inverse = ok ? rsqrtf(square) : 0.0f;
+    m.mx[k] = ok ? x * inverse : 0.0f;
+    m.my[k] = ok ? y * inverse : 0.0f;
+    m.mz[k] = ok ? z * inverse : 1.0f;
+    const float raw = o[5 * k + 3];
+    const float soft = raw > 20.0f ? raw : __logf(1.0f + __expf(raw));
+    m.kappa[k] = fminf(fmaxf(soft, kKappaMin), kKappaMax);
+    m.weight[k] = __expf(o[5 * k + 4] - top);
     total += m.weight[k];
   }
+  const float inverse = __builtin_amdgcn_rcpf(total);
 #pragma unroll
-  for (int k = 0; k < kComponents; ++k) m.weight[k] /= total;
+  for (int k = 0; k < kComponents; ++k) {
+    m.weight[k] *= inverse;
+    const float kappa = m.kappa[k];
+    m.scale[k] = m.weight[k] * kappa * 0.15915494309189533577f * __builtin_amdgcn_rcpf(1.0f - __expf(-2.0f * kappa));
+  }
   return m;
 }
 
@@ -237,7 +277,7 @@ __device__ inline float mixture_density(const Mixture &m, float wx, float wy, fl
   float q = 0.0f;
 #pragma unroll
   for (int k = 0; k < kComponents; ++k)
-    q += m.weight[k] * expf(log_vmf(m.kappa[k], m.mx[k] * wx + m.my[k] * wy + m.mz[k] * wz));
+    q += m.scale[k] * __expf(m.kappa[k] * (m.mx[k] * wx + m.my[k] * wy + m.mz[k] * wz - 1.0f));
   return q;
 }
 
@@ -292,14 +332,20 @@ __device__ inline void mixture_sample(const Mixture &m, float u0, float u1, floa
 }
 
 // Scattering events logged by the plain tracer for training (device pointers).
+// A dispatch logs in two passes over the same histories: the counting pass writes each
+// photon's number of logged events into starts[photon]; the host turns them into first slots
+// (prefix sum from the samples already held, clamped to the capacity); the writing pass stores
+// event k of a photon at starts[photon] + k. Samples end up in photon, then event order.
 struct EventLog {
   float *features;    // [capacity][kInputs]
   float *directions;  // [capacity][3] direction after the event
   float *cumulative;  // [capacity] detector weight of the history up to the event
   uint32_t *photon;   // [capacity] photon of the dispatch
   float *totals;      // [photons] detector weight of each whole history
-  unsigned *count;    // events logged (may exceed capacity; extra events are dropped)
+  uint32_t *starts;   // [photons] events to log (counting pass), then first slots
+  unsigned *count;    // samples held after the dispatch (set by the host)
   uint32_t capacity, photons;
+  uint32_t counting;  // 1 in the counting pass
 };
 
 // Host interface of a proposal handle, for the transport translation unit.

@@ -10,8 +10,9 @@
 // One wave trains on 64 samples per launch block: the forward pass runs on the matrix cores
 // (proposal.h), keeping the hidden pre-activations in a scratch buffer; the backward pass
 // chains W^T products the same way, and the weight gradients dW = dZ H^T come from two
-// 128 x 64 bf16 tiles in LDS (samples on the reduction side, 16 per instruction) and one
-// coalesced float atomic per gradient element and wave.
+// 128 x 64 bf16 tiles in LDS (samples on the reduction side, 16 per instruction). Each wave
+// writes its whole gradient and loss to a partial row of its own; a second kernel sums the rows
+// in wave order, so a training run is reproducible to the bit (no float atomics).
 
 #include <hip/hip_runtime.h>
 
@@ -34,6 +35,8 @@ namespace {
 // 16-byte LDS read phase hit distinct banks.
 constexpr int kTileStride = 72;
 constexpr int kZPerWave = 2 * 4 * 4096;
+// Waves of one gradient launch (65,536 samples); larger batches run in chunks of this many.
+constexpr uint32_t kChunkWaves = 1024;
 
 struct Proposal {
   int device = -1;
@@ -42,12 +45,16 @@ struct Proposal {
   float *bias = nullptr;
   float *z = nullptr;
   size_t z_waves = 0;
+  float *partial = nullptr;     // [waves][kParams] gradient of each wave of a launch
+  double *loss_part = nullptr;  // [waves] loss of each wave
   // Samples: features [n][12], directions [n][3], weights [n].
   float *features = nullptr, *directions = nullptr, *weights = nullptr;
   size_t capacity = 0, count = 0;
   // Per-event contribution so far and per-photon totals of a logging dispatch.
   float *cumulative = nullptr, *totals = nullptr;
-  uint32_t *photon = nullptr;
+  uint32_t *photon = nullptr, *starts = nullptr;
+  double *target_parts = nullptr;  // per-block sums of the training weights of a dispatch
+  size_t target_blocks = 0;
   unsigned *log_count = nullptr;
   size_t photon_capacity = 0;
   double *loss = nullptr;
@@ -175,19 +182,18 @@ __device__ inline float shuffle_row(const float (&v)[Count], int src, int row) {
   return out;
 }
 
-__device__ inline float bf16_round(float x) { return bf16_value(bf16_bits(x)); }
-
 // One training block: 64 samples (sample indices from `indices`, or hashed from (seed, step)).
 __global__ void __launch_bounds__(64)
     train_kernel(Net net, const float *__restrict__ features, const float *__restrict__ directions,
                  const float *__restrict__ weights, const uint32_t *__restrict__ indices, uint32_t count,
-                 uint32_t batch, uint64_t seed, float scale, float *__restrict__ grad,
-                 float *__restrict__ z_scratch, double *__restrict__ loss_sum) {
+                 uint32_t base, uint32_t batch, uint64_t seed, float scale, float *__restrict__ partial,
+                 float *__restrict__ z_scratch, double *__restrict__ loss_part) {
   __shared__ __attribute__((aligned(16))) uint16_t dz_tile[128 * kTileStride];
   __shared__ __attribute__((aligned(16))) uint16_t h_tile[128 * kTileStride];
   const int lane = __lane_id();
   const int h = lane >> 5, n = lane & 31;
-  const uint32_t sample = blockIdx.x * 64u + uint32_t(lane);
+  const uint32_t sample = base + blockIdx.x * 64u + uint32_t(lane);
+  float *grad = partial + size_t(blockIdx.x) * kParams;  // this wave's row: every element written once
   const bool valid = sample < batch;
   const uint32_t index = !valid ? 0u : indices ? indices[sample] : mix32(seed * 0x9E3779B97F4A7C15ull + sample) % count;
   float f[kInputs];
@@ -203,10 +209,10 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
   for (int nb = 0; nb < 2; ++nb) {
     bf16x8 x[8];
-    scatter_inputs(f, nb, x);
+    scatter_inputs(f, 32 * nb + n, x);
     floatx16 raw[2];
     forward_block(net, x, raw, z + nb * 16384);
-    gather_outputs(raw, h == nb, o);
+    gather_outputs(raw, n, h == nb, o);
   }
   float dy[kOutputs];
   const float loss = valid ? head_loss(o, wx, wy, wz, g, dy) : 0.0f;
@@ -216,7 +222,7 @@ __global__ void __launch_bounds__(64)
   }
   double total = loss;
   for (int offset = 32; offset > 0; offset >>= 1) total += __shfl_xor(total, offset, 64);
-  if (lane == 0) atomicAdd(loss_sum, total);
+  if (lane == 0) loss_part[blockIdx.x] = total;
 
   // dY as B operands (rows 16 s + slot_row(8 h + e) of 64, of the sample column 32 nb + n).
   bf16x8 d[2][8];
@@ -287,15 +293,14 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
         for (int v = 0; v < 16; ++v) {
           const int row = 32 * mb + 8 * (v >> 2) + 4 * h + (v & 3);
-          if (row < true_rows(l) && col < true_cols(l) && c[v] != 0.0f)
-            atomicAdd(grad + param_offset(l) + row * true_cols(l) + col, c[v]);
+          if (row < true_rows(l) && col < true_cols(l)) grad[param_offset(l) + row * true_cols(l) + col] = c[v];
         }
       }
     }
     for (int row = lane; row < true_rows(l); row += 64) {
       float sum = 0.0f;
       for (int t = 0; t < 64; ++t) sum += bf16_value(dz_tile[row * kTileStride + t]);
-      if (sum != 0.0f) atomicAdd(grad + param_offset(l) + true_rows(l) * true_cols(l) + row, sum);
+      grad[param_offset(l) + true_rows(l) * true_cols(l) + row] = sum;
     }
     __syncthreads();
     if (l == 0) break;
@@ -337,10 +342,10 @@ __global__ void __launch_bounds__(64) forward_kernel(Net net, const float *__res
 #pragma unroll
   for (int nb = 0; nb < 2; ++nb) {
     bf16x8 x[8];
-    scatter_inputs(f, nb, x);
+    scatter_inputs(f, 32 * nb + (lane & 31), x);
     floatx16 raw[2];
     forward_block(net, x, raw, nullptr);
-    gather_outputs(raw, (lane >> 5) == nb, o);
+    gather_outputs(raw, lane & 31, (lane >> 5) == nb, o);
   }
   if (sample < count) {
     for (int k = 0; k < kOutputs; ++k) out[size_t(sample) * kOutputs + k] = o[k];
@@ -360,10 +365,30 @@ __global__ void adam_kernel(float *__restrict__ params, const float *__restrict_
   params[i] -= lr * (mi / correction1) / (sqrtf(vi / correction2) + 1e-8f);
 }
 
-// Training weights of the events logged in one dispatch: G = photon total - contribution so far.
-__global__ void targets_kernel(const float *__restrict__ totals, const uint32_t *__restrict__ photon,
-                               const float *__restrict__ cumulative, float *__restrict__ weights,
-                               uint32_t first, uint32_t last, double *__restrict__ sum) {
+// Gradient of a launch: the waves' partial rows summed in wave order (into grad, or added to it
+// for the later chunks of a large batch), and their losses likewise.
+__global__ void sum_partials(const float *__restrict__ partial, const double *__restrict__ loss_part,
+                             uint32_t waves, bool add, float *__restrict__ grad, double *__restrict__ loss) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < kParams) {
+    float sum = 0.0f;
+    for (uint32_t w = 0; w < waves; ++w) sum += partial[size_t(w) * kParams + i];
+    grad[i] = add ? grad[i] + sum : sum;
+  }
+  if (i == 0) {
+    double sum = 0.0;
+    for (uint32_t w = 0; w < waves; ++w) sum += loss_part[w];
+    *loss = add ? *loss + sum : sum;
+  }
+}
+
+// Training weights of the events logged in one dispatch: G = photon total - contribution so far;
+// one sum per 256-event block (lanes, then the block's four waves, in a fixed order).
+__global__ void __launch_bounds__(256)
+    targets_kernel(const float *__restrict__ totals, const uint32_t *__restrict__ photon,
+                   const float *__restrict__ cumulative, float *__restrict__ weights, uint32_t first,
+                   uint32_t last, double *__restrict__ block_sums) {
+  __shared__ double waves[4];
   const uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
   double value = 0.0;
   if (i < last) {
@@ -372,7 +397,9 @@ __global__ void targets_kernel(const float *__restrict__ totals, const uint32_t 
     value = g;
   }
   for (int offset = 32; offset > 0; offset >>= 1) value += __shfl_xor(value, offset, 64);
-  if (__lane_id() == 0 && value != 0.0) atomicAdd(sum, value);
+  if (__lane_id() == 0) waves[threadIdx.x >> 6] = value;
+  __syncthreads();
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = ((waves[0] + waves[1]) + waves[2]) + waves[3];
 }
 
 Net view(const Proposal *p) { return Net{p->packed, p->packed_t, p->bias}; }
@@ -390,37 +417,48 @@ void release(Proposal *p) {
   if (p->device >= 0) (void)hipSetDevice(p->device);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   void *buffers[] = {p->params, p->grad, p->adam_m, p->adam_v, p->packed, p->packed_t, p->bias, p->z,
-                     p->features, p->directions, p->weights, p->cumulative, p->totals, p->photon,
-                     p->log_count, p->loss};
+                     p->partial, p->loss_part, p->features, p->directions, p->weights, p->cumulative,
+                     p->totals, p->photon, p->starts, p->target_parts, p->log_count, p->loss};
   for (void *b : buffers) (void)hipFree(b);
   delete p;
 }
 
+// Per-wave buffers of a gradient launch (pre-activations and partial gradients), kept for later
+// launches.
 bool ensure_scratch(Proposal *p, size_t waves, char *error, size_t size) {
   if (waves <= p->z_waves) return true;
-  (void)hipFree(p->z);
+  void *buffers[] = {p->z, p->partial, p->loss_part};
+  for (void *b : buffers) (void)hipFree(b);
   p->z = nullptr;
+  p->partial = nullptr;
+  p->loss_part = nullptr;
   p->z_waves = 0;
-  if (!ok(hipMalloc(&p->z, waves * kZPerWave * sizeof(float)), error, size, "allocate proposal scratch"))
+  if (!ok(hipMalloc(&p->z, waves * kZPerWave * sizeof(float)), error, size, "allocate proposal scratch") ||
+      !ok(hipMalloc(&p->partial, waves * kParams * sizeof(float)), error, size, "allocate partial gradients") ||
+      !ok(hipMalloc(&p->loss_part, waves * sizeof(double)), error, size, "allocate partial losses"))
     return false;
   p->z_waves = waves;
   return true;
 }
 
-// Gradient of the mean loss over `batch` samples into p->grad (zeroed first); returns the loss.
+// Gradient of the mean loss over `batch` samples into p->grad; returns the loss. Batches past
+// kChunkWaves waves run in chunks whose sums add up in chunk order.
 bool gradients(Proposal *p, const uint32_t *indices, uint32_t batch, uint64_t seed, double &loss,
                char *error, size_t size) {
-  const size_t waves = (batch + 63) / 64;
-  if (!ensure_scratch(p, waves, error, size)) return false;
+  const uint32_t total_waves = (batch + 63) / 64;
+  const uint32_t chunk = total_waves < kChunkWaves ? total_waves : kChunkWaves;
+  if (!ensure_scratch(p, chunk, error, size)) return false;
   const float scale = p->weight_sum > 0.0 ? float(double(p->count) / p->weight_sum / batch) : 0.0f;
-  if (!ok(hipMemsetAsync(p->grad, 0, kParams * sizeof(float), p->stream), error, size, "clear gradient") ||
-      !ok(hipMemsetAsync(p->loss, 0, sizeof(double), p->stream), error, size, "clear loss"))
-    return false;
-  hipLaunchKernelGGL(train_kernel, dim3(waves), dim3(64), 0, p->stream, view(p), p->features,
-                     p->directions, p->weights, indices, uint32_t(p->count), batch, seed, scale, p->grad,
-                     p->z, p->loss);
-  if (!ok(hipGetLastError(), error, size, "train launch") ||
-      !ok(hipMemcpyAsync(&loss, p->loss, sizeof(double), hipMemcpyDeviceToHost, p->stream), error, size,
+  for (uint32_t first = 0; first < total_waves; first += chunk) {
+    const uint32_t waves = total_waves - first < chunk ? total_waves - first : chunk;
+    hipLaunchKernelGGL(train_kernel, dim3(waves), dim3(64), 0, p->stream, view(p), p->features,
+                       p->directions, p->weights, indices, uint32_t(p->count), first * 64u, batch, seed, scale,
+                       p->partial, p->z, p->loss_part);
+    hipLaunchKernelGGL(sum_partials, dim3((kParams + 255) / 256), dim3(256), 0, p->stream, p->partial,
+                       p->loss_part, waves, first > 0, p->grad, p->loss);
+    if (!ok(hipGetLastError(), error, size, "train launch")) return false;
+  }
+  if (!ok(hipMemcpyAsync(&loss, p->loss, sizeof(double), hipMemcpyDeviceToHost, p->stream), error, size,
           "copy loss") ||
       !ok(hipStreamSynchronize(p->stream), error, size, "train"))
     return false;
@@ -436,8 +474,8 @@ Net proposal_net(void *handle) { return view(static_cast<Proposal *>(handle)); }
 
 EventLog proposal_log(void *handle) {
   Proposal *p = static_cast<Proposal *>(handle);
-  return EventLog{p->features, p->directions, p->cumulative, p->photon, p->totals, p->log_count,
-                  uint32_t(p->capacity), uint32_t(p->photon_capacity)};
+  return EventLog{p->features, p->directions, p->cumulative, p->photon, p->totals, p->starts, p->log_count,
+                  uint32_t(p->capacity), uint32_t(p->photon_capacity), 0u};
 }
 
 int proposal_device(void *handle) { return static_cast<Proposal *>(handle)->device; }
@@ -453,16 +491,24 @@ bool proposal_finish_log(void *handle, uint32_t first, hipStream_t stream, char 
     return false;
   const uint32_t last = logged < p->capacity ? logged : uint32_t(p->capacity);
   if (last > first) {
-    if (!ok(hipMemsetAsync(p->loss, 0, sizeof(double), stream), error, size, "clear sum")) return false;
-    hipLaunchKernelGGL(targets_kernel, dim3((last - first + 255) / 256), dim3(256), 0, stream, p->totals,
-                       p->photon, p->cumulative, p->weights, first, last, p->loss);
-    double sum = 0.0;
+    const size_t blocks = (last - first + 255) / 256;
+    if (blocks > p->target_blocks) {
+      (void)hipFree(p->target_parts);
+      p->target_parts = nullptr;
+      p->target_blocks = 0;
+      if (!ok(hipMalloc(&p->target_parts, blocks * sizeof(double)), error, size, "allocate target sums"))
+        return false;
+      p->target_blocks = blocks;
+    }
+    hipLaunchKernelGGL(targets_kernel, dim3(unsigned(blocks)), dim3(256), 0, stream, p->totals, p->photon,
+                       p->cumulative, p->weights, first, last, p->target_parts);
+    std::vector<double> sums(blocks);
     if (!ok(hipGetLastError(), error, size, "targets launch") ||
-        !ok(hipMemcpyAsync(&sum, p->loss, sizeof(double), hipMemcpyDeviceToHost, stream), error, size,
-            "copy target sum") ||
+        !ok(hipMemcpyAsync(sums.data(), p->target_parts, blocks * sizeof(double), hipMemcpyDeviceToHost, stream),
+            error, size, "copy target sums") ||
         !ok(hipStreamSynchronize(stream), error, size, "targets"))
       return false;
-    p->weight_sum += sum;
+    for (double value : sums) p->weight_sum += value;  // block order: reproducible
   }
   p->count = last;
   return true;
@@ -545,7 +591,7 @@ int monki_hip_proposal_reserve(void *handle, size_t events, size_t photons, char
   }
   if (!ok(hipSetDevice(p->device), error, error_size, "hipSetDevice")) return 1;
   void **buffers[] = {(void **)&p->features, (void **)&p->directions, (void **)&p->weights,
-                      (void **)&p->cumulative, (void **)&p->photon, (void **)&p->totals};
+                      (void **)&p->cumulative, (void **)&p->photon, (void **)&p->totals, (void **)&p->starts};
   for (void **b : buffers) {
     (void)hipFree(*b);
     *b = nullptr;
@@ -558,6 +604,7 @@ int monki_hip_proposal_reserve(void *handle, size_t events, size_t photons, char
       !ok(hipMalloc(&p->cumulative, events * sizeof(float)), error, error_size, "allocate samples") ||
       !ok(hipMalloc(&p->photon, events * sizeof(uint32_t)), error, error_size, "allocate samples") ||
       (photons && !ok(hipMalloc(&p->totals, photons * sizeof(float)), error, error_size, "allocate totals")) ||
+      (photons && !ok(hipMalloc(&p->starts, photons * sizeof(uint32_t)), error, error_size, "allocate log starts")) ||
       !ok(hipMemset(p->log_count, 0, sizeof(unsigned)), error, error_size, "clear log counter"))
     return 1;
   p->capacity = events;

@@ -1557,26 +1557,26 @@ __device__ inline void flush_counts(Counts &n, unsigned long long *counters) {
   n = Counts{0, 0, 0, 0, 0, 0, 0, 0};
 }
 
-// The proposal's mixture for the wave's guided events: the network on each 32-lane half with
-// guided events (matrix cores, all lanes together). Not inlined: the network needs most of the
-// register file, and a call saves the photon state around it once instead of the compiler
-// spilling it piecemeal through the network.
+// The proposal's mixture for one 32-event block of the wave's guided events: the network on
+// the matrix cores, all lanes together. Guided events are numbered by lane (rank r = guided
+// lanes below the lane); block nb holds ranks 32 nb .. 32 nb + 31, sample column r % 32 taking
+// the features of its lane, so a wave with at most 32 guided events runs one pass. Not
+// inlined: the network needs most of the register file, and a call saves the photon state
+// around it once instead of the compiler spilling it piecemeal through the network.
 __device__ __attribute__((noinline)) void guided_mixture(const monki_net::Net &net,
                                                           const float (&features)[monki_net::kInputs],
-                                                          unsigned long long guided_lanes, bool guided_event,
+                                                          unsigned long long guided_lanes, int nb, bool mine,
                                                           monki_net::Mixture &mixture) {
+  const int lane = __lane_id();
+  const int sample = 32 * nb + (lane & 31);
+  const int source = sample < __popcll(guided_lanes) ? monki_net::nth_set_lane(guided_lanes, sample) : -1;
+  monki_net::bf16x8 x[8];
+  monki_net::scatter_inputs(features, source, x);
+  monki_net::floatx16 raw[2];
+  monki_net::forward_block(net, x, raw, nullptr);
   float o[monki_net::kOutputs];
-#pragma unroll
-  for (int nb = 0; nb < 2; ++nb) {
-    // A 32-lane half without guided events skips its network pass (uniform branch).
-    if (((guided_lanes >> (32 * nb)) & 0xffffffffull) == 0) continue;
-    monki_net::bf16x8 x[8];
-    monki_net::scatter_inputs(features, nb, x);
-    monki_net::floatx16 raw[2];
-    monki_net::forward_block(net, x, raw, nullptr);
-    monki_net::gather_outputs(raw, (__lane_id() >> 5) == nb, o);
-  }
-  if (guided_event) mixture = monki_net::mixture_from(o);
+  monki_net::gather_outputs(raw, __popcll(guided_lanes & __lanemask_lt()) & 31, mine, o);
+  if (mine) mixture = monki_net::mixture_from(o);
 }
 
 // Weight-window copies (steered modes). Each lane keeps a stack of kSplitDepth entries in global
@@ -1657,10 +1657,11 @@ __device__ inline uint32_t pop_copy(uint32_t *entry, History<R> &h, R *history, 
 // capped to match (float32), which measured 1.35x faster than the compiler's own choice of
 // two waves per SIMD on the layered-cloud round. The guided mode evaluates the proposal network
 // for the wave's scattering events between the tally and the scatter phases (matrix cores,
-// all lanes together), and keeps two waves per SIMD for the network's registers.
+// all lanes together, 32 events per pass); the network's call fits the same budget without
+// spilling (box-cloud round 223 -> 193 ms against two waves per SIMD).
 template <typename R, bool TRACE, int MODE>
 __global__ void __launch_bounds__(kBlock32)
-    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? (MODE == kGuided ? 2 : 3) : 1)))
+    __attribute__((amdgpu_waves_per_eu(sizeof(R) == 4 ? 3 : 1)))
     transport_kernel(const Scene *__restrict__ scenes, const Controls *__restrict__ controls,
                      uint32_t scene_count, int row, R *banks, unsigned *failure, unsigned *work,
                      unsigned long long *counters, float *records, uint32_t record_count,
@@ -1677,6 +1678,7 @@ __global__ void __launch_bounds__(kBlock32)
                                     : nullptr;
   int top = 0;
   uint32_t tree = 0;
+  uint32_t logged = 0;  // events of the photon logged so far (logging mode)
   const bool window = STEER && stack && guidance.split_above > 0.0f;
   extern __shared__ unsigned char lds_raw[];
   // Layer rows of odd length (the largest nz of the scenes, | 1): lanes reading one row, or
@@ -1750,6 +1752,7 @@ __global__ void __launch_bounds__(kBlock32)
           ++n.photons;
           active = true;
           tree = 0;
+          logged = 0;
         }
       }
     }
@@ -1875,7 +1878,6 @@ __global__ void __launch_bounds__(kBlock32)
     const bool scatters = active && status == kRunning && !h.surface;
     float features[monki_net::kInputs];
     monki_net::Mixture mixture;
-    uint32_t slot = 0xffffffffu;
     if (MODE == kLog || MODE == kGuided) {
 #pragma unroll
       for (int k = 0; k < monki_net::kInputs; ++k) features[k] = 0.0f;
@@ -1886,16 +1888,10 @@ __global__ void __launch_bounds__(kBlock32)
                               ((guidance.species_mask >> (h.species + 1)) & 1u) && h.order <= guidance.max_order;
     const unsigned long long guided_lanes =
         MODE == kGuided && guidance.lobe_fraction < 1.0f ? __ballot(guided_event) : 0ull;
-    if (MODE == kGuided && guided_lanes) guided_mixture(net, features, guided_lanes, guided_event, mixture);
-    if (MODE == kLog) {
-      const unsigned long long logging = __ballot(scatters);
-      if (logging) {
-        const int leader = __ffsll(static_cast<long long>(logging)) - 1;
-        uint32_t base = 0;
-        if (__lane_id() == leader) base = atomicAdd(log.count, uint32_t(__popcll(logging)));
-        base = __shfl(base, leader, 64);
-        if (scatters) slot = base + uint32_t(__popcll(logging & __lanemask_lt()));
-      }
+    if (MODE == kGuided && guided_lanes) {
+      const int rank = __popcll(guided_lanes & __lanemask_lt());
+      for (int nb = 0; 32 * nb < __popcll(guided_lanes); ++nb)
+        guided_mixture(net, features, guided_lanes, nb, guided_event && (rank >> 5) == nb, mixture);
     }
     prefetch(h.rng);
     const ViewLobes lobes{&s, guidance.lobe_species, guidance.kappa};
@@ -1918,14 +1914,21 @@ __global__ void __launch_bounds__(kBlock32)
       push_copies<R>(stack + size_t(top) * split_stride, h, history, s.nz, copies - 1u);
       ++top;
     }
-    if (MODE == kLog && slot < log.capacity && status == kRunning) {
+    if (MODE == kLog && scatters && status == kRunning) {
+      // The counting pass only counts the photon's events; the writing pass puts its k-th event
+      // at the photon's start (prefix sum of the counts) + k: samples in photon, then event
+      // order, whatever the timing of the waves.
+      const uint32_t slot = log.counting ? 0xffffffffu : log.starts[id] + logged;
+      ++logged;
+      if (slot < log.capacity) {
 #pragma unroll
-      for (int k = 0; k < monki_net::kInputs; ++k) log.features[size_t(slot) * monki_net::kInputs + k] = features[k];
-      log.directions[size_t(slot) * 3] = float(h.p.a);
-      log.directions[size_t(slot) * 3 + 1] = float(h.p.b);
-      log.directions[size_t(slot) * 3 + 2] = float(h.p.c);
-      log.cumulative[slot] = h.contribution;
-      log.photon[slot] = id;
+        for (int k = 0; k < monki_net::kInputs; ++k) log.features[size_t(slot) * monki_net::kInputs + k] = features[k];
+        log.directions[size_t(slot) * 3] = float(h.p.a);
+        log.directions[size_t(slot) * 3 + 1] = float(h.p.b);
+        log.directions[size_t(slot) * 3 + 2] = float(h.p.c);
+        log.cumulative[slot] = h.contribution;
+        log.photon[slot] = id;
+      }
     }
     if (MERGE) {
       // Histories (and copies of split ones) that ended write out their open records before a
@@ -1952,7 +1955,10 @@ __global__ void __launch_bounds__(kBlock32)
       continue;
     }
     active = false;
-    if (MODE == kLog && id < log.photons) log.totals[id] = h.contribution;
+    if (MODE == kLog && id < log.photons) {
+      log.totals[id] = h.contribution;
+      if (log.counting) log.starts[id] = logged;
+    }
     if (TRACE && id < record_count) {
       float *out = records + size_t(id) * kTraceWords;
       out[0] = float(trace.events);
@@ -2567,7 +2573,7 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
     h->record_capacity = record_count;
   }
   monki_net::Net net{nullptr, nullptr, nullptr};
-  monki_net::EventLog log{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0};
+  monki_net::EventLog log{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, 0};
   uint32_t logged_before = 0;
   if (MODE == kLog || MODE == kGuided) {
     if (!proposal || monki_net::proposal_device(proposal) != h->device) {
@@ -2588,6 +2594,31 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
   const size_t lds = lds_bytes<R>(h, block, full_stokes(MODE));
   const uint32_t grid = persistent_grid<R, TRACE, MODE>(h, c.count, block, lds);
   if ((MODE == kGuided || MODE == kDirected) && !reserve_splits(h, grid, block, error, error_size)) return 1;
+  if (MODE == kLog) {
+    // Counting pass: the same histories (their streams are fixed by photon id), each photon's
+    // number of logged events into starts[id]; then starts become the photons' first slots.
+    log.counting = 1u;
+    if (!enqueue<R, TRACE, MODE>(h, c.rounds, grid, block, lds, record_count, net, log, error, error_size) ||
+        !finish(h, error, error_size))
+      return 1;
+    std::vector<uint32_t> starts(c.count);
+    if (!check(hipMemcpy(starts.data(), log.starts, c.count * sizeof(uint32_t), hipMemcpyDeviceToHost), error,
+               error_size, "copy logged counts"))
+      return 1;
+    uint64_t next = logged_before;
+    for (uint32_t i = 0; i < c.count; ++i) {
+      const uint64_t events = starts[i];
+      starts[i] = uint32_t(next < log.capacity ? next : log.capacity);
+      next += events;
+    }
+    const unsigned logged = unsigned(next < log.capacity ? next : log.capacity);
+    if (!check(hipMemcpy(log.starts, starts.data(), c.count * sizeof(uint32_t), hipMemcpyHostToDevice), error,
+               error_size, "upload log starts") ||
+        !check(hipMemcpy(log.count, &logged, sizeof(unsigned), hipMemcpyHostToDevice), error, error_size,
+               "set log count"))
+      return 1;
+    log.counting = 0u;
+  }
   (void)hipEventRecord(h->start, h->stream);
   if (!enqueue<R, TRACE, MODE>(h, c.rounds, grid, block, lds, record_count, net, log, error, error_size)) return 1;
   (void)hipEventRecord(h->stop, h->stream);

@@ -137,6 +137,38 @@ def test_logged_samples_are_well_formed():
     assert features[:, 11].min() == pytest.approx(1 / 32 - 1)
 
 
+def test_logging_and_training_are_reproducible():
+    """Two sessions with the same settings log the same samples in the same order (photon, then
+    event order, fixed by a counting pass) and train to bit-identical parameters (per-wave
+    gradients summed in wave order); a dispatch that overflows the capacity keeps the same
+    events too."""
+
+    from monki import SceneCase
+    from monki.execution import HIP, BatchControls
+    from monki.execution.proposal import ProposalSettings
+
+    settings = ProposalSettings(events=150_000, photons_per_column=2000, steps=40, batch=8192)
+    runs = []
+
+    for _ in range(2):
+        with HIP(seed=3, proposal=settings) as backend:
+            backend.prepare(SceneCase("box", small_box(), 200, 1))
+            samples = backend.network.read_samples()
+            parameters = backend.network.parameters
+            tallies = backend.run_batch(BatchControls(1, 200))
+            runs.append((samples, parameters, tallies, backend.provenance["proposal"]))
+
+    (a, pa, ta, ia), (b, pb, tb, ib) = runs
+    assert ia["training_events"] == ib["training_events"] == 150_000
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+    assert ia["loss_last"] == ib["loss_last"]
+    np.testing.assert_array_equal(pa, pb)
+    # Same histories, tallies summed by float32 atomics in a different order (Q and U are sums
+    # of terms of both signs: the floor is relative to the largest Stokes value).
+    np.testing.assert_allclose(ta.stokes, tb.stokes, rtol=1e-5, atol=1e-6 * np.abs(ta.stokes).max())
+
+
 def plain_reference(directory, scene, rounds, ppc):
     """The plain tracer with 8x the photons of the run under test. The plain estimator of the
     forward-peaked box cloud is heavy-tailed (rare, huge local-estimate contributions): at the

@@ -138,7 +138,7 @@ def spectral_line(steps: int, warmup: int) -> dict:
     from monki.execution.autotune import autotune
 
     cases = spectral_cases(steps + warmup + 1)
-    tuned = autotune(cases, blocks=(128, 256), shards=(4, 16), rounds=1)
+    tuned = autotune(cases, blocks=(128, 256), shards=(1, 4), rounds=1)
     scene = cases[0].scene
 
     with HIPBatch(block=tuned["block"], tally_shards=tuned["tally_shards"]) as batch:

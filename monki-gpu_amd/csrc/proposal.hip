@@ -170,16 +170,44 @@ __device__ inline float head_loss(const float *o, float wx, float wy, float wz, 
   return -g * log_q;
 }
 
-// Gather value of row `row` (lane-dependent) of the per-lane array v[count] held by lane src.
-template <int Count>
-__device__ inline float shuffle_row(const float (&v)[Count], int src, int row) {
-  float out = 0.0f;
-#pragma unroll
-  for (int k = 0; k < Count; ++k) {
-    const float value = __shfl(v[k], src, 64);
-    if (row == k) out = value;
+// Row `low` of the per-lane array v held by lane src for lanes 0..31, row `high` for lanes
+// 32..63 (rows fixed at compile time): two shuffles, one per half.
+template <int Low, int High, int Count>
+__device__ inline float shuffle_rows(const float (&v)[Count], int src) {
+  const float low = Low < Count ? __shfl(v[Low < Count ? Low : 0], src, 64) : 0.0f;
+  const float high = High < Count ? __shfl(v[High < Count ? High : 0], src, 64) : 0.0f;
+  return __lane_id() < 32 ? low : high;
+}
+
+// dY of sample column 32 nb + n as the B operand of k-step S: lane half h holds rows
+// 16 S + slot_row(8 h + e).
+template <int S, int E = 0>
+__device__ inline void dy_operand(const float (&dy)[kOutputs], int src, float (&e)[8]) {
+  if constexpr (E < 8) {
+    e[E] = shuffle_rows<16 * S + slot_row(E), 16 * S + slot_row(8 + E), kOutputs>(dy, src);
+    dy_operand<S, E + 1>(dy, src, e);
   }
-  return out;
+}
+
+template <int S = 0>
+__device__ inline void dy_operands(const float (&dy)[kOutputs], int src, bf16x8 (&d)[8]) {
+  if constexpr (S < 4) {
+    float e[8];
+    dy_operand<S>(dy, src, e);
+    d[S] = pack8(e);
+    dy_operands<S + 1>(dy, src, d);
+  }
+}
+
+// Input rows K + h (h = lane half) of sample src for the layer-0 H tile, K even.
+template <int K = 0>
+__device__ inline void input_rows(const float (&f)[kInputs], int src, int column, uint16_t *h_tile) {
+  if constexpr (K < 32) {
+    const int row = K + (__lane_id() >> 5);
+    const float value = shuffle_rows<K, K + 1, kInputs>(f, src);
+    h_tile[row * kTileStride + column] = row < kInputs ? bf16_bits(value) : 0;
+    input_rows<K + 2>(f, src, column, h_tile);
+  }
 }
 
 // One training block: 64 samples (sample indices from `indices`, or hashed from (seed, step)).
@@ -227,16 +255,7 @@ __global__ void __launch_bounds__(64)
   // dY as B operands (rows 16 s + slot_row(8 h + e) of 64, of the sample column 32 nb + n).
   bf16x8 d[2][8];
 #pragma unroll
-  for (int nb = 0; nb < 2; ++nb) {
-    const int src = 32 * nb + n;
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      float e[8];
-#pragma unroll
-      for (int j = 0; j < 8; ++j) e[j] = shuffle_row<kOutputs>(dy, src, 16 * s + slot_row(8 * h + j));
-      d[nb][s] = pack8(e);
-    }
-  }
+  for (int nb = 0; nb < 2; ++nb) dy_operands(dy, 32 * nb + n, d[nb]);
 
 #pragma unroll
   for (int l = kLayers - 1; l >= 0; --l) {
@@ -257,13 +276,7 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
     for (int nb = 0; nb < 2; ++nb) {
       if (l == 0) {
-        const int src = 32 * nb + n;
-#pragma unroll
-        for (int k = 0; k < 32; k += 2) {
-          const int row = k + h;
-          const float value = shuffle_row<kInputs>(f, src, row);
-          h_tile[row * kTileStride + 32 * nb + n] = row < kInputs ? bf16_bits(value) : 0;
-        }
+        input_rows(f, 32 * nb + n, 32 * nb + n, h_tile);
       } else {
         const float *zl = z + nb * 16384 + (l - 1) * 4096;
 #pragma unroll

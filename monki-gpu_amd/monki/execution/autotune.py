@@ -24,7 +24,7 @@ def signature(cases, device_name: str) -> str:
     return hashlib.sha256(json.dumps([device_name, shapes]).encode()).hexdigest()[:16]
 
 
-def autotune(cases, *, blocks=(64, 128, 256), shards=(8, 16, 32, 64), rounds: int = 3,
+def autotune(cases, *, blocks=(64, 128, 256), shards=(1, 8, 16, 32, 64), rounds: int = 3,
              photons_per_column: Optional[int] = None, device: int = 0, cache: Optional[Path] = CACHE,
              seed: int = 23274) -> dict:
     """Return {"block", "tally_shards", "seconds_per_round", "table", "cached"} for the batch.

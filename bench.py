@@ -28,6 +28,7 @@ to 1 % (SE ~ rounds^-1/2).
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-amf]
        python bench.py --layered-cloud          (the 30x20x35 layered-cloud round instead)
+       python bench.py --race [--amf-rounds R]  (only the box-cloud race)
        python bench.py --copy-calibration | --summarize-traffic DIR | --summarize-profile DB
 """
 
@@ -475,6 +476,7 @@ def main() -> int:
     parser.add_argument("--no-amf", action="store_true", help="skip the time-to-1%% AMF SE legs")
     parser.add_argument("--amf-rounds", type=int, default=128, help="round cap of the box-cloud race")
     parser.add_argument("--layered-cloud", action="store_true", help="time the layered-cloud round instead")
+    parser.add_argument("--race", action="store_true", help="run only the box-cloud race (plain, directed, guided)")
     parser.add_argument("--no-layered", action="store_true", help="skip the secondary layered-cloud line")
     parser.add_argument("--copy-calibration", action="store_true", help="run the copy kernel only")
     parser.add_argument("--summarize-traffic", type=Path, metavar="DIR")
@@ -512,6 +514,10 @@ def main() -> int:
     if args.layered_cloud:
         leg = layered_cloud_line(args.steps, args.warmup)
         print(json.dumps(leg), flush=True)
+        return 0
+
+    if args.race:
+        print(json.dumps(amf_race(args.amf_rounds)), flush=True)
         return 0
 
     print(f"bench: spectral batch, {args.warmup} warm-up + {args.steps} timed rounds", file=sys.stderr, flush=True)

@@ -164,6 +164,15 @@ int monki_hip_proposal_set_samples(void *proposal, const float *features, const 
                                    const float *weights, size_t count, char *error,
                                    size_t error_size);
 
+/* Path target of the training weights (no reference counterpart): with mix > 0, logging also
+ * records, per event, the layer paths the photon still travels before each later estimate,
+ * weighted by the estimate's detector weight and by layer_weights[k] (one per layer of the
+ * scene, at least nz), and a sample's training weight becomes (1 - mix) G + mix s P, G the
+ * detector weight after the event, P that path weight, s = sum G / sum P over the dispatch.
+ * mix 0 turns it off. Returns nonzero (message in error) for mix outside [0, 1] or no weights. */
+int monki_hip_proposal_set_path_target(void *proposal, const float *layer_weights, size_t layers, float mix,
+                                       char *error, size_t error_size);
+
 /* Sample count and sum of sample weights; copy the first `count` samples back. */
 int monki_hip_proposal_samples(void *proposal, uint64_t *count, double *weight_sum);
 int monki_hip_proposal_read_samples(void *proposal, float *features, float *directions,

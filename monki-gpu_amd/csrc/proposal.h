@@ -336,6 +336,10 @@ __device__ inline void mixture_sample(const Mixture &m, float u0, float u1, floa
 // photon's number of logged events into starts[photon]; the host turns them into first slots
 // (prefix sum from the samples already held, clamped to the capacity); the writing pass stores
 // event k of a photon at starts[photon] + k. Samples end up in photon, then event order.
+// With layer weights w (a path target), an estimate of detector weight C at an event where the
+// history's layer paths are L adds C (w . L) to the photon's path contribution; the samples keep
+// it and w . L at the event, the photons its total, so that the path the photon still travels
+// before each later estimate, weighted by that estimate, is known after the dispatch.
 struct EventLog {
   float *features;    // [capacity][kInputs]
   float *directions;  // [capacity][3] direction after the event
@@ -346,6 +350,11 @@ struct EventLog {
   unsigned *count;    // samples held after the dispatch (set by the host)
   uint32_t capacity, photons;
   uint32_t counting;  // 1 in the counting pass
+  const float *layer_weight;  // [layer_count] weights of the layer paths, or null (no path target)
+  uint32_t layer_count;
+  float *path_cumulative;     // [capacity] path contribution of the history up to the event
+  float *event_path;          // [capacity] w . L at the event
+  float *path_totals;         // [photons] path contribution of each whole history
 };
 
 // Host interface of a proposal handle, for the transport translation unit.

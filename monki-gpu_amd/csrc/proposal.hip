@@ -7,10 +7,10 @@
 // sum G: drawn from the phase function and weighted by what followed, the fitted q approaches
 // the product of the phase function and the photon's future contribution.
 //
-// One wave trains on 64 samples per launch block: the forward pass runs on the matrix cores
+// One wave trains on 32 samples per launch block: the forward pass runs on the matrix cores
 // (proposal.h), keeping the hidden pre-activations in a scratch buffer; the backward pass
 // chains W^T products the same way, and the weight gradients dW = dZ H^T come from two
-// 128 x 64 bf16 tiles in LDS (samples on the reduction side, 16 per instruction). Each wave
+// 128 x 32 bf16 tiles in LDS (samples on the reduction side, 16 per instruction). Each wave
 // writes its whole gradient and loss to a partial row of its own; a second kernel sums the rows
 // in wave order, so a training run is reproducible to the bit (no float atomics).
 
@@ -31,11 +31,13 @@ using namespace monki_net;
 
 namespace {
 
-// bf16 per tile row: 64 samples + 8, so that rows start 16-byte aligned and the 16 lanes of a
+// Samples per training wave: one 32-sample block, the matrix cores' N.
+constexpr int kWaveSamples = 32;
+// bf16 per tile row: 32 samples + 8, so that rows start 16-byte aligned and the 16 lanes of a
 // 16-byte LDS read phase hit distinct banks.
-constexpr int kTileStride = 72;
-constexpr int kZPerWave = 2 * 4 * 4096;
-// Waves of one gradient launch (65,536 samples); larger batches run in chunks of this many.
+constexpr int kTileStride = kWaveSamples + 8;
+constexpr int kZPerWave = 4 * 4096;  // pre-activations of the four hidden layers, D order
+// Waves of one gradient launch (32,768 samples); larger batches run in chunks of this many.
 constexpr uint32_t kChunkWaves = 1024;
 
 struct Proposal {
@@ -55,6 +57,11 @@ struct Proposal {
   uint32_t *photon = nullptr, *starts = nullptr;
   double *target_parts = nullptr;  // per-block sums of the training weights of a dispatch
   size_t target_blocks = 0;
+  // Path target: layer weights, the share `path_mix` of the training weight that comes from the
+  // weighted path, and the logged path contributions (see EventLog).
+  float *layer_weight = nullptr, *path_cumulative = nullptr, *event_path = nullptr, *path_totals = nullptr;
+  float path_mix = 0.0f;
+  uint32_t layer_count = 0;
   unsigned *log_count = nullptr;
   size_t photon_capacity = 0;
   double *loss = nullptr;
@@ -210,7 +217,9 @@ __device__ inline void input_rows(const float (&f)[kInputs], int src, int column
   }
 }
 
-// One training block: 64 samples (sample indices from `indices`, or hashed from (seed, step)).
+// One training wave: 32 samples (sample indices from `indices`, or hashed from (seed, step)), one
+// per sample column n = lane % 32; both lane halves hold the same sample (the matrix cores' B and
+// D layouts split each column over the halves), and only the lower half counts its loss.
 __global__ void __launch_bounds__(64)
     train_kernel(Net net, const float *__restrict__ features, const float *__restrict__ directions,
                  const float *__restrict__ weights, const uint32_t *__restrict__ indices, uint32_t count,
@@ -220,7 +229,7 @@ __global__ void __launch_bounds__(64)
   __shared__ __attribute__((aligned(16))) uint16_t h_tile[128 * kTileStride];
   const int lane = __lane_id();
   const int h = lane >> 5, n = lane & 31;
-  const uint32_t sample = base + blockIdx.x * 64u + uint32_t(lane);
+  const uint32_t sample = base + blockIdx.x * uint32_t(kWaveSamples) + uint32_t(n);
   float *grad = partial + size_t(blockIdx.x) * kParams;  // this wave's row: every element written once
   const bool valid = sample < batch;
   const uint32_t index = !valid ? 0u : indices ? indices[sample] : mix32(seed * 0x9E3779B97F4A7C15ull + sample) % count;
@@ -234,13 +243,12 @@ __global__ void __launch_bounds__(64)
   float *z = z_scratch + size_t(blockIdx.x) * kZPerWave;
 
   float o[kOutputs];
-#pragma unroll
-  for (int nb = 0; nb < 2; ++nb) {
+  {
     bf16x8 x[8];
-    scatter_inputs(f, 32 * nb + n, x);
+    scatter_inputs(f, n, x);
     floatx16 raw[2];
-    forward_block(net, x, raw, z + nb * 16384);
-    gather_outputs(raw, n, h == nb, o);
+    forward_block(net, x, raw, z);
+    gather_outputs(raw, n, true, o);
   }
   float dy[kOutputs];
   const float loss = valid ? head_loss(o, wx, wy, wz, g, dy) : 0.0f;
@@ -248,54 +256,46 @@ __global__ void __launch_bounds__(64)
 #pragma unroll
     for (int k = 0; k < kOutputs; ++k) dy[k] = 0.0f;
   }
-  double total = loss;
+  double total = h == 0 ? loss : 0.0;
   for (int offset = 32; offset > 0; offset >>= 1) total += __shfl_xor(total, offset, 64);
   if (lane == 0) loss_part[blockIdx.x] = total;
 
-  // dY as B operands (rows 16 s + slot_row(8 h + e) of 64, of the sample column 32 nb + n).
-  bf16x8 d[2][8];
-#pragma unroll
-  for (int nb = 0; nb < 2; ++nb) dy_operands(dy, 32 * nb + n, d[nb]);
+  // dY as B operands (rows 16 s + slot_row(8 h + e) of 64, of sample column n).
+  bf16x8 d[8];
+  dy_operands(dy, n, d);
 
 #pragma unroll
   for (int l = kLayers - 1; l >= 0; --l) {
     const int r_steps = rows(l) / 16;
     // dZ_{l+1} tile [row][sample] from the B operands.
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb) {
+    for (int s = 0; s < r_steps; ++s) {
+      const bf16x8 v = d[s];
 #pragma unroll
-      for (int s = 0; s < r_steps; ++s) {
-        const bf16x8 v = d[nb][s];
-#pragma unroll
-        for (int j = 0; j < 8; ++j)
-          dz_tile[(16 * s + slot_row(8 * h + j)) * kTileStride + 32 * nb + n] = uint16_t(v[j]);
-      }
+      for (int j = 0; j < 8; ++j) dz_tile[(16 * s + slot_row(8 * h + j)) * kTileStride + n] = uint16_t(v[j]);
     }
     // H_l tile [feature][sample]: inputs for l = 0, else SiLU of the saved pre-activations.
     const int c_blocks = (cols(l) + 31) / 32;
+    if (l == 0) {
+      input_rows(f, n, n, h_tile);
+    } else {
+      const float *zl = z + (l - 1) * 4096;
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb) {
-      if (l == 0) {
-        input_rows(f, 32 * nb + n, 32 * nb + n, h_tile);
-      } else {
-        const float *zl = z + nb * 16384 + (l - 1) * 4096;
+      for (int mb = 0; mb < 4; ++mb) {
 #pragma unroll
-        for (int mb = 0; mb < 4; ++mb) {
-#pragma unroll
-          for (int v = 0; v < 16; ++v) {
-            const int row = 32 * mb + 8 * (v >> 2) + 4 * h + (v & 3);
-            h_tile[row * kTileStride + 32 * nb + n] = bf16_bits(silu(zl[(mb * 16 + v) * 64 + lane]));
-          }
+        for (int v = 0; v < 16; ++v) {
+          const int row = 32 * mb + 8 * (v >> 2) + 4 * h + (v & 3);
+          h_tile[row * kTileStride + n] = bf16_bits(silu(zl[(mb * 16 + v) * 64 + lane]));
         }
       }
     }
     __syncthreads();
-    // dW_l = dZ_{l+1} H_l^T over the 64 samples; db_l = row sums of dZ_{l+1}.
+    // dW_l = dZ_{l+1} H_l^T over the 32 samples; db_l = row sums of dZ_{l+1}.
     for (int mb = 0; mb < rows(l) / 32; ++mb) {
       for (int cb = 0; cb < c_blocks; ++cb) {
         floatx16 c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-        for (int s = 0; s < 4; ++s) {
+        for (int s = 0; s < kWaveSamples / 16; ++s) {
           // Samples 16 s + 8 h .. + 7 of a dZ row and of an H row (any sample order serves, as
           // long as both operands use the same one).
           const bf16x8 av = *reinterpret_cast<const bf16x8 *>(dz_tile + (32 * mb + n) * kTileStride + 16 * s + 8 * h);
@@ -312,34 +312,31 @@ __global__ void __launch_bounds__(64)
     }
     for (int row = lane; row < true_rows(l); row += 64) {
       float sum = 0.0f;
-      for (int t = 0; t < 64; ++t) sum += bf16_value(dz_tile[row * kTileStride + t]);
+      for (int t = 0; t < kWaveSamples; ++t) sum += bf16_value(dz_tile[row * kTileStride + t]);
       grad[param_offset(l) + true_rows(l) * true_cols(l) + row] = sum;
     }
     __syncthreads();
     if (l == 0) break;
     // dH_l = W_l^T dZ_{l+1}, then dZ_l = dH_l * silu'(Z_l): chained like the forward pass.
+    const float *zl = z + (l - 1) * 4096;
+    bf16x8 next[8];
 #pragma unroll
-    for (int nb = 0; nb < 2; ++nb) {
-      const float *zl = z + nb * 16384 + (l - 1) * 4096;
-      bf16x8 next[8];
+    for (int mb = 0; mb < 4; ++mb) {
+      floatx16 c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-      for (int mb = 0; mb < 4; ++mb) {
-        floatx16 c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+      for (int s = 0; s < r_steps; ++s)
+        c = mfma(net.packed_t[packed_t_offset(l) + (mb * r_steps + s) * 64 + lane], d[s], c);
 #pragma unroll
-        for (int s = 0; s < r_steps; ++s)
-          c = mfma(net.packed_t[packed_t_offset(l) + (mb * r_steps + s) * 64 + lane], d[nb][s], c);
+      for (int j = 0; j < 2; ++j) {
+        float e[8];
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-          float e[8];
-#pragma unroll
-          for (int k = 0; k < 8; ++k) e[k] = c[8 * j + k] * silu_grad(zl[(mb * 16 + 8 * j + k) * 64 + lane]);
-          next[2 * mb + j] = pack8(e);
-        }
-        __builtin_amdgcn_sched_barrier(0);
+        for (int k = 0; k < 8; ++k) e[k] = c[8 * j + k] * silu_grad(zl[(mb * 16 + 8 * j + k) * 64 + lane]);
+        next[2 * mb + j] = pack8(e);
       }
-#pragma unroll
-      for (int s = 0; s < 8; ++s) d[nb][s] = next[s];
+      __builtin_amdgcn_sched_barrier(0);
     }
+#pragma unroll
+    for (int s = 0; s < 8; ++s) d[s] = next[s];
   }
 }
 
@@ -378,41 +375,78 @@ __global__ void adam_kernel(float *__restrict__ params, const float *__restrict_
   params[i] -= lr * (mi / correction1) / (sqrtf(vi / correction2) + 1e-8f);
 }
 
-// Gradient of a launch: the waves' partial rows summed in wave order (into grad, or added to it
-// for the later chunks of a large batch), and their losses likewise.
-__global__ void sum_partials(const float *__restrict__ partial, const double *__restrict__ loss_part,
-                             uint32_t waves, bool add, float *__restrict__ grad, double *__restrict__ loss) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < kParams) {
-    float sum = 0.0f;
-    for (uint32_t w = 0; w < waves; ++w) sum += partial[size_t(w) * kParams + i];
-    grad[i] = add ? grad[i] + sum : sum;
+// Gradient of a launch: the waves' partial rows summed in a fixed order (into grad, or added to
+// it for the later chunks of a large batch), and their losses likewise. A 256-thread block covers
+// 64 parameters; its four waves sum interleaved quarters of the rows (wave q: rows q, q + 4, ...),
+// then the quarters add in order.
+__global__ void __launch_bounds__(256)
+    sum_partials(const float *__restrict__ partial, const double *__restrict__ loss_part, uint32_t waves, bool add,
+                 float *__restrict__ grad, double *__restrict__ loss) {
+  __shared__ float quarters[4][64];
+  const int quarter = threadIdx.x >> 6;
+  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+  float sum = 0.0f;
+  if (i < kParams)
+    for (uint32_t w = quarter; w < waves; w += 4) sum += partial[size_t(w) * kParams + i];
+  quarters[quarter][threadIdx.x & 63] = sum;
+  __syncthreads();
+  if (quarter == 0 && i < kParams) {
+    const float total = ((quarters[0][threadIdx.x] + quarters[1][threadIdx.x]) + quarters[2][threadIdx.x]) +
+                        quarters[3][threadIdx.x];
+    grad[i] = add ? grad[i] + total : total;
   }
-  if (i == 0) {
-    double sum = 0.0;
-    for (uint32_t w = 0; w < waves; ++w) sum += loss_part[w];
-    *loss = add ? *loss + sum : sum;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    double total = 0.0;
+    for (uint32_t w = 0; w < waves; ++w) total += loss_part[w];
+    *loss = add ? *loss + total : total;
   }
 }
 
-// Training weights of the events logged in one dispatch: G = photon total - contribution so far;
-// one sum per 256-event block (lanes, then the block's four waves, in a fixed order).
+// Training weights of the events logged in one dispatch: G = photon total - contribution so far
+// (the detector weight the photon contributed after the event) and, with a path target,
+// P = (path total - path contribution so far) - (w . L at the event) G >= 0: the weighted path
+// the photon still travelled before each later estimate, times that estimate's weight. P goes
+// into event_path in place. One sum of each per 256-event block (lanes, then the block's four
+// waves, in a fixed order).
 __global__ void __launch_bounds__(256)
     targets_kernel(const float *__restrict__ totals, const uint32_t *__restrict__ photon,
-                   const float *__restrict__ cumulative, float *__restrict__ weights, uint32_t first,
-                   uint32_t last, double *__restrict__ block_sums) {
-  __shared__ double waves[4];
+                   const float *__restrict__ cumulative, const float *__restrict__ path_totals,
+                   const float *__restrict__ path_cumulative, float *__restrict__ event_path,
+                   float *__restrict__ weights, uint32_t first, uint32_t last, bool path,
+                   double *__restrict__ block_sums) {
+  __shared__ double waves[2][4];
   const uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
-  double value = 0.0;
+  double value = 0.0, path_value = 0.0;
   if (i < last) {
     const float g = fmaxf(0.0f, totals[photon[i]] - cumulative[i]);
     weights[i] = g;
     value = g;
+    if (path) {
+      const float p = fmaxf(0.0f, (path_totals[photon[i]] - path_cumulative[i]) - event_path[i] * g);
+      event_path[i] = p;
+      path_value = p;
+    }
   }
-  for (int offset = 32; offset > 0; offset >>= 1) value += __shfl_xor(value, offset, 64);
-  if (__lane_id() == 0) waves[threadIdx.x >> 6] = value;
+  for (int offset = 32; offset > 0; offset >>= 1) {
+    value += __shfl_xor(value, offset, 64);
+    path_value += __shfl_xor(path_value, offset, 64);
+  }
+  if (__lane_id() == 0) {
+    waves[0][threadIdx.x >> 6] = value;
+    waves[1][threadIdx.x >> 6] = path_value;
+  }
   __syncthreads();
-  if (threadIdx.x == 0) block_sums[blockIdx.x] = ((waves[0] + waves[1]) + waves[2]) + waves[3];
+  if (threadIdx.x < 2)
+    block_sums[2 * blockIdx.x + threadIdx.x] =
+        ((waves[threadIdx.x][0] + waves[threadIdx.x][1]) + waves[threadIdx.x][2]) + waves[threadIdx.x][3];
+}
+
+// Training weight with a path target: (1 - mix) G + mix scale P, scale = sum G / sum P over the
+// dispatch, so both parts carry the same total.
+__global__ void mix_targets(float *__restrict__ weights, const float *__restrict__ path_weights, uint32_t first,
+                            uint32_t last, float detector_share, float path_share) {
+  const uint32_t i = first + blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < last) weights[i] = detector_share * weights[i] + path_share * path_weights[i];
 }
 
 Net view(const Proposal *p) { return Net{p->packed, p->packed_t, p->bias}; }
@@ -431,7 +465,8 @@ void release(Proposal *p) {
   if (p->stream) (void)hipStreamDestroy(p->stream);
   void *buffers[] = {p->params, p->grad, p->adam_m, p->adam_v, p->packed, p->packed_t, p->bias, p->z,
                      p->partial, p->loss_part, p->features, p->directions, p->weights, p->cumulative,
-                     p->totals, p->photon, p->starts, p->target_parts, p->log_count, p->loss};
+                     p->totals, p->photon, p->starts, p->target_parts, p->log_count, p->loss,
+                     p->layer_weight, p->path_cumulative, p->event_path, p->path_totals};
   for (void *b : buffers) (void)hipFree(b);
   delete p;
 }
@@ -458,16 +493,16 @@ bool ensure_scratch(Proposal *p, size_t waves, char *error, size_t size) {
 // kChunkWaves waves run in chunks whose sums add up in chunk order.
 bool gradients(Proposal *p, const uint32_t *indices, uint32_t batch, uint64_t seed, double &loss,
                char *error, size_t size) {
-  const uint32_t total_waves = (batch + 63) / 64;
+  const uint32_t total_waves = (batch + kWaveSamples - 1) / kWaveSamples;
   const uint32_t chunk = total_waves < kChunkWaves ? total_waves : kChunkWaves;
   if (!ensure_scratch(p, chunk, error, size)) return false;
   const float scale = p->weight_sum > 0.0 ? float(double(p->count) / p->weight_sum / batch) : 0.0f;
   for (uint32_t first = 0; first < total_waves; first += chunk) {
     const uint32_t waves = total_waves - first < chunk ? total_waves - first : chunk;
     hipLaunchKernelGGL(train_kernel, dim3(waves), dim3(64), 0, p->stream, view(p), p->features,
-                       p->directions, p->weights, indices, uint32_t(p->count), first * 64u, batch, seed, scale,
-                       p->partial, p->z, p->loss_part);
-    hipLaunchKernelGGL(sum_partials, dim3((kParams + 255) / 256), dim3(256), 0, p->stream, p->partial,
+                       p->directions, p->weights, indices, uint32_t(p->count), first * uint32_t(kWaveSamples), batch,
+                       seed, scale, p->partial, p->z, p->loss_part);
+    hipLaunchKernelGGL(sum_partials, dim3((kParams + 63) / 64), dim3(256), 0, p->stream, p->partial,
                        p->loss_part, waves, first > 0, p->grad, p->loss);
     if (!ok(hipGetLastError(), error, size, "train launch")) return false;
   }
@@ -487,8 +522,10 @@ Net proposal_net(void *handle) { return view(static_cast<Proposal *>(handle)); }
 
 EventLog proposal_log(void *handle) {
   Proposal *p = static_cast<Proposal *>(handle);
+  const bool path = p->path_mix > 0.0f && p->layer_weight;
   return EventLog{p->features, p->directions, p->cumulative, p->photon, p->totals, p->starts, p->log_count,
-                  uint32_t(p->capacity), uint32_t(p->photon_capacity), 0u};
+                  uint32_t(p->capacity), uint32_t(p->photon_capacity), 0u, path ? p->layer_weight : nullptr,
+                  path ? p->layer_count : 0u, p->path_cumulative, p->event_path, p->path_totals};
 }
 
 int proposal_device(void *handle) { return static_cast<Proposal *>(handle)->device; }
@@ -509,19 +546,33 @@ bool proposal_finish_log(void *handle, uint32_t first, hipStream_t stream, char 
       (void)hipFree(p->target_parts);
       p->target_parts = nullptr;
       p->target_blocks = 0;
-      if (!ok(hipMalloc(&p->target_parts, blocks * sizeof(double)), error, size, "allocate target sums"))
+      if (!ok(hipMalloc(&p->target_parts, 2 * blocks * sizeof(double)), error, size, "allocate target sums"))
         return false;
       p->target_blocks = blocks;
     }
+    const bool path = p->path_mix > 0.0f && p->layer_weight;
     hipLaunchKernelGGL(targets_kernel, dim3(unsigned(blocks)), dim3(256), 0, stream, p->totals, p->photon,
-                       p->cumulative, p->weights, first, last, p->target_parts);
-    std::vector<double> sums(blocks);
+                       p->cumulative, p->path_totals, p->path_cumulative, p->event_path, p->weights, first, last,
+                       path, p->target_parts);
+    std::vector<double> sums(2 * blocks);
     if (!ok(hipGetLastError(), error, size, "targets launch") ||
-        !ok(hipMemcpyAsync(sums.data(), p->target_parts, blocks * sizeof(double), hipMemcpyDeviceToHost, stream),
+        !ok(hipMemcpyAsync(sums.data(), p->target_parts, 2 * blocks * sizeof(double), hipMemcpyDeviceToHost,
+                           stream),
             error, size, "copy target sums") ||
         !ok(hipStreamSynchronize(stream), error, size, "targets"))
       return false;
-    for (double value : sums) p->weight_sum += value;  // block order: reproducible
+    double detector = 0.0, path_sum = 0.0;
+    for (size_t b = 0; b < blocks; ++b) {  // block order: reproducible
+      detector += sums[2 * b];
+      path_sum += sums[2 * b + 1];
+    }
+    if (path && path_sum > 0.0) {
+      const float share = p->path_mix;
+      hipLaunchKernelGGL(mix_targets, dim3(unsigned(blocks)), dim3(256), 0, stream, p->weights, p->event_path,
+                         first, last, 1.0f - share, float(share * detector / path_sum));
+      if (!ok(hipGetLastError(), error, size, "mix targets launch")) return false;
+    }
+    p->weight_sum += detector;
   }
   p->count = last;
   return true;
@@ -604,7 +655,8 @@ int monki_hip_proposal_reserve(void *handle, size_t events, size_t photons, char
   }
   if (!ok(hipSetDevice(p->device), error, error_size, "hipSetDevice")) return 1;
   void **buffers[] = {(void **)&p->features, (void **)&p->directions, (void **)&p->weights,
-                      (void **)&p->cumulative, (void **)&p->photon, (void **)&p->totals, (void **)&p->starts};
+                      (void **)&p->cumulative, (void **)&p->photon, (void **)&p->totals, (void **)&p->starts,
+                      (void **)&p->path_cumulative, (void **)&p->event_path, (void **)&p->path_totals};
   for (void **b : buffers) {
     (void)hipFree(*b);
     *b = nullptr;
@@ -618,6 +670,9 @@ int monki_hip_proposal_reserve(void *handle, size_t events, size_t photons, char
       !ok(hipMalloc(&p->photon, events * sizeof(uint32_t)), error, error_size, "allocate samples") ||
       (photons && !ok(hipMalloc(&p->totals, photons * sizeof(float)), error, error_size, "allocate totals")) ||
       (photons && !ok(hipMalloc(&p->starts, photons * sizeof(uint32_t)), error, error_size, "allocate log starts")) ||
+      (photons && !ok(hipMalloc(&p->path_totals, photons * sizeof(float)), error, error_size, "allocate path totals")) ||
+      !ok(hipMalloc(&p->path_cumulative, events * sizeof(float)), error, error_size, "allocate path samples") ||
+      !ok(hipMalloc(&p->event_path, events * sizeof(float)), error, error_size, "allocate path samples") ||
       !ok(hipMemset(p->log_count, 0, sizeof(unsigned)), error, error_size, "clear log counter"))
     return 1;
   p->capacity = events;
@@ -645,6 +700,26 @@ int monki_hip_proposal_set_samples(void *handle, const float *features, const fl
   for (size_t i = 0; i < count; ++i) sum += weights[i];
   p->count = count;
   p->weight_sum = sum;
+  return 0;
+}
+
+int monki_hip_proposal_set_path_target(void *handle, const float *layer_weights, size_t layers, float mix,
+                                       char *error, size_t error_size) {
+  Proposal *p = static_cast<Proposal *>(handle);
+  if (!p || !(mix >= 0.0f && mix <= 1.0f) || (mix > 0.0f && (!layer_weights || layers == 0 || layers > 256))) {
+    fail(error, error_size, "Invalid path target (mix in [0, 1], 1-256 layer weights)");
+    return 1;
+  }
+  if (!ok(hipSetDevice(p->device), error, error_size, "hipSetDevice")) return 1;
+  (void)hipFree(p->layer_weight);
+  p->layer_weight = nullptr;
+  p->path_mix = mix;
+  p->layer_count = mix > 0.0f ? uint32_t(layers) : 0u;
+  if (mix > 0.0f &&
+      (!ok(hipMalloc(&p->layer_weight, layers * sizeof(float)), error, error_size, "allocate layer weights") ||
+       !ok(hipMemcpy(p->layer_weight, layer_weights, layers * sizeof(float), hipMemcpyHostToDevice), error,
+           error_size, "upload layer weights")))
+    return 1;
   return 0;
 }
 

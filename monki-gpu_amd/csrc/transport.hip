@@ -1679,6 +1679,9 @@ __global__ void __launch_bounds__(kBlock32)
   int top = 0;
   uint32_t tree = 0;
   uint32_t logged = 0;  // events of the photon logged so far (logging mode)
+  // Logging with a path target: sum over the photon's estimates so far of their detector weight
+  // times the weighted path at their event, and the weighted path at the last event.
+  float path_contribution = 0.0f, logged_path = 0.0f;
   const bool window = STEER && stack && guidance.split_above > 0.0f;
   extern __shared__ unsigned char lds_raw[];
   // Layer rows of odd length (the largest nz of the scenes, | 1): lanes reading one row, or
@@ -1753,6 +1756,7 @@ __global__ void __launch_bounds__(kBlock32)
           active = true;
           tree = 0;
           logged = 0;
+          path_contribution = 0.0f;
         }
       }
     }
@@ -1796,6 +1800,13 @@ __global__ void __launch_bounds__(kBlock32)
       // The local estimates toward every view, lane by lane (the detector walks of one lane run
       // back to back, so the wave waits once for its longest lane).
       Estimate<R> e0{R(0), R(0), R(0), R(0), -1}, e1 = e0, e2 = e0;
+      // Logging with a path target: the weighted path of the history up to this event (its
+      // layer paths times the layer weights) and the detector weight before its estimates.
+      float event_path = 0.0f, weight_before = 0.0f;
+      if (MODE == kLog && log.layer_weight && event) {
+        for (int k = 0; k < s.nz; ++k) event_path += float(history[k]) * log.layer_weight[k];
+        weight_before = h.contribution;
+      }
       if (event) {
 #pragma unroll 1
         for (int view = 0; view < s.views; ++view) {
@@ -1809,6 +1820,10 @@ __global__ void __launch_bounds__(kBlock32)
           else
             e2 = e;
         }
+      }
+      if (MODE == kLog && log.layer_weight && event) {
+        path_contribution += (h.contribution - weight_before) * event_path;
+        logged_path = event_path;
       }
       // Per view: the event's local estimate joins the open record of the same pixel and layer,
       // or the open record is written out first (another pixel, or the photon left its layer)
@@ -1928,6 +1943,10 @@ __global__ void __launch_bounds__(kBlock32)
         log.directions[size_t(slot) * 3 + 2] = float(h.p.c);
         log.cumulative[slot] = h.contribution;
         log.photon[slot] = id;
+        if (log.layer_weight) {
+          log.path_cumulative[slot] = path_contribution;
+          log.event_path[slot] = logged_path;
+        }
       }
     }
     if (MERGE) {
@@ -1957,6 +1976,7 @@ __global__ void __launch_bounds__(kBlock32)
     active = false;
     if (MODE == kLog && id < log.photons) {
       log.totals[id] = h.contribution;
+      if (log.layer_weight) log.path_totals[id] = path_contribution;
       if (log.counting) log.starts[id] = logged;
     }
     if (TRACE && id < record_count) {
@@ -2573,7 +2593,7 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
     h->record_capacity = record_count;
   }
   monki_net::Net net{nullptr, nullptr, nullptr};
-  monki_net::EventLog log{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0, 0};
+  monki_net::EventLog log{};
   uint32_t logged_before = 0;
   if (MODE == kLog || MODE == kGuided) {
     if (!proposal || monki_net::proposal_device(proposal) != h->device) {
@@ -2585,6 +2605,10 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
       log = monki_net::proposal_log(proposal);
       if (!log.features || c.count > log.photons) {
         set_error(error, error_size, "Reserve proposal samples for this dispatch first");
+        return 1;
+      }
+      if (log.layer_weight && log.layer_count < uint32_t(s.nz)) {
+        set_error(error, error_size, "The path target needs a weight for every layer of the scene");
         return 1;
       }
       logged_before = monki_net::proposal_samples(proposal);

@@ -169,6 +169,21 @@ def test_logging_and_training_are_reproducible():
     np.testing.assert_allclose(ta.stokes, tb.stokes, rtol=1e-5, atol=1e-6 * np.abs(ta.stokes).max())
 
 
+# Gates of the agreement tests. The layer AMF of the small box at 48 rounds x 200 photons per
+# column is a heavy-tailed ratio estimator: the plain tracer itself, at that budget against its
+# own run at 8x the photons, reaches max |z| 4.1-5.6 over its 4,896 (view, layer, pixel) cells and
+# chi2/dof 1.25-1.36 (six seeds). The AMF gate on max |z| sits above that range; the chi-square
+# gate is the one a bias fails (the same plain run with its AMF scaled by 1.005: chi2/dof 39,
+# max |z| 127; many cells have relative SEs far below the median 1.2%).
+RADIANCE_MAX_Z, AMF_MAX_Z, MAX_CHI2_PER_DOF = 5.5, 7.0, 1.5
+
+
+def assert_agreement(report):
+    for name, stats in report.items():
+        assert stats["max_abs_z"] < (AMF_MAX_Z if name == "AMF_layer" else RADIANCE_MAX_Z), name
+        assert 0.6 < stats["chi_square_per_dof"] < MAX_CHI2_PER_DOF, name
+
+
 def plain_reference(directory, scene, rounds, ppc):
     """The plain tracer with 8x the photons of the run under test. The plain estimator of the
     forward-peaked box cloud is heavy-tailed (rare, huge local-estimate contributions): at the
@@ -225,9 +240,7 @@ def test_guided_tracer_agrees_with_plain_tracer(tmp_path, guidance):
         report[name] = z_statistics(a, se_a, b, se_b, floor=1e-12)
 
     print(json.dumps(report))
-    for name, stats in report.items():
-        assert stats["max_abs_z"] < 5.5, name
-        assert 0.6 < stats["chi_square_per_dof"] < 1.5, name
+    assert_agreement(report)
 
 
 DIRECTED = {
@@ -267,9 +280,7 @@ def test_directed_tracer_agrees_with_plain_tracer(tmp_path, configuration):
         report[name] = z_statistics(a, se_a, b, se_b, floor=1e-12)
 
     print(json.dumps(report))
-    for name, stats in report.items():
-        assert stats["max_abs_z"] < 5.5, name
-        assert 0.6 < stats["chi_square_per_dof"] < 1.5, name
+    assert_agreement(report)
 
 
 def _relative_amf_se(result):

@@ -456,10 +456,13 @@ class HIP(Backend):
         start = time.perf_counter()
         network = Proposal(self._native, self.device, settings.seed)
         self._network = network
+        if settings.path_target > 0.0 and not self.save_path:
+            raise ValueError("a path target for the proposal needs the layer paths (save_path=True)")
+
         logged = log_rounds(
             self._native, self._handle, network, settings, columns,
             (self.seed + TRAINING_SEED_OFFSET + settings.seed) % 2**32, self.max_steps,
-            self.photons_per_dispatch, self._tally_count,
+            self.photons_per_dispatch, self._tally_count, self._shape,
         )  # fmt: skip
         logging_seconds = time.perf_counter() - start
         losses = network.train(settings.steps, settings.batch, settings.learning_rate, settings.seed)
@@ -475,6 +478,7 @@ class HIP(Backend):
             "settings": {key: getattr(settings, key) for key in settings.__dataclass_fields__},
             "logging_rounds": logged["rounds"],
             "training_events": logged["events"],
+            **({"layer_weights": logged["layer_weights"]} if "layer_weights" in logged else {}),
             "logging_seconds": logging_seconds,
             "training_seconds": time.perf_counter() - start - logging_seconds,
             "loss_first": float(losses[0]) if len(losses) else None,

@@ -68,6 +68,12 @@ class ProposalSettings:
     ``surface``. Weight window: photons heavier than ``split_above`` are split, lighter than
     ``roulette_below`` play Russian roulette for ``roulette_weight`` (``split_above`` 0 turns
     the window off, which is refused for tabulated species beyond order 4).
+
+    Training target: a sample's weight is the detector weight G the photon contributed after the
+    event; with ``path_target`` p > 0 it is (1 - p) G + p s P, P the layer path the photon still
+    travelled before each later estimate, weighted by that estimate's detector weight and by
+    1 / (the layer's path tally in a plain round), s = sum G / sum P: the network then also
+    values directions that feed the layers whose paths are rare.
     """
 
     events: int = 4_000_000
@@ -84,6 +90,7 @@ class ProposalSettings:
     split_above: float = 2.0
     roulette_below: float = 0.005
     roulette_weight: float = 0.01
+    path_target: float = 0.0
 
     def species_mask(self) -> int:
         if self.guided_kinds is None:
@@ -114,6 +121,9 @@ class ProposalSettings:
         if not 0.0 <= float(self.lobe_fraction) <= 1.0:
             raise ValueError("lobe_fraction must be in [0, 1]")
 
+        if not 0.0 <= float(self.path_target) <= 1.0:
+            raise ValueError("path_target must be in [0, 1]")
+
         if self.guided_kinds is not None:
             for kind in self.guided_kinds:
                 if kind != "gas" and not (type(kind) is int and 0 <= kind < 4):
@@ -133,6 +143,7 @@ def declare(native) -> None:
     native.monki_hip_proposal_reserve.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]
     native.monki_hip_proposal_set_samples.argtypes = [ctypes.c_void_p, _F, _F, _F, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]
     native.monki_hip_proposal_samples.argtypes = [ctypes.c_void_p, _U64, _D]
+    native.monki_hip_proposal_set_path_target.argtypes = [ctypes.c_void_p, _F, ctypes.c_size_t, ctypes.c_float, ctypes.c_char_p, ctypes.c_size_t]
     native.monki_hip_proposal_read_samples.argtypes = [ctypes.c_void_p, _F, _F, _F, ctypes.c_size_t]
     native.monki_hip_proposal_gradients.argtypes = [ctypes.c_void_p, ctypes.c_size_t, _F, _D, ctypes.c_char_p, ctypes.c_size_t]
     native.monki_hip_proposal_train.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float, ctypes.c_uint64, _D, ctypes.c_char_p, ctypes.c_size_t]
@@ -153,6 +164,7 @@ def declare(native) -> None:
     for name in (
         "monki_hip_proposal_set_parameters", "monki_hip_proposal_get_parameters", "monki_hip_proposal_reserve",
         "monki_hip_proposal_set_samples", "monki_hip_proposal_samples", "monki_hip_proposal_read_samples",
+        "monki_hip_proposal_set_path_target",
         "monki_hip_proposal_gradients", "monki_hip_proposal_train", "monki_hip_proposal_forward",
         "monki_hip_attach_proposal", "monki_hip_run_guided", "monki_hip_trace_guided", "monki_hip_run_logged",
         "monki_hip_set_directed", "monki_hip_run_directed", "monki_hip_set_steering",
@@ -287,6 +299,12 @@ class Proposal:
             error,
         )  # fmt: skip
 
+    def set_path_target(self, layer_weights, mix: float) -> None:
+        weights = np.ascontiguousarray(layer_weights, dtype=np.float32)
+        error = ctypes.create_string_buffer(1024)
+        self._check(self.native.monki_hip_proposal_set_path_target(self.handle, weights.ctypes.data_as(_F), weights.size,
+                                                                   float(mix), error, len(error)), error)  # fmt: skip
+
     def samples(self):
         """(sample count, sum of weights)."""
 
@@ -352,9 +370,21 @@ class Proposal:
         self.handle = None
 
 
+def path_layer_weights(tallies: np.ndarray, layout: tuple) -> np.ndarray:
+    """1 / (path tally of each layer, summed over pixels and views) of one round's tallies in the
+    output order [Stokes (nmat, nx, ny, na) | paths (nx, ny, nz, na) | weights (nx, ny, na)];
+    layers without path get 0. layout = (nmat, nx, ny, nz, na)."""
+
+    nmat, nx, ny, nz, na = layout
+    start = nmat * nx * ny * na
+    paths = tallies[start:start + nx * ny * nz * na].reshape((nx, ny, nz, na), order="F").sum(axis=(0, 1, 3))
+    return np.where(paths > 0.0, 1.0 / np.where(paths > 0.0, paths, 1.0), 0.0)
+
+
 def log_rounds(native, session, proposal: Proposal, settings: ProposalSettings, columns: int, seed: int,
-               max_steps: int, photons_per_dispatch: int, tally_count: int) -> dict:
-    """Run plain logging rounds (their own seed) until ``settings.events`` samples exist."""
+               max_steps: int, photons_per_dispatch: int, tally_count: int, layout: tuple = None) -> dict:
+    """Run plain logging rounds (their own seed) until ``settings.events`` samples exist.
+    layout = (nmat, nx, ny, nz, na) of the scene, needed for a path target."""
 
     per_round = settings.photons_per_column * columns
     dispatch = min(per_round, photons_per_dispatch)
@@ -363,6 +393,17 @@ def log_rounds(native, session, proposal: Proposal, settings: ProposalSettings, 
     error = ctypes.create_string_buffer(8192)
     round_index = 0
     events = 0
+    layer_weights = None
+
+    if settings.path_target > 0.0:
+        # Layer weights from a plain round of the first dispatch's photons on streams of their own:
+        # 1 / (the layer's path tally summed over pixels and views); its samples are dropped.
+        command = (ctypes.c_uint32 * 8)(dispatch, 0, 1, (seed + 1) % 2**32, max_steps, 0, per_round, 1)
+        if native.monki_hip_run_logged(session, command, out.ctypes.data_as(_D), proposal.handle, error, len(error)):
+            raise RuntimeError(error.value.decode() or "logging round failed")
+        layer_weights = path_layer_weights(out, layout)
+        proposal.reserve(settings.events, dispatch)
+        proposal.set_path_target(layer_weights, settings.path_target)
 
     while events < settings.events and round_index < 4096:
         round_index += 1
@@ -380,5 +421,10 @@ def log_rounds(native, session, proposal: Proposal, settings: ProposalSettings, 
             if events >= settings.events:
                 break
 
-    return {"rounds": round_index, "events": events, "weight_sum": proposal.samples()[1]}
+    result = {"rounds": round_index, "events": events, "weight_sum": proposal.samples()[1]}
+
+    if layer_weights is not None:
+        result["layer_weights"] = [float(w) for w in layer_weights]
+
+    return result
 

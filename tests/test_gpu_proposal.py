@@ -137,6 +137,38 @@ def test_logged_samples_are_well_formed():
     assert features[:, 11].min() == pytest.approx(1 / 32 - 1)
 
 
+def test_path_target_logging_is_well_formed():
+    """With a path target the logging records layer weights from a plain round (one per layer,
+    positive where the layer has path) and training weights that mix the path weight with the
+    detector weight: finite, non-negative, not all zero, and different from the detector weights
+    alone on the same streams."""
+
+    from monki import SceneCase
+    from monki.execution import HIP
+    from monki.execution.proposal import ProposalSettings
+
+    scene = small_box()
+    weights = {}
+
+    for mix in (0.0, 1.0):
+        settings = ProposalSettings(events=100_000, photons_per_column=300, steps=1, batch=1024, path_target=mix)
+
+        with HIP(proposal=settings) as backend:
+            backend.prepare(SceneCase("box", scene, 100, 1))
+            features, _, weights[mix] = backend.network.read_samples()
+            info = backend.provenance["proposal"]
+
+        if mix > 0:
+            layer_weights = np.asarray(info["layer_weights"])
+            assert layer_weights.size == scene.nz
+            assert np.all(layer_weights >= 0) and np.all(layer_weights[:3] > 0)
+
+    for mix, values in weights.items():
+        assert values.size == 100_000 and np.all(np.isfinite(values)) and np.all(values >= 0) and values.sum() > 0
+    # Same samples (the layer-weight round runs on streams of its own), other training weights.
+    assert not np.allclose(weights[0.0], weights[1.0])
+
+
 def test_logging_and_training_are_reproducible():
     """Two sessions with the same settings log the same samples in the same order (photon, then
     event order, fixed by a counting pass) and train to bit-identical parameters (per-wave

@@ -612,6 +612,34 @@ def test_unbounded_steering_of_tabulated_species_needs_the_weight_window(setting
     kind()
 
 
+def test_path_target_settings_and_layer_weights():
+    """path_target is a share in [0, 1]; the layer weights are 1 / (the layer's path tally summed
+    over pixels and views), read from the documented output order, 0 for layers without path."""
+
+    from monki.execution import ProposalSettings
+    from monki.execution.hip_tables import tally_count
+    from monki.execution.proposal import path_layer_weights
+
+    assert ProposalSettings().path_target == 0.0
+    ProposalSettings(path_target=1.0)
+
+    for bad in (-0.1, 1.5):
+        with pytest.raises(ValueError, match="path_target"):
+            ProposalSettings(path_target=bad)
+
+    nmat, nx, ny, nz, na = 3, 4, 2, 5, 3
+    count = tally_count(nmat, nx, ny, nz, na, True)
+    tallies = np.zeros(count)
+    paths = np.zeros((nx, ny, nz, na))
+    paths[:, :, 0, :] = 0.25  # layer 0: total 0.25 x 24
+    paths[1, 0, 2, 1] = 3.0  # layer 2: one cell
+    start = nmat * nx * ny * na
+    tallies[start:start + paths.size] = paths.ravel(order="F")
+    tallies[start + paths.size:] = 7.0  # weights, not read
+    weights = path_layer_weights(tallies, (nmat, nx, ny, nz, na))
+    np.testing.assert_allclose(weights, [1 / 6.0, 0.0, 1 / 3.0, 0.0, 0.0])
+
+
 def test_decomposition_settings_layout_and_product_variables():
     """save_scats: exclusive with layer paths, matched against the fixture line, tallies split
     in the documented order, and the Report_ncdf variables in the metadata."""

@@ -29,6 +29,7 @@ to 1 % (SE ~ rounds^-1/2).
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-amf]
        python bench.py --layered-cloud          (the 30x20x35 layered-cloud round instead)
        python bench.py --race [--amf-rounds R]  (only the box-cloud race)
+       python bench.py --batch [--steps K]      (only the 64-scene box-cloud batch)
        python bench.py --copy-calibration | --summarize-traffic DIR | --summarize-profile DB
 """
 
@@ -135,11 +136,40 @@ def spectral_cases(rounds: int):
 def spectral_line(steps: int, warmup: int) -> dict:
     """Timed rounds of the 64-wavelength batch with the autotuned launch shape."""
 
+    return batch_line(spectral_cases(steps + warmup + 1), steps, warmup, shards=(1, 4))
+
+
+BOX_BATCH_PPC = 1000
+
+
+def box_batch_line(steps: int = 3, warmup: int = 1) -> dict:
+    """The 64-scene box-cloud batch (one seed: SZA U(0, 70), albedo U(0.02, 0.3), cloud optical
+    depth lognormal with median 15, grids 16-30 x 12-20 x 35), 1,000 photons per column."""
+
+    from monki import SceneCase, configs
+
+    cases = [SceneCase(f"box-{i:02d}", scene, BOX_BATCH_PPC, steps + warmup + 1)
+             for i, scene in enumerate(configs.box_cloud_batch(64))]  # fmt: skip
+    run = batch_line(cases, steps, warmup, shards=(1, 8))
+    return {
+        "workload": "64 box-cloud scenes from one seed (grids 16-30 x 12-20 x 35, SZA U(0, 70), albedo U(0.02, 0.3), "
+        f"cloud tau lognormal median 15), I/Q/U, 3 views, layer paths, {BOX_BATCH_PPC} photons/column, one "
+        "graph-replayed launch per round",
+        "photon_events_per_s": run["value"],
+        "ms_per_round": 1e3 * run["elapsed"] / steps,
+        "device_ms_per_round": 1e3 * run["device_round"],
+        "photons_per_round": run["photons"] // steps,
+        "autotune": run["autotune"],
+    }
+
+
+def batch_line(cases, steps: int, warmup: int, shards=(1, 4)) -> dict:
+    """Timed rounds of a batch (rounds accumulated on the device) with the autotuned shape."""
+
     from monki.execution import HIPBatch
     from monki.execution.autotune import autotune
 
-    cases = spectral_cases(steps + warmup + 1)
-    tuned = autotune(cases, blocks=(128, 256), shards=(1, 4), rounds=1)
+    tuned = autotune(cases, blocks=(128, 256), shards=shards, rounds=1)
     scene = cases[0].scene
 
     with HIPBatch(block=tuned["block"], tally_shards=tuned["tally_shards"]) as batch:
@@ -477,7 +507,8 @@ def main() -> int:
     parser.add_argument("--amf-rounds", type=int, default=128, help="round cap of the box-cloud race")
     parser.add_argument("--layered-cloud", action="store_true", help="time the layered-cloud round instead")
     parser.add_argument("--race", action="store_true", help="run only the box-cloud race (plain, directed, guided)")
-    parser.add_argument("--no-layered", action="store_true", help="skip the secondary layered-cloud line")
+    parser.add_argument("--batch", action="store_true", help="time only the 64-scene box-cloud batch")
+    parser.add_argument("--no-layered", action="store_true", help="skip the secondary layered-cloud and batch lines")
     parser.add_argument("--copy-calibration", action="store_true", help="run the copy kernel only")
     parser.add_argument("--summarize-traffic", type=Path, metavar="DIR")
     parser.add_argument("--summarize-profile", type=Path, metavar="DB")
@@ -518,6 +549,10 @@ def main() -> int:
 
     if args.race:
         print(json.dumps(amf_race(args.amf_rounds)), flush=True)
+        return 0
+
+    if args.batch:
+        print(json.dumps(box_batch_line(args.steps, args.warmup)), flush=True)
         return 0
 
     print(f"bench: spectral batch, {args.warmup} warm-up + {args.steps} timed rounds", file=sys.stderr, flush=True)
@@ -580,6 +615,8 @@ def main() -> int:
     if not args.no_layered:
         print("bench: layered-cloud round", file=sys.stderr, flush=True)
         line["layered_cloud"] = layered_cloud_line()
+        print("bench: 64-scene box-cloud batch", file=sys.stderr, flush=True)
+        line["box_cloud_batch"] = box_batch_line()
 
     if not args.no_cpu:
         from monki import configs

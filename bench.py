@@ -30,6 +30,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-amf]
        python bench.py --layered-cloud          (the 30x20x35 layered-cloud round instead)
        python bench.py --race [--amf-rounds R]  (only the box-cloud race)
        python bench.py --batch [--steps K]      (only the 64-scene box-cloud batch)
+       python bench.py --train [--steps K]      (only the proposal's Adam steps, 16,384 samples each)
        python bench.py --copy-calibration | --summarize-traffic DIR | --summarize-profile DB
 """
 
@@ -386,6 +387,46 @@ def amf_race(rounds: int = 128, photons_per_column: int = 10_000) -> dict:
     return result
 
 
+# Network arithmetic per training sample: forward (2 x 56,360 multiply-adds less the biases),
+# input gradients of layers 1-4 and weight gradients of all five layers, about 332 kFLOP.
+TRAIN_FLOP_PER_SAMPLE = 2 * (12 * 128 + 3 * 128 * 128 + 128 * 40) * 3 - 2 * 12 * 128
+TRAIN_BATCH = 16_384
+
+
+def train_line(steps: int, warmup: int) -> dict:
+    """Adam steps of the learned proposal on 1e6 synthetic samples (uniform features, isotropic
+    directions, weights in [0.1, 2)): wall time per step of one `train` call."""
+
+    import numpy as np
+
+    from monki.execution.hip import load_library
+    from monki.execution.proposal import Proposal
+
+    native = load_library()
+    proposal = Proposal(native, 0, seed=3)
+    rng = np.random.default_rng(1)
+    count = 1_000_000
+    features = rng.uniform(-1.0, 1.0, (count, 12)).astype(np.float32)
+    directions = rng.normal(size=(count, 3))
+    directions /= np.linalg.norm(directions, axis=1, keepdims=True)
+    weights = rng.uniform(0.1, 2.0, count).astype(np.float32)
+
+    try:
+        proposal.set_samples(features, directions.astype(np.float32), weights)
+        proposal.train(max(warmup, 1), TRAIN_BATCH, 1e-3)
+        start = time.perf_counter()
+        losses = proposal.train(steps, TRAIN_BATCH, 1e-3)
+        seconds = (time.perf_counter() - start) / steps
+    finally:
+        proposal.close()
+
+    return {"workload": "proposal training", "batch": TRAIN_BATCH, "steps": steps,
+            "ms_per_step": 1e3 * seconds, "samples_per_s": TRAIN_BATCH / seconds,
+            "gflop_per_step": TRAIN_FLOP_PER_SAMPLE * TRAIN_BATCH / 1e9,
+            "tflops_wall": TRAIN_FLOP_PER_SAMPLE * TRAIN_BATCH / seconds / 1e12,
+            "loss_first": float(losses[0]), "loss_last": float(losses[-1])}  # fmt: skip
+
+
 COPY_BYTES = 1 << 30
 TRAFFIC_FILES = {"spectral": ROOT / "profiles" / "traffic_spectral_field.json",
                  "layered-cloud": ROOT / "profiles" / "traffic_layered_cloud.json"}  # fmt: skip
@@ -508,6 +549,7 @@ def main() -> int:
     parser.add_argument("--layered-cloud", action="store_true", help="time the layered-cloud round instead")
     parser.add_argument("--race", action="store_true", help="run only the box-cloud race (plain, directed, guided)")
     parser.add_argument("--batch", action="store_true", help="time only the 64-scene box-cloud batch")
+    parser.add_argument("--train", action="store_true", help="time only the proposal's Adam steps")
     parser.add_argument("--no-layered", action="store_true", help="skip the secondary layered-cloud and batch lines")
     parser.add_argument("--copy-calibration", action="store_true", help="run the copy kernel only")
     parser.add_argument("--summarize-traffic", type=Path, metavar="DIR")
@@ -553,6 +595,10 @@ def main() -> int:
 
     if args.batch:
         print(json.dumps(box_batch_line(args.steps, args.warmup)), flush=True)
+        return 0
+
+    if args.train:
+        print(json.dumps(train_line(args.steps, args.warmup)), flush=True)
         return 0
 
     print(f"bench: spectral batch, {args.warmup} warm-up + {args.steps} timed rounds", file=sys.stderr, flush=True)

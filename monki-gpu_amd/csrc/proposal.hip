@@ -7,12 +7,14 @@
 // sum G: drawn from the phase function and weighted by what followed, the fitted q approaches
 // the product of the phase function and the photon's future contribution.
 //
-// One wave trains on 32 samples per launch block: the forward pass runs on the matrix cores
-// (proposal.h), keeping the hidden pre-activations in a scratch buffer; the backward pass
-// chains W^T products the same way, and the weight gradients dW = dZ H^T come from two
-// 128 x 32 bf16 tiles in LDS (samples on the reduction side, 16 per instruction). Each wave
-// writes its whole gradient and loss to a partial row of its own; a second kernel sums the rows
-// in wave order, so a training run is reproducible to the bit (no float atomics).
+// A training step runs three kernels. train_kernel: one wave per 32-sample block runs the forward
+// pass on the matrix cores (proposal.h), keeping the hidden pre-activations in a scratch buffer,
+// the loss head, and the backward pass, which chains W^T products the same way; per layer it
+// stores the block's dZ rows and H columns (bf16, transposed through LDS) in global memory in the
+// matrix cores' operand order, and its loss. weight_gradients: dW = dZ H^T with the samples on
+// the reduction side, one wave per 32 x 32 tile and 512 samples (split K); the waves of the first
+// column tile also form db = dZ 1 the same way. sum_gradients: the split-K partials and the losses
+// summed in a fixed order, so a training run is reproducible to the bit (no float atomics).
 
 #include <hip/hip_runtime.h>
 
@@ -39,6 +41,20 @@ constexpr int kTileStride = kWaveSamples + 8;
 constexpr int kZPerWave = 4 * 4096;  // pre-activations of the four hidden layers, D order
 // Waves of one gradient launch (32,768 samples); larger batches run in chunks of this many.
 constexpr uint32_t kChunkWaves = 1024;
+// Weight-gradient operands and tiles. Layer l's dZ operands come in rows(l) / 32 blocks of 32
+// rows, its H operands in col_tiles(l) blocks of 32 columns; per block, one operand (8 bf16 per
+// lane) per 16-sample chunk. dW tiles: 32 x 32 blocks of every layer, layer by layer.
+__host__ __device__ constexpr int row_tiles(int l) { return rows(l) / 32; }
+__host__ __device__ constexpr int col_tiles(int l) { return (cols(l) + 31) / 32; }
+__host__ __device__ constexpr int dz_block_offset(int l) { return l == 0 ? 0 : dz_block_offset(l - 1) + row_tiles(l - 1); }
+__host__ __device__ constexpr int h_block_offset(int l) { return l == 0 ? 0 : h_block_offset(l - 1) + col_tiles(l - 1); }
+__host__ __device__ constexpr int tile_offset(int l) {
+  return l == 0 ? 0 : tile_offset(l - 1) + row_tiles(l - 1) * col_tiles(l - 1);
+}
+constexpr int kDzBlocks = dz_block_offset(kLayers);
+constexpr int kHBlocks = h_block_offset(kLayers);
+constexpr int kTiles = tile_offset(kLayers);
+constexpr uint32_t kGroupChunks = 32;  // 16-sample chunks per weight-gradient wave (512 samples)
 
 struct Proposal {
   int device = -1;
@@ -47,7 +63,9 @@ struct Proposal {
   float *bias = nullptr;
   float *z = nullptr;
   size_t z_waves = 0;
-  float *partial = nullptr;     // [waves][kParams] gradient of each wave of a launch
+  bf16x8 *dz_ops = nullptr, *h_ops = nullptr;  // weight-gradient operands of a launch
+  float *tile_part = nullptr;   // [kTiles][groups][64][16] split-K partial dW tiles, D order
+  float *bias_part = nullptr;   // [kDzBlocks][groups][32] split-K partial db blocks
   double *loss_part = nullptr;  // [waves] loss of each wave
   // Samples: features [n][12], directions [n][3], weights [n].
   float *features = nullptr, *directions = nullptr, *weights = nullptr;
@@ -223,14 +241,14 @@ __device__ inline void input_rows(const float (&f)[kInputs], int src, int column
 __global__ void __launch_bounds__(64)
     train_kernel(Net net, const float *__restrict__ features, const float *__restrict__ directions,
                  const float *__restrict__ weights, const uint32_t *__restrict__ indices, uint32_t count,
-                 uint32_t base, uint32_t batch, uint64_t seed, float scale, float *__restrict__ partial,
-                 float *__restrict__ z_scratch, double *__restrict__ loss_part) {
+                 uint32_t base, uint32_t batch, uint64_t seed, float scale, bf16x8 *__restrict__ dz_ops,
+                 bf16x8 *__restrict__ h_ops, uint32_t chunks, float *__restrict__ z_scratch,
+                 double *__restrict__ loss_part) {
   __shared__ __attribute__((aligned(16))) uint16_t dz_tile[128 * kTileStride];
   __shared__ __attribute__((aligned(16))) uint16_t h_tile[128 * kTileStride];
   const int lane = __lane_id();
   const int h = lane >> 5, n = lane & 31;
   const uint32_t sample = base + blockIdx.x * uint32_t(kWaveSamples) + uint32_t(n);
-  float *grad = partial + size_t(blockIdx.x) * kParams;  // this wave's row: every element written once
   const bool valid = sample < batch;
   const uint32_t index = !valid ? 0u : indices ? indices[sample] : mix32(seed * 0x9E3779B97F4A7C15ull + sample) % count;
   float f[kInputs];
@@ -290,30 +308,17 @@ __global__ void __launch_bounds__(64)
       }
     }
     __syncthreads();
-    // dW_l = dZ_{l+1} H_l^T over the 32 samples; db_l = row sums of dZ_{l+1}.
-    for (int mb = 0; mb < rows(l) / 32; ++mb) {
-      for (int cb = 0; cb < c_blocks; ++cb) {
-        floatx16 c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // The operands of dW_l = dZ_{l+1} H_l^T for this block's two 16-sample chunks (any sample
+    // order serves, as long as both operands use the same one).
 #pragma unroll
-        for (int s = 0; s < kWaveSamples / 16; ++s) {
-          // Samples 16 s + 8 h .. + 7 of a dZ row and of an H row (any sample order serves, as
-          // long as both operands use the same one).
-          const bf16x8 av = *reinterpret_cast<const bf16x8 *>(dz_tile + (32 * mb + n) * kTileStride + 16 * s + 8 * h);
-          const bf16x8 bv = *reinterpret_cast<const bf16x8 *>(h_tile + (32 * cb + n) * kTileStride + 16 * s + 8 * h);
-          c = mfma(av, bv, c);
-        }
-        const int col = 32 * cb + n;
-#pragma unroll
-        for (int v = 0; v < 16; ++v) {
-          const int row = 32 * mb + 8 * (v >> 2) + 4 * h + (v & 3);
-          if (row < true_rows(l) && col < true_cols(l)) grad[param_offset(l) + row * true_cols(l) + col] = c[v];
-        }
-      }
-    }
-    for (int row = lane; row < true_rows(l); row += 64) {
-      float sum = 0.0f;
-      for (int t = 0; t < kWaveSamples; ++t) sum += bf16_value(dz_tile[row * kTileStride + t]);
-      grad[param_offset(l) + true_rows(l) * true_cols(l) + row] = sum;
+    for (int s = 0; s < kWaveSamples / 16; ++s) {
+      const size_t chunk = size_t(blockIdx.x) * (kWaveSamples / 16) + s;
+      for (int mb = 0; mb < rows(l) / 32; ++mb)
+        dz_ops[(size_t(dz_block_offset(l) + mb) * chunks + chunk) * 64 + lane] =
+            *reinterpret_cast<const bf16x8 *>(dz_tile + (32 * mb + n) * kTileStride + 16 * s + 8 * h);
+      for (int cb = 0; cb < c_blocks; ++cb)
+        h_ops[(size_t(h_block_offset(l) + cb) * chunks + chunk) * 64 + lane] =
+            *reinterpret_cast<const bf16x8 *>(h_tile + (32 * cb + n) * kTileStride + 16 * s + 8 * h);
     }
     __syncthreads();
     if (l == 0) break;
@@ -375,30 +380,92 @@ __global__ void adam_kernel(float *__restrict__ params, const float *__restrict_
   params[i] -= lr * (mi / correction1) / (sqrtf(vi / correction2) + 1e-8f);
 }
 
-// Gradient of a launch: the waves' partial rows summed in a fixed order (into grad, or added to
-// it for the later chunks of a large batch), and their losses likewise. A 256-thread block covers
-// 64 parameters; its four waves sum interleaved quarters of the rows (wave q: rows q, q + 4, ...),
-// then the quarters add in order.
-__global__ void __launch_bounds__(256)
-    sum_partials(const float *__restrict__ partial, const double *__restrict__ loss_part, uint32_t waves, bool add,
-                 float *__restrict__ grad, double *__restrict__ loss) {
-  __shared__ float quarters[4][64];
-  const int quarter = threadIdx.x >> 6;
-  const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-  float sum = 0.0f;
-  if (i < kParams)
-    for (uint32_t w = quarter; w < waves; w += 4) sum += partial[size_t(w) * kParams + i];
-  quarters[quarter][threadIdx.x & 63] = sum;
-  __syncthreads();
-  if (quarter == 0 && i < kParams) {
-    const float total = ((quarters[0][threadIdx.x] + quarters[1][threadIdx.x]) + quarters[2][threadIdx.x]) +
-                        quarters[3][threadIdx.x];
-    grad[i] = add ? grad[i] + total : total;
+// dW tiles, split K: one wave per (tile, group of kGroupChunks 16-sample chunks), the partial
+// tile in D order [lane][register]; waves of column tile 0 also sum their dZ rows (B = ones).
+__global__ void __launch_bounds__(64)
+    weight_gradients(const bf16x8 *__restrict__ dz_ops, const bf16x8 *__restrict__ h_ops, uint32_t chunks,
+                     uint32_t groups, float *__restrict__ tile_part, float *__restrict__ bias_part) {
+  const int lane = __lane_id();
+  const uint32_t work = blockIdx.x;
+  const int tile = int(work / groups);
+  const uint32_t group = work % groups;
+  int l = 0;
+  while (l + 1 < kLayers && tile >= tile_offset(l + 1)) ++l;
+  const int local = tile - tile_offset(l);
+  const int mb = local / col_tiles(l), cb = local % col_tiles(l);
+  const bf16x8 *a = dz_ops + size_t(dz_block_offset(l) + mb) * chunks * 64 + lane;
+  const bf16x8 *b = h_ops + size_t(h_block_offset(l) + cb) * chunks * 64 + lane;
+  const uint32_t first = group * kGroupChunks;
+  const uint32_t last = first + kGroupChunks < chunks ? first + kGroupChunks : chunks;
+  floatx16 c = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+  if (cb == 0) {
+    const short one = 0x3F80;  // bf16 1.0
+    const bf16x8 ones = {one, one, one, one, one, one, one, one};
+    floatx16 rows = {0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t chunk = first; chunk < last; ++chunk) {
+      const bf16x8 x = a[size_t(chunk) * 64];
+      c = mfma(x, b[size_t(chunk) * 64], c);
+      rows = mfma(x, ones, rows);
+    }
+    // Column 0 of the row sums: lanes 0 and 32, rows 8 (v / 4) + 4 (lane / 32) + v % 4.
+    if ((lane & 31) == 0) {
+      float *out = bias_part + (size_t(dz_block_offset(l) + mb) * groups + group) * 32;
+#pragma unroll
+      for (int v = 0; v < 16; ++v) out[8 * (v >> 2) + 4 * (lane >> 5) + (v & 3)] = rows[v];
+    }
+  } else {
+    for (uint32_t chunk = first; chunk < last; ++chunk) c = mfma(a[size_t(chunk) * 64], b[size_t(chunk) * 64], c);
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    double total = 0.0;
-    for (uint32_t w = 0; w < waves; ++w) total += loss_part[w];
-    *loss = add ? *loss + total : total;
+  *reinterpret_cast<floatx16 *>(tile_part + (size_t(work) * 64 + lane) * 16) = c;
+}
+
+// Gradient of a launch (into grad, or added to it for the later chunks of a large batch), 256
+// threads a block: blocks [0, 4 kTiles) sum a quarter of one dW tile's split-K partials in group
+// order (block 4 tile + j, thread t: D element 256 j + t), blocks [4 kTiles, 4 kTiles + kDzBlocks)
+// one db block's, and the last block the waves' losses (strided partial sums, then a fixed tree).
+constexpr uint32_t kSumBlocks = 4 * kTiles + kDzBlocks + 1;
+__global__ void __launch_bounds__(256)
+    sum_gradients(const float *__restrict__ tile_part, const float *__restrict__ bias_part,
+                  const double *__restrict__ loss_part, uint32_t waves, uint32_t groups, bool add,
+                  float *__restrict__ grad, double *__restrict__ loss) {
+  const int t = threadIdx.x;
+  const int block = blockIdx.x;
+  if (block < 4 * kTiles) {
+    const int tile = block >> 2, e = 256 * (block & 3) + t;
+    int l = 0;
+    while (l + 1 < kLayers && tile >= tile_offset(l + 1)) ++l;
+    const int mb = (tile - tile_offset(l)) / col_tiles(l), cb = (tile - tile_offset(l)) % col_tiles(l);
+    float sum = 0.0f;
+    for (uint32_t g = 0; g < groups; ++g) sum += tile_part[(size_t(tile) * groups + g) * 1024 + e];
+    // D order: element 16 lane + v holds row 8 (v / 4) + 4 (lane / 32) + v % 4, column lane % 32.
+    const int lane = e >> 4, v = e & 15;
+    const int row = 32 * mb + 8 * (v >> 2) + 4 * (lane >> 5) + (v & 3), col = 32 * cb + (lane & 31);
+    if (row < true_rows(l) && col < true_cols(l)) {
+      float &out = grad[param_offset(l) + row * true_cols(l) + col];
+      out = add ? out + sum : sum;
+    }
+  } else if (block < 4 * kTiles + kDzBlocks) {
+    const int b = block - 4 * kTiles;
+    int l = 0;
+    while (l + 1 < kLayers && b >= dz_block_offset(l + 1)) ++l;
+    const int row = 32 * (b - dz_block_offset(l)) + t;
+    if (t < 32 && row < true_rows(l)) {
+      float sum = 0.0f;
+      for (uint32_t g = 0; g < groups; ++g) sum += bias_part[(size_t(b) * groups + g) * 32 + t];
+      float &out = grad[param_offset(l) + true_rows(l) * true_cols(l) + row];
+      out = add ? out + sum : sum;
+    }
+  } else {
+    __shared__ double partial[256];
+    double sum = 0.0;
+    for (uint32_t w = t; w < waves; w += 256) sum += loss_part[w];
+    partial[t] = sum;
+    __syncthreads();
+    for (int width = 128; width > 0; width >>= 1) {
+      if (t < width) partial[t] += partial[t + width];
+      __syncthreads();
+    }
+    if (t == 0) *loss = add ? *loss + partial[0] : partial[0];
   }
 }
 
@@ -464,34 +531,46 @@ void release(Proposal *p) {
   if (p->device >= 0) (void)hipSetDevice(p->device);
   if (p->stream) (void)hipStreamDestroy(p->stream);
   void *buffers[] = {p->params, p->grad, p->adam_m, p->adam_v, p->packed, p->packed_t, p->bias, p->z,
-                     p->partial, p->loss_part, p->features, p->directions, p->weights, p->cumulative,
+                     p->dz_ops, p->h_ops, p->bias_part, p->tile_part, p->loss_part, p->features,
+                     p->directions, p->weights, p->cumulative,
                      p->totals, p->photon, p->starts, p->target_parts, p->log_count, p->loss,
                      p->layer_weight, p->path_cumulative, p->event_path, p->path_totals};
   for (void *b : buffers) (void)hipFree(b);
   delete p;
 }
 
-// Per-wave buffers of a gradient launch (pre-activations and partial gradients), kept for later
-// launches.
+// Per-wave buffers of a gradient launch (pre-activations, weight-gradient operands, bias rows,
+// losses) and its split-K tiles, kept for later launches.
 bool ensure_scratch(Proposal *p, size_t waves, char *error, size_t size) {
   if (waves <= p->z_waves) return true;
-  void *buffers[] = {p->z, p->partial, p->loss_part};
+  void *buffers[] = {p->z, p->dz_ops, p->h_ops, p->tile_part, p->bias_part, p->loss_part};
   for (void *b : buffers) (void)hipFree(b);
   p->z = nullptr;
-  p->partial = nullptr;
+  p->dz_ops = p->h_ops = nullptr;
+  p->tile_part = p->bias_part = nullptr;
   p->loss_part = nullptr;
   p->z_waves = 0;
+  const size_t chunks = waves * (kWaveSamples / 16);
+  const size_t groups = (chunks + kGroupChunks - 1) / kGroupChunks;
   if (!ok(hipMalloc(&p->z, waves * kZPerWave * sizeof(float)), error, size, "allocate proposal scratch") ||
-      !ok(hipMalloc(&p->partial, waves * kParams * sizeof(float)), error, size, "allocate partial gradients") ||
+      !ok(hipMalloc(&p->dz_ops, size_t(kDzBlocks) * chunks * 64 * sizeof(bf16x8)), error, size,
+          "allocate gradient operands") ||
+      !ok(hipMalloc(&p->h_ops, size_t(kHBlocks) * chunks * 64 * sizeof(bf16x8)), error, size,
+          "allocate gradient operands") ||
+      !ok(hipMalloc(&p->tile_part, size_t(kTiles) * groups * 64 * 16 * sizeof(float)), error, size,
+          "allocate gradient tiles") ||
+      !ok(hipMalloc(&p->bias_part, size_t(kDzBlocks) * groups * 32 * sizeof(float)), error, size,
+          "allocate bias tiles") ||
       !ok(hipMalloc(&p->loss_part, waves * sizeof(double)), error, size, "allocate partial losses"))
     return false;
   p->z_waves = waves;
   return true;
 }
 
-// Gradient of the mean loss over `batch` samples into p->grad; returns the loss. Batches past
-// kChunkWaves waves run in chunks whose sums add up in chunk order.
-bool gradients(Proposal *p, const uint32_t *indices, uint32_t batch, uint64_t seed, double &loss,
+// Launches the gradient of the mean loss over `batch` samples into p->grad and the loss into
+// device_loss, on p->stream (no synchronization). Batches past kChunkWaves waves run in chunks
+// whose sums add up in chunk order.
+bool gradients(Proposal *p, const uint32_t *indices, uint32_t batch, uint64_t seed, double *device_loss,
                char *error, size_t size) {
   const uint32_t total_waves = (batch + kWaveSamples - 1) / kWaveSamples;
   const uint32_t chunk = total_waves < kChunkWaves ? total_waves : kChunkWaves;
@@ -499,17 +578,33 @@ bool gradients(Proposal *p, const uint32_t *indices, uint32_t batch, uint64_t se
   const float scale = p->weight_sum > 0.0 ? float(double(p->count) / p->weight_sum / batch) : 0.0f;
   for (uint32_t first = 0; first < total_waves; first += chunk) {
     const uint32_t waves = total_waves - first < chunk ? total_waves - first : chunk;
+    const uint32_t chunks = waves * uint32_t(kWaveSamples / 16);
+    const uint32_t groups = (chunks + kGroupChunks - 1) / kGroupChunks;
     hipLaunchKernelGGL(train_kernel, dim3(waves), dim3(64), 0, p->stream, view(p), p->features,
                        p->directions, p->weights, indices, uint32_t(p->count), first * uint32_t(kWaveSamples), batch,
-                       seed, scale, p->partial, p->z, p->loss_part);
-    hipLaunchKernelGGL(sum_partials, dim3((kParams + 63) / 64), dim3(256), 0, p->stream, p->partial,
-                       p->loss_part, waves, first > 0, p->grad, p->loss);
+                       seed, scale, p->dz_ops, p->h_ops, chunks, p->z, p->loss_part);
+    hipLaunchKernelGGL(weight_gradients, dim3(uint32_t(kTiles) * groups), dim3(64), 0, p->stream, p->dz_ops,
+                       p->h_ops, chunks, groups, p->tile_part, p->bias_part);
+    hipLaunchKernelGGL(sum_gradients, dim3(kSumBlocks), dim3(256), 0, p->stream, p->tile_part,
+                       p->bias_part, p->loss_part, waves, groups, first > 0, p->grad, device_loss);
     if (!ok(hipGetLastError(), error, size, "train launch")) return false;
   }
-  if (!ok(hipMemcpyAsync(&loss, p->loss, sizeof(double), hipMemcpyDeviceToHost, p->stream), error, size,
-          "copy loss") ||
-      !ok(hipStreamSynchronize(p->stream), error, size, "train"))
-    return false;
+  return true;
+}
+
+bool train_steps(Proposal *p, uint32_t steps, uint32_t batch, float learning_rate, uint64_t seed,
+                 double *device_losses, char *error, size_t error_size) {
+  const float beta1 = 0.9f, beta2 = 0.999f;
+  for (uint32_t i = 0; i < steps; ++i) {
+    if (!gradients(p, nullptr, batch, seed * 1000003ull + p->step, device_losses + i, error, error_size))
+      return false;
+    ++p->step;
+    const float c1 = 1.0f - powf(beta1, float(p->step));
+    const float c2 = 1.0f - powf(beta2, float(p->step));
+    hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, p->stream, p->params, p->grad,
+                       p->adam_m, p->adam_v, kParams, learning_rate, beta1, beta2, c1, c2);
+    if (!ok(hipGetLastError(), error, error_size, "adam launch") || !repack(p, error, error_size)) return false;
+  }
   return true;
 }
 
@@ -759,9 +854,12 @@ int monki_hip_proposal_gradients(void *handle, size_t count, float *grad, double
       ok(hipMalloc(&indices, count * sizeof(uint32_t)), error, error_size, "allocate indices") &&
       ok(hipMemcpy(indices, host.data(), count * sizeof(uint32_t), hipMemcpyHostToDevice), error, error_size,
          "upload indices") &&
-      gradients(p, indices, uint32_t(count), 0, value, error, error_size) &&
-      ok(hipMemcpy(grad, p->grad, kParams * sizeof(float), hipMemcpyDeviceToHost), error, error_size,
-         "copy gradient");
+      gradients(p, indices, uint32_t(count), 0, p->loss, error, error_size) &&
+      ok(hipMemcpyAsync(&value, p->loss, sizeof(double), hipMemcpyDeviceToHost, p->stream), error, error_size,
+         "copy loss") &&
+      ok(hipMemcpyAsync(grad, p->grad, kParams * sizeof(float), hipMemcpyDeviceToHost, p->stream), error,
+         error_size, "copy gradient") &&
+      ok(hipStreamSynchronize(p->stream), error, error_size, "gradient");
   (void)hipFree(indices);
   if (loss) *loss = value;
   return good ? 0 : 1;
@@ -775,19 +873,20 @@ int monki_hip_proposal_train(void *handle, uint32_t steps, uint32_t batch, float
     return 1;
   }
   if (!ok(hipSetDevice(p->device), error, error_size, "hipSetDevice")) return 1;
-  const float beta1 = 0.9f, beta2 = 0.999f;
-  for (uint32_t i = 0; i < steps; ++i) {
-    double loss = 0.0;
-    if (!gradients(p, nullptr, batch, seed * 1000003ull + p->step, loss, error, error_size)) return 1;
-    ++p->step;
-    const float c1 = 1.0f - powf(beta1, float(p->step));
-    const float c2 = 1.0f - powf(beta2, float(p->step));
-    hipLaunchKernelGGL(adam_kernel, dim3((kParams + 255) / 256), dim3(256), 0, p->stream, p->params, p->grad,
-                       p->adam_m, p->adam_v, kParams, learning_rate, beta1, beta2, c1, c2);
-    if (!ok(hipGetLastError(), error, error_size, "adam launch") || !repack(p, error, error_size)) return 1;
-    if (losses) losses[i] = loss;
-  }
-  return ok(hipStreamSynchronize(p->stream), error, error_size, "train") ? 0 : 1;
+  // Each step's loss lands in device memory; one copy after the last step (no per-step wait).
+  double *device_losses = nullptr;
+  if (!ok(hipMalloc(&device_losses, size_t(steps) * sizeof(double)), error, error_size, "allocate losses"))
+    return 1;
+  const int status = train_steps(p, steps, batch, learning_rate, seed, device_losses, error, error_size) &&
+                             (!losses || ok(hipMemcpyAsync(losses, device_losses, size_t(steps) * sizeof(double),
+                                                           hipMemcpyDeviceToHost, p->stream),
+                                            error, error_size, "copy losses")) &&
+                             ok(hipStreamSynchronize(p->stream), error, error_size, "train")
+                         ? 0
+                         : 1;
+  (void)hipStreamSynchronize(p->stream);
+  (void)hipFree(device_losses);
+  return status;
 }
 
 int monki_hip_proposal_forward(void *handle, const float *features, size_t count, float *out, char *error,

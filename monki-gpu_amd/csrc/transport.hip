@@ -551,7 +551,8 @@ template <typename R> __device__ inline R boundary_distance(const Photon<R> &p) 
 
 // ADVANCE_internal (mc.f90:1531-1580): move to the nearest face, x before y before z on ties,
 // periodic in x and y. Returns the distance travelled.
-template <typename R> __device__ inline R advance(const Scene &s, Photon<R> &p) {
+template <typename R, bool LAYERS = false>
+__device__ inline R advance(const Scene &s, Photon<R> &p, const float *layers = nullptr, int row = 0) {
   const R eps = R(1e-16);
   const R aa = fabs(p.a), ab = fabs(p.b), ac = fabs(p.c);
   const R dx = p.rx / fmax(aa, eps);
@@ -574,7 +575,8 @@ template <typename R> __device__ inline R advance(const Scene &s, Photon<R> &p) 
     p.k += p.c >= R(0) ? 1 : -1;
     p.rx = fmax(R(0), p.rx - aa * d);
     p.ry = fmax(R(0), p.ry - ab * d);
-    if (p.k >= 0 && p.k < s.nz) p.rz = R(s.zunit[p.k]);
+    // Layer thickness from the wave's LDS layer table where the caller has one.
+    if (p.k >= 0 && p.k < s.nz) p.rz = LAYERS ? R(layers[row + p.k]) : R(s.zunit[p.k]);
   }
   return d;
 }
@@ -639,9 +641,16 @@ __device__ inline DetectorHit detector_walk(const Scene &s, const Photon<R> &eve
   change_direction<R>(s, ray, va, vb, mu, R(s.phi));
   tau = R(0);
   uint32_t steps = 0;
+  // The cell walk is software-pipelined: a cell's extinction is loaded when the walk enters the
+  // cell and added to tau one cell later, so the load's latency overlaps the next face
+  // crossing instead of stalling every step (the sum and its order are unchanged). The factored
+  // parts stay separate until the add for the same reason.
+  R held_d = R(0);
+  float held_part = 0.0f, held_gas = 0.0f;
   while (ray.k >= 0 && ray.k < s.nz - 1) {
     if (++steps > max_steps) return DetectorHit{0, 0, false};
     if (!DOWN && ray.k >= s.uniform_top) {
+      tau += held_d * R(held_part + held_gas);
       // Layers at and above uniform_top are uniform: the wave's LDS layer table holds their
       // extinction, height above and optical depth above.
       const R height_above = LAYERS ? R(layers[2 * row + ray.k]) : R(s.height_above[ray.k]);
@@ -660,10 +669,16 @@ __device__ inline DetectorHit detector_walk(const Scene &s, const Photon<R> &eve
     }
     const int layer = ray.k;
     const int cell = cell_index<R>(s, ray.i, ray.j, layer);
-    const R d = advance<R>(s, ray);
-    tau += d * R(cell_extinction(s, cell, layer));
+    const float part = s.species_beta ? s.species_beta[cell] : s.beta[cell];
+    const float gas = s.species_beta ? s.gas_beta[layer] : 0.0f;
+    const R d = advance<R, LAYERS>(s, ray, layers, row);
+    tau += held_d * R(held_part + held_gas);
+    held_d = d;
+    held_part = part;
+    held_gas = gas;
     ++n.detector;
   }
+  tau += held_d * R(held_part + held_gas);
   return DetectorHit{ray.i, ray.j, true};
 }
 
@@ -902,7 +917,7 @@ __device__ inline int walk(const Scene &s, const Controls &c, unsigned *failure,
       break;
     }
     const int old_layer = p.k;
-    const R d = advance<R>(s, p);
+    const R d = advance<R, LAYERS>(s, p, layers, row);
     ++n.walk;
     if (s.paths) {
       history[old_layer] += d;

@@ -535,6 +535,14 @@ def summarize_profile(database: Path) -> str:
         for n, d, g, w, lds, sc, v, acc, sg in dispatches
         if "transport" in n or "reduce" in n or "assemble" in n
     ]
+    # The proposal's kernels run thousands of times: the launch shape of the first of each.
+    first = {}
+
+    for n, d, g, w, lds, sc, v, acc, sg in dispatches:
+        if any(key in n for key in ("train_kernel", "weight_gradients", "sum_gradients", "pack_kernel", "adam_kernel")):
+            first.setdefault(short(n), f"| `{short(n)}` (first) | {d / 1e6:.3f} | {g} | {w} | {lds} | {sc} | {v} | {acc} | {sg} |")
+
+    lines += list(first.values())
     return "\n".join(lines) + "\n"
 
 

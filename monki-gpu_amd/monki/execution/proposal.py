@@ -58,7 +58,9 @@ class ProposalSettings:
 
     Training: ``events`` scattering events logged from the plain tracer (``photons_per_column``
     per logging round), then ``steps`` Adam steps over random minibatches of ``batch`` samples
-    at ``learning_rate``; ``seed`` initializes the network and the logging streams.
+    at ``learning_rate``; ``seed`` initializes the network and the logging streams. The
+    defaults log 10^8 events (80 bytes each on the device, 8 GB) and draw about one epoch of
+    them (6,000 x 16,384 samples).
 
     Use: a steered event draws its direction from the phase function with probability
     ``alpha``, else from the proposal: ``lobe_fraction`` x lobes around the view directions
@@ -76,9 +78,9 @@ class ProposalSettings:
     values directions that feed the layers whose paths are rare.
     """
 
-    events: int = 4_000_000
+    events: int = 100_000_000
     photons_per_column: int = 2_000
-    steps: int = 1_500
+    steps: int = 6_000
     batch: int = 16_384
     learning_rate: float = 1e-3
     alpha: float = 0.9

@@ -86,6 +86,11 @@ double monki_hip_gpu_seconds(void *handle);
 /* Seconds of the transport kernel alone in the last run, from HIP events on its stream. */
 double monki_hip_kernel_seconds(void *handle);
 
+/* Layer paths each lane of a plain float32 launch keeps in LDS: the largest nz | 1 of the
+ * session's scenes, or fewer when whole rows would leave less than three waves per SIMD (the
+ * upper layers' paths then live in global memory). A diagnostic with no reference counterpart. */
+int monki_hip_layer_rows(void *handle);
+
 /* Device name of the handle (kernels/metal/bridge.m:140-143). */
 const char *monki_hip_device_name(void *handle);
 

@@ -54,10 +54,14 @@ enum Status : int { kRunning = 0, kEscaped = 1, kAbsorbed = 2, kFailed = 3 };
 // kBackward4: backward transport (nmat 1/3 and nmat 4). kScats, kScats4: the single/multiple-
 // scattering decomposition (calc_radiances_and_scats) in place of the layer paths; kBackScats,
 // kBackScats4 the same with backward transport. kSurface, kSurface4: the plain tracer toward
-// a detector at the ground.
+// a detector at the ground. kWide: the plain tracer (float32, forward, TOA, nmat 1 or 3) for
+// scenes whose layer rows would leave fewer than three waves per SIMD: LDS holds the lower
+// layers' paths of each lane, the upper (uniform) layers' paths are the first flight's whole
+// crossings from TOA (a layer index per lane) until the photon has an event up there or flies
+// back down through them, then a row of the lane's own in global memory (same values).
 enum Mode : int {
   kPlain = 0, kLog = 1, kGuided = 2, kDirected = 3, kStokes4 = 4, kBackward = 5, kBackward4 = 6,
-  kScats = 7, kScats4 = 8, kSurface = 9, kSurface4 = 10, kBackScats = 11, kBackScats4 = 12
+  kScats = 7, kScats4 = 8, kSurface = 9, kSurface4 = 10, kBackScats = 11, kBackScats4 = 12, kWide = 13
 };
 constexpr bool full_stokes(int mode) {
   return mode == kStokes4 || mode == kBackward4 || mode == kScats4 || mode == kSurface4 || mode == kBackScats4;
@@ -788,7 +792,7 @@ template <typename R, bool V4> __device__ inline bool finite_stokes(const Stokes
 // Launch at TOA in the vacuum layer of column photon mod (nx*ny) (mc.f90:351-464).
 template <typename R>
 __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R *history,
-                              History<R> &h) {
+                              History<R> &h, int held = 1 << 30) {
   const uint32_t identity = c.first + id;
   h.round_offset = identity / c.per_round;
   const uint32_t photon = identity % c.per_round;
@@ -805,7 +809,7 @@ __device__ void launch_photon(const Scene &s, const Controls &c, uint32_t id, R 
   p.rx = R(s.xunit) * R(uniform(h.rng));
   p.ry = R(s.yunit) * R(uniform(h.rng));
   p.rz = R(s.zunit[p.k]);
-  for (int k = 0; k < s.nz; ++k) history[k] = R(0);
+  for (int k = 0; k < s.nz && k < held; ++k) history[k] = R(0);
   h.st = Stokes<R>{R(1), R(0), R(0)};
   h.w2 = Stokes<R>{R(0), R(1), R(0), R(0)};
   h.w3 = Stokes<R>{R(0), R(0), R(1), R(0)};
@@ -837,6 +841,42 @@ __device__ inline int opposite_grid_pixel(const Scene &s, const Photon<R> &p, R 
   return max(i, 0) + s.nx * max(j, 0);
 }
 
+// Paths of a kWide lane in the upper layers low .. nz - 1 (all horizontally uniform). While
+// `residual` is clear, the photon's first flight from TOA crossed layers descent .. nz - 1
+// whole (descent nz: none yet) and the other upper layers hold nothing; once set, the lane's
+// row in global memory holds every upper layer's path (layer k at k - low). `initial`: no
+// event yet, the current flight is the first one.
+struct Upper {
+  int descent;
+  bool residual, initial;
+};
+
+// A whole crossing of layer k at direction cosine c, as the free-path walk measures it.
+template <typename R>
+__device__ inline R whole_crossing(const float *layers, int row, int k, R c) {
+  return R(layers[row + k]) / fmax(fabs(c), R(1e-16));
+}
+
+// From here on the lane's global row holds the upper paths: the first flight's crossings.
+template <typename R>
+__device__ inline void upper_materialize(const Scene &s, Upper &u, float *upper, int low, const float *layers,
+                                         int row) {
+  if (u.residual) return;
+  const R c0 = R(s.launch_c);
+  for (int k = low; k < s.nz; ++k)
+    upper[k - low] = k >= u.descent ? float(whole_crossing<R>(layers, row, k, c0)) : 0.0f;
+  u.residual = true;
+  u.descent = s.nz;
+}
+
+// The lane's path in upper layer k.
+template <typename R>
+__device__ inline R upper_path(const Scene &s, const Upper &u, const float *upper, int low, const float *layers,
+                               int row, int k) {
+  if (u.residual) return R(upper[k - low]);
+  return k >= u.descent ? whole_crossing<R>(layers, row, k, R(s.launch_c)) : R(0);
+}
+
 // Advance one photon to its next collision or surface hit (mc.f90:476-677). Returns kRunning
 // at the event (`surface` set for a surface hit), kEscaped or kFailed. layer_path: the path
 // the flight added to the layer it ends in (with layer paths), 0 on entry.
@@ -844,12 +884,29 @@ __device__ inline int opposite_grid_pixel(const Scene &s, const Photon<R> &p, R 
 // horizontally uniform layers (-1 for varying ones), thickness, height above and optical depth
 // above, read instead of the global tables on every layer crossing of the free path and by
 // the detector walk.
-template <typename R, bool LAYERS>
+// WIDE (kWide): the paths of layers low .. nz - 1 live outside LDS (Upper); a flight's
+// crossings there are added when it ends in an event up there or flies back down below `low`,
+// never for a flight that escapes.
+template <typename R, bool LAYERS, bool WIDE = false>
 __device__ inline int walk(const Scene &s, const Controls &c, unsigned *failure, R *history, History<R> &h,
-                           Counts &n, const float *layers, int row, bool &surface, R &layer_path) {
+                           Counts &n, const float *layers, int row, bool &surface, R &layer_path,
+                           Upper *u = nullptr, float *upper = nullptr, int low = 1 << 30) {
   const R eps = R(1e-16);
   Photon<R> &p = h.p;
   double slide = 0.0;  // flight through uniform layers not yet applied to the x/y cells
+  // WIDE: upper layers this flight crossed (pending_lo .. pending_hi, none while lo > hi), the
+  // first of them with a length of its own (the flight may start inside it).
+  int pending_lo = s.nz, pending_hi = -1, pending_start = -1;
+  R pending_first = R(0);
+  auto add_pending = [&]() {
+    if (WIDE && pending_lo <= pending_hi) {
+      upper_materialize<R>(s, *u, upper, low, layers, row);
+      for (int k = pending_lo; k <= pending_hi; ++k)
+        upper[k - low] += float(k == pending_start ? pending_first : whole_crossing<R>(layers, row, k, p.c));
+      pending_lo = s.nz;
+      pending_hi = -1;
+    }
+  };
   auto flush_slide = [&]() {
     if (slide != 0.0) {
       n.walk += slide_axis<R>(p.i, p.rx, p.a, slide, s.xunit, s.nx) +
@@ -881,7 +938,22 @@ __device__ inline int walk(const Scene &s, const Controls &c, unsigned *failure,
       // layers (the direction does not change within a flight).
       slide += double(d);
       if (s.paths) {
-        history[p.k] += d;
+        if (!WIDE || p.k < low) {
+          history[p.k] += d;
+        } else if (event) {
+          add_pending();
+          upper_materialize<R>(s, *u, upper, low, layers, row);
+          upper[p.k - low] += float(d);
+        } else if (u->initial) {
+          u->descent = p.k;  // a whole crossing of the first flight from TOA
+        } else {
+          if (pending_lo > pending_hi) {
+            pending_start = p.k;
+            pending_first = d;
+          }
+          pending_lo = min(pending_lo, p.k);
+          pending_hi = max(pending_hi, p.k);
+        }
         h.path_total += d;
         layer_path += d;
       }
@@ -894,6 +966,8 @@ __device__ inline int walk(const Scene &s, const Controls &c, unsigned *failure,
       h.remains -= d * extinction;
       p.k += p.c >= R(0) ? 1 : -1;
       if (p.k >= s.nz) return kEscaped;
+      // Back down below the upper layers: this flight's crossings there are complete.
+      if (WIDE && p.k == low - 1 && p.c < R(0)) add_pending();
       if (p.k >= 0) {
         p.rz = LAYERS ? R(layers[row + p.k]) : R(s.zunit[p.k]);
         layer_path = R(0);
@@ -1448,11 +1522,16 @@ template <typename R> struct Flight {
 // the lanes' own values (read from the source lane). The source lanes run one after another:
 // the per-view constants are set up once, outside that loop. Record offsets within a bank are
 // 32-bit (session creation keeps them below 2^31 / 8, pixel indexes below 2^24): 24-bit
-// multiplies instead of 64-bit index arithmetic.
-template <typename R, bool V4, bool DOWN>
+// multiplies instead of 64-bit index arithmetic. WIDE (kWide): the LDS rows hold layers
+// 0 .. low - 1; a source lane's upper paths are its first flight's whole crossings from TOA
+// (layers from its descent index up) or its row in global memory (wave_upper, `stride` floats
+// per lane) once it has one (upper_word: descent | residual << 7).
+template <typename R, bool V4, bool DOWN, bool WIDE = false>
 __device__ inline void flush_records(const Scene &s, R *__restrict__ banks, int shard, uint64_t lanes, int views,
                                      const Record<R> &wave_records, const R *wave_history, int row, int open_layer,
-                                     uint32_t round_offset, const Flight<R> &flight, Counts &n) {
+                                     uint32_t round_offset, const Flight<R> &flight, Counts &n, int low = 1 << 30,
+                                     const float *wave_upper = nullptr, int stride = 0, int upper_word = 0,
+                                     const float *layers = nullptr, int table_row = 0) {
   const int lane = __lane_id();
   const int record = s.paths ? s.nz + s.nmat : s.nmat;
   const int stokes_at = s.paths ? s.nz : 0;
@@ -1462,6 +1541,9 @@ __device__ inline void flush_records(const Scene &s, R *__restrict__ banks, int 
   const uint32_t view_values = uint32_t(s.nx * s.ny) * uint32_t(s.record_stride);
   const R lane_layer = lane < s.nz - 1 ? R(s.zunit[lane]) : R(0);
   const R lane_thickness = lane < s.nz ? R(s.zunit[lane]) : R(0);
+  // WIDE: this lane's layer crossed whole by a first flight from TOA.
+  const bool upper_lane = WIDE && path_lane && lane >= low;
+  const R lane_descent = upper_lane ? whole_crossing<R>(layers, table_row, lane, R(s.launch_c)) : R(0);
   R geometry[kMaxViews], layer_length[kMaxViews];
   uint32_t lane_index[kMaxViews];
 #pragma unroll
@@ -1472,7 +1554,8 @@ __device__ inline void flush_records(const Scene &s, R *__restrict__ banks, int 
   }
   // The source lane's small words in one register: views (3 bits), layer + 1 (7), stale (1),
   // end layer + 1 (7); layers stay below 64.
-  const int word = (views & 7) | ((open_layer + 1) << 3) | (int(flight.stale) << 10) | ((flight.end_layer + 1) << 11);
+  const int word = (views & 7) | ((open_layer + 1) << 3) | (int(flight.stale) << 10) | ((flight.end_layer + 1) << 11) |
+                   (WIDE ? upper_word << 18 : 0);
   for (uint64_t mask = lanes; mask; mask &= mask - 1) {
     const int src = __ffsll(static_cast<long long>(mask)) - 1;
     const int packed = __builtin_amdgcn_readlane(word, src);
@@ -1481,14 +1564,20 @@ __device__ inline void flush_records(const Scene &s, R *__restrict__ banks, int 
     const uint32_t round = uint32_t(__builtin_amdgcn_readlane(int(round_offset), src));
     const Record<R> source = wave_records.at(src);
     R *bank = banks + s.bank_base + (size_t(round) * s.shards + shard) * size_t(s.bank_count);
-    R path = path_lane ? wave_history[src * row + lane] : R(0);
+    R path = path_lane && !upper_lane ? wave_history[src * row + lane] : R(0);
+    if (upper_lane) {
+      const int descent = (packed >> 18) & 127;
+      path = (packed >> 25) & 1 ? R(wave_upper[size_t(src) * stride + (lane - low)])
+                                : (lane >= descent ? lane_descent : R(0));
+    }
     if ((packed >> 10) & 1) {
       const int end = ((packed >> 11) & 127) - 1;
       const bool between = end > layer ? (lane > layer && lane < end) : (lane < layer && lane > end);
       const R crossed = between ? lane_thickness * read_lane(flight.inverse, src) : R(0);
       const R added = lane == end ? read_lane(flight.end_path, src) : crossed;
-      // (A layer first entered on this flight comes back as rounding around zero.)
-      if (path_lane) path = fmax(R(0), path - added);
+      // (A layer first entered on this flight comes back as rounding around zero.) WIDE: an
+      // escaping flight's upper crossings were never added.
+      if (path_lane && !(upper_lane && end >= s.nz)) path = fmax(R(0), path - added);
     }
     const bool at_layer = lane == layer, beyond = DOWN ? lane < layer : lane > layer;
 #pragma unroll
@@ -1681,8 +1770,12 @@ __global__ void __launch_bounds__(kBlock32)
                      uint32_t scene_count, int row, R *banks, unsigned *failure, unsigned *work,
                      unsigned long long *counters, float *records, uint32_t record_count,
                      monki_net::Net net, monki_net::EventLog log, Guidance guidance, uint32_t *splits,
-                     int split_stride) {
+                     int split_stride, int wide) {
   constexpr bool V4 = full_stokes(MODE);
+  // kWide: LDS layer rows of `wide` values (layers 0 .. wide - 1); splits / split_stride are the
+  // lanes' rows of upper-layer paths in global memory.
+  constexpr bool WIDE = MODE == kWide;
+  const int hrow = WIDE ? wide : row;
   constexpr bool BACK = backward(MODE);
   constexpr bool SCATS = decomposition(MODE);
   constexpr bool DOWN = downward(MODE);
@@ -1701,8 +1794,14 @@ __global__ void __launch_bounds__(kBlock32)
   extern __shared__ unsigned char lds_raw[];
   // Layer rows of odd length (the largest nz of the scenes, | 1): lanes reading one row, or
   // each its own row at one layer, touch distinct LDS banks.
-  R *history = reinterpret_cast<R *>(lds_raw) + threadIdx.x * row;
-  const R *wave_history = reinterpret_cast<R *>(lds_raw) + (threadIdx.x & ~63u) * row;
+  R *history = reinterpret_cast<R *>(lds_raw) + threadIdx.x * hrow;
+  const R *wave_history = reinterpret_cast<R *>(lds_raw) + (threadIdx.x & ~63u) * hrow;
+  float *const upper = WIDE ? reinterpret_cast<float *>(splits) + size_t(blockIdx.x * blockDim.x + threadIdx.x) * split_stride
+                            : nullptr;
+  const float *const wave_upper =
+      WIDE ? reinterpret_cast<const float *>(splits) + size_t(blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) * split_stride
+           : nullptr;
+  Upper u{0, false, false};
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
   // Scenes of a batch: waves start spread over them, each wave works on one scene at a time
@@ -1713,7 +1812,7 @@ __global__ void __launch_bounds__(kBlock32)
   const int block = int(blockDim.x);
   // After the layer rows: the block's pending local estimates (scattering decomposition) or
   // open records (every other mode), struct of arrays over the block's lanes.
-  R *const area = reinterpret_cast<R *>(lds_raw) + size_t(block) * row;
+  R *const area = reinterpret_cast<R *>(lds_raw) + size_t(block) * hrow;
   const Pending<R> wave_pending{area + (threadIdx.x & ~63u),
                                 reinterpret_cast<int *>(area + size_t(block) * pending_reals(V4)) +
                                     (threadIdx.x & ~63u),
@@ -1765,7 +1864,8 @@ __global__ void __launch_bounds__(kBlock32)
       if (!active) {
         id = base + uint32_t(__popcll(idle & __lanemask_lt()));
         if (id < c.count) {
-          launch_photon<R>(s, c, id, history, h);
+          launch_photon<R>(s, c, id, history, h, hrow);
+          if (WIDE) u = Upper{s.nz, false, true};
           trace = Trace<R>{R(0), R(0), R(0), R(1), R(0), R(0), R(0), R(0)};
           ++n.photons;
           active = true;
@@ -1797,7 +1897,10 @@ __global__ void __launch_bounds__(kBlock32)
     } else {
       bool surface = false;
       R layer_path = R(0);
-      if (active) status = walk<R, LAYERS>(s, c, failure, history, h, n, layers, row, surface, layer_path);
+      if (active) {
+        status = walk<R, LAYERS, WIDE>(s, c, failure, history, h, n, layers, row, surface, layer_path, &u, upper, hrow);
+        if (WIDE) u.initial = false;
+      }
       // A flight that left the layer of the open records changed the layer paths the records
       // hold outside that layer: their write-out takes this flight's part off again.
       flight = Flight<R>{active && open_layer >= 0 && h.p.k != open_layer, h.p.k, layer_path,
@@ -1810,7 +1913,7 @@ __global__ void __launch_bounds__(kBlock32)
         // All views share one azimuth: cos and sin of the photon-view azimuth difference, once.
         delta = h.p.phi - R(s.phi);
         sin_cos(delta, &sin_delta, &cos_delta);
-        path = history[h.p.k];
+        path = WIDE && h.p.k >= hrow ? upper_path<R>(s, u, upper, hrow, layers, row, h.p.k) : history[h.p.k];
       }
       // The local estimates toward every view, lane by lane (the detector walks of one lane run
       // back to back, so the wave waits once for its longest lane).
@@ -1855,8 +1958,9 @@ __global__ void __launch_bounds__(kBlock32)
       if (flushing) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        flush_records<R, V4, DOWN>(s, banks, shard, flushing, flushed, wave_records, wave_history, row, open_layer,
-                                   h.round_offset, flight, n);
+        flush_records<R, V4, DOWN, WIDE>(s, banks, shard, flushing, flushed, wave_records, wave_history, hrow,
+                                         open_layer, h.round_offset, flight, n, hrow, wave_upper, split_stride,
+                                         u.descent | (int(u.residual) << 7), layers, row);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
       }
@@ -1972,8 +2076,9 @@ __global__ void __launch_bounds__(kBlock32)
       if (ending) {
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
-        flush_records<R, V4, DOWN>(s, banks, shard, ending, (1 << s.views) - 1, wave_records, wave_history, row,
-                                   open_layer, h.round_offset, flight, n);
+        flush_records<R, V4, DOWN, WIDE>(s, banks, shard, ending, (1 << s.views) - 1, wave_records, wave_history,
+                                         hrow, open_layer, h.round_offset, flight, n, hrow, wave_upper, split_stride,
+                                         u.descent | (int(u.residual) << 7), layers, row);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
       }
@@ -2204,6 +2309,12 @@ struct Session {
   int steer = kPlain;  // batch rounds: kPlain, kDirected or kGuided
   uint32_t *splits = nullptr;  // weight-window copies of the steered modes, per lane
   size_t split_words = 0;
+  // kWide: LDS layer rows of `wide` values (0: every plain launch keeps whole rows in LDS) and
+  // each lane's row of upper-layer paths in global memory (`upper_stride` floats).
+  int wide = 0;
+  int upper_stride = 0;
+  float *upper = nullptr;
+  uint32_t upper_blocks = 0;  // blocks whose lanes have a global row
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   // Rounds accumulated on the device (batch sessions with layer paths): per-scene sums and
@@ -2230,6 +2341,7 @@ void release(Session *s) {
   if (!s) return;
   if (s->device >= 0) (void)hipSetDevice(s->device);
   if (s->graph_exec) (void)hipGraphExecDestroy(s->graph_exec);
+  if (s->upper) (void)hipFree(s->upper);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   if (s->accumulate_exec) (void)hipGraphExecDestroy(s->accumulate_exec);
   if (s->accumulate_graph) (void)hipGraphDestroy(s->accumulate_graph);
@@ -2333,6 +2445,17 @@ bool parse_tables(const float *t, size_t count, const float *base, Scene &s, cha
       return false;
     }
     *span.target = base + offset;
+  }
+  // Every layer from uniform_top up is horizontally uniform (the detector walk's prefix and the
+  // split layer rows of tall scenes rely on it).
+  {
+    const float *flags = t + (s.uniform - base);
+    for (int k = s.uniform_top; k < s.nz; ++k) {
+      if (flags[k] == 0.0f) {
+        set_error(error, size, "A layer above the uniform top is not uniform");
+        return false;
+      }
+    }
   }
   // Factored extinction (words 64-65): one tabulated species, both parts in range, and on the
   // host pass every cell reproduces the extinction and fraction tables bit for bit.
@@ -2468,12 +2591,16 @@ bool enqueue(Session *h, uint32_t rounds, uint32_t grid, int block, size_t lds, 
   }
   launch_fill(h->failure, 1 + scene_count, h->stream);
   R *banks = static_cast<R *>(h->banks);
+  // A kWide grid never exceeds the blocks whose lanes have a row in global memory (persistent
+  // blocks loop over the photons, so fewer blocks only take longer).
+  if (MODE == kWide && grid > h->upper_blocks) grid = h->upper_blocks;
   if (grid) {
     hipLaunchKernelGGL((transport_kernel<R, TRACE, MODE>), dim3(grid), dim3(block), lds, h->stream,
                        h->d_scenes, h->d_controls, scene_count, h->row, banks, h->failure, h->work,
                        h->counters, TRACE ? h->records : nullptr, uint32_t(TRACE ? record_count : 0), net, log,
-                       MODE == kGuided || MODE == kDirected ? h->guidance : unsteered(), h->splits,
-                       kSplitWords + h->row);
+                       MODE == kGuided || MODE == kDirected ? h->guidance : unsteered(),
+                       MODE == kWide ? reinterpret_cast<uint32_t *>(h->upper) : h->splits,
+                       MODE == kWide ? h->upper_stride : kSplitWords + h->row, MODE == kWide ? h->wide : 0);
   }
   if (!check(hipGetLastError(), error, error_size, "transport launch")) return false;
   for (uint32_t i = 0; i < scene_count; ++i) {
@@ -2516,6 +2643,38 @@ size_t lds_bytes_for(int row, int block, bool v4) {
 template <typename R>
 size_t lds_bytes(const Session *h, int block, bool v4 = false) {
   return lds_bytes_for<R>(h->row, block, v4);
+}
+
+// kWide: history rows of `wide` values, the per-wave layer tables still `row` long.
+size_t lds_bytes_wide(int wide, int row, int block) {
+  return (size_t(wide) * sizeof(float) + pending_bytes<float>(false)) * block + size_t(block / 64) * 4 * row * sizeof(float);
+}
+
+// LDS of a launch in mode MODE.
+template <typename R, int MODE>
+size_t launch_lds(const Session *h, int block) {
+  return MODE == kWide ? lds_bytes_wide(h->wide, h->row, block) : lds_bytes<R>(h, block, full_stokes(MODE));
+}
+
+// Plain float32 launches of scenes whose whole layer rows leave fewer than three waves per SIMD
+// (the budget of the kernel's registers) run as kWide when that restores three waves: the
+// longest odd LDS row that fits, at least the highest varying layer of every scene (so that the
+// cell walks stay in LDS). Forward TOA scenes of nmat 1 or 3 only. Returns the row, or 0.
+int wide_row(const Session *h, int block, size_t lds_per_cu) {
+  int uniform_top = 0;
+  for (const Scene &s : h->scenes) {
+    if (s.nmat == 4 || s.backward || s.scats || s.detector_surface) return 0;
+    uniform_top = s.uniform_top > uniform_top ? s.uniform_top : uniform_top;
+  }
+  const size_t waves = 12;  // three per SIMD
+  const size_t blocks = (waves * 64 + block - 1) / block;
+  const size_t budget = lds_per_cu / blocks;
+  if (lds_bytes_for<float>(h->row, block, false) <= budget) return 0;
+  for (int wide = h->row - 2; wide >= 1; wide -= 2) {
+    if (lds_bytes_wide(wide, h->row, block) > budget) continue;
+    return wide >= uniform_top && wide >= 1 ? wide : 0;
+  }
+  return 0;
 }
 
 bool finish(Session *h, char *error, size_t error_size) {
@@ -2585,6 +2744,9 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
       return launch<R, TRACE, kStokes4>(h, controls, out, records, record_count, error, error_size, proposal);
     }
   }
+  if constexpr (MODE == kPlain && sizeof(R) == 4) {
+    if (h->wide) return launch<R, TRACE, kWide>(h, controls, out, records, record_count, error, error_size, proposal);
+  }
   Controls c{controls[0], controls[1], controls[2], controls[3],
              controls[4], controls[5], controls[6], controls[7]};
   const Scene &s = h->scenes[0];
@@ -2630,7 +2792,7 @@ int launch(Session *h, const uint32_t *controls, double *out, float *records, si
     }
   }
   const int block = block_size<R>(h);
-  const size_t lds = lds_bytes<R>(h, block, full_stokes(MODE));
+  const size_t lds = launch_lds<R, MODE>(h, block);
   const uint32_t grid = persistent_grid<R, TRACE, MODE>(h, c.count, block, lds);
   if ((MODE == kGuided || MODE == kDirected) && !reserve_splits(h, grid, block, error, error_size)) return 1;
   if (MODE == kLog) {
@@ -2802,6 +2964,20 @@ Session *create_session(uint32_t device, const float *tables, const size_t *offs
   else
     snprintf(h->name, sizeof(h->name), "%s, %d CUs", prop.gcnArchName, prop.multiProcessorCount);
   h->compute_units = prop.multiProcessorCount;
+  // Scenes too tall for whole LDS rows at three waves per SIMD: the kWide split and the lanes'
+  // global rows, enough for every block the LDS lets a CU hold.
+  const size_t lds_per_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
+  h->wide = wide_row(h, h->block, lds_per_cu);
+  if (h->wide) {
+    h->upper_stride = nz - h->wide;
+    const size_t resident = lds_per_cu / lds_bytes_wide(h->wide, h->row, h->block);
+    h->upper_blocks = uint32_t(size_t(h->compute_units > 0 ? h->compute_units : 1) * (resident ? resident : 1));
+    const size_t floats = size_t(h->upper_blocks) * size_t(h->block) * size_t(h->upper_stride);
+    if (!check(hipMalloc(&h->upper, floats * sizeof(float)), error, error_size, "allocate upper-layer rows")) {
+      release(h);
+      return nullptr;
+    }
+  }
   return h;
 }
 
@@ -2919,7 +3095,8 @@ int batch_round(Session *h, uint32_t round, uint32_t seed, uint32_t max_steps, c
              error, error_size, "upload controls"))
     return 1;
   const int block = block_size<float>(h);
-  const size_t lds = lds_bytes<float>(h, block);
+  const bool wide = h->steer == kPlain && h->wide;
+  const size_t lds = wide ? lds_bytes_wide(h->wide, h->row, block) : lds_bytes<float>(h, block);
   if (h->steer == kGuided && (!h->proposal || monki_net::proposal_device(h->proposal) != h->device)) {
     set_error(error, error_size, "A proposal on the session's device is required");
     return 1;
@@ -2928,6 +3105,7 @@ int batch_round(Session *h, uint32_t round, uint32_t seed, uint32_t max_steps, c
       !total                   ? 0u
       : h->steer == kGuided   ? persistent_grid<float, false, kGuided>(h, total, block, lds)
       : h->steer == kDirected ? persistent_grid<float, false, kDirected>(h, total, block, lds)
+      : wide                  ? persistent_grid<float, false, kWide>(h, total, block, lds)
                               : persistent_grid<float, false, kPlain>(h, total, block, lds);
   // Stacks for a full-occupancy grid: a captured round keeps their address.
   if (h->steer != kPlain &&
@@ -2947,6 +3125,7 @@ int batch_round(Session *h, uint32_t round, uint32_t seed, uint32_t max_steps, c
         h->steer == kGuided   ? enqueue<float, false, kGuided>(h, 1, grid, block, lds, 0, net, log, error, error_size)
         : h->steer == kDirected ? enqueue<float, false, kDirected>(h, 1, grid, block, lds, 0, net, log, error,
                                                                    error_size)
+        : wide                  ? enqueue<float, false, kWide>(h, 1, grid, block, lds, 0, net, log, error, error_size)
                                 : enqueue<float, false, kPlain>(h, 1, grid, block, lds, 0, net, log, error, error_size);
     if (queued && accumulate) enqueue_accumulate(h);
     return queued;
@@ -3077,7 +3256,7 @@ bool static_enqueue(const StaticRound &r, char *error, size_t error_size) {
                        1u, row, r.banks, r.flags, r.flags + 1, r.counters, nullptr, 0u,
                        monki_net::Net{nullptr, nullptr, nullptr},
                        monki_net::EventLog{nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, 0, 0},
-                       unsteered(), nullptr, 0);
+                       unsteered(), nullptr, 0, 0);
   }
   const size_t summed = size_t(s.bank_count) * r.rounds;
   hipLaunchKernelGGL(reduce_shards<float>, dim3(unsigned((summed + 255) / 256)), dim3(256), 0, 0, r.banks, r.reduced,
@@ -3445,6 +3624,12 @@ double monki_hip_gpu_seconds(void *handle) {
 
 double monki_hip_kernel_seconds(void *handle) {
   return handle ? static_cast<Session *>(handle)->kernel_seconds : 0.0;
+}
+
+int monki_hip_layer_rows(void *handle) {
+  if (!handle) return 0;
+  const Session *h = static_cast<Session *>(handle);
+  return h->wide ? h->wide : h->row;
 }
 
 const char *monki_hip_device_name(void *handle) {

@@ -105,6 +105,8 @@ def load_library(path: Path = LIBRARY):
     native.monki_hip_gpu_seconds.restype = ctypes.c_double
     native.monki_hip_kernel_seconds.argtypes = [ctypes.c_void_p]
     native.monki_hip_kernel_seconds.restype = ctypes.c_double
+    native.monki_hip_layer_rows.argtypes = [ctypes.c_void_p]
+    native.monki_hip_layer_rows.restype = ctypes.c_int
     native.monki_hip_device_name.argtypes = [ctypes.c_void_p]
     native.monki_hip_device_name.restype = ctypes.c_char_p
     native.monki_hip_close.argtypes = [ctypes.c_void_p]

@@ -293,6 +293,47 @@ def test_spectral_field_matches_oracle(oracle, name, precision):
     check_tallies(name, gpu_tallies, cpu_tallies, gpu, cpu, rtol=2e-2 if precision == 32 else 1e-4)
 
 
+@pytest.mark.parametrize("name", ["spectral-300", "spectral-500"])
+def test_split_layer_rows_equal_whole_rows(name):
+    """59 layers + vacuum: the session keeps the lower layers' paths in LDS and the upper
+    (uniform) layers' in global memory, for three waves per SIMD instead of two; its round
+    equals the round of the static entry point, whose launch keeps whole rows in LDS, on the
+    same Philox streams (300 nm: the most Rayleigh events in the upper layers)."""
+
+    from monki.execution.hip import DOUBLE_POINTER, FLOAT_POINTER, load_library
+    from monki.execution.hip_tables import pack_tables, tally_count
+
+    native = load_library()
+    scene = SCENES[name]
+    packed, shape = pack_tables(scene, shards=4)
+    count = tally_count(*shape, True, False)
+    per_round = 400 * scene.nx * scene.ny
+    controls = (ctypes.c_uint32 * 8)(per_round, 0, 1, 23274, 100_000, 0, per_round, 1)
+    error = ctypes.create_string_buffer(4096)
+    whole = np.zeros(count)
+    status = native.monki_hip_run_static(0, packed.ctypes.data_as(FLOAT_POINTER), packed.size, controls,
+                                         whole.ctypes.data_as(DOUBLE_POINTER), error, len(error))  # fmt: skip
+    assert status == 0, error.value.decode()
+    split = np.zeros(count)
+    handle = native.monki_hip_create(0, packed.ctypes.data_as(FLOAT_POINTER), packed.size, count * 4, 1, error,
+                                     len(error))  # fmt: skip
+    assert handle, error.value.decode()
+
+    try:
+        rows = native.monki_hip_layer_rows(handle)
+        print(f"{name}: nz {scene.nz}, {rows} layer paths per lane in LDS", flush=True)
+        assert rows < (scene.nz | 1)
+        status = native.monki_hip_run(handle, controls, split.ctypes.data_as(DOUBLE_POINTER), error, len(error))
+        assert status == 0, error.value.decode()
+    finally:
+        native.monki_hip_close(handle)
+
+    # Identical histories; float32 bank sums in another order, and an escaping flight's upper
+    # crossings never added instead of added and taken off again.
+    np.testing.assert_allclose(split, whole, rtol=2e-5, atol=2e-6 * np.abs(whole).max())
+    assert np.abs(whole).max() > 0
+
+
 def test_circular_polarization_matches_oracle(oracle):
     """The V tallies (nmat 4) of the float64 build follow the oracle's on the same streams.
 

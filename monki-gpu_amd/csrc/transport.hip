@@ -846,9 +846,13 @@ __device__ inline int opposite_grid_pixel(const Scene &s, const Photon<R> &p, R 
 // whole (descent nz: none yet) and the other upper layers hold nothing; once set, the lane's
 // row in global memory holds every upper layer's path (layer k at k - low). `initial`: no
 // event yet, the current flight is the first one.
+// One word: descent (bits 0-6) | residual << 7 | initial << 8.
 struct Upper {
-  int descent;
-  bool residual, initial;
+  uint32_t word;
+  __device__ int descent() const { return int(word & 127u); }
+  __device__ bool residual() const { return (word >> 7) & 1u; }
+  __device__ bool initial() const { return (word >> 8) & 1u; }
+  __device__ void set_descent(int k) { word = (word & ~127u) | uint32_t(k); }
 };
 
 // A whole crossing of layer k at direction cosine c, as the free-path walk measures it.
@@ -861,20 +865,20 @@ __device__ inline R whole_crossing(const float *layers, int row, int k, R c) {
 template <typename R>
 __device__ inline void upper_materialize(const Scene &s, Upper &u, float *upper, int low, const float *layers,
                                          int row) {
-  if (u.residual) return;
+  if (u.residual()) return;
   const R c0 = R(s.launch_c);
+  const int descent = u.descent();
   for (int k = low; k < s.nz; ++k)
-    upper[k - low] = k >= u.descent ? float(whole_crossing<R>(layers, row, k, c0)) : 0.0f;
-  u.residual = true;
-  u.descent = s.nz;
+    upper[k - low] = k >= descent ? float(whole_crossing<R>(layers, row, k, c0)) : 0.0f;
+  u.word = (u.word & 256u) | 128u | uint32_t(s.nz);
 }
 
 // The lane's path in upper layer k.
 template <typename R>
 __device__ inline R upper_path(const Scene &s, const Upper &u, const float *upper, int low, const float *layers,
                                int row, int k) {
-  if (u.residual) return R(upper[k - low]);
-  return k >= u.descent ? whole_crossing<R>(layers, row, k, R(s.launch_c)) : R(0);
+  if (u.residual()) return R(upper[k - low]);
+  return k >= u.descent() ? whole_crossing<R>(layers, row, k, R(s.launch_c)) : R(0);
 }
 
 // Advance one photon to its next collision or surface hit (mc.f90:476-677). Returns kRunning
@@ -944,8 +948,8 @@ __device__ inline int walk(const Scene &s, const Controls &c, unsigned *failure,
           add_pending();
           upper_materialize<R>(s, *u, upper, low, layers, row);
           upper[p.k - low] += float(d);
-        } else if (u->initial) {
-          u->descent = p.k;  // a whole crossing of the first flight from TOA
+        } else if (u->initial()) {
+          u->set_descent(p.k);  // a whole crossing of the first flight from TOA
         } else {
           if (pending_lo > pending_hi) {
             pending_start = p.k;
@@ -1796,12 +1800,13 @@ __global__ void __launch_bounds__(kBlock32)
   // each its own row at one layer, touch distinct LDS banks.
   R *history = reinterpret_cast<R *>(lds_raw) + threadIdx.x * hrow;
   const R *wave_history = reinterpret_cast<R *>(lds_raw) + (threadIdx.x & ~63u) * hrow;
-  float *const upper = WIDE ? reinterpret_cast<float *>(splits) + size_t(blockIdx.x * blockDim.x + threadIdx.x) * split_stride
-                            : nullptr;
+  // (The wave's first lane index is wave-uniform: the row base stays in scalar registers.)
+  const uint32_t wave_lane0 = __builtin_amdgcn_readfirstlane(blockIdx.x * blockDim.x + (threadIdx.x & ~63u));
   const float *const wave_upper =
-      WIDE ? reinterpret_cast<const float *>(splits) + size_t(blockIdx.x * blockDim.x + (threadIdx.x & ~63u)) * split_stride
-           : nullptr;
-  Upper u{0, false, false};
+      WIDE ? reinterpret_cast<const float *>(splits) + size_t(wave_lane0) * split_stride : nullptr;
+  float *const upper =
+      WIDE ? reinterpret_cast<float *>(splits) + (size_t(wave_lane0) + __lane_id()) * split_stride : nullptr;
+  Upper u{0u};
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t waves = (gridDim.x * blockDim.x) >> 6;
   // Scenes of a batch: waves start spread over them, each wave works on one scene at a time
@@ -1865,7 +1870,7 @@ __global__ void __launch_bounds__(kBlock32)
         id = base + uint32_t(__popcll(idle & __lanemask_lt()));
         if (id < c.count) {
           launch_photon<R>(s, c, id, history, h, hrow);
-          if (WIDE) u = Upper{s.nz, false, true};
+          if (WIDE) u = Upper{uint32_t(s.nz) | 256u};
           trace = Trace<R>{R(0), R(0), R(0), R(1), R(0), R(0), R(0), R(0)};
           ++n.photons;
           active = true;
@@ -1899,7 +1904,7 @@ __global__ void __launch_bounds__(kBlock32)
       R layer_path = R(0);
       if (active) {
         status = walk<R, LAYERS, WIDE>(s, c, failure, history, h, n, layers, row, surface, layer_path, &u, upper, hrow);
-        if (WIDE) u.initial = false;
+        if (WIDE) u.word &= ~256u;
       }
       // A flight that left the layer of the open records changed the layer paths the records
       // hold outside that layer: their write-out takes this flight's part off again.
@@ -1960,7 +1965,7 @@ __global__ void __launch_bounds__(kBlock32)
         __builtin_amdgcn_wave_barrier();
         flush_records<R, V4, DOWN, WIDE>(s, banks, shard, flushing, flushed, wave_records, wave_history, hrow,
                                          open_layer, h.round_offset, flight, n, hrow, wave_upper, split_stride,
-                                         u.descent | (int(u.residual) << 7), layers, row);
+                                         int(u.word & 255u), layers, row);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
       }
@@ -2078,7 +2083,7 @@ __global__ void __launch_bounds__(kBlock32)
         __builtin_amdgcn_wave_barrier();
         flush_records<R, V4, DOWN, WIDE>(s, banks, shard, ending, (1 << s.views) - 1, wave_records, wave_history,
                                          hrow, open_layer, h.round_offset, flight, n, hrow, wave_upper, split_stride,
-                                         u.descent | (int(u.residual) << 7), layers, row);
+                                         int(u.word & 255u), layers, row);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
       }

@@ -585,6 +585,36 @@ __device__ inline R advance(const Scene &s, Photon<R> &p, const float *layers = 
   return d;
 }
 
+// advance with the reciprocal direction magnitudes given (a ray whose direction does not change).
+template <typename R, bool LAYERS = false>
+__device__ inline R advance_by(const Scene &s, Photon<R> &p, R ia, R ib, R ic, const float *layers = nullptr,
+                               int row = 0) {
+  const R aa = fabs(p.a), ab = fabs(p.b), ac = fabs(p.c);
+  const R dx = p.rx * ia;
+  const R dy = p.ry * ib;
+  const R dz = p.rz * ic;
+  const R d = fmin(dx, fmin(dy, dz));
+  if (dx == d) {
+    p.i += p.a >= R(0) ? 1 : -1;
+    p.i = p.i < 0 ? s.nx - 1 : (p.i >= s.nx ? 0 : p.i);
+    p.rx = R(s.xunit);
+    p.ry = fmax(R(0), p.ry - ab * d);
+    p.rz = fmax(R(0), p.rz - ac * d);
+  } else if (dy == d) {
+    p.j += p.b >= R(0) ? 1 : -1;
+    p.j = p.j < 0 ? s.ny - 1 : (p.j >= s.ny ? 0 : p.j);
+    p.rx = fmax(R(0), p.rx - aa * d);
+    p.ry = R(s.yunit);
+    p.rz = fmax(R(0), p.rz - ac * d);
+  } else {
+    p.k += p.c >= R(0) ? 1 : -1;
+    p.rx = fmax(R(0), p.rx - aa * d);
+    p.ry = fmax(R(0), p.ry - ab * d);
+    if (p.k >= 0 && p.k < s.nz) p.rz = LAYERS ? R(layers[row + p.k]) : R(s.zunit[p.k]);
+  }
+  return d;
+}
+
 // x reduced to [0, span): floor-based (fmod's exact remainder loop is not needed at the
 // quotients of a periodic grid, and costs far more).
 template <typename T> __device__ inline T periodic(T x, T span) {
@@ -651,6 +681,8 @@ __device__ inline DetectorHit detector_walk(const Scene &s, const Photon<R> &eve
   // parts stay separate until the add for the same reason.
   R held_d = R(0);
   float held_part = 0.0f, held_gas = 0.0f;
+  const R ia = R(1) / fmax(fabs(ray.a), R(1e-16)), ib = R(1) / fmax(fabs(ray.b), R(1e-16)),
+          ic = R(1) / fmax(fabs(ray.c), R(1e-16));
   while (ray.k >= 0 && ray.k < s.nz - 1) {
     if (++steps > max_steps) return DetectorHit{0, 0, false};
     if (!DOWN && ray.k >= s.uniform_top) {
@@ -675,7 +707,7 @@ __device__ inline DetectorHit detector_walk(const Scene &s, const Photon<R> &eve
     const int cell = cell_index<R>(s, ray.i, ray.j, layer);
     const float part = s.species_beta ? s.species_beta[cell] : s.beta[cell];
     const float gas = s.species_beta ? s.gas_beta[layer] : 0.0f;
-    const R d = advance<R, LAYERS>(s, ray, layers, row);
+    const R d = advance_by<R, LAYERS>(s, ray, ia, ib, ic, layers, row);
     tau += held_d * R(held_part + held_gas);
     held_d = d;
     held_part = part;

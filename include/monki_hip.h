@@ -206,6 +206,11 @@ void monki_hip_proposal_close(void *proposal);
 int monki_hip_attach_proposal(void *handle, void *proposal, float alpha, uint32_t species_mask,
                               uint32_t max_order, char *error, size_t error_size);
 
+/* Guided tracer: the scattering orders (1 .. orders) at which the network's mixture is part of
+ * the proposal; at higher orders the view lobes alone (the network does not run). The default
+ * is every order. No reference counterpart (the reference has no learned proposal). */
+int monki_hip_set_network_orders(void *handle, uint32_t orders, char *error, size_t error_size);
+
 /* monki_hip_run with the attached proposal: directions from the phase function with
  * probability alpha, else from the mixture q; weights times p / (alpha p + (1 - alpha) q). */
 int monki_hip_run_guided(void *handle, const uint32_t *controls, double *out, char *error,

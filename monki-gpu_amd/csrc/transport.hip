@@ -100,10 +100,15 @@ struct Guidance {
   float lobe_fraction;
   int surface;
   float split_above, roulette_below, roulette_weight;
+  // Guided mode: scattering orders up to which the network's mixture takes part in the proposal
+  // (beyond them the view lobes alone, and the network does not run).
+  uint32_t network_orders;
 };
 
 // Unsteered runs.
-__host__ __device__ constexpr Guidance unsteered() { return Guidance{1.0f, 0u, 0u, 0.0f, -1, 0.0f, 0, 0.0f, 0.0f, 0.0f}; }
+__host__ __device__ constexpr Guidance unsteered() {
+  return Guidance{1.0f, 0u, 0u, 0.0f, -1, 0.0f, 0, 0.0f, 0.0f, 0.0f, 0xFFFFFFFFu};
+}
 
 struct SpeciesTables {
   int lut_size;
@@ -1404,7 +1409,10 @@ __device__ int scatter_photon(const Scene &s, unsigned *failure, History<R> &h, 
     R cosine, one_minus, one_plus;
     int interval = -1;  // table interval of the sampled cosine (phase-function draws)
     bool from_phase = true;
-    const bool guide = STEER && ((guidance.species_mask >> (h.species + 1)) & 1u) && h.order <= guidance.max_order;
+    // Guided mode beyond the network's orders: the lobes alone, or no steering when the proposal
+    // has no lobes (lobe_fraction 0).
+    const bool guide = STEER && ((guidance.species_mask >> (h.species + 1)) & 1u) && h.order <= guidance.max_order &&
+                       (MODE != kGuided || h.order <= guidance.network_orders || guidance.lobe_fraction > 0.0f);
     if (guide) from_phase = uniform(h.rng) < alpha;
     if (from_phase) {
       R dphi;
@@ -2049,14 +2057,16 @@ __global__ void __launch_bounds__(kBlock32)
     const bool scatters = active && status == kRunning && !h.surface;
     float features[monki_net::kInputs];
     monki_net::Mixture mixture;
+    // The network runs when some lane's event is guided (kind and order within the guidance,
+    // order within the network's orders).
+    const bool guided_event = MODE == kGuided && scatters &&
+                              ((guidance.species_mask >> (h.species + 1)) & 1u) && h.order <= guidance.max_order &&
+                              h.order <= guidance.network_orders;
     if (MODE == kLog || MODE == kGuided) {
 #pragma unroll
       for (int k = 0; k < monki_net::kInputs; ++k) features[k] = 0.0f;
-      if (scatters) event_features<R>(s, h, features);
+      if (MODE == kLog ? scatters : guided_event) event_features<R>(s, h, features);
     }
-    // The network runs when some lane's event is guided (kind and order within the guidance).
-    const bool guided_event = MODE == kGuided && scatters &&
-                              ((guidance.species_mask >> (h.species + 1)) & 1u) && h.order <= guidance.max_order;
     const unsigned long long guided_lanes =
         MODE == kGuided && guidance.lobe_fraction < 1.0f ? __ballot(guided_event) : 0ull;
     if (MODE == kGuided && guided_lanes) {
@@ -2070,7 +2080,8 @@ __global__ void __launch_bounds__(kBlock32)
       if (active && status == kRunning)
         status = scatter_photon<R, TRACE, MODE>(s, failure, h, n, trace, &lobes, &lobes, guidance);
     } else if (MODE == kGuided) {
-      const GuidedProposal guided{&mixture, lobes, guidance.lobe_fraction};
+      const GuidedProposal guided{&mixture, lobes,
+                                  h.order <= guidance.network_orders ? guidance.lobe_fraction : 1.0f};
       if (active && status == kRunning)
         status = scatter_photon<R, TRACE, MODE>(s, failure, h, n, trace, &guided, &lobes, guidance);
     } else if (active && status == kRunning) {
@@ -3554,6 +3565,17 @@ int monki_hip_attach_proposal(void *handle, void *proposal, float alpha, uint32_
   h->guidance.species_mask = species_mask;
   h->guidance.max_order = max_order;
   h->steer = proposal ? kGuided : kPlain;
+  drop_graphs(h);
+  return 0;
+}
+
+int monki_hip_set_network_orders(void *handle, uint32_t orders, char *error, size_t error_size) {
+  Session *h = static_cast<Session *>(handle);
+  if (!h || !orders) {
+    set_error(error, error_size, "Invalid session or network orders (at least 1)");
+    return 1;
+  }
+  h->guidance.network_orders = orders;
   drop_graphs(h);
   return 0;
 }

@@ -67,7 +67,10 @@ class ProposalSettings:
     shaped by the sharpest tabulated phase function + (1 - ``lobe_fraction``) x the network's
     vMF mixture. Steered events: atmospheric events of ``guided_kinds`` ("gas" and/or tabulated
     species indices; None for every kind) up to ``max_order``, and Lambertian reflections when
-    ``surface``. Weight window: photons heavier than ``split_above`` are split, lighter than
+    ``surface``. The network's mixture is part of the proposal at the first ``network_orders``
+    scattering orders only; beyond them the lobes alone (no steering when ``lobe_fraction`` is
+    0), and the network does not run: on the box cloud the mixture lowers the variance at the
+    first two orders and raises it at higher ones. Weight window: photons heavier than ``split_above`` are split, lighter than
     ``roulette_below`` play Russian roulette for ``roulette_weight`` (``split_above`` 0 turns
     the window off, which is refused for tabulated species beyond order 4).
 
@@ -93,6 +96,7 @@ class ProposalSettings:
     roulette_below: float = 0.005
     roulette_weight: float = 0.01
     path_target: float = 0.0
+    network_orders: int = 2
 
     def species_mask(self) -> int:
         if self.guided_kinds is None:
@@ -113,6 +117,7 @@ class ProposalSettings:
             ("batch", self.batch, 64, 1 << 24),
             ("seed", self.seed, 0, 2**32 - 1),
             ("max_order", self.max_order, 1, 2**32 - 1),
+            ("network_orders", self.network_orders, 1, 2**32 - 1),
         ):
             if type(value) is not int or not low <= value <= high:
                 raise ValueError(f"{name} must be an integer in [{low}, {high}]")
@@ -153,6 +158,7 @@ def declare(native) -> None:
     native.monki_hip_proposal_close.argtypes = [ctypes.c_void_p]
     native.monki_hip_proposal_close.restype = None
     native.monki_hip_attach_proposal.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_float, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]
+    native.monki_hip_set_network_orders.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_char_p, ctypes.c_size_t]
     native.monki_hip_run_guided.argtypes = [ctypes.c_void_p, _U, _D, ctypes.c_char_p, ctypes.c_size_t]
     native.monki_hip_trace_guided.argtypes = [ctypes.c_void_p, _U, _D, _F, ctypes.c_size_t, ctypes.c_char_p, ctypes.c_size_t]
     native.monki_hip_run_logged.argtypes = [ctypes.c_void_p, _U, _D, ctypes.c_void_p, ctypes.c_char_p, ctypes.c_size_t]
@@ -169,7 +175,7 @@ def declare(native) -> None:
         "monki_hip_proposal_set_path_target",
         "monki_hip_proposal_gradients", "monki_hip_proposal_train", "monki_hip_proposal_forward",
         "monki_hip_attach_proposal", "monki_hip_run_guided", "monki_hip_trace_guided", "monki_hip_run_logged",
-        "monki_hip_set_directed", "monki_hip_run_directed", "monki_hip_set_steering",
+        "monki_hip_set_directed", "monki_hip_run_directed", "monki_hip_set_steering", "monki_hip_set_network_orders",
     ):  # fmt: skip
         getattr(native, name).restype = ctypes.c_int
 
@@ -243,6 +249,12 @@ def apply_steering(native, handle, settings, lobe_species: int, lobe_fraction: f
         handle, lobe_species, lobe_fraction, int(settings.surface), settings.split_above, settings.roulette_below,
         settings.roulette_weight, error, len(error),
     ):  # fmt: skip
+        raise RuntimeError(error.value.decode())
+
+    # The guided tracer's network takes part in the proposal up to its orders.
+    orders = getattr(settings, "network_orders", None)
+
+    if orders is not None and native.monki_hip_set_network_orders(handle, min(orders, 2**32 - 1), error, len(error)):
         raise RuntimeError(error.value.decode())
 
 

@@ -232,12 +232,16 @@ def plain_reference(directory, scene, rounds, ppc):
 
 
 GUIDANCE = {
-    # Every kind and order, the surface too, view lobes + network, weight window on (defaults).
+    # Every kind and order, the surface too, view lobes + network at the first two orders (lobes
+    # alone beyond), weight window on (defaults).
     "defaults": dict(),
     # Network only (no view lobes), every kind and order, weight window on.
-    "network-only": dict(lobe_fraction=0.0, surface=False),
+    "network-only": dict(lobe_fraction=0.0, surface=False, network_orders=2**32 - 1),
     # Rayleigh/gas events only, every order, network only.
-    "gas-all-orders": dict(alpha=0.3, guided_kinds=("gas",), lobe_fraction=0.0, surface=False, split_above=0.0),
+    "gas-all-orders": dict(alpha=0.3, guided_kinds=("gas",), lobe_fraction=0.0, surface=False, split_above=0.0,
+                           network_orders=2**32 - 1),
+    # Network only at the first two orders, no steering beyond them.
+    "network-two-orders": dict(lobe_fraction=0.0, surface=False),
 }
 
 

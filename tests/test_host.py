@@ -612,6 +612,21 @@ def test_unbounded_steering_of_tabulated_species_needs_the_weight_window(setting
     kind()
 
 
+def test_network_orders_setting():
+    """network_orders: a positive integer, 2 by default (the orders at which the network's
+    mixture is part of the guided proposal)."""
+
+    from monki.execution import ProposalSettings
+
+    assert ProposalSettings().network_orders == 2
+    ProposalSettings(network_orders=1)
+    ProposalSettings(network_orders=2**32 - 1)
+
+    for bad in (0, -1, 2.0, 2**32):
+        with pytest.raises(ValueError, match="network_orders"):
+            ProposalSettings(network_orders=bad)
+
+
 def test_path_target_settings_and_layer_weights():
     """path_target is a share in [0, 1]; the layer weights are 1 / (the layer's path tally summed
     over pixels and views), read from the documented output order, 0 for layers without path."""

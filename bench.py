@@ -24,11 +24,13 @@ layered-cloud, 8,000 photons per round, AMFConvergence(), 8,192-round cap, media
 after a warm-up; validation/benchmark_amf_convergence.py:27-36) against its published Apple M4
 times; (b) the 3-D box cloud (30 x 20 x 35, tau 20, I/Q/U) with the plain, detector-directed
 and guided tracers, each to a fixed round cap with the worst relative AMF SE and the projection
-to 1 % (SE ~ rounds^-1/2).
+to 1 % (SE ~ rounds^-1/2); (c) the same race on four wavelengths of the spectral field in one
+batch per tracer.
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-amf]
        python bench.py --layered-cloud          (the 30x20x35 layered-cloud round instead)
        python bench.py --race [--amf-rounds R]  (only the box-cloud race)
+       python bench.py --spectral-race          (only the spectral-field race)
        python bench.py --batch [--steps K]      (only the 64-scene box-cloud batch)
        python bench.py --train [--steps K]      (only the proposal's Adam steps, 16,384 samples each)
        python bench.py --copy-calibration | --summarize-traffic DIR | --summarize-profile DB
@@ -387,6 +389,76 @@ def amf_race(rounds: int = 128, photons_per_column: int = 10_000) -> dict:
     return result
 
 
+SPECTRAL_RACE_ROUNDS = 32
+
+
+def spectral_race(rounds: int = SPECTRAL_RACE_ROUNDS) -> dict:
+    """Plain, detector-directed and guided tracers on every 16th wavelength of the spectral field
+    (300, 351, 402 and 452 nm of the 64; 128 x 128 x 60, 16 photons per column), each a batch of
+    the four scenes accumulated on the device, to the same round cap: the relative layer-AMF SE of
+    every cell at the cap and the projected seconds to 1 % (training counted once)."""
+
+    import numpy as np
+
+    from monki import SceneCase, configs
+    from monki.execution import HIPBatch
+    from monki.execution.proposal import DirectedSettings, ProposalSettings
+
+    scenes = configs.spectral_scenes(64)[::16]
+    cases = [SceneCase(f"spectral-{i:02d}", scene, SPECTRAL_PPC, rounds) for i, scene in enumerate(scenes)]
+    legs = {}
+
+    for name, backend in (("plain", {}), ("directed", {"directed": DirectedSettings()}),
+                          ("guided", {"proposal": ProposalSettings()})):  # fmt: skip
+        with HIPBatch(block=128, tally_shards=1, **backend) as batch:
+            batch.prepare(cases)
+            proposal = batch.provenance.get("proposal", {})
+            fixed = proposal.get("logging_seconds", 0.0) + proposal.get("training_seconds", 0.0)
+            device = 0.0
+            start = time.perf_counter()
+
+            for index in range(1, rounds + 1):
+                batch.accumulate_round(index)
+                device += batch.kernel_seconds()
+
+            elapsed = time.perf_counter() - start
+            relative = []
+
+            for number, case in enumerate(cases):
+                cells = batch.moments(number).amf_moments().relative_error(case.scene.zunit[:-1])
+                relative.append(cells[np.isfinite(cells)].ravel())
+
+        finite = np.sort(np.concatenate(relative))
+        worst, p999 = float(finite[-1]), float(finite[int(0.999 * (finite.size - 1))])
+        leg = {"seconds": elapsed, "device_seconds": device, "rounds": rounds, "max_relative_se": worst,
+               "p999_relative_se": p999, "worst_decile_mean_relative_se": float(finite[-max(1, finite.size // 10):].mean()),
+               "median_relative_se": float(np.median(finite)), "cells": int(finite.size)}  # fmt: skip
+
+        if proposal:
+            leg["training"] = {key: proposal[key] for key in ("training_events", "logging_seconds", "training_seconds")}
+
+        for key, se in (("", worst), ("_p999", p999)):
+            needed = rounds * max(se / 0.01, 1.0) ** 2
+            leg[f"projected_rounds{key}"] = needed
+            leg[f"projected_seconds{key}"] = fixed + elapsed / rounds * needed
+
+        legs[name] = leg
+
+    plain = legs["plain"]
+    result = {"workload": "spectral field 128x128x60 at 300, 351, 402 and 452 nm in one batch, I/Q/U, 3 views, "
+                          f"layer AMF tallies, {SPECTRAL_PPC} photons/column, cap {rounds} rounds",  # fmt: skip
+              **legs}
+
+    for name in ("directed", "guided"):
+        leg = legs[name]
+        result[f"{name}_speedup_to_1pct"] = plain["projected_seconds"] / leg["projected_seconds"]
+        result[f"{name}_speedup_to_1pct_p999"] = plain["projected_seconds_p999"] / leg["projected_seconds_p999"]
+        result[f"{name}_worst_decile_variance_ratio"] = (plain["worst_decile_mean_relative_se"] /
+                                                         leg["worst_decile_mean_relative_se"]) ** 2  # fmt: skip
+
+    return result
+
+
 # Network arithmetic per training sample: forward (2 x 56,360 multiply-adds less the biases),
 # input gradients of layers 1-4 and weight gradients of all five layers, about 332 kFLOP.
 TRAIN_FLOP_PER_SAMPLE = 2 * (12 * 128 + 3 * 128 * 128 + 128 * 40) * 3 - 2 * 12 * 128
@@ -556,6 +628,8 @@ def main() -> int:
     parser.add_argument("--amf-rounds", type=int, default=128, help="round cap of the box-cloud race")
     parser.add_argument("--layered-cloud", action="store_true", help="time the layered-cloud round instead")
     parser.add_argument("--race", action="store_true", help="run only the box-cloud race (plain, directed, guided)")
+    parser.add_argument("--spectral-race", action="store_true",
+                        help="run only the spectral-field race (plain, directed, guided; four wavelengths)")
     parser.add_argument("--batch", action="store_true", help="time only the 64-scene box-cloud batch")
     parser.add_argument("--train", action="store_true", help="time only the proposal's Adam steps")
     parser.add_argument("--no-layered", action="store_true", help="skip the secondary layered-cloud and batch lines")
@@ -599,6 +673,10 @@ def main() -> int:
 
     if args.race:
         print(json.dumps(amf_race(args.amf_rounds)), flush=True)
+        return 0
+
+    if args.spectral_race:
+        print(json.dumps(spectral_race()), flush=True)
         return 0
 
     if args.batch:
@@ -663,7 +741,10 @@ def main() -> int:
         fixtures = reference_fixture_amf()
         print("bench: box-cloud race, plain / directed / guided", file=sys.stderr, flush=True)
         race = amf_race(args.amf_rounds)
-        line["time_to_1pct_amf_se"] = {"reference_fixtures": fixtures, "box_cloud_guided_vs_plain": race}
+        print("bench: spectral-field race, plain / directed / guided", file=sys.stderr, flush=True)
+        spectral = spectral_race()
+        line["time_to_1pct_amf_se"] = {"reference_fixtures": fixtures, "box_cloud_guided_vs_plain": race,
+                                       "spectral_guided_vs_plain": spectral}  # fmt: skip
         line["guided_vs_plain_speedup_to_1pct"] = race["speedup_to_1pct"]
 
     if not args.no_layered:

@@ -293,8 +293,18 @@ def test_spectral_field_matches_oracle(oracle, name, precision):
     check_tallies(name, gpu_tallies, cpu_tallies, gpu, cpu, rtol=2e-2 if precision == 32 else 1e-4)
 
 
-@pytest.mark.parametrize("name", ["spectral-300", "spectral-500"])
-def test_split_layer_rows_equal_whole_rows(name):
+SPLIT_CASES = {
+    "spectral-300": ("spectral-300", {}),
+    "spectral-500": ("spectral-500", {}),
+    # Intensity only, no layer paths (the records carry the Stokes slot alone).
+    "spectral-300-I-no-paths": ("spectral-300", {"nmat": 1, "save_path": False}),
+    # Result grid at the ground (the line of sight back from each event).
+    "spectral-400-ground-grid": ("spectral-400", {"register": "surface"}),
+}
+
+
+@pytest.mark.parametrize("case_name", list(SPLIT_CASES))
+def test_split_layer_rows_equal_whole_rows(case_name):
     """59 layers + vacuum: the session keeps the lower layers' paths in LDS and the upper
     (uniform) layers' in global memory, for three waves per SIMD instead of two; its round
     equals the round of the static entry point, whose launch keeps whole rows in LDS, on the
@@ -304,9 +314,10 @@ def test_split_layer_rows_equal_whole_rows(name):
     from monki.execution.hip_tables import pack_tables, tally_count
 
     native = load_library()
-    scene = SCENES[name]
+    name, changes = SPLIT_CASES[case_name]
+    scene = replace(SCENES[name], **changes)
     packed, shape = pack_tables(scene, shards=4)
-    count = tally_count(*shape, True, False)
+    count = tally_count(*shape, scene.save_path, False)
     per_round = 400 * scene.nx * scene.ny
     controls = (ctypes.c_uint32 * 8)(per_round, 0, 1, 23274, 100_000, 0, per_round, 1)
     error = ctypes.create_string_buffer(4096)
@@ -321,7 +332,7 @@ def test_split_layer_rows_equal_whole_rows(name):
 
     try:
         rows = native.monki_hip_layer_rows(handle)
-        print(f"{name}: nz {scene.nz}, {rows} layer paths per lane in LDS", flush=True)
+        print(f"{case_name}: nz {scene.nz}, {rows} layer paths per lane in LDS", flush=True)
         assert rows < (scene.nz | 1)
         status = native.monki_hip_run(handle, controls, split.ctypes.data_as(DOUBLE_POINTER), error, len(error))
         assert status == 0, error.value.decode()

@@ -1575,7 +1575,7 @@ __device__ inline void flush_records(const Scene &s, R *__restrict__ banks, int 
                                      const Record<R> &wave_records, const R *wave_history, int row, int open_layer,
                                      uint32_t round_offset, const Flight<R> &flight, Counts &n, int low = 1 << 30,
                                      const float *wave_upper = nullptr, int stride = 0, int upper_word = 0,
-                                     const float *layers = nullptr, int table_row = 0) {
+                                     const float *layers = nullptr, int table_row = 0, int open_high = -1) {
   const int lane = __lane_id();
   const int record = s.paths ? s.nz + s.nmat : s.nmat;
   const int stokes_at = s.paths ? s.nz : 0;
@@ -1599,7 +1599,7 @@ __device__ inline void flush_records(const Scene &s, R *__restrict__ banks, int 
   // The source lane's small words in one register: views (3 bits), layer + 1 (7), stale (1),
   // end layer + 1 (7); layers stay below 64.
   const int word = (views & 7) | ((open_layer + 1) << 3) | (int(flight.stale) << 10) | ((flight.end_layer + 1) << 11) |
-                   (WIDE ? upper_word << 18 : 0);
+                   (WIDE ? (upper_word << 18) | (int(open_high > open_layer) << 26) : 0);
   for (uint64_t mask = lanes; mask; mask &= mask - 1) {
     const int src = __ffsll(static_cast<long long>(mask)) - 1;
     const int packed = __builtin_amdgcn_readlane(word, src);
@@ -1614,16 +1614,19 @@ __device__ inline void flush_records(const Scene &s, R *__restrict__ banks, int 
       path = (packed >> 25) & 1 ? R(wave_upper[size_t(src) * stride + (lane - low)])
                                 : (lane >= descent ? lane_descent : R(0));
     }
+    // WIDE: records of two adjacent layers, layer and high (E of high in the fifth field).
+    const int high = WIDE ? layer + ((packed >> 26) & 1) : layer;
     if ((packed >> 10) & 1) {
       const int end = ((packed >> 11) & 127) - 1;
-      const bool between = end > layer ? (lane > layer && lane < end) : (lane < layer && lane > end);
+      const bool between = end > high ? (lane > high && lane < end) : (lane < layer && lane > end);
       const R crossed = between ? lane_thickness * read_lane(flight.inverse, src) : R(0);
       const R added = lane == end ? read_lane(flight.end_path, src) : crossed;
       // (A layer first entered on this flight comes back as rounding around zero.) WIDE: an
       // escaping flight's upper crossings were never added.
       if (path_lane && !(upper_lane && end >= s.nz)) path = fmax(R(0), path - added);
     }
-    const bool at_layer = lane == layer, beyond = DOWN ? lane < layer : lane > layer;
+    const bool at_layer = lane == layer, at_high = WIDE && high != layer && lane == high;
+    const bool beyond = DOWN ? lane < layer : lane > high;
 #pragma unroll
     for (int view = 0; view < kMaxViews; ++view) {
       if (!((chosen >> view) & 1)) continue;
@@ -1633,7 +1636,9 @@ __device__ inline void flush_records(const Scene &s, R *__restrict__ banks, int 
       stokes = component == 1 ? source.q(view) : stokes;
       stokes = component == 2 ? source.u(view) : stokes;
       if (V4) stokes = component == 3 ? source.v(view) : stokes;
-      const R value = path_lane ? (at_layer ? source.e(view) : weight * (path + (beyond ? layer_length[view] : R(0))))
+      const R value = path_lane ? (at_layer  ? source.e(view)
+                                   : at_high ? source.v(view)
+                                             : weight * (path + (beyond ? layer_length[view] : R(0))))
                                 : stokes;
       if (pixel >= 0 && tally_lane && value != R(0)) {
         add_tally<R>(bank + (__umul24(uint32_t(pixel), uint32_t(s.record_stride)) + lane_index[view]), value);
@@ -1858,13 +1863,16 @@ __global__ void __launch_bounds__(kBlock32)
   // After the layer rows: the block's pending local estimates (scattering decomposition) or
   // open records (every other mode), struct of arrays over the block's lanes.
   R *const area = reinterpret_cast<R *>(lds_raw) + size_t(block) * hrow;
+  // kWide records span up to two adjacent layers: the record area of the full Stokes vector,
+  // whose fifth field holds the E of the pair's upper layer.
+  constexpr bool PAIRS = WIDE;
   const Pending<R> wave_pending{area + (threadIdx.x & ~63u),
-                                reinterpret_cast<int *>(area + size_t(block) * pending_reals(V4)) +
+                                reinterpret_cast<int *>(area + size_t(block) * pending_reals(V4 || PAIRS)) +
                                     (threadIdx.x & ~63u),
                                 block};
   const Pending<R> pending = wave_pending.at(__lane_id());
   const Record<R> wave_records{area + (threadIdx.x & ~63u),
-                               reinterpret_cast<int *>(area + size_t(block) * record_reals(V4)) +
+                               reinterpret_cast<int *>(area + size_t(block) * record_reals(V4 || PAIRS)) +
                                    (threadIdx.x & ~63u),
                                block};
   const Record<R> record = wave_records.at(__lane_id());
@@ -1872,7 +1880,7 @@ __global__ void __launch_bounds__(kBlock32)
   // vector's rows leave no room at three blocks per CU).
   constexpr bool LAYERS = !V4;
   float *layers = reinterpret_cast<float *>(reinterpret_cast<unsigned char *>(area) +
-                                            size_t(block) * pending_bytes<R>(V4)) +
+                                            size_t(block) * pending_bytes<R>(V4 || PAIRS)) +
                   (threadIdx.x >> 6) * 4 * row;
   uint32_t layers_scene = 0xffffffffu;
   Counts n{0, 0, 0, 0, 0, 0, 0, 0};
@@ -1883,6 +1891,8 @@ __global__ void __launch_bounds__(kBlock32)
   // Layer of the lane's open records, -1 while it has none (every mode but the decomposition).
   constexpr bool MERGE = !SCATS;
   int open_layer = -1;
+  // PAIRS: the upper layer of the records' pair (open_layer itself for one-layer records).
+  int open_high = -1;
 
   for (;;) {
     const Scene &s = scenes[current];
@@ -1948,8 +1958,9 @@ __global__ void __launch_bounds__(kBlock32)
       }
       // A flight that left the layer of the open records changed the layer paths the records
       // hold outside that layer: their write-out takes this flight's part off again.
-      flight = Flight<R>{active && open_layer >= 0 && h.p.k != open_layer, h.p.k, layer_path,
-                         R(1) / fmax(fabs(h.p.c), R(1e-16))};
+      flight = Flight<R>{active && open_layer >= 0 && (PAIRS ? h.p.k < open_layer || h.p.k > open_high
+                                                             : h.p.k != open_layer),
+                         h.p.k, layer_path, R(1) / fmax(fabs(h.p.c), R(1e-16))};
       if (active && status == kRunning) status = interact<R, TRACE, false>(s, h, n, trace, surface);
       const bool event = active && status == kRunning;
       R rz = R(0), delta = R(0), sin_delta = R(0), cos_delta = R(0), path = R(0);
@@ -1988,62 +1999,158 @@ __global__ void __launch_bounds__(kBlock32)
         path_contribution += (h.contribution - weight_before) * event_path;
         logged_path = event_path;
       }
-      // Per view: the event's local estimate joins the open record of the same pixel and layer,
-      // or the open record is written out first (another pixel, or the photon left its layer)
-      // and the event opens a new one. One write-out for all views of all lanes. (Lanes read
-      // each other's LDS rows and records: every access of the wave in order around it.)
-      int flushed = 0;
+      if constexpr (PAIRS) {
+        // Records of up to two adjacent layers. A record whose events all lie in one layer takes
+        // in the next layer up or down when the photon's flight went straight there and its
+        // pixel stays (or the view gets nothing from the event): its past events' E for the new
+        // layer is S (that layer's path before this flight [+ the full detector-ray length when
+        // the layer lies above theirs]); every event then adds C1 (path + detector-ray length)
+        // to the E of both layers. The layer paths outside the pair stay those of the events.
+        const int k = h.p.k;
+        const bool left = flight.stale;
+        const bool adjacent = event && left && open_layer == open_high && (k == open_layer + 1 || k == open_layer - 1);
+        int survive = 0;
 #pragma unroll 1
-      for (int view = 0; view < s.views; ++view) {
-        const int pixel = view == 0 ? e0.pixel : (view == 1 ? e1.pixel : e2.pixel);
-        const int held = event && open_layer >= 0 ? record.pixel(view) : -1;
-        if (held >= 0 && (flight.stale || (pixel >= 0 && pixel != held))) flushed |= 1 << view;
-      }
-      const unsigned long long flushing = __ballot(flushed != 0);
-      if (flushing) {
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        flush_records<R, V4, DOWN, WIDE>(s, banks, shard, flushing, flushed, wave_records, wave_history, hrow,
-                                         open_layer, h.round_offset, flight, n, hrow, wave_upper, split_stride,
-                                         int(u.word & 255u), layers, row);
-        __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-      }
-      bool open = false;
+        for (int view = 0; view < s.views; ++view) {
+          const int pixel = view == 0 ? e0.pixel : (view == 1 ? e1.pixel : e2.pixel);
+          const int held = event && open_layer >= 0 ? record.pixel(view) : -1;
+          if (held >= 0 && (!left || adjacent) && (pixel < 0 || pixel == held)) survive |= 1 << view;
+        }
+        const bool extend = adjacent && survive != 0;
+        int flushed = 0;
 #pragma unroll 1
-      for (int view = 0; view < s.views; ++view) {
-        const Estimate<R> e = view == 0 ? e0 : (view == 1 ? e1 : e2);
-        const int held = event && open_layer >= 0 ? record.pixel(view) : -1;
-        const bool same = !flight.stale && held == e.pixel;
-        if (e.pixel >= 0) {
-          const R part = e.weight * (path + rz * R(s.view_inv_mu[view]));
-          if (same) {
-            record.e(view) += part;
-            record.s(view) += e.weight;
-            record.q(view) += e.q;
-            record.u(view) += e.u;
-            if (V4) record.v(view) += e.v;
-          } else {
-            record.e(view) = part;
-            record.s(view) = e.weight;
-            record.q(view) = e.q;
-            record.u(view) = e.u;
-            if (V4) record.v(view) = e.v;
-            record.pixel(view) = e.pixel;
+        for (int view = 0; view < s.views; ++view) {
+          const int held = event && open_layer >= 0 ? record.pixel(view) : -1;
+          if (held >= 0 && !((survive >> view) & 1)) flushed |= 1 << view;
+        }
+        const unsigned long long flushing = __ballot(flushed != 0);
+        if (flushing) {
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          flush_records<R, V4, DOWN, WIDE>(s, banks, shard, flushing, flushed, wave_records, wave_history, hrow,
+                                           open_layer, h.round_offset, flight, n, hrow, wave_upper, split_stride,
+                                           int(u.word & 255u), layers, row, open_high);
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        }
+        const bool fresh = left || open_layer < 0;
+        const int low = extend ? min(open_layer, k) : (fresh ? k : open_layer);
+        const int high = extend ? max(open_layer, k) : (fresh ? k : open_high);
+        auto path_at = [&](int layer) {
+          return layer >= hrow ? upper_path<R>(s, u, upper, hrow, layers, row, layer) : history[layer];
+        };
+        R path_low = R(0), path_high = R(0);
+        if (event) {
+          path_low = path_at(low);
+          path_high = high != low ? path_at(high) : path_low;
+        }
+        const R path_new = k == low ? path_low : path_high;
+        bool open = false;
+#pragma unroll 1
+        for (int view = 0; view < s.views; ++view) {
+          const Estimate<R> e = view == 0 ? e0 : (view == 1 ? e1 : e2);
+          const bool keep = (survive >> view) & 1;
+          const R inv_mu = R(s.view_inv_mu[view]);
+          if (keep && extend) {
+            // The past events' E for the layer the record takes in.
+            const R before = path_new - flight.end_path;
+            const R past = record.s(view) * (k > open_layer ? before + R(layers[row + k]) * inv_mu : before);
+            if (k > open_layer) {
+              record.v(view) = past;
+            } else {
+              record.v(view) = record.e(view);
+              record.e(view) = past;
+            }
           }
-          open = true;
-        } else if (event) {
-          // Nothing toward this view: a record of the same layer stays open.
-          if (held < 0 || flight.stale) record.pixel(view) = -1;
-          open = open || (held >= 0 && !flight.stale);
+          if (e.pixel >= 0) {
+            // Detector-ray length in the event layer (its part above the event), the full
+            // layer above it, none below.
+            const R part_low = e.weight * (path_low + (low == k ? rz * inv_mu : R(0)));
+            const R part_high =
+                e.weight * (path_high + (high == k ? rz * inv_mu : R(layers[row + high]) * inv_mu));
+            if (keep) {
+              record.e(view) += part_low;
+              if (high != low) record.v(view) += part_high;
+              record.s(view) += e.weight;
+              record.q(view) += e.q;
+              record.u(view) += e.u;
+            } else {
+              record.e(view) = part_low;
+              record.v(view) = high != low ? part_high : R(0);
+              record.s(view) = e.weight;
+              record.q(view) = e.q;
+              record.u(view) = e.u;
+              record.pixel(view) = e.pixel;
+            }
+            open = true;
+          } else if (event) {
+            if (!keep) record.pixel(view) = -1;
+            open = open || keep;
+          }
+        }
+        if (event) {
+          open_layer = open ? low : -1;
+          open_high = open ? high : -1;
+          flight.stale = false;
+        }
+      } else {
+        // Per view: the event's local estimate joins the open record of the same pixel and layer,
+        // or the open record is written out first (another pixel, or the photon left its layer)
+        // and the event opens a new one. One write-out for all views of all lanes. (Lanes read
+        // each other's LDS rows and records: every access of the wave in order around it.)
+        int flushed = 0;
+  #pragma unroll 1
+        for (int view = 0; view < s.views; ++view) {
+          const int pixel = view == 0 ? e0.pixel : (view == 1 ? e1.pixel : e2.pixel);
+          const int held = event && open_layer >= 0 ? record.pixel(view) : -1;
+          if (held >= 0 && (flight.stale || (pixel >= 0 && pixel != held))) flushed |= 1 << view;
+        }
+        const unsigned long long flushing = __ballot(flushed != 0);
+        if (flushing) {
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          flush_records<R, V4, DOWN, WIDE>(s, banks, shard, flushing, flushed, wave_records, wave_history, hrow,
+                                           open_layer, h.round_offset, flight, n, hrow, wave_upper, split_stride,
+                                           int(u.word & 255u), layers, row);
+          __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+        }
+        bool open = false;
+  #pragma unroll 1
+        for (int view = 0; view < s.views; ++view) {
+          const Estimate<R> e = view == 0 ? e0 : (view == 1 ? e1 : e2);
+          const int held = event && open_layer >= 0 ? record.pixel(view) : -1;
+          const bool same = !flight.stale && held == e.pixel;
+          if (e.pixel >= 0) {
+            const R part = e.weight * (path + rz * R(s.view_inv_mu[view]));
+            if (same) {
+              record.e(view) += part;
+              record.s(view) += e.weight;
+              record.q(view) += e.q;
+              record.u(view) += e.u;
+              if (V4) record.v(view) += e.v;
+            } else {
+              record.e(view) = part;
+              record.s(view) = e.weight;
+              record.q(view) = e.q;
+              record.u(view) = e.u;
+              if (V4) record.v(view) = e.v;
+              record.pixel(view) = e.pixel;
+            }
+            open = true;
+          } else if (event) {
+            // Nothing toward this view: a record of the same layer stays open.
+            if (held < 0 || flight.stale) record.pixel(view) = -1;
+            open = open || (held >= 0 && !flight.stale);
+          }
+        }
+        if (event) {
+          // The records now hold the layer paths of this event.
+          open_layer = open ? h.p.k : -1;
+          flight.stale = false;
         }
       }
-      if (event) {
-        // The records now hold the layer paths of this event.
-        open_layer = open ? h.p.k : -1;
-        flight.stale = false;
       }
-    }
     // Russian roulette of the weight window, after the event's local estimates (a photon steered
     // toward a view has used its small weight there) and before its scattering.
     if (window && active && status == kRunning && h.steer < R(guidance.roulette_below)) {
@@ -2126,11 +2233,11 @@ __global__ void __launch_bounds__(kBlock32)
         __builtin_amdgcn_wave_barrier();
         flush_records<R, V4, DOWN, WIDE>(s, banks, shard, ending, (1 << s.views) - 1, wave_records, wave_history,
                                          hrow, open_layer, h.round_offset, flight, n, hrow, wave_upper, split_stride,
-                                         int(u.word & 255u), layers, row);
+                                         int(u.word & 255u), layers, row, open_high);
         __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
         __builtin_amdgcn_wave_barrier();
       }
-      if (ends) open_layer = -1;
+      if (ends) open_layer = open_high = -1;
     }
     if (!active || status == kRunning) continue;
     if (STEER && top > 0) {
@@ -2695,7 +2802,8 @@ size_t lds_bytes(const Session *h, int block, bool v4 = false) {
 
 // kWide: history rows of `wide` values, the per-wave layer tables still `row` long.
 size_t lds_bytes_wide(int wide, int row, int block) {
-  return (size_t(wide) * sizeof(float) + pending_bytes<float>(false)) * block + size_t(block / 64) * 4 * row * sizeof(float);
+  // Records of two layers: the five-field record area.
+  return (size_t(wide) * sizeof(float) + pending_bytes<float>(true)) * block + size_t(block / 64) * 4 * row * sizeof(float);
 }
 
 // LDS of a launch in mode MODE.
@@ -2704,23 +2812,40 @@ size_t launch_lds(const Session *h, int block) {
   return MODE == kWide ? lds_bytes_wide(h->wide, h->row, block) : lds_bytes<R>(h, block, full_stokes(MODE));
 }
 
+// Blocks of `block` lanes a CU holds for the plain float32 tracer with `lds` bytes of LDS per
+// block (the occupancy calculator: registers, LDS and its allocation granule).
+int resident_blocks(bool wide, int block, size_t lds) {
+  int blocks = 0;
+  const hipError_t status =
+      wide ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<float, false, kWide>, block, lds)
+           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, transport_kernel<float, false, kPlain>, block, lds);
+  return status == hipSuccess ? blocks : 0;
+}
+
+// Whether `blocks` blocks with `lds` bytes each fit a CU's LDS with 512 bytes to spare per
+// block (measured on gfx950: 26,720 and 26,528 B per 128-lane block run six to a CU, 27,040 B,
+// which the occupancy calculator also places six to a CU, ran a fifth slower).
+bool lds_fits(size_t lds, int blocks, size_t lds_per_cu) {
+  return size_t(blocks) * (lds + 512) <= lds_per_cu;
+}
+
 // Plain float32 launches of scenes whose whole layer rows leave fewer than three waves per SIMD
 // (the budget of the kernel's registers) run as kWide when that restores three waves: the
-// longest odd LDS row that fits, at least the highest varying layer of every scene (so that the
-// cell walks stay in LDS). Forward TOA scenes of nmat 1 or 3 only. Returns the row, or 0.
+// longest odd LDS row with which both the occupancy calculator and the LDS granules give three
+// waves per SIMD, at least the highest varying layer of every scene (so that the cell walks
+// stay in LDS). Forward TOA scenes of nmat 1 or 3 only. Returns the row, or 0.
 int wide_row(const Session *h, int block, size_t lds_per_cu) {
   int uniform_top = 0;
   for (const Scene &s : h->scenes) {
     if (s.nmat == 4 || s.backward || s.scats || s.detector_surface) return 0;
     uniform_top = s.uniform_top > uniform_top ? s.uniform_top : uniform_top;
   }
-  const size_t waves = 12;  // three per SIMD
-  const size_t blocks = (waves * 64 + block - 1) / block;
-  const size_t budget = lds_per_cu / blocks;
-  if (lds_bytes_for<float>(h->row, block, false) <= budget) return 0;
-  for (int wide = h->row - 2; wide >= 1; wide -= 2) {
-    if (lds_bytes_wide(wide, h->row, block) > budget) continue;
-    return wide >= uniform_top && wide >= 1 ? wide : 0;
+  const int needed = (12 * 64 + block - 1) / block;  // blocks per CU for three waves per SIMD
+  const size_t whole = lds_bytes_for<float>(h->row, block, false);
+  if (resident_blocks(false, block, whole) >= needed && lds_fits(whole, needed, lds_per_cu)) return 0;
+  for (int wide = h->row - 2; wide >= uniform_top && wide >= 1; wide -= 2) {
+    const size_t lds = lds_bytes_wide(wide, h->row, block);
+    if (resident_blocks(true, block, lds) >= needed && lds_fits(lds, needed, lds_per_cu)) return wide;
   }
   return 0;
 }
@@ -3018,7 +3143,7 @@ Session *create_session(uint32_t device, const float *tables, const size_t *offs
   h->wide = wide_row(h, h->block, lds_per_cu);
   if (h->wide) {
     h->upper_stride = nz - h->wide;
-    const size_t resident = lds_per_cu / lds_bytes_wide(h->wide, h->row, h->block);
+    const size_t resident = size_t(resident_blocks(true, h->block, lds_bytes_wide(h->wide, h->row, h->block)));
     h->upper_blocks = uint32_t(size_t(h->compute_units > 0 ? h->compute_units : 1) * (resident ? resident : 1));
     const size_t floats = size_t(h->upper_blocks) * size_t(h->block) * size_t(h->upper_stride);
     if (!check(hipMalloc(&h->upper, floats * sizeof(float)), error, error_size, "allocate upper-layer rows")) {

@@ -2822,17 +2822,18 @@ int resident_blocks(bool wide, int block, size_t lds) {
   return status == hipSuccess ? blocks : 0;
 }
 
-// Whether `blocks` blocks with `lds` bytes each fit a CU's LDS with 512 bytes to spare per
-// block (measured on gfx950: 26,720 and 26,528 B per 128-lane block run six to a CU, 27,040 B,
-// which the occupancy calculator also places six to a CU, ran a fifth slower).
+// Whether `blocks` blocks with `lds` bytes each fit a CU's LDS with 2 KB of it to spare. Measured
+// on gfx950 (160 KB per CU): six 128-lane blocks of 26,016, 26,528 or 26,720 B and three 256-lane
+// blocks of 53,056 B run together; six of 27,040 B and three of 54,080 B, which the occupancy
+// calculator also places together, ran a fifth and a half slower.
 bool lds_fits(size_t lds, int blocks, size_t lds_per_cu) {
-  return size_t(blocks) * (lds + 512) <= lds_per_cu;
+  return size_t(blocks) * lds + 2048 <= lds_per_cu;
 }
 
 // Plain float32 launches of scenes whose whole layer rows leave fewer than three waves per SIMD
 // (the budget of the kernel's registers) run as kWide when that restores three waves: the
-// longest odd LDS row with which both the occupancy calculator and the LDS granules give three
-// waves per SIMD, at least the highest varying layer of every scene (so that the cell walks
+// longest odd LDS row with which both the occupancy calculator and the measured LDS limit give
+// three waves per SIMD, at least the highest varying layer of every scene (so that the cell walks
 // stay in LDS). Forward TOA scenes of nmat 1 or 3 only. Returns the row, or 0.
 int wide_row(const Session *h, int block, size_t lds_per_cu) {
   int uniform_top = 0;

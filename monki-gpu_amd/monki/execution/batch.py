@@ -21,7 +21,7 @@ import numpy as np
 from ..input import ReferenceCase, SceneCase
 from ..metadata import file_hash, scene_metadata
 from ..preparation import UPSTREAM, prepare_case
-from .hip import DOUBLE_POINTER, FLOAT_POINTER, HIP, LIBRARY, UINT_POINTER, load_library
+from .hip import DOUBLE_POINTER, FLOAT_POINTER, HIP, LIBRARY, UINT_POINTER, all_finite, load_library
 from .hip_tables import pack_tables, tally_count
 from .proposal import DirectedSettings, Proposal, ProposalSettings, apply_steering, sharpest_species
 from .tallies import BatchControls, RawTallies
@@ -299,7 +299,7 @@ class HIPBatch:
                 int(self.use_graph), error, len(error),
             )  # fmt: skip
 
-            if status or not np.all(np.isfinite(out)):
+            if status or not all_finite(out):
                 raise RuntimeError(error.value.decode() or "Nonfinite HIP batch tallies")
         except BaseException:
             self.close()
@@ -307,6 +307,7 @@ class HIPBatch:
 
         self._round = round_index
         self.provenance["gpu_rounds"] += 1
+        # Every array returned is a view of this round's own buffer (no copies).
         results, start = [], 0
 
         for case, shape, count in zip(self._cases, self._shapes, self._counts):
@@ -314,13 +315,13 @@ class HIPBatch:
             start += count
             nmat, nx, ny, nz, views = shape
             stokes_count = nmat * nx * ny * views
-            stokes = values[:stokes_count].reshape((nmat, nx, ny, views), order="F").copy()
+            stokes = values[:stokes_count].reshape((nmat, nx, ny, views), order="F")
             paths = weights = None
 
             if self.save_path:
                 path_count = nx * ny * nz * views
-                paths = values[stokes_count : stokes_count + path_count].reshape((nx, ny, nz, views), order="F").copy()
-                weights = values[stokes_count + path_count :].reshape((nx, ny, views), order="F").copy()
+                paths = values[stokes_count : stokes_count + path_count].reshape((nx, ny, nz, views), order="F")
+                weights = values[stokes_count + path_count :].reshape((nx, ny, views), order="F")
 
             results.append(RawTallies(BatchControls(round_index, case.photons_per_column), stokes, paths, weights))
 
@@ -361,7 +362,7 @@ class HIPBatch:
                 int(self.use_graph), error, len(error),
             )  # fmt: skip
 
-            if status or not np.all(np.isfinite(stokes)):
+            if status or not all_finite(stokes):
                 raise RuntimeError(error.value.decode() or "Nonfinite HIP batch tallies")
         except BaseException:
             self.close()
@@ -369,13 +370,14 @@ class HIPBatch:
 
         self._round = round_index
         self.provenance["gpu_rounds"] += 1
+        # Views of this round's own buffer (no copies).
         results, start = [], 0
 
         for shape, size, on in zip(self._shapes, sizes, active):
             values = stokes[start : start + size]
             start += size
             nmat, nx, ny, _, views = shape
-            results.append(values.reshape((nmat, nx, ny, views), order="F").copy() if on else None)
+            results.append(values.reshape((nmat, nx, ny, views), order="F") if on else None)
 
         return results
 

@@ -729,7 +729,7 @@ def main() -> int:
             "basis": "algorithmic bytes per round / device time per round (HIP events around the replayed round)",
             "measured_hbm_frac": traffic / run["device_round"] / 1e9 / HBM_PEAK_GBS if traffic else None,
             "binding": {"kind": "latency and lane divergence of the walks (SQ counters in the committed profile: VALU "
-                                "busy 0.54 at lane utilization 0.35), then the tally's record atomics: one wave-wide "
+                                "busy 0.51 at lane utilization 0.34), then the tally's record atomics: one wave-wide "
                                 "float atomic instruction per record write-out (a pixel's 63-value record of layer "
                                 "paths and Stokes components)",
                         "record_instructions_per_s": records_rate,

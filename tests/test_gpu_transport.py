@@ -306,9 +306,10 @@ SPLIT_CASES = {
 @pytest.mark.parametrize("case_name", list(SPLIT_CASES))
 def test_split_layer_rows_equal_whole_rows(case_name):
     """59 layers + vacuum: the session keeps the lower layers' paths in LDS and the upper
-    (uniform) layers' in global memory, for three waves per SIMD instead of two; its round
-    equals the round of the static entry point, whose launch keeps whole rows in LDS, on the
-    same Philox streams (300 nm: the most Rayleigh events in the upper layers)."""
+    (uniform) layers' in global memory, for three waves per SIMD instead of two, and its open
+    tally records may span two adjacent layers; its round equals the round of the static entry
+    point, whose launch keeps whole rows in LDS and one-layer records, on the same Philox
+    streams (300 nm: the most Rayleigh events in the upper layers)."""
 
     from monki.execution.hip import DOUBLE_POINTER, FLOAT_POINTER, load_library
     from monki.execution.hip_tables import pack_tables, tally_count

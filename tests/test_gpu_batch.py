@@ -55,6 +55,33 @@ def test_batched_round_equals_single_scene_sessions(block):
                 assert_tallies_equal(batched[index - 1][number], alone)
 
 
+def test_round_arrays_survive_later_rounds_and_close():
+    """Returned tallies alias no buffer of the session: later rounds and close leave them as
+    they were (the reference's ownership rule, tallies.py:21-27, test_metal.py:126, :327-329)."""
+
+    from monki.execution import HIPBatch
+
+    batch_cases = cases(rounds=4)
+
+    with HIPBatch(tally_shards=8) as batch:
+        batch.prepare(batch_cases)
+        first = batch.run_round(1)
+        saved = [(r.stokes.copy(), r.weighted_paths.copy(), r.weights.copy()) for r in first]
+        stokes = batch.accumulate_round(2)
+        stokes_saved = [a.copy() for a in stokes]
+        batch.run_round(3)
+        batch.accumulate_round(4)
+
+    for r, (a, b, c) in zip(first, saved):
+        np.testing.assert_array_equal(r.stokes, a)
+        np.testing.assert_array_equal(r.weighted_paths, b)
+        np.testing.assert_array_equal(r.weights, c)
+
+    for a, b in zip(stokes, stokes_saved):
+        np.testing.assert_array_equal(a, b)
+        assert np.abs(a).max() > 0
+
+
 def test_graph_replays_equal_direct_launches():
     """The captured round, replayed several times with direct launches of another session and
     host work in between, equals the directly launched round every time."""

@@ -2099,7 +2099,7 @@ __global__ void __launch_bounds__(kBlock32)
         // and the event opens a new one. One write-out for all views of all lanes. (Lanes read
         // each other's LDS rows and records: every access of the wave in order around it.)
         int flushed = 0;
-  #pragma unroll 1
+#pragma unroll 1
         for (int view = 0; view < s.views; ++view) {
           const int pixel = view == 0 ? e0.pixel : (view == 1 ? e1.pixel : e2.pixel);
           const int held = event && open_layer >= 0 ? record.pixel(view) : -1;
@@ -2116,7 +2116,7 @@ __global__ void __launch_bounds__(kBlock32)
           __builtin_amdgcn_wave_barrier();
         }
         bool open = false;
-  #pragma unroll 1
+#pragma unroll 1
         for (int view = 0; view < s.views; ++view) {
           const Estimate<R> e = view == 0 ? e0 : (view == 1 ? e1 : e2);
           const int held = event && open_layer >= 0 ? record.pixel(view) : -1;
